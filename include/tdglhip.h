@@ -1,0 +1,179 @@
+/*
+ * tdglhip.h — C ABI of the MI355X ladder-cycle stepper for the N-band TDGL solver (release 2.0).
+ *
+ * The library replaces the device half of gpu/TDGLGPU/src/TDGLGPU.jl: a caller builds a stepper
+ * description from its own Stepper (the coefficients that gtdgl/src/scheme.jl:250-434 derives),
+ * uploads a State, steps it n times on the GPU and downloads it.  Every pointer is a plain host
+ * pointer; arrays are column-major (Julia order, i fastest) in exactly the shapes of
+ * gtdgl/src/state.jl:51-60, complex values interleaved (re, im) as ComplexF64.
+ *
+ * Grid conventions (gtdgl/src/grid.jl:109-127): Nx, Ny, Nz cells with padding, nodes 1..N+1 per
+ * axis, the block x1..x2+1 (nodes), 1-based indices as in the reference.
+ *
+ * Errors: every entry point returns 0 on success and a negative TDGL_E_* code otherwise; the
+ * message of the last failure on the calling thread is tdgl_last_error().  Calls are synchronous:
+ * when tdgl_step returns, the device state is final.  One handle must not be used from two
+ * threads at once.
+ */
+#ifndef TDGLHIP_H
+#define TDGLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDGL_ABI_VERSION 1
+#define TDGL_MAX_BANDS 4
+#define TDGL_MAX_STAGES 64
+
+enum {
+  TDGL_OK = 0,
+  TDGL_E_ARG = -1,         /* a bad argument (sizes, pointers, versions) */
+  TDGL_E_UNSUPPORTED = -2, /* an option this version does not run (tdgl_supported says which) */
+  TDGL_E_DEVICE = -3,      /* a HIP runtime error */
+  TDGL_E_NOMEM = -4        /* device allocation failed */
+};
+
+/* Field solve of the step (gtdgl/src/step.jl:25-29). */
+enum {
+  TDGL_FIELD_EXPLICIT = 0,      /* φ from field_increment! only */
+  TDGL_FIELD_SEPARABLE = 1,     /* implicit curl-curl, SepSolver (implicit.jl:62-104) */
+  TDGL_FIELD_PCG = 2,           /* implicit with conductivity per link, PCGSolver (implicit.jl:279-446) */
+  TDGL_FIELD_EXPLICIT_SCALED = 3 /* explicit, scale_increment! by D (implicit.jl:337-342) */
+};
+
+/* Self-field boundary (scheme.jl:414-415, field.jl:189-228). */
+enum { TDGL_BC_SLAB = 0, TDGL_BC_BAR = 1 };
+
+/* Coefficients of one band for one (sub)step: BandCoef (gtdgl/src/scheme.jl:21-28). */
+typedef struct {
+  double kx, ky, kz; /* dt/(m_i Γ̂ dx_i²) */
+  double c;          /* dt/Γ */
+  double eps;        /* ε = a − T */
+  double beta;       /* β */
+  double gamma2;     /* γ̃² of KWT (0 = plain TDGL) */
+  double cpl[TDGL_MAX_BANDS]; /* (dt/Γ̂) γ_µν, cpl[µ] = 0 */
+} tdgl_bandcoef;
+
+/* The stepper: what Stepper{N} (scheme.jl:167-198) holds and the step reads. */
+typedef struct {
+  int32_t abi_version;           /* TDGL_ABI_VERSION */
+  int32_t nbands;                /* N, 1..TDGL_MAX_BANDS */
+  int32_t Nx, Ny, Nz;            /* cells, padding included */
+  int32_t x1, x2, y1, y2, z1, z2;
+  double dx, dy, dz;
+  double dt;
+
+  tdgl_bandcoef bc[TDGL_MAX_BANDS];  /* full-step coefficients (st.bc) */
+  tdgl_bandcoef bcs[TDGL_MAX_BANDS]; /* per substep / stage (st.bcs) */
+  int32_t nsub[TDGL_MAX_BANDS];      /* Euler substeps (st.nsub) */
+  int32_t stages[TDGL_MAX_BANDS];    /* RKL1/RKC1 stages (st.rkl; 0 = Euler) */
+  int32_t rkc[TDGL_MAX_BANDS];       /* 1: the stage weights below are RKC1 (RKCCoef), 0: RKL1 */
+  double divisor[TDGL_MAX_BANDS];    /* q of the stages (s(s+1)/2 or RKCCoef.q), or nsub */
+  double rkc_mu[TDGL_MAX_BANDS][TDGL_MAX_STAGES]; /* RKCCoef.μ (index j-1) */
+  double rkc_nu[TDGL_MAX_BANDS][TDGL_MAX_STAGES]; /* RKCCoef.ν */
+  double rkc_c[TDGL_MAX_BANDS][TDGL_MAX_STAGES];  /* RKCCoef.c */
+  double wa[3][TDGL_MAX_BANDS];      /* supercurrent weight g_µ along x, y, z (st.wa) */
+
+  /* FieldCoef (scheme.jl:149-153) */
+  double d1, sxy, sxz, syx, syz, szx, szy, a;
+  int32_t field_mode;                /* TDGL_FIELD_* */
+  int32_t selffield_bc;              /* TDGL_BC_* */
+  int32_t extrap, rotframe, hard;    /* Stepper flags */
+
+  /* PCGSolver (implicit.jl:142-189) */
+  double pcg_tol;
+  int32_t pcg_maxit;
+  int32_t pcg_fast;                  /* PCGFast present (warm extrapolation + flexible β) */
+  int32_t pcg_f32;                   /* Float32 separable preconditioner */
+  int32_t pcg_extrap;                /* 2x_n − x_{n−1} first guess (with pcg_fast) */
+
+  /* Host arrays, copied at creation.  Masks (grid.jl:135-143) are node-shaped
+   * (Nx+1, Ny+1, Nz+1); D are link-shaped; bar profiles as bar_field (field.jl:151-180). */
+  const double *f;
+  const double *taux, *tauy, *tauz;
+  const double *Dx, *Dy, *Dz;        /* field_mode PCG / EXPLICIT_SCALED, else NULL */
+  const double *bar_hy_z;            /* (Ny+1, 2) */
+  const double *bar_hy_y;            /* (2, Nz) */
+  const double *bar_hz_z;            /* (Ny, 2) */
+  const double *bar_hz_y;            /* (2, Nz+1) */
+  const double *bar_hz_x;            /* (Ny, Nz+1) */
+  const double *bar_hy_x;            /* (Ny+1, Nz) */
+} tdgl_desc;
+
+/* Host views of a State (gtdgl/src/state.jl:51-60), Julia shapes:
+ *   G[µ], F[µ]      ComplexF64 (Nx+1, Ny+1, Nz+1)
+ *   thx, phx, Jx    Float64 (Nx, Ny+1, Nz+1);  Ux, Vx ComplexF64, same shape
+ *   thy, phy, Jy    Float64 (Nx+1, Ny, Nz+1);  Uy, Vy
+ *   thz, phz, Jz    Float64 (Nx+1, Ny+1, Nz);  Uz, Vz
+ *   hx (Nx+1, Ny, Nz), hy (Nx, Ny+1, Nz), hz (Nx, Ny, Nz+1)          Float64 */
+typedef struct {
+  double *G[TDGL_MAX_BANDS];
+  double *F[TDGL_MAX_BANDS];
+  double *thx, *thy, *thz;
+  double *phx, *phy, *phz;
+  double *Ux, *Uy, *Uz;
+  double *Vx, *Vy, *Vz;
+  double *Jx, *Jy, *Jz;
+  double *hx, *hy, *hz;
+} tdgl_state;
+
+/* Stepper-side PCG state that a checkpoint holds (implicit.jl:142-169, _pcg_snap :196-197):
+ * x and xprev on the ranges rx = (2:Nx-1, 2:Ny, 2:Nz), ry = (2:Nx, 2:Ny-1, 2:Nz),
+ * rz = (2:Nx, 2:Ny, 2:Nz-1), column-major. */
+typedef struct {
+  double *x[3];
+  double *xprev[3];
+  int32_t warm, have2, iters;
+} tdgl_pcg;
+
+typedef struct {
+  int64_t steps;        /* steps taken by this handle */
+  int64_t calls;        /* tdgl_step calls (cycles) */
+  int64_t pcg_iters;    /* PCG iterations summed over steps */
+  int32_t last_iters;   /* iterations of the last solve (PCGSolver.iters) */
+  double device_ms;     /* device time of the steps, HIP events */
+  int64_t device_bytes; /* device memory held by the handle */
+} tdgl_stats;
+
+typedef struct tdgl_handle tdgl_handle;
+
+/* Replaces supported(st) (TDGLGPU.jl:39-44): 1 if the description runs, else 0 and the
+ * first missing option in why (NUL-terminated, at most why_len bytes). */
+int tdgl_supported(const tdgl_desc *desc, char *why, size_t why_len);
+
+/* Replaces GPUStepper(st, backend) (TDGLGPU.jl:1299-1316): device copies of the coefficients,
+ * masks and work arrays, on the current HIP device. */
+int tdgl_create(const tdgl_desc *desc, tdgl_handle **out);
+int tdgl_destroy(tdgl_handle *h);
+
+/* Replaces device_state / device_state! (TDGLGPU.jl:64-74) and host_state! : the whole State. */
+int tdgl_upload_state(tdgl_handle *h, const tdgl_state *s);
+int tdgl_download_state(tdgl_handle *h, tdgl_state *s);
+
+/* Replaces gpu_steps!(g, gs, n; H, HJ) (TDGLGPU.jl:2065-2071): n steps of step!
+ * (gtdgl/src/step.jl:13-35), synchronous. */
+int tdgl_step(tdgl_handle *h, int32_t n, double H, double HJ);
+
+/* Replaces _pcg_up! / _pcg_down! (TDGLGPU.jl:217-224): PCG history in and out. */
+int tdgl_get_pcg(tdgl_handle *h, tdgl_pcg *p);
+int tdgl_set_pcg(tdgl_handle *h, const tdgl_pcg *p);
+
+/* Counters (execution_stats, TDGLGPU.jl:2086-2087, per handle). */
+int tdgl_get_stats(tdgl_handle *h, tdgl_stats *st);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *tdgl_last_error(void);
+
+/* Number of HIP devices visible (0 if none); selects device `dev` for later tdgl_create. */
+int tdgl_device_count(void);
+int tdgl_set_device(int32_t dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TDGLHIP_H */

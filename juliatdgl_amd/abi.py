@@ -1,0 +1,168 @@
+"""ctypes mirror of include/tdglhip.h: the description, state and PCG structs, and the packing of
+a host Stepper / State into them.  Shared by the HIP binding (juliatdgl_amd.hip) and by any other
+implementation of the same C ABI."""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from .scheme import FIELD_PCG, FIELD_EXPLICIT_SCALED, Stepper
+from .state import FIELDS, State
+
+ABI_VERSION = 1
+MAXB = 4
+MAXS = 64
+dp = ct.POINTER(ct.c_double)
+
+
+class BandCoefC(ct.Structure):
+    _fields_ = [("kx", ct.c_double), ("ky", ct.c_double), ("kz", ct.c_double), ("c", ct.c_double),
+                ("eps", ct.c_double), ("beta", ct.c_double), ("gamma2", ct.c_double),
+                ("cpl", ct.c_double * MAXB)]
+
+
+class DescC(ct.Structure):
+    _fields_ = [("abi_version", ct.c_int32), ("nbands", ct.c_int32),
+                ("Nx", ct.c_int32), ("Ny", ct.c_int32), ("Nz", ct.c_int32),
+                ("x1", ct.c_int32), ("x2", ct.c_int32), ("y1", ct.c_int32), ("y2", ct.c_int32),
+                ("z1", ct.c_int32), ("z2", ct.c_int32),
+                ("dx", ct.c_double), ("dy", ct.c_double), ("dz", ct.c_double), ("dt", ct.c_double),
+                ("bc", BandCoefC * MAXB), ("bcs", BandCoefC * MAXB),
+                ("nsub", ct.c_int32 * MAXB), ("stages", ct.c_int32 * MAXB), ("rkc", ct.c_int32 * MAXB),
+                ("divisor", ct.c_double * MAXB),
+                ("rkc_mu", (ct.c_double * MAXS) * MAXB), ("rkc_nu", (ct.c_double * MAXS) * MAXB),
+                ("rkc_c", (ct.c_double * MAXS) * MAXB),
+                ("wa", (ct.c_double * MAXB) * 3),
+                ("d1", ct.c_double), ("sxy", ct.c_double), ("sxz", ct.c_double), ("syx", ct.c_double),
+                ("syz", ct.c_double), ("szx", ct.c_double), ("szy", ct.c_double), ("a", ct.c_double),
+                ("field_mode", ct.c_int32), ("selffield_bc", ct.c_int32),
+                ("extrap", ct.c_int32), ("rotframe", ct.c_int32), ("hard", ct.c_int32),
+                ("pcg_tol", ct.c_double), ("pcg_maxit", ct.c_int32), ("pcg_fast", ct.c_int32),
+                ("pcg_f32", ct.c_int32), ("pcg_extrap", ct.c_int32),
+                ("f", dp), ("taux", dp), ("tauy", dp), ("tauz", dp),
+                ("Dx", dp), ("Dy", dp), ("Dz", dp),
+                ("bar_hy_z", dp), ("bar_hy_y", dp), ("bar_hz_z", dp), ("bar_hz_y", dp),
+                ("bar_hz_x", dp), ("bar_hy_x", dp)]
+
+
+class StateC(ct.Structure):
+    _fields_ = [("G", dp * MAXB), ("F", dp * MAXB)] + [(k, dp) for k in FIELDS]
+
+
+class PcgC(ct.Structure):
+    _fields_ = [("x", dp * 3), ("xprev", dp * 3), ("warm", ct.c_int32), ("have2", ct.c_int32),
+                ("iters", ct.c_int32)]
+
+
+class StatsC(ct.Structure):
+    _fields_ = [("steps", ct.c_int64), ("calls", ct.c_int64), ("pcg_iters", ct.c_int64),
+                ("last_iters", ct.c_int32), ("device_ms", ct.c_double), ("device_bytes", ct.c_int64)]
+
+
+def _ptr(a: np.ndarray):
+    if not (a.flags.f_contiguous and a.dtype in (np.float64, np.complex128)):
+        raise ValueError("arrays must be Fortran-contiguous float64/complex128")
+    return a.ctypes.data_as(dp)
+
+
+class PackedDesc:
+    """A DescC with the numpy arrays it points to kept alive."""
+
+    def __init__(self, st: Stepper):
+        d = DescC()
+        gr = st.grid
+        N = st.N
+        if N > MAXB:
+            raise ValueError(f"at most {MAXB} bands")
+        d.abi_version = ABI_VERSION
+        d.nbands = N
+        d.Nx, d.Ny, d.Nz = gr.Nx, gr.Ny, gr.Nz
+        d.x1, d.x2, d.y1, d.y2, d.z1, d.z2 = gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2
+        d.dx, d.dy, d.dz = gr.dx, gr.dy, gr.dz
+        d.dt = st.dt
+        for mu in range(N):
+            for dst, src in ((d.bc[mu], st.bc[mu]), (d.bcs[mu], st.bcs[mu])):
+                dst.kx, dst.ky, dst.kz, dst.c = src.kx, src.ky, src.kz, src.c
+                dst.eps, dst.beta, dst.gamma2 = src.eps, src.beta, src.g2
+                for nu in range(N):
+                    dst.cpl[nu] = src.cpl[nu] if nu != mu else 0.0
+            d.nsub[mu] = st.nsub[mu]
+            d.stages[mu] = st.rkl[mu]
+            d.divisor[mu] = st.divisor[mu]
+            kc = st.rkc[mu] if st.rkc is not None else None
+            d.rkc[mu] = 1 if kc is not None else 0
+            if kc is not None:
+                if len(kc.mu) > MAXS:
+                    raise ValueError(f"at most {MAXS} stages")
+                for j in range(len(kc.mu)):
+                    d.rkc_mu[mu][j] = kc.mu[j]
+                    d.rkc_nu[mu][j] = kc.nu[j]
+                    d.rkc_c[mu][j] = kc.c[j]
+            elif st.rkl[mu] > MAXS:
+                raise ValueError(f"at most {MAXS} stages")
+            for ax in range(3):
+                d.wa[ax][mu] = st.wa[ax][mu]
+        fc = st.fc
+        d.d1, d.sxy, d.sxz, d.syx, d.syz, d.szx, d.szy, d.a = (fc.d1, fc.sxy, fc.sxz, fc.syx, fc.syz, fc.szx,
+                                                               fc.szy, fc.a)
+        d.field_mode = st.field_mode
+        d.selffield_bc = 1 if st.selffield_bc == "bar" else 0
+        d.extrap, d.rotframe, d.hard = int(st.extrap), int(st.rotframe), int(st.hard)
+        d.pcg_tol, d.pcg_maxit = st.pcg_tol, st.pcg_maxit
+        d.pcg_fast, d.pcg_f32, d.pcg_extrap = int(st.pcg_fast), int(st.pcg_f32), int(st.pcg_extrap)
+        keep = []
+
+        def hold(a):
+            a = np.asfortranarray(a, dtype=np.float64)
+            keep.append(a)
+            return _ptr(a)
+
+        mk = st.masks
+        d.f, d.taux, d.tauy, d.tauz = hold(mk.f), hold(mk.tx), hold(mk.ty), hold(mk.tz)
+        if st.field_mode in (FIELD_PCG, FIELD_EXPLICIT_SCALED):
+            d.Dx, d.Dy, d.Dz = (hold(x) for x in st.D)
+        if st.bar is not None:
+            for k in ("hy_z", "hy_y", "hz_z", "hz_y", "hz_x", "hy_x"):
+                setattr(d, "bar_" + k, hold(st.bar[k]))
+        self.c = d
+        self._keep = keep
+
+
+def state_struct(s: State) -> StateC:
+    sc = StateC()
+    for mu in range(s.N):
+        sc.G[mu] = _ptr(s.G[mu])
+        sc.F[mu] = _ptr(s.F[mu])
+    for k in FIELDS:
+        setattr(sc, k, _ptr(getattr(s, k)))
+    return sc
+
+
+def pcg_ranges(gr):
+    """Shapes of the PCG range arrays rx, ry, rz (implicit.jl:52)."""
+    Nx, Ny, Nz = gr.Nx, gr.Ny, gr.Nz
+    return ((Nx - 2, Ny - 1, Nz - 1), (Nx - 1, Ny - 2, Nz - 1), (Nx - 1, Ny - 1, Nz - 2))
+
+
+class PcgState:
+    """Host copy of the PCG history (x, xprev, warm, have2, iters)."""
+
+    def __init__(self, gr):
+        sh = pcg_ranges(gr)
+        self.x = [np.zeros(s, order="F") for s in sh]
+        self.xprev = [np.zeros(s, order="F") for s in sh]
+        self.warm = False
+        self.have2 = False
+        self.iters = 0
+
+    def struct(self) -> PcgC:
+        p = PcgC()
+        for c in range(3):
+            p.x[c] = _ptr(self.x[c])
+            p.xprev[c] = _ptr(self.xprev[c])
+        p.warm, p.have2, p.iters = int(self.warm), int(self.have2), int(self.iters)
+        return p
+
+    def load(self, p: PcgC):
+        self.warm, self.have2, self.iters = bool(p.warm), bool(p.have2), int(p.iters)

@@ -1,0 +1,1188 @@
+// oracle.cpp — CPU restatement of the release-2.0 time step (gtdgl/src/step.jl:13-35).
+//
+// TEST INFRASTRUCTURE ONLY.  This is the parity oracle and the CPU baseline of the HIP stepper:
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it; the product
+// (juliatdgl_amd) never does.
+//
+// It restates, in the reference's loop order and order of additions, the options the HIP
+// stepper runs: N bands; Euler with substeps (extrapolated link roots), RKL1 and damped RKC1
+// stages; KWT; the rotating frame; supercurrent; field increment; the implicit curl-curl by the
+// separable solve or by PCG (exact, fast, Float32 preconditioner); explicit and D-scaled
+// explicit field; links; curl; slab and bar field boundaries; Neumann ghosts; advance.
+// Arrays are Julia-shaped (column-major, 1-based accessors) so every index reads as in the
+// reference.  Complex arithmetic is written out with the formulas of Julia's Base (no fused
+// multiply-add: build with -ffp-contract=off).
+//
+// Pinned by: the golden states of gtdgl/test/golden (k2_R13_explicit, k2_R4_prod, k25_R4_prod:
+// Euler + substeps + extrapolated links, separable and explicit field, slab BC); the closed
+// forms of gtdgl/test/test_physics.jl and TDGLGPU/test/test_invariants.jl (KWT 2x2 solve,
+// amplitude relaxation rates, uniform equilibrium, pure gauge); RKC polynomial identities; the
+// separable solve against a dense inverse of (I + aK); PCG against the separable solve at
+// D = 1; the manufactured rotations of gtdgl/test/rotation_fixture.jl.  The tests that do so
+// are tests/test_oracle_*.py.
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../include/tdglhip.h"
+
+namespace {
+
+struct C {
+  double re, im;
+};
+// Julia Base.complex arithmetic (complex.jl), no contraction
+inline C cmul(C a, C b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline C cadd(C a, C b) { return {a.re + b.re, a.im + b.im}; }
+inline C csub(C a, C b) { return {a.re - b.re, a.im - b.im}; }
+inline C rmul(double x, C a) { return {x * a.re, x * a.im}; }
+inline C conj(C a) { return {a.re, -a.im}; }
+inline double abs2(C a) { return a.re * a.re + a.im * a.im; }
+inline C cis(double x) { return {std::cos(x), std::sin(x)}; }
+inline double angle(C a) { return std::atan2(a.im, a.re); }
+// im * x * y with x real (the frame term): Complex(±0, x) * y
+inline C imx_mul(double x, C y) { return {0.0 * y.re - x * y.im, 0.0 * y.im + x * y.re}; }
+
+template <class T>
+struct A3 {
+  int n1 = 0, n2 = 0, n3 = 0;
+  std::vector<T> v;
+  A3() {}
+  A3(int a, int b, int c) : n1(a), n2(b), n3(c), v((size_t)a * b * c, T{}) {}
+  inline T &operator()(int i, int j, int k) { return v[(size_t)(i - 1) + (size_t)n1 * ((j - 1) + (size_t)n2 * (k - 1))]; }
+  inline const T &operator()(int i, int j, int k) const {
+    return v[(size_t)(i - 1) + (size_t)n1 * ((j - 1) + (size_t)n2 * (k - 1))];
+  }
+  size_t size() const { return v.size(); }
+};
+using A3c = A3<C>;
+using A3r = A3<double>;
+
+const double ROT_MIN2 = 1e-3;  // bands.jl:208
+const double ROT_MAXDT = 0.5;  // bands.jl:209
+
+inline double jclamp(double x, double lo, double hi) { return x > hi ? hi : (x < lo ? lo : x); }
+
+// kwt_update, bands.jl:135-140
+inline C kwt_update(C g, C d, double g2) {
+  double a2 = abs2(g);
+  double q = 1 + g2 * a2;
+  C dp = rmul(std::sqrt(q), d);
+  double s = cmul(conj(g), dp).re / q;
+  return cadd(g, csub(dp, rmul(s, rmul(g2, g))));
+}
+
+// _cis, bands.jl:289
+inline C scis(double a) {
+  if (std::fabs(a) <= 0.05) {
+    double a2 = a * a;
+    return {1 - a2 / 2, a * (1 - a2 / 6)};
+  }
+  return cis(a);
+}
+
+// 1D bases of the separable solve: V on nn nodes (cosine, DCT-II), U on nn-1 edges (sine,
+// DST-I), s the singular values of the forward difference in ascending order
+// (implicit.jl:15-22; the SVD there gives these bases up to signs that cancel in the solve).
+struct Basis {
+  int nn;
+  std::vector<double> V, U, s;  // V (nn x nn), U (nn-1 x nn-1), column-major
+};
+Basis make_basis(int nn) {
+  Basis b;
+  b.nn = nn;
+  b.V.assign((size_t)nn * nn, 0.0);
+  b.U.assign((size_t)(nn - 1) * (nn - 1), 0.0);
+  b.s.assign(nn, 0.0);
+  const double pi = 3.14159265358979323846;
+  for (int k = 0; k < nn; ++k) {
+    b.s[k] = k == 0 ? 0.0 : 2.0 * std::sin(pi * k / (2.0 * nn));
+    for (int i = 0; i < nn; ++i)
+      b.V[i + (size_t)nn * k] = k == 0 ? 1.0 / std::sqrt((double)nn)
+                                       : std::sqrt(2.0 / nn) * std::cos(pi * k * (i + 0.5) / nn);
+  }
+  for (int k = 1; k < nn; ++k)
+    for (int e = 0; e < nn - 1; ++e)
+      b.U[e + (size_t)(nn - 1) * (k - 1)] = -std::sqrt(2.0 / nn) * std::sin(pi * k * (e + 1.0) / nn);
+  return b;
+}
+
+// out = A transformed by M1' along dim 1, M2' along dim 2, M3' along dim 3 (_tx!, implicit.jl:30-37);
+// M_d is (n_d x m_d) column-major; `tr` transposes M_d (the inverse transform).
+template <class R>
+void tx(std::vector<R> &out, const std::vector<R> &A, int n1, int n2, int n3, const std::vector<R> &M1,
+        const std::vector<R> &M2, const std::vector<R> &M3, bool tr) {
+  // square bases: m_d = n_d
+  auto el = [&](const std::vector<R> &M, int n, int a, int b) -> R {  // (M or M') [a, b]
+    return tr ? M[b + (size_t)n * a] : M[a + (size_t)n * b];
+  };
+  std::vector<R> T1((size_t)n1 * n2 * n3), T2((size_t)n1 * n2 * n3);
+  // T1[m, j, k] = sum_n M1[n, m] A[n, j, k]
+  for (int k = 0; k < n3; ++k)
+    for (int j = 0; j < n2; ++j)
+      for (int m = 0; m < n1; ++m) {
+        R acc = 0;
+        for (int n = 0; n < n1; ++n) acc += el(M1, n1, n, m) * A[n + (size_t)n1 * (j + (size_t)n2 * k)];
+        T1[m + (size_t)n1 * (j + (size_t)n2 * k)] = acc;
+      }
+  // T2[i, m, k] = sum_n T1[i, n, k] M2[n, m]
+  for (int k = 0; k < n3; ++k)
+    for (int m = 0; m < n2; ++m)
+      for (int i = 0; i < n1; ++i) {
+        R acc = 0;
+        for (int n = 0; n < n2; ++n) acc += T1[i + (size_t)n1 * (n + (size_t)n2 * k)] * el(M2, n2, n, m);
+        T2[i + (size_t)n1 * (m + (size_t)n2 * k)] = acc;
+      }
+  // out[i, j, m] = sum_n T2[i, j, n] M3[n, m]
+  out.assign((size_t)n1 * n2 * n3, 0);
+  for (int m = 0; m < n3; ++m)
+    for (int j = 0; j < n2; ++j)
+      for (int i = 0; i < n1; ++i) {
+        R acc = 0;
+        for (int n = 0; n < n3; ++n) acc += T2[i + (size_t)n1 * (j + (size_t)n2 * n)] * el(M3, n3, n, m);
+        out[i + (size_t)n1 * (j + (size_t)n2 * m)] = acc;
+      }
+}
+
+struct Sep {
+  int Nx, Ny, Nz;
+  Basis bx, by, bz;
+  double a;
+  int ex[3], ey[3], ez[3];  // extents of rx, ry, rz
+  std::vector<float> V32[3], U32[3], s32[3];
+};
+
+// The mode projector of _sep_solve! (implicit.jl:67-78), in place on X, Y, Z.
+template <class R>
+void modes(std::vector<R> &X, std::vector<R> &Y, std::vector<R> &Z, const Sep &S, const R *sx, const R *sy,
+           const R *sz, R a) {
+  int nx = S.bx.nn, ny = S.by.nn, nz = S.bz.nn;
+  // X: (nx-1, ny, nz), Y: (nx, ny-1, nz), Z: (nx, ny, nz-1)
+  auto iX = [&](int mx, int my, int mz) { return (size_t)mx + (size_t)(nx - 1) * (my + (size_t)ny * mz); };
+  auto iY = [&](int mx, int my, int mz) { return (size_t)mx + (size_t)nx * (my + (size_t)(ny - 1) * mz); };
+  auto iZ = [&](int mx, int my, int mz) { return (size_t)mx + (size_t)nx * (my + (size_t)ny * mz); };
+  for (int mz = 0; mz < nz; ++mz)
+    for (int my = 0; my < ny; ++my)
+      for (int mx = 0; mx < nx; ++mx) {
+        R k1 = sx[mx], k2 = sy[my], k3 = sz[mz];
+        R k2s = k1 * k1 + k2 * k2 + k3 * k3;
+        if (k2s == 0) continue;
+        R x1 = mx > 0 ? X[iX(mx - 1, my, mz)] : R(0);
+        R y1 = my > 0 ? Y[iY(mx, my - 1, mz)] : R(0);
+        R z1 = mz > 0 ? Z[iZ(mx, my, mz - 1)] : R(0);
+        R d = (k1 * x1 + k2 * y1 + k3 * z1) / k2s;
+        R f = R(1) / (R(1) + a * k2s);
+        if (mx > 0) X[iX(mx - 1, my, mz)] = k1 * d + f * (x1 - k1 * d);
+        if (my > 0) Y[iY(mx, my - 1, mz)] = k2 * d + f * (y1 - k2 * d);
+        if (mz > 0) Z[iZ(mx, my, mz - 1)] = k3 * d + f * (z1 - k3 * d);
+      }
+}
+
+// (I + aK)^{-1} applied in place (_sep_solve!, implicit.jl:62-83)
+void sep_solve(const Sep &S, std::vector<double> &BX, std::vector<double> &BY, std::vector<double> &BZ) {
+  std::vector<double> X, Y, Z;
+  tx(X, BX, S.ex[0], S.ex[1], S.ex[2], S.bx.U, S.by.V, S.bz.V, false);
+  tx(Y, BY, S.ey[0], S.ey[1], S.ey[2], S.bx.V, S.by.U, S.bz.V, false);
+  tx(Z, BZ, S.ez[0], S.ez[1], S.ez[2], S.bx.V, S.by.V, S.bz.U, false);
+  modes<double>(X, Y, Z, S, S.bx.s.data(), S.by.s.data(), S.bz.s.data(), S.a);
+  tx(BX, X, S.ex[0], S.ex[1], S.ex[2], S.bx.U, S.by.V, S.bz.V, true);
+  tx(BY, Y, S.ey[0], S.ey[1], S.ey[2], S.bx.V, S.by.U, S.bz.V, true);
+  tx(BZ, Z, S.ez[0], S.ez[1], S.ez[2], S.bx.V, S.by.V, S.bz.U, true);
+}
+
+// The Float32 separable inverse (_precond32!, implicit.jl:353-373)
+void sep_solve32(const Sep &S, std::vector<double> *z, const std::vector<double> *r) {
+  std::vector<float> B[3], T[3];
+  for (int c = 0; c < 3; ++c) {
+    B[c].resize(r[c].size());
+    for (size_t i = 0; i < r[c].size(); ++i) B[c][i] = (float)r[c][i];
+  }
+  tx(T[0], B[0], S.ex[0], S.ex[1], S.ex[2], S.U32[0], S.V32[1], S.V32[2], false);
+  tx(T[1], B[1], S.ey[0], S.ey[1], S.ey[2], S.V32[0], S.U32[1], S.V32[2], false);
+  tx(T[2], B[2], S.ez[0], S.ez[1], S.ez[2], S.V32[0], S.V32[1], S.U32[2], false);
+  modes<float>(T[0], T[1], T[2], S, S.s32[0].data(), S.s32[1].data(), S.s32[2].data(), (float)S.a);
+  tx(B[0], T[0], S.ex[0], S.ex[1], S.ex[2], S.U32[0], S.V32[1], S.V32[2], true);
+  tx(B[1], T[1], S.ey[0], S.ey[1], S.ey[2], S.V32[0], S.U32[1], S.V32[2], true);
+  tx(B[2], T[2], S.ez[0], S.ez[1], S.ez[2], S.V32[0], S.V32[1], S.U32[2], true);
+  for (int c = 0; c < 3; ++c)
+    for (size_t i = 0; i < B[c].size(); ++i) z[c][i] = (double)B[c][i];
+}
+
+double dot(const std::vector<double> &a, const std::vector<double> &b) {
+  double s = 0;
+  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  return s;
+}
+
+}  // namespace
+
+struct orc {
+  tdgl_desc d;
+  int N;
+  int Nx, Ny, Nz, x1, x2, y1, y2, z1, z2;
+  // masks
+  A3r f, tx, ty, tz;
+  A3r Dx, Dy, Dz;
+  std::vector<double> bar[6];
+  // state (state.jl:51-60); the swaps of advance! swap these objects
+  std::vector<A3c> G, F;
+  A3r thx, thy, thz, phx, phy, phz;
+  A3c Ux, Uy, Uz, Vx, Vy, Vz;
+  A3r Jx, Jy, Jz, hx, hy, hz;
+  // workspace (scheme.jl:156-165)
+  A3c W, W2, L[6];
+  A3r Om, R[3];
+  // separable solve / PCG
+  Sep sep;
+  bool have_sep = false;
+  std::vector<double> BX, BY, BZ;
+  A3r P[3], Q[3], H[3];
+  std::vector<double> x[3], r[3], z[3], b[3], xprev[3], zold[3];
+  bool warm = false, have2 = false;
+  int iters = 0;
+  long long total_iters = 0;
+};
+
+namespace {
+
+// Link ranges the band stencil and its ghosts read (bands.jl:349-351)
+struct Rng {
+  int i0, i1, j0, j1, k0, k1;
+};
+Rng rx(const orc &o) { return {o.x1, o.x2, o.y1, o.y2 + 1, o.z1, o.z2 + 1}; }
+Rng ry(const orc &o) { return {o.x1, o.x2 + 1, o.y1, o.y2, o.z1, o.z2 + 1}; }
+Rng rz(const orc &o) { return {o.x1, o.x2 + 1, o.y1, o.y2 + 1, o.z1, o.z2}; }
+template <class F>
+void forr(const Rng &r, F fn) {
+  for (int k = r.k0; k <= r.k1; ++k)
+    for (int j = r.j0; j <= r.j1; ++j)
+      for (int i = r.i0; i <= r.i1; ++i) fn(i, j, k);
+}
+
+// _band_row! over the interior (bands.jl:71-97, one material, no drag, no heat field)
+void band_pass(orc &o, A3c &D, const A3c &S, int mu, const A3c &Lx, const A3c &Ly, const A3c &Lz,
+               const tdgl_bandcoef &cf) {
+  const int N = o.N;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int j = o.y1 + 1; j <= o.y2; ++j)
+      for (int i = o.x1 + 1; i <= o.x2; ++i) {
+        C g = S(i, j, k);
+        double fm = o.f(i, j, k);
+        if (o.d.hard && fm == 0.0) {
+          D(i, j, k) = {0.0, 0.0};
+          continue;
+        }
+        C vxm = conj(Lx(i - 1, j, k)), vym = conj(Ly(i, j - 1, k)), vzm = conj(Lz(i, j, k - 1));
+        C tx_ = rmul(cf.kx, cadd(csub(cmul(Lx(i, j, k), S(i + 1, j, k)), rmul(2.0, g)), cmul(vxm, S(i - 1, j, k))));
+        C ty_ = rmul(cf.ky, cadd(csub(cmul(Ly(i, j, k), S(i, j + 1, k)), rmul(2.0, g)), cmul(vym, S(i, j - 1, k))));
+        C tz_ = rmul(cf.kz, cadd(csub(cmul(Lz(i, j, k), S(i, j, k + 1)), rmul(2.0, g)), cmul(vzm, S(i, j, k - 1))));
+        C nl = rmul(fm * cf.eps - cf.beta * abs2(g), rmul(cf.c, g));
+        C acc = cadd(cadd(cadd(cadd(g, tx_), ty_), tz_), nl);
+        for (int nu = 0; nu < N; ++nu) {
+          if (nu == mu) continue;
+          acc = cadd(acc, rmul(cf.cpl[nu], o.G[nu](i, j, k)));
+        }
+        D(i, j, k) = cf.gamma2 > 0 ? kwt_update(g, csub(acc, g), cf.gamma2) : acc;
+      }
+}
+
+// _frame_rate! without noise (bands.jl:213-234)
+void frame_rate(orc &o, A3r &Om, A3c &Cc, const A3c &Y, double h, double dt) {
+  double wmax = ROT_MAXDT / dt;
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int j = o.y1 + 1; j <= o.y2; ++j)
+      for (int i = o.x1 + 1; i <= o.x2; ++i) {
+        C y = Y(i, j, k);
+        double a2 = abs2(y);
+        double w = a2 > ROT_MIN2 ? jclamp(cmul(conj(y), csub(Cc(i, j, k), y)).im / (a2 * h), -wmax, wmax) : 0.0;
+        Om(i, j, k) = w;
+        Cc(i, j, k) = csub(Cc(i, j, k), imx_mul(w * h, y));
+      }
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int j = o.y1 + 1; j <= o.y2; ++j) {
+      Om(o.x1, j, k) = Om(o.x1 + 1, j, k);
+      Om(o.x2 + 1, j, k) = Om(o.x2, j, k);
+    }
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int i = o.x1; i <= o.x2 + 1; ++i) {
+      Om(i, o.y1, k) = Om(i, o.y1 + 1, k);
+      Om(i, o.y2 + 1, k) = Om(i, o.y2, k);
+    }
+  for (int j = o.y1; j <= o.y2 + 1; ++j)
+    for (int i = o.x1; i <= o.x2 + 1; ++i) {
+      Om(i, j, o.z1) = Om(i, j, o.z1 + 1);
+      Om(i, j, o.z2 + 1) = Om(i, j, o.z2);
+    }
+}
+
+// _frame_term! without noise (bands.jl:274-286)
+void frame_term(orc &o, A3c &Cc, const A3c &S, const A3r &Om, double h) {
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int j = o.y1 + 1; j <= o.y2; ++j)
+      for (int i = o.x1 + 1; i <= o.x2; ++i) Cc(i, j, k) = csub(Cc(i, j, k), imx_mul(Om(i, j, k) * h, S(i, j, k)));
+}
+
+// _frame_turn! (bands.jl:321-327)
+void frame_turn(orc &o, A3c &F, const A3r &Om, double dt) {
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int j = o.y1 + 1; j <= o.y2; ++j)
+      for (int i = o.x1 + 1; i <= o.x2; ++i) F(i, j, k) = cmul(F(i, j, k), cis(Om(i, j, k) * dt));
+}
+
+// _frame_links! (bands.jl:292-297)
+void frame_links(orc &o, A3c &Lx, A3c &Ly, A3c &Lz, const A3r &Om, double tau) {
+  forr(rx(o), [&](int i, int j, int k) { Lx(i, j, k) = cmul(Lx(i, j, k), scis((Om(i + 1, j, k) - Om(i, j, k)) * tau)); });
+  forr(ry(o), [&](int i, int j, int k) { Ly(i, j, k) = cmul(Ly(i, j, k), scis((Om(i, j + 1, k) - Om(i, j, k)) * tau)); });
+  forr(rz(o), [&](int i, int j, int k) { Lz(i, j, k) = cmul(Lz(i, j, k), scis((Om(i, j, k + 1) - Om(i, j, k)) * tau)); });
+}
+
+// _link_rate! (bands.jl:303-310)
+void link_rate(orc &o, A3r &Rr, const A3c &V, const A3c &U, const A3r &Om, double dt, const Rng &rg, int oi, int oj,
+               int ok) {
+  forr(rg, [&](int i, int j, int k) {
+    C q = cmul(V(i, j, k), conj(U(i, j, k)));
+    double sf = -q.im;
+    double a = q.re > 0 && std::fabs(sf) <= 0.05 ? -(sf + sf * sf * sf / 6) : angle(q);
+    Rr(i, j, k) = a + (Om(i + oi, j + oj, k + ok) - Om(i, j, k)) * dt;
+  });
+}
+
+// _rotlinks! (bands.jl:311-316)
+void rotlinks(A3c &L, const A3c &V, const A3r &Rr, double c, const Rng &rg) {
+  forr(rg, [&](int i, int j, int k) { L(i, j, k) = cmul(V(i, j, k), scis(Rr(i, j, k) * c)); });
+}
+
+// _rot! (bands.jl:355-369)
+void rot(A3c &L, A3c &Rc, const A3c &V, const A3c &U, int n, const Rng &rg) {
+  forr(rg, [&](int i, int j, int k) {
+    C v = V(i, j, k);
+    L(i, j, k) = v;
+    C q = cmul(v, conj(U(i, j, k)));
+    double sf = -q.im;
+    if (q.re > 0 && std::fabs(sf) <= 0.05) {
+      double a = (sf + sf * sf * sf / 6) / n;
+      double a2 = a * a;
+      Rc(i, j, k) = {1 - a2 / 2, -a * (1 - a2 / 6)};
+    } else {
+      Rc(i, j, k) = cis(angle(q) / n);
+    }
+  });
+}
+
+// _advance_links! (bands.jl:371-376)
+void advance_links(orc &o, A3c &Lx, A3c &Ly, A3c &Lz, const A3c &Rx_, const A3c &Ry_, const A3c &Rz_) {
+  forr(rx(o), [&](int i, int j, int k) { Lx(i, j, k) = cmul(Lx(i, j, k), Rx_(i, j, k)); });
+  forr(ry(o), [&](int i, int j, int k) { Ly(i, j, k) = cmul(Ly(i, j, k), Ry_(i, j, k)); });
+  forr(rz(o), [&](int i, int j, int k) { Lz(i, j, k) = cmul(Lz(i, j, k), Rz_(i, j, k)); });
+}
+
+// _rotfrac! (bands.jl:409-423)
+void rotfrac(A3c &L, const A3c &V, const A3c &U, double c, const Rng &rg) {
+  forr(rg, [&](int i, int j, int k) {
+    C v = V(i, j, k);
+    C q = cmul(v, conj(U(i, j, k)));
+    double sf = -q.im;
+    if (q.re > 0 && std::fabs(sf) <= 0.05) {
+      double a = (sf + sf * sf * sf / 6) * c;
+      double a2 = a * a;
+      L(i, j, k) = cmul(v, C{1 - a2 / 2, -a * (1 - a2 / 6)});
+    } else {
+      L(i, j, k) = cmul(v, cis(angle(q) * c));
+    }
+  });
+}
+
+// _combine! (bands.jl:425-430)
+void combine(orc &o, A3c &Cc, const A3c &A, double mu, double nu) {
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int j = o.y1 + 1; j <= o.y2; ++j)
+      for (int i = o.x1 + 1; i <= o.x2; ++i) Cc(i, j, k) = cadd(rmul(mu, Cc(i, j, k)), rmul(nu, A(i, j, k)));
+}
+
+// ghosts! (bands.jl:524-539)
+void ghosts(orc &o, A3c &X, const A3c &Vx, const A3c &Vy, const A3c &Vz) {
+  for (int k = o.z1; k <= o.z2 + 1; ++k)
+    for (int j = o.y1; j <= o.y2 + 1; ++j) {
+      X(o.x1, j, k) = cmul(Vx(o.x1, j, k), X(o.x1 + 1, j, k));
+      X(o.x2 + 1, j, k) = cmul(conj(Vx(o.x2, j, k)), X(o.x2, j, k));
+    }
+  for (int k = o.z1; k <= o.z2 + 1; ++k)
+    for (int i = o.x1; i <= o.x2 + 1; ++i) {
+      X(i, o.y1, k) = cmul(Vy(i, o.y1, k), X(i, o.y1 + 1, k));
+      X(i, o.y2 + 1, k) = cmul(conj(Vy(i, o.y2, k)), X(i, o.y2, k));
+    }
+  for (int j = o.y1; j <= o.y2 + 1; ++j)
+    for (int i = o.x1; i <= o.x2 + 1; ++i) {
+      X(i, j, o.z1) = cmul(Vz(i, j, o.z1), X(i, j, o.z1 + 1));
+      X(i, j, o.z2 + 1) = cmul(conj(Vz(i, j, o.z2)), X(i, j, o.z2));
+    }
+}
+
+// _band_substeps! (bands.jl:378-406)
+void band_substeps(orc &o, int mu, int n) {
+  const tdgl_bandcoef &cf = o.d.bcs[mu];
+  A3c &F = o.F[mu];
+  A3c &W = o.W;
+  A3c *Lx, *Ly, *Lz;
+  if (o.d.extrap) {
+    Lx = &o.L[0];
+    Ly = &o.L[1];
+    Lz = &o.L[2];
+    rot(o.L[0], o.L[3], o.Vx, o.Ux, n, rx(o));
+    rot(o.L[1], o.L[4], o.Vy, o.Uy, n, ry(o));
+    rot(o.L[2], o.L[5], o.Vz, o.Uz, n, rz(o));
+  } else {
+    Lx = &o.Vx;
+    Ly = &o.Vy;
+    Lz = &o.Vz;
+  }
+  const A3c *src = &o.G[mu];
+  bool rt = o.d.rotframe;
+  double h = o.d.dt / n;
+  for (int m = 1; m <= n; ++m) {
+    A3c *dst = ((n - m) % 2 == 0) ? &F : &W;
+    band_pass(o, *dst, *src, mu, *Lx, *Ly, *Lz, cf);
+    if (rt) {
+      if (m == 1)
+        frame_rate(o, o.Om, *dst, *src, h, o.d.dt);
+      else
+        frame_term(o, *dst, *src, o.Om, h);
+    }
+    if (m < n) {
+      if (o.d.extrap) advance_links(o, *Lx, *Ly, *Lz, o.L[3], o.L[4], o.L[5]);
+      if (rt) frame_links(o, *Lx, *Ly, *Lz, o.Om, h);
+      ghosts(o, *dst, *Lx, *Ly, *Lz);
+    }
+    src = dst;
+  }
+  if (rt) frame_turn(o, F, o.Om, o.d.dt);
+}
+
+// _band_rkl! without noise (bands.jl:464-521)
+void band_rkl(orc &o, int mu, int ns) {
+  const tdgl_bandcoef &cf = o.d.bcs[mu];
+  bool kc = o.d.rkc[mu] != 0;
+  A3c *Lx, *Ly, *Lz;
+  if (o.d.extrap) {
+    o.L[0] = o.Vx;
+    o.L[1] = o.Vy;
+    o.L[2] = o.Vz;
+    Lx = &o.L[0];
+    Ly = &o.L[1];
+    Lz = &o.L[2];
+  } else {
+    Lx = &o.Vx;
+    Ly = &o.Vy;
+    Lz = &o.Vz;
+  }
+  A3c *buf[3] = {&o.W, &o.W2, &o.F[mu]};
+  bool rt = o.d.rotframe;
+  // h = dt/q, q = ns(ns+1)/2 (RKL1) or RKCCoef.q (bands.jl:468, 477)
+  double qd = kc ? o.d.divisor[mu] : (double)(ns * (ns + 1) / 2);
+  double h = o.d.dt / qd;
+  const A3c *A = &o.G[mu];
+  const A3c *B = &o.G[mu];
+  A3c *Cc = buf[0];
+  for (int j = 1; j <= ns; ++j) {
+    Cc = buf[(j - 1) % 3];
+    band_pass(o, *Cc, *B, mu, *Lx, *Ly, *Lz, cf);
+    if (rt) {
+      if (j == 1)
+        frame_rate(o, o.Om, *Cc, *B, h, o.d.dt);
+      else
+        frame_term(o, *Cc, *B, o.Om, h);
+    }
+    if (rt && j == 1 && o.d.extrap && ns > 1) {
+      link_rate(o, o.R[0], o.Vx, o.Ux, o.Om, o.d.dt, rx(o), 1, 0, 0);
+      link_rate(o, o.R[1], o.Vy, o.Uy, o.Om, o.d.dt, ry(o), 0, 1, 0);
+      link_rate(o, o.R[2], o.Vz, o.Uz, o.Om, o.d.dt, rz(o), 0, 0, 1);
+    }
+    if (j >= 2) {
+      double mj, nj;
+      if (kc) {
+        mj = o.d.rkc_mu[mu][j - 1];
+        nj = o.d.rkc_nu[mu][j - 1];
+      } else {
+        mj = (2.0 * j - 1) / j;
+        nj = (1.0 - j) / j;
+      }
+      combine(o, *Cc, *A, mj, nj);
+    }
+    if (j < ns) {
+      if (o.d.extrap) {
+        double c = kc ? o.d.rkc_c[mu][j - 1] : (double)(j * (j + 1)) / (double)(ns * (ns + 1));
+        if (rt) {
+          rotlinks(*Lx, o.Vx, o.R[0], c, rx(o));
+          rotlinks(*Ly, o.Vy, o.R[1], c, ry(o));
+          rotlinks(*Lz, o.Vz, o.R[2], c, rz(o));
+        } else {
+          rotfrac(*Lx, o.Vx, o.Ux, c, rx(o));
+          rotfrac(*Ly, o.Vy, o.Uy, c, ry(o));
+          rotfrac(*Lz, o.Vz, o.Uz, c, rz(o));
+        }
+      }
+      ghosts(o, *Cc, *Lx, *Ly, *Lz);
+    }
+    A = B;
+    B = Cc;
+  }
+  if (Cc != &o.F[mu]) o.F[mu] = *Cc;
+  if (rt) frame_turn(o, o.F[mu], o.Om, o.d.dt);
+}
+
+// update_bands! (bands.jl:330-346)
+void update_bands(orc &o) {
+  for (int mu = 0; mu < o.N; ++mu) {
+    int n = o.d.nsub[mu];
+    if (o.d.stages[mu] > 0) {
+      band_rkl(o, mu, o.d.stages[mu]);
+    } else if (n == 1) {
+      band_pass(o, o.F[mu], o.G[mu], mu, o.Vx, o.Vy, o.Vz, o.d.bc[mu]);
+      if (o.d.rotframe) {
+        frame_rate(o, o.Om, o.F[mu], o.G[mu], o.d.dt, o.d.dt);
+        frame_turn(o, o.F[mu], o.Om, o.d.dt);
+      }
+    } else {
+      band_substeps(o, mu, n);
+    }
+  }
+}
+
+// supercurrent! (field.jl:19-46), one material, no drag
+inline double jlink(const orc &o, int ax, C v, int i, int j, int k, int i2, int j2, int k2) {
+  double acc = o.d.wa[ax][0] * cmul(cmul(conj(o.G[0](i, j, k)), v), o.G[0](i2, j2, k2)).im;
+  for (int mu = 1; mu < o.N; ++mu) acc += o.d.wa[ax][mu] * cmul(cmul(conj(o.G[mu](i, j, k)), v), o.G[mu](i2, j2, k2)).im;
+  return acc;
+}
+void supercurrent(orc &o) {
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+#pragma omp parallel for schedule(static)
+  for (int k = 2; k <= Nz; ++k) {
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 1; i <= Nx; ++i) o.Jx(i, j, k) = jlink(o, 0, o.Vx(i, j, k), i, j, k, i + 1, j, k);
+    for (int j = 1; j <= Ny; ++j)
+      for (int i = 2; i <= Nx; ++i) o.Jy(i, j, k) = jlink(o, 1, o.Vy(i, j, k), i, j, k, i, j + 1, k);
+  }
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k <= Nz; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 2; i <= Nx; ++i) o.Jz(i, j, k) = jlink(o, 2, o.Vz(i, j, k), i, j, k, i, j, k + 1);
+}
+
+// field_increment! (field.jl:84-102)
+void field_increment(orc &o) {
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+  const tdgl_desc &d = o.d;
+#pragma omp parallel for schedule(static)
+  for (int k = 2; k <= Nz; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 1; i <= Nx; ++i)
+        o.phx(i, j, k) = ((o.thx(i, j, k) + (d.d1 * o.tx(i, j, k)) * o.Jx(i, j, k)) - d.sxy * (o.hz(i, j, k) - o.hz(i, j - 1, k))) +
+                         d.sxz * (o.hy(i, j, k) - o.hy(i, j, k - 1));
+#pragma omp parallel for schedule(static)
+  for (int k = 2; k <= Nz; ++k)
+    for (int j = 1; j <= Ny; ++j)
+      for (int i = 2; i <= Nx; ++i)
+        o.phy(i, j, k) = ((o.thy(i, j, k) + (d.d1 * o.ty(i, j, k)) * o.Jy(i, j, k)) - d.syz * (o.hx(i, j, k) - o.hx(i, j, k - 1))) +
+                         d.syx * (o.hz(i, j, k) - o.hz(i - 1, j, k));
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k <= Nz; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 2; i <= Nx; ++i)
+        o.phz(i, j, k) = ((o.thz(i, j, k) + (d.d1 * o.tz(i, j, k)) * o.Jz(i, j, k)) - d.szx * (o.hy(i, j, k) - o.hy(i - 1, j, k))) +
+                         d.szy * (o.hx(i, j, k) - o.hx(i, j - 1, k));
+}
+
+// gather / scatter of the ranges of the separable solve: rx = (2:Nx-1, 2:Ny, 2:Nz), ...
+void gather_diff(std::vector<double> &B, const A3r &ph, const A3r &th, int i0, int i1, int j0, int j1, int k0, int k1) {
+  size_t p = 0;
+  B.resize((size_t)(i1 - i0 + 1) * (j1 - j0 + 1) * (k1 - k0 + 1));
+  for (int k = k0; k <= k1; ++k)
+    for (int j = j0; j <= j1; ++j)
+      for (int i = i0; i <= i1; ++i) B[p++] = ph(i, j, k) - th(i, j, k);
+}
+void scatter_add(A3r &ph, const A3r &th, const std::vector<double> &B, int i0, int i1, int j0, int j1, int k0, int k1) {
+  size_t p = 0;
+  for (int k = k0; k <= k1; ++k)
+    for (int j = j0; j <= j1; ++j)
+      for (int i = i0; i <= i1; ++i) ph(i, j, k) = th(i, j, k) + B[p++];
+}
+
+// implicit_curl!(s, ::SepSolver) (implicit.jl:91-104)
+void implicit_sep(orc &o) {
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+  gather_diff(o.BX, o.phx, o.thx, 2, Nx - 1, 2, Ny, 2, Nz);
+  gather_diff(o.BY, o.phy, o.thy, 2, Nx, 2, Ny - 1, 2, Nz);
+  gather_diff(o.BZ, o.phz, o.thz, 2, Nx, 2, Ny, 2, Nz - 1);
+  sep_solve(o.sep, o.BX, o.BY, o.BZ);
+  scatter_add(o.phx, o.thx, o.BX, 2, Nx - 1, 2, Ny, 2, Nz);
+  scatter_add(o.phy, o.thy, o.BY, 2, Nx, 2, Ny - 1, 2, Nz);
+  scatter_add(o.phz, o.thz, o.BZ, 2, Nx, 2, Ny, 2, Nz - 1);
+}
+
+// _apply_A! (implicit.jl:212-236)
+void apply_A(orc &o) {
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+  const tdgl_desc &d = o.d;
+  A3r &px = o.P[0], &py = o.P[1], &pz = o.P[2];
+  A3r &qx = o.Q[0], &qy = o.Q[1], &qz = o.Q[2];
+  A3r &hx = o.H[0], &hy = o.H[1], &hz = o.H[2];
+  for (int k = 2; k <= Nz - 1; ++k)
+    for (int j = 2; j <= Ny - 1; ++j)
+      for (int i = 2; i <= Nx; ++i)
+        hx(i, j, k) = ((pz(i, j + 1, k) - pz(i, j, k)) - py(i, j, k + 1) + py(i, j, k)) / (d.dy * d.dz);
+  for (int k = 2; k <= Nz - 1; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 2; i <= Nx - 1; ++i)
+        hy(i, j, k) = ((px(i, j, k + 1) - px(i, j, k)) - pz(i + 1, j, k) + pz(i, j, k)) / (d.dz * d.dx);
+  for (int k = 2; k <= Nz; ++k)
+    for (int j = 2; j <= Ny - 1; ++j)
+      for (int i = 2; i <= Nx - 1; ++i)
+        hz(i, j, k) = ((py(i + 1, j, k) - py(i, j, k)) - px(i, j + 1, k) + px(i, j, k)) / (d.dx * d.dy);
+  for (int k = 2; k <= Nz; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 2; i <= Nx - 1; ++i)
+        qx(i, j, k) = o.Dx(i, j, k) * px(i, j, k) + d.sxy * (hz(i, j, k) - hz(i, j - 1, k)) - d.sxz * (hy(i, j, k) - hy(i, j, k - 1));
+  for (int k = 2; k <= Nz; ++k)
+    for (int j = 2; j <= Ny - 1; ++j)
+      for (int i = 2; i <= Nx; ++i)
+        qy(i, j, k) = o.Dy(i, j, k) * py(i, j, k) + d.syz * (hx(i, j, k) - hx(i, j, k - 1)) - d.syx * (hz(i, j, k) - hz(i - 1, j, k));
+  for (int k = 2; k <= Nz - 1; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 2; i <= Nx; ++i)
+        qz(i, j, k) = o.Dz(i, j, k) * pz(i, j, k) + d.szx * (hy(i, j, k) - hy(i - 1, j, k)) - d.szy * (hx(i, j, k) - hx(i, j - 1, k));
+}
+
+struct R3 {
+  int i0, i1, j0, j1, k0, k1;
+};
+R3 prange(const orc &o, int c) {
+  if (c == 0) return {2, o.Nx - 1, 2, o.Ny, 2, o.Nz};
+  if (c == 1) return {2, o.Nx, 2, o.Ny - 1, 2, o.Nz};
+  return {2, o.Nx, 2, o.Ny, 2, o.Nz - 1};
+}
+void scatter_full(A3r &full, const std::vector<double> &v, const R3 &r) {
+  std::fill(full.v.begin(), full.v.end(), 0.0);
+  size_t p = 0;
+  for (int k = r.k0; k <= r.k1; ++k)
+    for (int j = r.j0; j <= r.j1; ++j)
+      for (int i = r.i0; i <= r.i1; ++i) full(i, j, k) = v[p++];
+}
+template <class F>
+void over_range(const R3 &r, F fn) {
+  size_t p = 0;
+  for (int k = r.k0; k <= r.k1; ++k)
+    for (int j = r.j0; j <= r.j1; ++j)
+      for (int i = r.i0; i <= r.i1; ++i) fn(p++, i, j, k);
+}
+double dot3(const std::vector<double> *a, const std::vector<double> *b) { return dot(a[0], b[0]) + dot(a[1], b[1]) + dot(a[2], b[2]); }
+
+void precond(orc &o, std::vector<double> *zz, const std::vector<double> *rr, bool f32) {
+  if (f32) {
+    sep_solve32(o.sep, zz, rr);
+    return;
+  }
+  o.BX = rr[0];
+  o.BY = rr[1];
+  o.BZ = rr[2];
+  sep_solve(o.sep, o.BX, o.BY, o.BZ);
+  zz[0] = o.BX;
+  zz[1] = o.BY;
+  zz[2] = o.BZ;
+}
+
+// _pcg_zero_update! (implicit.jl:251-271)
+void pcg_zero_update(orc &o) {
+  if (o.d.pcg_fast || o.d.pcg_f32) {
+    if (o.warm && (o.d.pcg_extrap && o.d.pcg_fast)) {
+      for (int c = 0; c < 3; ++c) o.xprev[c] = o.x[c];
+      o.have2 = true;
+    } else {
+      o.have2 = false;
+    }
+  }
+  for (int c = 0; c < 3; ++c) std::fill(o.x[c].begin(), o.x[c].end(), 0.0);
+  o.warm = true;
+  o.iters = 0;
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+  for (int k = 2; k <= Nz; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 2; i <= Nx - 1; ++i) o.phx(i, j, k) = o.thx(i, j, k);
+  for (int k = 2; k <= Nz; ++k)
+    for (int j = 2; j <= Ny - 1; ++j)
+      for (int i = 2; i <= Nx; ++i) o.phy(i, j, k) = o.thy(i, j, k);
+  for (int k = 2; k <= Nz - 1; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 2; i <= Nx; ++i) o.phz(i, j, k) = o.thz(i, j, k);
+}
+
+// implicit_curl!(s, ::PCGSolver) and _implicit_curl_fast! (implicit.jl:279-334, 376-446)
+void implicit_pcg(orc &o) {
+  const tdgl_desc &d = o.d;
+  const bool fast = d.pcg_fast || d.pcg_f32;  // PCGFast present
+  const bool f32 = d.pcg_f32 != 0;
+  const bool extrap = d.pcg_extrap && d.pcg_fast;
+  R3 rg[3] = {prange(o, 0), prange(o, 1), prange(o, 2)};
+  A3r *ph[3] = {&o.phx, &o.phy, &o.phz};
+  A3r *th[3] = {&o.thx, &o.thy, &o.thz};
+  A3r *Dd[3] = {&o.Dx, &o.Dy, &o.Dz};
+  for (int c = 0; c < 3; ++c) {
+    o.b[c].resize((size_t)(rg[c].i1 - rg[c].i0 + 1) * (rg[c].j1 - rg[c].j0 + 1) * (rg[c].k1 - rg[c].k0 + 1));
+    over_range(rg[c], [&](size_t p, int i, int j, int k) { o.b[c][p] = (*ph[c])(i, j, k) - (*th[c])(i, j, k); });
+  }
+  for (int c = 0; c < 3; ++c)
+    for (size_t i = 0; i < ph[c]->v.size(); ++i) ph[c]->v[i] = th[c]->v[i] + (ph[c]->v[i] - th[c]->v[i]) / Dd[c]->v[i];
+  double nb = std::sqrt(dot3(o.b, o.b));
+  if (nb == 0) {
+    pcg_zero_update(o);
+    return;
+  }
+  if (!fast) {
+    if (!o.warm) precond(o, o.x, o.b, false);
+  } else {
+    if (!o.warm) {
+      precond(o, o.x, o.b, false);
+      o.have2 = false;
+    } else if (!extrap) {
+    } else if (o.have2) {
+      for (int c = 0; c < 3; ++c)
+        for (size_t i = 0; i < o.x[c].size(); ++i) {
+          double v = o.x[c][i];
+          o.x[c][i] = 2 * v - o.xprev[c][i];
+          o.xprev[c][i] = v;
+        }
+    } else {
+      for (int c = 0; c < 3; ++c) o.xprev[c] = o.x[c];
+      o.have2 = true;
+    }
+  }
+  o.warm = true;
+  for (int c = 0; c < 3; ++c) scatter_full(o.P[c], o.x[c], rg[c]);
+  apply_A(o);
+  for (int c = 0; c < 3; ++c) {
+    o.r[c].resize(o.b[c].size());
+    over_range(rg[c], [&](size_t p, int i, int j, int k) { o.r[c][p] = o.b[c][p] - o.Q[c](i, j, k); });
+  }
+  int it = 0;
+  if (nb > 0 && std::sqrt(dot3(o.r, o.r)) > d.pcg_tol * nb) {
+    for (int c = 0; c < 3; ++c) o.z[c].resize(o.r[c].size());
+    precond(o, o.z, o.r, fast && f32);
+    for (int c = 0; c < 3; ++c) scatter_full(o.P[c], o.z[c], rg[c]);
+    if (fast && f32)
+      for (int c = 0; c < 3; ++c) o.zold[c] = o.z[c];
+    double rz = dot3(o.r, o.z);
+    while (it < d.pcg_maxit) {
+      ++it;
+      apply_A(o);
+      double pAp = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        double s = 0;
+        over_range(rg[c], [&](size_t, int i, int j, int k) { s += o.P[c](i, j, k) * o.Q[c](i, j, k); });
+        pAp += s;
+      }
+      double alpha = rz / pAp;
+      for (int c = 0; c < 3; ++c)
+        over_range(rg[c], [&](size_t p, int i, int j, int k) {
+          o.x[c][p] += alpha * o.P[c](i, j, k);
+          o.r[c][p] -= alpha * o.Q[c](i, j, k);
+        });
+      if (std::sqrt(dot3(o.r, o.r)) <= d.pcg_tol * nb) break;
+      precond(o, o.z, o.r, fast && f32);
+      double rz2 = dot3(o.r, o.z);
+      double beta;
+      if (fast && f32) {
+        beta = (rz2 - dot3(o.r, o.zold)) / rz;
+        for (int c = 0; c < 3; ++c) o.zold[c] = o.z[c];
+      } else {
+        beta = rz2 / rz;
+      }
+      rz = rz2;
+      for (int c = 0; c < 3; ++c)
+        over_range(rg[c], [&](size_t p, int i, int j, int k) { o.P[c](i, j, k) = o.z[c][p] + beta * o.P[c](i, j, k); });
+    }
+  }
+  o.iters = it;
+  o.total_iters += it;
+  for (int c = 0; c < 3; ++c)
+    over_range(rg[c], [&](size_t p, int i, int j, int k) { (*ph[c])(i, j, k) = (*th[c])(i, j, k) + o.x[c][p]; });
+}
+
+// scale_increment! (implicit.jl:337-342)
+void scale_increment(orc &o) {
+  A3r *ph[3] = {&o.phx, &o.phy, &o.phz};
+  A3r *th[3] = {&o.thx, &o.thy, &o.thz};
+  A3r *Dd[3] = {&o.Dx, &o.Dy, &o.Dz};
+  for (int c = 0; c < 3; ++c)
+    for (size_t i = 0; i < ph[c]->v.size(); ++i) ph[c]->v[i] = th[c]->v[i] + (ph[c]->v[i] - th[c]->v[i]) / Dd[c]->v[i];
+}
+
+// links! (field.jl:105-121)
+void links(orc &o) {
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k <= Nz; ++k) {
+    if (2 <= k && k <= Nz) {
+      for (int j = 2; j <= Ny; ++j)
+        for (int i = 1; i <= Nx; ++i) o.Ux(i, j, k) = cis(-o.phx(i, j, k));
+      for (int j = 1; j <= Ny; ++j)
+        for (int i = 2; i <= Nx; ++i) o.Uy(i, j, k) = cis(-o.phy(i, j, k));
+    }
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 2; i <= Nx; ++i) o.Uz(i, j, k) = cis(-o.phz(i, j, k));
+  }
+}
+
+// curl_h! (field.jl:124-137)
+void curl_h(orc &o) {
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+  const double dx = o.d.dx, dy = o.d.dy, dz = o.d.dz;
+  for (int k = 2; k <= Nz - 1; ++k)
+    for (int j = 2; j <= Ny - 1; ++j)
+      for (int i = 2; i <= Nx; ++i)
+        o.hx(i, j, k) = ((o.phz(i, j + 1, k) - o.phz(i, j, k)) - o.phy(i, j, k + 1) + o.phy(i, j, k)) / (dy * dz);
+  for (int k = 2; k <= Nz - 1; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 2; i <= Nx - 1; ++i)
+        o.hy(i, j, k) = ((o.phx(i, j, k + 1) - o.phx(i, j, k)) - o.phz(i + 1, j, k) + o.phz(i, j, k)) / (dz * dx);
+  for (int k = 2; k <= Nz; ++k)
+    for (int j = 2; j <= Ny - 1; ++j)
+      for (int i = 2; i <= Nx - 1; ++i)
+        o.hz(i, j, k) = ((o.phy(i + 1, j, k) - o.phy(i, j, k)) - o.phx(i, j + 1, k) + o.phx(i, j, k)) / (dx * dy);
+}
+
+// field_bc! (field.jl:189-228), no tilt, no open boundary
+void field_bc(orc &o, double H, double HJ) {
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+  A3r &hx = o.hx, &hy = o.hy, &hz = o.hz;
+  for (int j = 1; j <= Ny; ++j)
+    for (int i = 1; i <= Nx; ++i) hz(i, j, 1) = hz(i, j, Nz + 1) = H;
+  for (int k = 1; k <= Nz + 1; ++k)
+    for (int j = 1; j <= Ny; ++j) hz(1, j, k) = hz(Nx, j, k) = H;
+  for (int k = 1; k <= Nz + 1; ++k)
+    for (int i = 1; i <= Nx; ++i) hz(i, 1, k) = hz(i, Ny, k) = H;
+  for (int k = 1; k <= Nz; ++k)
+    for (int j = 1; j <= Ny; ++j) hx(1, j, k) = hx(Nx + 1, j, k) = 0.0;
+  for (int k = 1; k <= Nz; ++k)
+    for (int i = 1; i <= Nx + 1; ++i) hx(i, 1, k) = hx(i, Ny, k) = 0.0;
+  for (int k = 1; k <= Nz; ++k)
+    for (int i = 1; i <= Nx; ++i) hy(i, 1, k) = hy(i, Ny + 1, k) = 0.0;
+  if (o.d.selffield_bc == TDGL_BC_SLAB) {
+    for (int j = 1; j <= Ny + 1; ++j)
+      for (int i = 1; i <= Nx; ++i) {
+        hy(i, j, 1) = -HJ;
+        hy(i, j, Nz) = +HJ;
+      }
+    return;
+  }
+  const std::vector<double> &hy_z = o.bar[0], &hy_y = o.bar[1], &hz_z = o.bar[2], &hz_y = o.bar[3], &hz_x = o.bar[4],
+                            &hy_x = o.bar[5];
+  // hy_z (Ny+1, 2)
+  for (int c = 0; c < 2; ++c) {
+    int k = c == 0 ? 1 : Nz;
+    for (int j = 1; j <= Ny + 1; ++j)
+      for (int i = 1; i <= Nx; ++i) hy(i, j, k) = HJ * hy_z[(j - 1) + (size_t)(Ny + 1) * c];
+  }
+  // hy_y (2, Nz)
+  for (int c = 0; c < 2; ++c) {
+    int j = c == 0 ? 1 : Ny + 1;
+    for (int k = 1; k <= Nz; ++k)
+      for (int i = 1; i <= Nx; ++i) hy(i, j, k) = HJ * hy_y[c + 2 * (size_t)(k - 1)];
+  }
+  // hz_z (Ny, 2)
+  for (int c = 0; c < 2; ++c) {
+    int k = c == 0 ? 1 : Nz + 1;
+    for (int j = 1; j <= Ny; ++j)
+      for (int i = 1; i <= Nx; ++i) hz(i, j, k) = H + HJ * hz_z[(j - 1) + (size_t)Ny * c];
+  }
+  // hz_y (2, Nz+1)
+  for (int c = 0; c < 2; ++c) {
+    int j = c == 0 ? 1 : Ny;
+    for (int k = 1; k <= Nz + 1; ++k)
+      for (int i = 1; i <= Nx; ++i) hz(i, j, k) = H + HJ * hz_y[c + 2 * (size_t)(k - 1)];
+  }
+  // hz_x (Ny, Nz+1)
+  for (int c = 0; c < 2; ++c) {
+    int i = c == 0 ? 1 : Nx;
+    for (int k = 1; k <= Nz + 1; ++k)
+      for (int j = 1; j <= Ny; ++j) hz(i, j, k) = H + HJ * hz_x[(j - 1) + (size_t)Ny * (k - 1)];
+  }
+  // hy_x (Ny+1, Nz)
+  for (int c = 0; c < 2; ++c) {
+    int i = c == 0 ? 1 : Nx;
+    for (int k = 1; k <= Nz; ++k)
+      for (int j = 1; j <= Ny + 1; ++j) hy(i, j, k) = HJ * hy_x[(j - 1) + (size_t)(Ny + 1) * (k - 1)];
+  }
+}
+
+// neumann! (field.jl:231-245) and advance! (field.jl:248-263)
+void neumann(orc &o) {
+  for (int mu = 0; mu < o.N; ++mu) ghosts(o, o.F[mu], o.Ux, o.Uy, o.Uz);
+}
+void advance(orc &o) {
+  for (int mu = 0; mu < o.N; ++mu)
+    for (int k = o.z1; k <= o.z2 + 1; ++k)
+      for (int j = o.y1; j <= o.y2 + 1; ++j)
+        for (int i = o.x1; i <= o.x2 + 1; ++i) o.G[mu](i, j, k) = o.F[mu](i, j, k);
+  std::swap(o.thx, o.phx);
+  std::swap(o.thy, o.phy);
+  std::swap(o.thz, o.phz);
+  std::swap(o.Vx, o.Ux);
+  std::swap(o.Vy, o.Uy);
+  std::swap(o.Vz, o.Uz);
+}
+
+void step(orc &o, double H, double HJ) {
+  update_bands(o);
+  supercurrent(o);
+  field_increment(o);
+  switch (o.d.field_mode) {
+    case TDGL_FIELD_SEPARABLE: implicit_sep(o); break;
+    case TDGL_FIELD_PCG: implicit_pcg(o); break;
+    case TDGL_FIELD_EXPLICIT_SCALED: scale_increment(o); break;
+    default: break;
+  }
+  links(o);
+  curl_h(o);
+  field_bc(o, H, HJ);
+  neumann(o);
+  advance(o);
+}
+
+template <class T>
+void load(A3<T> &a, const double *p) {
+  std::memcpy(a.v.data(), p, a.v.size() * sizeof(T));
+}
+template <class T>
+void save(const A3<T> &a, double *p) {
+  std::memcpy(p, a.v.data(), a.v.size() * sizeof(T));
+}
+
+}  // namespace
+
+extern "C" {
+
+orc *orc_create(const tdgl_desc *d) {
+  orc *o = new orc();
+  o->d = *d;
+  o->N = d->nbands;
+  o->Nx = d->Nx;
+  o->Ny = d->Ny;
+  o->Nz = d->Nz;
+  o->x1 = d->x1;
+  o->x2 = d->x2;
+  o->y1 = d->y1;
+  o->y2 = d->y2;
+  o->z1 = d->z1;
+  o->z2 = d->z2;
+  const int Nx = d->Nx, Ny = d->Ny, Nz = d->Nz;
+  o->f = A3r(Nx + 1, Ny + 1, Nz + 1);
+  o->tx = A3r(Nx + 1, Ny + 1, Nz + 1);
+  o->ty = A3r(Nx + 1, Ny + 1, Nz + 1);
+  o->tz = A3r(Nx + 1, Ny + 1, Nz + 1);
+  load(o->f, d->f);
+  load(o->tx, d->taux);
+  load(o->ty, d->tauy);
+  load(o->tz, d->tauz);
+  if (d->Dx) {
+    o->Dx = A3r(Nx, Ny + 1, Nz + 1);
+    o->Dy = A3r(Nx + 1, Ny, Nz + 1);
+    o->Dz = A3r(Nx + 1, Ny + 1, Nz);
+    load(o->Dx, d->Dx);
+    load(o->Dy, d->Dy);
+    load(o->Dz, d->Dz);
+  }
+  if (d->selffield_bc == TDGL_BC_BAR) {
+    size_t n[6] = {(size_t)(Ny + 1) * 2, (size_t)2 * Nz, (size_t)Ny * 2, (size_t)2 * (Nz + 1), (size_t)Ny * (Nz + 1),
+                   (size_t)(Ny + 1) * Nz};
+    const double *src[6] = {d->bar_hy_z, d->bar_hy_y, d->bar_hz_z, d->bar_hz_y, d->bar_hz_x, d->bar_hy_x};
+    for (int q = 0; q < 6; ++q) o->bar[q].assign(src[q], src[q] + n[q]);
+  }
+  for (int mu = 0; mu < o->N; ++mu) {
+    o->G.emplace_back(Nx + 1, Ny + 1, Nz + 1);
+    o->F.emplace_back(Nx + 1, Ny + 1, Nz + 1);
+  }
+  o->thx = o->phx = o->Jx = A3r(Nx, Ny + 1, Nz + 1);
+  o->thy = o->phy = o->Jy = A3r(Nx + 1, Ny, Nz + 1);
+  o->thz = o->phz = o->Jz = A3r(Nx + 1, Ny + 1, Nz);
+  o->Ux = o->Vx = A3c(Nx, Ny + 1, Nz + 1);
+  o->Uy = o->Vy = A3c(Nx + 1, Ny, Nz + 1);
+  o->Uz = o->Vz = A3c(Nx + 1, Ny + 1, Nz);
+  o->hx = A3r(Nx + 1, Ny, Nz);
+  o->hy = A3r(Nx, Ny + 1, Nz);
+  o->hz = A3r(Nx, Ny, Nz + 1);
+  o->W = o->W2 = A3c(Nx + 1, Ny + 1, Nz + 1);
+  o->L[0] = o->L[3] = A3c(Nx, Ny + 1, Nz + 1);
+  o->L[1] = o->L[4] = A3c(Nx + 1, Ny, Nz + 1);
+  o->L[2] = o->L[5] = A3c(Nx + 1, Ny + 1, Nz);
+  o->Om = A3r(Nx + 1, Ny + 1, Nz + 1);
+  o->R[0] = A3r(Nx, Ny + 1, Nz + 1);
+  o->R[1] = A3r(Nx + 1, Ny, Nz + 1);
+  o->R[2] = A3r(Nx + 1, Ny + 1, Nz);
+  if (d->field_mode == TDGL_FIELD_SEPARABLE || d->field_mode == TDGL_FIELD_PCG) {
+    Sep &S = o->sep;
+    S.Nx = Nx;
+    S.Ny = Ny;
+    S.Nz = Nz;
+    S.bx = make_basis(Nx - 1);
+    S.by = make_basis(Ny - 1);
+    S.bz = make_basis(Nz - 1);
+    S.a = d->a;
+    int ex[3] = {Nx - 2, Ny - 1, Nz - 1}, ey[3] = {Nx - 1, Ny - 2, Nz - 1}, ez[3] = {Nx - 1, Ny - 1, Nz - 2};
+    std::memcpy(S.ex, ex, sizeof ex);
+    std::memcpy(S.ey, ey, sizeof ey);
+    std::memcpy(S.ez, ez, sizeof ez);
+    const Basis *bb[3] = {&S.bx, &S.by, &S.bz};
+    for (int c = 0; c < 3; ++c) {
+      S.V32[c].assign(bb[c]->V.begin(), bb[c]->V.end());
+      S.U32[c].assign(bb[c]->U.begin(), bb[c]->U.end());
+      S.s32[c].assign(bb[c]->s.begin(), bb[c]->s.end());
+    }
+    o->have_sep = true;
+  }
+  if (d->field_mode == TDGL_FIELD_PCG) {
+    o->P[0] = o->Q[0] = A3r(Nx, Ny + 1, Nz + 1);
+    o->P[1] = o->Q[1] = A3r(Nx + 1, Ny, Nz + 1);
+    o->P[2] = o->Q[2] = A3r(Nx + 1, Ny + 1, Nz);
+    o->H[0] = A3r(Nx + 1, Ny, Nz);
+    o->H[1] = A3r(Nx, Ny + 1, Nz);
+    o->H[2] = A3r(Nx, Ny, Nz + 1);
+    for (int c = 0; c < 3; ++c) {
+      R3 r = prange(*o, c);
+      size_t n = (size_t)(r.i1 - r.i0 + 1) * (r.j1 - r.j0 + 1) * (r.k1 - r.k0 + 1);
+      o->x[c].assign(n, 0.0);
+      o->xprev[c].assign(n, 0.0);
+      o->zold[c].assign(n, 0.0);
+    }
+  }
+  return o;
+}
+
+void orc_destroy(orc *o) { delete o; }
+
+void orc_set_state(orc *o, const tdgl_state *s) {
+  for (int mu = 0; mu < o->N; ++mu) {
+    load(o->G[mu], s->G[mu]);
+    load(o->F[mu], s->F[mu]);
+  }
+  load(o->thx, s->thx);
+  load(o->thy, s->thy);
+  load(o->thz, s->thz);
+  load(o->phx, s->phx);
+  load(o->phy, s->phy);
+  load(o->phz, s->phz);
+  load(o->Ux, s->Ux);
+  load(o->Uy, s->Uy);
+  load(o->Uz, s->Uz);
+  load(o->Vx, s->Vx);
+  load(o->Vy, s->Vy);
+  load(o->Vz, s->Vz);
+  load(o->Jx, s->Jx);
+  load(o->Jy, s->Jy);
+  load(o->Jz, s->Jz);
+  load(o->hx, s->hx);
+  load(o->hy, s->hy);
+  load(o->hz, s->hz);
+}
+
+void orc_get_state(orc *o, tdgl_state *s) {
+  for (int mu = 0; mu < o->N; ++mu) {
+    save(o->G[mu], s->G[mu]);
+    save(o->F[mu], s->F[mu]);
+  }
+  save(o->thx, s->thx);
+  save(o->thy, s->thy);
+  save(o->thz, s->thz);
+  save(o->phx, s->phx);
+  save(o->phy, s->phy);
+  save(o->phz, s->phz);
+  save(o->Ux, s->Ux);
+  save(o->Uy, s->Uy);
+  save(o->Uz, s->Uz);
+  save(o->Vx, s->Vx);
+  save(o->Vy, s->Vy);
+  save(o->Vz, s->Vz);
+  save(o->Jx, s->Jx);
+  save(o->Jy, s->Jy);
+  save(o->Jz, s->Jz);
+  save(o->hx, s->hx);
+  save(o->hy, s->hy);
+  save(o->hz, s->hz);
+}
+
+void orc_step(orc *o, int n, double H, double HJ) {
+  for (int q = 0; q < n; ++q) step(*o, H, HJ);
+}
+
+void orc_get_pcg(orc *o, tdgl_pcg *p) {
+  for (int c = 0; c < 3; ++c) {
+    if (p->x[c]) std::memcpy(p->x[c], o->x[c].data(), o->x[c].size() * 8);
+    if (p->xprev[c]) std::memcpy(p->xprev[c], o->xprev[c].data(), o->xprev[c].size() * 8);
+  }
+  p->warm = o->warm;
+  p->have2 = o->have2;
+  p->iters = o->iters;
+}
+
+void orc_set_pcg(orc *o, const tdgl_pcg *p) {
+  for (int c = 0; c < 3; ++c) {
+    if (p->x[c]) std::memcpy(o->x[c].data(), p->x[c], o->x[c].size() * 8);
+    if (p->xprev[c]) std::memcpy(o->xprev[c].data(), p->xprev[c], o->xprev[c].size() * 8);
+  }
+  o->warm = p->warm != 0;
+  o->have2 = p->have2 != 0;
+  o->iters = p->iters;
+}
+
+long long orc_total_iters(orc *o) { return o->total_iters; }
+
+// The separable inverse alone, in place on range arrays (tests).
+void orc_sep_solve(orc *o, double *BX, double *BY, double *BZ) {
+  std::vector<double> X(BX, BX + o->BX.size()), Y, Z;
+  const Sep &S = o->sep;
+  size_t nx = (size_t)S.ex[0] * S.ex[1] * S.ex[2], ny = (size_t)S.ey[0] * S.ey[1] * S.ey[2],
+         nz = (size_t)S.ez[0] * S.ez[1] * S.ez[2];
+  X.assign(BX, BX + nx);
+  Y.assign(BY, BY + ny);
+  Z.assign(BZ, BZ + nz);
+  sep_solve(S, X, Y, Z);
+  std::memcpy(BX, X.data(), nx * 8);
+  std::memcpy(BY, Y.data(), ny * 8);
+  std::memcpy(BZ, Z.data(), nz * 8);
+}
+
+// The 1D bases (tests): V (nn x nn), U (nn-1 x nn-1), s (nn)
+void orc_basis(int nn, double *V, double *U, double *s) {
+  Basis b = make_basis(nn);
+  std::memcpy(V, b.V.data(), b.V.size() * 8);
+  std::memcpy(U, b.U.data(), b.U.size() * 8);
+  std::memcpy(s, b.s.data(), b.s.size() * 8);
+}
+
+// One KWT node update (tests: the 2x2 system of test_physics.jl:51-59)
+void orc_kwt_update(const double *g, const double *d, double g2, double *out) {
+  C r = kwt_update(C{g[0], g[1]}, C{d[0], d[1]}, g2);
+  out[0] = r.re;
+  out[1] = r.im;
+}
+
+// One temporal link rotation (tests: rotation_fixture.jl semantics): mode 0 _rotfrac!, 1 _rot! root
+void orc_rotation(int mode, const double *v, const double *u, double c, int n, double *out) {
+  A3c L(1, 1, 1), R(1, 1, 1), V(1, 1, 1), U(1, 1, 1);
+  V(1, 1, 1) = C{v[0], v[1]};
+  U(1, 1, 1) = C{u[0], u[1]};
+  Rng r{1, 1, 1, 1, 1, 1};
+  if (mode == 0) {
+    rotfrac(L, V, U, c, r);
+    out[0] = L(1, 1, 1).re;
+    out[1] = L(1, 1, 1).im;
+  } else {
+    rot(L, R, V, U, n, r);
+    out[0] = R(1, 1, 1).re;
+    out[1] = R(1, 1, 1).im;
+  }
+}
+
+}  // extern "C"

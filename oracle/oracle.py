@@ -1,0 +1,124 @@
+"""ctypes binding of oracle/_build/liboracle.so (the C++ restatement of gtdgl's step!).
+
+TEST INFRASTRUCTURE ONLY: the parity checker and the CPU baseline.  Built by `make -C oracle`.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+from juliatdgl_amd.abi import PackedDesc, PcgC, PcgState, state_struct
+from juliatdgl_amd.scheme import Stepper
+from juliatdgl_amd.state import State
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ct.CDLL(LIB)
+        L.orc_create.restype = ct.c_void_p
+        L.orc_create.argtypes = [ct.c_void_p]
+        L.orc_destroy.argtypes = [ct.c_void_p]
+        L.orc_set_state.argtypes = [ct.c_void_p, ct.c_void_p]
+        L.orc_get_state.argtypes = [ct.c_void_p, ct.c_void_p]
+        L.orc_step.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_double]
+        L.orc_get_pcg.argtypes = [ct.c_void_p, ct.c_void_p]
+        L.orc_set_pcg.argtypes = [ct.c_void_p, ct.c_void_p]
+        L.orc_total_iters.argtypes = [ct.c_void_p]
+        L.orc_total_iters.restype = ct.c_longlong
+        L.orc_sep_solve.argtypes = [ct.c_void_p] + [ct.c_void_p] * 3
+        L.orc_basis.argtypes = [ct.c_int] + [ct.c_void_p] * 3
+        L.orc_kwt_update.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_void_p]
+        L.orc_rotation.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int, ct.c_void_p]
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's row loops (same bits with any number)."""
+    import ctypes.util
+    gomp = ct.CDLL(ctypes.util.find_library("gomp") or "libgomp.so.1")
+    gomp.omp_set_num_threads(int(n))
+
+
+class Oracle:
+    """The CPU step on a host State: set_state / step / get_state, as the C ABI does on the GPU."""
+
+    def __init__(self, st: Stepper):
+        self.st = st
+        self._desc = PackedDesc(st)
+        self.h = lib().orc_create(ct.byref(self._desc.c))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def set_state(self, s: State) -> None:
+        self._s = s.copy()
+        lib().orc_set_state(self.h, ct.byref(state_struct(self._s)))
+
+    def get_state(self) -> State:
+        out = self._s.copy()
+        lib().orc_get_state(self.h, ct.byref(state_struct(out)))
+        return out
+
+    def step(self, n: int, H: float, HJ: float) -> None:
+        lib().orc_step(self.h, int(n), float(H), float(HJ))
+
+    def get_pcg(self) -> PcgState:
+        p = PcgState(self.st.grid)
+        c = p.struct()
+        lib().orc_get_pcg(self.h, ct.byref(c))
+        p.load(c)
+        return p
+
+    def set_pcg(self, p: PcgState) -> None:
+        lib().orc_set_pcg(self.h, ct.byref(p.struct()))
+
+    def total_iters(self) -> int:
+        return int(lib().orc_total_iters(self.h))
+
+    def sep_solve(self, BX, BY, BZ):
+        """(I + aK)^{-1} on range arrays (the stepper must have the separable or PCG field)."""
+        out = [np.array(b, dtype=np.float64, order="F", copy=True) for b in (BX, BY, BZ)]
+        lib().orc_sep_solve(self.h, *(o.ctypes.data for o in out))
+        return out
+
+
+def basis(nn: int):
+    V = np.zeros((nn, nn), order="F")
+    U = np.zeros((nn - 1, nn - 1), order="F")
+    s = np.zeros(nn)
+    lib().orc_basis(nn, V.ctypes.data, U.ctypes.data, s.ctypes.data)
+    return V, U, s
+
+
+def kwt_update(g: complex, d: complex, g2: float) -> complex:
+    a = np.array([g.real, g.imag])
+    b = np.array([d.real, d.imag])
+    o = np.zeros(2)
+    lib().orc_kwt_update(a.ctypes.data, b.ctypes.data, g2, o.ctypes.data)
+    return complex(o[0], o[1])
+
+
+def rotation(mode: int, v: complex, u: complex, c: float = 0.0, n: int = 1) -> complex:
+    a = np.array([v.real, v.imag])
+    b = np.array([u.real, u.imag])
+    o = np.zeros(2)
+    lib().orc_rotation(mode, a.ctypes.data, b.ctypes.data, float(c), int(n), o.ctypes.data)
+    return complex(o[0], o[1])
