@@ -1,0 +1,125 @@
+"""ctypes binding of the HIP stepper library (juliatdgl_amd/_lib/libtdglhip.so, C ABI of
+include/tdglhip.h).  There is no fallback: if the library is missing or no HIP device is present,
+the calls raise."""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import threading
+
+import numpy as np
+
+from .abi import PackedDesc, PcgState, StatsC, state_struct
+from .scheme import Stepper
+from .state import State
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_lib", "libtdglhip.so")
+
+SYMBOLS = ("tdgl_supported", "tdgl_create", "tdgl_destroy", "tdgl_upload_state", "tdgl_download_state", "tdgl_step",
+           "tdgl_get_pcg", "tdgl_set_pcg", "tdgl_get_stats", "tdgl_last_error", "tdgl_device_count",
+           "tdgl_set_device")
+
+_lib = None
+_lock = threading.Lock()
+
+
+class TDGLError(RuntimeError):
+    pass
+
+
+def load():
+    """The library, loaded once.  Raises if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB):
+                raise TDGLError(f"the HIP stepper library is not built ({LIB}); run python -m juliatdgl_amd.build")
+            L = ct.CDLL(LIB)
+            vp = ct.c_void_p
+            L.tdgl_supported.argtypes = [vp, ct.c_char_p, ct.c_size_t]
+            L.tdgl_supported.restype = ct.c_int
+            L.tdgl_create.argtypes = [vp, ct.POINTER(vp)]
+            L.tdgl_destroy.argtypes = [vp]
+            L.tdgl_upload_state.argtypes = [vp, vp]
+            L.tdgl_download_state.argtypes = [vp, vp]
+            L.tdgl_step.argtypes = [vp, ct.c_int32, ct.c_double, ct.c_double]
+            L.tdgl_get_pcg.argtypes = [vp, vp]
+            L.tdgl_set_pcg.argtypes = [vp, vp]
+            L.tdgl_get_stats.argtypes = [vp, vp]
+            L.tdgl_last_error.restype = ct.c_char_p
+            L.tdgl_device_count.restype = ct.c_int
+            L.tdgl_set_device.argtypes = [ct.c_int32]
+            _lib = L
+        return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().tdgl_last_error().decode(errors="replace")
+        raise TDGLError(f"{what} failed ({rc}): {msg}")
+
+
+def device_count() -> int:
+    return int(load().tdgl_device_count())
+
+
+def supported_desc(st: Stepper):
+    """(ok, why) of the C ABI's tdgl_supported for this stepper."""
+    pd = PackedDesc(st)
+    buf = ct.create_string_buffer(256)
+    ok = load().tdgl_supported(ct.byref(pd.c), buf, len(buf))
+    return bool(ok), buf.value.decode()
+
+
+class HipStepper:
+    """One device stepper: coefficients, masks, work arrays and the state of one run."""
+
+    def __init__(self, st: Stepper):
+        self.st = st
+        self._desc = PackedDesc(st)
+        L = load()
+        if L.tdgl_device_count() < 1:
+            raise TDGLError("no HIP device")
+        h = ct.c_void_p()
+        _check(L.tdgl_create(ct.byref(self._desc.c), ct.byref(h)), "tdgl_create")
+        self.h = h
+        self._shape = None
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            load().tdgl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, s: State) -> None:
+        self._shape = s.copy()
+        _check(load().tdgl_upload_state(self.h, ct.byref(state_struct(s))), "tdgl_upload_state")
+
+    def download(self, into: State = None) -> State:
+        out = into if into is not None else self._shape.copy()
+        _check(load().tdgl_download_state(self.h, ct.byref(state_struct(out))), "tdgl_download_state")
+        return out
+
+    def step(self, n: int, H: float, HJ: float) -> None:
+        _check(load().tdgl_step(self.h, int(n), float(H), float(HJ)), "tdgl_step")
+
+    def get_pcg(self) -> PcgState:
+        p = PcgState(self.st.grid)
+        c = p.struct()
+        _check(load().tdgl_get_pcg(self.h, ct.byref(c)), "tdgl_get_pcg")
+        p.load(c)
+        return p
+
+    def set_pcg(self, p: PcgState) -> None:
+        _check(load().tdgl_set_pcg(self.h, ct.byref(p.struct())), "tdgl_set_pcg")
+
+    def stats(self) -> dict:
+        c = StatsC()
+        _check(load().tdgl_get_stats(self.h, ct.byref(c)), "tdgl_get_stats")
+        return {k: getattr(c, k) for k, _ in StatsC._fields_}

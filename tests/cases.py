@@ -1,0 +1,108 @@
+"""Stepper cases of the parity tests, after the reference's step comparison
+(gpu/TDGLGPU/test/compare_cpu.jl:28-38, 44-63): the two-band MgB2-like model of
+gtdgl/materials/mgb2_model_II.toml at κ = 2 on a 24x12x6 WEDGE box, with one scheme option
+changed per case, plus three bands, KWT and anisotropic masses; and the comparison itself
+(compare_cpu.jl:65-109): 5 CPU steps off the seed, then n steps of the oracle and of the HIP
+stepper from the same state (and the same PCG history), every field compared, vector fields
+against the size of the whole vector (:91-96)."""
+from __future__ import annotations
+
+import numpy as np
+
+from juliatdgl_amd.scheme import Band, Grid, Model, box_masks, make_stepper, wedge_masks
+from juliatdgl_amd.state import initial_state
+
+
+def mgb2(kappa=2.0, gkwt=0.0, sigma_n=1.0):
+    """mgb2_model_II.toml: σ (m=1, Γ=1, a=1), π (ξ ratio 4, a=0.59), γ12 = 1e-3."""
+    R, a, beta = 4.0, 0.59, 1.0
+    m = 1 / (R * R * (a / beta))
+    return Model.make((Band.make(gkwt=gkwt), Band.make(m=m, Gamma=1.0, a=a, gkwt=gkwt)),
+                      [[0, 1e-3], [1e-3, 0]], kappa=kappa, sigma_n=sigma_n)
+
+
+def three_frustrated(kappa=2.0):
+    g = -0.1
+    return Model.make((Band.make(), Band.make(), Band.make()), [[0, g, g], [g, 0, g], [g, g, 0]], kappa=kappa)
+
+
+def three_aniso(kappa=2.0):
+    """Three bands with anisotropic masses m_z = 4 m_xy: a = 1, 0.6, 0.3, ξ = 1, 1.8, 3."""
+    bands = []
+    for a, xi in ((1.0, 1.0), (0.6, 1.8), (0.3, 3.0)):
+        m = 1 / (xi * xi * a)
+        bands.append(Band.make(a=a, m_axes=(m, m, 4 * m)))
+    gm = [[0, -0.1, -0.05], [-0.1, 0, -0.02], [-0.05, -0.02, 0]]
+    return Model.make(tuple(bands), gm, kappa=kappa)
+
+
+SMALL = dict(nx=24, ny=12, nz=6, dx=0.5)
+
+# name -> (model factory, mesh overrides, scheme keywords, tolerance)
+CASES = {
+    "production": (mgb2, {}, {}, 1e-10),
+    "euler_substeps": (mgb2, {}, dict(rkl=False), 1e-10),
+    "no_extrap": (mgb2, {}, dict(extrap=False), 1e-10),
+    "slab": (mgb2, {}, dict(selffield_bc="slab"), 1e-10),
+    "rkl": (mgb2, {}, dict(rkl=True), 1e-10),
+    "pcg_fast_f32": (mgb2, dict(buf=2), dict(sigma_vac=0.01), 1e-7),
+    "pcg_tight_f32": (mgb2, dict(buf=2), dict(sigma_vac=0.01, pcg_tol=1e-12), 1e-10),
+    "pcg_exact": (mgb2, dict(buf=2), dict(sigma_vac=0.01, pcg_fast=False, pcg_f32=False, pcg_tol=1e-12), 1e-10),
+    "explicit_padding": (mgb2, dict(buf=2), dict(sigma_vac=0.01, implicit=False), 1e-10),
+    "explicit": (mgb2, {}, dict(implicit=False), 1e-10),
+    "three_frustrated": (three_frustrated, {}, {}, 1e-10),
+    "kwt10": (lambda: mgb2(gkwt=10.0), {}, {}, 1e-10),
+    "kwt10_euler": (lambda: mgb2(gkwt=10.0), {}, dict(rkl=False), 1e-10),
+    "kwt10_pcg": (lambda: mgb2(gkwt=10.0), dict(buf=2), dict(sigma_vac=0.01, pcg_tol=1e-12), 1e-10),
+    "three_aniso": (three_aniso, {}, {}, 1e-10),
+    "hard": (mgb2, {}, dict(hard=True), 1e-10),
+}
+
+
+def make_case(name, **extra):
+    factory, mesh, scheme, tol = CASES[name]
+    md = factory()
+    m = dict(SMALL)
+    m.update(mesh)
+    gr = Grid.make(m["nx"], m["ny"], m["nz"], dx=m["dx"], buf=m.get("buf", 1))
+    mk = wedge_masks(gr)
+    kw = dict(scheme)
+    kw.update(extra)
+    st = make_stepper(md, gr, mk, **kw)
+    return st, tol
+
+
+def maxrel(a, b):
+    d = np.max(np.abs(a - b)) if a.size else 0.0
+    m = np.max(np.abs(b)) if b.size else 0.0
+    return d if m == 0 else d / m
+
+
+def compare_states(sg, s):
+    """Per-field errors of compare_cpu.jl:88-96: Ψ_µ against its own size, θ, h and J against
+    the size of their whole vector."""
+    out = {}
+    for mu in range(s.N):
+        out[f"G{mu + 1}"] = maxrel(sg.G[mu], s.G[mu])
+    for qs in (("thx", "thy", "thz"), ("hx", "hy", "hz"), ("Jx", "Jy", "Jz"), ("phx", "phy", "phz")):
+        m = max(np.max(np.abs(getattr(s, q))) for q in qs)
+        for q in qs:
+            d = np.max(np.abs(getattr(sg, q) - getattr(s, q)))
+            out[q] = d if m == 0 else d / m
+    for q in ("Vx", "Vy", "Vz", "Ux", "Uy", "Uz"):
+        out[q] = np.max(np.abs(getattr(sg, q) - getattr(s, q)))
+    return out
+
+
+def seed_state(st, H=0.03, J=0.3, warm_steps=5):
+    """initial_state and `warm_steps` oracle steps off the seed (compare_cpu.jl:74-75)."""
+    from oracle.oracle import Oracle
+    s0 = initial_state(st.model, st.grid, st.masks, H)
+    HJ = st.selffield(J)
+    o = Oracle(st)
+    o.set_state(s0)
+    if warm_steps:
+        o.step(warm_steps, H, HJ)
+    s = o.get_state()
+    pcg = o.get_pcg() if st.field_mode == 2 else None
+    return s, pcg, H, HJ
