@@ -1,0 +1,186 @@
+"""Ladder-cycle throughput of the HIP stepper on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg1]
+
+A "step" of this benchmark is one ladder cycle: lMax = 40 time steps of the release-2.0 step
+(the n that TDGL's ladder hands to its stepper hook, gtdgl/src/protocol.jl:28, 409), with the
+state resident on the device.  The workload is the two-band KWT stepper on 128x64x16 cells with
+an insulating buffer and the PCG field solve to relative residual 1e-10 (juliatdgl_amd.configs.cfg1).
+Prints one JSON line: cell-cycles/s (nx·ny·nz × cycles / s), the roofline of the dominant
+kernel (HIP events around its launches over the timed cycles), and the CPU oracle timed on a
+bounded sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from juliatdgl_amd import configs  # noqa: E402
+from juliatdgl_amd.hip import HipStepper, device_count  # noqa: E402
+from juliatdgl_amd.state import seeded_state  # noqa: E402
+
+HBM_PEAK = 8.0e12       # B/s, MI355X HBM3E
+FP64_PEAK = 78.6e12     # flop/s, MI355X FP64 (vector and matrix)
+LMAX = 40
+
+
+def kernel_bytes(st, name: str, launches: int, nsteps: int, stats: dict) -> float:
+    """Algorithmic bytes (each array a kernel must touch counted once) summed over the launches
+    of one kernel class in `nsteps` time steps."""
+    gr = st.grid
+    N = st.N
+    interior = (gr.x2 - gr.x1) * (gr.y2 - gr.y1) * (gr.z2 - gr.z1)
+    nodes = (gr.Nx + 1) * (gr.Ny + 1) * (gr.Nz + 1)
+    rng = (gr.Nx - 2) * (gr.Ny - 1) * (gr.Nz - 1) + (gr.Nx - 1) * (gr.Ny - 2) * (gr.Nz - 1) + \
+        (gr.Nx - 1) * (gr.Ny - 1) * (gr.Nz - 2)
+    if name == "band_stage":
+        # S, D, the other bands, f, three links (+ Y_{j-2} and Ω on most stages)
+        per = 16 + 16 + 16 * (N - 1) + 8 + 48 + 16 + 8
+        return per * interior * launches
+    if name in ("sep_transform_f64",):
+        return 16.0 * rng * launches
+    if name == "sep_transform_f32":
+        return 8.0 * rng * launches
+    if name == "pcg_apply_A":
+        return 24.0 * rng * launches
+    if name == "current_field":
+        return (16 * N + 48 + 24 * 6 + 24 + 24) * nodes * launches
+    if name == "links_curl_bc":
+        return 96.0 * nodes * launches
+    return 0.0
+
+
+def kernel_flops(st, name: str, launches: int) -> float:
+    gr = st.grid
+    e = [(gr.Nx - 2, gr.Ny - 1, gr.Nz - 1), (gr.Nx - 1, gr.Ny - 2, gr.Nz - 1), (gr.Nx - 1, gr.Ny - 1, gr.Nz - 2)]
+    if name.startswith("sep_transform"):
+        # one pass along one axis: 2 L flop per element; the three axes alternate
+        per_axis = [sum(2.0 * ex[a] * ex[0] * ex[1] * ex[2] for ex in e) for a in range(3)]
+        return np.mean(per_axis) * launches
+    return 0.0
+
+
+def cpu_baseline(st, s0, H, HJ, nsteps: int, threads: int):
+    """The oracle (CPU restatement of the step) on `nsteps` time steps of the same workload."""
+    from oracle.oracle import Oracle, set_threads
+    set_threads(threads)
+    o = Oracle(st)
+    o.set_state(s0)
+    t = time.perf_counter()
+    o.step(nsteps, H, HJ)
+    dt = time.perf_counter() - t
+    cells = st.grid.cells
+    return cells * (nsteps / LMAX) / dt, dt
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed ladder cycles")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed ladder cycles")
+    ap.add_argument("--workload", default="cfg1")
+    ap.add_argument("--cpu-steps", type=int, default=8, help="time steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-out", default=None, help="write the per-kernel profile (JSON) here")
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: a run is one device's work and this build has no multi-GPU path "
+              "(--gpus 1 only)", file=sys.stderr)
+        return 2
+    if device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        return 1
+    if args.workload == "cfg1":
+        st, p = configs.cfg1()
+        workload = ("two-band KWT (u=Gamma/sigma_n=5.79, gkwt=10) TDGL, 128x64x16 cells, dx=0.5, buf 2 insulating "
+                    "(sigma_vac/sigma_n=0.01), PCG tol 1e-10 (Float32 preconditioner, FP64 outer), 30-degree wedge, "
+                    "H=0.2, J=0.1, dt_scale=0.1, ladder cycle = 40 steps")
+    elif args.workload == "cfg3":
+        st, p = configs.cfg3_like()
+        workload = "three-band anisotropic TDGL, 256x128x32 cells, buf 8 insulating, PCG tol 1e-8, H=0.2, J=0.1"
+    else:
+        raise SystemExit(f"unknown workload {args.workload}")
+    H, HJ = p["H"], st.selffield(p["J"])
+    s0 = seeded_state(st.model, st.grid, st.masks, H, seed=20261202)
+    dev = HipStepper(st)
+    dev.upload(s0)
+    print(f"[bench] {args.workload}: dt={st.dt:.5f} stages={st.rkl} nsub={st.nsub} grid={st.grid.Nx}x{st.grid.Ny}x"
+          f"{st.grid.Nz} padded", file=sys.stderr, flush=True)
+    for q in range(args.warmup):
+        dev.step(LMAX, H, HJ)
+        print(f"[bench] warmup cycle {q + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    st0 = dev.stats()
+    t = time.perf_counter()
+    for q in range(args.steps):
+        dev.step(LMAX, H, HJ)
+    wall = time.perf_counter() - t
+    st1 = dev.stats()
+    chk = dev.download()
+    gmax = max(float(np.abs(g).max()) for g in chk.G)
+    if not np.isfinite(gmax) or gmax > 5:
+        print(f"[bench] the state diverged (max|psi| = {gmax})", file=sys.stderr)
+        return 3
+    dev_ms = st1["device_ms"] - st0["device_ms"]
+    iters = st1["pcg_iters"] - st0["pcg_iters"]
+    nsteps = args.steps * LMAX
+    print(f"[bench] timed {args.steps} cycles: wall {wall:.3f}s device {dev_ms / 1e3:.3f}s, pcg iters/step "
+          f"{iters / nsteps:.2f}", file=sys.stderr, flush=True)
+    cells = st.grid.cells
+    value = cells * args.steps / wall
+    # per-kernel device time: the same number of cycles again with HIP events around every launch
+    dev.profile(True)
+    for q in range(args.steps):
+        dev.step(LMAX, H, HJ)
+    prof = dev.profile_get()
+    dev.profile(False)
+    tot = sum(ms for _, ms in prof.values())
+    rows = sorted(prof.items(), key=lambda kv: -kv[1][1])
+    for name, (nl, ms) in rows:
+        if nl:
+            print(f"[bench]   {name:20s} {nl:8d} launches {ms:10.3f} ms ({100 * ms / tot:5.1f}%) "
+                  f"{1e3 * ms / nl:8.2f} us/launch", file=sys.stderr)
+    dom, (nl, ms) = rows[0]
+    avg_s = ms / nl / 1e3
+    byts = kernel_bytes(st, dom, nl, nsteps, prof) / nl
+    flops = kernel_flops(st, dom, nl) / nl
+    if byts > 0 and (flops == 0 or byts / flops > 1 / 9.8):
+        roof = {"bound": "hbm", "achieved": byts / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": byts / avg_s / HBM_PEAK, "traffic": None, "kernel": dom, "bytes_per_launch": byts,
+                "avg_launch_us": avg_s * 1e6}
+    else:
+        roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                "frac": flops / avg_s / FP64_PEAK, "traffic": None, "kernel": dom, "flops_per_launch": flops,
+                "bytes_per_launch": byts, "avg_launch_us": avg_s * 1e6,
+                "note": "dense FP64 transform pass on the vector ALUs (FP64 vector peak = FP64 matrix peak)"}
+    if args.profile_out:
+        with open(args.profile_out, "w") as fh:
+            json.dump({k: {"launches": v[0], "ms": v[1]} for k, v in prof.items()}, fh, indent=1)
+    cpu = None
+    if not args.no_cpu:
+        s1 = dev.download()
+        v1, t1 = cpu_baseline(st, s1, H, HJ, args.cpu_steps, 1)
+        cpu = {"value": v1, "unit": "cell-cycles/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} time steps (of 40 per cycle) of the same workload from the GPU's state "
+                         f"after warmup, C++ oracle single-threaded, {t1:.1f} s; host {os.cpu_count()} logical cpus"}
+        print(f"[bench] cpu oracle 1 thread: {v1:.3e} cell-cycles/s ({t1:.1f}s for {args.cpu_steps} steps)",
+              file=sys.stderr, flush=True)
+    out = {"metric": "cell-cycles/s", "value": value, "unit": "cell-cycles/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded psi, notch mask)",
+           "config": {"workload": workload, "cells": cells, "lmax": LMAX, "pcg_iters_per_step": iters / nsteps,
+                      "device_ms_per_cycle": dev_ms / args.steps},
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

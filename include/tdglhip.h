@@ -137,6 +137,9 @@ typedef struct {
   int32_t last_iters;   /* iterations of the last solve (PCGSolver.iters) */
   double device_ms;     /* device time of the steps, HIP events */
   int64_t device_bytes; /* device memory held by the handle */
+  int64_t graph_cycles; /* tdgl_step calls replayed from a captured graph */
+  int64_t rollbacks;    /* captured cycles redone eagerly (a solve needed more iterations) */
+  int32_t pcg_budget;   /* PCG iterations per solve in the captured cycle */
 } tdgl_stats;
 
 typedef struct tdgl_handle tdgl_handle;
@@ -164,6 +167,18 @@ int tdgl_set_pcg(tdgl_handle *h, const tdgl_pcg *p);
 
 /* Counters (execution_stats, TDGLGPU.jl:2086-2087, per handle). */
 int tdgl_get_stats(tdgl_handle *h, tdgl_stats *st);
+
+/* Per-kernel device time (no counterpart in the reference: the measurement of this library).
+ * While enabled, every launch is bracketed by HIP events on the handle's stream; the totals
+ * are per kernel class (tdgl_kernel_name).  Enabling clears them. */
+#define TDGL_NKERNELS 13
+typedef struct {
+  int64_t launches[TDGL_NKERNELS];
+  double ms[TDGL_NKERNELS];
+} tdgl_profile;
+int tdgl_profile_enable(tdgl_handle *h, int32_t on);
+int tdgl_profile_get(tdgl_handle *h, tdgl_profile *p);
+const char *tdgl_kernel_name(int32_t k);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *tdgl_last_error(void);

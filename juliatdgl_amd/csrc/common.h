@@ -77,5 +77,6 @@ struct Ranges {
 };
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+__device__ __forceinline__ int cdiv_d(int a, int b) { return (a + b - 1) / b; }
 
 }  // namespace tdgl

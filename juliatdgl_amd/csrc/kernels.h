@@ -9,12 +9,21 @@ namespace tdgl {
 // ---------------------------------------------------------------- band update (bands.jl)
 enum FrameMode { FRAME_NONE = 0, FRAME_RATE = 1, FRAME_TERM = 2 };
 
+// How a stage gets its links (computed per use from V, U and Ω, never stored):
+enum LinkSource {
+  LS_DIRECT = 0,    // V                                         (first stage; extrap off)
+  LS_ROTFRAC = 1,   // V (V Ū)^c                                 (_rotfrac!, bands.jl:409-423)
+  LS_ROTLINKS = 2,  // V _cis(R c), R = arg(V Ū) + (Ω[+e] − Ω) dt (_link_rate! :303-310, _rotlinks! :311-316)
+  LS_SUBSTEP = 3    // V Q^(m−1) with Q = (V Ū)^(1/n), each power times the frame's _cis((Ω[+e] − Ω) h)
+                    //  (_rot! :355-369, _advance_links! :371-376, _frame_links! :292-297)
+};
+
 struct StageArgs {
   cplx *D;                     // output (interior nodes)
   const cplx *S;               // source Y_{j-1}
   const cplx *A;               // Y_{j-2} (combine) or null
   const cplx *G[TDGL_MAX_BANDS];  // the bands at the start of the step (coupling)
-  const cplx *Lx, *Ly, *Lz;    // links of this stage
+  const cplx *V[3], *U[3];     // links of this step and of the previous one
   const double *f;             // condensation mask
   double *Om;                  // rotating-frame rate (written by FRAME_RATE, read otherwise)
   double kx, ky, kz, c, eps, beta, gamma2;
@@ -28,30 +37,12 @@ struct StageArgs {
   int turn;                    // F = C·cis(Ω dt) (end of the band's step)
   double dt;
   int hard;
+  int lsrc;                    // LinkSource
+  double lc;                   // fraction c of the step (ROTFRAC, ROTLINKS)
+  int msub, nsub;              // substep m of n (SUBSTEP)
+  int lframe;                  // SUBSTEP: the frame's factor on each power
 };
 hipError_t launch_stage(const Geo &g, const StageArgs &a, hipStream_t s);
-
-enum LinkMode {
-  LINK_ROTFRAC = 0,   // L = V (V Ū)^c                     (_rotfrac!, bands.jl:409-423)
-  LINK_ROTLINKS = 1,  // L = V _cis(R c)                    (_rotlinks!, :311-316)
-  LINK_RATE = 2,      // R = arg(V Ū) + (Ω[+e] − Ω) dt       (_link_rate!, :303-310)
-  LINK_ROOT = 3,      // L = V, Q = (V Ū)^(1/n)              (_rot!, :355-369)
-  LINK_ADVANCE = 4    // L = L·Q [· _cis((Ω[+e] − Ω) τ)]      (_advance_links! + _frame_links!)
-};
-struct LinkArgs {
-  int mode;
-  cplx *L[3];
-  cplx *Q[3];            // substep roots (complex)
-  double *R[3];          // rotframe link rates (real)
-  const cplx *V[3];
-  const cplx *U[3];
-  const double *Om;
-  double c;              // fraction of the step / τ
-  double dt;
-  int n;                 // substeps
-  int frame;             // ADVANCE: also the frame links
-};
-hipError_t launch_links(const Geo &g, const LinkArgs &a, hipStream_t s);
 
 // ---------------------------------------------------------------- field (field.jl)
 struct FieldArgs {
@@ -63,7 +54,8 @@ struct FieldArgs {
   const double *h[3];
   const double *tau[3];
   const double *D[3];    // scaled modes: φ = θ + (φₑ − θ)/D on every link
-  double *B;             // implicit modes: φₑ − θ on the ranges (compact)
+  double *B[3];          // implicit modes: φₑ − θ on the ranges (node-shaped, zero elsewhere)
+  double *part;          // PCG: block partials of b·b (null otherwise)
   Ranges rg;
   double wa[3][TDGL_MAX_BANDS];
   double d1, sxy, sxz, syx, syz, szx, szy;
@@ -72,15 +64,15 @@ struct FieldArgs {
   int gather;            // write B
 };
 hipError_t launch_field(const Geo &g, const FieldArgs &a, hipStream_t s);
+int field_blocks(const Geo &g);
 
 struct CurlArgs {
   const double *ph[3];
   cplx *U[3];
   double *h[3];
-  double H, HJ;
+  const double *hp;  // (H, HJ) in device memory, set before each call of the step
   int bar;
   const double *bar_hy_z, *bar_hy_y, *bar_hz_z, *bar_hz_y, *bar_hz_x, *bar_hy_x;
-  double idydz, idzdx, idxdy;  // 1/(dy dz) etc. are NOT used: the reference divides by (dy*dz)
   double dydz, dzdx, dxdy;
 };
 hipError_t launch_curl(const Geo &g, const CurlArgs &a, hipStream_t s);
@@ -92,80 +84,111 @@ struct GhostArgs {
 };
 hipError_t launch_ghosts(const Geo &g, const GhostArgs &a, hipStream_t s);
 
-// φ = θ + B on the ranges (implicit_curl!, implicit.jl:98-102 and :328-332)
-hipError_t launch_scatter(const Geo &g, const Ranges &rg, const double *B, const double *const th[3],
-                          double *const ph[3], hipStream_t s);
+// device-to-device copies of up to 32 arrays in one launch
+struct CopyList {
+  const double *src[32];
+  double *dst[32];
+  int n[32];
+  int cnt;
+};
+hipError_t launch_copies(const CopyList &c, hipStream_t s);
 
-// ---------------------------------------------------------------- separable solve (implicit.jl)
-// One pass of the 1D transforms along `axis` for the three components: out = M' in (the
-// forward product with the basis of each component on that axis) with A[n*L + m] = M(n, m).
+// ---------------------------------------------------------------- implicit solve (implicit.jl)
+// Scalars, flags and reduction partials of one PCG solve
+enum { SC_NB = 0, SC_RZ0, SC_RZ1, SC_TOL, SC_COUNT };
+enum {
+  FL_NZ = 0,      // the source is not zero
+  FL_ACT0,        // the iterations run while set: iteration q reads ACT[q % 2], its convergence
+  FL_ACT1,        //   check writes ACT[(q + 1) % 2]
+  FL_IT,          // iterations of this solve
+  FL_MAXIT,
+  FL_OVF,         // a solve of the cycle was still active after its launched iterations
+  FL_ITSUM,       // iterations summed over the cycle
+  FL_ITMAX,       // the most iterations of one solve in the cycle
+  FL_COUNT
+};
+// partial sums, two per block, one region per reduction (every consumer re-sums them in block
+// order: the same value in every block, no fence and no extra launch)
+enum { PART_NB = 0, PART_RR, PART_PQ, PART_RZ, PART_N };
+constexpr int MAX_PART = 16384;
+struct PcgDev {
+  double *sc;
+  int *fl;
+  double *part;   // PART_N regions of 2 * MAX_PART
+};
+
+// One component of a range vector: element (i0, i1, i2) at p + i0 + s1 i1 + s2 i2
+struct View {
+  void *p;
+  int s1, s2;
+};
+
+// The convergence check of a first preconditioner pass (FIN of implicit.jl:305, 319, 426)
+enum { CONV_NONE = 0, CONV_RR0 = 1, CONV_RR = 2 };
+struct Conv {
+  int mode;
+  const int *in;   // the incoming flag (NZ for RR0, ACT[q % 2] for RR)
+  int *out;        // ACT of the iterations that follow
+  int nparts;      // partials of r·r
+};
+
+// One pass of the 1D transforms along `axis` for the three components, out = M' in with
+// A[n*L + m] = M(n, m).  `addth`: out is φ and receives θ + value (implicit.jl:98-102).
+// `dot`: the pass writes z and the partials of r·z (and r·zold) at its elements.
 struct TxPass {
-  const void *in[3];
-  void *out[3];
+  View in[3], out[3];
   const void *A[3];
   int e[3][3];
   int axis;
+  int addth;
+  View th[3];
+  int dot;          // 1: r·z, 2: r·z and r·zold
+  View r[3], zold[3];
+  Conv conv;
+  PcgDev d;
 };
-template <class T>
+template <class TI, class TA, class TO>
 hipError_t launch_tx(const TxPass &p, const int *gate, hipStream_t s);
-// The same pass from double input to T (first pass of the Float32 solve) and from T to double.
-hipError_t launch_tx_d2f(const TxPass &p, const int *gate, hipStream_t s);
-hipError_t launch_tx_f2d(const TxPass &p, const int *gate, hipStream_t s);
+int tx_parts(const TxPass &p);
 
-struct ModeArgs {
-  void *X, *Y, *Z;
+// The last forward pass (axis 2) fused with the per-wavevector projector: one thread per mode.
+struct Tx2Modes {
+  const void *in;        // compact range vector after the axis-0 and axis-1 passes (T)
+  void *out;             // compact range vector of projected modes (T)
+  const void *Az[2];     // axis-2 forward matrices: [0] V_z (X and Y), [1] U_z (Z)
   const void *sx, *sy, *sz;
-  int nx, ny, nz;  // basis sizes (nodes) per axis
+  int off[3];
+  int nx, ny, nz;
   double a;
 };
 template <class T>
-hipError_t launch_modes(const ModeArgs &m, const int *gate, hipStream_t s);
+hipError_t launch_tx2_modes(const Tx2Modes &m, const int *gate, hipStream_t s);
 
-// ---------------------------------------------------------------- PCG (implicit.jl:279-446)
-// Device scalars of one solve
-enum {
-  SC_NB = 0, SC_RZ, SC_PAP, SC_ALPHA, SC_RR, SC_RZ2, SC_RZO, SC_BETA, SC_TOLNB,
-  SC_COUNT
+// PCG kernels over the box i, j, k in 2..N (the union of the ranges), node-shaped vectors
+struct Vec3 {
+  double *c[3];
 };
-enum { FL_ACTIVE = 0, FL_COLD, FL_IT, FL_MAXIT, FL_COUNT };
-struct PcgDev {
-  double *sc;       // SC_*
-  int *fl;          // FL_*: ACTIVE (iterations run while 1), COLD gate, iteration counter
-  double *part;     // partial sums (NPART x 2)
-};
-hipError_t launch_apply_A(const Geo &g, const Ranges &rg, const double *P, double *Q, const double *const D[3],
-                          double sxy, double sxz, double syx, double syz, double szx, double szy, double dx, double dy,
-                          double dz, const int *gate, hipStream_t s);
-// partial dot products over the whole range vector: part[b] = a·b (and a·c if c)
-hipError_t launch_dot(const double *a, const double *b, const double *c, int n, double *part, const int *gate,
-                      hipStream_t s);
-int dot_blocks();
-// what to do with the reduced sums
-enum {
-  FIN_NB = 0,     // nb = sqrt(Σ); ACTIVE = nb > 0 (COLD gate too)
-  FIN_RR0,        // first residual: ACTIVE = nb > 0 && sqrt(Σ) > tol nb
-  FIN_RZ0,        // rz = Σ
-  FIN_ALPHA,      // pAp = Σ; α = rz/pAp; it++
-  FIN_RR,         // rr = Σ; converged or maxit -> ACTIVE = 0
-  FIN_BETA,       // rz2 = Σ0 (rzo = Σ1): β; rz = rz2
-};
-hipError_t launch_finish(int what, PcgDev d, int flexible, const int *gate, hipStream_t s);
-// elementwise vector ops over the whole range vector
-enum {
-  V_COPY = 0,       // y = x
-  V_XR,             // x += α P, r −= α Q
-  V_P,              // P = z + β P  (and zold = z if flexible)
-  V_RESID,          // r = b − Q
-  V_START           // warm start: extrapolation or zero source
-};
-struct VecArgs {
-  int op, n;
-  double *x, *r, *P, *zold, *xprev;
-  const double *Q, *z, *b;
-  const double *sc;
+struct PcgArgs {
+  Geo g;
+  Ranges rg;
+  PcgDev d;
+  const double *D[3];
+  double sxy, sxz, syx, syz, szx, szy, dydz, dzdx, dxdy;
+  Vec3 x, r, b, z, Pold, Pnew, Q, zold, xprev;
+  int q;             // iteration of the solve (ap, xr)
   int flexible;
-  int warm, extrap, have2;
+  int warm, extrap, have2;   // start
+  int nparts_in;     // partials the kernel consumes
+  const double *th[3];       // finish: φ = θ + x
+  double *ph[3];
+  int act_slot;      // finish: the flag that tells an unfinished solve
 };
-hipError_t launch_vec(const VecArgs &v, const int *gate, hipStream_t s);
+int pcg_box_blocks(const Geo &g);
+hipError_t launch_pcg_start(const PcgArgs &a, hipStream_t s);    // nb, warm start or zero source
+hipError_t launch_pcg_resid(const PcgArgs &a, hipStream_t s);    // Q = A x, r = b − Q, r·r
+hipError_t launch_pcg_ap(const PcgArgs &a, hipStream_t s);       // β, P = z + βP, Q = A P, p·Ap
+hipError_t launch_pcg_xr(const PcgArgs &a, hipStream_t s);       // α, x += αP, r −= αQ, r·r
+hipError_t launch_pcg_finish(const PcgArgs &a, hipStream_t s);   // φ = θ + x, the cycle's counts
+hipError_t launch_pcg_cycle_reset(PcgDev d, hipStream_t s);
 
 }  // namespace tdgl

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -57,6 +58,20 @@ Bases make_bases(int nn) {
   return b;
 }
 
+// kernel classes of the per-kernel profile (tdgl_profile_get)
+enum { K_STAGE = 0, K_LINKS, K_FIELD, K_CURL, K_GHOSTS, K_SCATTER, K_TX, K_TX32, K_MODES, K_APPLYA, K_DOT, K_FINISH, K_VEC, K_COUNT };
+const char *const K_NAMES[K_COUNT] = {"band_stage", "stage_links", "current_field", "links_curl_bc", "neumann_ghosts",
+                                      "solve_scatter", "sep_transform_f64", "sep_transform_f32", "sep_modes", "pcg_apply_A",
+                                      "pcg_dot", "pcg_finish", "pcg_vector"};
+static_assert(K_COUNT == TDGL_NKERNELS, "kernel classes");
+
+struct HostSnap {
+  cplx *G[TDGL_MAX_BANDS], *F[TDGL_MAX_BANDS];
+  double *th[3], *ph[3];
+  cplx *U[3], *V[3];
+  int parity;
+  bool warm, have2;
+};
 }  // namespace
 
 struct tdgl_handle {
@@ -74,8 +89,8 @@ struct tdgl_handle {
   cplx *U[3] = {}, *V[3] = {};
   double *J[3] = {}, *h[3] = {};
   // workspace
-  cplx *W = nullptr, *W2 = nullptr, *L[3] = {}, *Q[3] = {};
-  double *Om = nullptr, *R[3] = {};
+  cplx *W = nullptr, *W2 = nullptr;
+  double *Om = nullptr;
   // masks and tables
   double *f = nullptr, *tau[3] = {}, *D[3] = {};
   double *bar[6] = {};
@@ -84,18 +99,38 @@ struct tdgl_handle {
   float *Af[3][2][2] = {};
   double *sd[3] = {};
   float *sf[3] = {};
-  double *B = nullptr, *T1 = nullptr, *T2 = nullptr;
+  double *T1 = nullptr, *T2 = nullptr;
   float *T32a = nullptr, *T32b = nullptr;
-  // PCG
-  double *x = nullptr, *r = nullptr, *z = nullptr, *P = nullptr, *Qv = nullptr, *xprev = nullptr, *zold = nullptr;
+  // right-hand side of the implicit solve and the PCG vectors: node-shaped, zero outside the ranges
+  Vec3 B{}, x{}, r{}, z{}, P{}, P2{}, Qv{}, xprev{}, zold{};
   PcgDev pd{};
   bool warm = false, have2 = false;
   int last_iters = 0;
+  int *flh = nullptr;      // pinned host copy of the flags
+  // (H, HJ) of the step: device copy read by the field boundary, pinned host staging
+  double *hp = nullptr, *hph = nullptr;
+  // the captured ladder cycle
+  bool graphs = true, capturing = false;
+  int parity = 0;          // swaps of advance! modulo 2
+  int budget = 0;          // PCG iterations per solve in the graph (0: not known yet)
+  long long gkey = -1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  HostSnap gend{};
+  double *bk[32] = {};     // device copy of a cycle's start (roll back)
+  int64_t rollbacks = 0, graph_cycles = 0;
   // counters
   int64_t steps = 0, calls = 0, pcg_iters = 0;
   double device_ms = 0;
   // host staging
   std::vector<unsigned char> stage;
+  // per-kernel profile (HIP events around each launch while enabled)
+  bool prof = false;
+  std::vector<hipEvent_t> evs;
+  std::vector<int> evcls;
+  size_t evn = 0;
+  int64_t launches[K_COUNT] = {};
+  double kms[K_COUNT] = {};
 };
 
 namespace {
@@ -153,6 +188,46 @@ int get(tdgl_handle *h, void *host, const void *dev, int n1, int n2, int n3, siz
     int rc_ = (expr);      \
     if (rc_) return rc_;   \
   } while (0)
+
+int prof_flush(tdgl_handle *h) {
+  if (h->evn == 0) return 0;
+  HIPCHK(hipStreamSynchronize(h->s));
+  for (size_t q = 0; q < h->evn; q += 2) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, h->evs[q], h->evs[q + 1]));
+    h->kms[h->evcls[q / 2]] += ms;
+    h->launches[h->evcls[q / 2]] += 1;
+  }
+  h->evn = 0;
+  return 0;
+}
+int prof_mark(tdgl_handle *h, int cls, bool begin) {
+  if (begin && h->evn + 2 > 16384) CHK(prof_flush(h));
+  if (h->evn + 1 >= h->evs.size()) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    h->evs.push_back(e);
+    HIPCHK(hipEventCreate(&e));
+    h->evs.push_back(e);
+  }
+  if (begin) {
+    if (h->evcls.size() <= h->evn / 2) h->evcls.resize(h->evn / 2 + 1);
+    h->evcls[h->evn / 2] = cls;
+    HIPCHK(hipEventRecord(h->evs[h->evn], h->s));
+  } else {
+    HIPCHK(hipEventRecord(h->evs[h->evn + 1], h->s));
+    h->evn += 2;
+  }
+  return 0;
+}
+// a launch, with its events when the profile is on
+#define PL(cls, expr)                                                                                 \
+  do {                                                                                                \
+    if (h->prof) CHK(prof_mark(h, (cls), true));                                                      \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess) return fail(TDGL_E_DEVICE, "launch %s: %s", #expr, hipGetErrorString(e_));  \
+    if (h->prof) CHK(prof_mark(h, (cls), false));                                                     \
+  } while (0)
 #define LCHK(expr)                                                                                   \
   do {                                                                                               \
     hipError_t e_ = (expr);                                                                          \
@@ -163,6 +238,10 @@ int get(tdgl_handle *h, void *host, const void *dev, int n1, int n2, int n3, siz
 StageArgs stage_base(tdgl_handle *h, int mu, const tdgl_bandcoef &cf) {
   StageArgs a{};
   for (int nu = 0; nu < h->d.nbands; ++nu) a.G[nu] = h->G[nu];
+  for (int c = 0; c < 3; ++c) {
+    a.V[c] = h->V[c];
+    a.U[c] = h->U[c];
+  }
   a.f = h->f;
   a.Om = h->Om;
   a.kx = cf.kx;
@@ -178,26 +257,13 @@ StageArgs stage_base(tdgl_handle *h, int mu, const tdgl_bandcoef &cf) {
   a.wmax = 0.5 / h->d.dt;  // _ROT_MAXDT / dt (bands.jl:209, 216)
   a.dt = h->d.dt;
   a.hard = h->d.hard;
-  return a;
-}
-
-LinkArgs link_base(tdgl_handle *h, int mode) {
-  LinkArgs a{};
-  a.mode = mode;
-  for (int c = 0; c < 3; ++c) {
-    a.L[c] = h->L[c];
-    a.Q[c] = h->Q[c];
-    a.R[c] = h->R[c];
-    a.V[c] = h->V[c];
-    a.U[c] = h->U[c];
-  }
-  a.Om = h->Om;
-  a.dt = h->d.dt;
+  a.lsrc = LS_DIRECT;
   return a;
 }
 
 // _band_rkl! (bands.jl:464-521): stage j writes buf[(j - ns) mod 3] of (F, W, W2) so that the
-// last stage lands in F and no stage writes a buffer it reads.
+// last stage lands in F and no stage writes a buffer it reads.  The links of stage j >= 2 are
+// V (V Ū)^c_j, or V _cis(R c_j) in the rotating frame, evaluated inside the stage kernel.
 int band_rkl(tdgl_handle *h, int mu, int ns) {
   const tdgl_desc &d = h->d;
   const bool kc = d.rkc[mu] != 0;
@@ -212,10 +278,11 @@ int band_rkl(tdgl_handle *h, int mu, int ns) {
     a.D = C;
     a.S = B;
     a.A = A;
-    const bool own = ex && j >= 2;
-    a.Lx = own ? h->L[0] : h->V[0];
-    a.Ly = own ? h->L[1] : h->V[1];
-    a.Lz = own ? h->L[2] : h->V[2];
+    if (ex && j >= 2) {
+      // the links of the previous stage's fraction c_{j-1} (set after stage j-1, bands.jl:502-513)
+      a.lsrc = rot ? LS_ROTLINKS : LS_ROTFRAC;
+      a.lc = kc ? d.rkc_c[mu][j - 2] : (double)((j - 1) * j) / (double)(ns * (ns + 1));
+    }
     a.ghost_fly = j >= 2;
     a.frame = rot ? (j == 1 ? FRAME_RATE : FRAME_TERM) : FRAME_NONE;
     a.h = hh;
@@ -225,49 +292,36 @@ int band_rkl(tdgl_handle *h, int mu, int ns) {
       a.cnu = kc ? d.rkc_nu[mu][j - 1] : (1.0 - j) / j;
     }
     a.turn = rot && j == ns;
-    LCHK(launch_stage(h->g, a, h->s));
-    if (rot && j == 1 && ex && ns > 1) LCHK(launch_links(h->g, link_base(h, LINK_RATE), h->s));
-    if (j < ns && ex) {
-      LinkArgs la = link_base(h, rot ? LINK_ROTLINKS : LINK_ROTFRAC);
-      la.c = kc ? d.rkc_c[mu][j - 1] : (double)(j * (j + 1)) / (double)(ns * (ns + 1));
-      LCHK(launch_links(h->g, la, h->s));
-    }
+    PL(K_STAGE, launch_stage(h->g, a, h->s));
     A = B;
     B = C;
   }
   return 0;
 }
 
-// _band_substeps! (bands.jl:378-406)
+// _band_substeps! (bands.jl:378-406): substep m >= 2 uses V Q^(m-1) (times the frame's turn of
+// each link per substep), evaluated inside the stage kernel.
 int band_substeps(tdgl_handle *h, int mu, int n) {
   const tdgl_desc &d = h->d;
   const bool rot = d.rotframe, ex = d.extrap;
   const double hh = d.dt / n;
-  if (ex) {
-    LinkArgs la = link_base(h, LINK_ROOT);
-    la.n = n;
-    LCHK(launch_links(h->g, la, h->s));
-  }
   const cplx *src = h->G[mu];
   for (int m = 1; m <= n; ++m) {
     cplx *dst = ((n - m) % 2 == 0) ? h->F[mu] : h->W;
     StageArgs a = stage_base(h, mu, d.bcs[mu]);
     a.D = dst;
     a.S = src;
-    a.Lx = ex ? h->L[0] : h->V[0];
-    a.Ly = ex ? h->L[1] : h->V[1];
-    a.Lz = ex ? h->L[2] : h->V[2];
+    if (ex && m >= 2) {
+      a.lsrc = LS_SUBSTEP;
+      a.msub = m;
+      a.nsub = n;
+      a.lframe = rot;
+    }
     a.ghost_fly = m >= 2;
     a.frame = rot ? (m == 1 ? FRAME_RATE : FRAME_TERM) : FRAME_NONE;
     a.h = hh;
     a.turn = rot && m == n;
-    LCHK(launch_stage(h->g, a, h->s));
-    if (m < n && ex) {
-      LinkArgs la = link_base(h, LINK_ADVANCE);
-      la.frame = rot;
-      la.c = hh;
-      LCHK(launch_links(h->g, la, h->s));
-    }
+    PL(K_STAGE, launch_stage(h->g, a, h->s));
     src = dst;
   }
   return 0;
@@ -283,13 +337,10 @@ int update_bands(tdgl_handle *h) {
       StageArgs a = stage_base(h, mu, d.bc[mu]);
       a.D = h->F[mu];
       a.S = h->G[mu];
-      a.Lx = h->V[0];
-      a.Ly = h->V[1];
-      a.Lz = h->V[2];
       a.frame = d.rotframe ? FRAME_RATE : FRAME_NONE;
       a.h = d.dt;
       a.turn = d.rotframe;
-      LCHK(launch_stage(h->g, a, h->s));
+      PL(K_STAGE, launch_stage(h->g, a, h->s));
     } else {
       CHK(band_substeps(h, mu, d.nsub[mu]));
     }
@@ -298,175 +349,222 @@ int update_bands(tdgl_handle *h) {
 }
 
 // ---------------------------------------------------------------------------- separable solve
-TxPass tx_pass(tdgl_handle *h, const void *in, void *out, int axis, bool inv, bool f32) {
+View node_view(tdgl_handle *h, const double *base) {
+  return View{(void *)(base + h->g.at(2, 2, 2)), h->g.px, h->g.px * h->g.py};
+}
+View compact_view(tdgl_handle *h, const void *base, int c, size_t es) {
+  return View{(void *)((const unsigned char *)base + (size_t)h->rg.off[c] * es), h->rg.e[c][0],
+              h->rg.e[c][0] * h->rg.e[c][1]};
+}
+
+TxPass tx_pass(tdgl_handle *h, int axis, bool inv, bool f32) {
   TxPass p{};
   p.axis = axis;
+  p.d = h->pd;
   for (int c = 0; c < 3; ++c) {
-    const size_t off = (size_t)h->rg.off[c];
-    const size_t es_in = 8, es_out = 8;
-    (void)es_in;
-    (void)es_out;
-    p.in[c] = in;
-    p.out[c] = out;
     for (int q = 0; q < 3; ++q) p.e[c][q] = h->rg.e[c][q];
     const int which = (axis == c) ? 1 : 0;  // U on the component's own axis, V elsewhere
     p.A[c] = f32 ? (const void *)h->Af[axis][which][inv] : (const void *)h->Ad[axis][which][inv];
-    (void)off;
   }
   return p;
 }
 
-// offsets applied per component with the element size of each side
-void tx_offsets(TxPass &p, const Ranges &rg, size_t es_in, size_t es_out) {
-  for (int c = 0; c < 3; ++c) {
-    p.in[c] = (const unsigned char *)p.in[c] + (size_t)rg.off[c] * es_in;
-    p.out[c] = (unsigned char *)p.out[c] + (size_t)rg.off[c] * es_out;
-  }
-}
+// What the last inverse pass does with the solution
+struct SepOut {
+  Vec3 *out;          // a node-shaped vector
+  bool addth;         // or φ = θ + value
+  int dot;            // with the partials of r·z (1) and r·zold (2)
+};
 
-// out = (I + aK)^{-1} in on range vectors (_sep_solve!, implicit.jl:62-83; out may be in)
-int sep_solve(tdgl_handle *h, const double *in, double *out, const int *gate) {
-  const double *src[6] = {in, h->T1, h->T2, h->T1, h->T2, h->T1};
-  double *dst[6] = {h->T1, h->T2, h->T1, h->T2, h->T1, out};
-  for (int q = 0; q < 6; ++q) {
-    const int axis = q % 3;
-    const bool inv = q >= 3;
-    TxPass p = tx_pass(h, src[q], dst[q], axis, inv, false);
-    tx_offsets(p, h->rg, 8, 8);
-    LCHK(launch_tx<double>(p, gate, h->s));
-    if (q == 2) {
-      ModeArgs m{h->T1 + h->rg.off[0], h->T1 + h->rg.off[1], h->T1 + h->rg.off[2], h->sd[0], h->sd[1], h->sd[2],
-                 h->g.Nx - 1, h->g.Ny - 1, h->g.Nz - 1, h->d.a};
-      LCHK(launch_modes<double>(m, gate, h->s));
+// (I + aK)^{-1} of the node-shaped vector `in` (_sep_solve!, implicit.jl:62-83, or in Float32
+// _precond32!, :353-373): forward passes along x and y, the pass along z fused with the mode
+// projector, the three inverse passes.  `conv`: the convergence check of the first pass.
+int sep_solve(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int *gate, bool f32, Conv conv) {
+  void *Ta = f32 ? (void *)h->T32a : (void *)h->T1;
+  void *Tb = f32 ? (void *)h->T32b : (void *)h->T2;
+  const size_t es = f32 ? 4 : 8;
+  {
+    TxPass p = tx_pass(h, 0, false, f32);
+    for (int c = 0; c < 3; ++c) {
+      p.in[c] = node_view(h, in.c[c]);
+      p.out[c] = compact_view(h, Ta, c, es);
     }
+    p.conv = conv;
+    if (f32)
+      PL(K_TX32, (launch_tx<double, float, float>(p, gate, h->s)));
+    else
+      PL(K_TX, (launch_tx<double, double, double>(p, gate, h->s)));
+  }
+  const int *g2 = conv.mode != CONV_NONE ? conv.out : gate;
+  {
+    TxPass p = tx_pass(h, 1, false, f32);
+    for (int c = 0; c < 3; ++c) {
+      p.in[c] = compact_view(h, Ta, c, es);
+      p.out[c] = compact_view(h, Tb, c, es);
+    }
+    if (f32)
+      PL(K_TX32, (launch_tx<float, float, float>(p, g2, h->s)));
+    else
+      PL(K_TX, (launch_tx<double, double, double>(p, g2, h->s)));
+  }
+  {
+    Tx2Modes m{};
+    m.in = Tb;
+    m.out = Ta;
+    m.Az[0] = f32 ? (const void *)h->Af[2][0][0] : (const void *)h->Ad[2][0][0];
+    m.Az[1] = f32 ? (const void *)h->Af[2][1][0] : (const void *)h->Ad[2][1][0];
+    m.sx = f32 ? (const void *)h->sf[0] : (const void *)h->sd[0];
+    m.sy = f32 ? (const void *)h->sf[1] : (const void *)h->sd[1];
+    m.sz = f32 ? (const void *)h->sf[2] : (const void *)h->sd[2];
+    for (int c = 0; c < 3; ++c) m.off[c] = h->rg.off[c];
+    m.nx = h->g.Nx - 1;
+    m.ny = h->g.Ny - 1;
+    m.nz = h->g.Nz - 1;
+    m.a = h->d.a;
+    if (f32)
+      PL(K_MODES, launch_tx2_modes<float>(m, g2, h->s));
+    else
+      PL(K_MODES, launch_tx2_modes<double>(m, g2, h->s));
+  }
+  for (int ax = 0; ax < 2; ++ax) {
+    TxPass p = tx_pass(h, ax, true, f32);
+    void *src = ax == 0 ? Ta : Tb, *dst = ax == 0 ? Tb : Ta;
+    for (int c = 0; c < 3; ++c) {
+      p.in[c] = compact_view(h, src, c, es);
+      p.out[c] = compact_view(h, dst, c, es);
+    }
+    if (f32)
+      PL(K_TX32, (launch_tx<float, float, float>(p, g2, h->s)));
+    else
+      PL(K_TX, (launch_tx<double, double, double>(p, g2, h->s)));
+  }
+  {
+    TxPass p = tx_pass(h, 2, true, f32);
+    for (int c = 0; c < 3; ++c) {
+      p.in[c] = compact_view(h, Ta, c, es);
+      if (o.addth) {
+        p.out[c] = node_view(h, h->ph[c]);
+        p.th[c] = node_view(h, h->th[c]);
+      } else {
+        p.out[c] = node_view(h, o.out->c[c]);
+      }
+      if (o.dot) {
+        p.r[c] = node_view(h, h->r.c[c]);
+        p.zold[c] = node_view(h, h->zold.c[c]);
+      }
+    }
+    p.addth = o.addth;
+    p.dot = o.dot;
+    if (f32)
+      PL(K_TX32, (launch_tx<float, float, double>(p, g2, h->s)));
+    else
+      PL(K_TX, (launch_tx<double, double, double>(p, g2, h->s)));
   }
   return 0;
 }
 
-// The Float32 separable inverse (_precond32!, implicit.jl:353-373)
-int sep_solve32(tdgl_handle *h, const double *in, double *out, const int *gate) {
-  const void *src[6] = {in, h->T32a, h->T32b, h->T32a, h->T32b, h->T32a};
-  void *dst[6] = {h->T32a, h->T32b, h->T32a, h->T32b, h->T32a, out};
-  for (int q = 0; q < 6; ++q) {
-    const int axis = q % 3;
-    const bool inv = q >= 3;
-    TxPass p = tx_pass(h, src[q], dst[q], axis, inv, true);
-    if (q == 0) {
-      tx_offsets(p, h->rg, 8, 4);
-      LCHK(launch_tx_d2f(p, gate, h->s));
-    } else if (q == 5) {
-      tx_offsets(p, h->rg, 4, 8);
-      LCHK(launch_tx_f2d(p, gate, h->s));
-    } else {
-      tx_offsets(p, h->rg, 4, 4);
-      LCHK(launch_tx<float>(p, gate, h->s));
-    }
-    if (q == 2) {
-      ModeArgs m{h->T32a + h->rg.off[0], h->T32a + h->rg.off[1], h->T32a + h->rg.off[2], h->sf[0], h->sf[1], h->sf[2],
-                 h->g.Nx - 1, h->g.Ny - 1, h->g.Nz - 1, h->d.a};
-      LCHK(launch_modes<float>(m, gate, h->s));
-    }
-  }
-  return 0;
-}
+int sep_last_parts(tdgl_handle *h) { return tx_parts(tx_pass(h, 2, true, false)); }
 
 // ---------------------------------------------------------------------------- PCG
-int pcg_iteration(tdgl_handle *h, bool flexible, bool f32) {
+PcgArgs pcg_args(tdgl_handle *h) {
   const tdgl_desc &d = h->d;
-  const int n = h->rg.total;
-  const int *act = h->pd.fl + FL_ACTIVE;
-  LCHK(launch_apply_A(h->g, h->rg, h->P, h->Qv, h->D, d.sxy, d.sxz, d.syx, d.syz, d.szx, d.szy, d.dx, d.dy, d.dz, act, h->s));
-  LCHK(launch_dot(h->P, h->Qv, nullptr, n, h->pd.part, act, h->s));
-  LCHK(launch_finish(FIN_ALPHA, h->pd, 0, act, h->s));
-  VecArgs v{};
-  v.n = n;
-  v.x = h->x;
-  v.r = h->r;
-  v.P = h->P;
-  v.Q = h->Qv;
-  v.z = h->z;
-  v.zold = h->zold;
-  v.sc = h->pd.sc;
-  v.flexible = flexible;
-  v.op = V_XR;
-  LCHK(launch_vec(v, act, h->s));
-  LCHK(launch_dot(h->r, h->r, nullptr, n, h->pd.part, act, h->s));
-  LCHK(launch_finish(FIN_RR, h->pd, 0, act, h->s));
-  CHK(f32 ? sep_solve32(h, h->r, h->z, act) : sep_solve(h, h->r, h->z, act));
-  LCHK(launch_dot(h->r, h->z, flexible ? h->zold : nullptr, n, h->pd.part, act, h->s));
-  LCHK(launch_finish(FIN_BETA, h->pd, flexible, act, h->s));
-  v.op = V_P;
-  LCHK(launch_vec(v, act, h->s));
+  PcgArgs a{};
+  a.g = h->g;
+  a.rg = h->rg;
+  a.d = h->pd;
+  for (int c = 0; c < 3; ++c) {
+    a.D[c] = h->D[c];
+    a.th[c] = h->th[c];
+    a.ph[c] = h->ph[c];
+  }
+  a.sxy = d.sxy;
+  a.sxz = d.sxz;
+  a.syx = d.syx;
+  a.syz = d.syz;
+  a.szx = d.szx;
+  a.szy = d.szy;
+  a.dydz = d.dy * d.dz;
+  a.dzdx = d.dz * d.dx;
+  a.dxdy = d.dx * d.dy;
+  a.x = h->x;
+  a.r = h->r;
+  a.b = h->B;
+  a.z = h->z;
+  a.Q = h->Qv;
+  a.zold = h->zold;
+  a.xprev = h->xprev;
+  return a;
+}
+
+// Iteration q of the loop of _implicit_curl_fast! / implicit_curl! (implicit.jl:416-437,
+// 309-325): the direction and A P, the update of x and r, the preconditioner (whose first pass
+// decides on convergence, whose last pass gives r·z).  Gated by ACT[q % 2].
+int pcg_iteration(tdgl_handle *h, int q, bool flexible) {
+  PcgArgs a = pcg_args(h);
+  a.q = q;
+  a.flexible = flexible;
+  a.Pnew = (q % 2 == 0) ? h->P : h->P2;
+  a.Pold = (q % 2 == 0) ? h->P2 : h->P;
+  a.nparts_in = sep_last_parts(h);
+  PL(K_APPLYA, launch_pcg_ap(a, h->s));
+  a.nparts_in = pcg_box_blocks(h->g);
+  PL(K_VEC, launch_pcg_xr(a, h->s));
+  Conv cv{CONV_RR, h->pd.fl + FL_ACT0 + (q & 1), h->pd.fl + FL_ACT0 + ((q + 1) & 1), pcg_box_blocks(h->g)};
+  SepOut o{&h->z, false, flexible ? 2 : 1};
+  CHK(sep_solve(h, h->r, o, nullptr, flexible, cv));
   return 0;
 }
 
 // implicit_curl!(s, ::PCGSolver) / _implicit_curl_fast! (implicit.jl:279-446), after the field
-// kernel has written b = φₑ − θ (h->B) and scaled φ by D.
+// kernel has written b = φₑ − θ and the partials of b·b, and scaled φ by D.  Eager: the host
+// reads the flags between batches of iterations.  Captured: `budget` iterations, each gated.
 int implicit_pcg(tdgl_handle *h) {
   const tdgl_desc &d = h->d;
   const bool fast = d.pcg_fast || d.pcg_f32;
   const bool f32 = d.pcg_f32 != 0;
   const bool extrap = d.pcg_extrap && d.pcg_fast;
   const bool flexible = fast && f32;
-  const int n = h->rg.total;
-  const int *nz = h->pd.fl + FL_COLD;
-  const int *act = h->pd.fl + FL_ACTIVE;
-  LCHK(launch_dot(h->B, h->B, nullptr, n, h->pd.part, nullptr, h->s));
-  LCHK(launch_finish(FIN_NB, h->pd, 0, nullptr, h->s));
-  VecArgs v{};
-  v.n = n;
-  v.x = h->x;
-  v.r = h->r;
-  v.P = h->P;
-  v.Q = h->Qv;
-  v.z = h->z;
-  v.b = h->B;
-  v.zold = h->zold;
-  v.xprev = h->xprev;
-  v.sc = h->pd.sc;
-  v.flexible = flexible;
-  v.warm = h->warm;
-  v.extrap = fast && extrap;
-  v.have2 = h->have2;
-  v.op = V_START;
-  LCHK(launch_vec(v, nullptr, h->s));
-  if (!h->warm) CHK(sep_solve(h, h->B, h->x, nz));  // cold start: x = M⁻¹ b in Float64
-  const bool warm_before = h->warm;
-  h->have2 = fast && warm_before && extrap;
-  h->warm = true;
-  LCHK(launch_apply_A(h->g, h->rg, h->x, h->Qv, h->D, d.sxy, d.sxz, d.syx, d.syz, d.szx, d.szy, d.dx, d.dy, d.dz, nz, h->s));
-  v.op = V_RESID;
-  LCHK(launch_vec(v, nz, h->s));
-  LCHK(launch_dot(h->r, h->r, nullptr, n, h->pd.part, nz, h->s));
-  LCHK(launch_finish(FIN_RR0, h->pd, 0, nz, h->s));
-  CHK(flexible ? sep_solve32(h, h->r, h->z, act) : sep_solve(h, h->r, h->z, act));
-  v.op = V_COPY;
-  LCHK(launch_vec(v, act, h->s));
-  LCHK(launch_dot(h->r, h->z, nullptr, n, h->pd.part, act, h->s));
-  LCHK(launch_finish(FIN_RZ0, h->pd, 0, act, h->s));
-  // iterations in batches; the host reads the flags between batches only
-  int launched = 0;
-  int batch = h->last_iters > 0 ? h->last_iters : 4;
-  int fl[FL_COUNT];
-  for (;;) {
-    HIPCHK(hipMemcpyAsync(fl, h->pd.fl, sizeof fl, hipMemcpyDeviceToHost, h->s));
-    HIPCHK(hipStreamSynchronize(h->s));
-    if (!fl[FL_ACTIVE] || launched >= d.pcg_maxit) break;
-    const int nb = std::min(batch, d.pcg_maxit - launched);
-    for (int q = 0; q < nb; ++q) CHK(pcg_iteration(h, flexible, flexible));
-    launched += nb;
-    batch = 2;
+  const int *nz = h->pd.fl + FL_NZ;
+  PcgArgs a = pcg_args(h);
+  a.warm = h->warm;
+  a.extrap = fast && extrap;
+  a.have2 = h->have2;
+  a.nparts_in = field_blocks(h->g);
+  PL(K_VEC, launch_pcg_start(a, h->s));
+  if (!h->warm) {  // cold start: x = M⁻¹ b in Float64
+    SepOut o{&h->x, false, 0};
+    CHK(sep_solve(h, h->B, o, nz, false, Conv{CONV_NONE, nullptr, nullptr, 0}));
   }
-  h->last_iters = fl[FL_IT];
-  h->pcg_iters += fl[FL_IT];
-  const double *th[3] = {h->th[0], h->th[1], h->th[2]};
-  double *ph[3] = {h->ph[0], h->ph[1], h->ph[2]};
-  LCHK(launch_scatter(h->g, h->rg, h->x, th, ph, h->s));
+  h->have2 = fast && h->warm && extrap;
+  h->warm = true;
+  PL(K_APPLYA, launch_pcg_resid(a, h->s));
+  {
+    Conv cv{CONV_RR0, nz, h->pd.fl + FL_ACT0, pcg_box_blocks(h->g)};
+    SepOut o{&h->z, false, 1};
+    CHK(sep_solve(h, h->r, o, nullptr, flexible, cv));
+  }
+  int q = 0;
+  if (h->capturing) {
+    for (; q < h->budget; ++q) CHK(pcg_iteration(h, q, flexible));
+  } else {
+    int batch = h->last_iters > 0 ? h->last_iters : 4;
+    for (;;) {
+      HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
+      HIPCHK(hipStreamSynchronize(h->s));
+      if (!h->flh[FL_ACT0 + (q & 1)] || q >= d.pcg_maxit) break;
+      const int nbt = std::min(batch, d.pcg_maxit - q);
+      for (int t = 0; t < nbt; ++t, ++q) CHK(pcg_iteration(h, q, flexible));
+      batch = 2;
+    }
+    h->last_iters = h->flh[FL_IT];
+  }
+  a.act_slot = q & 1;
+  PL(K_SCATTER, launch_pcg_finish(a, h->s));
   return 0;
 }
 
 // ---------------------------------------------------------------------------- one step
-int step_once(tdgl_handle *h, double H, double HJ) {
+int step_once(tdgl_handle *h) {
   const tdgl_desc &d = h->d;
   CHK(update_bands(h));
   FieldArgs fa{};
@@ -479,8 +577,8 @@ int step_once(tdgl_handle *h, double H, double HJ) {
     fa.h[c] = h->h[c];
     fa.tau[c] = h->tau[c];
     fa.D[c] = h->D[c];
+    fa.B[c] = h->B.c[c];
   }
-  fa.B = h->B;
   fa.rg = h->rg;
   for (int ax = 0; ax < 3; ++ax)
     for (int mu = 0; mu < TDGL_MAX_BANDS; ++mu) fa.wa[ax][mu] = d.wa[ax][mu];
@@ -494,12 +592,11 @@ int step_once(tdgl_handle *h, double H, double HJ) {
   fa.nb = d.nbands;
   fa.scale = d.field_mode == TDGL_FIELD_PCG || d.field_mode == TDGL_FIELD_EXPLICIT_SCALED;
   fa.gather = d.field_mode == TDGL_FIELD_PCG || d.field_mode == TDGL_FIELD_SEPARABLE;
-  LCHK(launch_field(h->g, fa, h->s));
+  fa.part = d.field_mode == TDGL_FIELD_PCG ? h->pd.part + (size_t)PART_NB * 2 * MAX_PART : nullptr;
+  PL(K_FIELD, launch_field(h->g, fa, h->s));
   if (d.field_mode == TDGL_FIELD_SEPARABLE) {
-    CHK(sep_solve(h, h->B, h->B, nullptr));
-    const double *th[3] = {h->th[0], h->th[1], h->th[2]};
-    double *ph[3] = {h->ph[0], h->ph[1], h->ph[2]};
-    LCHK(launch_scatter(h->g, h->rg, h->B, th, ph, h->s));
+    SepOut o{nullptr, true, 0};
+    CHK(sep_solve(h, h->B, o, nullptr, false, Conv{CONV_NONE, nullptr, nullptr, 0}));
   } else if (d.field_mode == TDGL_FIELD_PCG) {
     CHK(implicit_pcg(h));
   }
@@ -509,8 +606,7 @@ int step_once(tdgl_handle *h, double H, double HJ) {
     ca.U[c] = h->U[c];
     ca.h[c] = h->h[c];
   }
-  ca.H = H;
-  ca.HJ = HJ;
+  ca.hp = h->hp;
   ca.bar = d.selffield_bc == TDGL_BC_BAR;
   ca.bar_hy_z = h->bar[0];
   ca.bar_hy_y = h->bar[1];
@@ -521,20 +617,149 @@ int step_once(tdgl_handle *h, double H, double HJ) {
   ca.dydz = d.dy * d.dz;
   ca.dzdx = d.dz * d.dx;
   ca.dxdy = d.dx * d.dy;
-  LCHK(launch_curl(h->g, ca, h->s));
+  PL(K_CURL, launch_curl(h->g, ca, h->s));
   GhostArgs ga{};
   for (int mu = 0; mu < d.nbands; ++mu) ga.F[mu] = h->F[mu];
   for (int c = 0; c < 3; ++c) ga.U[c] = h->U[c];
   ga.nb = d.nbands;
-  LCHK(launch_ghosts(h->g, ga, h->s));
+  PL(K_GHOSTS, launch_ghosts(h->g, ga, h->s));
   // advance! (field.jl:248-263): F equals G outside the block, so G <- F is a swap
   for (int mu = 0; mu < d.nbands; ++mu) std::swap(h->G[mu], h->F[mu]);
   for (int c = 0; c < 3; ++c) {
     std::swap(h->th[c], h->ph[c]);
     std::swap(h->V[c], h->U[c]);
   }
+  h->parity ^= 1;
   return 0;
 }
+
+// ---------------------------------------------------------------------------- a ladder cycle
+// The arrays a step reads at its start (the rest is rewritten before it is read)
+CopyList backup_list(tdgl_handle *h, bool save) {
+  CopyList c{};
+  const int n = h->g.n;
+  auto add = [&](double *live, double *bk, int len) {
+    c.src[c.cnt] = save ? live : bk;
+    c.dst[c.cnt] = save ? bk : live;
+    c.n[c.cnt] = len;
+    ++c.cnt;
+  };
+  int q = 0;
+  for (int mu = 0; mu < h->d.nbands; ++mu) add((double *)h->G[mu], h->bk[q++], 2 * n);
+  for (int k = 0; k < 3; ++k) {
+    add(h->th[k], h->bk[q++], n);
+    add(h->ph[k], h->bk[q++], n);
+    add((double *)h->U[k], h->bk[q++], 2 * n);
+    add((double *)h->V[k], h->bk[q++], 2 * n);
+    add(h->J[k], h->bk[q++], n);
+    add(h->h[k], h->bk[q++], n);
+  }
+  if (h->d.field_mode == TDGL_FIELD_PCG)
+    for (int k = 0; k < 3; ++k) {
+      add(h->x.c[k], h->bk[q++], n);
+      add(h->xprev.c[k], h->bk[q++], n);
+    }
+  return c;
+}
+
+HostSnap snap(tdgl_handle *h) {
+  HostSnap s{};
+  for (int mu = 0; mu < TDGL_MAX_BANDS; ++mu) {
+    s.G[mu] = h->G[mu];
+    s.F[mu] = h->F[mu];
+  }
+  for (int c = 0; c < 3; ++c) {
+    s.th[c] = h->th[c];
+    s.ph[c] = h->ph[c];
+    s.U[c] = h->U[c];
+    s.V[c] = h->V[c];
+  }
+  s.parity = h->parity;
+  s.warm = h->warm;
+  s.have2 = h->have2;
+  return s;
+}
+void restore(tdgl_handle *h, const HostSnap &s) {
+  for (int mu = 0; mu < TDGL_MAX_BANDS; ++mu) {
+    h->G[mu] = s.G[mu];
+    h->F[mu] = s.F[mu];
+  }
+  for (int c = 0; c < 3; ++c) {
+    h->th[c] = s.th[c];
+    h->ph[c] = s.ph[c];
+    h->U[c] = s.U[c];
+    h->V[c] = s.V[c];
+  }
+  h->parity = s.parity;
+  h->warm = s.warm;
+  h->have2 = s.have2;
+}
+
+int steps_eager(tdgl_handle *h, int n) {
+  for (int q = 0; q < n; ++q) CHK(step_once(h));
+  return 0;
+}
+
+int destroy_graph(tdgl_handle *h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  return 0;
+}
+
+// n steps as one captured graph: captured once per (n, pointer parity, warm, have2, budget),
+// replayed after.  With the PCG, the graph holds `budget` gated iterations per solve; a cycle in
+// which a solve was still active after them is rolled back from a device copy of its start and
+// stepped eagerly, and the budget grows.
+int cycle_graph(tdgl_handle *h, int n) {
+  const bool pcg = h->d.field_mode == TDGL_FIELD_PCG;
+  const long long key = ((((long long)n * 2 + h->parity) * 2 + h->warm) * 2 + h->have2) * 1024 + (pcg ? h->budget : 0);
+  const HostSnap start = snap(h);
+  if (!h->gexec || h->gkey != key) {
+    CHK(destroy_graph(h));
+    h->capturing = true;
+    HIPCHK(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
+    int rc = steps_eager(h, n);
+    hipGraph_t gph = nullptr;
+    hipError_t e = hipStreamEndCapture(h->s, &gph);
+    h->capturing = false;
+    if (rc) {
+      if (gph) (void)hipGraphDestroy(gph);
+      restore(h, start);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(TDGL_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    h->graph = gph;
+    HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    h->gkey = key;
+    h->gend = snap(h);
+    restore(h, start);
+  }
+  if (pcg) {
+    PL(K_VEC, launch_copies(backup_list(h, true), h->s));
+    PL(K_VEC, launch_pcg_cycle_reset(h->pd, h->s));
+  }
+  HIPCHK(hipGraphLaunch(h->gexec, h->s));
+  restore(h, h->gend);
+  if (pcg) {
+    HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    if (h->flh[FL_OVF]) {
+      // the roll back: the start state into the start pointers, then the eager steps
+      restore(h, start);
+      PL(K_VEC, launch_copies(backup_list(h, false), h->s));
+      h->budget = std::min(h->d.pcg_maxit, std::max(2 * h->budget, h->flh[FL_ITMAX] + 4));
+      h->rollbacks += 1;
+      PL(K_VEC, launch_pcg_cycle_reset(h->pd, h->s));
+      CHK(steps_eager(h, n));
+      HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
+    }
+  }
+  return 0;
+}
+
+int d_maxit(tdgl_handle *h) { return h->d.pcg_maxit; }
 
 int check_desc(const tdgl_desc *d, char *why, size_t n) {
   auto no = [&](const char *m) {
@@ -652,14 +877,7 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     }
     if (stages || subs) ALLOC(h->W, n);
     if (stages) ALLOC(h->W2, n);
-    if (d.extrap && (stages || subs))
-      for (int c = 0; c < 3; ++c) ALLOC(h->L[c], n);
-    if (d.extrap && subs)
-      for (int c = 0; c < 3; ++c) ALLOC(h->Q[c], n);
-    if (d.rotframe) {
-      ALLOC(h->Om, n);
-      for (int c = 0; c < 3; ++c) ALLOC(h->R[c], n);
-    }
+    if (d.rotframe) ALLOC(h->Om, n);
     CHK(put(h, h->f, d.f, d.Nx + 1, d.Ny + 1, d.Nz + 1, 8));
     const double *taus[3] = {d.taux, d.tauy, d.tauz};
     for (int c = 0; c < 3; ++c) CHK(put(h, h->tau[c], taus[c], d.Nx + 1, d.Ny + 1, d.Nz + 1, 8));
@@ -708,30 +926,50 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
         ALLOC(h->sf[ax], nn[ax]);
         CHK(upload_vec(h->sf[ax], s32));
       }
-      ALLOC(h->B, rg.total);
+      for (int c = 0; c < 3; ++c) ALLOC(h->B.c[c], n);
       ALLOC(h->T1, rg.total);
       ALLOC(h->T2, rg.total);
     }
     if (d.field_mode == TDGL_FIELD_PCG) {
-      ALLOC(h->x, rg.total);
-      ALLOC(h->r, rg.total);
-      ALLOC(h->z, rg.total);
-      ALLOC(h->P, rg.total);
-      ALLOC(h->Qv, rg.total);
-      ALLOC(h->xprev, rg.total);
-      ALLOC(h->zold, rg.total);
+      Vec3 *vs[8] = {&h->x, &h->r, &h->z, &h->P, &h->P2, &h->Qv, &h->xprev, &h->zold};
+      for (Vec3 *v : vs)
+        for (int c = 0; c < 3; ++c) ALLOC(v->c[c], n);
       ALLOC(h->T32a, rg.total);
       ALLOC(h->T32b, rg.total);
       ALLOC(h->pd.sc, SC_COUNT);
       ALLOC(h->pd.fl, FL_COUNT);
-      ALLOC(h->pd.part, 2 * dot_blocks());
+      ALLOC(h->pd.part, (size_t)PART_N * 2 * MAX_PART);
+      if (pcg_box_blocks(g) > MAX_PART || field_blocks(g) > MAX_PART) return fail(TDGL_E_UNSUPPORTED, "grid too large for the PCG reductions");
       std::vector<double> sc(SC_COUNT, 0.0);
-      sc[SC_TOLNB] = d.pcg_tol;
+      sc[SC_TOL] = d.pcg_tol;
       CHK(upload_vec(h->pd.sc, sc));
       std::vector<int> fl(FL_COUNT, 0);
       fl[FL_MAXIT] = d.pcg_maxit;
       CHK(upload_vec(h->pd.fl, fl));
     }
+    ALLOC(h->hp, 2);
+    HIPCHK(hipHostMalloc((void **)&h->hph, 2 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&h->flh, FL_COUNT * sizeof(int), hipHostMallocDefault));
+    // the copy of a cycle's start for the roll back of a captured PCG cycle
+    if (d.field_mode == TDGL_FIELD_PCG) {
+      int q = 0;
+      for (int mu = 0; mu < d.nbands; ++mu) ALLOC(h->bk[q++], 2 * (size_t)n);
+      for (int k = 0; k < 3; ++k) {
+        ALLOC(h->bk[q++], n);
+        ALLOC(h->bk[q++], n);
+        ALLOC(h->bk[q++], 2 * (size_t)n);
+        ALLOC(h->bk[q++], 2 * (size_t)n);
+        ALLOC(h->bk[q++], n);
+        ALLOC(h->bk[q++], n);
+      }
+      for (int k = 0; k < 6; ++k) ALLOC(h->bk[q++], n);
+    }
+    // TDGL_HIP_GRAPH=0: every step launched from the host; TDGL_HIP_BUDGET=k: the captured PCG
+    // starts with k iterations per solve instead of learning it from an eager first cycle
+    const char *ge = std::getenv("TDGL_HIP_GRAPH");
+    h->graphs = !(ge && ge[0] == '0');
+    const char *gb = std::getenv("TDGL_HIP_BUDGET");
+    if (gb && d.field_mode == TDGL_FIELD_PCG) h->budget = std::max(1, std::min(d.pcg_maxit, std::atoi(gb)));
     return 0;
   };
   int rc = run();
@@ -744,6 +982,10 @@ int tdgl_destroy(tdgl_handle *h) {
   if (!h) return 0;
   if (h->s) (void)hipStreamSynchronize(h->s);
   for (void *p : h->allocs) (void)hipFree(p);
+  destroy_graph(h);
+  if (h->hph) (void)hipHostFree(h->hph);
+  if (h->flh) (void)hipHostFree(h->flh);
+  for (hipEvent_t e : h->evs) (void)hipEventDestroy(e);
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
   if (h->s) (void)hipStreamDestroy(h->s);
@@ -828,10 +1070,31 @@ int tdgl_step(tdgl_handle *h, int32_t n, double H, double HJ) {
   if (!h) return fail(TDGL_E_ARG, "null handle");
   if (n < 0) return fail(TDGL_E_ARG, "negative step count");
   if (!std::isfinite(H) || !std::isfinite(HJ)) return fail(TDGL_E_ARG, "non-finite H or HJ");
+  const bool pcg = h->d.field_mode == TDGL_FIELD_PCG;
+  h->hph[0] = H;
+  h->hph[1] = HJ;
+  HIPCHK(hipMemcpyAsync(h->hp, h->hph, 2 * sizeof(double), hipMemcpyHostToDevice, h->s));
   HIPCHK(hipEventRecord(h->e0, h->s));
-  for (int q = 0; q < n; ++q) CHK(step_once(h, H, HJ));
+  if (n > 0) {
+    const bool eager = h->prof || !h->graphs || (pcg && h->budget == 0);
+    if (eager) {
+      if (pcg) PL(K_VEC, launch_pcg_cycle_reset(h->pd, h->s));
+      CHK(steps_eager(h, n));
+      if (pcg) HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
+    } else {
+      CHK(cycle_graph(h, n));
+      h->graph_cycles += 1;
+    }
+  }
   HIPCHK(hipEventRecord(h->e1, h->s));
   HIPCHK(hipStreamSynchronize(h->s));
+  if (h->prof) CHK(prof_flush(h));
+  if (pcg && n > 0) {
+    h->pcg_iters += h->flh[FL_ITSUM];
+    h->last_iters = h->flh[FL_IT];
+    // the graph's iterations per solve: the most one solve took, and a margin
+    if (h->budget == 0) h->budget = std::min(d_maxit(h), std::max(4, h->flh[FL_ITMAX] + 3));
+  }
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, h->e0, h->e1));
   h->device_ms += ms;
@@ -844,10 +1107,18 @@ int tdgl_get_pcg(tdgl_handle *h, tdgl_pcg *p) {
   if (!h || !p) return fail(TDGL_E_ARG, "null argument");
   if (h->d.field_mode != TDGL_FIELD_PCG) return fail(TDGL_E_ARG, "the stepper has no PCG solver");
   HIPCHK(hipStreamSynchronize(h->s));
+  std::vector<double> node((size_t)h->g.n);
   for (int c = 0; c < 3; ++c) {
-    const size_t n = (size_t)h->rg.e[c][0] * h->rg.e[c][1] * h->rg.e[c][2];
-    if (p->x[c]) HIPCHK(hipMemcpy(p->x[c], h->x + h->rg.off[c], n * 8, hipMemcpyDeviceToHost));
-    if (p->xprev[c]) HIPCHK(hipMemcpy(p->xprev[c], h->xprev + h->rg.off[c], n * 8, hipMemcpyDeviceToHost));
+    double *dst[2] = {p->x[c], p->xprev[c]};
+    const double *src[2] = {h->x.c[c], h->xprev.c[c]};
+    for (int w = 0; w < 2; ++w) {
+      if (!dst[w]) continue;
+      HIPCHK(hipMemcpy(node.data(), src[w], node.size() * 8, hipMemcpyDeviceToHost));
+      const int e0 = h->rg.e[c][0], e1 = h->rg.e[c][1], e2 = h->rg.e[c][2];
+      for (int k = 0; k < e2; ++k)
+        for (int j = 0; j < e1; ++j)
+          for (int i = 0; i < e0; ++i) dst[w][i + (size_t)e0 * (j + (size_t)e1 * k)] = node[h->g.at(i + 2, j + 2, k + 2)];
+    }
   }
   p->warm = h->warm;
   p->have2 = h->have2;
@@ -859,10 +1130,19 @@ int tdgl_set_pcg(tdgl_handle *h, const tdgl_pcg *p) {
   if (!h || !p) return fail(TDGL_E_ARG, "null argument");
   if (h->d.field_mode != TDGL_FIELD_PCG) return fail(TDGL_E_ARG, "the stepper has no PCG solver");
   HIPCHK(hipStreamSynchronize(h->s));
+  std::vector<double> node((size_t)h->g.n);
   for (int c = 0; c < 3; ++c) {
-    const size_t n = (size_t)h->rg.e[c][0] * h->rg.e[c][1] * h->rg.e[c][2];
-    if (p->x[c]) HIPCHK(hipMemcpy(h->x + h->rg.off[c], p->x[c], n * 8, hipMemcpyHostToDevice));
-    if (p->xprev[c]) HIPCHK(hipMemcpy(h->xprev + h->rg.off[c], p->xprev[c], n * 8, hipMemcpyHostToDevice));
+    const double *src[2] = {p->x[c], p->xprev[c]};
+    double *dst[2] = {h->x.c[c], h->xprev.c[c]};
+    for (int w = 0; w < 2; ++w) {
+      if (!src[w]) continue;
+      std::fill(node.begin(), node.end(), 0.0);
+      const int e0 = h->rg.e[c][0], e1 = h->rg.e[c][1], e2 = h->rg.e[c][2];
+      for (int k = 0; k < e2; ++k)
+        for (int j = 0; j < e1; ++j)
+          for (int i = 0; i < e0; ++i) node[h->g.at(i + 2, j + 2, k + 2)] = src[w][i + (size_t)e0 * (j + (size_t)e1 * k)];
+      HIPCHK(hipMemcpy(dst[w], node.data(), node.size() * 8, hipMemcpyHostToDevice));
+    }
   }
   h->warm = p->warm != 0;
   h->have2 = p->have2 != 0;
@@ -872,6 +1152,9 @@ int tdgl_set_pcg(tdgl_handle *h, const tdgl_pcg *p) {
 
 int tdgl_get_stats(tdgl_handle *h, tdgl_stats *st) {
   if (!h || !st) return fail(TDGL_E_ARG, "null argument");
+  st->graph_cycles = h->graph_cycles;
+  st->rollbacks = h->rollbacks;
+  st->pcg_budget = h->budget;
   st->steps = h->steps;
   st->calls = h->calls;
   st->pcg_iters = h->pcg_iters;
@@ -880,5 +1163,29 @@ int tdgl_get_stats(tdgl_handle *h, tdgl_stats *st) {
   st->device_bytes = h->bytes;
   return 0;
 }
+
+int tdgl_profile_enable(tdgl_handle *h, int32_t on) {
+  if (!h) return fail(TDGL_E_ARG, "null handle");
+  CHK(prof_flush(h));
+  if (on && !h->prof)
+    for (int k = 0; k < K_COUNT; ++k) {
+      h->launches[k] = 0;
+      h->kms[k] = 0;
+    }
+  h->prof = on != 0;
+  return 0;
+}
+
+int tdgl_profile_get(tdgl_handle *h, tdgl_profile *p) {
+  if (!h || !p) return fail(TDGL_E_ARG, "null argument");
+  CHK(prof_flush(h));
+  for (int k = 0; k < K_COUNT; ++k) {
+    p->launches[k] = h->launches[k];
+    p->ms[k] = h->kms[k];
+  }
+  return 0;
+}
+
+const char *tdgl_kernel_name(int32_t k) { return (k >= 0 && k < K_COUNT) ? K_NAMES[k] : ""; }
 
 }  // extern "C"

@@ -9,7 +9,7 @@ import threading
 
 import numpy as np
 
-from .abi import PackedDesc, PcgState, StatsC, state_struct
+from .abi import NKERNELS, PackedDesc, PcgState, ProfileC, StatsC, state_struct
 from .scheme import Stepper
 from .state import State
 
@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "_lib", "libtdglhip.so")
 
 SYMBOLS = ("tdgl_supported", "tdgl_create", "tdgl_destroy", "tdgl_upload_state", "tdgl_download_state", "tdgl_step",
            "tdgl_get_pcg", "tdgl_set_pcg", "tdgl_get_stats", "tdgl_last_error", "tdgl_device_count",
-           "tdgl_set_device")
+           "tdgl_set_device", "tdgl_profile_enable", "tdgl_profile_get", "tdgl_kernel_name")
 
 _lib = None
 _lock = threading.Lock()
@@ -50,6 +50,10 @@ def load():
             L.tdgl_last_error.restype = ct.c_char_p
             L.tdgl_device_count.restype = ct.c_int
             L.tdgl_set_device.argtypes = [ct.c_int32]
+            L.tdgl_profile_enable.argtypes = [vp, ct.c_int32]
+            L.tdgl_profile_get.argtypes = [vp, vp]
+            L.tdgl_kernel_name.argtypes = [ct.c_int32]
+            L.tdgl_kernel_name.restype = ct.c_char_p
             _lib = L
         return _lib
 
@@ -118,6 +122,16 @@ class HipStepper:
 
     def set_pcg(self, p: PcgState) -> None:
         _check(load().tdgl_set_pcg(self.h, ct.byref(p.struct())), "tdgl_set_pcg")
+
+    def profile(self, on: bool) -> None:
+        """Per-kernel HIP-event timing of every launch while on (clears the totals)."""
+        _check(load().tdgl_profile_enable(self.h, int(on)), "tdgl_profile_enable")
+
+    def profile_get(self) -> dict:
+        c = ProfileC()
+        _check(load().tdgl_profile_get(self.h, ct.byref(c)), "tdgl_profile_get")
+        L = load()
+        return {L.tdgl_kernel_name(k).decode(): (int(c.launches[k]), float(c.ms[k])) for k in range(NKERNELS)}
 
     def stats(self) -> dict:
         c = StatsC()

@@ -110,40 +110,57 @@ Basis make_basis(int nn) {
 }
 
 // out = A transformed by M1' along dim 1, M2' along dim 2, M3' along dim 3 (_tx!, implicit.jl:30-37);
-// M_d is (n_d x m_d) column-major; `tr` transposes M_d (the inverse transform).
+// M_d is (n_d x n_d) column-major; `tr` transposes M_d (the inverse transform).  Every output
+// element is the sum over n in increasing order; the loops run the contiguous index innermost
+// (vectorised across independent outputs, which keeps each element's order of additions).
 template <class R>
 void tx(std::vector<R> &out, const std::vector<R> &A, int n1, int n2, int n3, const std::vector<R> &M1,
         const std::vector<R> &M2, const std::vector<R> &M3, bool tr) {
-  // square bases: m_d = n_d
-  auto el = [&](const std::vector<R> &M, int n, int a, int b) -> R {  // (M or M') [a, b]
-    return tr ? M[b + (size_t)n * a] : M[a + (size_t)n * b];
+  // Mt[d][n*L + m] = (M_d or M_d')(n, m): the row of the n-th input across the outputs m
+  auto rows = [&](const std::vector<R> &M, int L) {
+    std::vector<R> t((size_t)L * L);
+    for (int n = 0; n < L; ++n)
+      for (int m = 0; m < L; ++m) t[(size_t)n * L + m] = tr ? M[m + (size_t)L * n] : M[n + (size_t)L * m];
+    return t;
   };
+  const std::vector<R> R1 = rows(M1, n1), R2 = rows(M2, n2), R3 = rows(M3, n3);
   std::vector<R> T1((size_t)n1 * n2 * n3), T2((size_t)n1 * n2 * n3);
   // T1[m, j, k] = sum_n M1[n, m] A[n, j, k]
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < n3; ++k)
-    for (int j = 0; j < n2; ++j)
-      for (int m = 0; m < n1; ++m) {
-        R acc = 0;
-        for (int n = 0; n < n1; ++n) acc += el(M1, n1, n, m) * A[n + (size_t)n1 * (j + (size_t)n2 * k)];
-        T1[m + (size_t)n1 * (j + (size_t)n2 * k)] = acc;
+    for (int j = 0; j < n2; ++j) {
+      R *o = &T1[(size_t)n1 * (j + (size_t)n2 * k)];
+      const R *a = &A[(size_t)n1 * (j + (size_t)n2 * k)];
+      for (int m = 0; m < n1; ++m) o[m] = 0;
+      for (int n = 0; n < n1; ++n) {
+        const R an = a[n];
+        const R *r = &R1[(size_t)n * n1];
+        for (int m = 0; m < n1; ++m) o[m] += r[m] * an;
       }
+    }
   // T2[i, m, k] = sum_n T1[i, n, k] M2[n, m]
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < n3; ++k)
-    for (int m = 0; m < n2; ++m)
-      for (int i = 0; i < n1; ++i) {
-        R acc = 0;
-        for (int n = 0; n < n2; ++n) acc += T1[i + (size_t)n1 * (n + (size_t)n2 * k)] * el(M2, n2, n, m);
-        T2[i + (size_t)n1 * (m + (size_t)n2 * k)] = acc;
+    for (int m = 0; m < n2; ++m) {
+      R *o = &T2[(size_t)n1 * (m + (size_t)n2 * k)];
+      for (int i = 0; i < n1; ++i) o[i] = 0;
+      for (int n = 0; n < n2; ++n) {
+        const R w = R2[(size_t)n * n2 + m];
+        const R *t = &T1[(size_t)n1 * (n + (size_t)n2 * k)];
+        for (int i = 0; i < n1; ++i) o[i] += t[i] * w;
       }
+    }
   // out[i, j, m] = sum_n T2[i, j, n] M3[n, m]
   out.assign((size_t)n1 * n2 * n3, 0);
-  for (int m = 0; m < n3; ++m)
-    for (int j = 0; j < n2; ++j)
-      for (int i = 0; i < n1; ++i) {
-        R acc = 0;
-        for (int n = 0; n < n3; ++n) acc += T2[i + (size_t)n1 * (j + (size_t)n2 * n)] * el(M3, n3, n, m);
-        out[i + (size_t)n1 * (j + (size_t)n2 * m)] = acc;
-      }
+#pragma omp parallel for schedule(static)
+  for (int m = 0; m < n3; ++m) {
+    R *o = &out[(size_t)n1 * n2 * m];
+    for (int n = 0; n < n3; ++n) {
+      const R w = R3[(size_t)n * n3 + m];
+      const R *t = &T2[(size_t)n1 * n2 * n];
+      for (size_t q = 0; q < (size_t)n1 * n2; ++q) o[q] += t[q] * w;
+    }
+  }
 }
 
 struct Sep {

