@@ -1,0 +1,262 @@
+# ============================================================================
+#  TDGLHIP — the ladder cycle of TDGL 2.0 on an MI355X through the C ABI of libtdglhip
+#  (include/tdglhip.h).  The same exports as TDGLGPU (gpu/TDGLGPU/src/TDGLGPU.jl:29), so that
+#  run_gpu.jl-style drivers switch by `TDGLHIP.enable!()` in place of `TDGLGPU.enable!(backend)`.
+#
+#      gs = GPUStepper(st)            # st: a TDGL.Stepper the HIP stepper runs (supported(st))
+#      g  = device_state(gs, s)       # s: a TDGL.State
+#      gpu_steps!(g, gs, n; H, HJ)    # n steps on the device
+#      host_state!(s, g)              # back into s
+#
+#  The library path: ENV["TDGLHIP_LIB"], else juliatdgl_amd/_lib/libtdglhip.so of this repository.
+# ============================================================================
+module TDGLHIP
+
+using Libdl
+using TDGL
+using TDGL: State, Stepper, SepSolver, PCGSolver
+
+export GPUStepper, GState, device_state, device_state!, host_state!, gpu_step!, gpu_steps!, supported, enable!,
+       disable!, execution_stats
+
+const MAXB = 4
+const MAXS = 64
+const _LIBPATH = Ref{String}("")
+libpath() = isempty(_LIBPATH[]) ? get(ENV, "TDGLHIP_LIB",
+    normpath(joinpath(@__DIR__, "..", "..", "..", "juliatdgl_amd", "_lib", "libtdglhip.so"))) : _LIBPATH[]
+
+# ---- the C structs (include/tdglhip.h), field for field -------------------------------------
+struct BandCoefC
+    kx::Float64; ky::Float64; kz::Float64; c::Float64; eps::Float64; beta::Float64; gamma2::Float64
+    cpl::NTuple{MAXB,Float64}
+end
+struct DescC
+    abi_version::Int32; nbands::Int32
+    Nx::Int32; Ny::Int32; Nz::Int32
+    x1::Int32; x2::Int32; y1::Int32; y2::Int32; z1::Int32; z2::Int32
+    dx::Float64; dy::Float64; dz::Float64; dt::Float64
+    bc::NTuple{MAXB,BandCoefC}; bcs::NTuple{MAXB,BandCoefC}
+    nsub::NTuple{MAXB,Int32}; stages::NTuple{MAXB,Int32}; rkc::NTuple{MAXB,Int32}
+    divisor::NTuple{MAXB,Float64}
+    rkc_mu::NTuple{MAXB,NTuple{MAXS,Float64}}; rkc_nu::NTuple{MAXB,NTuple{MAXS,Float64}}
+    rkc_c::NTuple{MAXB,NTuple{MAXS,Float64}}
+    wa::NTuple{3,NTuple{MAXB,Float64}}
+    d1::Float64; sxy::Float64; sxz::Float64; syx::Float64; syz::Float64; szx::Float64; szy::Float64; a::Float64
+    field_mode::Int32; selffield_bc::Int32
+    extrap::Int32; rotframe::Int32; hard::Int32
+    pcg_tol::Float64; pcg_maxit::Int32; pcg_fast::Int32; pcg_f32::Int32; pcg_extrap::Int32
+    f::Ptr{Float64}; taux::Ptr{Float64}; tauy::Ptr{Float64}; tauz::Ptr{Float64}
+    Dx::Ptr{Float64}; Dy::Ptr{Float64}; Dz::Ptr{Float64}
+    bar_hy_z::Ptr{Float64}; bar_hy_y::Ptr{Float64}; bar_hz_z::Ptr{Float64}; bar_hz_y::Ptr{Float64}
+    bar_hz_x::Ptr{Float64}; bar_hy_x::Ptr{Float64}
+end
+struct StateC
+    G::NTuple{MAXB,Ptr{Float64}}; F::NTuple{MAXB,Ptr{Float64}}
+    thx::Ptr{Float64}; thy::Ptr{Float64}; thz::Ptr{Float64}
+    phx::Ptr{Float64}; phy::Ptr{Float64}; phz::Ptr{Float64}
+    Ux::Ptr{Float64}; Uy::Ptr{Float64}; Uz::Ptr{Float64}
+    Vx::Ptr{Float64}; Vy::Ptr{Float64}; Vz::Ptr{Float64}
+    Jx::Ptr{Float64}; Jy::Ptr{Float64}; Jz::Ptr{Float64}
+    hx::Ptr{Float64}; hy::Ptr{Float64}; hz::Ptr{Float64}
+end
+struct PcgC
+    x::NTuple{3,Ptr{Float64}}; xprev::NTuple{3,Ptr{Float64}}
+    warm::Int32; have2::Int32; iters::Int32
+end
+
+_pad(t, n, z) = ntuple(i -> i <= length(t) ? t[i] : z, n)
+_bc(b, N, µ) = BandCoefC(b.kx, b.ky, b.kz, b.c, b.ε, b.β, b.γ2, _pad(ntuple(ν -> ν == µ ? 0.0 : b.cpl[ν], N), MAXB, 0.0))
+const _ZBC = BandCoefC(0, 0, 0, 0, 0, 0, 0, ntuple(_ -> 0.0, MAXB))
+
+"""
+    supported(st) -> (ok::Bool, why::String)
+
+Whether the HIP stepper runs the step of this Stepper (TDGLGPU.jl:39-44), and if not the first
+option it does not have.
+"""
+function supported(st::Stepper{N}) where N
+    N <= MAXB || return (false, "more than $MAXB bands")
+    st.heat === nothing || return (false, "the heat model")
+    st.drag === nothing || return (false, "interband drag")
+    st.noise === nothing || return (false, "thermal noise")
+    st.mt === nothing || return (false, "materials per region")
+    st.qpx === nothing || return (false, "normal fluids with a finite exchange")
+    st.open === nothing || return (false, "the open boundary")
+    st.masks.vac === nothing || return (false, "a sculpted sample")
+    any(>(MAXS), st.rkl) && return (false, "more than $MAXS stages")
+    (true, "")
+end
+
+# the description of st, and the arrays it points to (kept alive by the caller)
+function _desc(st::Stepper{N}) where N
+    gr = st.grid; mk = st.masks; fc = st.fc
+    keep = Any[]
+    hold(a) = (b = Array{Float64}(a); push!(keep, b); pointer(b))
+    mode = st.imp isa PCGSolver ? 2 : st.imp isa SepSolver ? 1 : st.D !== nothing ? 3 : 0
+    D = mode == 2 ? st.imp.D : mode == 3 ? st.D : nothing
+    pcg = mode == 2 ? st.imp : nothing
+    fast = pcg !== nothing && pcg.fast !== nothing
+    rk = ntuple(µ -> st.rkc !== nothing && st.rkc[µ] !== nothing ? st.rkc[µ] : nothing, N)
+    div = ntuple(N) do µ
+        st.rkl[µ] > 0 ? (rk[µ] !== nothing ? rk[µ].q : Float64(st.rkl[µ] * (st.rkl[µ] + 1) ÷ 2)) : Float64(st.nsub[µ])
+    end
+    tab(f) = _pad(ntuple(µ -> rk[µ] === nothing ? ntuple(_ -> 0.0, MAXS) : _pad(Tuple(f(rk[µ])), MAXS, 0.0), N),
+                  MAXB, ntuple(_ -> 0.0, MAXS))
+    bar = st.bar
+    bp(k) = bar === nothing ? Ptr{Float64}(0) : hold(getproperty(bar, k))
+    d = DescC(1, N, gr.Nx, gr.Ny, gr.Nz, gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2, gr.dx, gr.dy, gr.dz, st.dt,
+              _pad(ntuple(µ -> _bc(st.bc[µ], N, µ), N), MAXB, _ZBC), _pad(ntuple(µ -> _bc(st.bcs[µ], N, µ), N), MAXB, _ZBC),
+              _pad(Int32.(st.nsub), MAXB, Int32(0)), _pad(Int32.(st.rkl), MAXB, Int32(0)),
+              _pad(ntuple(µ -> Int32(rk[µ] !== nothing), N), MAXB, Int32(0)), _pad(div, MAXB, 0.0),
+              tab(k -> k.μ), tab(k -> k.ν), tab(k -> k.c),
+              ntuple(i -> _pad(st.wa[i], MAXB, 0.0), 3),
+              fc.d1, fc.sxy, fc.sxz, fc.syx, fc.syz, fc.szx, fc.szy, fc.a,
+              mode, bar === nothing ? 0 : 1, st.extrap, st.rotframe, st.hard,
+              pcg === nothing ? 0.0 : pcg.tol, pcg === nothing ? 0 : pcg.maxit,
+              Int32(fast), Int32(fast && pcg.fast.f32), Int32(fast && pcg.fast.extrap),
+              hold(mk.f), hold(mk.τx), hold(mk.τy), hold(mk.τz),
+              D === nothing ? Ptr{Float64}(0) : hold(D[1]), D === nothing ? Ptr{Float64}(0) : hold(D[2]),
+              D === nothing ? Ptr{Float64}(0) : hold(D[3]),
+              bp(:hy_z), bp(:hy_y), bp(:hz_z), bp(:hz_y), bp(:hz_x), bp(:hy_x))
+    d, keep
+end
+
+_lib() = (Libdl.dlopen(libpath()); libpath())
+function _check(rc, what)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:tdgl_last_error, libpath()), Cstring, ()))
+    error("$what failed ($rc): $msg")
+end
+
+"""
+    GPUStepper(st) -> GPUStepper
+
+The device copy of the stepper `st` (GPUStepper(st, backend), TDGLGPU.jl:1299-1316): coefficients,
+masks, work arrays and one run's state.  Errors when `st` has an option the HIP stepper does not run.
+"""
+mutable struct GPUStepper{N}
+    st::Stepper{N}
+    h::Ptr{Cvoid}
+end
+function GPUStepper(st::Stepper{N}) where N
+    ok, why = supported(st)
+    ok || error("TDGLHIP does not run $why yet (the CPU step does: TDGL.step!)")
+    _lib()
+    d, keep = _desc(st)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve keep begin
+        _check(ccall((:tdgl_create, libpath()), Cint, (Ref{DescC}, Ref{Ptr{Cvoid}}), d, out), "tdgl_create")
+    end
+    gs = GPUStepper{N}(st, out[])
+    finalizer(g -> g.h == C_NULL || (ccall((:tdgl_destroy, libpath()), Cint, (Ptr{Cvoid},), g.h); g.h = C_NULL), gs)
+    gs
+end
+
+"""The state of a run on the device (held by its GPUStepper's handle)."""
+struct GState{N}
+    gs::GPUStepper{N}
+end
+
+_ptr(a) = Ptr{Float64}(pointer(a))
+_statec(s::State{N}) where N =
+    StateC(_pad(ntuple(µ -> _ptr(s.G[µ]), N), MAXB, Ptr{Float64}(0)), _pad(ntuple(µ -> _ptr(s.F[µ]), N), MAXB, Ptr{Float64}(0)),
+           _ptr(s.θx), _ptr(s.θy), _ptr(s.θz), _ptr(s.φx), _ptr(s.φy), _ptr(s.φz),
+           _ptr(s.Ux), _ptr(s.Uy), _ptr(s.Uz), _ptr(s.Vx), _ptr(s.Vy), _ptr(s.Vz),
+           _ptr(s.Jx), _ptr(s.Jy), _ptr(s.Jz), _ptr(s.hx), _ptr(s.hy), _ptr(s.hz))
+
+"""device_state(gs, s) (TDGLGPU.jl:64-66): the state `s` on the device."""
+function device_state(gs::GPUStepper{N}, s::State{N}) where N
+    g = GState{N}(gs)
+    device_state!(g, s)
+end
+"""device_state!(g, s) (TDGLGPU.jl:69-74)."""
+function device_state!(g::GState{N}, s::State{N}) where N
+    GC.@preserve s _check(ccall((:tdgl_upload_state, libpath()), Cint, (Ptr{Cvoid}, Ref{StateC}), g.gs.h, _statec(s)),
+                          "tdgl_upload_state")
+    g
+end
+"""host_state!(s, g): every field of the device state into `s`."""
+function host_state!(s::State{N}, g::GState{N}) where N
+    GC.@preserve s _check(ccall((:tdgl_download_state, libpath()), Cint, (Ptr{Cvoid}, Ref{StateC}), g.gs.h, _statec(s)),
+                          "tdgl_download_state")
+    s
+end
+"""gpu_step!(g, gs; H, HJ): one step (step!, gtdgl/src/step.jl:13-35) on the device."""
+gpu_step!(g::GState, gs::GPUStepper; H::Float64, HJ::Float64) = gpu_steps!(g, gs, 1; H=H, HJ=HJ)
+"""gpu_steps!(g, gs, n; H, HJ) (TDGLGPU.jl:2065-2071): n steps, synchronous."""
+function gpu_steps!(g::GState, gs::GPUStepper, n::Integer; H::Float64, HJ::Float64)
+    _check(ccall((:tdgl_step, libpath()), Cint, (Ptr{Cvoid}, Int32, Float64, Float64), gs.h, Int32(n), H, HJ), "tdgl_step")
+    g
+end
+
+# the PCG history of the Stepper to and from the device (_pcg_up!/_pcg_down!, TDGLGPU.jl:217-224)
+function _pcg_up!(gs::GPUStepper, S::PCGSolver)
+    f = S.fast
+    xp = f === nothing ? S.x : f.xprev
+    p = PcgC(ntuple(c -> _ptr(S.x[c]), 3), ntuple(c -> _ptr(xp[c]), 3), S.warm[1], f === nothing ? 0 : f.have2[1],
+             S.iters[1])
+    GC.@preserve S _check(ccall((:tdgl_set_pcg, libpath()), Cint, (Ptr{Cvoid}, Ref{PcgC}), gs.h, p), "tdgl_set_pcg")
+end
+function _pcg_down!(S::PCGSolver, gs::GPUStepper)
+    f = S.fast
+    xp = f === nothing ? [zeros(size(S.x[c])) for c in 1:3] : f.xprev
+    p = Ref(PcgC(ntuple(c -> _ptr(S.x[c]), 3), ntuple(c -> _ptr(xp[c]), 3), 0, 0, 0))
+    GC.@preserve S xp _check(ccall((:tdgl_get_pcg, libpath()), Cint, (Ptr{Cvoid}, Ptr{PcgC}), gs.h, p), "tdgl_get_pcg")
+    S.warm[1] = p[].warm != 0
+    f === nothing || (f.have2[1] = p[].have2 != 0)
+    S.iters[1] = p[].iters
+    S
+end
+
+# ---- the ladder hook (TDGLGPU.jl:2076-2144) -----------------------------------------------------
+const _LAST = Ref{Any}(nothing)
+const _DEV = Ref{Any}(nothing)
+const _FALLBACK = Ref(:cpu)
+const _COUNTS = Dict(:bc => 0, :bs => 0, :cc => 0, :cs => 0)
+
+"""execution_stats() -> (backend, backend_cycles, backend_steps, cpu_fallback_cycles, cpu_fallback_steps)"""
+execution_stats() = (backend="HIP", backend_cycles=_COUNTS[:bc], backend_steps=_COUNTS[:bs],
+                     cpu_fallback_cycles=_COUNTS[:cc], cpu_fallback_steps=_COUNTS[:cs])
+
+"""
+    enable!(; fallback=:cpu)
+
+Every ladder of TDGL steps its cycles on the HIP stepper (TDGL.STEPS, gtdgl/src/protocol.jl:301-316); an
+unsupported Stepper runs on the CPU with a warning, or throws with `fallback=:error`.
+"""
+function enable!(; fallback::Symbol=:cpu)
+    fallback in (:cpu, :error) || throw(ArgumentError("fallback must be :cpu or :error"))
+    _LAST[] = nothing; _DEV[] = nothing; _FALLBACK[] = fallback
+    for k in keys(_COUNTS); _COUNTS[k] = 0; end
+    TDGL.STEPS[] = (s, st, n; H, HJ) -> _hip_cycle!(s, st, n; H=H, HJ=HJ)
+    nothing
+end
+disable!() = (TDGL.STEPS[] = nothing; _LAST[] = nothing; _DEV[] = nothing; nothing)
+
+function _hip_cycle!(s::State, st::Stepper, n::Int; H::Float64, HJ::Float64)
+    if _LAST[] === nothing || _LAST[][1] !== st
+        ok, why = supported(st)
+        if !ok
+            _FALLBACK[] === :error && throw(ArgumentError("TDGLHIP: $why is unsupported; CPU fallback is disabled"))
+            @warn "TDGLHIP: $why is unsupported; this stepper runs on the CPU"
+        end
+        _LAST[] = (st, ok ? GPUStepper(st) : nothing)
+        _DEV[] = nothing
+    end
+    gs = _LAST[][2]
+    if gs === nothing
+        for _ in 1:n; step!(s, st; H=H, HJ=HJ); end
+        _COUNTS[:cc] += 1; _COUNTS[:cs] += n
+        return s
+    end
+    _DEV[] === nothing || _DEV[][1] !== s ? (_DEV[] = (s, device_state(gs, s))) : device_state!(_DEV[][2], s)
+    g = _DEV[][2]
+    st.imp isa PCGSolver && _pcg_up!(gs, st.imp)
+    gpu_steps!(g, gs, n; H=H, HJ=HJ)
+    st.imp isa PCGSolver && _pcg_down!(st.imp, gs)
+    host_state!(s, g)
+    _COUNTS[:bc] += 1; _COUNTS[:bs] += n
+    s
+end
+
+end # module

@@ -186,7 +186,8 @@ struct PcgArgs {
 int pcg_box_blocks(const Geo &g);
 hipError_t launch_pcg_start(const PcgArgs &a, hipStream_t s);    // nb, warm start or zero source
 hipError_t launch_pcg_resid(const PcgArgs &a, hipStream_t s);    // Q = A x, r = b − Q, r·r
-hipError_t launch_pcg_ap(const PcgArgs &a, hipStream_t s);       // β, P = z + βP, Q = A P, p·Ap
+hipError_t launch_pcg_p(const PcgArgs &a, hipStream_t s);        // β, P = z + βP, zold = z
+hipError_t launch_pcg_ap(const PcgArgs &a, hipStream_t s);       // Q = A P, p·Ap
 hipError_t launch_pcg_xr(const PcgArgs &a, hipStream_t s);       // α, x += αP, r −= αQ, r·r
 hipError_t launch_pcg_finish(const PcgArgs &a, hipStream_t s);   // φ = θ + x, the cycle's counts
 hipError_t launch_pcg_cycle_reset(PcgDev d, hipStream_t s);

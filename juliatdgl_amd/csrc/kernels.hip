@@ -461,7 +461,10 @@ __device__ __forceinline__ double *region_w(const PcgDev &d, int r) { return d.p
 // a small GEMM per component, Out(m, o) = Σ_n A(n, m) In(n, o) over the "other" index o of the
 // range.  LDS tiles of 32 (m) x 32 (o) outputs with 32-deep chunks of n, each thread a 2 x 2
 // register tile; every output sums over n in increasing order (the order of the CPU step).
-constexpr int TXM = 32, TXO = 32, TXK = 32;
+constexpr int TXM = 32, TXO = 32;
+// contraction depth staged per round: 128 (Float32) or 64 (Float64) values of n, 33.8 KB of LDS
+template <class TA> struct TxDepth { static constexpr int K = 64; };
+template <> struct TxDepth<float> { static constexpr int K = 128; };
 
 // (i0, i1, i2) of (n along the axis, o)
 __device__ __forceinline__ void tx_coords(int axis, int e0, int e1, int n, int o, int &i0, int &i1, int &i2) {
@@ -483,6 +486,7 @@ __device__ __forceinline__ int voff(const View &v, int i0, int i1, int i2) { ret
 
 template <class TI, class TA, class TO>
 __global__ void __launch_bounds__(256) k_tx(TxPass p, const int *gate) {
+  constexpr int TXK = TxDepth<TA>::K;
   __shared__ TA As[TXK][TXM + 1];
   __shared__ TA Is[TXK][TXO + 1];
   __shared__ double sh[RED_T];
@@ -544,6 +548,7 @@ __global__ void __launch_bounds__(256) k_tx(TxPass p, const int *gate) {
       }
       __syncthreads();
       const int kmax = (L - n0) < TXK ? (L - n0) : TXK;
+#pragma unroll 4
       for (int kk = 0; kk < kmax; ++kk) {
         const TA a0 = As[kk][tm], a1 = As[kk][tm + 1];
         const TA i0 = Is[kk][to], i1 = Is[kk][to + 1];
@@ -645,6 +650,7 @@ __global__ void __launch_bounds__(256) k_tx2_modes(Tx2Modes m, const int *gate) 
     const T *col = in + m.off[0] + (mx - 1) + (nx - 1) * my;
     const int st = (nx - 1) * ny;
     T acc = 0;
+#pragma unroll 8
     for (int n = 0; n < nz; ++n) acc += col[st * n] * AV[n * nz + mz];
     x1 = acc;
   }
@@ -652,6 +658,7 @@ __global__ void __launch_bounds__(256) k_tx2_modes(Tx2Modes m, const int *gate) 
     const T *col = in + m.off[1] + mx + nx * (my - 1);
     const int st = nx * (ny - 1);
     T acc = 0;
+#pragma unroll 8
     for (int n = 0; n < nz; ++n) acc += col[st * n] * AV[n * nz + mz];
     y1 = acc;
   }
@@ -659,6 +666,7 @@ __global__ void __launch_bounds__(256) k_tx2_modes(Tx2Modes m, const int *gate) 
     const T *col = in + m.off[2] + mx + nx * my;
     const int st = nx * ny;
     T acc = 0;
+#pragma unroll 8
     for (int n = 0; n < nz - 1; ++n) acc += col[st * n] * AU[n * (nz - 1) + (mz - 1)];
     z1 = acc;
   }
@@ -797,8 +805,8 @@ __global__ void __launch_bounds__(RED_T) k_pcg_resid(PcgArgs a) {
 }
 
 // iteration q: rz (and β) from the partials of the preconditioner's last pass, P = z + βP
-// (P = z on the first), zold = z, Q = A P and the partials of P·Q (implicit.jl:416-437)
-__global__ void __launch_bounds__(RED_T) k_pcg_ap(PcgArgs a) {
+// (P = z on the first) and zold = z (implicit.jl:412-413, 428-436)
+__global__ void __launch_bounds__(RED_T) k_pcg_p(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
   __shared__ double sh[RED_T];
   const int tid = threadIdx.x;
@@ -811,21 +819,29 @@ __global__ void __launch_bounds__(RED_T) k_pcg_ap(PcgArgs a) {
     beta = a.flexible ? (rz2 - rzo) / rz : rz2 / rz;
   }
   if (b.bid == 0 && tid == 0) a.d.sc[SC_RZ0 + (a.q & 1)] = rz2;
-  PDir P{};
   for (int comp = 0; comp < 3; ++comp) {
-    P.z[comp] = a.z.c[comp];
-    P.Po[comp] = a.Pold.c[comp];
+    if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+    const int c = b.c;
+    const double zv = a.z.c[comp][c];
+    a.Pnew.c[comp][c] = a.q == 0 ? zv : zv + beta * a.Pold.c[comp][c];
+    if (a.flexible) a.zold.c[comp][c] = zv;
   }
-  P.beta = beta;
-  P.first = a.q == 0;
+}
+
+// iteration q: Q = A P and the partials of P·Q (implicit.jl:418-420)
+__global__ void __launch_bounds__(RED_T) k_pcg_ap(PcgArgs a) {
+  if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
+  __shared__ double sh[RED_T];
+  const int tid = threadIdx.x;
+  const BoxIdx b = box_idx(a.g);
+  PDir P{};
+  for (int comp = 0; comp < 3; ++comp) P.z[comp] = a.Pnew.c[comp];
   double s = 0.0;
   for (int comp = 0; comp < 3; ++comp) {
     if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
-    const double p = P.at(comp, b.c);
+    const double p = a.Pnew.c[comp][b.c];
     const double Aq = apply_one(a, P, comp, b.i, b.j, b.k, b.c, p);
     a.Q.c[comp][b.c] = Aq;
-    a.Pnew.c[comp][b.c] = p;
-    if (a.flexible) a.zold.c[comp][b.c] = a.z.c[comp][b.c];
     s += p * Aq;
   }
   const double r0 = block_sum_t(s, sh, tid);
@@ -883,6 +899,7 @@ static hipError_t launch_box(void (*k)(PcgArgs), const PcgArgs &a, hipStream_t s
 }
 hipError_t launch_pcg_start(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_start, a, s); }
 hipError_t launch_pcg_resid(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_resid, a, s); }
+hipError_t launch_pcg_p(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_p, a, s); }
 hipError_t launch_pcg_ap(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_ap, a, s); }
 hipError_t launch_pcg_xr(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_xr, a, s); }
 hipError_t launch_pcg_finish(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_finish, a, s); }

@@ -506,6 +506,7 @@ int pcg_iteration(tdgl_handle *h, int q, bool flexible) {
   a.Pnew = (q % 2 == 0) ? h->P : h->P2;
   a.Pold = (q % 2 == 0) ? h->P2 : h->P;
   a.nparts_in = sep_last_parts(h);
+  PL(K_VEC, launch_pcg_p(a, h->s));
   PL(K_APPLYA, launch_pcg_ap(a, h->s));
   a.nparts_in = pcg_box_blocks(h->g);
   PL(K_VEC, launch_pcg_xr(a, h->s));
