@@ -164,6 +164,45 @@ struct Tx2Modes {
 template <class T>
 hipError_t launch_tx2_modes(const Tx2Modes &m, const int *gate, hipStream_t s);
 
+// Two passes of the transforms in one launch, for each z-plane of each component: along x with
+// X, then along y with Y (out(m, q) = Σ_j Y(j, q) Σ_i X(i, m) in(i, j), each sum in increasing
+// order, the x result rounded to T as a pass of its own would store it).  One block per
+// (component, plane, slice of PLANE_MS(T) rows of m).  The y extent is at most PLANE_LMAX.
+constexpr int PLANE_LMAX = 256;
+struct PlanePass {
+  View in[3], out[3];
+  const void *X[3], *Y[3];
+  int e[3][3];
+  int addth;
+  View th[3];
+  int dot;
+  View r[3], zold[3];
+  Conv conv;
+  PcgDev d;
+};
+template <class TI, class T, class TO>
+hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s);
+template <class T>
+int plane_parts(const PlanePass &p);
+template <class T>
+bool plane_fits(const int e[3][3]);
+
+// The z pass, the per-wavevector projector and the inverse z pass in one launch: a block holds
+// whole z-columns of modes (floor(256 / nz) columns of nz threads), so the inverse pass reads
+// the projected modes from LDS.  nz <= 256.
+struct ZModes {
+  const void *in;        // compact range vector after the forward x and y passes (T)
+  void *out;             // compact range vector before the inverse x and y passes (T)
+  const void *Az[2];     // forward z: [0] V_z (X and Y), [1] U_z (Z)
+  const void *Ai[2];     // inverse z
+  const void *sx, *sy, *sz;
+  int off[3];
+  int nx, ny, nz;
+  double a;
+};
+template <class T>
+hipError_t launch_zmodes(const ZModes &m, const int *gate, hipStream_t s);
+
 // PCG kernels over the box i, j, k in 2..N (the union of the ranges), node-shaped vectors
 struct Vec3 {
   double *c[3];
@@ -188,6 +227,7 @@ hipError_t launch_pcg_start(const PcgArgs &a, hipStream_t s);    // nb, warm sta
 hipError_t launch_pcg_resid(const PcgArgs &a, hipStream_t s);    // Q = A x, r = b − Q, r·r
 hipError_t launch_pcg_p(const PcgArgs &a, hipStream_t s);        // β, P = z + βP, zold = z
 hipError_t launch_pcg_ap(const PcgArgs &a, hipStream_t s);       // Q = A P, p·Ap
+hipError_t launch_pcg_pap(const PcgArgs &a, hipStream_t s);      // β, P = z + βP, zold = z, Q = A P, p·Ap
 hipError_t launch_pcg_xr(const PcgArgs &a, hipStream_t s);       // α, x += αP, r −= αQ, r·r
 hipError_t launch_pcg_finish(const PcgArgs &a, hipStream_t s);   // φ = θ + x, the cycle's counts
 hipError_t launch_pcg_cycle_reset(PcgDev d, hipStream_t s);

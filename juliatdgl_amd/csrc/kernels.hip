@@ -432,25 +432,73 @@ hipError_t launch_copies(const CopyList &c, hipStream_t s) {
 // Deterministic sums: a fixed assignment of elements to threads, a fixed LDS tree per block,
 // block partials in block order.  A consumer sums a producer's partials itself (every block the
 // same value) after the producer's kernel has ended.
-constexpr int RED_T = 256;
+constexpr int RED_T = 256;  // threads of every reducing block: four wavefronts
 
+// Σ over a wavefront by xor butterflies: every lane ends with the same bits (each step adds the
+// same two values in both lanes of a pair)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Σ over a block of RED_T threads: the wavefront sums, then the four in a fixed order.  `sh`
+// holds at least 8 doubles.
 __device__ __forceinline__ double block_sum_t(double v, double *sh, int tid) {
-  sh[tid] = v;
+  v = wave_sum(v);
+  if ((tid & 63) == 0) sh[tid >> 6] = v;
   __syncthreads();
-  for (int w = RED_T / 2; w > 0; w >>= 1) {
-    if (tid < w) sh[tid] += sh[tid + w];
-    __syncthreads();
-  }
-  const double r = sh[0];
+  const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
   __syncthreads();
   return r;
 }
+__device__ __forceinline__ void block_sum2_t(double &a, double &b, double *sh, int tid) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((tid & 63) == 0) {
+    sh[tid >> 6] = a;
+    sh[4 + (tid >> 6)] = b;
+  }
+  __syncthreads();
+  a = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  b = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+  __syncthreads();
+}
 
-// Σ over the partials [slot][which] of a region, identical in every block
+// Σ over the partials [slot][which] of a region, identical in every block.  The loads of a
+// thread are issued together, eight at a time (one memory latency for up to 2048 partials).
+constexpr int PART_UNROLL = 8;
 __device__ __forceinline__ double sum_parts(const double *part, int n, int which, double *sh, int tid) {
   double s = 0.0;
-  for (int b = tid; b < n; b += RED_T) s += part[2 * b + which];
+  for (int b0 = 0; b0 < n; b0 += PART_UNROLL * RED_T) {
+    double v[PART_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PART_UNROLL; ++u) {
+      v[u] = part[2 * min(b0 + tid + u * RED_T, n - 1) + which];  // clamped: no load waits under a condition
+    }
+#pragma unroll
+    for (int u = 0; u < PART_UNROLL; ++u) s += (b0 + tid + u * RED_T < n) ? v[u] : 0.0;
+  }
   return block_sum_t(s, sh, tid);
+}
+// both partials of every slot
+__device__ __forceinline__ void sum_parts2(const double *part, int n, double &s0, double &s1, double *sh, int tid) {
+  s0 = 0.0;
+  s1 = 0.0;
+  for (int b0 = 0; b0 < n; b0 += PART_UNROLL * RED_T) {
+    double2 v[PART_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PART_UNROLL; ++u) {
+      v[u] = reinterpret_cast<const double2 *>(part)[min(b0 + tid + u * RED_T, n - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < PART_UNROLL; ++u) {
+      const bool in = b0 + tid + u * RED_T < n;
+      s0 += in ? v[u].x : 0.0;
+      s1 += in ? v[u].y : 0.0;
+    }
+  }
+  block_sum2_t(s0, s1, sh, tid);
 }
 
 __device__ __forceinline__ const double *region(const PcgDev &d, int r) { return d.part + (size_t)r * 2 * MAX_PART; }
@@ -487,9 +535,12 @@ __device__ __forceinline__ int voff(const View &v, int i0, int i1, int i2) { ret
 template <class TI, class TA, class TO>
 __global__ void __launch_bounds__(256) k_tx(TxPass p, const int *gate) {
   constexpr int TXK = TxDepth<TA>::K;
+  constexpr int NL = TXK * TXM / 256;  // loads per thread of each operand per chunk
+  static_assert(TXM == 32 && TXO == 32 && NL * 256 == TXK * TXM, "tile shape");
   __shared__ TA As[TXK][TXM + 1];
   __shared__ TA Is[TXK][TXO + 1];
-  __shared__ double sh[RED_T];
+  __shared__ double sh[8];
+  __shared__ int colb[TXO];
   const int tid = threadIdx.x;
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   if (p.conv.mode != CONV_NONE) {
@@ -515,38 +566,77 @@ __global__ void __launch_bounds__(256) k_tx(TxPass p, const int *gate) {
   const int axis = p.axis;
   const int L = axis == 0 ? e0 : (axis == 1 ? e1 : e2);
   const int O = (e0 * e1 * e2) / L;
+  const int eo = axis == 0 ? e1 : e0;  // the first "other" extent: o = c1 + eo c2
   const int m0 = blockIdx.y * TXM, o0 = blockIdx.x * TXO;
   double d0 = 0.0, d1 = 0.0;
   if (m0 < L && o0 < O) {
     const TI *in = (const TI *)p.in[comp].p;
     const TA *A = (const TA *)p.A[comp];
-    const int tm = (tid / 16) * 2, to = (tid % 16) * 2;
-    TA acc[2][2] = {{0, 0}, {0, 0}};
-    for (int n0 = 0; n0 < L; n0 += TXK) {
-      for (int t = tid; t < TXK * TXM; t += 256) {
-        const int kk = t / TXM, mm = t % TXM;
-        const int n = n0 + kk, m = m0 + mm;
-        As[kk][mm] = (n < L && m < L) ? A[n * L + m] : TA(0);
-      }
-      for (int t = tid; t < TXK * TXO; t += 256) {
-        int kk, oo;
-        if (axis == 0) {
-          kk = t % TXK;
-          oo = t / TXK;
-        } else {
-          kk = t / TXO;
-          oo = t % TXO;
-        }
-        const int n = n0 + kk, o = o0 + oo;
-        TA v = TA(0);
-        if (n < L && o < O) {
-          int i0, i1, i2;
-          tx_coords(axis, e0, e1, n, o, i0, i1, i2);
-          v = (TA)in[voff(p.in[comp], i0, i1, i2)];
-        }
-        Is[kk][oo] = v;
+    const View vin = p.in[comp];
+    // where this thread's input elements start and how they step along n
+    int ibase, istep, ikk, ioo;
+    if (axis == 0) {
+      // column oo of the tile starts at colb[oo]; the thread loads n = kk, oo = tid/TXK + r 256/TXK
+      if (tid < TXO) {
+        const int o = o0 + tid;
+        colb[tid] = o < O ? voff(vin, 0, o % e1, o / e1) : -1;
       }
       __syncthreads();
+      ikk = tid % TXK;
+      ioo = tid / TXK;
+      ibase = 0;
+      istep = 1;
+    } else {
+      // one column per thread (oo = tid % 32), n = kk = tid/32 + 8 r
+      ioo = tid % TXO;
+      ikk = tid / TXO;
+      const int o = o0 + ioo;
+      const int c1 = o % e0, c2 = o / e0;
+      ibase = o < O ? (axis == 1 ? voff(vin, c1, 0, c2) : voff(vin, c1, c2, 0)) : -1;
+      istep = axis == 1 ? vin.s1 : vin.s2;
+    }
+    const int amm = tid % TXM, akk = tid / TXM;
+    TA ra[NL];
+    TI ri[NL];  // converted when stored to LDS, so no load waits inside the fetch
+    auto fetch = [&](int n0) {
+#pragma unroll
+      for (int r = 0; r < NL; ++r) {
+        const int n = n0 + akk + r * (256 / TXM);
+        ra[r] = A[min(n, L - 1) * L + min(m0 + amm, L - 1)];  // clamped; padded at the LDS store
+      }
+#pragma unroll
+      for (int r = 0; r < NL; ++r) {
+        int n, base;
+        if (axis == 0) {
+          n = n0 + ikk;
+          base = colb[ioo + r * (256 / TXK)];
+        } else {
+          n = n0 + ikk + r * (256 / TXO);
+          base = ibase;
+        }
+        ri[r] = in[(base >= 0 ? base : 0) + min(n, L - 1) * istep];
+      }
+    };
+    // thread tile: m fastest for the x pass (its output is contiguous along m), o fastest else
+    const int tm = axis == 0 ? (tid % 16) * 2 : (tid / 16) * 2;
+    const int to = axis == 0 ? (tid / 16) * 2 : (tid % 16) * 2;
+    TA acc[2][2] = {{0, 0}, {0, 0}};
+    fetch(0);
+    for (int n0 = 0; n0 < L; n0 += TXK) {
+#pragma unroll
+      for (int r = 0; r < NL; ++r) {
+        const int na = n0 + akk + r * (256 / TXM);
+        As[akk + r * (256 / TXM)][amm] = (na < L && m0 + amm < L) ? ra[r] : TA(0);
+        if (axis == 0) {
+          const bool ok = n0 + ikk < L && colb[ioo + r * (256 / TXK)] >= 0;
+          Is[ikk][ioo + r * (256 / TXK)] = ok ? (TA)ri[r] : TA(0);
+        } else {
+          const bool ok = n0 + ikk + r * (256 / TXO) < L && ibase >= 0;
+          Is[ikk + r * (256 / TXO)][ioo] = ok ? (TA)ri[r] : TA(0);
+        }
+      }
+      __syncthreads();
+      if (n0 + TXK < L) fetch(n0 + TXK);  // the next chunk's loads overlap this chunk's products
       const int kmax = (L - n0) < TXK ? (L - n0) : TXK;
 #pragma unroll 4
       for (int kk = 0; kk < kmax; ++kk) {
@@ -559,34 +649,45 @@ __global__ void __launch_bounds__(256) k_tx(TxPass p, const int *gate) {
       }
       __syncthreads();
     }
-    for (int dm = 0; dm < 2; ++dm)
-      for (int dout = 0; dout < 2; ++dout) {
-        const int m = m0 + tm + dm, o = o0 + to + dout;
-        if (m >= L || o >= O) continue;
-        int i0, i1, i2;
-        tx_coords(axis, e0, e1, m, o, i0, i1, i2);
-        const int q = voff(p.out[comp], i0, i1, i2);
+    // epilogue: the coordinates of the four outputs, every load, then the stores
+    int qo[2][2];
+    bool ok[2][2];
+    double tv[2][2] = {}, rv[2][2] = {}, zv[2][2] = {};
+    for (int dout = 0; dout < 2; ++dout) {
+      const int o = o0 + to + dout;
+      const int c1 = o % eo, c2 = o / eo;
+      for (int dm = 0; dm < 2; ++dm) {
+        const int m = m0 + tm + dm;
+        ok[dm][dout] = m < L && o < O;
+        const int i0 = axis == 0 ? m : c1, i1 = axis == 0 ? c1 : (axis == 1 ? m : c2), i2 = axis == 2 ? m : c2;
+        qo[dm][dout] = voff(p.out[comp], i0, i1, i2);
+        if (!ok[dm][dout]) continue;
+        if (p.addth) tv[dm][dout] = ((const double *)p.th[comp].p)[voff(p.th[comp], i0, i1, i2)];
+        if (p.dot) rv[dm][dout] = ((const double *)p.r[comp].p)[voff(p.r[comp], i0, i1, i2)];
+        if (p.dot == 2) zv[dm][dout] = ((const double *)p.zold[comp].p)[voff(p.zold[comp], i0, i1, i2)];
+      }
+    }
+    for (int dout = 0; dout < 2; ++dout)
+      for (int dm = 0; dm < 2; ++dm) {
+        if (!ok[dm][dout]) continue;
         if (p.addth) {
-          const double v = ((const double *)p.th[comp].p)[voff(p.th[comp], i0, i1, i2)] + (double)acc[dm][dout];
-          ((double *)p.out[comp].p)[q] = v;
+          ((double *)p.out[comp].p)[qo[dm][dout]] = tv[dm][dout] + (double)acc[dm][dout];
         } else {
           const TO v = (TO)acc[dm][dout];
-          ((TO *)p.out[comp].p)[q] = v;
+          ((TO *)p.out[comp].p)[qo[dm][dout]] = v;
           if (p.dot) {
-            const double rv = ((const double *)p.r[comp].p)[voff(p.r[comp], i0, i1, i2)];
-            d0 += rv * (double)v;
-            if (p.dot == 2) d1 += rv * ((const double *)p.zold[comp].p)[voff(p.zold[comp], i0, i1, i2)];
+            d0 += rv[dm][dout] * (double)v;
+            if (p.dot == 2) d1 += rv[dm][dout] * zv[dm][dout];
           }
         }
       }
   }
   if (p.dot) {
-    const double r0 = block_sum_t(d0, sh, tid);
-    const double r1 = block_sum_t(d1, sh, tid);
+    block_sum2_t(d0, d1, sh, tid);
     if (tid == 0) {
       double *pr = region_w(p.d, PART_RZ);
-      pr[2 * bid] = r0;
-      pr[2 * bid + 1] = r1;
+      pr[2 * bid] = d0;
+      pr[2 * bid + 1] = d1;
     }
   }
 }
@@ -644,32 +745,27 @@ __global__ void __launch_bounds__(256) k_tx2_modes(Tx2Modes m, const int *gate) 
   const T *AV = (const T *)m.Az[0], *AU = (const T *)m.Az[1];
   const T k1 = ((const T *)m.sx)[mx], k2 = ((const T *)m.sy)[my], k3 = ((const T *)m.sz)[mz];
   const bool hx = mx > 0, hy = my > 0, hz = mz > 0;
-  // X: extents (nx-1, ny, nz); Y: (nx, ny-1, nz); Z: (nx, ny, nz-1)
-  T x1 = 0, y1 = 0, z1 = 0;
-  if (hx) {
-    const T *col = in + m.off[0] + (mx - 1) + (nx - 1) * my;
-    const int st = (nx - 1) * ny;
-    T acc = 0;
-#pragma unroll 8
-    for (int n = 0; n < nz; ++n) acc += col[st * n] * AV[n * nz + mz];
-    x1 = acc;
+  // X: extents (nx-1, ny, nz); Y: (nx, ny-1, nz); Z: (nx, ny, nz-1).  The columns of an absent
+  // component are read at a clamped index and dropped, so every load is unconditional.
+  const T *colx = in + m.off[0] + (hx ? mx - 1 : 0) + (nx - 1) * my;
+  const T *coly = in + m.off[1] + mx + nx * (hy ? my - 1 : 0);
+  const T *colz = in + m.off[2] + mx + nx * my;
+  const int stx = (nx - 1) * ny, sty = nx * (ny - 1), stz = nx * ny;
+  const int mzu = hz ? mz - 1 : 0;
+  T ax = 0, ay = 0, az = 0;
+#pragma unroll 16
+  for (int n = 0; n < nz - 1; ++n) {
+    const T v = AV[n * nz + mz];
+    ax += colx[stx * n] * v;
+    ay += coly[sty * n] * v;
+    az += colz[stz * n] * AU[n * (nz - 1) + mzu];
   }
-  if (hy) {
-    const T *col = in + m.off[1] + mx + nx * (my - 1);
-    const int st = nx * (ny - 1);
-    T acc = 0;
-#pragma unroll 8
-    for (int n = 0; n < nz; ++n) acc += col[st * n] * AV[n * nz + mz];
-    y1 = acc;
+  {
+    const T v = AV[(nz - 1) * nz + mz];
+    ax += colx[stx * (nz - 1)] * v;
+    ay += coly[sty * (nz - 1)] * v;
   }
-  if (hz) {
-    const T *col = in + m.off[2] + mx + nx * my;
-    const int st = nx * ny;
-    T acc = 0;
-#pragma unroll 8
-    for (int n = 0; n < nz - 1; ++n) acc += col[st * n] * AU[n * (nz - 1) + (mz - 1)];
-    z1 = acc;
-  }
+  T x1 = hx ? ax : T(0), y1 = hy ? ay : T(0), z1 = hz ? az : T(0);
   const T k2s = k1 * k1 + k2 * k2 + k3 * k3;
   if (k2s != 0) project<T>(k1, k2, k3, (T)m.a, x1, y1, z1, hx, hy, hz);
   if (hx) out[m.off[0] + (mx - 1) + (nx - 1) * (my + ny * mz)] = x1;
@@ -684,6 +780,511 @@ hipError_t launch_tx2_modes(const Tx2Modes &m, const int *gate, hipStream_t s) {
 }
 template hipError_t launch_tx2_modes<double>(const Tx2Modes &, const int *, hipStream_t);
 template hipError_t launch_tx2_modes<float>(const Tx2Modes &, const int *, hipStream_t);
+
+// Whether a transform launch runs: the convergence check of a first preconditioner pass
+// (implicit.jl:305, 319, 426; it writes the flag of the iterations that follow) or a gate flag.
+// Uniform over the block.
+__device__ __forceinline__ bool pass_active(const Conv &cv, const int *gate, const PcgDev &d, double *sh, int tid,
+                                            int bid) {
+  if (cv.mode != CONV_NONE) {
+    if (!*cv.in) {
+      if (bid == 0 && tid == 0) *cv.out = 0;
+      return false;
+    }
+    const double rr = sum_parts(region(d, PART_RR), cv.nparts, 0, sh, tid);
+    const double tol = d.sc[SC_TOL], nb = d.sc[SC_NB];
+    int active;
+    if (cv.mode == CONV_RR0)
+      active = (nb > 0 && sqrt(rr) > tol * nb) ? 1 : 0;
+    else
+      active = (sqrt(rr) <= tol * nb || d.fl[FL_IT] >= d.fl[FL_MAXIT]) ? 0 : 1;
+    if (bid == 0 && tid == 0) *cv.out = active;
+    return active != 0;
+  }
+  return !(gate && !*gate);
+}
+
+// ---------------------------------------------------------------------------- x and y in one launch
+// Per block: rows m0..m0+MS of one plane.  Phase 1 stages IC rows of the input plane (all of y)
+// and the matching IC x MS block of X, each thread accumulating RM rows x (y/32) columns of
+// S = X' in; S goes to LDS (transposed); phase 2 stages IC rows of Y and accumulates Y' S the
+// same way.  The next chunk's loads are in flight while the current one is multiplied.
+template <class T> struct PlaneCfg;
+template <> struct PlaneCfg<float> { static constexpr int MS = 32, IC = 32, RM = 4; };
+template <> struct PlaneCfg<double> { static constexpr int MS = 16, IC = 16, RM = 2; };
+
+// LDS rows are one element longer than the tile (bank-conflict-free staging and S stores)
+template <class T>
+static size_t plane_lds(int l1) {
+  constexpr int MS = PlaneCfg<T>::MS, IC = PlaneCfg<T>::IC;
+  const int l1p = (l1 + 31) / 32 * 32;
+  return (size_t)(IC * MS + IC * (l1p + 1) + l1p * (MS + 1)) * sizeof(T);
+}
+
+// TJ = ceil(y extent / 32) column slots per thread, a compile-time count so that the LDS reads
+// of one step of the contraction are issued together
+template <class TI, class T, class TO, int TJ>
+__global__ void __launch_bounds__(256) k_plane(PlanePass p, const int *gate) {
+  constexpr int MS = PlaneCfg<T>::MS, IC = PlaneCfg<T>::IC, RM = PlaneCfg<T>::RM;
+  static_assert(MS == 8 * RM && 256 % IC == 0 && (IC * MS) % 256 == 0, "plane tile");
+  constexpr int NX = IC * MS / 256;     // staged X values per thread
+  constexpr int L1P = 32 * TJ;          // staged columns (zero beyond the y extent)
+  constexpr int L1S = L1P + 1;          // LDS row of the staged input and of Y
+  constexpr int MSS = MS + 1;           // LDS row of S
+  constexpr int NB = IC * L1P / 256;    // staged input values per thread (ceil)
+  extern __shared__ __align__(16) unsigned char plane_smem[];
+  __shared__ double sh[8];
+  const int tid = threadIdx.x;
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (!pass_active(p.conv, gate, p.d, sh, tid, bid)) return;
+  const int comp = blockIdx.z, plane = blockIdx.y, m0 = blockIdx.x * MS;
+  const int e0 = p.e[comp][0], e1 = p.e[comp][1], e2 = p.e[comp][2];
+  double d0 = 0.0, d1 = 0.0;
+  if (plane < e2 && m0 < e0) {
+    T *Xs = (T *)plane_smem;  // [IC][MS]
+    T *Bs = Xs + IC * MS;     // [IC][L1S]: input rows (phase 1), rows of Y (phase 2)
+    T *St = Bs + IC * L1S;    // [L1P][MSS]: S transposed
+    const TI *in = (const TI *)p.in[comp].p + (size_t)p.in[comp].s2 * plane;
+    const int s1 = p.in[comp].s1;
+    const T *X = (const T *)p.X[comp], *Y = (const T *)p.Y[comp];
+    const int rg = tid / 32, lane = tid % 32;
+    T rx[NX];
+    TI rb[NB];
+    // input element (ii, j): ii = tid % IC, j = tid / IC + r (256 / IC), so a row of x is read by
+    // consecutive threads
+    auto fetch1 = [&](int i0) {  // clamped addresses; the padding is applied at the LDS store
+#pragma unroll
+      for (int r = 0; r < NX; ++r) {
+        const int t = tid + 256 * r, ii = t / MS, mm = t % MS;
+        rx[r] = X[min(i0 + ii, e0 - 1) * e0 + min(m0 + mm, e0 - 1)];
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int ii = tid % IC, j = tid / IC + r * (256 / IC);
+        rb[r] = in[min(i0 + ii, e0 - 1) + s1 * min(j, e1 - 1)];
+      }
+    };
+    T acc[RM][TJ];
+#pragma unroll
+    for (int u = 0; u < RM; ++u)
+#pragma unroll
+      for (int t = 0; t < TJ; ++t) acc[u][t] = T(0);
+    fetch1(0);
+    for (int i0 = 0; i0 < e0; i0 += IC) {
+#pragma unroll
+      for (int r = 0; r < NX; ++r) {
+        const int t = tid + 256 * r, ii = t / MS, mm = t % MS;
+        Xs[t] = (i0 + ii < e0 && m0 + mm < e0) ? rx[r] : T(0);
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int ii = tid % IC, j = tid / IC + r * (256 / IC);
+        if (j < L1P) Bs[ii * L1S + j] = (i0 + ii < e0 && j < e1) ? (T)rb[r] : T(0);
+      }
+      __syncthreads();
+      if (i0 + IC < e0) fetch1(i0 + IC);
+      const int kmax = (e0 - i0) < IC ? (e0 - i0) : IC;
+#pragma unroll 2
+      for (int ii = 0; ii < kmax; ++ii) {
+        T xv[RM], bv[TJ];
+#pragma unroll
+        for (int u = 0; u < RM; ++u) xv[u] = Xs[ii * MS + rg * RM + u];
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) bv[t] = Bs[ii * L1S + lane + 32 * t];
+#pragma unroll
+        for (int t = 0; t < TJ; ++t)
+#pragma unroll
+          for (int u = 0; u < RM; ++u) acc[u][t] += xv[u] * bv[t];
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < TJ; ++t)
+#pragma unroll
+      for (int u = 0; u < RM; ++u) St[(lane + 32 * t) * MSS + rg * RM + u] = acc[u][t];
+    // phase 2: out(m, q) = Σ_j Y(j, q) S(m, j); row jj of Y is staged by thread q = tid
+    T ry[IC];
+    auto fetch2 = [&](int j0) {
+#pragma unroll
+      for (int r = 0; r < IC; ++r) ry[r] = Y[min(j0 + r, e1 - 1) * e1 + min(tid, e1 - 1)];
+    };
+#pragma unroll
+    for (int u = 0; u < RM; ++u)
+#pragma unroll
+      for (int t = 0; t < TJ; ++t) acc[u][t] = T(0);
+    fetch2(0);
+    for (int j0 = 0; j0 < e1; j0 += IC) {
+      if (tid < L1P)
+#pragma unroll
+        for (int r = 0; r < IC; ++r) Bs[r * L1S + tid] = (j0 + r < e1 && tid < e1) ? ry[r] : T(0);
+      __syncthreads();
+      if (j0 + IC < e1) fetch2(j0 + IC);
+      const int kmax = (e1 - j0) < IC ? (e1 - j0) : IC;
+#pragma unroll 2
+      for (int jj = 0; jj < kmax; ++jj) {
+        T sv[RM], bv[TJ];
+#pragma unroll
+        for (int u = 0; u < RM; ++u) sv[u] = St[(j0 + jj) * MSS + rg * RM + u];
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) bv[t] = Bs[jj * L1S + lane + 32 * t];
+#pragma unroll
+        for (int t = 0; t < TJ; ++t)
+#pragma unroll
+          for (int u = 0; u < RM; ++u) acc[u][t] += sv[u] * bv[t];
+      }
+      __syncthreads();
+    }
+    // epilogue: (m, q, plane) for the thread's outputs; every load, then the stores
+    const View vo = p.out[comp];
+#pragma unroll
+    for (int t = 0; t < TJ; ++t) {
+      const int q = lane + 32 * t;
+      double tv[RM], rv[RM], zv[RM];
+      bool ok[RM];
+#pragma unroll
+      for (int u = 0; u < RM; ++u) {
+        const int m = m0 + rg * RM + u;
+        ok[u] = m < e0 && q < e1;
+        tv[u] = rv[u] = zv[u] = 0.0;
+        if (!ok[u]) continue;
+        if (p.addth) tv[u] = ((const double *)p.th[comp].p)[voff(p.th[comp], m, q, plane)];
+        if (p.dot) rv[u] = ((const double *)p.r[comp].p)[voff(p.r[comp], m, q, plane)];
+        if (p.dot == 2) zv[u] = ((const double *)p.zold[comp].p)[voff(p.zold[comp], m, q, plane)];
+      }
+#pragma unroll
+      for (int u = 0; u < RM; ++u) {
+        if (!ok[u]) continue;
+        const int m = m0 + rg * RM + u;
+        const int qo = voff(vo, m, q, plane);
+        if (p.addth) {
+          ((double *)vo.p)[qo] = tv[u] + (double)acc[u][t];
+        } else {
+          const TO v = (TO)acc[u][t];
+          ((TO *)vo.p)[qo] = v;
+          if (p.dot) {
+            d0 += rv[u] * (double)v;
+            if (p.dot == 2) d1 += rv[u] * zv[u];
+          }
+        }
+      }
+    }
+  }
+  if (p.dot) {
+    block_sum2_t(d0, d1, sh, tid);
+    if (tid == 0) {
+      double *pr = region_w(p.d, PART_RZ);
+      pr[2 * bid] = d0;
+      pr[2 * bid + 1] = d1;
+    }
+  }
+}
+
+// The Float32 plane pass on the matrix cores: the same two products as k_plane, each as 16 x 16
+// tiles of v_mfma_f32_16x16x4f32 (A: lane l holds (m = l%16, k = l/16); B: (k = l/16, n = l%16);
+// D: (m = 4(l/16) + r, n = l%16)).  A block's 32 rows x 32 TJ columns are 4 TJ tiles, TJ per
+// wavefront.  Sums are grouped by the instruction (four products at a time, rounded once):
+// the Float32 preconditioner, whose result the FP64 outer PCG only needs as an approximation.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Rows of the staged chunks: 64 while the y extent is at most 128 (fewer dependent load rounds),
+// 32 above (the LDS and register budget of the wider tiles)
+template <int TJ> struct PlaneMfCfg { static constexpr int IC = TJ <= 4 ? 64 : 32; };
+
+template <class TI, class TO, int TJ>
+__global__ void __launch_bounds__(256) k_plane_mf(PlanePass p, const int *gate) {
+  constexpr int MS = 32, IC = PlaneMfCfg<TJ>::IC;
+  constexpr int NX = IC * MS / 256;     // staged X values per thread
+  constexpr int L1P = 32 * TJ;          // staged columns (zero beyond the y extent)
+  constexpr int L1S = L1P + 1;          // LDS row of the staged input and of Y
+  constexpr int XSS = MS + 1;           // LDS row of X
+  constexpr int MSS = MS + 1;           // LDS row of S (transposed)
+  constexpr int NB = IC * L1P / 256;    // staged input values per thread
+  extern __shared__ __align__(16) unsigned char plane_smem[];
+  __shared__ double sh[8];
+  const int tid = threadIdx.x;
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (!pass_active(p.conv, gate, p.d, sh, tid, bid)) return;
+  const int comp = blockIdx.z, plane = blockIdx.y, m0 = blockIdx.x * MS;
+  const int e0 = p.e[comp][0], e1 = p.e[comp][1], e2 = p.e[comp][2];
+  double d0 = 0.0, d1 = 0.0;
+  if (plane < e2 && m0 < e0) {
+    float *Xs = (float *)plane_smem;  // [IC][XSS]
+    float *Bs = Xs + IC * XSS;        // [IC][L1S]
+    float *St = Bs + IC * L1S;        // [L1P][MSS]
+    const TI *in = (const TI *)p.in[comp].p + (size_t)p.in[comp].s2 * plane;
+    const int s1 = p.in[comp].s1;
+    const float *X = (const float *)p.X[comp], *Y = (const float *)p.Y[comp];
+    const int w = tid / 64, l = tid % 64, lr = l % 16, lk = l / 16;
+    float rx[NX];
+    TI rb[NB];
+    float ry[IC];
+    // Every load reads a clamped, in-bounds address unconditionally; the padding is applied where
+    // the value is stored to LDS (a load under a divergent condition would wait for memory).
+    auto fetch1 = [&](int i0) {
+#pragma unroll
+      for (int r = 0; r < NX; ++r) {
+        const int t = tid + 256 * r, ii = t / MS, mm = t % MS;
+        rx[r] = X[min(i0 + ii, e0 - 1) * e0 + min(m0 + mm, e0 - 1)];
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int ii = tid % IC, j = tid / IC + r * (256 / IC);
+        rb[r] = in[min(i0 + ii, e0 - 1) + s1 * min(j, e1 - 1)];
+      }
+    };
+    // row j0 + r of Y, column q = tid (Y does not depend on the data: its first chunk is
+    // requested with the first input chunk)
+    auto fetch2 = [&](int j0) {
+#pragma unroll
+      for (int r = 0; r < IC; ++r) ry[r] = Y[min(j0 + r, e1 - 1) * e1 + min(tid, e1 - 1)];
+    };
+    v4f acc[TJ];
+#pragma unroll
+    for (int t = 0; t < TJ; ++t) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    fetch1(0);
+    fetch2(0);
+    for (int i0 = 0; i0 < e0; i0 += IC) {
+#pragma unroll
+      for (int r = 0; r < NX; ++r) {
+        const int t = tid + 256 * r, ii = t / MS, mm = t % MS;
+        Xs[ii * XSS + mm] = (i0 + ii < e0 && m0 + mm < e0) ? rx[r] : 0.0f;
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int ii = tid % IC, j = tid / IC + r * (256 / IC);
+        Bs[ii * L1S + j] = (i0 + ii < e0 && j < e1) ? (float)rb[r] : 0.0f;
+      }
+      __syncthreads();
+      if (i0 + IC < e0) fetch1(i0 + IC);
+#pragma unroll 2
+      for (int k0 = 0; k0 < IC; k0 += 4) {
+        const float a0 = Xs[(k0 + lk) * XSS + lr], a1 = Xs[(k0 + lk) * XSS + 16 + lr];
+#pragma unroll
+        for (int s = 0; s < TJ; ++s) {
+          const int t = w * TJ + s, mt = t & 1, jt = t >> 1;
+          const float b = Bs[(k0 + lk) * L1S + jt * 16 + lr];
+          acc[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(mt ? a1 : a0, b, acc[s], 0, 0, 0);
+        }
+      }
+      __syncthreads();
+    }
+    // S(m, j) to LDS, transposed: lane l holds m = mt 16 + 4 lk + r, j = jt 16 + lr
+#pragma unroll
+    for (int s = 0; s < TJ; ++s) {
+      const int t = w * TJ + s, mt = t & 1, jt = t >> 1;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) St[(jt * 16 + lr) * MSS + mt * 16 + 4 * lk + r] = acc[s][r];
+      acc[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    // phase 2: out(m, q) = Σ_j S(m, j) Y(j, q); row jj of Y is staged by thread q = tid
+    for (int j0 = 0; j0 < e1; j0 += IC) {
+      if (tid < L1P)
+#pragma unroll
+        for (int r = 0; r < IC; ++r) Bs[r * L1S + tid] = (j0 + r < e1 && tid < e1) ? ry[r] : 0.0f;
+      __syncthreads();
+      if (j0 + IC < e1) fetch2(j0 + IC);
+      // rows of S at or beyond the staged columns are zero
+#pragma unroll 2
+      for (int k0 = 0; k0 < IC; k0 += 4) {
+        const int jr = j0 + k0 + lk;
+        const float a0 = jr < L1P ? St[jr * MSS + lr] : 0.0f, a1 = jr < L1P ? St[jr * MSS + 16 + lr] : 0.0f;
+#pragma unroll
+        for (int s = 0; s < TJ; ++s) {
+          const int t = w * TJ + s, mt = t & 1, qt = t >> 1;
+          const float b = Bs[(k0 + lk) * L1S + qt * 16 + lr];
+          acc[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(mt ? a1 : a0, b, acc[s], 0, 0, 0);
+        }
+      }
+      __syncthreads();
+    }
+    // epilogue: every load, then the stores
+    const View vo = p.out[comp];
+#pragma unroll
+    for (int s = 0; s < TJ; ++s) {
+      const int t = w * TJ + s, mt = t & 1, qt = t >> 1;
+      const int q = qt * 16 + lr;
+      double tv[4], rv[4], zv[4];
+      bool ok[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + mt * 16 + 4 * lk + r;
+        ok[r] = m < e0 && q < e1;
+        tv[r] = rv[r] = zv[r] = 0.0;
+        if (!ok[r]) continue;
+        if (p.addth) tv[r] = ((const double *)p.th[comp].p)[voff(p.th[comp], m, q, plane)];
+        if (p.dot) rv[r] = ((const double *)p.r[comp].p)[voff(p.r[comp], m, q, plane)];
+        if (p.dot == 2) zv[r] = ((const double *)p.zold[comp].p)[voff(p.zold[comp], m, q, plane)];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (!ok[r]) continue;
+        const int m = m0 + mt * 16 + 4 * lk + r;
+        const int qo = voff(vo, m, q, plane);
+        if (p.addth) {
+          ((double *)vo.p)[qo] = tv[r] + (double)acc[s][r];
+        } else {
+          const TO v = (TO)acc[s][r];
+          ((TO *)vo.p)[qo] = v;
+          if (p.dot) {
+            d0 += rv[r] * (double)v;
+            if (p.dot == 2) d1 += rv[r] * zv[r];
+          }
+        }
+      }
+    }
+  }
+  if (p.dot) {
+    block_sum2_t(d0, d1, sh, tid);
+    if (tid == 0) {
+      double *pr = region_w(p.d, PART_RZ);
+      pr[2 * bid] = d0;
+      pr[2 * bid + 1] = d1;
+    }
+  }
+}
+
+static size_t plane_mf_lds(int l1) {
+  const int l1p = (l1 + 31) / 32 * 32;
+  const int ic = l1p <= 128 ? 64 : 32;  // PlaneMfCfg
+  return (size_t)(ic * 33 + ic * (l1p + 1) + l1p * 33) * sizeof(float);
+}
+
+template <class T>
+static dim3 plane_grid(const PlanePass &p) {
+  int gx = 1, gy = 1;
+  for (int c = 0; c < 3; ++c) {
+    gx = cdiv(p.e[c][0], PlaneCfg<T>::MS) > gx ? cdiv(p.e[c][0], PlaneCfg<T>::MS) : gx;
+    gy = p.e[c][2] > gy ? p.e[c][2] : gy;
+  }
+  return dim3(gx, gy, 3);
+}
+template <class T>
+int plane_parts(const PlanePass &p) {
+  const dim3 g = plane_grid<T>(p);
+  return (int)(g.x * g.y * g.z);
+}
+template <class T>
+bool plane_fits(const int e[3][3]) {
+  for (int c = 0; c < 3; ++c)
+    if (e[c][1] > PLANE_LMAX || e[c][0] < 1 || e[c][1] < 1 || e[c][2] < 1) return false;
+  int l1 = 0;
+  for (int c = 0; c < 3; ++c) l1 = e[c][1] > l1 ? e[c][1] : l1;
+  return (sizeof(T) == 4 ? plane_mf_lds(l1) : plane_lds<T>(l1)) <= 64 * 1024;
+}
+
+template <class TI, class T, class TO>
+hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
+  if (!plane_fits<T>(p.e) || plane_parts<T>(p) > MAX_PART) return hipErrorInvalidValue;
+  int l1 = 0;
+  for (int c = 0; c < 3; ++c) l1 = p.e[c][1] > l1 ? p.e[c][1] : l1;
+  const dim3 gr = plane_grid<T>(p);
+  const bool mf = sizeof(T) == 4;  // Float32 on the matrix cores
+  const size_t lds = mf ? plane_mf_lds(l1) : plane_lds<T>(l1);
+  switch ((l1 + 31) / 32) {
+#define TDGL_PLANE_CASE(tj)                                                                     \
+  case tj:                                                                                      \
+    if constexpr (sizeof(T) == 4)                                                               \
+      hipLaunchKernelGGL((k_plane_mf<TI, TO, tj>), gr, dim3(256), lds, s, p, gate);             \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_plane<TI, T, TO, tj>), gr, dim3(256), lds, s, p, gate);             \
+    break;
+    TDGL_PLANE_CASE(1)
+    TDGL_PLANE_CASE(2)
+    TDGL_PLANE_CASE(3)
+    TDGL_PLANE_CASE(4)
+    TDGL_PLANE_CASE(5)
+    TDGL_PLANE_CASE(6)
+    TDGL_PLANE_CASE(7)
+    TDGL_PLANE_CASE(8)
+#undef TDGL_PLANE_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+template hipError_t launch_plane<double, float, float>(const PlanePass &, const int *, hipStream_t);
+template hipError_t launch_plane<float, float, double>(const PlanePass &, const int *, hipStream_t);
+template hipError_t launch_plane<double, double, double>(const PlanePass &, const int *, hipStream_t);
+template int plane_parts<float>(const PlanePass &);
+template int plane_parts<double>(const PlanePass &);
+template bool plane_fits<float>(const int e[3][3]);
+template bool plane_fits<double>(const int e[3][3]);
+
+// ---------------------------------------------------------------------------- z, modes, inverse z
+// Thread (column tc, mode mz) transforms the three z-columns that meet at mode (mx, my, mz)
+// (X at (mx-1, my), Y at (mx, my-1), Z at (mx, my)), projects them and leaves the projected modes
+// in LDS; then thread (tc, n) takes the inverse z transform of those columns at z index n.  Every
+// projected mode of a column comes from the same block, so no other launch is needed between.
+template <class T>
+__global__ void __launch_bounds__(256) k_zmodes(ZModes m, const int *gate) {
+  if (gate && !*gate) return;
+  __shared__ T Mx[256], My[256], Mz[256];
+  const int nx = m.nx, ny = m.ny, nz = m.nz;
+  const int C = 256 / nz;
+  const int t = threadIdx.x;
+  const int tc = t / nz, mz = t % nz;
+  const int col = blockIdx.x * C + tc;
+  const bool live = tc < C && col < nx * ny;
+  const int mx = live ? col % nx : 0, my = live ? col / nx : 0;
+  const bool hx = mx > 0, hy = my > 0, hz = mz > 0;
+  const T *in = (const T *)m.in;
+  if (live) {
+    const T *AV = (const T *)m.Az[0], *AU = (const T *)m.Az[1];
+    const T k1 = ((const T *)m.sx)[mx], k2 = ((const T *)m.sy)[my], k3 = ((const T *)m.sz)[mz];
+    const T *colx = in + m.off[0] + (hx ? mx - 1 : 0) + (nx - 1) * my;
+    const T *coly = in + m.off[1] + mx + nx * (hy ? my - 1 : 0);
+    const T *colz = in + m.off[2] + mx + nx * my;
+    const int stx = (nx - 1) * ny, sty = nx * (ny - 1), stz = nx * ny;
+    const int mzu = hz ? mz - 1 : 0;
+    T ax = 0, ay = 0, az = 0;
+#pragma unroll 16
+    for (int n = 0; n < nz - 1; ++n) {
+      const T v = AV[n * nz + mz];
+      ax += colx[stx * n] * v;
+      ay += coly[sty * n] * v;
+      az += colz[stz * n] * AU[n * (nz - 1) + mzu];
+    }
+    {
+      const T v = AV[(nz - 1) * nz + mz];
+      ax += colx[stx * (nz - 1)] * v;
+      ay += coly[sty * (nz - 1)] * v;
+    }
+    T x1 = hx ? ax : T(0), y1 = hy ? ay : T(0), z1 = hz ? az : T(0);
+    const T k2s = k1 * k1 + k2 * k2 + k3 * k3;
+    if (k2s != 0) project<T>(k1, k2, k3, (T)m.a, x1, y1, z1, hx, hy, hz);
+    Mx[t] = x1;
+    My[t] = y1;
+    Mz[t] = z1;
+  }
+  __syncthreads();
+  if (!live) return;
+  const int n = mz;  // the z index this thread writes
+  const T *BV = (const T *)m.Ai[0], *BU = (const T *)m.Ai[1];
+  const int b0 = tc * nz;
+  T ox = 0, oy = 0, oz = 0;
+#pragma unroll 8
+  for (int q = 0; q < nz; ++q) {
+    const T v = BV[q * nz + n];
+    ox += Mx[b0 + q] * v;
+    oy += My[b0 + q] * v;
+  }
+  const int nu = n < nz - 1 ? n : 0;
+#pragma unroll 8
+  for (int q = 1; q < nz; ++q) oz += Mz[b0 + q] * BU[(q - 1) * (nz - 1) + nu];
+  T *out = (T *)m.out;
+  if (hx) out[m.off[0] + (mx - 1) + (nx - 1) * (my + ny * n)] = ox;
+  if (hy) out[m.off[1] + mx + nx * ((my - 1) + (ny - 1) * n)] = oy;
+  if (n < nz - 1) out[m.off[2] + mx + nx * (my + ny * n)] = oz;
+}
+
+template <class T>
+hipError_t launch_zmodes(const ZModes &m, const int *gate, hipStream_t s) {
+  if (m.nz < 2 || m.nz > 256 || m.nx < 1 || m.ny < 1) return hipErrorInvalidValue;
+  const int C = 256 / m.nz;
+  hipLaunchKernelGGL((k_zmodes<T>), dim3(cdiv(m.nx * m.ny, C)), dim3(256), 0, s, m, gate);
+  return hipGetLastError();
+}
+template hipError_t launch_zmodes<float>(const ZModes &, const int *, hipStream_t);
+template hipError_t launch_zmodes<double>(const ZModes &, const int *, hipStream_t);
 
 // ============================================================================ PCG
 // Kernels over the box i, j, k in 2..N (the union of the three ranges) with 1D blocks of 64 x 4
@@ -706,74 +1307,95 @@ __device__ __forceinline__ BoxIdx box_idx(const Geo &g) {
 }
 int pcg_box_blocks(const Geo &g) { return cdiv(g.Nx - 1, 64) * cdiv(g.Ny - 1, 4) * (g.Nz - 1); }
 
-// h of the direction on the ranges of _apply_A! (implicit.jl:216-224), zero elsewhere; P is the
-// direction of this kernel: z + β Pold (or a vector itself when Pold is null)
-struct PDir {
-  const double *z[3], *Po[3];
-  double beta;
-  int first;
-  __device__ __forceinline__ double at(int comp, int c) const {
-    if (!Po[0]) return z[comp][c];
-    return first ? z[comp][c] : z[comp][c] + beta * Po[comp][c];
-  }
+// Q = (D + aK) P of the three components at box node (i, j, k) (_apply_A!, implicit.jl:216-234):
+// h = curl P on the plaquettes of its ranges (zero elsewhere), then D p + s (curl h).  The 24
+// values of P the stencil reads are gathered unconditionally (a box node's neighbours lie inside
+// the node-shaped arrays, and P is zero outside its ranges), so they are in flight together; the
+// ranges then select by value.
+enum {
+  P0C, P0ZP, P0ZM, P0XMZP, P0XM, P0YP, P0YM, P0XMYP,
+  P1C, P1ZP, P1ZM, P1YMZP, P1YM, P1XP, P1YMXP, P1XM,
+  P2C, P2YP, P2ZMYP, P2ZM, P2YM, P2XP, P2ZMXP, P2XM, NSTEN
 };
-__device__ __forceinline__ double hx_of(const Geo &g, const PDir &P, int i, int j, int k, int c, double dydz) {
-  if (!(i >= 2 && i <= g.Nx && j >= 2 && j <= g.Ny - 1 && k >= 2 && k <= g.Nz - 1)) return 0.0;
-  const int sy = g.px, sz = g.px * g.py;
-  return ((P.at(2, c + sy) - P.at(2, c)) - P.at(1, c + sz) + P.at(1, c)) / dydz;
+__device__ __forceinline__ void gather24(const double *P0, const double *P1, const double *P2, int c, int sy, int sz,
+                                         double v[NSTEN]) {
+  v[P0C] = P0[c], v[P0ZP] = P0[c + sz], v[P0ZM] = P0[c - sz], v[P0XMZP] = P0[c - 1 + sz], v[P0XM] = P0[c - 1];
+  v[P0YP] = P0[c + sy], v[P0YM] = P0[c - sy], v[P0XMYP] = P0[c - 1 + sy];
+  v[P1C] = P1[c], v[P1ZP] = P1[c + sz], v[P1ZM] = P1[c - sz], v[P1YMZP] = P1[c - sy + sz], v[P1YM] = P1[c - sy];
+  v[P1XP] = P1[c + 1], v[P1YMXP] = P1[c - sy + 1], v[P1XM] = P1[c - 1];
+  v[P2C] = P2[c], v[P2YP] = P2[c + sy], v[P2ZMYP] = P2[c - sz + sy], v[P2ZM] = P2[c - sz], v[P2YM] = P2[c - sy];
+  v[P2XP] = P2[c + 1], v[P2ZMXP] = P2[c - sz + 1], v[P2XM] = P2[c - 1];
 }
-__device__ __forceinline__ double hy_of(const Geo &g, const PDir &P, int i, int j, int k, int c, double dzdx) {
-  if (!(i >= 2 && i <= g.Nx - 1 && j >= 2 && j <= g.Ny && k >= 2 && k <= g.Nz - 1)) return 0.0;
-  const int sz = g.px * g.py;
-  return ((P.at(0, c + sz) - P.at(0, c)) - P.at(2, c + 1) + P.at(2, c)) / dzdx;
+__device__ __forceinline__ void apply3v(const PcgArgs &a, const double v[NSTEN], const double dv[3], int i, int j,
+                                        int k, double q[3]) {
+  const int Nx = a.g.Nx, Ny = a.g.Ny, Nz = a.g.Nz;
+  // the ranges of hx (2:Nx, 2:Ny-1, 2:Nz-1), hy (2:Nx-1, 2:Ny, 2:Nz-1), hz (2:Nx-1, 2:Ny-1, 2:Nz)
+  auto okx = [&](int ii, int jj, int kk) { return ii >= 2 && ii <= Nx && jj >= 2 && jj <= Ny - 1 && kk >= 2 && kk <= Nz - 1; };
+  auto oky = [&](int ii, int jj, int kk) { return ii >= 2 && ii <= Nx - 1 && jj >= 2 && jj <= Ny && kk >= 2 && kk <= Nz - 1; };
+  auto okz = [&](int ii, int jj, int kk) { return ii >= 2 && ii <= Nx - 1 && jj >= 2 && jj <= Ny - 1 && kk >= 2 && kk <= Nz; };
+  const double hx_c = okx(i, j, k) ? ((v[P2YP] - v[P2C]) - v[P1ZP] + v[P1C]) / a.dydz : 0.0;
+  const double hx_zm = okx(i, j, k - 1) ? ((v[P2ZMYP] - v[P2ZM]) - v[P1C] + v[P1ZM]) / a.dydz : 0.0;
+  const double hx_ym = okx(i, j - 1, k) ? ((v[P2C] - v[P2YM]) - v[P1YMZP] + v[P1YM]) / a.dydz : 0.0;
+  const double hy_c = oky(i, j, k) ? ((v[P0ZP] - v[P0C]) - v[P2XP] + v[P2C]) / a.dzdx : 0.0;
+  const double hy_zm = oky(i, j, k - 1) ? ((v[P0C] - v[P0ZM]) - v[P2ZMXP] + v[P2ZM]) / a.dzdx : 0.0;
+  const double hy_xm = oky(i - 1, j, k) ? ((v[P0XMZP] - v[P0XM]) - v[P2C] + v[P2XM]) / a.dzdx : 0.0;
+  const double hz_c = okz(i, j, k) ? ((v[P1XP] - v[P1C]) - v[P0YP] + v[P0C]) / a.dxdy : 0.0;
+  const double hz_ym = okz(i, j - 1, k) ? ((v[P1YMXP] - v[P1YM]) - v[P0C] + v[P0YM]) / a.dxdy : 0.0;
+  const double hz_xm = okz(i - 1, j, k) ? ((v[P1C] - v[P1XM]) - v[P0XMYP] + v[P0XM]) / a.dxdy : 0.0;
+  q[0] = dv[0] * v[P0C] + a.sxy * (hz_c - hz_ym) - a.sxz * (hy_c - hy_zm);
+  q[1] = dv[1] * v[P1C] + a.syz * (hx_c - hx_zm) - a.syx * (hz_c - hz_xm);
+  q[2] = dv[2] * v[P2C] + a.szx * (hy_c - hy_xm) - a.szy * (hx_c - hx_ym);
 }
-__device__ __forceinline__ double hz_of(const Geo &g, const PDir &P, int i, int j, int k, int c, double dxdy) {
-  if (!(i >= 2 && i <= g.Nx - 1 && j >= 2 && j <= g.Ny - 1 && k >= 2 && k <= g.Nz)) return 0.0;
-  const int sy = g.px;
-  return ((P.at(1, c + 1) - P.at(1, c)) - P.at(0, c + sy) + P.at(0, c)) / dxdy;
+__device__ __forceinline__ void apply3(const PcgArgs &a, const double *P0, const double *P1, const double *P2, int i,
+                                       int j, int k, int c, double q[3], double pc[3]) {
+  double v[NSTEN];
+  gather24(P0, P1, P2, c, a.g.px, a.g.px * a.g.py, v);
+  const double dv[3] = {a.D[0][c], a.D[1][c], a.D[2][c]};
+  apply3v(a, v, dv, i, j, k, q);
+  pc[0] = v[P0C];
+  pc[1] = v[P1C];
+  pc[2] = v[P2C];
 }
 
-// q = (D + aK) p of one component at (i, j, k) (_apply_A!, implicit.jl:226-234)
-__device__ __forceinline__ double apply_one(const PcgArgs &a, const PDir &P, int comp, int i, int j, int k, int c,
-                                            double p) {
-  const Geo &g = a.g;
-  const int sy = g.px, sz = g.px * g.py;
-  if (comp == 0)
-    return a.D[0][c] * p + a.sxy * (hz_of(g, P, i, j, k, c, a.dxdy) - hz_of(g, P, i, j - 1, k, c - sy, a.dxdy)) -
-           a.sxz * (hy_of(g, P, i, j, k, c, a.dzdx) - hy_of(g, P, i, j, k - 1, c - sz, a.dzdx));
-  if (comp == 1)
-    return a.D[1][c] * p + a.syz * (hx_of(g, P, i, j, k, c, a.dydz) - hx_of(g, P, i, j, k - 1, c - sz, a.dydz)) -
-           a.syx * (hz_of(g, P, i, j, k, c, a.dxdy) - hz_of(g, P, i - 1, j, k, c - 1, a.dxdy));
-  return a.D[2][c] * p + a.szx * (hy_of(g, P, i, j, k, c, a.dzdx) - hy_of(g, P, i - 1, j, k, c - 1, a.dzdx)) -
-         a.szy * (hx_of(g, P, i, j, k, c, a.dydz) - hx_of(g, P, i, j - 1, k, c - sy, a.dydz));
-}
+// a box thread inside the node array (the box's blocks overhang it along x and y)
+__device__ __forceinline__ bool box_live(const Geo &g, const BoxIdx &b) { return b.i <= g.Nx && b.j <= g.Ny; }
+// the index a thread loads from: its node, or the box's first node for an overhanging thread, so
+// every load is unconditional (a load under a divergent condition waits for memory)
+__device__ __forceinline__ int box_load_c(const Geo &g, const BoxIdx &b) { return box_live(g, b) ? b.c : g.at(2, 2, 2); }
 
 // _pcg_zero_update! (implicit.jl:251-271) when the source is zero, else the warm start of
 // _implicit_curl_fast! (:391-402); nb from the field kernel's partials
 __global__ void __launch_bounds__(RED_T) k_pcg_start(PcgArgs a) {
-  __shared__ double sh[RED_T];
+  __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const double nb = sqrt(sum_parts(region(a.d, PART_NB), a.nparts_in, 0, sh, tid));
   const BoxIdx b = box_idx(a.g);
+  const bool live = box_live(a.g, b);
+  const int cl = box_load_c(a.g, b);
+  double xv[3], pv[3];
+  for (int comp = 0; comp < 3; ++comp) {
+    xv[comp] = a.x.c[comp][cl];
+    pv[comp] = a.xprev.c[comp][cl];
+  }
+  const double nb = sqrt(sum_parts(region(a.d, PART_NB), a.nparts_in, 0, sh, tid));
   if (b.bid == 0 && tid == 0) {
     a.d.sc[SC_NB] = nb;
     a.d.fl[FL_NZ] = nb > 0 ? 1 : 0;
     a.d.fl[FL_IT] = 0;
   }
+  if (!live) return;
   for (int comp = 0; comp < 3; ++comp) {
     if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
     double *x = a.x.c[comp], *xp = a.xprev.c[comp];
     const int c = b.c;
     if (nb == 0) {
-      if (a.warm && a.extrap) xp[c] = x[c];
+      if (a.warm && a.extrap) xp[c] = xv[comp];
       x[c] = 0.0;
     } else if (a.warm && a.extrap) {
       if (a.have2) {
-        const double v = x[c];
-        x[c] = 2 * v - xp[c];
-        xp[c] = v;
+        x[c] = 2 * xv[comp] - pv[comp];
+        xp[c] = xv[comp];
       } else {
-        xp[c] = x[c];
+        xp[c] = xv[comp];
       }
     }
   }
@@ -782,19 +1404,21 @@ __global__ void __launch_bounds__(RED_T) k_pcg_start(PcgArgs a) {
 // the start of the iterations: Q = A x, r = b − Q and the partials of r·r (implicit.jl:407-410)
 __global__ void __launch_bounds__(RED_T) k_pcg_resid(PcgArgs a) {
   if (!a.d.fl[FL_NZ]) return;
-  __shared__ double sh[RED_T];
+  __shared__ double sh[8];
   const int tid = threadIdx.x;
   const BoxIdx b = box_idx(a.g);
-  PDir P{};
-  for (int comp = 0; comp < 3; ++comp) P.z[comp] = a.x.c[comp];
   double s = 0.0;
-  for (int comp = 0; comp < 3; ++comp) {
-    if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
-    const double Aq = apply_one(a, P, comp, b.i, b.j, b.k, b.c, a.x.c[comp][b.c]);
-    a.Q.c[comp][b.c] = Aq;
-    const double rv = a.b.c[comp][b.c] - Aq;
-    a.r.c[comp][b.c] = rv;
-    s += rv * rv;
+  if (box_live(a.g, b)) {
+    double q[3], pc[3], bv[3];
+    for (int comp = 0; comp < 3; ++comp) bv[comp] = a.b.c[comp][b.c];
+    apply3(a, a.x.c[0], a.x.c[1], a.x.c[2], b.i, b.j, b.k, b.c, q, pc);
+    for (int comp = 0; comp < 3; ++comp) {
+      if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+      a.Q.c[comp][b.c] = q[comp];
+      const double rv = bv[comp] - q[comp];
+      a.r.c[comp][b.c] = rv;
+      s += rv * rv;
+    }
   }
   const double r0 = block_sum_t(s, sh, tid);
   if (tid == 0) {
@@ -808,41 +1432,95 @@ __global__ void __launch_bounds__(RED_T) k_pcg_resid(PcgArgs a) {
 // (P = z on the first) and zold = z (implicit.jl:412-413, 428-436)
 __global__ void __launch_bounds__(RED_T) k_pcg_p(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
-  __shared__ double sh[RED_T];
+  __shared__ double sh[8];
   const int tid = threadIdx.x;
   const BoxIdx b = box_idx(a.g);
-  const double rz2 = sum_parts(region(a.d, PART_RZ), a.nparts_in, 0, sh, tid);
+  const bool live = box_live(a.g, b);
+  const int cl = box_load_c(a.g, b);
+  double zv[3], pv[3] = {};
+  for (int comp = 0; comp < 3; ++comp) {
+    zv[comp] = a.z.c[comp][cl];
+    if (a.q > 0) pv[comp] = a.Pold.c[comp][cl];
+  }
+  double rz2, rzo;
+  sum_parts2(region(a.d, PART_RZ), a.nparts_in, rz2, rzo, sh, tid);
   double beta = 0.0;
   if (a.q > 0) {
-    const double rzo = a.flexible ? sum_parts(region(a.d, PART_RZ), a.nparts_in, 1, sh, tid) : 0.0;
     const double rz = a.d.sc[SC_RZ0 + ((a.q + 1) & 1)];
     beta = a.flexible ? (rz2 - rzo) / rz : rz2 / rz;
   }
   if (b.bid == 0 && tid == 0) a.d.sc[SC_RZ0 + (a.q & 1)] = rz2;
+  if (!live) return;
   for (int comp = 0; comp < 3; ++comp) {
     if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
     const int c = b.c;
-    const double zv = a.z.c[comp][c];
-    a.Pnew.c[comp][c] = a.q == 0 ? zv : zv + beta * a.Pold.c[comp][c];
-    if (a.flexible) a.zold.c[comp][c] = zv;
+    a.Pnew.c[comp][c] = a.q == 0 ? zv[comp] : zv[comp] + beta * pv[comp];
+    if (a.flexible) a.zold.c[comp][c] = zv[comp];
   }
 }
 
 // iteration q: Q = A P and the partials of P·Q (implicit.jl:418-420)
 __global__ void __launch_bounds__(RED_T) k_pcg_ap(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
-  __shared__ double sh[RED_T];
+  __shared__ double sh[8];
   const int tid = threadIdx.x;
   const BoxIdx b = box_idx(a.g);
-  PDir P{};
-  for (int comp = 0; comp < 3; ++comp) P.z[comp] = a.Pnew.c[comp];
   double s = 0.0;
-  for (int comp = 0; comp < 3; ++comp) {
-    if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
-    const double p = a.Pnew.c[comp][b.c];
-    const double Aq = apply_one(a, P, comp, b.i, b.j, b.k, b.c, p);
-    a.Q.c[comp][b.c] = Aq;
-    s += p * Aq;
+  if (box_live(a.g, b)) {
+    double q[3], pc[3];
+    apply3(a, a.Pnew.c[0], a.Pnew.c[1], a.Pnew.c[2], b.i, b.j, b.k, b.c, q, pc);
+    for (int comp = 0; comp < 3; ++comp) {
+      if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+      a.Q.c[comp][b.c] = q[comp];
+      s += pc[comp] * q[comp];
+    }
+  }
+  const double r0 = block_sum_t(s, sh, tid);
+  if (tid == 0) {
+    double *pr = region_w(a.d, PART_PQ);
+    pr[2 * b.bid] = r0;
+    pr[2 * b.bid + 1] = 0.0;
+  }
+}
+
+// iteration q in one launch: β from the partials of the preconditioner's last pass, the direction
+// P = z + βPold (P = z on the first) at the node and at every neighbour the operator reads, then
+// Q = A P and the partials of P·Q; the node's P and zold = z are stored (implicit.jl:412-420,
+// 428-436).  z and Pold are zero outside the ranges, so P is too.
+__global__ void __launch_bounds__(RED_T) k_pcg_pap(PcgArgs a) {
+  if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
+  __shared__ double sh[8];
+  const int tid = threadIdx.x;
+  const BoxIdx b = box_idx(a.g);
+  const bool live = box_live(a.g, b);
+  const int sy = a.g.px, sz = a.g.px * a.g.py;
+  const int cl = box_load_c(a.g, b);
+  double vz[NSTEN], vp[NSTEN], dv[3];
+  gather24(a.z.c[0], a.z.c[1], a.z.c[2], cl, sy, sz, vz);
+  if (a.q > 0) gather24(a.Pold.c[0], a.Pold.c[1], a.Pold.c[2], cl, sy, sz, vp);
+  for (int comp = 0; comp < 3; ++comp) dv[comp] = a.D[comp][cl];
+  double rz2, rzo;
+  sum_parts2(region(a.d, PART_RZ), a.nparts_in, rz2, rzo, sh, tid);
+  double beta = 0.0;
+  if (a.q > 0) {
+    const double rz = a.d.sc[SC_RZ0 + ((a.q + 1) & 1)];
+    beta = a.flexible ? (rz2 - rzo) / rz : rz2 / rz;
+  }
+  if (b.bid == 0 && tid == 0) a.d.sc[SC_RZ0 + (a.q & 1)] = rz2;
+  double s = 0.0;
+  if (live) {
+    double v[NSTEN], q[3];
+    for (int n = 0; n < NSTEN; ++n) v[n] = a.q == 0 ? vz[n] : vz[n] + beta * vp[n];
+    apply3v(a, v, dv, b.i, b.j, b.k, q);
+    const int cc[3] = {P0C, P1C, P2C};
+    for (int comp = 0; comp < 3; ++comp) {
+      if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+      const double pv = v[cc[comp]];
+      a.Pnew.c[comp][b.c] = pv;
+      if (a.flexible) a.zold.c[comp][b.c] = vz[cc[comp]];
+      a.Q.c[comp][b.c] = q[comp];
+      s += pv * q[comp];
+    }
   }
   const double r0 = block_sum_t(s, sh, tid);
   if (tid == 0) {
@@ -855,21 +1533,31 @@ __global__ void __launch_bounds__(RED_T) k_pcg_ap(PcgArgs a) {
 // iteration q: α = rz/pAp, x += αP, r −= αQ and the partials of r·r; IT += 1
 __global__ void __launch_bounds__(RED_T) k_pcg_xr(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
-  __shared__ double sh[RED_T];
+  __shared__ double sh[8];
   const int tid = threadIdx.x;
   const BoxIdx b = box_idx(a.g);
+  const bool live = box_live(a.g, b);
+  const int cl = box_load_c(a.g, b);
+  double xv[3], pv[3], rv[3], qv[3];
+  for (int comp = 0; comp < 3; ++comp) {
+    xv[comp] = a.x.c[comp][cl];
+    pv[comp] = a.Pnew.c[comp][cl];
+    rv[comp] = a.r.c[comp][cl];
+    qv[comp] = a.Q.c[comp][cl];
+  }
   const double pAp = sum_parts(region(a.d, PART_PQ), a.nparts_in, 0, sh, tid);
   const double al = a.d.sc[SC_RZ0 + (a.q & 1)] / pAp;
   if (b.bid == 0 && tid == 0) a.d.fl[FL_IT] += 1;
   double s = 0.0;
-  for (int comp = 0; comp < 3; ++comp) {
-    if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
-    const int c = b.c;
-    a.x.c[comp][c] += al * a.Pnew.c[comp][c];
-    const double rv = a.r.c[comp][c] - al * a.Q.c[comp][c];
-    a.r.c[comp][c] = rv;
-    s += rv * rv;
-  }
+  if (live)
+    for (int comp = 0; comp < 3; ++comp) {
+      if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+      const int c = b.c;
+      a.x.c[comp][c] = xv[comp] + al * pv[comp];
+      const double r1 = rv[comp] - al * qv[comp];
+      a.r.c[comp][c] = r1;
+      s += r1 * r1;
+    }
   const double r0 = block_sum_t(s, sh, tid);
   if (tid == 0) {
     double *pr = region_w(a.d, PART_RR);
@@ -881,8 +1569,9 @@ __global__ void __launch_bounds__(RED_T) k_pcg_xr(PcgArgs a) {
 // φ = θ + x on the ranges (implicit.jl:440-444) and the cycle's counts of the solve
 __global__ void __launch_bounds__(RED_T) k_pcg_finish(PcgArgs a) {
   const BoxIdx b = box_idx(a.g);
-  for (int comp = 0; comp < 3; ++comp)
-    if (a.rg.in(comp, b.i, b.j, b.k)) a.ph[comp][b.c] = a.th[comp][b.c] + a.x.c[comp][b.c];
+  if (box_live(a.g, b))
+    for (int comp = 0; comp < 3; ++comp)
+      if (a.rg.in(comp, b.i, b.j, b.k)) a.ph[comp][b.c] = a.th[comp][b.c] + a.x.c[comp][b.c];
   if (b.bid == 0 && threadIdx.x == 0) {
     int *fl = a.d.fl;
     if (fl[FL_NZ] && fl[FL_ACT0 + (a.act_slot & 1)]) fl[FL_OVF] = 1;
@@ -901,6 +1590,7 @@ hipError_t launch_pcg_start(const PcgArgs &a, hipStream_t s) { return launch_box
 hipError_t launch_pcg_resid(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_resid, a, s); }
 hipError_t launch_pcg_p(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_p, a, s); }
 hipError_t launch_pcg_ap(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_ap, a, s); }
+hipError_t launch_pcg_pap(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_pap, a, s); }
 hipError_t launch_pcg_xr(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_xr, a, s); }
 hipError_t launch_pcg_finish(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_finish, a, s); }
 

@@ -101,6 +101,7 @@ struct tdgl_handle {
   float *sf[3] = {};
   double *T1 = nullptr, *T2 = nullptr;
   float *T32a = nullptr, *T32b = nullptr;
+  bool plane32 = false, plane64 = false;  // the fused transform launches fit (per precision)
   // right-hand side of the implicit solve and the PCG vectors: node-shaped, zero outside the ranges
   Vec3 B{}, x{}, r{}, z{}, P{}, P2{}, Qv{}, xprev{}, zold{};
   PcgDev pd{};
@@ -369,6 +370,39 @@ TxPass tx_pass(tdgl_handle *h, int axis, bool inv, bool f32) {
   return p;
 }
 
+// Forward (x then y) or inverse (x then y) transforms of the plane launch: U on a component's own
+// axis, V on the others
+PlanePass plane_pass(tdgl_handle *h, bool inv, bool f32) {
+  PlanePass p{};
+  p.d = h->pd;
+  for (int c = 0; c < 3; ++c) {
+    for (int q = 0; q < 3; ++q) p.e[c][q] = h->rg.e[c][q];
+    const int wx = c == 0 ? 1 : 0, wy = c == 1 ? 1 : 0;
+    p.X[c] = f32 ? (const void *)h->Af[0][wx][inv] : (const void *)h->Ad[0][wx][inv];
+    p.Y[c] = f32 ? (const void *)h->Af[1][wy][inv] : (const void *)h->Ad[1][wy][inv];
+  }
+  return p;
+}
+
+ZModes zmodes(tdgl_handle *h, const void *in, void *out, bool f32) {
+  ZModes m{};
+  m.in = in;
+  m.out = out;
+  for (int w = 0; w < 2; ++w) {
+    m.Az[w] = f32 ? (const void *)h->Af[2][w][0] : (const void *)h->Ad[2][w][0];
+    m.Ai[w] = f32 ? (const void *)h->Af[2][w][1] : (const void *)h->Ad[2][w][1];
+  }
+  m.sx = f32 ? (const void *)h->sf[0] : (const void *)h->sd[0];
+  m.sy = f32 ? (const void *)h->sf[1] : (const void *)h->sd[1];
+  m.sz = f32 ? (const void *)h->sf[2] : (const void *)h->sd[2];
+  for (int c = 0; c < 3; ++c) m.off[c] = h->rg.off[c];
+  m.nx = h->g.Nx - 1;
+  m.ny = h->g.Ny - 1;
+  m.nz = h->g.Nz - 1;
+  m.a = h->d.a;
+  return m;
+}
+
 // What the last inverse pass does with the solution
 struct SepOut {
   Vec3 *out;          // a node-shaped vector
@@ -383,6 +417,45 @@ int sep_solve(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int *gate, 
   void *Ta = f32 ? (void *)h->T32a : (void *)h->T1;
   void *Tb = f32 ? (void *)h->T32b : (void *)h->T2;
   const size_t es = f32 ? 4 : 8;
+  if (f32 ? h->plane32 : h->plane64) {
+    PlanePass f = plane_pass(h, false, f32);
+    for (int c = 0; c < 3; ++c) {
+      f.in[c] = node_view(h, in.c[c]);
+      f.out[c] = compact_view(h, Ta, c, es);
+    }
+    f.conv = conv;
+    if (f32)
+      PL(K_TX32, (launch_plane<double, float, float>(f, gate, h->s)));
+    else
+      PL(K_TX, (launch_plane<double, double, double>(f, gate, h->s)));
+    const int *g2 = conv.mode != CONV_NONE ? conv.out : gate;
+    ZModes m = zmodes(h, Ta, Tb, f32);
+    if (f32)
+      PL(K_MODES, launch_zmodes<float>(m, g2, h->s));
+    else
+      PL(K_MODES, launch_zmodes<double>(m, g2, h->s));
+    PlanePass b = plane_pass(h, true, f32);
+    for (int c = 0; c < 3; ++c) {
+      b.in[c] = compact_view(h, Tb, c, es);
+      if (o.addth) {
+        b.out[c] = node_view(h, h->ph[c]);
+        b.th[c] = node_view(h, h->th[c]);
+      } else {
+        b.out[c] = node_view(h, o.out->c[c]);
+      }
+      if (o.dot) {
+        b.r[c] = node_view(h, h->r.c[c]);
+        b.zold[c] = node_view(h, h->zold.c[c]);
+      }
+    }
+    b.addth = o.addth;
+    b.dot = o.dot;
+    if (f32)
+      PL(K_TX32, (launch_plane<float, float, double>(b, g2, h->s)));
+    else
+      PL(K_TX, (launch_plane<double, double, double>(b, g2, h->s)));
+    return 0;
+  }
   {
     TxPass p = tx_pass(h, 0, false, f32);
     for (int c = 0; c < 3; ++c) {
@@ -463,7 +536,11 @@ int sep_solve(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int *gate, 
   return 0;
 }
 
-int sep_last_parts(tdgl_handle *h) { return tx_parts(tx_pass(h, 2, true, false)); }
+// the partials of r·z written by the last launch of a preconditioner solve
+int sep_last_parts(tdgl_handle *h, bool f32) {
+  if (f32 ? h->plane32 : h->plane64) return f32 ? plane_parts<float>(plane_pass(h, true, true)) : plane_parts<double>(plane_pass(h, true, false));
+  return tx_parts(tx_pass(h, 2, true, false));
+}
 
 // ---------------------------------------------------------------------------- PCG
 PcgArgs pcg_args(tdgl_handle *h) {
@@ -505,9 +582,8 @@ int pcg_iteration(tdgl_handle *h, int q, bool flexible) {
   a.flexible = flexible;
   a.Pnew = (q % 2 == 0) ? h->P : h->P2;
   a.Pold = (q % 2 == 0) ? h->P2 : h->P;
-  a.nparts_in = sep_last_parts(h);
-  PL(K_VEC, launch_pcg_p(a, h->s));
-  PL(K_APPLYA, launch_pcg_ap(a, h->s));
+  a.nparts_in = sep_last_parts(h, flexible);
+  PL(K_APPLYA, launch_pcg_pap(a, h->s));
   a.nparts_in = pcg_box_blocks(h->g);
   PL(K_VEC, launch_pcg_xr(a, h->s));
   Conv cv{CONV_RR, h->pd.fl + FL_ACT0 + (q & 1), h->pd.fl + FL_ACT0 + ((q + 1) & 1), pcg_box_blocks(h->g)};
@@ -971,6 +1047,14 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     h->graphs = !(ge && ge[0] == '0');
     const char *gb = std::getenv("TDGL_HIP_BUDGET");
     if (gb && d.field_mode == TDGL_FIELD_PCG) h->budget = std::max(1, std::min(d.pcg_maxit, std::atoi(gb)));
+    // TDGL_HIP_PLANE=0: the separable solve as six single-axis passes even where the fused
+    // x-y and z-mode launches fit
+    const char *gp = std::getenv("TDGL_HIP_PLANE");
+    const bool zfits = d.Nz - 1 >= 2 && d.Nz - 1 <= 256;
+    if (!(gp && gp[0] == '0') && zfits) {
+      h->plane32 = plane_fits<float>(rg.e);
+      h->plane64 = plane_fits<double>(rg.e);
+    }
     return 0;
   };
   int rc = run();

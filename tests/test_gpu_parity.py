@@ -108,3 +108,33 @@ def test_hip_graph_cycles_match_oracle(name, budget, monkeypatch):
     assert stt["graph_cycles"] >= (2 if budget is None else 3)
     if budget is not None and st.field_mode == 2:
         assert stt["rollbacks"] >= 1
+
+
+SINGLE_AXIS_CASES = ["production", "pcg_fast_f32", "pcg_exact", "three_aniso"]
+
+
+@pytest.mark.parametrize("name", SINGLE_AXIS_CASES)
+def test_hip_single_axis_transforms_match_oracle(name, monkeypatch):
+    """The separable solve as six single-axis launches (the path for planes wider than the fused
+    x-y launch holds) against the oracle, as the default fused path is above."""
+    from oracle.oracle import Oracle
+    HipStepper = _hip()
+    monkeypatch.setenv("TDGL_HIP_PLANE", "0")
+    st, tol = make_case(name)
+    s0, pcg, H, HJ = seed_state(st)
+    o = Oracle(st)
+    o.set_state(s0)
+    if pcg is not None:
+        o.set_pcg(pcg)
+    o.step(12, H, HJ)
+    ref = o.get_state()
+    print(f"[gpu] single-axis case {name}", flush=True)
+    dev = HipStepper(st)
+    dev.upload(s0)
+    if pcg is not None:
+        dev.set_pcg(pcg)
+    dev.step(12, H, HJ)
+    errs = compare_states(dev.download(), ref)
+    core = {k: v for k, v in errs.items() if not k.startswith(("U", "V"))}
+    print(f"    worst {max(core, key=core.get)} {max(core.values()):.3e}", flush=True)
+    assert max(core.values()) <= tol, errs
