@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define TDGL_ABI_VERSION 1
+#define TDGL_ABI_VERSION 2
 #define TDGL_MAX_BANDS 4
 #define TDGL_MAX_STAGES 64
 
@@ -102,7 +102,33 @@ typedef struct {
   const double *bar_hz_y;            /* (2, Nz+1) */
   const double *bar_hz_x;            /* (Ny, Nz+1) */
   const double *bar_hy_x;            /* (Ny+1, Nz) */
+
+  /* Normal fluids with a finite interband exchange (qp_exchange, gtdgl/src/exchange.jl):
+   * TDGL_QP_NONE (one normal fluid), TDGL_QP_TWO (QPExchange: one potential δ = μ_π − μ_σ, solved
+   * from σn∇²δ − λ0 δ = C_π/s_π − C_σ/s_σ) or TDGL_QP_N (QPExchangeN: N potentials through the
+   * qp_nmodes = N modes of S⁻¹L, each σn∇²m − λ_k m = r_k).  The unknowns are the nodes
+   * i = 2..Nx, j = qp_j0+1..qp_j0+qp_nj, k = qp_k0+1..qp_k0+qp_nk (QPSolve's region:
+   * exchange.jl:40-47); their 1D operators' eigenbases (_lap1d, exchange.jl:148-166) are
+   * the caller's own: qp_Vx (Nx-1)², qp_Wx = inv(qp_Vx), qp_ex (Nx-1), and likewise for y (qp_nj)
+   * and z (qp_nk), column-major.  Uniform σn and λ only (no materials, no sculpted vacuum, no
+   * drag). */
+  int32_t qp_mode;
+  int32_t qp_nmodes;
+  int32_t qp_j0, qp_nj, qp_k0, qp_nk;
+  double qp_sigma;                        /* σn */
+  double qp_s[TDGL_MAX_BANDS];            /* shares s_µ of σn */
+  double qp_turn[TDGL_MAX_BANDS];         /* TWO: m_µ dt with m = (−s_π, s_σ) (qp_turn!, :400-412) */
+  double qp_lambda[TDGL_MAX_BANDS];       /* TWO: λ0; N: the eigenvalues λ_k of S⁻¹L */
+  double qp_P[TDGL_MAX_BANDS][TDGL_MAX_BANDS];    /* N: μ_µ = Σ_k P[µ][k] m_k */
+  double qp_Pinv[TDGL_MAX_BANDS][TDGL_MAX_BANDS]; /* N: m_k = Σ_µ Pinv[k][µ] R_µ */
+  double qp_gamma[TDGL_MAX_BANDS][TDGL_MAX_BANDS]; /* the Josephson matrix γ_µν (_conv, :316-337) */
+  const double *qp_Vx, *qp_Wx, *qp_ex;
+  const double *qp_Vy, *qp_Wy, *qp_ey;
+  const double *qp_Vz, *qp_Wz, *qp_ez;
 } tdgl_desc;
+
+/* Normal-fluid models (tdgl_desc.qp_mode) */
+enum { TDGL_QP_NONE = 0, TDGL_QP_TWO = 1, TDGL_QP_N = 2 };
 
 /* Host views of a State (gtdgl/src/state.jl:51-60), Julia shapes:
  *   G[µ], F[µ]      ComplexF64 (Nx+1, Ny+1, Nz+1)

@@ -10,7 +10,7 @@ import numpy as np
 from .scheme import FIELD_PCG, FIELD_EXPLICIT_SCALED, Stepper
 from .state import FIELDS, State
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAXB = 4
 MAXS = 64
 dp = ct.POINTER(ct.c_double)
@@ -43,7 +43,14 @@ class DescC(ct.Structure):
                 ("f", dp), ("taux", dp), ("tauy", dp), ("tauz", dp),
                 ("Dx", dp), ("Dy", dp), ("Dz", dp),
                 ("bar_hy_z", dp), ("bar_hy_y", dp), ("bar_hz_z", dp), ("bar_hz_y", dp),
-                ("bar_hz_x", dp), ("bar_hy_x", dp)]
+                ("bar_hz_x", dp), ("bar_hy_x", dp),
+                ("qp_mode", ct.c_int32), ("qp_nmodes", ct.c_int32),
+                ("qp_j0", ct.c_int32), ("qp_nj", ct.c_int32), ("qp_k0", ct.c_int32), ("qp_nk", ct.c_int32),
+                ("qp_sigma", ct.c_double), ("qp_s", ct.c_double * MAXB), ("qp_turn", ct.c_double * MAXB),
+                ("qp_lambda", ct.c_double * MAXB), ("qp_P", (ct.c_double * MAXB) * MAXB),
+                ("qp_Pinv", (ct.c_double * MAXB) * MAXB), ("qp_gamma", (ct.c_double * MAXB) * MAXB),
+                ("qp_Vx", dp), ("qp_Wx", dp), ("qp_ex", dp), ("qp_Vy", dp), ("qp_Wy", dp), ("qp_ey", dp),
+                ("qp_Vz", dp), ("qp_Wz", dp), ("qp_ez", dp)]
 
 
 class StateC(ct.Structure):
@@ -133,6 +140,22 @@ class PackedDesc:
         if st.bar is not None:
             for k in ("hy_z", "hy_y", "hz_z", "hz_y", "hz_x", "hy_x"):
                 setattr(d, "bar_" + k, hold(st.bar[k]))
+        q = st.qp
+        if q is not None:
+            d.qp_mode, d.qp_nmodes = q.mode, q.K
+            d.qp_j0, d.qp_nj, d.qp_k0, d.qp_nk = q.j0, q.nj, q.k0, q.nk
+            d.qp_sigma = st.model.sigma_n
+            for mu in range(N):
+                d.qp_s[mu] = q.s[mu]
+                d.qp_turn[mu] = q.turn[mu]
+                for nu in range(N):
+                    d.qp_P[mu][nu] = q.P[mu, nu]
+                    d.qp_Pinv[mu][nu] = q.Pinv[mu, nu]
+                    d.qp_gamma[mu][nu] = st.model.gamma[mu][nu]
+            for k in range(q.K):
+                d.qp_lambda[k] = q.lam[k]
+            for name, arr in zip(("Vx", "Wx", "ex", "Vy", "Wy", "ey", "Vz", "Wz", "ez"), q.bases):
+                setattr(d, "qp_" + name, hold(arr))
         self.c = d
         self._keep = keep
 

@@ -41,6 +41,13 @@ struct StageArgs {
   double lc;                   // fraction c of the step (ROTFRAC, ROTLINKS)
   int msub, nsub;              // substep m of n (SUBSTEP)
   int lframe;                  // SUBSTEP: the frame's factor on each power
+  // normal fluids: the band's last stage turns it by its potential (qp_turn!, exchange.jl:400-412,
+  // 533-543), e^{-i m_µ dt δ} (TWO) or e^{-i dt Σ_k P[µ][k] m_k} (N), m read from the solved
+  // modes on the region of the unknowns (nodes i = 2..Nx, j0+1.., k0+1..)
+  int qp;                      // TDGL_QP_*
+  const double *qpM[TDGL_MAX_BANDS];
+  int qpK, qn1, qnj, qj0, qk0;
+  double qpc[TDGL_MAX_BANDS];  // TWO: [0] = m_µ dt; N: P[µ][k]
 };
 hipError_t launch_stage(const Geo &g, const StageArgs &a, hipStream_t s);
 
@@ -169,10 +176,12 @@ hipError_t launch_tx2_modes(const Tx2Modes &m, const int *gate, hipStream_t s);
 // order, the x result rounded to T as a pass of its own would store it).  One block per
 // (component, plane, slice of PLANE_MS(T) rows of m).  The y extent is at most PLANE_LMAX.
 constexpr int PLANE_LMAX = 256;
+constexpr int PLANE_MAXC = 4;  // components (the three of a link vector, or the modes of a potential)
 struct PlanePass {
-  View in[3], out[3];
-  const void *X[3], *Y[3];
-  int e[3][3];
+  int ncomp;                   // 0 means 3
+  View in[PLANE_MAXC], out[PLANE_MAXC];
+  const void *X[PLANE_MAXC], *Y[PLANE_MAXC];
+  int e[PLANE_MAXC][3];
   int addth;
   View th[3];
   int dot;
@@ -185,7 +194,7 @@ hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s);
 template <class T>
 int plane_parts(const PlanePass &p);
 template <class T>
-bool plane_fits(const int e[3][3]);
+bool plane_fits(const int e[][3], int ncomp);
 
 // The z pass, the per-wavevector projector and the inverse z pass in one launch: a block holds
 // whole z-columns of modes (floor(256 / nz) columns of nz threads), so the inverse pass reads
@@ -202,6 +211,37 @@ struct ZModes {
 };
 template <class T>
 hipError_t launch_zmodes(const ZModes &m, const int *gate, hipStream_t s);
+
+// ---------------------------------------------------------------- normal fluids (exchange.jl)
+// The sources of the potentials on the region of the unknowns (compact (Nx-1) x nj x nk, zero
+// off the block): TWO, S = C_2/s_2 − C_1/s_1; N, m_k = Σ_µ Pinv[k][µ] (C_µ/s_µ − Σ_ν C_ν)
+// (qp_potential!, exchange.jl:344-360, 477-498)
+struct QpConv {
+  const cplx *G[TDGL_MAX_BANDS];
+  const cplx *V[3];
+  const double *tau[3];
+  double *S[TDGL_MAX_BANDS];
+  double wa[3][TDGL_MAX_BANDS];
+  double gam[TDGL_MAX_BANDS][TDGL_MAX_BANDS];
+  double s[TDGL_MAX_BANDS];
+  double Pinv[TDGL_MAX_BANDS][TDGL_MAX_BANDS];
+  double rx2, ry2, rz2;
+  int nb, mode, K;
+  int n1, nj, nk, j0, k0;
+};
+hipError_t launch_qp_conv(const Geo &g, const QpConv &a, hipStream_t s);
+
+// The z pass, the division by σn(e_x + e_y + e_z) − λ_k and the inverse z pass of each mode
+// (_qp_sep!, exchange.jl:295-302; the x and y passes are plane launches)
+struct QpZ {
+  const double *in;      // K compact region vectors, mode k at k * n
+  double *out;
+  const double *Wz, *Vz, *ex, *ey, *ez;
+  double sigma;
+  double lam[TDGL_MAX_BANDS];
+  int K, n1, nj, nk;
+};
+hipError_t launch_qp_zcol(const QpZ &a, hipStream_t s);
 
 // PCG kernels over the box i, j, k in 2..N (the union of the ranges), node-shaped vectors
 struct Vec3 {

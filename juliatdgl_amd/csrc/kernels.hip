@@ -125,6 +125,18 @@ __global__ void __launch_bounds__(256) k_stage(Geo g, StageArgs a) {
   }
   if (a.combine) p = cadd(rmul(a.cmu, p), rmul(a.cnu, a.A[c]));
   if (a.turn) p = cmul(p, dcis(om * a.dt));
+  if (a.qp) {
+    const int q = (i - 2) + a.qn1 * ((j - a.qj0 - 1) + a.qnj * (k - a.qk0 - 1));
+    double ang;
+    if (a.qp == TDGL_QP_TWO) {
+      ang = -a.qpc[0] * a.qpM[0][q];
+    } else {
+      double pot = 0.0;
+      for (int kk = 0; kk < a.qpK; ++kk) pot += a.qpc[kk] * a.qpM[kk][q];
+      ang = -a.dt * pot;
+    }
+    p = cmul(p, dcis(ang));
+  }
   a.D[c] = p;
 }
 
@@ -1149,14 +1161,16 @@ static size_t plane_mf_lds(int l1) {
   return (size_t)(ic * 33 + ic * (l1p + 1) + l1p * 33) * sizeof(float);
 }
 
+static int plane_ncomp(const PlanePass &p) { return p.ncomp > 0 ? p.ncomp : 3; }
 template <class T>
 static dim3 plane_grid(const PlanePass &p) {
   int gx = 1, gy = 1;
-  for (int c = 0; c < 3; ++c) {
+  const int nc = plane_ncomp(p);
+  for (int c = 0; c < nc; ++c) {
     gx = cdiv(p.e[c][0], PlaneCfg<T>::MS) > gx ? cdiv(p.e[c][0], PlaneCfg<T>::MS) : gx;
     gy = p.e[c][2] > gy ? p.e[c][2] : gy;
   }
-  return dim3(gx, gy, 3);
+  return dim3(gx, gy, nc);
 }
 template <class T>
 int plane_parts(const PlanePass &p) {
@@ -1164,19 +1178,21 @@ int plane_parts(const PlanePass &p) {
   return (int)(g.x * g.y * g.z);
 }
 template <class T>
-bool plane_fits(const int e[3][3]) {
-  for (int c = 0; c < 3; ++c)
+bool plane_fits(const int e[][3], int ncomp) {
+  if (ncomp < 1 || ncomp > PLANE_MAXC) return false;
+  for (int c = 0; c < ncomp; ++c)
     if (e[c][1] > PLANE_LMAX || e[c][0] < 1 || e[c][1] < 1 || e[c][2] < 1) return false;
   int l1 = 0;
-  for (int c = 0; c < 3; ++c) l1 = e[c][1] > l1 ? e[c][1] : l1;
+  for (int c = 0; c < ncomp; ++c) l1 = e[c][1] > l1 ? e[c][1] : l1;
   return (sizeof(T) == 4 ? plane_mf_lds(l1) : plane_lds<T>(l1)) <= 64 * 1024;
 }
 
 template <class TI, class T, class TO>
 hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
-  if (!plane_fits<T>(p.e) || plane_parts<T>(p) > MAX_PART) return hipErrorInvalidValue;
+  const int nc = plane_ncomp(p);
+  if (!plane_fits<T>(p.e, nc) || plane_parts<T>(p) > MAX_PART) return hipErrorInvalidValue;
   int l1 = 0;
-  for (int c = 0; c < 3; ++c) l1 = p.e[c][1] > l1 ? p.e[c][1] : l1;
+  for (int c = 0; c < nc; ++c) l1 = p.e[c][1] > l1 ? p.e[c][1] : l1;
   const dim3 gr = plane_grid<T>(p);
   const bool mf = sizeof(T) == 4;  // Float32 on the matrix cores
   const size_t lds = mf ? plane_mf_lds(l1) : plane_lds<T>(l1);
@@ -1206,8 +1222,8 @@ template hipError_t launch_plane<float, float, double>(const PlanePass &, const 
 template hipError_t launch_plane<double, double, double>(const PlanePass &, const int *, hipStream_t);
 template int plane_parts<float>(const PlanePass &);
 template int plane_parts<double>(const PlanePass &);
-template bool plane_fits<float>(const int e[3][3]);
-template bool plane_fits<double>(const int e[3][3]);
+template bool plane_fits<float>(const int e[][3], int);
+template bool plane_fits<double>(const int e[][3], int);
 
 // ---------------------------------------------------------------------------- z, modes, inverse z
 // Thread (column tc, mode mz) transforms the three z-columns that meet at mode (mx, my, mz)
@@ -1285,6 +1301,97 @@ hipError_t launch_zmodes(const ZModes &m, const int *gate, hipStream_t s) {
 }
 template hipError_t launch_zmodes<float>(const ZModes &, const int *, hipStream_t);
 template hipError_t launch_zmodes<double>(const ZModes &, const int *, hipStream_t);
+
+// ============================================================================ normal fluids
+// C_µ at a block node (_conv, exchange.jl:316-337, no drag): the τ-masked divergence of the band's
+// link supercurrents and its Josephson exchange, in the reference's order of additions
+__device__ __forceinline__ double qp_conv1(const Geo &g, const QpConv &a, int mu, int c, int sy, int sz) {
+  const cplx *P = a.G[mu];
+  const cplx p = cconj(P[c]);
+  const double gx = a.wa[0][mu], gy = a.wa[1][mu], gz = a.wa[2][mu];
+  const double *tx = a.tau[0], *ty = a.tau[1], *tz = a.tau[2];
+  const cplx *Vx = a.V[0], *Vy = a.V[1], *Vz = a.V[2];
+  double v = ((gx * (tx[c] * cmul(cmul(p, Vx[c]), P[c + 1]).y + tx[c - 1] * cmul(cmul(p, cconj(Vx[c - 1])), P[c - 1]).y) *
+                   a.rx2 +
+               gy * (ty[c] * cmul(cmul(p, Vy[c]), P[c + sy]).y + ty[c - sy] * cmul(cmul(p, cconj(Vy[c - sy])), P[c - sy]).y) *
+                   a.ry2) +
+              gz * (tz[c] * cmul(cmul(p, Vz[c]), P[c + sz]).y + tz[c - sz] * cmul(cmul(p, cconj(Vz[c - sz])), P[c - sz]).y) *
+                  a.rz2);
+  for (int nu = 0; nu < a.nb; ++nu) {
+    if (nu == mu) continue;
+    v += a.gam[mu][nu] * cmul(p, a.G[nu][c]).y;
+  }
+  return v;
+}
+
+// One thread per unknown of the region; block nodes (faces included) get the source, the rest 0
+__global__ void __launch_bounds__(256) k_qp_conv(Geo g, QpConv a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = a.n1 * a.nj * a.nk;
+  if (t >= n) return;
+  const int ix = t % a.n1, jy = (t / a.n1) % a.nj, kz = t / (a.n1 * a.nj);
+  const int i = ix + 2, j = a.j0 + 1 + jy, k = a.k0 + 1 + kz;
+  double src[TDGL_MAX_BANDS] = {0.0, 0.0, 0.0, 0.0};
+  if (i >= g.x1 && i <= g.x2 + 1 && j >= g.y1 && j <= g.y2 + 1 && k >= g.z1 && k <= g.z2 + 1) {
+    const int c = g.at(i, j, k), sy = g.px, sz = g.px * g.py;
+    if (a.mode == TDGL_QP_TWO) {
+      const double c1 = qp_conv1(g, a, 0, c, sy, sz), c2 = qp_conv1(g, a, 1, c, sy, sz);
+      src[0] = c2 / a.s[1] - c1 / a.s[0];
+    } else {
+      double cv[TDGL_MAX_BANDS], C = 0.0, R[TDGL_MAX_BANDS];
+      for (int mu = 0; mu < a.nb; ++mu) cv[mu] = qp_conv1(g, a, mu, c, sy, sz);
+      for (int mu = 0; mu < a.nb; ++mu) C += cv[mu];
+      for (int mu = 0; mu < a.nb; ++mu) R[mu] = cv[mu] / a.s[mu] - C;
+      for (int kk = 0; kk < a.K; ++kk) {
+        double m = 0.0;
+        for (int mu = 0; mu < a.nb; ++mu) m += a.Pinv[kk][mu] * R[mu];
+        src[kk] = m;
+      }
+    }
+  }
+  for (int kk = 0; kk < a.K; ++kk) a.S[kk][t] = src[kk];
+}
+
+hipError_t launch_qp_conv(const Geo &g, const QpConv &a, hipStream_t s) {
+  const int n = a.n1 * a.nj * a.nk;
+  hipLaunchKernelGGL(k_qp_conv, dim3(cdiv(n, 256)), dim3(256), 0, s, g, a);
+  return hipGetLastError();
+}
+
+// Thread (column tc, z index kz) of mode blockIdx.y: forward z with W_z, the division, then the
+// inverse z of the block's columns from LDS with V_z (sums in increasing index)
+__global__ void __launch_bounds__(256) k_qp_zcol(QpZ a) {
+  __shared__ double Tm[256];
+  const int nk = a.nk, C = 256 / nk;
+  const int t = threadIdx.x, tc = t / nk, kz = t % nk;
+  const int col = blockIdx.x * C + tc;
+  const size_t n = (size_t)a.n1 * a.nj * nk;
+  const int mode = blockIdx.y;
+  const bool live = tc < C && col < a.n1 * a.nj;
+  const int ix = live ? col % a.n1 : 0, jy = live ? col / a.n1 : 0;
+  const size_t cs = (size_t)a.n1 * a.nj;
+  const double *in = a.in + mode * n + ix + (size_t)a.n1 * jy;
+  if (live) {
+    double acc = 0.0;
+#pragma unroll 8
+    for (int q = 0; q < nk; ++q) acc += a.Wz[kz + (size_t)nk * q] * in[cs * q];
+    acc /= a.sigma * (a.ex[ix] + a.ey[jy] + a.ez[kz]) - a.lam[mode];
+    Tm[t] = acc;
+  }
+  __syncthreads();
+  if (!live) return;
+  double acc = 0.0;
+#pragma unroll 8
+  for (int q = 0; q < nk; ++q) acc += a.Vz[kz + (size_t)nk * q] * Tm[tc * nk + q];
+  a.out[mode * n + ix + (size_t)a.n1 * jy + cs * kz] = acc;
+}
+
+hipError_t launch_qp_zcol(const QpZ &a, hipStream_t s) {
+  if (a.nk < 1 || a.nk > 256 || a.K < 1 || a.K > TDGL_MAX_BANDS) return hipErrorInvalidValue;
+  const int C = 256 / a.nk;
+  hipLaunchKernelGGL(k_qp_zcol, dim3(cdiv(a.n1 * a.nj, C), a.K), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
 
 // ============================================================================ PCG
 // Kernels over the box i, j, k in 2..N (the union of the three ranges) with 1D blocks of 64 x 4

@@ -59,8 +59,8 @@ Bases make_bases(int nn) {
 }
 
 // kernel classes of the per-kernel profile (tdgl_profile_get)
-enum { K_STAGE = 0, K_LINKS, K_FIELD, K_CURL, K_GHOSTS, K_SCATTER, K_TX, K_TX32, K_MODES, K_APPLYA, K_DOT, K_FINISH, K_VEC, K_COUNT };
-const char *const K_NAMES[K_COUNT] = {"band_stage", "stage_links", "current_field", "links_curl_bc", "neumann_ghosts",
+enum { K_STAGE = 0, K_QP, K_FIELD, K_CURL, K_GHOSTS, K_SCATTER, K_TX, K_TX32, K_MODES, K_APPLYA, K_DOT, K_FINISH, K_VEC, K_COUNT };
+const char *const K_NAMES[K_COUNT] = {"band_stage", "normal_fluids", "current_field", "links_curl_bc", "neumann_ghosts",
                                       "solve_scatter", "sep_transform_f64", "sep_transform_f32", "sep_modes", "pcg_apply_A",
                                       "pcg_dot", "pcg_finish", "pcg_vector"};
 static_assert(K_COUNT == TDGL_NKERNELS, "kernel classes");
@@ -102,6 +102,11 @@ struct tdgl_handle {
   double *T1 = nullptr, *T2 = nullptr;
   float *T32a = nullptr, *T32b = nullptr;
   bool plane32 = false, plane64 = false;  // the fused transform launches fit (per precision)
+  // normal fluids: the sources / solved modes (region-shaped, one per mode), work, the bases
+  double *qS[TDGL_MAX_BANDS] = {};
+  double *qT1 = nullptr, *qT2 = nullptr;
+  double *qV[3] = {}, *qW[3] = {}, *qe[3] = {};
+  int qn = 0;                  // unknowns of the region
   // right-hand side of the implicit solve and the PCG vectors: node-shaped, zero outside the ranges
   Vec3 B{}, x{}, r{}, z{}, P{}, P2{}, Qv{}, xprev{}, zold{};
   PcgDev pd{};
@@ -262,6 +267,23 @@ StageArgs stage_base(tdgl_handle *h, int mu, const tdgl_bandcoef &cf) {
   return a;
 }
 
+// the turn of band µ by its normal-fluid potential, on the launch that writes its new values
+void stage_qp(tdgl_handle *h, StageArgs &a, int mu) {
+  const tdgl_desc &d = h->d;
+  if (d.qp_mode == TDGL_QP_NONE) return;
+  a.qp = d.qp_mode;
+  a.qpK = d.qp_nmodes;
+  for (int k = 0; k < d.qp_nmodes; ++k) a.qpM[k] = h->qS[k];
+  a.qn1 = d.Nx - 1;
+  a.qnj = d.qp_nj;
+  a.qj0 = d.qp_j0;
+  a.qk0 = d.qp_k0;
+  if (d.qp_mode == TDGL_QP_TWO)
+    a.qpc[0] = d.qp_turn[mu];
+  else
+    for (int k = 0; k < d.qp_nmodes; ++k) a.qpc[k] = d.qp_P[mu][k];
+}
+
 // _band_rkl! (bands.jl:464-521): stage j writes buf[(j - ns) mod 3] of (F, W, W2) so that the
 // last stage lands in F and no stage writes a buffer it reads.  The links of stage j >= 2 are
 // V (V Ū)^c_j, or V _cis(R c_j) in the rotating frame, evaluated inside the stage kernel.
@@ -293,6 +315,7 @@ int band_rkl(tdgl_handle *h, int mu, int ns) {
       a.cnu = kc ? d.rkc_nu[mu][j - 1] : (1.0 - j) / j;
     }
     a.turn = rot && j == ns;
+    if (j == ns) stage_qp(h, a, mu);
     PL(K_STAGE, launch_stage(h->g, a, h->s));
     A = B;
     B = C;
@@ -322,6 +345,7 @@ int band_substeps(tdgl_handle *h, int mu, int n) {
     a.frame = rot ? (m == 1 ? FRAME_RATE : FRAME_TERM) : FRAME_NONE;
     a.h = hh;
     a.turn = rot && m == n;
+    if (m == n) stage_qp(h, a, mu);
     PL(K_STAGE, launch_stage(h->g, a, h->s));
     src = dst;
   }
@@ -341,6 +365,7 @@ int update_bands(tdgl_handle *h) {
       a.frame = d.rotframe ? FRAME_RATE : FRAME_NONE;
       a.h = d.dt;
       a.turn = d.rotframe;
+      stage_qp(h, a, mu);
       PL(K_STAGE, launch_stage(h->g, a, h->s));
     } else {
       CHK(band_substeps(h, mu, d.nsub[mu]));
@@ -640,9 +665,83 @@ int implicit_pcg(tdgl_handle *h) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------- normal fluids
+// qp_potential! (exchange.jl:344-389, 477-516) from the state at the start of the step: the
+// sources, then for every mode the x-y plane launch (W_x, W_y), the z launch with the division
+// and the inverse z, and the inverse x-y plane launch (V_x, V_y) into the modes the stages read
+int qp_potential(tdgl_handle *h) {
+  const tdgl_desc &d = h->d;
+  const int K = d.qp_nmodes, n1 = d.Nx - 1, nj = d.qp_nj, nk = d.qp_nk;
+  QpConv c{};
+  for (int mu = 0; mu < d.nbands; ++mu) c.G[mu] = h->G[mu];
+  for (int a = 0; a < 3; ++a) {
+    c.V[a] = h->V[a];
+    c.tau[a] = h->tau[a];
+    for (int mu = 0; mu < TDGL_MAX_BANDS; ++mu) c.wa[a][mu] = d.wa[a][mu];
+  }
+  for (int k = 0; k < K; ++k) c.S[k] = h->qS[k];
+  for (int mu = 0; mu < TDGL_MAX_BANDS; ++mu) {
+    c.s[mu] = d.qp_s[mu];
+    for (int nu = 0; nu < TDGL_MAX_BANDS; ++nu) {
+      c.gam[mu][nu] = d.qp_gamma[mu][nu];
+      c.Pinv[mu][nu] = d.qp_Pinv[mu][nu];
+    }
+  }
+  c.rx2 = 1 / (d.dx * d.dx);
+  c.ry2 = 1 / (d.dy * d.dy);
+  c.rz2 = 1 / (d.dz * d.dz);
+  c.nb = d.nbands;
+  c.mode = d.qp_mode;
+  c.K = K;
+  c.n1 = n1;
+  c.nj = nj;
+  c.nk = nk;
+  c.j0 = d.qp_j0;
+  c.k0 = d.qp_k0;
+  PL(K_QP, launch_qp_conv(h->g, c, h->s));
+  const size_t n = (size_t)h->qn;
+  PlanePass f{};
+  f.ncomp = K;
+  for (int k = 0; k < K; ++k) {
+    f.e[k][0] = n1;
+    f.e[k][1] = nj;
+    f.e[k][2] = nk;
+    f.in[k] = View{(void *)h->qS[k], n1, n1 * nj};
+    f.out[k] = View{(void *)(h->qT1 + k * n), n1, n1 * nj};
+    f.X[k] = h->qW[0];
+    f.Y[k] = h->qW[1];
+  }
+  PL(K_QP, (launch_plane<double, double, double>(f, nullptr, h->s)));
+  QpZ z{};
+  z.in = h->qT1;
+  z.out = h->qT2;
+  z.Wz = h->qW[2];
+  z.Vz = h->qV[2];
+  z.ex = h->qe[0];
+  z.ey = h->qe[1];
+  z.ez = h->qe[2];
+  z.sigma = d.qp_sigma;
+  for (int k = 0; k < K; ++k) z.lam[k] = d.qp_lambda[k];
+  z.K = K;
+  z.n1 = n1;
+  z.nj = nj;
+  z.nk = nk;
+  PL(K_QP, launch_qp_zcol(z, h->s));
+  PlanePass b = f;
+  for (int k = 0; k < K; ++k) {
+    b.in[k] = View{(void *)(h->qT2 + k * n), n1, n1 * nj};
+    b.out[k] = View{(void *)h->qS[k], n1, n1 * nj};
+    b.X[k] = h->qV[0];
+    b.Y[k] = h->qV[1];
+  }
+  PL(K_QP, (launch_plane<double, double, double>(b, nullptr, h->s)));
+  return 0;
+}
+
 // ---------------------------------------------------------------------------- one step
 int step_once(tdgl_handle *h) {
   const tdgl_desc &d = h->d;
+  if (d.qp_mode != TDGL_QP_NONE) CHK(qp_potential(h));
   CHK(update_bands(h));
   FieldArgs fa{};
   for (int mu = 0; mu < d.nbands; ++mu) fa.G[mu] = h->G[mu];
@@ -866,6 +965,19 @@ int check_desc(const tdgl_desc *d, char *why, size_t n) {
   if (d->selffield_bc == TDGL_BC_BAR && !(d->bar_hy_z && d->bar_hy_y && d->bar_hz_z && d->bar_hz_y && d->bar_hz_x && d->bar_hy_x))
     return no("bar profiles");
   if (!(d->f && d->taux && d->tauy && d->tauz)) return no("masks");
+  if (d->qp_mode != TDGL_QP_NONE) {
+    if (d->qp_mode != TDGL_QP_TWO && d->qp_mode != TDGL_QP_N) return no("normal-fluid model");
+    if (d->qp_mode == TDGL_QP_TWO && (d->nbands != 2 || d->qp_nmodes != 1)) return no("two normal fluids need two bands");
+    if (d->qp_mode == TDGL_QP_N && (d->qp_nmodes != d->nbands || d->nbands < 2)) return no("normal-fluid modes");
+    if (d->qp_j0 < 0 || d->qp_nj < 1 || d->qp_j0 + d->qp_nj > d->Ny + 1 || d->qp_j0 > d->y1 - 1 ||
+        d->qp_j0 + d->qp_nj < d->y2 + 1 || d->qp_k0 < 0 || d->qp_nk < 1 || d->qp_k0 + d->qp_nk > d->Nz + 1 ||
+        d->qp_k0 > d->z1 - 1 || d->qp_k0 + d->qp_nk < d->z2 + 1)
+      return no("normal-fluid region");
+    if (d->qp_nk > 256 || d->qp_nj > PLANE_LMAX || d->Nx - 1 < 1) return no("normal-fluid region larger than the transform tiles");
+    if (!(d->qp_Vx && d->qp_Wx && d->qp_ex && d->qp_Vy && d->qp_Wy && d->qp_ey && d->qp_Vz && d->qp_Wz && d->qp_ez))
+      return no("normal-fluid bases");
+    if (!(d->qp_sigma > 0)) return no("normal-fluid σn");
+  }
   const double cells = (double)(((d->Nx + 1 + 7) / 8) * 8) * (d->Ny + 1) * (d->Nz + 1);
   if (cells > 2.0e9) return no("grid too large for 32-bit indexing");
   return 1;
@@ -1024,6 +1136,34 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
       fl[FL_MAXIT] = d.pcg_maxit;
       CHK(upload_vec(h->pd.fl, fl));
     }
+    if (d.qp_mode != TDGL_QP_NONE) {
+      const int n1 = d.Nx - 1, nj = d.qp_nj, nk = d.qp_nk, K = d.qp_nmodes;
+      h->qn = n1 * nj * nk;
+      for (int k = 0; k < K; ++k) ALLOC(h->qS[k], (size_t)h->qn);
+      ALLOC(h->qT1, (size_t)h->qn * K);
+      ALLOC(h->qT2, (size_t)h->qn * K);
+      const int nn[3] = {n1, nj, nk};
+      const double *V[3] = {d.qp_Vx, d.qp_Vy, d.qp_Vz}, *W[3] = {d.qp_Wx, d.qp_Wy, d.qp_Wz},
+                   *E[3] = {d.qp_ex, d.qp_ey, d.qp_ez};
+      for (int a = 0; a < 3; ++a) {
+        const size_t m = (size_t)nn[a] * nn[a];
+        ALLOC(h->qV[a], m);
+        ALLOC(h->qW[a], m);
+        ALLOC(h->qe[a], (size_t)nn[a]);
+        CHK(upload_vec(h->qV[a], std::vector<double>(V[a], V[a] + m)));
+        CHK(upload_vec(h->qW[a], std::vector<double>(W[a], W[a] + m)));
+        CHK(upload_vec(h->qe[a], std::vector<double>(E[a], E[a] + nn[a])));
+      }
+      PlanePass t{};
+      t.ncomp = K;
+      for (int k = 0; k < K; ++k) {
+        t.e[k][0] = n1;
+        t.e[k][1] = nj;
+        t.e[k][2] = nk;
+      }
+      if (!plane_fits<double>(t.e, K) || plane_parts<double>(t) > MAX_PART)
+        return fail(TDGL_E_UNSUPPORTED, "normal-fluid region larger than the transform tiles");
+    }
     ALLOC(h->hp, 2);
     HIPCHK(hipHostMalloc((void **)&h->hph, 2 * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&h->flh, FL_COUNT * sizeof(int), hipHostMallocDefault));
@@ -1052,8 +1192,8 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     const char *gp = std::getenv("TDGL_HIP_PLANE");
     const bool zfits = d.Nz - 1 >= 2 && d.Nz - 1 <= 256;
     if (!(gp && gp[0] == '0') && zfits) {
-      h->plane32 = plane_fits<float>(rg.e);
-      h->plane64 = plane_fits<double>(rg.e);
+      h->plane32 = plane_fits<float>(rg.e, 3);
+      h->plane64 = plane_fits<double>(rg.e, 3);
     }
     return 0;
   };

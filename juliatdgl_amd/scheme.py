@@ -533,6 +533,112 @@ def bar_field(gr: Grid, nq: int = 8):
 FIELD_EXPLICIT, FIELD_SEPARABLE, FIELD_PCG, FIELD_EXPLICIT_SCALED = 0, 1, 2, 3
 
 
+QP_NONE, QP_TWO, QP_N = 0, 1, 2
+
+
+def lap1d(n: int, h: float, dirichlet: bool):
+    """The 1D second difference on n nodes with spacing h, Dirichlet (the n unknowns between two
+    fixed ends) or Neumann with mirror ghosts (2/h² to the only neighbour at each end); its
+    eigenbasis V (columns), W = V⁻¹ and the eigenvalues (_lap1d, exchange.jl:148-166), eigenvalues
+    in increasing order."""
+    L = np.zeros((n, n))
+    for a in range(n):
+        L[a, a] = -2 / h ** 2
+        if a > 0:
+            L[a, a - 1] = 1 / h ** 2
+        if a < n - 1:
+            L[a, a + 1] = 1 / h ** 2
+    if not dirichlet and n > 1:
+        L[0, 1] = 2 / h ** 2
+        L[n - 1, n - 2] = 2 / h ** 2
+    e, V = np.linalg.eig(L)
+    e, V = np.real(e), np.real(V)
+    o = np.argsort(e, kind="stable")
+    e, V = e[o], V[:, o]
+    return np.asfortranarray(V), np.asfortranarray(np.linalg.inv(V)), e.copy()
+
+
+@dataclass
+class QPData:
+    """QPExchange / QPExchangeN (exchange.jl:130-191, 425-475) for a uniform, not sculpted sample:
+    what qp_potential! and qp_turn! read."""
+    mode: int
+    K: int                  # Helmholtz problems per step
+    s: tuple                # shares of σn
+    G: float                # two bands: χ_σ ν (the exchange conductance)
+    lam: tuple              # λ0, or the eigenvalues of S⁻¹L
+    P: np.ndarray           # N: μ = P m
+    Pinv: np.ndarray        # N: m = Pinv R
+    turn: tuple             # two bands: m_µ dt, m = (−s_π, s_σ)
+    j0: int
+    nj: int
+    k0: int
+    nk: int
+    bases: tuple            # (Vx, Wx, ex, Vy, Wy, ey, Vz, Wz, ez)
+    Gm: np.ndarray = None   # N: the conductances G_µν
+
+
+def qp_exchange_data(md, gr, nu, split, conductor: bool, dt: float) -> QPData:
+    """QPExchange(md, gr; ν, split, conductor) for a number ν (two bands) or QPExchangeN for a
+    matrix of rates (exchange.jl:170-191, 450-475)."""
+    N = md.N
+    b = md.bands
+    if conductor:
+        jr, kr = (gr.y1, gr.y2 + 1), (gr.z1, gr.z2 + 1)
+    else:
+        jr, kr = (1, gr.Ny + 1), (1, gr.Nz + 1)
+    nj, nk = jr[1] - jr[0] + 1, kr[1] - kr[0] + 1
+    bases = lap1d(gr.Nx - 1, gr.dx, True) + lap1d(nj, gr.dy, False) + lap1d(nk, gr.dz, False)
+    if np.ndim(nu) == 0:
+        if N != 2:
+            raise ValueError(f"qp_exchange: a number is for two bands; with {N} bands give the matrix of rates")
+        nu = float(nu)
+        if not nu >= 0:
+            raise ValueError("qp_exchange: the rate must be >= 0")
+        w = tuple(b[mu].stiffness() * b[mu].Gamma / b[mu].beta for mu in range(2)) if split is None else \
+            tuple(float(x) for x in split)
+        if len(w) != 2 or not all(x > 0 for x in w):
+            raise ValueError("qp_split: two positive shares")
+        sh = (w[0] / (w[0] + w[1]), w[1] / (w[0] + w[1]))
+        chis = sh[0] * md.sigma_n * b[0].Gamma / b[0].stiffness()
+        G = chis * nu
+        lam0 = G * (1 / sh[0] + 1 / sh[1])
+        m = (-sh[1], sh[0])
+        return QPData(QP_TWO, 1, sh, G, (lam0,), np.zeros((N, N)), np.zeros((N, N)),
+                      tuple(m[mu] * dt for mu in range(2)), jr[0] - 1, nj, kr[0] - 1, nk, bases)
+    nu = np.asarray(nu, dtype=float)
+    if nu.shape != (N, N):
+        raise ValueError(f"qp_exchange: an {N}x{N} matrix of rates for {N} bands")
+    if not (all(nu[mu, mu] == 0 for mu in range(N)) and (nu >= 0).all()):
+        raise ValueError("qp_exchange: rates >= 0 with a zero diagonal")
+    w = tuple(b[mu].stiffness() * b[mu].Gamma / b[mu].beta for mu in range(N)) if split is None else \
+        tuple(float(x) for x in split)
+    if len(w) != N or not all(x > 0 for x in w):
+        raise ValueError(f"qp_split: {N} positive shares")
+    tot = 0.0
+    for x in w:
+        tot += x
+    sh = tuple(x / tot for x in w)
+    chi = [sh[mu] * md.sigma_n * b[mu].Gamma / b[mu].stiffness() for mu in range(N)]
+    G = np.array([[chi[mu] * nu[mu, n] for n in range(N)] for mu in range(N)])
+    if not np.allclose(G, G.T, rtol=1e-10, atol=1e-14):
+        raise ValueError("qp_exchange: the rates must satisfy detailed balance, χ_µν_µν = χ_νν_νµ")
+    G = (G + G.T) / 2
+    L = -G.copy()
+    for mu in range(N):
+        acc = 0.0
+        for n in range(N):
+            if n != mu:
+                acc += G[mu, n]
+        L[mu, mu] = acc
+    rs = np.sqrt(np.array(sh))
+    lam, Q = np.linalg.eigh(L / np.outer(rs, rs))
+    P = Q / rs[:, None]
+    Pinv = Q.T * rs[None, :]
+    return QPData(QP_N, N, sh, 0.0, tuple(float(max(x, 0.0)) for x in lam), P, Pinv, tuple(0.0 for _ in range(N)),
+                  jr[0] - 1, nj, kr[0] - 1, nk, bases, G)
+
+
 @dataclass
 class Stepper:
     """The coefficients of Stepper{N} (scheme.jl:167-198) that the step reads."""
@@ -564,6 +670,7 @@ class Stepper:
     pcg_f32: bool = True
     pcg_extrap: bool = True
     selffield_bc: str = "bar"
+    qp: Optional[QPData] = None
 
     @property
     def N(self) -> int:
@@ -577,7 +684,8 @@ class Stepper:
 def make_stepper(md: Model, gr: Grid, mk: Masks, *, implicit=True, substeps=None, dt_scale=0.5,
                  extrap=True, hard=False, sigma_vac=None, pcg_tol=1e-8, selffield_bc="bar",
                  rkl="rkc", rotframe=None, rkl_margin=None, rkc_damping=RKC_EPS, pcg_fast=True,
-                 pcg_f32=True, pcg_extrap=True, dt=None, pcg_maxit=500, **unsupported) -> Stepper:
+                 pcg_f32=True, pcg_extrap=True, dt=None, pcg_maxit=500, qp_exchange=None, qp_split=None,
+                 **unsupported) -> Stepper:
     """Stepper(md, gr, mk; kw...) (scheme.jl:250-434) for the options the HIP stepper runs.
 
     `dt` (an extension): a fixed time step in place of the derived one; the substep and stage
@@ -688,6 +796,7 @@ def make_stepper(md: Model, gr: Grid, mk: Masks, *, implicit=True, substeps=None
             raise NotImplementedError("selffield_bc = open is not run by the HIP stepper")
         raise ValueError("selffield_bc must be slab, bar or open")
     bar = bar_field(gr) if sbc == "bar" else None
+    qp = None if qp_exchange is None else qp_exchange_data(md, gr, qp_exchange, qp_split, pad, dtv)
     return Stepper(md, gr, mk, dtv, tuple(nsub), tuple(stg), bc, bcs, w, wa, fc, mode, D, bar, bool(extrap), rotframe,
                    bool(hard), limit, float(dt_scale), rkl_margin, rkcc, tuple(float(divisor(mu)) for mu in range(N)),
-                   float(pcg_tol), int(pcg_maxit), bool(pcg_fast), bool(pcg_f32), bool(pcg_extrap), sbc)
+                   float(pcg_tol), int(pcg_maxit), bool(pcg_fast), bool(pcg_f32), bool(pcg_extrap), sbc, qp)

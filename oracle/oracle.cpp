@@ -8,7 +8,8 @@
 // stepper runs: N bands; Euler with substeps (extrapolated link roots), RKL1 and damped RKC1
 // stages; KWT; the rotating frame; supercurrent; field increment; the implicit curl-curl by the
 // separable solve or by PCG (exact, fast, Float32 preconditioner); explicit and D-scaled
-// explicit field; links; curl; slab and bar field boundaries; Neumann ghosts; advance.
+// explicit field; links; curl; slab and bar field boundaries; Neumann ghosts; advance; two and N
+// normal fluids with a finite exchange (uniform, separable: exchange.jl).
 // Arrays are Julia-shaped (column-major, 1-based accessors) so every index reads as in the
 // reference.  Complex arithmetic is written out with the formulas of Julia's Base (no fused
 // multiply-add: build with -ffp-contract=off).
@@ -260,6 +261,11 @@ struct orc {
   bool warm = false, have2 = false;
   int iters = 0;
   long long total_iters = 0;
+  // normal fluids (exchange.jl): the bases of the region, and the solved modes (region-shaped)
+  int qp = TDGL_QP_NONE, qK = 0, qj0 = 0, qnj = 0, qk0 = 0, qnk = 0;
+  std::vector<double> qVx, qWx, qex, qVy, qWy, qey, qVz, qWz, qez;
+  std::vector<double> qM[TDGL_MAX_BANDS], qT1, qT2;
+  std::vector<double> qSrc[TDGL_MAX_BANDS];  // the sources of the last solve (before it)
 };
 
 namespace {
@@ -951,8 +957,128 @@ void advance(orc &o) {
   std::swap(o.Vz, o.Uz);
 }
 
+// ---------------------------------------------------------------- normal fluids (exchange.jl)
+// C_µ on a block node: the divergence of band µ's τ-masked link supercurrents plus the
+// Josephson exchange with the other bands (_conv, exchange.jl:316-337; no drag)
+double qp_conv(const orc &o, int mu, int i, int j, int k) {
+  const A3c &P = o.G[mu];
+  const C p = conj(P(i, j, k));
+  const double gx = o.d.wa[0][mu], gy = o.d.wa[1][mu], gz = o.d.wa[2][mu];
+  const double rx2 = 1 / (o.d.dx * o.d.dx), ry2 = 1 / (o.d.dy * o.d.dy), rz2 = 1 / (o.d.dz * o.d.dz);
+  double c = ((gx * (o.tx(i, j, k) * cmul(cmul(p, o.Vx(i, j, k)), P(i + 1, j, k)).im +
+                     o.tx(i - 1, j, k) * cmul(cmul(p, conj(o.Vx(i - 1, j, k))), P(i - 1, j, k)).im) * rx2 +
+               gy * (o.ty(i, j, k) * cmul(cmul(p, o.Vy(i, j, k)), P(i, j + 1, k)).im +
+                     o.ty(i, j - 1, k) * cmul(cmul(p, conj(o.Vy(i, j - 1, k))), P(i, j - 1, k)).im) * ry2) +
+              gz * (o.tz(i, j, k) * cmul(cmul(p, o.Vz(i, j, k)), P(i, j, k + 1)).im +
+                    o.tz(i, j, k - 1) * cmul(cmul(p, conj(o.Vz(i, j, k - 1))), P(i, j, k - 1)).im) * rz2);
+  for (int nu = 0; nu < o.N; ++nu) {
+    if (nu == mu) continue;
+    c += o.d.qp_gamma[mu][nu] * cmul(p, o.G[nu](i, j, k)).im;
+  }
+  return c;
+}
+
+// B = A transformed by Mx along x, My along y, Mz along z (_tx3!, exchange.jl:304-311); M
+// column-major n x n, B(i, ..) = Σ_a M(i, a) A(a, ..) with the sum in increasing a
+void qp_tx3(std::vector<double> &B, const std::vector<double> &A, std::vector<double> &T, int n1, int n2, int n3,
+            const std::vector<double> &Mx, const std::vector<double> &My, const std::vector<double> &Mz) {
+  auto at = [&](int i, int j, int k) { return (size_t)i + (size_t)n1 * (j + (size_t)n2 * k); };
+  for (int k = 0; k < n3; ++k)
+    for (int j = 0; j < n2; ++j)
+      for (int i = 0; i < n1; ++i) {
+        double acc = 0.0;
+        for (int a = 0; a < n1; ++a) acc += Mx[i + (size_t)n1 * a] * A[at(a, j, k)];
+        T[at(i, j, k)] = acc;
+      }
+  for (int k = 0; k < n3; ++k)
+    for (int j = 0; j < n2; ++j)
+      for (int i = 0; i < n1; ++i) {
+        double acc = 0.0;
+        for (int b = 0; b < n2; ++b) acc += T[at(i, b, k)] * My[j + (size_t)n2 * b];
+        B[at(i, j, k)] = acc;
+      }
+  for (int k = 0; k < n3; ++k)
+    for (int j = 0; j < n2; ++j)
+      for (int i = 0; i < n1; ++i) {
+        double acc = 0.0;
+        for (int c = 0; c < n3; ++c) acc += B[at(i, j, c)] * Mz[k + (size_t)n3 * c];
+        T[at(i, j, k)] = acc;
+      }
+  B = T;
+}
+
+// the modes, divided, back (_qp_sep!, exchange.jl:295-302)
+void qp_sep(orc &o, std::vector<double> &X, double lam) {
+  const int n1 = o.Nx - 1, n2 = o.qnj, n3 = o.qnk;
+  qp_tx3(o.qT2, X, o.qT1, n1, n2, n3, o.qWx, o.qWy, o.qWz);
+  for (int k = 0; k < n3; ++k)
+    for (int j = 0; j < n2; ++j)
+      for (int i = 0; i < n1; ++i) {
+        double &t = o.qT2[(size_t)i + (size_t)n1 * (j + (size_t)n2 * k)];
+        t /= o.d.qp_sigma * (o.qex[i] + o.qey[j] + o.qez[k]) - lam;
+      }
+  qp_tx3(X, o.qT2, o.qT1, n1, n2, n3, o.qVx, o.qVy, o.qVz);
+}
+
+// the potentials from the state at the start of the step (qp_potential!, exchange.jl:344-389 for
+// two bands, :477-516 for N): the source on every block node, the separable solves
+void qp_potential(orc &o) {
+  const int n1 = o.Nx - 1, nj = o.qnj, nk = o.qnk, N = o.N;
+  const size_t n = (size_t)n1 * nj * nk;
+  for (int q = 0; q < o.qK; ++q) o.qM[q].assign(n, 0.0);
+  std::vector<double> R[TDGL_MAX_BANDS];
+  if (o.qp == TDGL_QP_N)
+    for (int mu = 0; mu < N; ++mu) R[mu].assign(n, 0.0);
+  for (int k = o.z1; k <= o.z2 + 1; ++k)
+    for (int j = o.y1; j <= o.y2 + 1; ++j)
+      for (int i = o.x1; i <= o.x2 + 1; ++i) {
+        const size_t q = (size_t)(i - 2) + (size_t)n1 * ((j - o.qj0 - 1) + (size_t)nj * (k - o.qk0 - 1));
+        if (o.qp == TDGL_QP_TWO) {
+          const double c1 = qp_conv(o, 0, i, j, k), c2 = qp_conv(o, 1, i, j, k);
+          o.qM[0][q] = c2 / o.d.qp_s[1] - c1 / o.d.qp_s[0];
+        } else {
+          double c[TDGL_MAX_BANDS], C = 0.0;
+          for (int mu = 0; mu < N; ++mu) c[mu] = qp_conv(o, mu, i, j, k);
+          for (int mu = 0; mu < N; ++mu) C += c[mu];
+          for (int mu = 0; mu < N; ++mu) R[mu][q] = c[mu] / o.d.qp_s[mu] - C;
+        }
+      }
+  if (o.qp == TDGL_QP_N)
+    for (int kk = 0; kk < o.qK; ++kk)
+      for (size_t q = 0; q < n; ++q) {
+        double m = 0.0;
+        for (int mu = 0; mu < N; ++mu) m += o.d.qp_Pinv[kk][mu] * R[mu][q];
+        o.qM[kk][q] = m;
+      }
+  for (int kk = 0; kk < o.qK; ++kk) o.qSrc[kk] = o.qM[kk];
+  for (int kk = 0; kk < o.qK; ++kk) qp_sep(o, o.qM[kk], o.d.qp_lambda[kk]);
+}
+
+// every band turns by its potential on the nodes the band update writes (qp_turn!,
+// exchange.jl:400-412 and :533-543)
+void qp_turn(orc &o) {
+  const int n1 = o.Nx - 1, nj = o.qnj;
+  for (int mu = 0; mu < o.N; ++mu)
+    for (int k = o.z1 + 1; k <= o.z2; ++k)
+      for (int j = o.y1 + 1; j <= o.y2; ++j)
+        for (int i = o.x1 + 1; i <= o.x2; ++i) {
+          const size_t q = (size_t)(i - 2) + (size_t)n1 * ((j - o.qj0 - 1) + (size_t)nj * (k - o.qk0 - 1));
+          double a;
+          if (o.qp == TDGL_QP_TWO) {
+            a = -o.d.qp_turn[mu] * o.qM[0][q];
+          } else {
+            double pot = 0.0;
+            for (int kk = 0; kk < o.qK; ++kk) pot += o.d.qp_P[mu][kk] * o.qM[kk][q];
+            a = -o.d.dt * pot;
+          }
+          o.F[mu](i, j, k) = cmul(o.F[mu](i, j, k), cis(a));
+        }
+}
+
 void step(orc &o, double H, double HJ) {
+  if (o.qp != TDGL_QP_NONE) qp_potential(o);
   update_bands(o);
+  if (o.qp != TDGL_QP_NONE) qp_turn(o);
   supercurrent(o);
   field_increment(o);
   switch (o.d.field_mode) {
@@ -1059,6 +1185,26 @@ orc *orc_create(const tdgl_desc *d) {
     }
     o->have_sep = true;
   }
+  if (d->qp_mode != TDGL_QP_NONE) {
+    o->qp = d->qp_mode;
+    o->qK = d->qp_nmodes;
+    o->qj0 = d->qp_j0;
+    o->qnj = d->qp_nj;
+    o->qk0 = d->qp_k0;
+    o->qnk = d->qp_nk;
+    const int n1 = Nx - 1, nj = d->qp_nj, nk = d->qp_nk;
+    o->qVx.assign(d->qp_Vx, d->qp_Vx + (size_t)n1 * n1);
+    o->qWx.assign(d->qp_Wx, d->qp_Wx + (size_t)n1 * n1);
+    o->qex.assign(d->qp_ex, d->qp_ex + n1);
+    o->qVy.assign(d->qp_Vy, d->qp_Vy + (size_t)nj * nj);
+    o->qWy.assign(d->qp_Wy, d->qp_Wy + (size_t)nj * nj);
+    o->qey.assign(d->qp_ey, d->qp_ey + nj);
+    o->qVz.assign(d->qp_Vz, d->qp_Vz + (size_t)nk * nk);
+    o->qWz.assign(d->qp_Wz, d->qp_Wz + (size_t)nk * nk);
+    o->qez.assign(d->qp_ez, d->qp_ez + nk);
+    o->qT1.assign((size_t)n1 * nj * nk, 0.0);
+    o->qT2.assign((size_t)n1 * nj * nk, 0.0);
+  }
   if (d->field_mode == TDGL_FIELD_PCG) {
     o->P[0] = o->Q[0] = A3r(Nx, Ny + 1, Nz + 1);
     o->P[1] = o->Q[1] = A3r(Nx + 1, Ny, Nz + 1);
@@ -1078,6 +1224,18 @@ orc *orc_create(const tdgl_desc *d) {
 }
 
 void orc_destroy(orc *o) { delete o; }
+
+// the solved modes of the last normal-fluid potential (δ for two bands), region-shaped
+void orc_qp_modes(orc *o, int k, double *out) {
+  if (k >= 0 && k < o->qK) std::memcpy(out, o->qM[k].data(), o->qM[k].size() * sizeof(double));
+}
+void orc_qp_source(orc *o, int k, double *out) {
+  if (k >= 0 && k < o->qK) std::memcpy(out, o->qSrc[k].data(), o->qSrc[k].size() * sizeof(double));
+}
+// the potentials of the state as it is now (qp_potential! without a step)
+void orc_qp_potential(orc *o) {
+  if (o->qp != TDGL_QP_NONE) qp_potential(*o);
+}
 
 void orc_set_state(orc *o, const tdgl_state *s) {
   for (int mu = 0; mu < o->N; ++mu) {

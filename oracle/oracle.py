@@ -44,6 +44,9 @@ def lib():
         L.orc_basis.argtypes = [ct.c_int] + [ct.c_void_p] * 3
         L.orc_kwt_update.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_void_p]
         L.orc_rotation.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int, ct.c_void_p]
+        L.orc_qp_modes.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p]
+        L.orc_qp_source.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p]
+        L.orc_qp_potential.argtypes = [ct.c_void_p]
         _lib = L
     return _lib
 
@@ -92,6 +95,32 @@ class Oracle:
 
     def total_iters(self) -> int:
         return int(lib().orc_total_iters(self.h))
+
+    def _qp_region(self):
+        q = self.st.qp
+        return (self.st.grid.Nx - 1, q.nj, q.nk)
+
+    def qp_potential(self):
+        """qp_potential! on the current state (no step); returns the solved modes (δ for two bands),
+        each region-shaped ((Nx-1), nj, nk), and the sources they were solved from."""
+        lib().orc_qp_potential(self.h)
+        return self.qp_modes(), self.qp_sources()
+
+    def qp_modes(self):
+        out = []
+        for k in range(self.st.qp.K):
+            a = np.zeros(self._qp_region(), order="F")
+            lib().orc_qp_modes(self.h, k, a.ctypes.data)
+            out.append(a)
+        return out
+
+    def qp_sources(self):
+        out = []
+        for k in range(self.st.qp.K):
+            a = np.zeros(self._qp_region(), order="F")
+            lib().orc_qp_source(self.h, k, a.ctypes.data)
+            out.append(a)
+        return out
 
     def sep_solve(self, BX, BY, BZ):
         """(I + aK)^{-1} on range arrays (the stepper must have the separable or PCG field)."""

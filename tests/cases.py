@@ -36,6 +36,17 @@ def three_aniso(kappa=2.0):
     return Model.make(tuple(bands), gm, kappa=kappa)
 
 
+def qp_rates(r):
+    """Exchange rates with G_µν = r for every pair (detailed balance, test_physics.jl:404):
+    ν_µν = r / χ_µ with χ_µ = s_µ σn Γ_µ / g_µ and the dirty shares s_µ ∝ g_µ Γ_µ / β_µ."""
+    def rates(md):
+        w = [b.stiffness() * b.Gamma / b.beta for b in md.bands]
+        s = [x / sum(w) for x in w]
+        chi = [s[m] * md.sigma_n * md.bands[m].Gamma / md.bands[m].stiffness() for m in range(md.N)]
+        return [[0.0 if m == n else r / chi[m] for n in range(md.N)] for m in range(md.N)]
+    return rates
+
+
 SMALL = dict(nx=24, ny=12, nz=6, dx=0.5)
 
 # name -> (model factory, mesh overrides, scheme keywords, tolerance)
@@ -56,6 +67,9 @@ CASES = {
     "kwt10_pcg": (lambda: mgb2(gkwt=10.0), dict(buf=2), dict(sigma_vac=0.01, pcg_tol=1e-12), 1e-10),
     "three_aniso": (three_aniso, {}, {}, 1e-10),
     "hard": (mgb2, {}, dict(hard=True), 1e-10),
+    "qp_two": (mgb2, {}, dict(qp_exchange=1.0), 1e-10),
+    "qp_two_kwt_pcg": (lambda: mgb2(gkwt=10.0), dict(buf=2), dict(sigma_vac=0.01, pcg_tol=1e-12, qp_exchange=0.5), 1e-10),
+    "qp_three": (three_frustrated, {}, dict(qp_exchange=qp_rates(0.5)), 1e-10),
 }
 
 
@@ -66,7 +80,7 @@ def make_case(name, **extra):
     m.update(mesh)
     gr = Grid.make(m["nx"], m["ny"], m["nz"], dx=m["dx"], buf=m.get("buf", 1))
     mk = wedge_masks(gr)
-    kw = dict(scheme)
+    kw = {k: (v(md) if callable(v) else v) for k, v in scheme.items()}
     kw.update(extra)
     st = make_stepper(md, gr, mk, **kw)
     return st, tol

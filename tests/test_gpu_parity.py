@@ -71,7 +71,8 @@ def test_hip_matches_oracle(name):
         assert abs(gi - oi) <= max(2, oi // 4), (gi, oi)
 
 
-GRAPH_CASES = ["production", "euler_substeps", "pcg_fast_f32", "pcg_exact", "kwt10_pcg", "three_aniso", "kwt10_euler"]
+GRAPH_CASES = ["production", "euler_substeps", "pcg_fast_f32", "pcg_exact", "kwt10_pcg", "three_aniso", "kwt10_euler",
+               "qp_two_kwt_pcg", "qp_three"]
 
 
 @pytest.mark.parametrize("budget", [None, 1])
