@@ -45,8 +45,9 @@ enum {
   TDGL_FIELD_EXPLICIT_SCALED = 3 /* explicit, scale_increment! by D (implicit.jl:337-342) */
 };
 
-/* Self-field boundary (scheme.jl:414-415, field.jl:189-228). */
-enum { TDGL_BC_SLAB = 0, TDGL_BC_BAR = 1 };
+/* Self-field boundary (scheme.jl:414-416, field.jl:189-228): OPEN is the bar field plus h_δ, the
+ * free-space field of the box's own currents beyond it (gtdgl/src/openbc.jl). */
+enum { TDGL_BC_SLAB = 0, TDGL_BC_BAR = 1, TDGL_BC_OPEN = 2 };
 
 /* Coefficients of one band for one (sub)step: BandCoef (gtdgl/src/scheme.jl:21-28). */
 typedef struct {
@@ -125,6 +126,16 @@ typedef struct {
   const double *qp_Vx, *qp_Wx, *qp_ex;
   const double *qp_Vy, *qp_Wy, *qp_ey;
   const double *qp_Vz, *qp_Wz, *qp_ez;
+
+  /* Open boundary (selffield_bc = TDGL_BC_OPEN; OpenBC, openbc.jl:109-140): h_δ recomputed every
+   * open_every steps from δJ = κ² ∇_h × h − HJ·J_bar and ramped in between.  The bar profiles
+   * above are required too. */
+  int32_t open_every;
+  int32_t open_R;              /* the table covers offsets 0..R per axis (1/4πr beyond) */
+  double open_fac;             /* dx³/κ² */
+  double open_kappa2;          /* κ² */
+  const double *open_lgf;      /* lattice_green(R): (R+1)³, column-major */
+  const double *open_Jbar;     /* (Ny+1, Nz+1): κ² curl of the x-face bar field per unit HJ */
 } tdgl_desc;
 
 /* Normal-fluid models (tdgl_desc.qp_mode) */

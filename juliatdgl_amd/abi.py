@@ -50,7 +50,9 @@ class DescC(ct.Structure):
                 ("qp_lambda", ct.c_double * MAXB), ("qp_P", (ct.c_double * MAXB) * MAXB),
                 ("qp_Pinv", (ct.c_double * MAXB) * MAXB), ("qp_gamma", (ct.c_double * MAXB) * MAXB),
                 ("qp_Vx", dp), ("qp_Wx", dp), ("qp_ex", dp), ("qp_Vy", dp), ("qp_Wy", dp), ("qp_ey", dp),
-                ("qp_Vz", dp), ("qp_Wz", dp), ("qp_ez", dp)]
+                ("qp_Vz", dp), ("qp_Wz", dp), ("qp_ez", dp),
+                ("open_every", ct.c_int32), ("open_R", ct.c_int32), ("open_fac", ct.c_double),
+                ("open_kappa2", ct.c_double), ("open_lgf", dp), ("open_Jbar", dp)]
 
 
 class StateC(ct.Structure):
@@ -122,7 +124,7 @@ class PackedDesc:
         d.d1, d.sxy, d.sxz, d.syx, d.syz, d.szx, d.szy, d.a = (fc.d1, fc.sxy, fc.sxz, fc.syx, fc.syz, fc.szx,
                                                                fc.szy, fc.a)
         d.field_mode = st.field_mode
-        d.selffield_bc = 1 if st.selffield_bc == "bar" else 0
+        d.selffield_bc = {"slab": 0, "bar": 1, "open": 2}[st.selffield_bc]
         d.extrap, d.rotframe, d.hard = int(st.extrap), int(st.rotframe), int(st.hard)
         d.pcg_tol, d.pcg_maxit = st.pcg_tol, st.pcg_maxit
         d.pcg_fast, d.pcg_f32, d.pcg_extrap = int(st.pcg_fast), int(st.pcg_f32), int(st.pcg_extrap)
@@ -156,6 +158,10 @@ class PackedDesc:
                 d.qp_lambda[k] = q.lam[k]
             for name, arr in zip(("Vx", "Wx", "ex", "Vy", "Wy", "ey", "Vz", "Wz", "ez"), q.bases):
                 setattr(d, "qp_" + name, hold(arr))
+        ob = st.open
+        if ob is not None:
+            d.open_every, d.open_R, d.open_fac, d.open_kappa2 = ob.every, ob.R, ob.fac, ob.kappa2
+            d.open_lgf, d.open_Jbar = hold(ob.lgf), hold(ob.Jbar)
         self.c = d
         self._keep = keep
 

@@ -81,6 +81,12 @@ struct CurlArgs {
   int bar;
   const double *bar_hy_z, *bar_hy_y, *bar_hz_z, *bar_hz_y, *bar_hz_x, *bar_hy_x;
   double dydz, dzdx, dxdy;
+  // open boundary: h_δ of the last two recomputations (h-shaped), ramped on the six faces after
+  // the bar field (_open_add!, openbc.jl:218-231) by f = min((count + 1)/every, 1)
+  int open;
+  const double *hold[3], *hnew[3];
+  const int *ofl;
+  int every;
 };
 hipError_t launch_curl(const Geo &g, const CurlArgs &a, hipStream_t s);
 
@@ -91,12 +97,13 @@ struct GhostArgs {
 };
 hipError_t launch_ghosts(const Geo &g, const GhostArgs &a, hipStream_t s);
 
-// device-to-device copies of up to 32 arrays in one launch
+// device-to-device copies of up to 48 arrays in one launch
 struct CopyList {
-  const double *src[32];
-  double *dst[32];
-  int n[32];
+  const double *src[48];
+  double *dst[48];
+  int n[48];
   int cnt;
+  const int *gate;   // copy only while *gate (null: always)
 };
 hipError_t launch_copies(const CopyList &c, hipStream_t s);
 
@@ -211,6 +218,67 @@ struct ZModes {
 };
 template <class T>
 hipError_t launch_zmodes(const ZModes &m, const int *gate, hipStream_t s);
+
+// ---------------------------------------------------------------- open boundary (openbc.jl)
+// the clock: COUNT (−1 before the first step), RECOMP and FIRST of this step
+enum { OF_COUNT = 0, OF_RECOMP, OF_FIRST, OF_N };
+hipError_t launch_open_clock(int *ofl, int every, hipStream_t s);
+
+struct OpenDJ {
+  const double *h[3];      // hx, hy, hz (node-shaped)
+  double *dj[3];           // δJ per link lattice, compact (n1, n2, n3)
+  int n[3][3];
+  const double *Jbar;      // (Ny+1, Nz+1)
+  const double *HJp;       // HJ of the step (device)
+  const int *ofl;
+  double k2, dx, dy, dz;
+};
+hipError_t launch_open_dj(const Geo &g, const OpenDJ &a, hipStream_t s);
+
+// One axis of the FFTs: length M = Π rad, twiddles ω^k = e^{−2πik/M}
+struct FftAxis {
+  int M, nrad;
+  int rad[24];
+  const cplx *tw;
+};
+// Lines of length M along one axis: line (l0, l1) starts at l0·l0stride + l1·l1stride, element e at
+// e·s.  The first nin elements are read (zeros beyond), the first nout written (times `scale`, the
+// real part when out_real).  With ghat, the forward transform is multiplied by Ĝ (real, same line
+// geometry) and transformed back.
+struct FftPass {
+  const void *in;
+  int in_real;
+  long long in_s, in_l0, in_l1;
+  void *out;
+  int out_real;
+  double scale;
+  long long out_s, out_l0, out_l1;
+  int nl0, nl1, nin, nout, dir;
+  int lines_fast;          // consecutive threads over lines (a line's elements are far apart)
+  const double *ghat;
+  long long g_s, g_l0, g_l1;
+  int lpb;                 // set by the launcher
+  FftAxis ax;
+};
+hipError_t launch_fft(FftPass p, const int *gate, hipStream_t s);
+
+struct OpenGreen {         // fac·G on the padded box M0 x M1 x M2 (real)
+  double *out;
+  int M[3];
+  const double *lgf;
+  int R;
+  double fac;
+};
+hipError_t launch_open_green(const OpenGreen &a, hipStream_t s);
+
+struct OpenFace {
+  const double *theta[3];  // θ_δ per lattice, compact
+  int n[3][3];
+  double *hnew[3];
+  const int *ofl;
+  double dx, dy, dz;
+};
+hipError_t launch_open_face(const Geo &g, const OpenFace &a, hipStream_t s);
 
 // ---------------------------------------------------------------- normal fluids (exchange.jl)
 // The sources of the potentials on the region of the unknowns (compact (Nx-1) x nj x nk, zero

@@ -265,6 +265,8 @@ __global__ void __launch_bounds__(256) k_curl(Geo g, CurlArgs a) {
   const int sy = g.px, sz = g.px * g.py;
   const double *phx = a.ph[0], *phy = a.ph[1], *phz = a.ph[2];
   const double H = a.hp[0], HJ = a.hp[1];
+  // the point of the open boundary's ramp (after this step's clock)
+  const double openf = a.open ? fmin((double)(a.ofl[OF_COUNT] + 1) / (double)a.every, 1.0) : 0.0;
   if (i <= Nx && j >= 2 && j <= Ny && k >= 2 && k <= Nz) a.U[0][c] = dcis(-phx[c]);
   if (i >= 2 && i <= Nx && j <= Ny && k >= 2 && k <= Nz) a.U[1][c] = dcis(-phy[c]);
   if (i >= 2 && i <= Nx && j >= 2 && j <= Ny && k <= Nz) a.U[2][c] = dcis(-phz[c]);
@@ -278,6 +280,11 @@ __global__ void __launch_bounds__(256) k_curl(Geo g, CurlArgs a) {
     }
     if (i == 1 || i == Nx + 1 || j == 1 || j == Ny) {
       v = 0.0;
+      set = true;
+    }
+    if (a.open && i >= 2 && i <= Nx && (j == 1 || j == Ny || k == 1 || k == Nz)) {
+      const double o = a.hold[0][c];
+      v = o + openf * (a.hnew[0][c] - o);
       set = true;
     }
     if (set) a.h[0][c] = v;
@@ -318,6 +325,10 @@ __global__ void __launch_bounds__(256) k_curl(Geo g, CurlArgs a) {
         set = true;
       }
     }
+    if (a.open && (((k == 1 || k == Nz) && i >= 2 && i <= Nx - 1) || i == 1 || i == Nx)) {
+      const double o = a.hold[1][c];
+      v += o + openf * (a.hnew[1][c] - o);
+    }
     if (set) a.h[1][c] = v;
   }
   // hz (Nx, Ny, Nz+1)
@@ -336,6 +347,10 @@ __global__ void __launch_bounds__(256) k_curl(Geo g, CurlArgs a) {
       if (k == 1 || k == Nz + 1) v = H + HJ * a.bar_hz_z[(j - 1) + Ny * (k == 1 ? 0 : 1)];
       if (j == 1 || j == Ny) v = H + HJ * a.bar_hz_y[(j == 1 ? 0 : 1) + 2 * (k - 1)];
       if (i == 1 || i == Nx) v = H + HJ * a.bar_hz_x[(j - 1) + Ny * (k - 1)];
+    }
+    if (a.open && (((j == 1 || j == Ny) && i >= 2 && i <= Nx - 1) || i == 1 || i == Nx)) {
+      const double o = a.hold[2][c];
+      v += o + openf * (a.hnew[2][c] - o);
     }
     if (set) a.h[2][c] = v;
   }
@@ -426,6 +441,7 @@ hipError_t launch_ghosts(const Geo &g, const GhostArgs &a, hipStream_t s) {
 __global__ void __launch_bounds__(256) k_copies(CopyList c) {
   const int a = blockIdx.y;
   if (a >= c.cnt) return;
+  if (c.gate && !*c.gate) return;
   const double *src = c.src[a];
   double *dst = c.dst[a];
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < c.n[a]; t += gridDim.x * blockDim.x) dst[t] = src[t];

@@ -107,6 +107,16 @@ struct tdgl_handle {
   double *qT1 = nullptr, *qT2 = nullptr;
   double *qV[3] = {}, *qW[3] = {}, *qe[3] = {};
   int qn = 0;                  // unknowns of the region
+  // open boundary: the clock, h_δ of the last two recomputations (h-shaped), δJ and θ_δ per link
+  // lattice (compact boxes), the padded transform (M0 x M1 x n2 complex) and Ĝ (real) per lattice
+  int *ofl = nullptr;
+  double *ohold[3] = {}, *ohnew[3] = {};
+  double *odj[3] = {}, *oth[3] = {}, *oghat[3] = {};
+  cplx *obuf[3] = {};
+  cplx *otw[3][3] = {};
+  int on[3][3] = {}, oM[3][3] = {};
+  FftAxis oax[3][3] = {};
+  double *olgf = nullptr, *oJbar = nullptr;
   // right-hand side of the implicit solve and the PCG vectors: node-shaped, zero outside the ranges
   Vec3 B{}, x{}, r{}, z{}, P{}, P2{}, Qv{}, xprev{}, zold{};
   PcgDev pd{};
@@ -123,7 +133,7 @@ struct tdgl_handle {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   HostSnap gend{};
-  double *bk[32] = {};     // device copy of a cycle's start (roll back)
+  double *bk[48] = {};     // device copy of a cycle's start (roll back)
   int64_t rollbacks = 0, graph_cycles = 0;
   // counters
   int64_t steps = 0, calls = 0, pcg_iters = 0;
@@ -665,6 +675,233 @@ int implicit_pcg(tdgl_handle *h) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------- open boundary
+// the smallest 2^a 3^b 5^c >= n
+int smooth_size(int n) {
+  for (int m = std::max(1, n);; ++m) {
+    int r = m;
+    for (int f : {2, 3, 5})
+      while (r % f == 0) r /= f;
+    if (r == 1) return m;
+  }
+}
+// the radices of M in the order the line FFT applies them (4s, then 2, 3s, 5s)
+void fft_radices(int M, FftAxis &ax) {
+  ax.M = M;
+  ax.nrad = 0;
+  int m = M;
+  for (int r : {4, 2, 3, 5})
+    while (m % r == 0) {
+      ax.rad[ax.nrad++] = r;
+      m /= r;
+    }
+}
+
+// the five passes of one lattice's convolution: x forward from δJ, y forward, z forward × Ĝ and
+// back, y back, x back into θ_δ (the padding is never stored)
+int open_convolve(tdgl_handle *h, int c, const int *gate) {
+  const int n0 = h->on[c][0], n1 = h->on[c][1], n2 = h->on[c][2];
+  const int M0 = h->oM[c][0], M1 = h->oM[c][1], M2 = h->oM[c][2];
+  const long long P = (long long)M0 * M1;
+  FftPass p{};
+  // x forward: lines (j < n1, k < n2) of δJ
+  p.in = h->odj[c];
+  p.in_real = 1;
+  p.in_s = 1;
+  p.in_l0 = n0;
+  p.in_l1 = (long long)n0 * n1;
+  p.out = h->obuf[c];
+  p.scale = 1.0;
+  p.out_s = 1;
+  p.out_l0 = M0;
+  p.out_l1 = P;
+  p.nl0 = n1;
+  p.nl1 = n2;
+  p.nin = n0;
+  p.nout = M0;
+  p.dir = -1;
+  p.ax = h->oax[c][0];
+  PL(K_CURL, launch_fft(p, gate, h->s));
+  // y forward: lines (i < M0, k < n2)
+  FftPass q{};
+  q.in = q.out = h->obuf[c];
+  q.scale = 1.0;
+  q.in_s = q.out_s = M0;
+  q.in_l0 = q.out_l0 = 1;
+  q.in_l1 = q.out_l1 = P;
+  q.nl0 = M0;
+  q.nl1 = n2;
+  q.nin = n1;
+  q.nout = M1;
+  q.dir = -1;
+  q.lines_fast = 1;
+  q.ax = h->oax[c][1];
+  PL(K_CURL, launch_fft(q, gate, h->s));
+  // z: forward, × Ĝ, back; lines (i < M0, j < M1), only the box's planes kept
+  FftPass z = q;
+  z.in_s = z.out_s = P;
+  z.in_l0 = z.out_l0 = 1;
+  z.in_l1 = z.out_l1 = M0;
+  z.nl0 = M0;
+  z.nl1 = M1;
+  z.nin = n2;
+  z.nout = n2;
+  z.ghat = h->oghat[c];
+  z.g_s = P;
+  z.g_l0 = 1;
+  z.g_l1 = M0;
+  z.ax = h->oax[c][2];
+  PL(K_CURL, launch_fft(z, gate, h->s));
+  // y back
+  FftPass yb = q;
+  yb.nin = M1;
+  yb.nout = n1;
+  yb.dir = 1;
+  PL(K_CURL, launch_fft(yb, gate, h->s));
+  // x back into θ_δ, normalized
+  FftPass xb{};
+  xb.in = h->obuf[c];
+  xb.in_s = 1;
+  xb.in_l0 = M0;
+  xb.in_l1 = P;
+  xb.out = h->oth[c];
+  xb.out_real = 1;
+  xb.scale = 1.0 / ((double)M0 * M1 * M2);
+  xb.out_s = 1;
+  xb.out_l0 = n0;
+  xb.out_l1 = (long long)n0 * n1;
+  xb.nl0 = n1;
+  xb.nl1 = n2;
+  xb.nin = M0;
+  xb.nout = n0;
+  xb.dir = 1;
+  xb.ax = h->oax[c][0];
+  PL(K_CURL, launch_fft(xb, gate, h->s));
+  return 0;
+}
+
+// open_update! (openbc.jl:196-215) on the device: the clock, and on its steps h_δ anew
+int open_update(tdgl_handle *h) {
+  const tdgl_desc &d = h->d;
+  const int *re = h->ofl + OF_RECOMP, *first = h->ofl + OF_FIRST;
+  PL(K_CURL, launch_open_clock(h->ofl, d.open_every, h->s));
+  CopyList keep{};
+  for (int c = 0; c < 3; ++c) {
+    keep.src[c] = h->ohnew[c];
+    keep.dst[c] = h->ohold[c];
+    keep.n[c] = h->g.n;
+  }
+  keep.cnt = 3;
+  keep.gate = re;
+  PL(K_CURL, launch_copies(keep, h->s));
+  OpenDJ dj{};
+  for (int c = 0; c < 3; ++c) {
+    dj.h[c] = h->h[c];
+    dj.dj[c] = h->odj[c];
+    for (int a = 0; a < 3; ++a) dj.n[c][a] = h->on[c][a];
+  }
+  dj.Jbar = h->oJbar;
+  dj.HJp = h->hp + 1;
+  dj.ofl = h->ofl;
+  dj.k2 = d.open_kappa2;
+  dj.dx = d.dx;
+  dj.dy = d.dy;
+  dj.dz = d.dz;
+  PL(K_CURL, launch_open_dj(h->g, dj, h->s));
+  for (int c = 0; c < 3; ++c) CHK(open_convolve(h, c, re));
+  OpenFace f{};
+  for (int c = 0; c < 3; ++c) {
+    f.theta[c] = h->oth[c];
+    f.hnew[c] = h->ohnew[c];
+    for (int a = 0; a < 3; ++a) f.n[c][a] = h->on[c][a];
+  }
+  f.ofl = h->ofl;
+  f.dx = d.dx;
+  f.dy = d.dy;
+  f.dz = d.dz;
+  PL(K_CURL, launch_open_face(h->g, f, h->s));
+  keep.gate = first;  // the first time there is no ramp
+  PL(K_CURL, launch_copies(keep, h->s));
+  return 0;
+}
+
+// Ĝ of each lattice, once: fac·G on the padded box, its three forward transforms, the real part
+int open_green(tdgl_handle *h) {
+  const tdgl_desc &d = h->d;
+  for (int c = 0; c < 3; ++c) {
+    const int M0 = h->oM[c][0], M1 = h->oM[c][1], M2 = h->oM[c][2];
+    const size_t n = (size_t)M0 * M1 * M2;
+    double *g = nullptr;
+    cplx *G = nullptr;
+    HIPCHK(hipMalloc((void **)&g, n * sizeof(double)));
+    if (hipMalloc((void **)&G, n * sizeof(cplx)) != hipSuccess) {
+      (void)hipFree(g);
+      return fail(TDGL_E_NOMEM, "hipMalloc of the padded Green function failed");
+    }
+    OpenGreen og{};
+    og.out = g;
+    og.M[0] = M0;
+    og.M[1] = M1;
+    og.M[2] = M2;
+    og.lgf = h->olgf;
+    og.R = d.open_R;
+    og.fac = d.open_fac;
+    int rc = 0;
+    auto run = [&]() -> int {
+      LCHK(launch_open_green(og, h->s));
+      const long long P = (long long)M0 * M1;
+      FftPass p{};
+      p.in = g;
+      p.in_real = 1;
+      p.in_s = 1;
+      p.in_l0 = M0;
+      p.in_l1 = P;
+      p.out = G;
+      p.scale = 1.0;
+      p.out_s = 1;
+      p.out_l0 = M0;
+      p.out_l1 = P;
+      p.nl0 = M1;
+      p.nl1 = M2;
+      p.nin = p.nout = M0;
+      p.dir = -1;
+      p.ax = h->oax[c][0];
+      LCHK(launch_fft(p, nullptr, h->s));
+      FftPass q{};
+      q.in = q.out = G;
+      q.scale = 1.0;
+      q.in_s = q.out_s = M0;
+      q.in_l0 = q.out_l0 = 1;
+      q.in_l1 = q.out_l1 = P;
+      q.nl0 = M0;
+      q.nl1 = M2;
+      q.nin = q.nout = M1;
+      q.dir = -1;
+      q.lines_fast = 1;
+      q.ax = h->oax[c][1];
+      LCHK(launch_fft(q, nullptr, h->s));
+      FftPass z = q;
+      z.out = h->oghat[c];
+      z.out_real = 1;
+      z.in_s = z.out_s = P;
+      z.in_l0 = z.out_l0 = 1;
+      z.in_l1 = z.out_l1 = M0;
+      z.nl0 = M0;
+      z.nl1 = M1;
+      z.nin = z.nout = M2;
+      z.ax = h->oax[c][2];
+      LCHK(launch_fft(z, nullptr, h->s));
+      HIPCHK(hipStreamSynchronize(h->s));
+      return 0;
+    };
+    rc = run();
+    (void)hipFree(g);
+    (void)hipFree(G);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 // ---------------------------------------------------------------------------- normal fluids
 // qp_potential! (exchange.jl:344-389, 477-516) from the state at the start of the step: the
 // sources, then for every mode the x-y plane launch (W_x, W_y), the z launch with the division
@@ -741,6 +978,7 @@ int qp_potential(tdgl_handle *h) {
 // ---------------------------------------------------------------------------- one step
 int step_once(tdgl_handle *h) {
   const tdgl_desc &d = h->d;
+  if (d.selffield_bc == TDGL_BC_OPEN) CHK(open_update(h));
   if (d.qp_mode != TDGL_QP_NONE) CHK(qp_potential(h));
   CHK(update_bands(h));
   FieldArgs fa{};
@@ -783,7 +1021,16 @@ int step_once(tdgl_handle *h) {
     ca.h[c] = h->h[c];
   }
   ca.hp = h->hp;
-  ca.bar = d.selffield_bc == TDGL_BC_BAR;
+  ca.bar = d.selffield_bc == TDGL_BC_BAR || d.selffield_bc == TDGL_BC_OPEN;
+  if (d.selffield_bc == TDGL_BC_OPEN) {
+    ca.open = 1;
+    for (int c = 0; c < 3; ++c) {
+      ca.hold[c] = h->ohold[c];
+      ca.hnew[c] = h->ohnew[c];
+    }
+    ca.ofl = h->ofl;
+    ca.every = d.open_every;
+  }
   ca.bar_hy_z = h->bar[0];
   ca.bar_hy_y = h->bar[1];
   ca.bar_hz_z = h->bar[2];
@@ -835,6 +1082,13 @@ CopyList backup_list(tdgl_handle *h, bool save) {
       add(h->x.c[k], h->bk[q++], n);
       add(h->xprev.c[k], h->bk[q++], n);
     }
+  if (h->d.field_mode == TDGL_FIELD_PCG && h->d.selffield_bc == TDGL_BC_OPEN) {
+    for (int k = 0; k < 3; ++k) {
+      add(h->ohold[k], h->bk[q++], n);
+      add(h->ohnew[k], h->bk[q++], n);
+    }
+    add((double *)h->ofl, h->bk[q++], 2);  // the clock: four ints
+  }
   return c;
 }
 
@@ -962,8 +1216,17 @@ int check_desc(const tdgl_desc *d, char *why, size_t n) {
     return no("link conductivities D");
   if ((d->field_mode == TDGL_FIELD_SEPARABLE || d->field_mode == TDGL_FIELD_PCG) && !(d->dx == d->dy && d->dy == d->dz))
     return no("implicit curl-curl with dx != dy or dz");
-  if (d->selffield_bc == TDGL_BC_BAR && !(d->bar_hy_z && d->bar_hy_y && d->bar_hz_z && d->bar_hz_y && d->bar_hz_x && d->bar_hy_x))
+  if ((d->selffield_bc == TDGL_BC_BAR || d->selffield_bc == TDGL_BC_OPEN) &&
+      !(d->bar_hy_z && d->bar_hy_y && d->bar_hz_z && d->bar_hz_y && d->bar_hz_x && d->bar_hy_x))
     return no("bar profiles");
+  if (d->selffield_bc < 0 || d->selffield_bc > 2) return no("self-field boundary");
+  if (d->selffield_bc == TDGL_BC_OPEN) {
+    if (!(d->dx == d->dy && d->dy == d->dz)) return no("open boundary with dx != dy or dz");
+    if (d->open_every < 1 || d->open_R < 0 || !d->open_lgf || !d->open_Jbar) return no("open boundary tables");
+    const int nl[3] = {d->Nx + 1, d->Ny + 1, d->Nz + 1};
+    for (int a = 0; a < 3; ++a)
+      if (smooth_size(2 * nl[a] - 1) > 2048) return no("open boundary larger than the line FFT (2048 per padded axis)");
+  }
   if (!(d->f && d->taux && d->tauy && d->tauz)) return no("masks");
   if (d->qp_mode != TDGL_QP_NONE) {
     if (d->qp_mode != TDGL_QP_TWO && d->qp_mode != TDGL_QP_N) return no("normal-fluid model");
@@ -1164,6 +1427,42 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
       if (!plane_fits<double>(t.e, K) || plane_parts<double>(t) > MAX_PART)
         return fail(TDGL_E_UNSUPPORTED, "normal-fluid region larger than the transform tiles");
     }
+    if (d.selffield_bc == TDGL_BC_OPEN) {
+      const int nl[3][3] = {{d.Nx, d.Ny + 1, d.Nz + 1}, {d.Nx + 1, d.Ny, d.Nz + 1}, {d.Nx + 1, d.Ny + 1, d.Nz}};
+      const double pi = 3.14159265358979323846;
+      for (int c = 0; c < 3; ++c) {
+        for (int a = 0; a < 3; ++a) {
+          h->on[c][a] = nl[c][a];
+          const int M = smooth_size(2 * nl[c][a] - 1);
+          h->oM[c][a] = M;
+          fft_radices(M, h->oax[c][a]);
+          std::vector<double> tw(2 * (size_t)M);
+          for (int k = 0; k < M; ++k) {
+            tw[2 * k] = std::cos(2 * pi * k / M);
+            tw[2 * k + 1] = -std::sin(2 * pi * k / M);
+          }
+          ALLOC(h->otw[c][a], (size_t)M);
+          CHK(upload_vec((double *)h->otw[c][a], tw));
+          h->oax[c][a].tw = h->otw[c][a];
+        }
+        const size_t nb = (size_t)nl[c][0] * nl[c][1] * nl[c][2];
+        ALLOC(h->odj[c], nb);
+        ALLOC(h->oth[c], nb);
+        ALLOC(h->obuf[c], (size_t)h->oM[c][0] * h->oM[c][1] * nl[c][2]);
+        ALLOC(h->oghat[c], (size_t)h->oM[c][0] * h->oM[c][1] * h->oM[c][2]);
+        ALLOC(h->ohold[c], n);
+        ALLOC(h->ohnew[c], n);
+      }
+      ALLOC(h->ofl, 4);
+      CHK(upload_vec(h->ofl, std::vector<int>{-1, 0, 0, 0}));
+      const size_t nt = (size_t)(d.open_R + 1) * (d.open_R + 1) * (d.open_R + 1);
+      ALLOC(h->olgf, nt);
+      CHK(upload_vec(h->olgf, std::vector<double>(d.open_lgf, d.open_lgf + nt)));
+      const size_t nj = (size_t)(d.Ny + 1) * (d.Nz + 1);
+      ALLOC(h->oJbar, nj);
+      CHK(upload_vec(h->oJbar, std::vector<double>(d.open_Jbar, d.open_Jbar + nj)));
+      CHK(open_green(h));
+    }
     ALLOC(h->hp, 2);
     HIPCHK(hipHostMalloc((void **)&h->hph, 2 * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&h->flh, FL_COUNT * sizeof(int), hipHostMallocDefault));
@@ -1180,6 +1479,10 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
         ALLOC(h->bk[q++], n);
       }
       for (int k = 0; k < 6; ++k) ALLOC(h->bk[q++], n);
+      if (d.selffield_bc == TDGL_BC_OPEN) {
+        for (int k = 0; k < 6; ++k) ALLOC(h->bk[q++], n);
+        ALLOC(h->bk[q++], 2);
+      }
     }
     // TDGL_HIP_GRAPH=0: every step launched from the host; TDGL_HIP_BUDGET=k: the captured PCG
     // starts with k iterations per solve instead of learning it from an eager first cycle

@@ -536,6 +536,74 @@ FIELD_EXPLICIT, FIELD_SEPARABLE, FIELD_PCG, FIELD_EXPLICIT_SCALED = 0, 1, 2, 3
 QP_NONE, QP_TWO, QP_N = 0, 1, 2
 
 
+# ---------------------------------------------------------------- the open boundary (openbc.jl)
+def _scaled_besseli(R: int, x: float) -> np.ndarray:
+    """e^{-x} I_n(x), n = 0..R, by the trapezoid rule on (1/π)∫₀^π e^{x(cos θ − 1)} cos nθ dθ
+    (_scaled_besseli!, openbc.jl:43-56)."""
+    M = 32 + math.ceil(12 * math.sqrt(x))
+    n = np.arange(R + 1)
+    b = np.zeros(R + 1)
+    for m in range(M + 1):
+        th = math.pi * m / M
+        w = 0.5 if (m == 0 or m == M) else 1.0
+        e = w * math.exp(x * (math.cos(th) - 1))
+        b += e * np.cos(n * th)
+    return b / M
+
+
+def lattice_green(R: int) -> np.ndarray:
+    """G[0:R, 0:R, 0:R], the Green function of −Δ_h on the unit simple-cubic lattice
+    (lattice_green, openbc.jl:67-93): G(n) = ∫₀^∞ Π_i e^{−2t} I_{n_i}(2t) dt by 48-point
+    Gauss–Legendre on [0, 0.5, 2, 8, 32, 128, 512] and the asymptotic tail beyond."""
+    x, w = np.polynomial.legendre.leggauss(48)
+    edges = (0.0, 0.5, 2.0, 8.0, 32.0, 128.0, 512.0)
+    T = edges[-1]
+    G = np.zeros((R + 1, R + 1, R + 1))
+    for p in range(len(edges) - 1):
+        a, c = edges[p], edges[p + 1]
+        for q in range(len(x)):
+            t = (a + c) / 2 + (c - a) / 2 * x[q]
+            wt = (c - a) / 2 * w[q]
+            b = _scaled_besseli(R, 2 * t)
+            G += ((wt * b[:, None, None]) * b[None, :, None]) * b[None, None, :]
+    n = np.arange(R + 1)
+    c = ((4 * n[:, None, None] ** 2 - 1) + (4 * n[None, :, None] ** 2 - 1) + (4 * n[None, None, :] ** 2 - 1)) / 16
+    G += (4 * math.pi) ** -1.5 * (2 / math.sqrt(T) - c * (2 / 3) / T ** 1.5)
+    return np.asfortranarray(G)
+
+
+def lgf_far(i, j, k) -> float:
+    """1/(4πr) with the first correction of the cubic lattice (_lgf_far, openbc.jl:96-99)."""
+    r2 = float(i * i + j * j + k * k)
+    r = math.sqrt(r2)
+    return (1 + (5 * (i ** 4 + j ** 4 + k ** 4) / r2 ** 2 - 3) / (8 * r2)) / (4 * math.pi * r)
+
+
+@dataclass
+class OpenData:
+    """OpenBC (openbc.jl:109-140) as the step reads it."""
+    every: int
+    R: int
+    lgf: np.ndarray        # lattice_green(R)
+    fac: float             # dx³/κ²
+    kappa2: float
+    Jbar: np.ndarray       # (Ny+1, Nz+1)
+
+
+def open_data(gr, kappa: float, bar: dict, every: int, R: int = 8) -> OpenData:
+    if not (gr.dx == gr.dy == gr.dz):
+        raise ValueError(":open needs dx = dy = dz")
+    Ny, Nz = gr.Ny, gr.Nz
+    k2 = kappa ** 2
+    Jb = np.zeros((Ny + 1, Nz + 1), order="F")
+    hz_x, hy_x = bar["hz_x"], bar["hy_x"]
+    for k in range(2, Nz + 1):
+        for j in range(2, Ny + 1):
+            Jb[j - 1, k - 1] = k2 * ((hz_x[j - 1, k - 1] - hz_x[j - 2, k - 1]) / gr.dy -
+                                     (hy_x[j - 1, k - 1] - hy_x[j - 1, k - 2]) / gr.dz)
+    return OpenData(int(every), int(R), lattice_green(R), gr.dx ** 3 / k2, k2, Jb)
+
+
 def lap1d(n: int, h: float, dirichlet: bool):
     """The 1D second difference on n nodes with spacing h, Dirichlet (the n unknowns between two
     fixed ends) or Neumann with mirror ghosts (2/h² to the only neighbour at each end); its
@@ -671,6 +739,7 @@ class Stepper:
     pcg_extrap: bool = True
     selffield_bc: str = "bar"
     qp: Optional[QPData] = None
+    open: Optional[OpenData] = None
 
     @property
     def N(self) -> int:
@@ -685,7 +754,7 @@ def make_stepper(md: Model, gr: Grid, mk: Masks, *, implicit=True, substeps=None
                  extrap=True, hard=False, sigma_vac=None, pcg_tol=1e-8, selffield_bc="bar",
                  rkl="rkc", rotframe=None, rkl_margin=None, rkc_damping=RKC_EPS, pcg_fast=True,
                  pcg_f32=True, pcg_extrap=True, dt=None, pcg_maxit=500, qp_exchange=None, qp_split=None,
-                 **unsupported) -> Stepper:
+                 open_every=1.0, **unsupported) -> Stepper:
     """Stepper(md, gr, mk; kw...) (scheme.jl:250-434) for the options the HIP stepper runs.
 
     `dt` (an extension): a fixed time step in place of the derived one; the substep and stage
@@ -725,7 +794,7 @@ def make_stepper(md: Model, gr: Grid, mk: Masks, *, implicit=True, substeps=None
     if pad and not (0 < sigma_vac <= md.sigma_n):
         raise ValueError("σvac must be in (0, σn]")
     bfield = INF if implicit else (float(sigma_vac) if pad else md.sigma_n) * l / (4 * md.kappa ** 2)
-    bsc = 2 * md.sigma_n / lamL
+    bsc = 2 * md.sigma_n / lamL if lamL != 0 else math.copysign(INF, lamL)  # IEEE division, as Julia's
     ref = min(bfield, bsc, min(INF if substeps[mu] == 0 else bb[mu] * max(substeps[mu], 1) for mu in range(N)))
     nsub = [max(1, math.ceil(dt_scale * ref / bb[mu] - 1e-12)) if substeps[mu] == 0 else substeps[mu]
             for mu in range(N)]
@@ -791,12 +860,11 @@ def make_stepper(md: Model, gr: Grid, mk: Masks, *, implicit=True, substeps=None
     else:
         mode = FIELD_PCG if pad else FIELD_SEPARABLE
     sbc = str(selffield_bc).lower().lstrip(":")
-    if sbc not in ("slab", "bar"):
-        if sbc == "open":
-            raise NotImplementedError("selffield_bc = open is not run by the HIP stepper")
+    if sbc not in ("slab", "bar", "open"):
         raise ValueError("selffield_bc must be slab, bar or open")
-    bar = bar_field(gr) if sbc == "bar" else None
+    bar = bar_field(gr) if sbc in ("bar", "open") else None
+    ob = open_data(gr, md.kappa, bar, max(1, round(open_every / dtv))) if sbc == "open" else None
     qp = None if qp_exchange is None else qp_exchange_data(md, gr, qp_exchange, qp_split, pad, dtv)
     return Stepper(md, gr, mk, dtv, tuple(nsub), tuple(stg), bc, bcs, w, wa, fc, mode, D, bar, bool(extrap), rotframe,
                    bool(hard), limit, float(dt_scale), rkl_margin, rkcc, tuple(float(divisor(mu)) for mu in range(N)),
-                   float(pcg_tol), int(pcg_maxit), bool(pcg_fast), bool(pcg_f32), bool(pcg_extrap), sbc, qp)
+                   float(pcg_tol), int(pcg_maxit), bool(pcg_fast), bool(pcg_f32), bool(pcg_extrap), sbc, qp, ob)

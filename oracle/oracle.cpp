@@ -21,6 +21,7 @@
 // separable solve against a dense inverse of (I + aK); PCG against the separable solve at
 // D = 1; the manufactured rotations of gtdgl/test/rotation_fixture.jl.  The tests that do so
 // are tests/test_oracle_*.py.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -266,6 +267,10 @@ struct orc {
   std::vector<double> qVx, qWx, qex, qVy, qWy, qey, qVz, qWz, qez;
   std::vector<double> qM[TDGL_MAX_BANDS], qT1, qT2;
   std::vector<double> qSrc[TDGL_MAX_BANDS];  // the sources of the last solve (before it)
+  // open boundary (openbc.jl): the clock, δJ and θ_δ on the three link lattices, h_δ of the last
+  // two recomputations (h-shaped)
+  int ocount = -1;
+  A3r odJ[3], oth[3], oold[3], onew[3];
 };
 
 namespace {
@@ -877,6 +882,8 @@ void curl_h(orc &o) {
 }
 
 // field_bc! (field.jl:189-228), no tilt, no open boundary
+void open_add(orc &o);
+
 void field_bc(orc &o, double H, double HJ) {
   const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
   A3r &hx = o.hx, &hy = o.hy, &hz = o.hz;
@@ -938,6 +945,141 @@ void field_bc(orc &o, double H, double HJ) {
     for (int k = 1; k <= Nz; ++k)
       for (int j = 1; j <= Ny + 1; ++j) hy(i, j, k) = HJ * hy_x[(j - 1) + (size_t)(Ny + 1) * (k - 1)];
   }
+  if (o.d.selffield_bc == TDGL_BC_OPEN) open_add(o);
+}
+
+// ---------------------------------------------------------------- open boundary (openbc.jl)
+// G(i, j, k) of −Δ_h: the table within R, 1/4πr with the cubic correction beyond (_lgf, :101-102)
+double lgf(const orc &o, int i, int j, int k) {
+  i = std::abs(i);
+  j = std::abs(j);
+  k = std::abs(k);
+  const int R = o.d.open_R;
+  if (i <= R && j <= R && k <= R) return o.d.open_lgf[i + (size_t)(R + 1) * (j + (size_t)(R + 1) * k)];
+  const double r2 = (double)(i * i + j * j + k * k), r = std::sqrt(r2);
+  const double i4 = (double)i * i * i * i, j4 = (double)j * j * j * j, k4 = (double)k * k * k * k;
+  return (1 + (5 * (i4 + j4 + k4) / (r2 * r2) - 3) / (8 * r2)) / (4 * M_PI * r);
+}
+
+// δJ = κ² ∇_h × h − HJ·J_bar on the links of the box, the face-normal links left out
+// (_delta_current!, openbc.jl:143-160)
+void open_delta_current(orc &o, double HJ) {
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+  const double k2 = o.d.open_kappa2, dx = o.d.dx, dy = o.d.dy, dz = o.d.dz;
+  A3r &bx = o.odJ[0], &by = o.odJ[1], &bz = o.odJ[2];
+  std::fill(bx.v.begin(), bx.v.end(), 0.0);
+  std::fill(by.v.begin(), by.v.end(), 0.0);
+  std::fill(bz.v.begin(), bz.v.end(), 0.0);
+  const A3r &hx = o.hx, &hy = o.hy, &hz = o.hz;
+  for (int k = 2; k <= Nz; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 2; i <= Nx - 1; ++i)
+        bx(i, j, k) = k2 * ((hz(i, j, k) - hz(i, j - 1, k)) / dy - (hy(i, j, k) - hy(i, j, k - 1)) / dz) -
+                      HJ * o.d.open_Jbar[(j - 1) + (size_t)(Ny + 1) * (k - 1)];
+  for (int k = 2; k <= Nz; ++k)
+    for (int j = 2; j <= Ny - 1; ++j)
+      for (int i = 2; i <= Nx; ++i) by(i, j, k) = k2 * ((hx(i, j, k) - hx(i, j, k - 1)) / dz - (hz(i, j, k) - hz(i - 1, j, k)) / dx);
+  for (int k = 2; k <= Nz - 1; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 2; i <= Nx; ++i) bz(i, j, k) = k2 * ((hy(i, j, k) - hy(i - 1, j, k)) / dx - (hx(i, j, k) - hx(i, j - 1, k)) / dy);
+}
+
+// θ_δ = (dx³/κ²) G ∗ δJ on every link of the box (_convolve!, openbc.jl:163-172): the linear
+// convolution that the reference computes by zero padding and FFT, here as the direct sum
+void open_convolve(orc &o) {
+  for (int c = 0; c < 3; ++c) {
+    const A3r &b = o.odJ[c];
+    A3r &t = o.oth[c];
+    std::vector<int> si, sj, sk;
+    std::vector<double> sv;
+    for (int k = 1; k <= b.n3; ++k)
+      for (int j = 1; j <= b.n2; ++j)
+        for (int i = 1; i <= b.n1; ++i)
+          if (b(i, j, k) != 0.0) {
+            si.push_back(i);
+            sj.push_back(j);
+            sk.push_back(k);
+            sv.push_back(b(i, j, k));
+          }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= t.n3; ++k)
+      for (int j = 1; j <= t.n2; ++j)
+        for (int i = 1; i <= t.n1; ++i) {
+          double acc = 0.0;
+          for (size_t q = 0; q < sv.size(); ++q) acc += (o.d.open_fac * lgf(o, i - si[q], j - sj[q], k - sk[q])) * sv[q];
+          t(i, j, k) = acc;
+        }
+  }
+}
+
+// h_δ = ∇_h × θ_δ / area on the tangential plaquettes of the six faces (_face_field!, :175-188)
+void open_face_field(orc &o, A3r *h) {
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+  const double dx = o.d.dx, dy = o.d.dy, dz = o.d.dz;
+  const A3r &px = o.oth[0], &py = o.oth[1], &pz = o.oth[2];
+  auto hzv = [&](int i, int j, int k) { return ((py(i + 1, j, k) - py(i, j, k)) - px(i, j + 1, k) + px(i, j, k)) / (dx * dy); };
+  auto hxv = [&](int i, int j, int k) { return ((pz(i, j + 1, k) - pz(i, j, k)) - py(i, j, k + 1) + py(i, j, k)) / (dy * dz); };
+  auto hyv = [&](int i, int j, int k) { return ((px(i, j, k + 1) - px(i, j, k)) - pz(i + 1, j, k) + pz(i, j, k)) / (dz * dx); };
+  A3r &hx = h[0], &hy = h[1], &hz = h[2];
+  for (int j : {1, Ny})
+    for (int k = 1; k <= Nz + 1; ++k)
+      for (int i = 1; i <= Nx; ++i) hz(i, j, k) = hzv(i, j, k);
+  for (int j : {1, Ny})
+    for (int k = 1; k <= Nz; ++k)
+      for (int i = 1; i <= Nx + 1; ++i) hx(i, j, k) = hxv(i, j, k);
+  for (int k : {1, Nz})
+    for (int j = 1; j <= Ny + 1; ++j)
+      for (int i = 1; i <= Nx; ++i) hy(i, j, k) = hyv(i, j, k);
+  for (int k : {1, Nz})
+    for (int j = 1; j <= Ny; ++j)
+      for (int i = 1; i <= Nx + 1; ++i) hx(i, j, k) = hxv(i, j, k);
+  for (int i : {1, Nx})
+    for (int k = 1; k <= Nz + 1; ++k)
+      for (int j = 1; j <= Ny; ++j) hz(i, j, k) = hzv(i, j, k);
+  for (int i : {1, Nx})
+    for (int k = 1; k <= Nz; ++k)
+      for (int j = 1; j <= Ny + 1; ++j) hy(i, j, k) = hyv(i, j, k);
+}
+
+// open_update! (openbc.jl:196-215): the clock; every `every` steps (and the first time) h_δ anew
+void open_update(orc &o, double HJ) {
+  const bool first = o.ocount < 0;
+  o.ocount += 1;
+  if (o.ocount >= o.d.open_every || first) {
+    for (int c = 0; c < 3; ++c) o.oold[c] = o.onew[c];
+    open_delta_current(o, HJ);
+    open_convolve(o);
+    open_face_field(o, o.onew);
+    if (first)
+      for (int c = 0; c < 3; ++c) o.oold[c] = o.onew[c];
+    o.ocount = 0;
+  }
+}
+
+// h_δ at the current point of the ramp, on the six faces after the bar field (_open_add!, :218-231)
+void open_add(orc &o) {
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+  const double f = std::min((double)(o.ocount + 1) / (double)o.d.open_every, 1.0);
+  const A3r *ol = o.oold, *nw = o.onew;
+  auto rampv = [&](int c, int i, int j, int k) { return ol[c](i, j, k) + f * (nw[c](i, j, k) - ol[c](i, j, k)); };
+  for (int j : {1, Ny})
+    for (int k = 1; k <= Nz + 1; ++k)
+      for (int i = 2; i <= Nx - 1; ++i) o.hz(i, j, k) += rampv(2, i, j, k);
+  for (int j : {1, Ny})
+    for (int k = 1; k <= Nz; ++k)
+      for (int i = 2; i <= Nx; ++i) o.hx(i, j, k) = rampv(0, i, j, k);
+  for (int k : {1, Nz})
+    for (int j = 1; j <= Ny + 1; ++j)
+      for (int i = 2; i <= Nx - 1; ++i) o.hy(i, j, k) += rampv(1, i, j, k);
+  for (int k : {1, Nz})
+    for (int j = 1; j <= Ny; ++j)
+      for (int i = 2; i <= Nx; ++i) o.hx(i, j, k) = rampv(0, i, j, k);
+  for (int i : {1, Nx})
+    for (int k = 1; k <= Nz + 1; ++k)
+      for (int j = 1; j <= Ny; ++j) o.hz(i, j, k) += rampv(2, i, j, k);
+  for (int i : {1, Nx})
+    for (int k = 1; k <= Nz; ++k)
+      for (int j = 1; j <= Ny + 1; ++j) o.hy(i, j, k) += rampv(1, i, j, k);
 }
 
 // neumann! (field.jl:231-245) and advance! (field.jl:248-263)
@@ -1076,6 +1218,7 @@ void qp_turn(orc &o) {
 }
 
 void step(orc &o, double H, double HJ) {
+  if (o.d.selffield_bc == TDGL_BC_OPEN) open_update(o, HJ);
   if (o.qp != TDGL_QP_NONE) qp_potential(o);
   update_bands(o);
   if (o.qp != TDGL_QP_NONE) qp_turn(o);
@@ -1137,7 +1280,7 @@ orc *orc_create(const tdgl_desc *d) {
     load(o->Dy, d->Dy);
     load(o->Dz, d->Dz);
   }
-  if (d->selffield_bc == TDGL_BC_BAR) {
+  if (d->selffield_bc == TDGL_BC_BAR || d->selffield_bc == TDGL_BC_OPEN) {
     size_t n[6] = {(size_t)(Ny + 1) * 2, (size_t)2 * Nz, (size_t)Ny * 2, (size_t)2 * (Nz + 1), (size_t)Ny * (Nz + 1),
                    (size_t)(Ny + 1) * Nz};
     const double *src[6] = {d->bar_hy_z, d->bar_hy_y, d->bar_hz_z, d->bar_hz_y, d->bar_hz_x, d->bar_hy_x};
@@ -1185,6 +1328,14 @@ orc *orc_create(const tdgl_desc *d) {
     }
     o->have_sep = true;
   }
+  if (d->selffield_bc == TDGL_BC_OPEN) {
+    o->odJ[0] = o->oth[0] = A3r(Nx, Ny + 1, Nz + 1);
+    o->odJ[1] = o->oth[1] = A3r(Nx + 1, Ny, Nz + 1);
+    o->odJ[2] = o->oth[2] = A3r(Nx + 1, Ny + 1, Nz);
+    o->oold[0] = o->onew[0] = A3r(Nx + 1, Ny, Nz);
+    o->oold[1] = o->onew[1] = A3r(Nx, Ny + 1, Nz);
+    o->oold[2] = o->onew[2] = A3r(Nx, Ny, Nz + 1);
+  }
   if (d->qp_mode != TDGL_QP_NONE) {
     o->qp = d->qp_mode;
     o->qK = d->qp_nmodes;
@@ -1231,6 +1382,26 @@ void orc_qp_modes(orc *o, int k, double *out) {
 }
 void orc_qp_source(orc *o, int k, double *out) {
   if (k >= 0 && k < o->qK) std::memcpy(out, o->qSrc[k].data(), o->qSrc[k].size() * sizeof(double));
+}
+// the open boundary's clock and h_δ of its last two recomputations (h-shaped, hx hy hz each)
+int orc_open_state(orc *o, double *hold, double *hnew) {
+  if (o->d.selffield_bc != TDGL_BC_OPEN) return -1;
+  size_t q = 0;
+  for (int c = 0; c < 3; ++c) {
+    std::memcpy(hold + q, o->oold[c].v.data(), o->oold[c].size() * sizeof(double));
+    std::memcpy(hnew + q, o->onew[c].v.data(), o->onew[c].size() * sizeof(double));
+    q += o->oold[c].size();
+  }
+  return o->ocount;
+}
+// θ_δ and δJ of the last recomputation (link-shaped x, y, z each)
+void orc_open_fields(orc *o, double *theta, double *dj) {
+  size_t q = 0;
+  for (int c = 0; c < 3; ++c) {
+    std::memcpy(theta + q, o->oth[c].v.data(), o->oth[c].size() * sizeof(double));
+    std::memcpy(dj + q, o->odJ[c].v.data(), o->odJ[c].size() * sizeof(double));
+    q += o->oth[c].size();
+  }
 }
 // the potentials of the state as it is now (qp_potential! without a step)
 void orc_qp_potential(orc *o) {

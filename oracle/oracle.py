@@ -47,6 +47,9 @@ def lib():
         L.orc_qp_modes.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p]
         L.orc_qp_source.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p]
         L.orc_qp_potential.argtypes = [ct.c_void_p]
+        L.orc_open_state.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.orc_open_state.restype = ct.c_int
+        L.orc_open_fields.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p]
         _lib = L
     return _lib
 
@@ -122,11 +125,39 @@ class Oracle:
             out.append(a)
         return out
 
+    def open_state(self):
+        """(count, old, new) of the open boundary: h_δ of its last two recomputations, each the
+        three h-shaped arrays."""
+        gr = self.st.grid
+        sh = ((gr.Nx + 1, gr.Ny, gr.Nz), (gr.Nx, gr.Ny + 1, gr.Nz), (gr.Nx, gr.Ny, gr.Nz + 1))
+        n = sum(int(np.prod(x)) for x in sh)
+        a, b = np.zeros(n), np.zeros(n)
+        cnt = lib().orc_open_state(self.h, a.ctypes.data, b.ctypes.data)
+        return cnt, _split(a, sh), _split(b, sh)
+
+    def open_fields(self):
+        """θ_δ and δJ of the last recomputation, each the three link-shaped arrays."""
+        gr = self.st.grid
+        sh = ((gr.Nx, gr.Ny + 1, gr.Nz + 1), (gr.Nx + 1, gr.Ny, gr.Nz + 1), (gr.Nx + 1, gr.Ny + 1, gr.Nz))
+        n = sum(int(np.prod(x)) for x in sh)
+        a, b = np.zeros(n), np.zeros(n)
+        lib().orc_open_fields(self.h, a.ctypes.data, b.ctypes.data)
+        return _split(a, sh), _split(b, sh)
+
     def sep_solve(self, BX, BY, BZ):
         """(I + aK)^{-1} on range arrays (the stepper must have the separable or PCG field)."""
         out = [np.array(b, dtype=np.float64, order="F", copy=True) for b in (BX, BY, BZ)]
         lib().orc_sep_solve(self.h, *(o.ctypes.data for o in out))
         return out
+
+
+def _split(a, shapes):
+    out, q = [], 0
+    for sh in shapes:
+        n = int(np.prod(sh))
+        out.append(a[q:q + n].reshape(sh, order="F"))
+        q += n
+    return out
 
 
 def basis(nn: int):

@@ -73,4 +73,6 @@ def test_stepper_options_outside_the_path_are_refused():
     with pytest.raises(NotImplementedError):
         make_stepper(mgb2(), gr, box_masks(gr), noise=1e-3)
     with pytest.raises(NotImplementedError):
-        make_stepper(mgb2(), gr, box_masks(gr), selffield_bc="open")
+        make_stepper(mgb2(), gr, box_masks(gr), heat=1.0)
+    with pytest.raises(NotImplementedError):
+        make_stepper(mgb2(), gr, box_masks(gr), drag=0.1)

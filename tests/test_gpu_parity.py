@@ -72,7 +72,7 @@ def test_hip_matches_oracle(name):
 
 
 GRAPH_CASES = ["production", "euler_substeps", "pcg_fast_f32", "pcg_exact", "kwt10_pcg", "three_aniso", "kwt10_euler",
-               "qp_two_kwt_pcg", "qp_three"]
+               "qp_two_kwt_pcg", "qp_three", "open", "open_kwt_pcg"]
 
 
 @pytest.mark.parametrize("budget", [None, 1])
