@@ -202,6 +202,18 @@ int tdgl_step(tdgl_handle *h, int32_t n, double H, double HJ);
 int tdgl_get_pcg(tdgl_handle *h, tdgl_pcg *p);
 int tdgl_set_pcg(tdgl_handle *h, const tdgl_pcg *p);
 
+/* Stepper-side state of the open boundary that a checkpoint holds (OpenBC count, old, new;
+ * open_extra / restore_open!, openbc.jl:109-118): the clock (−1 before the first step) and h_δ of
+ * the last two recomputations, h-shaped: hx (Nx+1, Ny, Nz), hy (Nx, Ny+1, Nz), hz (Nx, Ny, Nz+1).
+ * Only the face values are meaningful.  TDGL_E_ARG for a stepper without the open boundary. */
+typedef struct {
+  int32_t count;
+  double *hold[3];
+  double *hnew[3];
+} tdgl_open;
+int tdgl_get_open(tdgl_handle *h, tdgl_open *o);
+int tdgl_set_open(tdgl_handle *h, const tdgl_open *o);
+
 /* Counters (execution_stats, TDGLGPU.jl:2086-2087, per handle). */
 int tdgl_get_stats(tdgl_handle *h, tdgl_stats *st);
 

@@ -49,6 +49,19 @@ struct DescC
     Dx::Ptr{Float64}; Dy::Ptr{Float64}; Dz::Ptr{Float64}
     bar_hy_z::Ptr{Float64}; bar_hy_y::Ptr{Float64}; bar_hz_z::Ptr{Float64}; bar_hz_y::Ptr{Float64}
     bar_hz_x::Ptr{Float64}; bar_hy_x::Ptr{Float64}
+    # normal fluids (exchange.jl)
+    qp_mode::Int32; qp_nmodes::Int32
+    qp_j0::Int32; qp_nj::Int32; qp_k0::Int32; qp_nk::Int32
+    qp_sigma::Float64
+    qp_s::NTuple{MAXB,Float64}; qp_turn::NTuple{MAXB,Float64}; qp_lambda::NTuple{MAXB,Float64}
+    qp_P::NTuple{MAXB,NTuple{MAXB,Float64}}; qp_Pinv::NTuple{MAXB,NTuple{MAXB,Float64}}
+    qp_gamma::NTuple{MAXB,NTuple{MAXB,Float64}}
+    qp_Vx::Ptr{Float64}; qp_Wx::Ptr{Float64}; qp_ex::Ptr{Float64}
+    qp_Vy::Ptr{Float64}; qp_Wy::Ptr{Float64}; qp_ey::Ptr{Float64}
+    qp_Vz::Ptr{Float64}; qp_Wz::Ptr{Float64}; qp_ez::Ptr{Float64}
+    # open boundary (openbc.jl)
+    open_every::Int32; open_R::Int32; open_fac::Float64; open_kappa2::Float64
+    open_lgf::Ptr{Float64}; open_Jbar::Ptr{Float64}
 end
 struct StateC
     G::NTuple{MAXB,Ptr{Float64}}; F::NTuple{MAXB,Ptr{Float64}}
@@ -80,8 +93,10 @@ function supported(st::Stepper{N}) where N
     st.drag === nothing || return (false, "interband drag")
     st.noise === nothing || return (false, "thermal noise")
     st.mt === nothing || return (false, "materials per region")
-    st.qpx === nothing || return (false, "normal fluids with a finite exchange")
-    st.open === nothing || return (false, "the open boundary")
+    if st.qpx !== nothing
+        st.qpx.ps.cond === nothing || return (false, "normal fluids in a sculpted sample")
+        st.qpx isa TDGL.QPExchange && st.qpx.var !== nothing && return (false, "normal fluids with materials")
+    end
     st.masks.vac === nothing || return (false, "a sculpted sample")
     any(>(MAXS), st.rkl) && return (false, "more than $MAXS stages")
     (true, "")
@@ -104,21 +119,54 @@ function _desc(st::Stepper{N}) where N
                   MAXB, ntuple(_ -> 0.0, MAXS))
     bar = st.bar
     bp(k) = bar === nothing ? Ptr{Float64}(0) : hold(getproperty(bar, k))
-    d = DescC(1, N, gr.Nx, gr.Ny, gr.Nz, gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2, gr.dx, gr.dy, gr.dz, st.dt,
+    d = DescC(2, N, gr.Nx, gr.Ny, gr.Nz, gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2, gr.dx, gr.dy, gr.dz, st.dt,
               _pad(ntuple(µ -> _bc(st.bc[µ], N, µ), N), MAXB, _ZBC), _pad(ntuple(µ -> _bc(st.bcs[µ], N, µ), N), MAXB, _ZBC),
               _pad(Int32.(st.nsub), MAXB, Int32(0)), _pad(Int32.(st.rkl), MAXB, Int32(0)),
               _pad(ntuple(µ -> Int32(rk[µ] !== nothing), N), MAXB, Int32(0)), _pad(div, MAXB, 0.0),
               tab(k -> k.μ), tab(k -> k.ν), tab(k -> k.c),
               ntuple(i -> _pad(st.wa[i], MAXB, 0.0), 3),
               fc.d1, fc.sxy, fc.sxz, fc.syx, fc.syz, fc.szx, fc.szy, fc.a,
-              mode, bar === nothing ? 0 : 1, st.extrap, st.rotframe, st.hard,
+              mode, st.open !== nothing ? 2 : bar === nothing ? 0 : 1, st.extrap, st.rotframe, st.hard,
               pcg === nothing ? 0.0 : pcg.tol, pcg === nothing ? 0 : pcg.maxit,
               Int32(fast), Int32(fast && pcg.fast.f32), Int32(fast && pcg.fast.extrap),
               hold(mk.f), hold(mk.τx), hold(mk.τy), hold(mk.τz),
               D === nothing ? Ptr{Float64}(0) : hold(D[1]), D === nothing ? Ptr{Float64}(0) : hold(D[2]),
               D === nothing ? Ptr{Float64}(0) : hold(D[3]),
-              bp(:hy_z), bp(:hy_y), bp(:hz_z), bp(:hz_y), bp(:hz_x), bp(:hy_x))
+              bp(:hy_z), bp(:hy_y), bp(:hz_z), bp(:hz_y), bp(:hz_x), bp(:hy_x),
+              _qp_fields(st, hold)..., _open_fields(st, hold)...)
     d, keep
+end
+
+const _Z4 = ntuple(_ -> 0.0, MAXB)
+const _Z44 = ntuple(_ -> _Z4, MAXB)
+_null() = Ptr{Float64}(0)
+
+# the QPExchange / QPExchangeN of st as the description's qp fields (exchange.jl:130-146, 425-444)
+function _qp_fields(st::Stepper{N}, hold) where N
+    q = st.qpx
+    q === nothing && return (Int32(0), Int32(0), Int32(0), Int32(0), Int32(0), Int32(0), 0.0, _Z4, _Z4, _Z4, _Z44, _Z44,
+                             _Z44, ntuple(_ -> _null(), 9)...)
+    _, nj, nk = q isa TDGL.QPExchange ? size(q.S) : size(q.M[1])
+    γ = _pad(ntuple(µ -> _pad(ntuple(ν -> st.model.γ[µ][ν], N), MAXB, 0.0), N), MAXB, _Z4)
+    bases = (hold(q.Vx), hold(q.Wx), hold(q.ex), hold(q.Vy), hold(q.Wy), hold(q.ey), hold(q.Vz), hold(q.Wz), hold(q.ez))
+    if q isa TDGL.QPExchange
+        m = (-q.s[2], q.s[1])
+        return (Int32(1), Int32(1), Int32(q.ps.j0 - 1), Int32(nj), Int32(q.ps.k0 - 1), Int32(nk), q.σn,
+                _pad(q.s, MAXB, 0.0), _pad((m[1] * st.dt, m[2] * st.dt), MAXB, 0.0), _pad((q.λ0,), MAXB, 0.0),
+                _Z44, _Z44, γ, bases...)
+    end
+    P = _pad(ntuple(µ -> _pad(ntuple(k -> q.P[µ, k], N), MAXB, 0.0), N), MAXB, _Z4)
+    Pinv = _pad(ntuple(k -> _pad(ntuple(µ -> q.Pinv[k, µ], N), MAXB, 0.0), N), MAXB, _Z4)
+    (Int32(2), Int32(N), Int32(q.ps.j0 - 1), Int32(nj), Int32(q.ps.k0 - 1), Int32(nk), q.σn, _pad(q.s, MAXB, 0.0), _Z4,
+     _pad(Tuple(q.λ), MAXB, 0.0), P, Pinv, γ, bases...)
+end
+
+# the OpenBC of st as the description's open fields (openbc.jl:109-140; R = 8, the table OpenBC uses)
+function _open_fields(st::Stepper, hold)
+    ob = st.open
+    ob === nothing && return (Int32(0), Int32(0), 0.0, 0.0, _null(), _null())
+    gr = st.grid; κ2 = st.model.κ^2
+    (Int32(ob.every), Int32(8), gr.dx^3 / κ2, κ2, hold(TDGL.lattice_green(8)), hold(ob.Jbar))
 end
 
 _lib() = (Libdl.dlopen(libpath()); libpath())

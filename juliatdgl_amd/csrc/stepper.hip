@@ -1678,6 +1678,37 @@ int tdgl_set_pcg(tdgl_handle *h, const tdgl_pcg *p) {
   return 0;
 }
 
+int tdgl_get_open(tdgl_handle *h, tdgl_open *o) {
+  if (!h || !o) return fail(TDGL_E_ARG, "null handle or open state");
+  if (h->d.selffield_bc != TDGL_BC_OPEN) return fail(TDGL_E_ARG, "the stepper has no open boundary");
+  const tdgl_desc &d = h->d;
+  const int sh[3][3] = {{d.Nx + 1, d.Ny, d.Nz}, {d.Nx, d.Ny + 1, d.Nz}, {d.Nx, d.Ny, d.Nz + 1}};
+  for (int c = 0; c < 3; ++c) {
+    if (!o->hold[c] || !o->hnew[c]) return fail(TDGL_E_ARG, "null open-boundary array");
+    CHK(get(h, o->hold[c], h->ohold[c], sh[c][0], sh[c][1], sh[c][2], 8));
+    CHK(get(h, o->hnew[c], h->ohnew[c], sh[c][0], sh[c][1], sh[c][2], 8));
+  }
+  int fl[4];
+  HIPCHK(hipMemcpy(fl, h->ofl, sizeof fl, hipMemcpyDeviceToHost));
+  o->count = fl[OF_COUNT];
+  return 0;
+}
+
+int tdgl_set_open(tdgl_handle *h, const tdgl_open *o) {
+  if (!h || !o) return fail(TDGL_E_ARG, "null handle or open state");
+  if (h->d.selffield_bc != TDGL_BC_OPEN) return fail(TDGL_E_ARG, "the stepper has no open boundary");
+  const tdgl_desc &d = h->d;
+  const int sh[3][3] = {{d.Nx + 1, d.Ny, d.Nz}, {d.Nx, d.Ny + 1, d.Nz}, {d.Nx, d.Ny, d.Nz + 1}};
+  for (int c = 0; c < 3; ++c) {
+    if (!o->hold[c] || !o->hnew[c]) return fail(TDGL_E_ARG, "null open-boundary array");
+    CHK(put(h, h->ohold[c], o->hold[c], sh[c][0], sh[c][1], sh[c][2], 8));
+    CHK(put(h, h->ohnew[c], o->hnew[c], sh[c][0], sh[c][1], sh[c][2], 8));
+  }
+  const int fl[4] = {o->count, 0, 0, 0};
+  HIPCHK(hipMemcpy(h->ofl, fl, sizeof fl, hipMemcpyHostToDevice));
+  return 0;
+}
+
 int tdgl_get_stats(tdgl_handle *h, tdgl_stats *st) {
   if (!h || !st) return fail(TDGL_E_ARG, "null argument");
   st->graph_cycles = h->graph_cycles;

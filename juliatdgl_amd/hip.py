@@ -9,7 +9,7 @@ import threading
 
 import numpy as np
 
-from .abi import NKERNELS, PackedDesc, PcgState, ProfileC, StatsC, state_struct
+from .abi import NKERNELS, OpenC, PackedDesc, PcgState, ProfileC, StatsC, state_struct
 from .scheme import Stepper
 from .state import State
 
@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_lib", "libtdglhip.so")
 
 SYMBOLS = ("tdgl_supported", "tdgl_create", "tdgl_destroy", "tdgl_upload_state", "tdgl_download_state", "tdgl_step",
-           "tdgl_get_pcg", "tdgl_set_pcg", "tdgl_get_stats", "tdgl_last_error", "tdgl_device_count",
+           "tdgl_get_pcg", "tdgl_set_pcg", "tdgl_get_open", "tdgl_set_open", "tdgl_get_stats", "tdgl_last_error", "tdgl_device_count",
            "tdgl_set_device", "tdgl_profile_enable", "tdgl_profile_get", "tdgl_kernel_name")
 
 _lib = None
@@ -122,6 +122,33 @@ class HipStepper:
 
     def set_pcg(self, p: PcgState) -> None:
         _check(load().tdgl_set_pcg(self.h, ct.byref(p.struct())), "tdgl_set_pcg")
+
+    def _open_shapes(self):
+        gr = self.st.grid
+        return ((gr.Nx + 1, gr.Ny, gr.Nz), (gr.Nx, gr.Ny + 1, gr.Nz), (gr.Nx, gr.Ny, gr.Nz + 1))
+
+    def get_open(self):
+        """(count, old, new) of the open boundary: its clock and h_δ of the last two
+        recomputations (h-shaped hx, hy, hz each)."""
+        sh = self._open_shapes()
+        old = [np.zeros(x, order="F") for x in sh]
+        new = [np.zeros(x, order="F") for x in sh]
+        c = OpenC()
+        for k in range(3):
+            c.hold[k] = old[k].ctypes.data_as(ct.POINTER(ct.c_double))
+            c.hnew[k] = new[k].ctypes.data_as(ct.POINTER(ct.c_double))
+        _check(load().tdgl_get_open(self.h, ct.byref(c)), "tdgl_get_open")
+        return int(c.count), old, new
+
+    def set_open(self, count: int, old, new) -> None:
+        old = [np.asfortranarray(x, dtype=np.float64) for x in old]
+        new = [np.asfortranarray(x, dtype=np.float64) for x in new]
+        c = OpenC()
+        c.count = int(count)
+        for k in range(3):
+            c.hold[k] = old[k].ctypes.data_as(ct.POINTER(ct.c_double))
+            c.hnew[k] = new[k].ctypes.data_as(ct.POINTER(ct.c_double))
+        _check(load().tdgl_set_open(self.h, ct.byref(c)), "tdgl_set_open")
 
     def profile(self, on: bool) -> None:
         """Per-kernel HIP-event timing of every launch while on (clears the totals)."""

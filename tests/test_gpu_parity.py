@@ -139,3 +139,42 @@ def test_hip_single_axis_transforms_match_oracle(name, monkeypatch):
     core = {k: v for k, v in errs.items() if not k.startswith(("U", "V"))}
     print(f"    worst {max(core, key=core.get)} {max(core.values()):.3e}", flush=True)
     assert max(core.values()) <= tol, errs
+
+
+def test_hip_open_boundary_resumes_and_is_odd_in_the_current():
+    """A :open run resumed from its checkpoint state (the clock and h_δ mid-ramp, tdgl_get_open /
+    tdgl_set_open) continues it exactly (test_openbc.jl:59-76); at H = 0 the two polarities are
+    the conjugate states (:45-55)."""
+    HipStepper = _hip()
+    from juliatdgl_amd.state import initial_state
+    st, _ = make_case("open")
+    H, HJ = 0.03, st.selffield(0.3)
+    s0 = initial_state(st.model, st.grid, st.masks, H)
+    n = 3 * st.open.every + 7
+    print(f"[gpu] open resume, every={st.open.every}, n={n}", flush=True)
+    a = HipStepper(st)
+    a.upload(s0)
+    a.step(2 * n, H, HJ)
+    sa = a.download()
+    b = HipStepper(st)
+    b.upload(s0)
+    b.step(n, H, HJ)
+    mid, cnt = b.download(), b.get_open()
+    c = HipStepper(st)
+    c.upload(mid)
+    c.set_open(*cnt)
+    c.step(n, H, HJ)
+    sc = c.download()
+    for mu in range(st.N):
+        assert np.array_equal(sa.G[mu], sc.G[mu])
+    assert np.array_equal(sa.hz, sc.hz) and np.array_equal(sa.thx, sc.thx)
+    out = []
+    for J in (0.3, -0.3):
+        d = HipStepper(st)
+        d.upload(initial_state(st.model, st.grid, st.masks, 0.0))
+        d.step(60, 0.0, st.selffield(J))
+        out.append(d.download())
+    p, m = out
+    for mu in range(st.N):
+        assert np.max(np.abs(p.G[mu] - np.conj(m.G[mu]))) < 1e-13
+    assert np.max(np.abs(p.hz + m.hz)) < 1e-13 and np.max(np.abs(p.hz)) > 1e-4
