@@ -125,7 +125,9 @@ def main(argv=None):
                     "H=0.2, J=0.1, dt_scale=0.1, ladder cycle = 40 steps")
     elif args.workload == "cfg3":
         st, p = configs.cfg3_like()
-        workload = "three-band anisotropic TDGL, 256x128x32 cells, buf 8 insulating, PCG tol 1e-8, H=0.2, J=0.1"
+        workload = ("three-band anisotropic (m_z = 4 m_xy) TDGL, 256x128x32 cells, dx=0.5, buf 8 insulating "
+                    "(sigma_vac/sigma_n=0.01), PCG tol 1e-8 (Float32 preconditioner, FP64 outer), open boundary "
+                    "(FFT convolution, every t_GL), notch, H=0.2, J=0.1, ladder cycle = 40 steps")
     else:
         raise SystemExit(f"unknown workload {args.workload}")
     if args.copy_probe:

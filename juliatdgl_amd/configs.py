@@ -51,10 +51,11 @@ def cfg1(nx=128, ny=64, nz=16, pcg_tol=1e-10, J=0.1, dt_scale=0.1):
     return st, dict(H=0.2, J=J)
 
 
-def cfg3_like(nx=256, ny=128, nz=32, buf=8, pcg_tol=1e-8):
+def cfg3_like(nx=256, ny=128, nz=32, buf=8, pcg_tol=1e-8, selffield_bc="open"):
     """Three bands (a = 1, 0.6, 0.3; ξ = 1, 1.8, 3; Josephson −0.1, −0.05, −0.02) with
     m_z = 4 m_xy on 256x128x32 cells and an insulating buffer of 8 cells (σvac = 0.01, PCG with
-    the Float32 preconditioner), bar field (the open boundary is not run yet)."""
+    the Float32 preconditioner and the FP64 outer iteration), the open boundary (h_δ by the
+    zero-padded FFT convolution, recomputed every t_GL)."""
     bands = []
     for a, xi in ((1.0, 1.0), (0.6, 1.8), (0.3, 3.0)):
         m = 1 / (xi * xi * a)
@@ -62,5 +63,5 @@ def cfg3_like(nx=256, ny=128, nz=32, buf=8, pcg_tol=1e-8):
     md = Model.make(tuple(bands), [[0, -0.1, -0.05], [-0.1, 0, -0.02], [-0.05, -0.02, 0]], kappa=2.5)
     gr = Grid.make(nx, ny, nz, dx=0.5, buf=buf)
     mk = notch_masks(gr, (nx // 2 - 4, nx // 2 + 4), (0, ny // 4))
-    st = make_stepper(md, gr, mk, sigma_vac=0.01, pcg_tol=pcg_tol, selffield_bc="bar")
+    st = make_stepper(md, gr, mk, sigma_vac=0.01, pcg_tol=pcg_tol, selffield_bc=selffield_bc)
     return st, dict(H=0.2, J=0.1)
