@@ -357,6 +357,11 @@ __global__ void __launch_bounds__(256) k_curl(Geo g, CurlArgs a) {
 }
 
 hipError_t launch_curl(const Geo &g, const CurlArgs &a, hipStream_t s) {
+  // every table the boundary reads must be there: a missing one is an error, not a device fault
+  if (a.bar && !(a.bar_hy_z && a.bar_hy_y && a.bar_hz_z && a.bar_hz_y && a.bar_hz_x && a.bar_hy_x))
+    return hipErrorInvalidValue;
+  if (a.open && !(a.ofl && a.hold[0] && a.hold[1] && a.hold[2] && a.hnew[0] && a.hnew[1] && a.hnew[2]))
+    return hipErrorInvalidValue;
   dim3 b(64, 4, 1), gr(cdiv(g.Nx + 1, 64), cdiv(g.Ny + 1, 4), g.Nz + 1);
   hipLaunchKernelGGL(k_curl, gr, b, 0, s, g, a);
   return hipGetLastError();

@@ -1341,7 +1341,8 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
         CHK(put(h, h->D[c], Ds[c], sh[c][0], sh[c][1], sh[c][2], 8));
       }
     }
-    if (d.selffield_bc == TDGL_BC_BAR) {
+    // the open boundary adds h_δ to the bar self-field of the transport current (openbc.jl:5-8, field.jl:221)
+    if (d.selffield_bc == TDGL_BC_BAR || d.selffield_bc == TDGL_BC_OPEN) {
       const size_t nn[6] = {(size_t)(d.Ny + 1) * 2, (size_t)2 * d.Nz, (size_t)d.Ny * 2, (size_t)2 * (d.Nz + 1),
                             (size_t)d.Ny * (d.Nz + 1), (size_t)(d.Ny + 1) * d.Nz};
       const double *src[6] = {d.bar_hy_z, d.bar_hy_y, d.bar_hz_z, d.bar_hz_y, d.bar_hz_x, d.bar_hy_x};
