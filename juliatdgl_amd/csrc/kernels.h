@@ -186,6 +186,7 @@ constexpr int PLANE_LMAX = 256;
 constexpr int PLANE_MAXC = 4;  // components (the three of a link vector, or the modes of a potential)
 struct PlanePass {
   int ncomp;                   // 0 means 3
+  int one;                     // Float32: the one-round kernel where it pays (1), forced with 2 or 3 m tiles, off (0)
   View in[PLANE_MAXC], out[PLANE_MAXC];
   const void *X[PLANE_MAXC], *Y[PLANE_MAXC];
   int e[PLANE_MAXC][3];

@@ -1176,6 +1176,250 @@ __global__ void __launch_bounds__(256) k_plane_mf(PlanePass p, const int *gate) 
   }
 }
 
+// Planes of at most 32 XC x 32 TJ elements in one memory round.  A block holds MT 16-row tiles of
+// m of one plane: the whole input plane, its MS columns of X and all of Y are requested before
+// anything waits (after the gate flag; the convergence sum of a first pass is taken while they
+// are in flight), staged as Float32 in LDS k-contiguous, so that one 16-byte read gives a lane
+// its operands of four v_mfma_f32_16x16x4f32 steps (group g, step u, lane k index lk: row
+// 16 g + 4 lk + u; the four products of one instruction are rounded once, as in k_plane_mf, but
+// grouped differently).  LDS rows are 4 (mod 8) words long: the 16-byte reads of 16 lanes are
+// bank-conflict free.  The next group's operands are read while the current group multiplies.
+// Blocks go to the XCDs in contiguous runs of the (m tile, plane, component) order, so one
+// plane's blocks share an L2.
+template <int XC, int TJ, int MT> struct PlaneMf1 {
+  static constexpr int E0 = 32 * XC, L1P = 32 * TJ, MS = 16 * MT;
+  static constexpr int E0S = E0 + 4, JS = L1P + 4;
+  static constexpr int NJ = 2 * TJ, NT = MT * NJ, S = (NT + 3) / 4;  // j tiles, tiles, tiles per wave
+  static constexpr int NB = E0 * L1P / 256, NX = E0 * MS / 256, NY = L1P * L1P / 256;  // loads per thread
+  static constexpr int BROW = E0S > JS ? E0S : JS;
+  // Xt [MS][E0S] | Bt [L1P][E0S], later Yt [L1P][JS] | St [MS][JS]
+  static constexpr int BOFF = MS * E0S, SOFF = BOFF + L1P * BROW;
+  static constexpr size_t lds = (size_t)(SOFF + MS * JS) * sizeof(float);
+};
+constexpr int PLANE1_XC = 5, PLANE1_TJ = 3;  // planes up to 160 x 96
+
+// the hardware block index to the block's place in (m tile, plane, component) order: the
+// hardware deals blocks to the 8 XCDs in turn, XCD x receives a contiguous run of that order
+__device__ __forceinline__ int xcd_run(int hw, int nb) {
+  const int q = nb / 8, r = nb % 8, x = hw % 8, s = hw / 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + s;
+}
+
+template <class TI, class TO, int XC, int TJ, int MT>
+__global__ void __launch_bounds__(256) k_plane_mf1(PlanePass p, const int *gate, int gx, int gy) {
+  using C = PlaneMf1<XC, TJ, MT>;
+  constexpr int MS = C::MS, E0 = C::E0, L1P = C::L1P, E0S = C::E0S, JS = C::JS, S = C::S, NT = C::NT;
+  extern __shared__ __align__(16) unsigned char plane_smem[];
+  __shared__ double sh[8];
+  const int tid = threadIdx.x;
+  const int nb = gridDim.x, bid = xcd_run(blockIdx.x, nb);
+  const int mb = bid % gx, plane = (bid / gx) % gy, comp = bid / (gx * gy), m0 = mb * MS;
+  // the gate flag first: a pass the gate turns off loads nothing
+  if (p.conv.mode != CONV_NONE) {
+    if (!*p.conv.in) {
+      if (bid == 0 && tid == 0) *p.conv.out = 0;
+      return;
+    }
+  } else if (gate && !*gate) {
+    return;
+  }
+  const int e0 = p.e[comp][0], e1 = p.e[comp][1], e2 = p.e[comp][2];
+  const int pl = min(plane, e2 - 1);  // blocks beyond this component's planes load from its last
+  const TI *in = (const TI *)p.in[comp].p + (size_t)p.in[comp].s2 * pl;
+  const int s1 = p.in[comp].s1;
+  const float *X = (const float *)p.X[comp], *Y = (const float *)p.Y[comp];
+  TI rb[C::NB];
+  float rx[C::NX], ry[C::NY];
+#pragma unroll
+  for (int r = 0; r < C::NB; ++r) {
+    const int t = tid + 256 * r, ii = t % E0, j = t / E0;
+    rb[r] = in[min(ii, e0 - 1) + s1 * min(j, e1 - 1)];
+  }
+#pragma unroll
+  for (int r = 0; r < C::NX; ++r) {
+    const int t = tid + 256 * r, ii = t / MS, mm = t % MS;
+    rx[r] = X[min(ii, e0 - 1) * e0 + min(m0 + mm, e0 - 1)];
+  }
+#pragma unroll
+  for (int r = 0; r < C::NY; ++r) {
+    const int t = tid + 256 * r, jj = t / L1P, q = t % L1P;
+    ry[r] = Y[min(jj, e1 - 1) * e1 + min(q, e1 - 1)];
+  }
+  if (p.conv.mode != CONV_NONE) {
+    const double rr = sum_parts(region(p.d, PART_RR), p.conv.nparts, 0, sh, tid);
+    const double tol = p.d.sc[SC_TOL], nrm = p.d.sc[SC_NB];
+    const int active = p.conv.mode == CONV_RR0 ? ((nrm > 0 && sqrt(rr) > tol * nrm) ? 1 : 0)
+                                               : ((sqrt(rr) <= tol * nrm || p.d.fl[FL_IT] >= p.d.fl[FL_MAXIT]) ? 0 : 1);
+    if (bid == 0 && tid == 0) *p.conv.out = active;
+    if (!active) return;
+  }
+  double d0 = 0.0, d1 = 0.0;
+  if (plane < e2 && m0 < e0) {
+    float *Xt = (float *)plane_smem, *Bt = Xt + C::BOFF, *St = Xt + C::SOFF;
+#pragma unroll
+    for (int r = 0; r < C::NX; ++r) {
+      const int t = tid + 256 * r, ii = t / MS, mm = t % MS;
+      Xt[mm * E0S + ii] = (ii < e0 && m0 + mm < e0) ? rx[r] : 0.0f;
+    }
+#pragma unroll
+    for (int r = 0; r < C::NB; ++r) {
+      const int t = tid + 256 * r, ii = t % E0, j = t / E0;
+      Bt[j * E0S + ii] = (ii < e0 && j < e1) ? (float)rb[r] : 0.0f;
+    }
+    __syncthreads();
+    const int w = tid / 64, l = tid % 64, lr = l % 16, lk = l / 16;
+    // this wave's tiles: t = w + 4 s, (mt, jt) = (t mod MT, t / MT); where the last is absent the
+    // wave multiplies a copy of tile NT - 1 (no longer than the waves that have it) and drops it
+    int ao[S], bo[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int t = min(w + 4 * s, NT - 1);
+      ao[s] = ((t % MT) * 16 + lr) * E0S + 4 * lk;
+      bo[s] = ((t / MT) * 16 + lr) * E0S + 4 * lk;
+    }
+    const bool last = w + 4 * (S - 1) < NT;
+    v4f acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    // phase 1: S(m, j) = Σ_i X(i, m) in(i, j), groups of 16 rows of i
+    const int G1 = (e0 + 15) / 16;
+    {
+      v4f A[S], B[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        A[s] = *(const v4f *)(Xt + ao[s]);
+        B[s] = *(const v4f *)(Bt + bo[s]);
+      }
+      for (int g = 0; g < G1; ++g) {
+        const int gn = min(g + 1, G1 - 1) * 16;
+        v4f An[S], Bn[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          An[s] = *(const v4f *)(Xt + ao[s] + gn);
+          Bn[s] = *(const v4f *)(Bt + bo[s] + gn);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int s = 0; s < S; ++s) acc[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[s][u], B[s][u], acc[s], 0, 0, 0);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          A[s] = An[s];
+          B[s] = Bn[s];
+        }
+      }
+    }
+    __syncthreads();  // every wave is done with the input plane
+    // S to LDS (St [m][j]), Y to LDS (Yt [q][j], over the input plane)
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int t = min(w + 4 * s, NT - 1), mt = t % MT, jt = t / MT;
+      if (s < S - 1 || last)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) St[(mt * 16 + 4 * lk + r) * JS + jt * 16 + lr] = acc[s][r];
+      acc[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    float *Yt = Bt;
+#pragma unroll
+    for (int r = 0; r < C::NY; ++r) {
+      const int t = tid + 256 * r, jj = t / L1P, q = t % L1P;
+      Yt[q * JS + jj] = (jj < e1 && q < e1) ? ry[r] : 0.0f;
+    }
+    // the epilogue's operands, in flight during phase 2 (clamped addresses)
+    double ev[S][4], zv[S][4];
+    const int ec = p.addth ? 1 : (p.dot ? 2 : 0);
+    const View ve = ec == 1 ? p.th[comp < 3 ? comp : 2] : p.r[comp < 3 ? comp : 2];
+    const View vz = p.zold[comp < 3 ? comp : 2];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ev[s][r] = zv[s][r] = 0.0;
+    if (ec) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int t = min(w + 4 * s, NT - 1), mt = t % MT, qt = t / MT;
+        const int qc = min(qt * 16 + lr, e1 - 1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          ev[s][r] = ((const double *)ve.p)[voff(ve, min(m0 + mt * 16 + 4 * lk + r, e0 - 1), qc, plane)];
+      }
+      if (p.dot == 2)
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const int t = min(w + 4 * s, NT - 1), mt = t % MT, qt = t / MT;
+          const int qc = min(qt * 16 + lr, e1 - 1);
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            zv[s][r] = ((const double *)vz.p)[voff(vz, min(m0 + mt * 16 + 4 * lk + r, e0 - 1), qc, plane)];
+        }
+    }
+    __syncthreads();
+    // phase 2: out(m, q) = Σ_j S(m, j) Y(j, q), groups of 16 j
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int t = min(w + 4 * s, NT - 1);
+      ao[s] = ((t % MT) * 16 + lr) * JS + 4 * lk;
+      bo[s] = ((t / MT) * 16 + lr) * JS + 4 * lk;
+    }
+    const int G2 = (e1 + 15) / 16;
+    {
+      v4f A[S], B[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        A[s] = *(const v4f *)(St + ao[s]);
+        B[s] = *(const v4f *)(Yt + bo[s]);
+      }
+      for (int g = 0; g < G2; ++g) {
+        const int gn = min(g + 1, G2 - 1) * 16;
+        v4f An[S], Bn[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          An[s] = *(const v4f *)(St + ao[s] + gn);
+          Bn[s] = *(const v4f *)(Yt + bo[s] + gn);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int s = 0; s < S; ++s) acc[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[s][u], B[s][u], acc[s], 0, 0, 0);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          A[s] = An[s];
+          B[s] = Bn[s];
+        }
+      }
+    }
+    const View vo = p.out[comp];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (!(s < S - 1 || last)) continue;
+      const int t = w + 4 * s, mt = t % MT, qt = t / MT;
+      const int q = qt * 16 + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + mt * 16 + 4 * lk + r;
+        if (m >= e0 || q >= e1) continue;
+        const int qo = voff(vo, m, q, plane);
+        if (ec == 1) {
+          ((double *)vo.p)[qo] = ev[s][r] + (double)acc[s][r];
+        } else {
+          const TO v = (TO)acc[s][r];
+          ((TO *)vo.p)[qo] = v;
+          d0 += ev[s][r] * (double)v;  // zero unless dot
+          d1 += ev[s][r] * zv[s][r];
+        }
+      }
+    }
+  }
+  if (p.dot) {
+    block_sum2_t(d0, d1, sh, tid);
+    if (tid == 0) {
+      double *pr = region_w(p.d, PART_RZ);
+      pr[2 * bid] = d0;
+      pr[2 * bid + 1] = d1;
+    }
+  }
+}
+
 static size_t plane_mf_lds(int l1) {
   const int l1p = (l1 + 31) / 32 * 32;
   const int ic = l1p <= 128 ? 64 : 32;  // PlaneMfCfg
@@ -1193,8 +1437,38 @@ static dim3 plane_grid(const PlanePass &p) {
   }
   return dim3(gx, gy, nc);
 }
+// The one-round Float32 kernel (p.one != 0): planes up to 32 PLANE1_XC x 32 PLANE1_TJ with x
+// extents above 64 and y extents above 32 (smaller planes keep k_plane_mf).  Blocks of 2 m tiles,
+// or of 3 where 2 would need more blocks than the chip has CUs and 3 needs fewer (p.one = 1), or
+// p.one m tiles (2 or 3).
+struct Plane1Pick {
+  int use, xc, tj, mt, gx, gy, nc;
+};
+static Plane1Pick plane1_pick(const PlanePass &p) {
+  Plane1Pick k{};
+  k.nc = plane_ncomp(p);
+  int l0 = 0, l1 = 0;
+  k.gy = 1;
+  for (int c = 0; c < k.nc; ++c) {
+    l0 = p.e[c][0] > l0 ? p.e[c][0] : l0;
+    l1 = p.e[c][1] > l1 ? p.e[c][1] : l1;
+    k.gy = p.e[c][2] > k.gy ? p.e[c][2] : k.gy;
+  }
+  k.use = p.one != 0 && l0 > 64 && l0 <= 32 * PLANE1_XC && l1 > 32 && l1 <= 32 * PLANE1_TJ;
+  if (!k.use) return k;
+  k.xc = cdiv(l0, 32);
+  k.tj = cdiv(l1, 32);
+  const int n2 = cdiv(l0, 32) * k.gy * k.nc, n3 = cdiv(l0, 48) * k.gy * k.nc;
+  k.mt = p.one >= 2 ? p.one : ((n2 > 256 && n3 < n2) ? 3 : 2);
+  k.gx = cdiv(l0, 16 * k.mt);
+  return k;
+}
 template <class T>
 int plane_parts(const PlanePass &p) {
+  if (sizeof(T) == 4) {
+    const Plane1Pick k = plane1_pick(p);
+    if (k.use) return k.gx * k.gy * k.nc;
+  }
   const dim3 g = plane_grid<T>(p);
   return (int)(g.x * g.y * g.z);
 }
@@ -1216,6 +1490,25 @@ hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
   for (int c = 0; c < nc; ++c) l1 = p.e[c][1] > l1 ? p.e[c][1] : l1;
   const dim3 gr = plane_grid<T>(p);
   const bool mf = sizeof(T) == 4;  // Float32 on the matrix cores
+  if constexpr (sizeof(T) == 4) {
+    const Plane1Pick k = plane1_pick(p);
+    if (k.use) {
+      const dim3 g1(k.gx * k.gy * k.nc);
+      switch ((k.xc * 4 + k.tj) * 4 + k.mt) {
+#define TDGL_PLANE1_CASE(x, t, m)                                                                                 \
+  case (x * 4 + t) * 4 + m:                                                                                       \
+    hipLaunchKernelGGL((k_plane_mf1<TI, TO, x, t, m>), g1, dim3(256), (PlaneMf1<x, t, m>::lds), s, p, gate, k.gx, \
+                       k.gy);                                                                                     \
+    break;
+        TDGL_PLANE1_CASE(3, 2, 2) TDGL_PLANE1_CASE(3, 3, 2) TDGL_PLANE1_CASE(3, 2, 3) TDGL_PLANE1_CASE(3, 3, 3)
+        TDGL_PLANE1_CASE(4, 2, 2) TDGL_PLANE1_CASE(4, 3, 2) TDGL_PLANE1_CASE(4, 2, 3) TDGL_PLANE1_CASE(4, 3, 3)
+        TDGL_PLANE1_CASE(5, 2, 2) TDGL_PLANE1_CASE(5, 3, 2) TDGL_PLANE1_CASE(5, 2, 3) TDGL_PLANE1_CASE(5, 3, 3)
+#undef TDGL_PLANE1_CASE
+        default: return hipErrorInvalidValue;
+      }
+      return hipGetLastError();
+    }
+  }
   const size_t lds = mf ? plane_mf_lds(l1) : plane_lds<T>(l1);
   switch ((l1 + 31) / 32) {
 #define TDGL_PLANE_CASE(tj)                                                                     \

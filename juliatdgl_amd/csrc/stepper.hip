@@ -102,6 +102,7 @@ struct tdgl_handle {
   double *T1 = nullptr, *T2 = nullptr;
   float *T32a = nullptr, *T32b = nullptr;
   bool plane32 = false, plane64 = false;  // the fused transform launches fit (per precision)
+  int plane1 = 1;                         // PlanePass::one of the Float32 launches
   // normal fluids: the sources / solved modes (region-shaped, one per mode), work, the bases
   double *qS[TDGL_MAX_BANDS] = {};
   double *qT1 = nullptr, *qT2 = nullptr;
@@ -410,6 +411,7 @@ TxPass tx_pass(tdgl_handle *h, int axis, bool inv, bool f32) {
 PlanePass plane_pass(tdgl_handle *h, bool inv, bool f32) {
   PlanePass p{};
   p.d = h->pd;
+  p.one = h->plane1;
   for (int c = 0; c < 3; ++c) {
     for (int q = 0; q < 3; ++q) p.e[c][q] = h->rg.e[c][q];
     const int wx = c == 0 ? 1 : 0, wy = c == 1 ? 1 : 0;
@@ -1493,6 +1495,10 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     if (gb && d.field_mode == TDGL_FIELD_PCG) h->budget = std::max(1, std::min(d.pcg_maxit, std::atoi(gb)));
     // TDGL_HIP_PLANE=0: the separable solve as six single-axis passes even where the fused
     // x-y and z-mode launches fit
+    // TDGL_HIP_PLANE1=0: the chunked Float32 x-y kernel also where the one-round kernel fits;
+    // =2 or =3: the one-round kernel with that many m tiles per block wherever it fits
+    const char *g1 = std::getenv("TDGL_HIP_PLANE1");
+    if (g1 && (g1[0] == '0' || g1[0] == '2' || g1[0] == '3')) h->plane1 = g1[0] - '0';
     const char *gp = std::getenv("TDGL_HIP_PLANE");
     const bool zfits = d.Nz - 1 >= 2 && d.Nz - 1 <= 256;
     if (!(gp && gp[0] == '0') && zfits) {
