@@ -189,6 +189,7 @@ struct PlanePass {
   int one;                     // Float32: the one-round kernel where it pays (1), forced with 2 or 3 m tiles, off (0)
   View in[PLANE_MAXC], out[PLANE_MAXC];
   const void *X[PLANE_MAXC], *Y[PLANE_MAXC];
+  const void *XT[PLANE_MAXC], *YT[PLANE_MAXC];  // their transposes (the one-round kernel)
   int e[PLANE_MAXC][3];
   int addth;
   View th[3];

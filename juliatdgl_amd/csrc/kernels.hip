@@ -1176,25 +1176,28 @@ __global__ void __launch_bounds__(256) k_plane_mf(PlanePass p, const int *gate) 
   }
 }
 
-// Planes of at most 32 XC x 32 TJ elements in one memory round.  A block holds MT 16-row tiles of
-// m of one plane: the whole input plane, its MS columns of X and all of Y are requested before
-// anything waits (after the gate flag; the convergence sum of a first pass is taken while they
-// are in flight), staged as Float32 in LDS k-contiguous, so that one 16-byte read gives a lane
+// Planes of at most 32 XC x 32 TJ elements in one memory round.  A block of eight wavefronts
+// holds MT 16-row tiles of m of one plane: the whole input plane and its MS columns of X are
+// requested before anything waits (after the gate flag; the convergence sum of a first pass is
+// taken while they are in flight), at most 63 loads per thread (what one wavefront can have in
+// flight); Y is requested next and arrives during the first product.  Both are staged as Float32 in LDS k-contiguous, so that one 16-byte read gives a lane
 // its operands of four v_mfma_f32_16x16x4f32 steps (group g, step u, lane k index lk: row
 // 16 g + 4 lk + u; the four products of one instruction are rounded once, as in k_plane_mf, but
 // grouped differently).  LDS rows are 4 (mod 8) words long: the 16-byte reads of 16 lanes are
 // bank-conflict free.  The next group's operands are read while the current group multiplies.
 // Blocks go to the XCDs in contiguous runs of the (m tile, plane, component) order, so one
 // plane's blocks share an L2.
+constexpr int PLANE1_T = 512;
 template <int XC, int TJ, int MT> struct PlaneMf1 {
   static constexpr int E0 = 32 * XC, L1P = 32 * TJ, MS = 16 * MT;
   static constexpr int E0S = E0 + 4, JS = L1P + 4;
-  static constexpr int NJ = 2 * TJ, NT = MT * NJ, S = (NT + 3) / 4;  // j tiles, tiles, tiles per wave
-  static constexpr int NB = E0 * L1P / 256, NX = E0 * MS / 256, NY = L1P * L1P / 256;  // loads per thread
+  static constexpr int NJ = 2 * TJ, NT = MT * NJ, S = (NT + 7) / 8;  // j tiles, tiles, tiles per wave
+  static constexpr int NB = E0 * L1P / PLANE1_T, NX = E0 * MS / PLANE1_T, NY = L1P * L1P / PLANE1_T;
   static constexpr int BROW = E0S > JS ? E0S : JS;
   // Xt [MS][E0S] | Bt [L1P][E0S], later Yt [L1P][JS] | St [MS][JS]
   static constexpr int BOFF = MS * E0S, SOFF = BOFF + L1P * BROW;
   static constexpr size_t lds = (size_t)(SOFF + MS * JS) * sizeof(float);
+  static_assert(NB + NX <= 63 && NY <= 63, "staging loads per thread");
 };
 constexpr int PLANE1_XC = 5, PLANE1_TJ = 3;  // planes up to 160 x 96
 
@@ -1205,12 +1208,107 @@ __device__ __forceinline__ int xcd_run(int hw, int nb) {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + s;
 }
 
+// Raw buffer loads: the base in scalar registers and a 32-bit byte offset per lane (one address
+// register per load instead of two); the hardware returns 0 beyond `bytes`.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, bytes, 0x00020000);
+}
+template <class T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int off);
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
+// sum_parts in a block of eight wavefronts: the first four load and sum as a block of RED_T
+// threads does (the same value, bit for bit, as every other reader of the partials)
+__device__ __forceinline__ double sum_parts_8w(const double *part, int n, int which, double *sh, int tid) {
+  double s = 0.0;
+  if (tid < RED_T) {
+    const __amdgpu_buffer_rsrc_t rp = buf_rsrc(part, 16 * n);
+    for (int b0 = 0; b0 < n; b0 += PART_UNROLL * RED_T) {
+      double v[PART_UNROLL];
+#pragma unroll
+      for (int u = 0; u < PART_UNROLL; ++u) v[u] = buf_load<double>(rp, 8 * (2 * min(b0 + tid + u * RED_T, n - 1) + which));
+#pragma unroll
+      for (int u = 0; u < PART_UNROLL; ++u) s += (b0 + tid + u * RED_T < n) ? v[u] : 0.0;
+    }
+  }
+  s = wave_sum(s);
+  if ((tid & 63) == 0) sh[tid >> 6] = s;
+  __syncthreads();
+  const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return r;
+}
+// both sums of eight wavefronts in a fixed order
+__device__ __forceinline__ void block_sum2_8w(double &a, double &b, double *sh, int tid) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((tid & 63) == 0) {
+    sh[tid >> 6] = a;
+    sh[8 + (tid >> 6)] = b;
+  }
+  __syncthreads();
+  a = ((sh[0] + sh[1]) + (sh[2] + sh[3])) + ((sh[4] + sh[5]) + (sh[6] + sh[7]));
+  b = ((sh[8] + sh[9]) + (sh[10] + sh[11])) + ((sh[12] + sh[13]) + (sh[14] + sh[15]));
+  __syncthreads();
+}
+
+// acc[s] += A_s' B_s over G groups of 16 rows for the first NS tiles of a wave (operands in LDS
+// at Ab + ao[s], Bb + bo[s], 16 rows further per group); the next group's operands are read
+// while the current group multiplies
+template <int NS, int S>
+__device__ __forceinline__ void mf_groups(v4f (&acc)[S], const float *Ab, const int (&ao)[S], const float *Bb,
+                                          const int (&bo)[S], int G) {
+  v4f A[NS], B[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    A[s] = *(const v4f *)(Ab + ao[s]);
+    B[s] = *(const v4f *)(Bb + bo[s]);
+  }
+  for (int g = 0; g < G; ++g) {
+    const int gn = min(g + 1, G - 1) * 16;
+    v4f An[NS], Bn[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      An[s] = *(const v4f *)(Ab + ao[s] + gn);
+      Bn[s] = *(const v4f *)(Bb + bo[s] + gn);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int s = 0; s < NS; ++s) acc[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[s][u], B[s][u], acc[s], 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      A[s] = An[s];
+      B[s] = Bn[s];
+    }
+  }
+}
+
+// mf_groups for the ns tiles of a wave (0 <= ns <= S, at least S - 2; wave-uniform)
+template <int S>
+__device__ __forceinline__ void mf_slots(int ns, v4f (&acc)[S], const float *Ab, const int (&ao)[S], const float *Bb,
+                                         const int (&bo)[S], int G) {
+  if (ns >= S) {
+    mf_groups<S, S>(acc, Ab, ao, Bb, bo, G);
+  } else if constexpr (S >= 2) {
+    if (ns == S - 1) {
+      mf_groups<S - 1, S>(acc, Ab, ao, Bb, bo, G);
+    } else if constexpr (S >= 3) {
+      if (ns == S - 2) mf_groups<S - 2, S>(acc, Ab, ao, Bb, bo, G);
+    }
+  }
+}
+
 template <class TI, class TO, int XC, int TJ, int MT>
-__global__ void __launch_bounds__(256) k_plane_mf1(PlanePass p, const int *gate, int gx, int gy) {
+__global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *gate, int gx, int gy) {
   using C = PlaneMf1<XC, TJ, MT>;
-  constexpr int MS = C::MS, E0 = C::E0, L1P = C::L1P, E0S = C::E0S, JS = C::JS, S = C::S, NT = C::NT;
+  constexpr int NTH = PLANE1_T, MS = C::MS, E0 = C::E0, L1P = C::L1P, E0S = C::E0S, JS = C::JS, S = C::S;
   extern __shared__ __align__(16) unsigned char plane_smem[];
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   const int tid = threadIdx.x;
   const int nb = gridDim.x, bid = xcd_run(blockIdx.x, nb);
   const int mb = bid % gx, plane = (bid / gx) % gy, comp = bid / (gx * gy), m0 = mb * MS;
@@ -1225,28 +1323,24 @@ __global__ void __launch_bounds__(256) k_plane_mf1(PlanePass p, const int *gate,
   }
   const int e0 = p.e[comp][0], e1 = p.e[comp][1], e2 = p.e[comp][2];
   const int pl = min(plane, e2 - 1);  // blocks beyond this component's planes load from its last
-  const TI *in = (const TI *)p.in[comp].p + (size_t)p.in[comp].s2 * pl;
   const int s1 = p.in[comp].s1;
-  const float *X = (const float *)p.X[comp], *Y = (const float *)p.Y[comp];
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc((const TI *)p.in[comp].p + (size_t)p.in[comp].s2 * pl,
+                                              (s1 * (e1 - 1) + e0) * (int)sizeof(TI));
+  const __amdgpu_buffer_rsrc_t rX = buf_rsrc(p.XT[comp], e0 * e0 * 4), rY = buf_rsrc(p.YT[comp], e1 * e1 * 4);
   TI rb[C::NB];
   float rx[C::NX], ry[C::NY];
 #pragma unroll
   for (int r = 0; r < C::NB; ++r) {
-    const int t = tid + 256 * r, ii = t % E0, j = t / E0;
-    rb[r] = in[min(ii, e0 - 1) + s1 * min(j, e1 - 1)];
+    const int t = tid + NTH * r, ii = t % E0, j = t / E0;
+    rb[r] = buf_load<TI>(rin, (min(ii, e0 - 1) + s1 * min(j, e1 - 1)) * (int)sizeof(TI));
   }
 #pragma unroll
   for (int r = 0; r < C::NX; ++r) {
-    const int t = tid + 256 * r, ii = t / MS, mm = t % MS;
-    rx[r] = X[min(ii, e0 - 1) * e0 + min(m0 + mm, e0 - 1)];
-  }
-#pragma unroll
-  for (int r = 0; r < C::NY; ++r) {
-    const int t = tid + 256 * r, jj = t / L1P, q = t % L1P;
-    ry[r] = Y[min(jj, e1 - 1) * e1 + min(q, e1 - 1)];
+    const int t = tid + NTH * r, ii = t % E0, mm = t / E0;
+    rx[r] = buf_load<float>(rX, (min(m0 + mm, e0 - 1) * e0 + min(ii, e0 - 1)) * 4);
   }
   if (p.conv.mode != CONV_NONE) {
-    const double rr = sum_parts(region(p.d, PART_RR), p.conv.nparts, 0, sh, tid);
+    const double rr = sum_parts_8w(region(p.d, PART_RR), p.conv.nparts, 0, sh, tid);
     const double tol = p.d.sc[SC_TOL], nrm = p.d.sc[SC_NB];
     const int active = p.conv.mode == CONV_RR0 ? ((nrm > 0 && sqrt(rr) > tol * nrm) ? 1 : 0)
                                                : ((sqrt(rr) <= tol * nrm || p.d.fl[FL_IT] >= p.d.fl[FL_MAXIT]) ? 0 : 1);
@@ -1258,63 +1352,45 @@ __global__ void __launch_bounds__(256) k_plane_mf1(PlanePass p, const int *gate,
     float *Xt = (float *)plane_smem, *Bt = Xt + C::BOFF, *St = Xt + C::SOFF;
 #pragma unroll
     for (int r = 0; r < C::NX; ++r) {
-      const int t = tid + 256 * r, ii = t / MS, mm = t % MS;
+      const int t = tid + NTH * r, ii = t % E0, mm = t / E0;
       Xt[mm * E0S + ii] = (ii < e0 && m0 + mm < e0) ? rx[r] : 0.0f;
     }
 #pragma unroll
     for (int r = 0; r < C::NB; ++r) {
-      const int t = tid + 256 * r, ii = t % E0, j = t / E0;
+      const int t = tid + NTH * r, ii = t % E0, j = t / E0;
       Bt[j * E0S + ii] = (ii < e0 && j < e1) ? (float)rb[r] : 0.0f;
     }
+    // Y, for phase 2: in flight during phase 1
+#pragma unroll
+    for (int r = 0; r < C::NY; ++r) {
+      const int t = tid + NTH * r, jj = t % L1P, q = t / L1P;
+      ry[r] = buf_load<float>(rY, (min(q, e1 - 1) * e1 + min(jj, e1 - 1)) * 4);
+    }
     __syncthreads();
-    const int w = tid / 64, l = tid % 64, lr = l % 16, lk = l / 16;
-    // this wave's tiles: t = w + 4 s, (mt, jt) = (t mod MT, t / MT); where the last is absent the
-    // wave multiplies a copy of tile NT - 1 (no longer than the waves that have it) and drops it
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid % 64, lr = l % 16, lk = l / 16;
+    // the tiles: MT m tiles by the 16-column j tiles the y extent needs; wave w takes t = w + 8 s,
+    // (mt, jt) = (t mod MT, t / MT), ns of them (wave-uniform)
+    const int NTr = MT * ((e1 + 15) / 16);
+    const int ns = w < NTr ? min(S, (NTr - w + 7) / 8) : 0;
     int ao[S], bo[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      const int t = min(w + 4 * s, NT - 1);
+      const int t = min(w + 8 * s, NTr - 1);
       ao[s] = ((t % MT) * 16 + lr) * E0S + 4 * lk;
       bo[s] = ((t / MT) * 16 + lr) * E0S + 4 * lk;
     }
-    const bool last = w + 4 * (S - 1) < NT;
     v4f acc[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) acc[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     // phase 1: S(m, j) = Σ_i X(i, m) in(i, j), groups of 16 rows of i
     const int G1 = (e0 + 15) / 16;
-    {
-      v4f A[S], B[S];
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        A[s] = *(const v4f *)(Xt + ao[s]);
-        B[s] = *(const v4f *)(Bt + bo[s]);
-      }
-      for (int g = 0; g < G1; ++g) {
-        const int gn = min(g + 1, G1 - 1) * 16;
-        v4f An[S], Bn[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-          An[s] = *(const v4f *)(Xt + ao[s] + gn);
-          Bn[s] = *(const v4f *)(Bt + bo[s] + gn);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int s = 0; s < S; ++s) acc[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[s][u], B[s][u], acc[s], 0, 0, 0);
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-          A[s] = An[s];
-          B[s] = Bn[s];
-        }
-      }
-    }
+    mf_slots<S>(ns, acc, Xt, ao, Bt, bo, G1);
     __syncthreads();  // every wave is done with the input plane
     // S to LDS (St [m][j]), Y to LDS (Yt [q][j], over the input plane)
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      const int t = min(w + 4 * s, NT - 1), mt = t % MT, jt = t / MT;
-      if (s < S - 1 || last)
+      const int t = min(w + 8 * s, NTr - 1), mt = t % MT, jt = t / MT;
+      if (s < ns)
 #pragma unroll
         for (int r = 0; r < 4; ++r) St[(mt * 16 + 4 * lk + r) * JS + jt * 16 + lr] = acc[s][r];
       acc[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
@@ -1322,7 +1398,7 @@ __global__ void __launch_bounds__(256) k_plane_mf1(PlanePass p, const int *gate,
     float *Yt = Bt;
 #pragma unroll
     for (int r = 0; r < C::NY; ++r) {
-      const int t = tid + 256 * r, jj = t / L1P, q = t % L1P;
+      const int t = tid + NTH * r, jj = t % L1P, q = t / L1P;
       Yt[q * JS + jj] = (jj < e1 && q < e1) ? ry[r] : 0.0f;
     }
     // the epilogue's operands, in flight during phase 2 (clamped addresses)
@@ -1330,6 +1406,8 @@ __global__ void __launch_bounds__(256) k_plane_mf1(PlanePass p, const int *gate,
     const int ec = p.addth ? 1 : (p.dot ? 2 : 0);
     const View ve = ec == 1 ? p.th[comp < 3 ? comp : 2] : p.r[comp < 3 ? comp : 2];
     const View vz = p.zold[comp < 3 ? comp : 2];
+    const int span = (ve.s1 * (e1 - 1) + ve.s2 * plane + e0) * 8, spanz = (vz.s1 * (e1 - 1) + vz.s2 * plane + e0) * 8;
+    const __amdgpu_buffer_rsrc_t re = buf_rsrc(ve.p, span), rz = buf_rsrc(vz.p, spanz);
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
@@ -1337,62 +1415,37 @@ __global__ void __launch_bounds__(256) k_plane_mf1(PlanePass p, const int *gate,
     if (ec) {
 #pragma unroll
       for (int s = 0; s < S; ++s) {
-        const int t = min(w + 4 * s, NT - 1), mt = t % MT, qt = t / MT;
+        const int t = min(w + 8 * s, NTr - 1), mt = t % MT, qt = t / MT;
         const int qc = min(qt * 16 + lr, e1 - 1);
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          ev[s][r] = ((const double *)ve.p)[voff(ve, min(m0 + mt * 16 + 4 * lk + r, e0 - 1), qc, plane)];
+          ev[s][r] = buf_load<double>(re, voff(ve, min(m0 + mt * 16 + 4 * lk + r, e0 - 1), qc, plane) * 8);
       }
       if (p.dot == 2)
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-          const int t = min(w + 4 * s, NT - 1), mt = t % MT, qt = t / MT;
+          const int t = min(w + 8 * s, NTr - 1), mt = t % MT, qt = t / MT;
           const int qc = min(qt * 16 + lr, e1 - 1);
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            zv[s][r] = ((const double *)vz.p)[voff(vz, min(m0 + mt * 16 + 4 * lk + r, e0 - 1), qc, plane)];
+            zv[s][r] = buf_load<double>(rz, voff(vz, min(m0 + mt * 16 + 4 * lk + r, e0 - 1), qc, plane) * 8);
         }
     }
     __syncthreads();
     // phase 2: out(m, q) = Σ_j S(m, j) Y(j, q), groups of 16 j
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      const int t = min(w + 4 * s, NT - 1);
+      const int t = min(w + 8 * s, NTr - 1);
       ao[s] = ((t % MT) * 16 + lr) * JS + 4 * lk;
       bo[s] = ((t / MT) * 16 + lr) * JS + 4 * lk;
     }
     const int G2 = (e1 + 15) / 16;
-    {
-      v4f A[S], B[S];
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        A[s] = *(const v4f *)(St + ao[s]);
-        B[s] = *(const v4f *)(Yt + bo[s]);
-      }
-      for (int g = 0; g < G2; ++g) {
-        const int gn = min(g + 1, G2 - 1) * 16;
-        v4f An[S], Bn[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-          An[s] = *(const v4f *)(St + ao[s] + gn);
-          Bn[s] = *(const v4f *)(Yt + bo[s] + gn);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int s = 0; s < S; ++s) acc[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[s][u], B[s][u], acc[s], 0, 0, 0);
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-          A[s] = An[s];
-          B[s] = Bn[s];
-        }
-      }
-    }
+    mf_slots<S>(ns, acc, St, ao, Yt, bo, G2);
     const View vo = p.out[comp];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      if (!(s < S - 1 || last)) continue;
-      const int t = w + 4 * s, mt = t % MT, qt = t / MT;
+      if (s >= ns) continue;
+      const int t = w + 8 * s, mt = t % MT, qt = t / MT;
       const int q = qt * 16 + lr;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -1411,7 +1464,7 @@ __global__ void __launch_bounds__(256) k_plane_mf1(PlanePass p, const int *gate,
     }
   }
   if (p.dot) {
-    block_sum2_t(d0, d1, sh, tid);
+    block_sum2_8w(d0, d1, sh, tid);
     if (tid == 0) {
       double *pr = region_w(p.d, PART_RZ);
       pr[2 * bid] = d0;
@@ -1454,7 +1507,9 @@ static Plane1Pick plane1_pick(const PlanePass &p) {
     l1 = p.e[c][1] > l1 ? p.e[c][1] : l1;
     k.gy = p.e[c][2] > k.gy ? p.e[c][2] : k.gy;
   }
-  k.use = p.one != 0 && l0 > 64 && l0 <= 32 * PLANE1_XC && l1 > 32 && l1 <= 32 * PLANE1_TJ;
+  bool tr = true;
+  for (int c = 0; c < k.nc; ++c) tr = tr && p.XT[c] && p.YT[c];
+  k.use = p.one != 0 && tr && l0 > 64 && l0 <= 32 * PLANE1_XC && l1 > 32 && l1 <= 32 * PLANE1_TJ;
   if (!k.use) return k;
   k.xc = cdiv(l0, 32);
   k.tj = cdiv(l1, 32);
@@ -1497,7 +1552,7 @@ hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
       switch ((k.xc * 4 + k.tj) * 4 + k.mt) {
 #define TDGL_PLANE1_CASE(x, t, m)                                                                                 \
   case (x * 4 + t) * 4 + m:                                                                                       \
-    hipLaunchKernelGGL((k_plane_mf1<TI, TO, x, t, m>), g1, dim3(256), (PlaneMf1<x, t, m>::lds), s, p, gate, k.gx, \
+    hipLaunchKernelGGL((k_plane_mf1<TI, TO, x, t, m>), g1, dim3(PLANE1_T), (PlaneMf1<x, t, m>::lds), s, p, gate, k.gx, \
                        k.gy);                                                                                     \
     break;
         TDGL_PLANE1_CASE(3, 2, 2) TDGL_PLANE1_CASE(3, 3, 2) TDGL_PLANE1_CASE(3, 2, 3) TDGL_PLANE1_CASE(3, 3, 3)

@@ -417,6 +417,9 @@ PlanePass plane_pass(tdgl_handle *h, bool inv, bool f32) {
     const int wx = c == 0 ? 1 : 0, wy = c == 1 ? 1 : 0;
     p.X[c] = f32 ? (const void *)h->Af[0][wx][inv] : (const void *)h->Ad[0][wx][inv];
     p.Y[c] = f32 ? (const void *)h->Af[1][wy][inv] : (const void *)h->Ad[1][wy][inv];
+    // the inverse basis is the forward one transposed
+    p.XT[c] = f32 ? (const void *)h->Af[0][wx][!inv] : (const void *)h->Ad[0][wx][!inv];
+    p.YT[c] = f32 ? (const void *)h->Af[1][wy][!inv] : (const void *)h->Ad[1][wy][!inv];
   }
   return p;
 }
