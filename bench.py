@@ -1,11 +1,11 @@
 """Ladder-cycle throughput of the HIP stepper on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kwt_pcg_128x64x16]
 
 A "step" of this benchmark is one ladder cycle: lMax = 40 time steps of the release-2.0 step
 (the n that TDGL's ladder hands to its stepper hook, gtdgl/src/protocol.jl:28, 409), with the
 state resident on the device.  The workload is the two-band KWT stepper on 128x64x16 cells with
-an insulating buffer and the PCG field solve to relative residual 1e-10 (juliatdgl_amd.configs.cfg1).
+an insulating buffer and the PCG field solve to relative residual 1e-10 (juliatdgl_amd.configs.kwt_pcg_128x64x16).
 Prints one JSON line: cell-cycles/s (nx·ny·nz × cycles / s), the roofline of the dominant
 kernel (HIP events around its launches over the timed cycles), and the CPU oracle timed on a
 bounded sample of the same workload on this host.
@@ -49,8 +49,12 @@ def kernel_bytes(st, name: str, launches: int, nsteps: int, stats: dict) -> floa
     if name == "sep_transform_f64":
         return 16.0 * rng * launches
     if name == "sep_transform_f32":
-        # FP64 in and FP32 out (forward) or FP32 in and FP64 out (inverse)
-        return 12.0 * rng * launches
+        # per PCG solve of `it` iterations: it + 1 forward launches (FP64 in, FP32 out: 12 B per
+        # element), the first inverse launch (FP32 in, FP64 out, r read for r·z: 20 B) and it
+        # inverse launches of the flexible iteration (r and z_old read: 28 B)
+        it = stats.get("iters_per_solve", 0.0)
+        per = (12.0 * (it + 1) + 20.0 + 28.0 * it) / (2.0 * (it + 1))
+        return per * rng * launches
     if name == "pcg_apply_A":
         return 24.0 * rng * launches
     if name == "current_field":
@@ -99,12 +103,45 @@ def cpu_baseline(st, s0, H, HJ, nsteps: int, threads: int):
     return cells * (nsteps / LMAX) / dt, dt
 
 
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the counter reads for a copy
+    of known size), or None where there is none."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_probe(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB through the HIP runtime (its copy kernel): the
+    reference dispatch whose FETCH_SIZE and WRITE_SIZE counters scale the counted HBM traffic."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for call in (lambda: hip.hipMalloc(ctypes.byref(a), n), lambda: hip.hipMalloc(ctypes.byref(b), n),
+                 lambda: hip.hipMemset(a, 1, n), lambda: hip.hipDeviceSynchronize(),
+                 lambda: hip.hipMemcpy(b, a, n, 3), lambda: hip.hipDeviceSynchronize(),  # 3: device to device
+                 lambda: hip.hipFree(a), lambda: hip.hipFree(b)):
+        rc = call()
+        if rc != 0:
+            raise RuntimeError(f"copy probe: HIP error {rc}")
+    print(f"[bench] copy probe: {mib} MiB read + {mib} MiB written", file=sys.stderr)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed ladder cycles")
     ap.add_argument("--warmup", type=int, default=2, help="untimed ladder cycles")
-    ap.add_argument("--workload", default="cfg1")
+    ap.add_argument("--workload", default="kwt_pcg_128x64x16", choices=("kwt_pcg_128x64x16", "open_3band_256x128x32"))
     ap.add_argument("--cpu-steps", type=int, default=8, help="time steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-out", default=None, help="write the per-kernel profile (JSON) here")
@@ -118,26 +155,20 @@ def main(argv=None):
     if device_count() < 1:
         print("no HIP device", file=sys.stderr)
         return 1
-    if args.workload == "cfg1":
-        st, p = configs.cfg1()
+    if args.workload == "kwt_pcg_128x64x16":
+        st, p = configs.kwt_pcg_128x64x16()
         workload = ("two-band KWT (u=Gamma/sigma_n=5.79, gkwt=10) TDGL, 128x64x16 cells, dx=0.5, buf 2 insulating "
                     "(sigma_vac/sigma_n=0.01), PCG tol 1e-10 (Float32 preconditioner, FP64 outer), 30-degree wedge, "
                     "H=0.2, J=0.1, dt_scale=0.1, ladder cycle = 40 steps")
-    elif args.workload == "cfg3":
-        st, p = configs.cfg3_like()
+    elif args.workload == "open_3band_256x128x32":
+        st, p = configs.open_3band_256x128x32()
         workload = ("three-band anisotropic (m_z = 4 m_xy) TDGL, 256x128x32 cells, dx=0.5, buf 8 insulating "
                     "(sigma_vac/sigma_n=0.01), PCG tol 1e-8 (Float32 preconditioner, FP64 outer), open boundary "
                     "(FFT convolution, every t_GL), notch, H=0.2, J=0.1, ladder cycle = 40 steps")
     else:
         raise SystemExit(f"unknown workload {args.workload}")
     if args.copy_probe:
-        import torch
-        a = torch.ones(args.copy_probe * (1 << 20) // 8, dtype=torch.float64, device="cuda:0")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        print(f"[bench] copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written", file=sys.stderr)
-        del a, b
+        copy_probe(args.copy_probe)
     H, HJ = p["H"], st.selffield(p["J"])
     s0 = seeded_state(st.model, st.grid, st.masks, H, seed=20261202)
     dev = HipStepper(st)
@@ -179,7 +210,7 @@ def main(argv=None):
                   f"{1e3 * ms / nl:8.2f} us/launch", file=sys.stderr)
     dom, (nl, ms) = rows[0]
     avg_s = ms / nl / 1e3
-    byts = kernel_bytes(st, dom, nl, nsteps, prof) / nl
+    byts = kernel_bytes(st, dom, nl, nsteps, {"iters_per_solve": iters / nsteps}) / nl
     flops = kernel_flops(st, dom, nl) / nl
     peak = FP32_MFMA_PEAK if dom == "sep_transform_f32" else FP64_PEAK
     if byts > 0 and (flops == 0 or flops / byts < peak / HBM_PEAK):
@@ -193,6 +224,8 @@ def main(argv=None):
                 "note": "fused x-y transform launch of the Float32 preconditioner on v_mfma_f32_16x16x4f32 "
                         "(2 (Lx + Ly) flop per element); durations from HIP events around every launch "
                         "over the same number of cycles as the timed region"}
+    traffic = measured_traffic(dom, args.workload)
+    roof["traffic"] = traffic
     if args.profile_out:
         with open(args.profile_out, "w") as fh:
             json.dump({k: {"launches": v[0], "ms": v[1]} for k, v in prof.items()}, fh, indent=1)

@@ -12,7 +12,7 @@ import math
 from .scheme import Band, Grid, Model, make_stepper, notch_masks, sloped_wedge_masks
 
 
-def two_band_cfg(kappa=2.5, Gamma=1.0, gkwt=0.0):
+def two_band_model(kappa=2.5, Gamma=1.0, gkwt=0.0):
     """Two bands: (a, β, ξ, Γ) = (1, 1, 1, Γ) and (0.5, 1, 2.5, Γ), Josephson −0.1, κ = 2.5."""
     a2, R2 = 0.5, 2.5
     return Model.make((Band.make(m=1.0, Gamma=Gamma, a=1.0, gkwt=gkwt),
@@ -20,10 +20,10 @@ def two_band_cfg(kappa=2.5, Gamma=1.0, gkwt=0.0):
                       [[0.0, -0.1], [-0.1, 0.0]], kappa=kappa, sigma_n=1.0)
 
 
-def cfg0():
+def notch_12x8x4():
     """12x8x4 cells, buf 1, dx 0.5; notch f = 0 on cells x in [5,7), y in [5,8); bar field,
     KWT and noise off, dt = 0.01, separable implicit field.  H = 0.3, J = 0.1."""
-    md = two_band_cfg()
+    md = two_band_model()
     gr = Grid.make(12, 8, 4, dx=0.5, buf=1)
     mk = notch_masks(gr, (5, 7), (5, 8))
     st = make_stepper(md, gr, mk, dt=0.01, selffield_bc="bar")
@@ -34,8 +34,8 @@ U_KWT = 5.79       # u = Γ/σn of a dirty BCS band (model.jl:52)
 GKWT = 10.0        # KWT γ̃ (the reference's example value, gtdgl/examples/kwt_k2.toml)
 
 
-def cfg1(nx=128, ny=64, nz=16, pcg_tol=1e-10, J=0.1, dt_scale=0.1):
-    """The two bands of cfg0 (Γ = 1) with KWT: u = Γ/σn = 5.79 (σn = 1/5.79) and γ̃ = 10, on
+def kwt_pcg_128x64x16(nx=128, ny=64, nz=16, pcg_tol=1e-10, J=0.1, dt_scale=0.1):
+    """The two bands of notch_12x8x4 (Γ = 1) with KWT: u = Γ/σn = 5.79 (σn = 1/5.79) and γ̃ = 10, on
     128x64x16 cells, dx 0.5, an insulating buffer of 2 cells (σvac/σn = 0.01: the PCG solve) to
     relative residual 1e-10, a wedge cut into face y1 whose sloped face is at 30 degrees, bar
     field, H = 0.2.  The stepper's own dt rule with dt_scale = 0.1: at the default 0.5 the staged
@@ -43,7 +43,7 @@ def cfg1(nx=128, ny=64, nz=16, pcg_tol=1e-10, J=0.1, dt_scale=0.1):
     field against the KWT-stiffened phase; 0.2 still diverges on this grid), at 0.1 the run is stable.  J is one rung of the
     0 → 0.3 ramp."""
     sn = 1 / U_KWT
-    base = two_band_cfg(Gamma=1.0, gkwt=GKWT)
+    base = two_band_model(Gamma=1.0, gkwt=GKWT)
     md = Model.make(base.bands, [[0.0, -0.1], [-0.1, 0.0]], kappa=2.5, sigma_n=sn)
     gr = Grid.make(nx, ny, nz, dx=0.5, buf=2)
     mk = sloped_wedge_masks(gr, angle_deg=30.0, depth_frac=0.35, x_frac=0.45)
@@ -51,7 +51,7 @@ def cfg1(nx=128, ny=64, nz=16, pcg_tol=1e-10, J=0.1, dt_scale=0.1):
     return st, dict(H=0.2, J=J)
 
 
-def cfg3_like(nx=256, ny=128, nz=32, buf=8, pcg_tol=1e-8, selffield_bc="open"):
+def open_3band_256x128x32(nx=256, ny=128, nz=32, buf=8, pcg_tol=1e-8, selffield_bc="open"):
     """Three bands (a = 1, 0.6, 0.3; ξ = 1, 1.8, 3; Josephson −0.1, −0.05, −0.02) with
     m_z = 4 m_xy on 256x128x32 cells and an insulating buffer of 8 cells (σvac = 0.01, PCG with
     the Float32 preconditioner and the FP64 outer iteration), the open boundary (h_δ by the

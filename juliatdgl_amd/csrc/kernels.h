@@ -4,7 +4,29 @@
 
 #include "common.h"
 
+#include <hip/hip_ext.h>
+
 namespace tdgl {
+
+// Every launch of the library goes through TDGL_LAUNCH.  While a profile pair is set (the per-kernel
+// profile, eager launches only) the runtime records it at the kernel's own start and end
+// (hipExtLaunchKernelGGL), so the pair times the kernel and not the gap before it.
+struct ProfPair {
+  hipEvent_t a, b;
+  bool used;
+};
+extern thread_local ProfPair *prof_pair;
+#define TDGL_LAUNCH(kern, grid, block, shmem, stream, ...)                                                 \
+  do {                                                                                                     \
+    if (::tdgl::prof_pair) {                                                                               \
+      hipExtLaunchKernelGGL(kern, grid, block, shmem, stream, ::tdgl::prof_pair->a, ::tdgl::prof_pair->b, 0, \
+                            __VA_ARGS__);                                                                  \
+      ::tdgl::prof_pair->used = true;                                                                      \
+    } else {                                                                                               \
+      hipLaunchKernelGGL(kern, grid, block, shmem, stream, __VA_ARGS__);                                   \
+    }                                                                                                      \
+  } while (0)
+
 
 // ---------------------------------------------------------------- band update (bands.jl)
 enum FrameMode { FRAME_NONE = 0, FRAME_RATE = 1, FRAME_TERM = 2 };

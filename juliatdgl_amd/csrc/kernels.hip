@@ -8,6 +8,8 @@
 
 namespace tdgl {
 
+thread_local ProfPair *prof_pair = nullptr;
+
 // ============================================================================ band stage
 // Rate of the frame at a block node: the faces copy the interior (precedence x, y, z), so the
 // value at any block node is the one of the nearest interior node (bands.jl:224-232).
@@ -144,7 +146,7 @@ hipError_t launch_stage(const Geo &g, const StageArgs &a, hipStream_t s) {
   const int ni = g.x2 - g.x1, nj = g.y2 - g.y1, nk = g.z2 - g.z1;
   if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
   dim3 b(64, 4, 1), gr(cdiv(ni, 64), cdiv(nj, 4), nk);
-  hipLaunchKernelGGL(k_stage, gr, b, 0, s, g, a);
+  TDGL_LAUNCH(k_stage, gr, b, 0, s, g, a);
   return hipGetLastError();
 }
 
@@ -248,7 +250,7 @@ int field_blocks(const Geo &g) { return cdiv(g.Nx + 1, 64) * cdiv(g.Ny + 1, 4) *
 hipError_t launch_field(const Geo &g, const FieldArgs &a, hipStream_t s) {
   if (a.part && field_blocks(g) > MAX_PART) return hipErrorInvalidValue;
   dim3 b(64, 4, 1), gr(cdiv(g.Nx + 1, 64), cdiv(g.Ny + 1, 4), g.Nz + 1);
-  hipLaunchKernelGGL(k_field, gr, b, 0, s, g, a);
+  TDGL_LAUNCH(k_field, gr, b, 0, s, g, a);
   return hipGetLastError();
 }
 
@@ -363,7 +365,7 @@ hipError_t launch_curl(const Geo &g, const CurlArgs &a, hipStream_t s) {
   if (a.open && !(a.ofl && a.hold[0] && a.hold[1] && a.hold[2] && a.hnew[0] && a.hnew[1] && a.hnew[2]))
     return hipErrorInvalidValue;
   dim3 b(64, 4, 1), gr(cdiv(g.Nx + 1, 64), cdiv(g.Ny + 1, 4), g.Nz + 1);
-  hipLaunchKernelGGL(k_curl, gr, b, 0, s, g, a);
+  TDGL_LAUNCH(k_curl, gr, b, 0, s, g, a);
   return hipGetLastError();
 }
 
@@ -439,7 +441,7 @@ __global__ void __launch_bounds__(256) k_ghosts(Geo g, GhostArgs a) {
 hipError_t launch_ghosts(const Geo &g, const GhostArgs &a, hipStream_t s) {
   const int ni = g.x2 - g.x1 + 2, nj = g.y2 - g.y1 + 2, nk = g.z2 - g.z1 + 2;
   const int n = 2 * (nj * nk + ni * nk + ni * nj);
-  hipLaunchKernelGGL(k_ghosts, dim3(cdiv(n, 256)), dim3(256), 0, s, g, a);
+  TDGL_LAUNCH(k_ghosts, dim3(cdiv(n, 256)), dim3(256), 0, s, g, a);
   return hipGetLastError();
 }
 
@@ -457,7 +459,7 @@ hipError_t launch_copies(const CopyList &c, hipStream_t s) {
   int m = 0;
   for (int a = 0; a < c.cnt; ++a) m = c.n[a] > m ? c.n[a] : m;
   const int bx = cdiv(m, 256) < 1024 ? cdiv(m, 256) : 1024;
-  hipLaunchKernelGGL(k_copies, dim3(bx, c.cnt), dim3(256), 0, s, c);
+  TDGL_LAUNCH(k_copies, dim3(bx, c.cnt), dim3(256), 0, s, c);
   return hipGetLastError();
 }
 
@@ -743,7 +745,7 @@ int tx_parts(const TxPass &p) {
 template <class TI, class TA, class TO>
 hipError_t launch_tx(const TxPass &p, const int *gate, hipStream_t s) {
   if (tx_parts(p) > MAX_PART) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_tx<TI, TA, TO>), tx_grid(p), dim3(256), 0, s, p, gate);
+  TDGL_LAUNCH((k_tx<TI, TA, TO>), tx_grid(p), dim3(256), 0, s, p, gate);
   return hipGetLastError();
 }
 template hipError_t launch_tx<double, double, double>(const TxPass &, const int *, hipStream_t);
@@ -808,7 +810,7 @@ __global__ void __launch_bounds__(256) k_tx2_modes(Tx2Modes m, const int *gate) 
 
 template <class T>
 hipError_t launch_tx2_modes(const Tx2Modes &m, const int *gate, hipStream_t s) {
-  hipLaunchKernelGGL((k_tx2_modes<T>), dim3(cdiv(m.nx * m.ny * m.nz, 256)), dim3(256), 0, s, m, gate);
+  TDGL_LAUNCH((k_tx2_modes<T>), dim3(cdiv(m.nx * m.ny * m.nz, 256)), dim3(256), 0, s, m, gate);
   return hipGetLastError();
 }
 template hipError_t launch_tx2_modes<double>(const Tx2Modes &, const int *, hipStream_t);
@@ -1552,7 +1554,7 @@ hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
       switch ((k.xc * 4 + k.tj) * 4 + k.mt) {
 #define TDGL_PLANE1_CASE(x, t, m)                                                                                 \
   case (x * 4 + t) * 4 + m:                                                                                       \
-    hipLaunchKernelGGL((k_plane_mf1<TI, TO, x, t, m>), g1, dim3(PLANE1_T), (PlaneMf1<x, t, m>::lds), s, p, gate, k.gx, \
+    TDGL_LAUNCH((k_plane_mf1<TI, TO, x, t, m>), g1, dim3(PLANE1_T), (PlaneMf1<x, t, m>::lds), s, p, gate, k.gx, \
                        k.gy);                                                                                     \
     break;
         TDGL_PLANE1_CASE(3, 2, 2) TDGL_PLANE1_CASE(3, 3, 2) TDGL_PLANE1_CASE(3, 2, 3) TDGL_PLANE1_CASE(3, 3, 3)
@@ -1569,9 +1571,9 @@ hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
 #define TDGL_PLANE_CASE(tj)                                                                     \
   case tj:                                                                                      \
     if constexpr (sizeof(T) == 4)                                                               \
-      hipLaunchKernelGGL((k_plane_mf<TI, TO, tj>), gr, dim3(256), lds, s, p, gate);             \
+      TDGL_LAUNCH((k_plane_mf<TI, TO, tj>), gr, dim3(256), lds, s, p, gate);             \
     else                                                                                        \
-      hipLaunchKernelGGL((k_plane<TI, T, TO, tj>), gr, dim3(256), lds, s, p, gate);             \
+      TDGL_LAUNCH((k_plane<TI, T, TO, tj>), gr, dim3(256), lds, s, p, gate);             \
     break;
     TDGL_PLANE_CASE(1)
     TDGL_PLANE_CASE(2)
@@ -1665,7 +1667,7 @@ template <class T>
 hipError_t launch_zmodes(const ZModes &m, const int *gate, hipStream_t s) {
   if (m.nz < 2 || m.nz > 256 || m.nx < 1 || m.ny < 1) return hipErrorInvalidValue;
   const int C = 256 / m.nz;
-  hipLaunchKernelGGL((k_zmodes<T>), dim3(cdiv(m.nx * m.ny, C)), dim3(256), 0, s, m, gate);
+  TDGL_LAUNCH((k_zmodes<T>), dim3(cdiv(m.nx * m.ny, C)), dim3(256), 0, s, m, gate);
   return hipGetLastError();
 }
 template hipError_t launch_zmodes<float>(const ZModes &, const int *, hipStream_t);
@@ -1723,7 +1725,7 @@ __global__ void __launch_bounds__(256) k_qp_conv(Geo g, QpConv a) {
 
 hipError_t launch_qp_conv(const Geo &g, const QpConv &a, hipStream_t s) {
   const int n = a.n1 * a.nj * a.nk;
-  hipLaunchKernelGGL(k_qp_conv, dim3(cdiv(n, 256)), dim3(256), 0, s, g, a);
+  TDGL_LAUNCH(k_qp_conv, dim3(cdiv(n, 256)), dim3(256), 0, s, g, a);
   return hipGetLastError();
 }
 
@@ -1758,7 +1760,7 @@ __global__ void __launch_bounds__(256) k_qp_zcol(QpZ a) {
 hipError_t launch_qp_zcol(const QpZ &a, hipStream_t s) {
   if (a.nk < 1 || a.nk > 256 || a.K < 1 || a.K > TDGL_MAX_BANDS) return hipErrorInvalidValue;
   const int C = 256 / a.nk;
-  hipLaunchKernelGGL(k_qp_zcol, dim3(cdiv(a.n1 * a.nj, C), a.K), dim3(256), 0, s, a);
+  TDGL_LAUNCH(k_qp_zcol, dim3(cdiv(a.n1 * a.nj, C), a.K), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -2059,7 +2061,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_finish(PcgArgs a) {
 static hipError_t launch_box(void (*k)(PcgArgs), const PcgArgs &a, hipStream_t s) {
   const int nb = pcg_box_blocks(a.g);
   if (nb > MAX_PART) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k, dim3(nb), dim3(RED_T), 0, s, a);
+  TDGL_LAUNCH(k, dim3(nb), dim3(RED_T), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_pcg_start(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_start, a, s); }
@@ -2079,7 +2081,7 @@ __global__ void k_pcg_cycle_reset(PcgDev d) {
 }
 
 hipError_t launch_pcg_cycle_reset(PcgDev d, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_cycle_reset, dim3(1), dim3(64), 0, s, d);
+  TDGL_LAUNCH(k_pcg_cycle_reset, dim3(1), dim3(64), 0, s, d);
   return hipGetLastError();
 }
 

@@ -22,7 +22,7 @@ __global__ void k_open_clock(int *ofl, int every) {
   ofl[OF_COUNT] = re ? 0 : c;
 }
 hipError_t launch_open_clock(int *ofl, int every, hipStream_t s) {
-  hipLaunchKernelGGL(k_open_clock, dim3(1), dim3(64), 0, s, ofl, every);
+  TDGL_LAUNCH(k_open_clock, dim3(1), dim3(64), 0, s, ofl, every);
   return hipGetLastError();
 }
 
@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(256) k_open_dj(Geo g, OpenDJ a) {
 hipError_t launch_open_dj(const Geo &g, const OpenDJ &a, hipStream_t s) {
   int m = 0;
   for (int c = 0; c < 3; ++c) m = std::max(m, a.n[c][0] * a.n[c][1] * a.n[c][2]);
-  hipLaunchKernelGGL(k_open_dj, dim3(cdiv(m, 256), 3), dim3(256), 0, s, g, a);
+  TDGL_LAUNCH(k_open_dj, dim3(cdiv(m, 256), 3), dim3(256), 0, s, g, a);
   return hipGetLastError();
 }
 
@@ -215,7 +215,7 @@ hipError_t launch_fft(FftPass p, const int *gate, hipStream_t s) {
     return hipErrorInvalidValue;
   p.lpb = fft_lpb(p.ax.M);
   const int nlines = p.nl0 * p.nl1;
-  hipLaunchKernelGGL(k_fft, dim3(cdiv(nlines, p.lpb)), dim3(256), 2 * FFT_LDS * sizeof(cplx), s, p, gate);
+  TDGL_LAUNCH(k_fft, dim3(cdiv(nlines, p.lpb)), dim3(256), 2 * FFT_LDS * sizeof(cplx), s, p, gate);
   return hipGetLastError();
 }
 
@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(256) k_open_green(OpenGreen a) {
 }
 hipError_t launch_open_green(const OpenGreen &a, hipStream_t s) {
   const long long n = (long long)a.M[0] * a.M[1] * a.M[2];
-  hipLaunchKernelGGL(k_open_green, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  TDGL_LAUNCH(k_open_green, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(256) k_open_face(Geo g, OpenFace a) {
     a.hnew[1][q] = ((th(0, i, j, k + 1) - th(0, i, j, k)) - th(2, i + 1, j, k) + th(2, i, j, k)) / (a.dz * a.dx);
 }
 hipError_t launch_open_face(const Geo &g, const OpenFace &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_open_face, dim3(cdiv(g.Nx + 1, 64), g.Ny + 1, g.Nz + 1), dim3(64), 0, s, g, a);
+  TDGL_LAUNCH(k_open_face, dim3(cdiv(g.Nx + 1, 64), g.Ny + 1, g.Nz + 1), dim3(64), 0, s, g, a);
   return hipGetLastError();
 }
 

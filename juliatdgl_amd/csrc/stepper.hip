@@ -101,6 +101,7 @@ struct tdgl_handle {
   float *sf[3] = {};
   double *T1 = nullptr, *T2 = nullptr;
   float *T32a = nullptr, *T32b = nullptr;
+  ProfPair ppair{};                       // the events of the launch being profiled
   bool plane32 = false, plane64 = false;  // the fused transform launches fit (per precision)
   int plane1 = 1;                         // PlanePass::one of the Float32 launches
   // normal fluids: the sources / solved modes (region-shaped, one per mode), work, the bases
@@ -230,9 +231,14 @@ int prof_mark(tdgl_handle *h, int cls, bool begin) {
   if (begin) {
     if (h->evcls.size() <= h->evn / 2) h->evcls.resize(h->evn / 2 + 1);
     h->evcls[h->evn / 2] = cls;
-    HIPCHK(hipEventRecord(h->evs[h->evn], h->s));
+    h->ppair = ProfPair{h->evs[h->evn], h->evs[h->evn + 1], false};
+    prof_pair = &h->ppair;  // the launch records the pair at the kernel's start and end
   } else {
-    HIPCHK(hipEventRecord(h->evs[h->evn + 1], h->s));
+    prof_pair = nullptr;
+    if (!h->ppair.used) {  // no kernel was launched: an empty interval
+      HIPCHK(hipEventRecord(h->evs[h->evn], h->s));
+      HIPCHK(hipEventRecord(h->evs[h->evn + 1], h->s));
+    }
     h->evn += 2;
   }
   return 0;
@@ -242,6 +248,7 @@ int prof_mark(tdgl_handle *h, int cls, bool begin) {
   do {                                                                                                \
     if (h->prof) CHK(prof_mark(h, (cls), true));                                                      \
     hipError_t e_ = (expr);                                                                           \
+    if (h->prof) prof_pair = nullptr;                                                                 \
     if (e_ != hipSuccess) return fail(TDGL_E_DEVICE, "launch %s: %s", #expr, hipGetErrorString(e_));  \
     if (h->prof) CHK(prof_mark(h, (cls), false));                                                     \
   } while (0)
