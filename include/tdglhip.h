@@ -214,13 +214,55 @@ typedef struct {
 int tdgl_get_open(tdgl_handle *h, tdgl_open *o);
 int tdgl_set_open(tdgl_handle *h, const tdgl_open *o);
 
+/* The voltmeter of a ladder (ladder, gtdgl/src/protocol.jl:341-352): the x-link planes vi and vo
+ * (1-based, x1 + delta_v and x2 - delta_v) and four masks over the lines (j, k), j in y1..y2+1
+ * fastest, k in z1..z2+1, one byte each: the live links of plane vi (Min) and of plane vo (Mou), the
+ * lines whose links vi..vo-1 are all live (line_ok), and those of them that are condensed
+ * throughout (line_f1); κ² of the model for the mid-plane current. */
+typedef struct {
+  int32_t vi, vo;
+  double kappa2;
+  const uint8_t *Min, *Mou, *line_ok, *line_f1;
+} tdgl_meter;
+
+/* One cycle's measurements of the ladder, from the state on the device (protocol.jl:416-464; the
+ * host reference reads them from the whole downloaded state every cycle):
+ *   V = Σ_Mou φx[vo] / |Mou| − Σ_Min φx[vi] / |Min|                              (:416, _plane_sum :148-155)
+ *   TA = the mean over line_ok of Σ_{ii=vi}^{vo-1} φx[ii]                          (:417, _theta_A :157-166)
+ *   T[µ] = the mean over line_ok of the unwrapped path phase of band µ, Tabs[µ] the sum of its |value| over
+ *          line_ok (_crossings_now :193-199), Tf1 the mean of band 1 over line_f1,
+ *          dmax and nbig the largest wrapped change of band 1 and the number above 2 since the rung began
+ *                                                                          (:418-440, _path_phase! :168-179, _unwrap! :181-189)
+ *   It, Is = the current through the mid plane, total and through the sample   (:442, observables.jl:102-114)
+ *   dif[µ] = max |(|Ψ_µ| − its value at the last cycle)| over the block, amax = max |Ψ_µ|   (:445-454)
+ *   diffh = max |hz − its value at the last cycle|                               (:459-463)
+ * Maxima propagate NaN, as Julia's max does (a diverged state shows in amax).
+ * `first` starts a rung (m = 1: the path phases become the reference, T, Tf1, dmax and nbig zero). */
+typedef struct {
+  double V, TA, Tf1;
+  double T[TDGL_MAX_BANDS];
+  double Tabs[TDGL_MAX_BANDS];
+  double dmax;
+  int32_t nbig;
+  int32_t pad;
+  double It, Is;
+  double dif[TDGL_MAX_BANDS];
+  double amax, diffh;
+} tdgl_cycle_record;
+/* Uploads the voltmeter and takes |Ψ_µ| on the block and hz of the current state as the last
+ * cycle's (Cooper0 and fieldOld, protocol.jl:371-372). */
+int tdgl_meter_begin(tdgl_handle *h, const tdgl_meter *m);
+/* The record of the cycle that just ended (after tdgl_step); TDGL_E_ARG before tdgl_meter_begin. */
+int tdgl_meter_cycle(tdgl_handle *h, int32_t first, tdgl_cycle_record *out);
+
 /* Counters (execution_stats, TDGLGPU.jl:2086-2087, per handle). */
 int tdgl_get_stats(tdgl_handle *h, tdgl_stats *st);
 
 /* Per-kernel device time (no counterpart in the reference: the measurement of this library).
- * While enabled, every launch is bracketed by HIP events on the handle's stream; the totals
- * are per kernel class (tdgl_kernel_name).  Enabling clears them. */
-#define TDGL_NKERNELS 13
+ * While enabled, steps are launched from the host and every launch records a pair of HIP events
+ * at the kernel's own start and end; the totals are per kernel class (tdgl_kernel_name).
+ * Enabling clears them. */
+#define TDGL_NKERNELS 14
 typedef struct {
   int64_t launches[TDGL_NKERNELS];
   double ms[TDGL_NKERNELS];

@@ -74,7 +74,19 @@ class StatsC(ct.Structure):
                 ("graph_cycles", ct.c_int64), ("rollbacks", ct.c_int64), ("pcg_budget", ct.c_int32)]
 
 
-NKERNELS = 13
+NKERNELS = 14
+
+
+class MeterC(ct.Structure):
+    _fields_ = [("vi", ct.c_int32), ("vo", ct.c_int32), ("kappa2", ct.c_double),
+                ("Min", ct.POINTER(ct.c_uint8)), ("Mou", ct.POINTER(ct.c_uint8)),
+                ("line_ok", ct.POINTER(ct.c_uint8)), ("line_f1", ct.POINTER(ct.c_uint8))]
+
+
+class CycleRecordC(ct.Structure):
+    _fields_ = [("V", ct.c_double), ("TA", ct.c_double), ("Tf1", ct.c_double), ("T", ct.c_double * MAXB),
+                ("Tabs", ct.c_double * MAXB), ("dmax", ct.c_double), ("nbig", ct.c_int32), ("pad", ct.c_int32), ("It", ct.c_double),
+                ("Is", ct.c_double), ("dif", ct.c_double * MAXB), ("amax", ct.c_double), ("diffh", ct.c_double)]
 
 
 class ProfileC(ct.Structure):

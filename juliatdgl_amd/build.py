@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libtdglhip.so")
-SOURCES = ("kernels.hip", "openbc.hip", "stepper.hip")
+SOURCES = ("kernels.hip", "openbc.hip", "meter.hip", "stepper.hip")
 ARCH = os.environ.get("TDGL_HIP_ARCH", "gfx950")
 
 

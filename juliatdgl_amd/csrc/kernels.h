@@ -359,6 +359,35 @@ hipError_t launch_pcg_start(const PcgArgs &a, hipStream_t s);    // nb, warm sta
 hipError_t launch_pcg_resid(const PcgArgs &a, hipStream_t s);    // Q = A x, r = b − Q, r·r
 hipError_t launch_pcg_p(const PcgArgs &a, hipStream_t s);        // β, P = z + βP, zold = z
 hipError_t launch_pcg_ap(const PcgArgs &a, hipStream_t s);       // Q = A P, p·Ap
+// ---------------------------------------------------------------- the ladder's per-cycle measurements
+// partial slots of a line block (and, from 0, of the other blocks' partials)
+enum { METER_PIN = 0, METER_POUT, METER_TA, METER_TF1, METER_DMAX, METER_NBIG, METER_T,
+       METER_TABS = METER_T + TDGL_MAX_BANDS, METER_Q = METER_TABS + TDGL_MAX_BANDS };
+// the record's slots (tdgl_cycle_record in this order, nbig as a double)
+enum { MR_V = 0, MR_TA, MR_TF1, MR_T, MR_TABS = MR_T + TDGL_MAX_BANDS, MR_DMAX = MR_TABS + TDGL_MAX_BANDS, MR_NBIG,
+       MR_IT, MR_IS, MR_DIF, MR_AMAX = MR_DIF + TDGL_MAX_BANDS, MR_DIFFH, MR_N };
+struct MeterArgs {
+  Geo g;
+  int N;
+  const cplx *G[TDGL_MAX_BANDS];  // Ψ of the state (the reference's F on the block)
+  const cplx *Vx;                 // V on x links
+  const double *phx, *hz, *hy;    // the reference's φx, hz, hy
+  double *cooper[TDGL_MAX_BANDS]; // |Ψ_µ| of the last cycle, block-compact (x fastest)
+  double *hold;                   // hz of the last cycle (node layout)
+  double *Dprev[TDGL_MAX_BANDS], *T[TDGL_MAX_BANDS];  // per line: last path phase, unwrapped phase
+  const unsigned char *Min, *Mou, *lok, *lf1;
+  int vi, vo, nl1, nl2;
+  int nin, nou, nlin, nlin_f1;
+  int mid;                        // oround(Nx / 2)
+  double ks2, dy, dz;
+  int first, init;                // first cycle of a rung; init: only take |Ψ| and hz
+  double *part;
+  int nlb, nnb, nhb;              // line, block-node and hz blocks (then one mid-plane block)
+};
+int meter_blocks(const MeterArgs &a);
+hipError_t launch_meter(const MeterArgs &a, hipStream_t s);
+hipError_t launch_meter_finish(const MeterArgs &a, double *rec, double *rung, hipStream_t s);
+
 hipError_t launch_pcg_pap(const PcgArgs &a, hipStream_t s);      // β, P = z + βP, zold = z, Q = A P, p·Ap
 hipError_t launch_pcg_xr(const PcgArgs &a, hipStream_t s);       // α, x += αP, r −= αQ, r·r
 hipError_t launch_pcg_finish(const PcgArgs &a, hipStream_t s);   // φ = θ + x, the cycle's counts

@@ -59,10 +59,10 @@ Bases make_bases(int nn) {
 }
 
 // kernel classes of the per-kernel profile (tdgl_profile_get)
-enum { K_STAGE = 0, K_QP, K_FIELD, K_CURL, K_GHOSTS, K_SCATTER, K_TX, K_TX32, K_MODES, K_APPLYA, K_DOT, K_FINISH, K_VEC, K_COUNT };
+enum { K_STAGE = 0, K_QP, K_FIELD, K_CURL, K_GHOSTS, K_SCATTER, K_TX, K_TX32, K_MODES, K_APPLYA, K_DOT, K_FINISH, K_VEC, K_METER, K_COUNT };
 const char *const K_NAMES[K_COUNT] = {"band_stage", "normal_fluids", "current_field", "links_curl_bc", "neumann_ghosts",
                                       "solve_scatter", "sep_transform_f64", "sep_transform_f32", "sep_modes", "pcg_apply_A",
-                                      "pcg_dot", "pcg_finish", "pcg_vector"};
+                                      "pcg_dot", "pcg_finish", "pcg_vector", "cycle_meter"};
 static_assert(K_COUNT == TDGL_NKERNELS, "kernel classes");
 
 struct HostSnap {
@@ -102,6 +102,15 @@ struct tdgl_handle {
   double *T1 = nullptr, *T2 = nullptr;
   float *T32a = nullptr, *T32b = nullptr;
   ProfPair ppair{};                       // the events of the launch being profiled
+  // F of the last upload (host layout): outside the block no step writes F, so it is what the
+  // reference's F holds there; `stepped`: a step ran since (then F equals G on the block)
+  std::vector<double> Fup[TDGL_MAX_BANDS];
+  bool stepped = false;
+  // the ladder's per-cycle measurements (tdgl_meter_begin): its own allocations
+  std::vector<void *> mallocs;
+  MeterArgs ma{};
+  double *mrec = nullptr, *mrung = nullptr;
+  bool meter = false;
   bool plane32 = false, plane64 = false;  // the fused transform launches fit (per precision)
   int plane1 = 1;                         // PlanePass::one of the Float32 launches
   // normal fluids: the sources / solved modes (region-shaped, one per mode), work, the bases
@@ -1527,6 +1536,7 @@ int tdgl_destroy(tdgl_handle *h) {
   if (!h) return 0;
   if (h->s) (void)hipStreamSynchronize(h->s);
   for (void *p : h->allocs) (void)hipFree(p);
+  for (void *p : h->mallocs) (void)hipFree(p);
   destroy_graph(h);
   if (h->hph) (void)hipHostFree(h->hph);
   if (h->flh) (void)hipHostFree(h->flh);
@@ -1561,7 +1571,9 @@ int tdgl_upload_state(tdgl_handle *h, const tdgl_state *s) {
           Fh[2 * q + 1] = s->G[mu][2 * q + 1];
         }
     CHK(put(h, h->F[mu], Fh.data(), Nx + 1, Ny + 1, Nz + 1, 16));
+    h->Fup[mu].assign(s->F[mu], s->F[mu] + Fh.size());
   }
+  h->stepped = false;
   (void)g;
   const int lx[3] = {Nx, Ny + 1, Nz + 1}, ly[3] = {Nx + 1, Ny, Nz + 1}, lz[3] = {Nx + 1, Ny + 1, Nz};
   const int *ls[3] = {lx, ly, lz};
@@ -1590,7 +1602,23 @@ int tdgl_download_state(tdgl_handle *h, tdgl_state *s) {
   for (int mu = 0; mu < d.nbands; ++mu) {
     if (!s->G[mu] || !s->F[mu]) return fail(TDGL_E_ARG, "null band array");
     CHK(get(h, s->G[mu], h->G[mu], Nx + 1, Ny + 1, Nz + 1, 16));
-    CHK(get(h, s->F[mu], h->F[mu], Nx + 1, Ny + 1, Nz + 1, 16));
+    // F as the reference leaves it (advance!, field.jl:248-255): after a step it equals G on the
+    // block (the swap left the step before in the device's F), and outside the block it is the
+    // F that was uploaded
+    const size_t ne = (size_t)(Nx + 1) * (Ny + 1) * (Nz + 1) * 2;
+    if (h->stepped)
+      std::memcpy(s->F[mu], s->G[mu], ne * 8);
+    else
+      CHK(get(h, s->F[mu], h->F[mu], Nx + 1, Ny + 1, Nz + 1, 16));
+    if (h->Fup[mu].size() == ne)
+      for (int k = 1; k <= Nz + 1; ++k)
+        for (int j = 1; j <= Ny + 1; ++j)
+          for (int i = 1; i <= Nx + 1; ++i) {
+            if (i >= d.x1 && i <= d.x2 + 1 && j >= d.y1 && j <= d.y2 + 1 && k >= d.z1 && k <= d.z2 + 1) continue;
+            const size_t q = (size_t)(i - 1) + (size_t)(Nx + 1) * ((j - 1) + (size_t)(Ny + 1) * (k - 1));
+            s->F[mu][2 * q] = h->Fup[mu][2 * q];
+            s->F[mu][2 * q + 1] = h->Fup[mu][2 * q + 1];
+          }
   }
   const int lx[3] = {Nx, Ny + 1, Nz + 1}, ly[3] = {Nx + 1, Ny, Nz + 1}, lz[3] = {Nx + 1, Ny + 1, Nz};
   const int *ls[3] = {lx, ly, lz};
@@ -1616,6 +1644,7 @@ int tdgl_step(tdgl_handle *h, int32_t n, double H, double HJ) {
   if (n < 0) return fail(TDGL_E_ARG, "negative step count");
   if (!std::isfinite(H) || !std::isfinite(HJ)) return fail(TDGL_E_ARG, "non-finite H or HJ");
   const bool pcg = h->d.field_mode == TDGL_FIELD_PCG;
+  if (n > 0) h->stepped = true;
   h->hph[0] = H;
   h->hph[1] = HJ;
   HIPCHK(hipMemcpyAsync(h->hp, h->hph, 2 * sizeof(double), hipMemcpyHostToDevice, h->s));
@@ -1723,6 +1752,124 @@ int tdgl_set_open(tdgl_handle *h, const tdgl_open *o) {
   }
   const int fl[4] = {o->count, 0, 0, 0};
   HIPCHK(hipMemcpy(h->ofl, fl, sizeof fl, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------- the ladder's meter
+extern "C++" {
+namespace {
+// the meter's arguments with the state's current arrays (the advance swaps them)
+MeterArgs meter_args(tdgl_handle *h, int first) {
+  MeterArgs a = h->ma;
+  for (int mu = 0; mu < h->d.nbands; ++mu) a.G[mu] = h->G[mu];
+  a.Vx = h->V[0];
+  a.phx = h->ph[0];
+  a.hz = h->h[2];
+  a.hy = h->h[1];
+  a.first = first ? 1 : 0;
+  a.init = 0;
+  return a;
+}
+template <class T>
+int malloc_meter(tdgl_handle *h, T **p, size_t n) {
+  void *q = nullptr;
+  if (hipMalloc(&q, (n ? n : 1) * sizeof(T)) != hipSuccess) return fail(TDGL_E_NOMEM, "hipMalloc of the meter failed");
+  h->mallocs.push_back(q);
+  *p = (T *)q;
+  return 0;
+}
+}  // namespace
+}  // extern "C++"
+
+int tdgl_meter_begin(tdgl_handle *h, const tdgl_meter *m) {
+  if (!h || !m || !m->Min || !m->Mou || !m->line_ok || !m->line_f1) return fail(TDGL_E_ARG, "null handle or meter");
+  const tdgl_desc &d = h->d;
+  const Geo &g = h->g;
+  if (m->vi < d.x1 || m->vo > d.x2 || m->vi >= m->vo) return fail(TDGL_E_ARG, "voltmeter planes %d, %d outside x1..x2", m->vi, m->vo);
+  HIPCHK(hipStreamSynchronize(h->s));
+  for (void *p : h->mallocs) (void)hipFree(p);
+  h->mallocs.clear();
+  h->meter = false;
+  MeterArgs a{};
+  a.g = g;
+  a.N = d.nbands;
+  a.vi = m->vi;
+  a.vo = m->vo;
+  a.nl1 = d.y2 - d.y1 + 2;
+  a.nl2 = d.z2 - d.z1 + 2;
+  const int nl = a.nl1 * a.nl2;
+  for (int q = 0; q < nl; ++q) {
+    a.nin += m->Min[q] != 0;
+    a.nou += m->Mou[q] != 0;
+    a.nlin += m->line_ok[q] != 0;
+    a.nlin_f1 += m->line_f1[q] != 0;
+  }
+  if (a.nlin == 0) return fail(TDGL_E_ARG, "no complete line between the voltmeter planes");
+  a.mid = (d.Nx + 1) / 2;  // oround(Nx / 2)
+  a.ks2 = m->kappa2;
+  a.dy = d.dy;
+  a.dz = d.dz;
+  const int nb = (d.x2 - d.x1 + 2) * nl;
+  a.nlb = cdiv(nl, 4);
+  a.nnb = cdiv(nb, 256);
+  a.nhb = cdiv(d.Nx * d.Ny * (d.Nz + 1), 256);
+  unsigned char *mk[4];
+  const uint8_t *src[4] = {m->Min, m->Mou, m->line_ok, m->line_f1};
+  for (int q = 0; q < 4; ++q) {
+    CHK(malloc_meter(h, &mk[q], (size_t)nl));
+    HIPCHK(hipMemcpy(mk[q], src[q], (size_t)nl, hipMemcpyHostToDevice));
+  }
+  a.Min = mk[0];
+  a.Mou = mk[1];
+  a.lok = mk[2];
+  a.lf1 = mk[3];
+  for (int mu = 0; mu < d.nbands; ++mu) {
+    CHK(malloc_meter(h, &a.Dprev[mu], (size_t)nl));
+    CHK(malloc_meter(h, &a.T[mu], (size_t)nl));
+    CHK(malloc_meter(h, &a.cooper[mu], (size_t)nb));
+    HIPCHK(hipMemset(a.Dprev[mu], 0, nl * sizeof(double)));
+    HIPCHK(hipMemset(a.T[mu], 0, nl * sizeof(double)));
+  }
+  CHK(malloc_meter(h, &a.hold, (size_t)g.n));
+  CHK(malloc_meter(h, &a.part, (size_t)meter_blocks(a) * METER_Q));
+  CHK(malloc_meter(h, &h->mrec, (size_t)MR_N));
+  CHK(malloc_meter(h, &h->mrung, 2));
+  HIPCHK(hipMemset(h->mrung, 0, 2 * sizeof(double)));
+  h->ma = a;
+  h->meter = true;
+  // Cooper0 and fieldOld of the current state
+  MeterArgs b = meter_args(h, 0);
+  b.init = 1;
+  LCHK(launch_meter(b, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+  return 0;
+}
+
+int tdgl_meter_cycle(tdgl_handle *h, int32_t first, tdgl_cycle_record *out) {
+  if (!h || !out) return fail(TDGL_E_ARG, "null handle or record");
+  if (!h->meter) return fail(TDGL_E_ARG, "tdgl_meter_begin has not been called");
+  const MeterArgs a = meter_args(h, first);
+  PL(K_METER, launch_meter(a, h->s));
+  PL(K_METER, launch_meter_finish(a, h->mrec, h->mrung, h->s));
+  double r[MR_N];
+  HIPCHK(hipMemcpyAsync(r, h->mrec, sizeof r, hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+  if (h->prof) CHK(prof_flush(h));
+  *out = tdgl_cycle_record{};
+  out->V = r[MR_V];
+  out->TA = r[MR_TA];
+  out->Tf1 = r[MR_TF1];
+  for (int mu = 0; mu < TDGL_MAX_BANDS; ++mu) {
+    out->T[mu] = r[MR_T + mu];
+    out->Tabs[mu] = r[MR_TABS + mu];
+    out->dif[mu] = r[MR_DIF + mu];
+  }
+  out->dmax = r[MR_DMAX];
+  out->nbig = (int32_t)r[MR_NBIG];
+  out->It = r[MR_IT];
+  out->Is = r[MR_IS];
+  out->amax = r[MR_AMAX];
+  out->diffh = r[MR_DIFFH];
   return 0;
 }
 

@@ -9,7 +9,7 @@ import threading
 
 import numpy as np
 
-from .abi import NKERNELS, OpenC, PackedDesc, PcgState, ProfileC, StatsC, state_struct
+from .abi import NKERNELS, CycleRecordC, MeterC, OpenC, PackedDesc, PcgState, ProfileC, StatsC, state_struct
 from .scheme import Stepper
 from .state import State
 
@@ -18,7 +18,8 @@ LIB = os.path.join(HERE, "_lib", "libtdglhip.so")
 
 SYMBOLS = ("tdgl_supported", "tdgl_create", "tdgl_destroy", "tdgl_upload_state", "tdgl_download_state", "tdgl_step",
            "tdgl_get_pcg", "tdgl_set_pcg", "tdgl_get_open", "tdgl_set_open", "tdgl_get_stats", "tdgl_last_error", "tdgl_device_count",
-           "tdgl_set_device", "tdgl_profile_enable", "tdgl_profile_get", "tdgl_kernel_name")
+           "tdgl_set_device", "tdgl_profile_enable", "tdgl_profile_get", "tdgl_kernel_name", "tdgl_meter_begin",
+           "tdgl_meter_cycle")
 
 _lib = None
 _lock = threading.Lock()
@@ -53,6 +54,8 @@ def load():
             L.tdgl_profile_enable.argtypes = [vp, ct.c_int32]
             L.tdgl_profile_get.argtypes = [vp, vp]
             L.tdgl_kernel_name.argtypes = [ct.c_int32]
+            L.tdgl_meter_begin.argtypes = [vp, vp]
+            L.tdgl_meter_cycle.argtypes = [vp, ct.c_int32, vp]
             L.tdgl_kernel_name.restype = ct.c_char_p
             _lib = L
         return _lib
@@ -149,6 +152,26 @@ class HipStepper:
             c.hold[k] = old[k].ctypes.data_as(ct.POINTER(ct.c_double))
             c.hnew[k] = new[k].ctypes.data_as(ct.POINTER(ct.c_double))
         _check(load().tdgl_set_open(self.h, ct.byref(c)), "tdgl_set_open")
+
+    def meter_begin(self, vm) -> None:
+        """Uploads the voltmeter `vm` (juliatdgl_amd.meter.Voltmeter) and takes |Ψ| and hz of the
+        state on the device as the last cycle's (tdgl_meter_begin)."""
+        self._vm = [np.ascontiguousarray(a.ravel(order="F"), dtype=np.uint8)
+                    for a in (vm.Min, vm.Mou, vm.line_ok, vm.line_f1)]
+        c = MeterC()
+        c.vi, c.vo, c.kappa2 = int(vm.vi), int(vm.vo), float(vm.kappa2)
+        c.Min, c.Mou, c.line_ok, c.line_f1 = [a.ctypes.data_as(ct.POINTER(ct.c_uint8)) for a in self._vm]
+        _check(load().tdgl_meter_begin(self.h, ct.byref(c)), "tdgl_meter_begin")
+
+    def meter_cycle(self, first: bool) -> dict:
+        """The record of the cycle that just ended (tdgl_meter_cycle); `first` starts a rung."""
+        r = CycleRecordC()
+        _check(load().tdgl_meter_cycle(self.h, int(first), ct.byref(r)), "tdgl_meter_cycle")
+        N = self.st.N
+        return {"V": r.V, "TA": r.TA, "Tf1": r.Tf1, "T": [r.T[k] for k in range(N)],
+                "Tabs": [r.Tabs[k] for k in range(N)], "dmax": r.dmax,
+                "nbig": int(r.nbig), "It": r.It, "Is": r.Is, "dif": [r.dif[k] for k in range(N)], "amax": r.amax,
+                "diffh": r.diffh}
 
     def profile(self, on: bool) -> None:
         """Per-kernel HIP-event timing of every launch while on (clears the totals)."""

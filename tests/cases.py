@@ -103,6 +103,7 @@ def compare_states(sg, s):
     out = {}
     for mu in range(s.N):
         out[f"G{mu + 1}"] = maxrel(sg.G[mu], s.G[mu])
+        out[f"F{mu + 1}"] = maxrel(sg.F[mu], s.F[mu])
     for qs in (("thx", "thy", "thz"), ("hx", "hy", "hz"), ("Jx", "Jy", "Jz"), ("phx", "phy", "phz")):
         m = max(np.max(np.abs(getattr(s, q))) for q in qs)
         for q in qs:

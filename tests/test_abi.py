@@ -32,14 +32,17 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layout_matches_header(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "tdglhip.h"\n'
-                   'int main(){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(tdgl_desc), sizeof(tdgl_state),'
-                   ' sizeof(tdgl_pcg), sizeof(tdgl_stats), offsetof(tdgl_desc, f), offsetof(tdgl_desc, pcg_tol));}\n')
+                   'int main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(tdgl_desc), sizeof(tdgl_state),'
+                   ' sizeof(tdgl_pcg), sizeof(tdgl_stats), offsetof(tdgl_desc, f), offsetof(tdgl_desc, pcg_tol),'
+                   ' sizeof(tdgl_meter), sizeof(tdgl_cycle_record), offsetof(tdgl_cycle_record, It),'
+                   ' offsetof(tdgl_cycle_record, diffh));}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     got = [int(x) for x in out]
     want = [ct.sizeof(abi.DescC), ct.sizeof(abi.StateC), ct.sizeof(abi.PcgC), ct.sizeof(abi.StatsC),
-            abi.DescC.f.offset, abi.DescC.pcg_tol.offset]
+            abi.DescC.f.offset, abi.DescC.pcg_tol.offset, ct.sizeof(abi.MeterC), ct.sizeof(abi.CycleRecordC),
+            abi.CycleRecordC.It.offset, abi.CycleRecordC.diffh.offset]
     assert got == want
 
 

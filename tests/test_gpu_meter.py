@@ -1,0 +1,129 @@
+"""The device meter (tdgl_meter_begin / tdgl_meter_cycle) against the oracle's restatement of a
+ladder's per-cycle measurements (oracle/meter.py), and the host logic of juliatdgl_amd.meter on the
+device against the reference's checks (gtdgl/test/test_protocol.jl:153-163).  Needs a HIP device."""
+import math
+
+import numpy as np
+import pytest
+
+from juliatdgl_amd.meter import Protocol, Voltmeter, crossing_times, run_rung
+from juliatdgl_amd.scheme import Grid, make_stepper, two_band, wedge_masks
+from juliatdgl_amd.state import initial_state
+from cases import make_case, seed_state
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("V", "TA", "Tf1", "It", "Is", "amax", "diffh", "dmax")
+
+
+def _hip():
+    from juliatdgl_amd.hip import HipStepper, device_count
+    if device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return HipStepper
+
+
+def _close(a, b, tol):
+    for k in KEYS:
+        assert a[k] == pytest.approx(b[k], rel=tol, abs=tol), (k, a[k], b[k])
+    for k in ("T", "Tabs", "dif"):
+        assert np.allclose(a[k], b[k], rtol=tol, atol=tol), (k, a[k], b[k])
+    assert a["nbig"] == b["nbig"]
+
+
+@pytest.mark.parametrize("name", ["production", "kwt10_pcg", "three_aniso"])
+def test_device_meter_matches_the_oracle_meter_on_the_same_states(name):
+    """Cycle by cycle, the record of the device equals the oracle's measurements of the state the
+    device holds (downloaded after each cycle), to the rounding of the sums' order."""
+    from oracle.meter import OracleMeter
+    HipStepper = _hip()
+    st, _ = make_case(name)
+    s0, pcg, H, HJ = seed_state(st)
+    vm = Voltmeter.of_ladder(st, 3)
+    print(f"[gpu] meter on {name}", flush=True)
+    dev = HipStepper(st)
+    dev.upload(s0)
+    if pcg is not None:
+        dev.set_pcg(pcg)
+    dev.meter_begin(vm)
+    om = OracleMeter(st, vm, dev.download())
+    for m in range(1, 6):
+        dev.step(8, H, HJ)
+        rd = dev.meter_cycle(m == 1)
+        ro = om.cycle(dev.download(), m == 1)
+        _close(rd, ro, 1e-12)
+        assert math.isfinite(rd["amax"]) and rd["amax"] > 0.1
+    assert abs(rd["It"]) > 0 and rd["diffh"] > 0
+
+
+def test_device_meter_follows_the_oracle_run():
+    """The device stepping and measuring against the oracle stepping and measuring: the records
+    of the two runs agree to the parity of the step."""
+    from oracle.meter import OracleBackend
+    from oracle.oracle import Oracle
+    HipStepper = _hip()
+    st, tol = make_case("production")
+    s0, pcg, H, HJ = seed_state(st)
+    vm = Voltmeter.of_ladder(st, 3)
+    o = Oracle(st)
+    o.set_state(s0)
+    ob = OracleBackend(o, st)
+    dev = HipStepper(st)
+    dev.upload(s0)
+    for b in (ob, dev):
+        b.meter_begin(vm)
+    for m in range(1, 5):
+        rs = []
+        for b in (ob, dev):
+            b.step(10, H, HJ)
+            rs.append(b.meter_cycle(m == 1))
+        _close(rs[1], rs[0], 1e-9)
+
+
+def test_device_crossing_times_matches_the_reference_test():
+    """No crossing far below the onset, a regular train far above (test_protocol.jl:153-163), with
+    the measurements on the device every 10 steps."""
+    HipStepper = _hip()
+    md = two_band(xi_ratio=4.0, kappa=25.0)
+    gr = Grid.make(24, 12, 4, dx=0.5)
+    mk = wedge_masks(gr)
+    st = make_stepper(md, gr, mk)
+    print("[gpu] crossing_times below and above the onset", flush=True)
+    dev = HipStepper(st)
+    dev.upload(initial_state(md, gr, mk, 0.03))
+    times, t, div = crossing_times(dev, st, 0.5, 0.03, 150.0, delta_v=6)
+    assert times == [] and not div and abs(t - 150.0) < 0.5
+    dev = HipStepper(st)
+    dev.upload(initial_state(md, gr, mk, 0.03))
+    times, t, div = crossing_times(dev, st, 1.4, 0.03, 600.0, delta_v=6, stop=4)
+    assert len(times) == 4 and times == sorted(times) and t < 600.0 and not div
+
+
+def test_device_rung_closes_as_the_oracle_rung():
+    """One ladder point at a current below and one above the onset (the κ = 25 box): the device
+    and the oracle close them with the same verdict after the same number of cycles."""
+    from oracle.meter import OracleBackend
+    from oracle.oracle import Oracle
+    HipStepper = _hip()
+    md = two_band(xi_ratio=4.0, kappa=25.0)
+    gr = Grid.make(24, 12, 4, dx=0.5)
+    mk = wedge_masks(gr)
+    st = make_stepper(md, gr, mk)
+    pr = Protocol(t_min_GL=20.0, t_min_GL_res=60.0, mMax=400, delta_v=6)
+    vm = Voltmeter.of_ladder(st, pr.delta_v)
+    for J in (0.5, 1.4):
+        print(f"[gpu] rung at J = {J}", flush=True)
+        o = Oracle(st)
+        o.set_state(initial_state(md, gr, mk, 0.03))
+        ob = OracleBackend(o, st)
+        dev = HipStepper(st)
+        dev.upload(initial_state(md, gr, mk, 0.03))
+        out = []
+        for b in (ob, dev):
+            b.meter_begin(vm)
+            out.append(run_rung(b, st, pr, J, 0.03))
+        ro, rd = out
+        print(f"    oracle {ro.conv} m={ro.m}, device {rd.conv} m={rd.m}", flush=True)
+        assert rd.conv == ro.conv and rd.m == ro.m
+        assert rd.conv == ("F" if J < 1 else "V")
+        assert np.allclose(rd.V, ro.V, rtol=1e-8, atol=1e-8)
