@@ -17,6 +17,7 @@ using TDGL
 using TDGL: State, Stepper, SepSolver, PCGSolver
 
 export GPUStepper, GState, device_state, device_state!, host_state!, gpu_step!, gpu_steps!, supported, enable!,
+       meter_begin!, meter_cycle,
        disable!, execution_stats
 
 const MAXB = 4
@@ -254,6 +255,62 @@ function _pcg_down!(S::PCGSolver, gs::GPUStepper)
     f === nothing || (f.have2[1] = p[].have2 != 0)
     S.iters[1] = p[].iters
     S
+end
+
+# ---- the ladder's per-cycle measurements on the device (protocol.jl:341-352, 416-464) -------------
+struct MeterC
+    vi::Int32
+    vo::Int32
+    kappa2::Float64
+    Min::Ptr{UInt8}
+    Mou::Ptr{UInt8}
+    line_ok::Ptr{UInt8}
+    line_f1::Ptr{UInt8}
+end
+
+struct CycleRecordC
+    V::Float64
+    TA::Float64
+    Tf1::Float64
+    T::NTuple{MAXB,Float64}
+    Tabs::NTuple{MAXB,Float64}
+    dmax::Float64
+    nbig::Int32
+    pad::Int32
+    It::Float64
+    Is::Float64
+    dif::NTuple{MAXB,Float64}
+    amax::Float64
+    diffh::Float64
+end
+
+"""
+    meter_begin!(gs, st, delta_v) -> masks
+
+The voltmeter of `ladder` (protocol.jl:341-352) on the device, and |Ψ| and hz of the device state as
+the last cycle's (Cooper0, fieldOld).  Keep the returned masks alive while the meter is used.
+"""
+function meter_begin!(gs::GPUStepper, st::Stepper, delta_v::Integer)
+    gr = st.grid; mk = st.masks
+    (; x1, x2, y1, y2, z1, z2) = gr
+    vi = x1 + delta_v; vo = x2 - delta_v
+    Min = UInt8[mk.τx[vi,j,k] > 0.5 for j in y1:y2+1, k in z1:z2+1]
+    Mou = UInt8[mk.τx[vo,j,k] > 0.5 for j in y1:y2+1, k in z1:z2+1]
+    ok = UInt8[all(mk.τx[ii,j,k] > 0.5 for ii in vi:vo-1) for j in y1:y2+1, k in z1:z2+1]
+    f1 = UInt8[ok[a,b] != 0 && all(mk.f[ii, y1+a-1, z1+b-1] >= 1 - 1e-12 for ii in vi:vo)
+               for a in axes(ok, 1), b in axes(ok, 2)]
+    m = Ref(MeterC(vi, vo, st.model.κ^2, pointer(Min), pointer(Mou), pointer(ok), pointer(f1)))
+    GC.@preserve Min Mou ok f1 _check(ccall((:tdgl_meter_begin, libpath()), Cint, (Ptr{Cvoid}, Ptr{MeterC}),
+                                            gs.h, m), "tdgl_meter_begin")
+    (Min, Mou, ok, f1)
+end
+
+"""meter_cycle(gs, first) -> CycleRecordC: the record of the cycle that just ended; `first` at m == 1."""
+function meter_cycle(gs::GPUStepper, first::Bool)
+    r = Ref{CycleRecordC}()
+    _check(ccall((:tdgl_meter_cycle, libpath()), Cint, (Ptr{Cvoid}, Int32, Ptr{CycleRecordC}), gs.h, Int32(first), r),
+           "tdgl_meter_cycle")
+    r[]
 end
 
 # ---- the ladder hook (TDGLGPU.jl:2076-2144) -----------------------------------------------------
