@@ -1601,11 +1601,16 @@ template bool plane_fits<double>(const int e[][3], int);
 // (X at (mx-1, my), Y at (mx, my-1), Z at (mx, my)), projects them and leaves the projected modes
 // in LDS; then thread (tc, n) takes the inverse z transform of those columns at z index n.  Every
 // projected mode of a column comes from the same block, so no other launch is needed between.
-template <class T>
+// The columns are staged in LDS first (thread (tc, n) loads z index n of its column), and with
+// LB the four z bases too, so every global load is issued before the first wait; the sums run in
+// the same order as from global memory.
+template <class T, bool LB>
 __global__ void __launch_bounds__(256) k_zmodes(ZModes m, const int *gate) {
   if (gate && !*gate) return;
-  __shared__ T Mx[256], My[256], Mz[256];
+  extern __shared__ __align__(16) unsigned char zm_smem[];
   const int nx = m.nx, ny = m.ny, nz = m.nz;
+  T *Cx = (T *)zm_smem, *Cy = Cx + 256, *Cz = Cy + 256, *Mx = Cz + 256, *My = Mx + 256, *Mz = My + 256;
+  T *Lav = Mz + 256, *Lbv = Lav + nz * nz, *Lau = Lbv + nz * nz, *Lbu = Lau + (nz - 1) * (nz - 1);
   const int C = 256 / nz;
   const int t = threadIdx.x;
   const int tc = t / nz, mz = t % nz;
@@ -1614,26 +1619,47 @@ __global__ void __launch_bounds__(256) k_zmodes(ZModes m, const int *gate) {
   const int mx = live ? col % nx : 0, my = live ? col / nx : 0;
   const bool hx = mx > 0, hy = my > 0, hz = mz > 0;
   const T *in = (const T *)m.in;
+  const int stx = (nx - 1) * ny, sty = nx * (ny - 1), stz = nx * ny;
   if (live) {
-    const T *AV = (const T *)m.Az[0], *AU = (const T *)m.Az[1];
-    const T k1 = ((const T *)m.sx)[mx], k2 = ((const T *)m.sy)[my], k3 = ((const T *)m.sz)[mz];
+    // z index mz of the three columns (clamped addresses, the padding selected after)
     const T *colx = in + m.off[0] + (hx ? mx - 1 : 0) + (nx - 1) * my;
     const T *coly = in + m.off[1] + mx + nx * (hy ? my - 1 : 0);
     const T *colz = in + m.off[2] + mx + nx * my;
-    const int stx = (nx - 1) * ny, sty = nx * (ny - 1), stz = nx * ny;
+    const T vx = colx[stx * mz], vy = coly[sty * mz], vz = colz[stz * min(mz, nz - 2)];
+    Cx[t] = vx;
+    Cy[t] = vy;
+    Cz[t] = mz < nz - 1 ? vz : T(0);
+  }
+  if (LB) {
+    const T *AV = (const T *)m.Az[0], *AU = (const T *)m.Az[1], *BV = (const T *)m.Ai[0], *BU = (const T *)m.Ai[1];
+    for (int e = t; e < nz * nz; e += 256) {
+      Lav[e] = AV[e];
+      Lbv[e] = BV[e];
+    }
+    for (int e = t; e < (nz - 1) * (nz - 1); e += 256) {
+      Lau[e] = AU[e];
+      Lbu[e] = BU[e];
+    }
+  }
+  __syncthreads();
+  const T *AV = LB ? Lav : (const T *)m.Az[0], *AU = LB ? Lau : (const T *)m.Az[1];
+  const T *BV = LB ? Lbv : (const T *)m.Ai[0], *BU = LB ? Lbu : (const T *)m.Ai[1];
+  const int b0 = tc * nz;
+  if (live) {
+    const T k1 = ((const T *)m.sx)[mx], k2 = ((const T *)m.sy)[my], k3 = ((const T *)m.sz)[mz];
     const int mzu = hz ? mz - 1 : 0;
     T ax = 0, ay = 0, az = 0;
-#pragma unroll 16
+#pragma unroll 8
     for (int n = 0; n < nz - 1; ++n) {
       const T v = AV[n * nz + mz];
-      ax += colx[stx * n] * v;
-      ay += coly[sty * n] * v;
-      az += colz[stz * n] * AU[n * (nz - 1) + mzu];
+      ax += Cx[b0 + n] * v;
+      ay += Cy[b0 + n] * v;
+      az += Cz[b0 + n] * AU[n * (nz - 1) + mzu];
     }
     {
       const T v = AV[(nz - 1) * nz + mz];
-      ax += colx[stx * (nz - 1)] * v;
-      ay += coly[sty * (nz - 1)] * v;
+      ax += Cx[b0 + nz - 1] * v;
+      ay += Cy[b0 + nz - 1] * v;
     }
     T x1 = hx ? ax : T(0), y1 = hy ? ay : T(0), z1 = hz ? az : T(0);
     const T k2s = k1 * k1 + k2 * k2 + k3 * k3;
@@ -1645,8 +1671,6 @@ __global__ void __launch_bounds__(256) k_zmodes(ZModes m, const int *gate) {
   __syncthreads();
   if (!live) return;
   const int n = mz;  // the z index this thread writes
-  const T *BV = (const T *)m.Ai[0], *BU = (const T *)m.Ai[1];
-  const int b0 = tc * nz;
   T ox = 0, oy = 0, oz = 0;
 #pragma unroll 8
   for (int q = 0; q < nz; ++q) {
@@ -1663,11 +1687,17 @@ __global__ void __launch_bounds__(256) k_zmodes(ZModes m, const int *gate) {
   if (n < nz - 1) out[m.off[2] + mx + nx * (my + ny * n)] = oz;
 }
 
+constexpr int ZM_LB_MAX = 64;  // z bases in LDS up to this nz
 template <class T>
 hipError_t launch_zmodes(const ZModes &m, const int *gate, hipStream_t s) {
   if (m.nz < 2 || m.nz > 256 || m.nx < 1 || m.ny < 1) return hipErrorInvalidValue;
   const int C = 256 / m.nz;
-  TDGL_LAUNCH((k_zmodes<T>), dim3(cdiv(m.nx * m.ny, C)), dim3(256), 0, s, m, gate);
+  const bool lb = m.nz <= ZM_LB_MAX;
+  const size_t lds = (6 * 256 + (lb ? 2 * (size_t)m.nz * m.nz + 2 * (size_t)(m.nz - 1) * (m.nz - 1) : 0)) * sizeof(T);
+  if (lb)
+    TDGL_LAUNCH((k_zmodes<T, true>), dim3(cdiv(m.nx * m.ny, C)), dim3(256), lds, s, m, gate);
+  else
+    TDGL_LAUNCH((k_zmodes<T, false>), dim3(cdiv(m.nx * m.ny, C)), dim3(256), lds, s, m, gate);
   return hipGetLastError();
 }
 template hipError_t launch_zmodes<float>(const ZModes &, const int *, hipStream_t);
