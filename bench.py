@@ -141,7 +141,8 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed ladder cycles")
     ap.add_argument("--warmup", type=int, default=2, help="untimed ladder cycles")
-    ap.add_argument("--workload", default="kwt_pcg_128x64x16", choices=("kwt_pcg_128x64x16", "open_3band_256x128x32"))
+    ap.add_argument("--workload", default="kwt_pcg_128x64x16",
+                    choices=("kwt_pcg_128x64x16", "open_3band_256x128x32", "pcg_wedge_96x48x24", "plain_wedge_96x48x24"))
     ap.add_argument("--cpu-steps", type=int, default=8, help="time steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-out", default=None, help="write the per-kernel profile (JSON) here")
@@ -165,8 +166,23 @@ def main(argv=None):
         workload = ("three-band anisotropic (m_z = 4 m_xy) TDGL, 256x128x32 cells, dx=0.5, buf 8 insulating "
                     "(sigma_vac/sigma_n=0.01), PCG tol 1e-8 (Float32 preconditioner, FP64 outer), open boundary "
                     "(FFT convolution, every t_GL), notch, H=0.2, J=0.1, ladder cycle = 40 steps")
+    elif args.workload == "pcg_wedge_96x48x24":
+        st, p = configs.pcg_wedge_96x48x24()
+        workload = ("the reference's validation run file pcg96.toml: mgb2_model_II at kappa=2, 96x48x24 cells, dx=0.5, "
+                    "buf 2 insulating (sigma_vac=0.01), PCG tol 1e-8 (Float32 preconditioner, FP64 outer), WEDGE, "
+                    "J=0.3, H=0.03, ladder cycle = 40 steps")
+    elif args.workload == "plain_wedge_96x48x24":
+        st, p = configs.plain_wedge_96x48x24()
+        workload = ("the reference's validation run file plain96.toml: mgb2_model_II at kappa=2, 96x48x24 cells, "
+                    "dx=0.5, buf 1, separable implicit field solve, WEDGE, J=0.3, H=0.03, ladder cycle = 40 steps")
     else:
         raise SystemExit(f"unknown workload {args.workload}")
+    # the reference's published T4 figures for these run files (ms per time step, median of three
+    # batches of 20 steps; gpu/TDGLGPU/validation/README.md:41-71)
+    published = {"pcg_wedge_96x48x24": {"t4_batch_ms_per_step": 26.903, "t4_cycle_ms_per_step": 21.040,
+                                        "laptop_1core_ms_per_step": 146.370},
+                 "plain_wedge_96x48x24": {"t4_batch_ms_per_step": 7.295, "t4_cycle_ms_per_step": 4.816,
+                                          "laptop_1core_ms_per_step": 22.574}}.get(args.workload)
     if args.copy_probe:
         copy_probe(args.copy_probe)
     H, HJ = p["H"], st.selffield(p["J"])
@@ -221,9 +237,11 @@ def main(argv=None):
         roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                 "frac": flops / avg_s / peak, "traffic": None, "kernel": dom, "flops_per_launch": flops,
                 "bytes_per_launch": byts, "avg_launch_us": avg_s * 1e6,
-                "note": "fused x-y transform launch of the Float32 preconditioner on v_mfma_f32_16x16x4f32 "
-                        "(2 (Lx + Ly) flop per element); durations from HIP events around every launch "
-                        "over the same number of cycles as the timed region"}
+                "note": ("fused x-y transform launch of the Float32 preconditioner on v_mfma_f32_16x16x4f32"
+                         if dom == "sep_transform_f32" else
+                         "fused x-y transform launch of the FP64 separable solve on the vector ALUs")
+                        + " (2 (Lx + Ly) flop per element); durations from HIP events recorded at each launch's "
+                          "start and end over the same number of cycles as the timed region"}
     traffic = measured_traffic(dom, args.workload)
     roof["traffic"] = traffic
     if args.profile_out:
@@ -243,11 +261,16 @@ def main(argv=None):
     hbm_cycle = balg * nodes * LMAX * args.steps / wall / HBM_PEAK
     out = {"metric": "cell-cycles/s", "value": value, "unit": "cell-cycles/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded psi = 1 + 0.01 (n1 + i n2), 30-degree wedge mask)",
+           "scaling": "weak",
+           "vs_baseline": (value / (st.grid.cells / (published["t4_cycle_ms_per_step"] * 1e-3) / LMAX)) if published else None, "dtype": "f64",
+           "data": "synthetic (seeded psi = 1 + 0.01 (n1 + i n2)), " + {"kwt_pcg_128x64x16": "30-degree sloped wedge mask",
+                                                                         "open_3band_256x128x32": "notch mask"}.get(
+               args.workload, "catalogue WEDGE mask"),
            "config": {"workload": workload, "cells": cells, "lmax": LMAX, "pcg_iters_per_step": iters / nsteps,
                       "device_ms_per_cycle": dev_ms / args.steps, "padded_nodes": nodes,
                       "cell_steps_per_s": value * LMAX, "b_alg_per_node_step": balg,
-                      "hbm_roofline_frac_cycle": hbm_cycle},
+                      "hbm_roofline_frac_cycle": hbm_cycle, "ms_per_time_step": 1e3 * wall / nsteps,
+                      **({"reference_published": published} if published else {})},
            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out))
     return 0

@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import math
 
-from .scheme import Band, Grid, Model, make_stepper, notch_masks, sloped_wedge_masks
+from .scheme import Band, Grid, Model, make_stepper, notch_masks, sloped_wedge_masks, two_band, wedge_masks
 
 
 def two_band_model(kappa=2.5, Gamma=1.0, gkwt=0.0):
@@ -65,3 +65,23 @@ def open_3band_256x128x32(nx=256, ny=128, nz=32, buf=8, pcg_tol=1e-8, selffield_
     mk = notch_masks(gr, (nx // 2 - 4, nx // 2 + 4), (0, ny // 4))
     st = make_stepper(md, gr, mk, sigma_vac=0.01, pcg_tol=pcg_tol, selffield_bc=selffield_bc)
     return st, dict(H=0.2, J=0.1)
+
+
+def pcg_wedge_96x48x24():
+    """The reference's validation run file gpu/TDGLGPU/validation/configs/pcg96.toml: material
+    mgb2_model_II (σ band and a π band of ξ ratio 4, a = 0.59, γ12 = 1e-3) at κ = 2, 96x48x24
+    cells, dx 0.5, buffer 2 with σvac = 0.01 (the insulating PCG with the scheme defaults: tol
+    1e-8, extrapolated first guess, Float32 preconditioner), the catalogue WEDGE, J = 0.3, H = 0.03."""
+    md = two_band(xi_ratio=4.0, kappa=2.0)
+    gr = Grid.make(96, 48, 24, dx=0.5, buf=2)
+    st = make_stepper(md, gr, wedge_masks(gr), sigma_vac=0.01)
+    return st, dict(H=0.03, J=0.3)
+
+
+def plain_wedge_96x48x24():
+    """validation/configs/plain96.toml: as pcg_wedge_96x48x24 with a one-cell normal-metal buffer
+    (the separable implicit solve)."""
+    md = two_band(xi_ratio=4.0, kappa=2.0)
+    gr = Grid.make(96, 48, 24, dx=0.5, buf=1)
+    st = make_stepper(md, gr, wedge_masks(gr))
+    return st, dict(H=0.03, J=0.3)

@@ -359,6 +359,27 @@ hipError_t launch_pcg_start(const PcgArgs &a, hipStream_t s);    // nb, warm sta
 hipError_t launch_pcg_resid(const PcgArgs &a, hipStream_t s);    // Q = A x, r = b − Q, r·r
 hipError_t launch_pcg_p(const PcgArgs &a, hipStream_t s);        // β, P = z + βP, zold = z
 hipError_t launch_pcg_ap(const PcgArgs &a, hipStream_t s);       // Q = A P, p·Ap
+// Raw buffer loads: the base in scalar registers and a 32-bit byte offset per lane (one address
+// register per load instead of two); the hardware returns 0 beyond `bytes`.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, bytes, 0x00020000);
+}
+template <class T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int off);
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ double2 buf_load_d2(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+// with a scalar (wave-uniform) byte offset added
+template <class T> __device__ __forceinline__ T buf_load_s(__amdgpu_buffer_rsrc_t r, int off, int soff);
+template <> __device__ __forceinline__ double buf_load_s<double>(__amdgpu_buffer_rsrc_t r, int off, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+}
+
 // ---------------------------------------------------------------- the ladder's per-cycle measurements
 // partial slots of a line block (and, from 0, of the other blocks' partials)
 enum { METER_PIN = 0, METER_POUT, METER_TA, METER_TF1, METER_DMAX, METER_NBIG, METER_T,

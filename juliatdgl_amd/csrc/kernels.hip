@@ -1210,19 +1210,6 @@ __device__ __forceinline__ int xcd_run(int hw, int nb) {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + s;
 }
 
-// Raw buffer loads: the base in scalar registers and a 32-bit byte offset per lane (one address
-// register per load instead of two); the hardware returns 0 beyond `bytes`.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, int bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, bytes, 0x00020000);
-}
-template <class T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int off);
-template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int off) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int off) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
-
 // sum_parts in a block of eight wavefronts: the first four load and sum as a block of RED_T
 // threads does (the same value, bit for bit, as every other reader of the partials)
 __device__ __forceinline__ double sum_parts_8w(const double *part, int n, int which, double *sh, int tid) {
@@ -1834,6 +1821,21 @@ __device__ __forceinline__ void gather24(const double *P0, const double *P1, con
   v[P2C] = P2[c], v[P2YP] = P2[c + sy], v[P2ZMYP] = P2[c - sz + sy], v[P2ZM] = P2[c - sz], v[P2YM] = P2[c - sy];
   v[P2XP] = P2[c + 1], v[P2ZMXP] = P2[c - sz + 1], v[P2XM] = P2[c - 1];
 }
+// The same gather as raw buffer loads: one 32-bit lane offset (c) shared by the 24 loads, the
+// stencil offsets in scalar registers, the base K elements before the arrays so every offset is
+// non-negative (K = sz + sy + 1; c - sz >= 0 for the box's nodes).  Node arrays of n elements.
+__device__ __forceinline__ void gather24b(const double *P0, const double *P1, const double *P2, int c, int sy, int sz,
+                                          int n, double v[NSTEN]) {
+  const int K = sz + sy + 1, vo = c * 8, bytes = (n + K) * 8;
+  const __amdgpu_buffer_rsrc_t r0 = buf_rsrc(P0 - K, bytes), r1 = buf_rsrc(P1 - K, bytes), r2 = buf_rsrc(P2 - K, bytes);
+  auto L = [&](__amdgpu_buffer_rsrc_t r, int d) { return buf_load_s<double>(r, vo, (K + d) * 8); };
+  v[P0C] = L(r0, 0), v[P0ZP] = L(r0, sz), v[P0ZM] = L(r0, -sz), v[P0XMZP] = L(r0, -1 + sz), v[P0XM] = L(r0, -1);
+  v[P0YP] = L(r0, sy), v[P0YM] = L(r0, -sy), v[P0XMYP] = L(r0, -1 + sy);
+  v[P1C] = L(r1, 0), v[P1ZP] = L(r1, sz), v[P1ZM] = L(r1, -sz), v[P1YMZP] = L(r1, -sy + sz), v[P1YM] = L(r1, -sy);
+  v[P1XP] = L(r1, 1), v[P1YMXP] = L(r1, -sy + 1), v[P1XM] = L(r1, -1);
+  v[P2C] = L(r2, 0), v[P2YP] = L(r2, sy), v[P2ZMYP] = L(r2, -sz + sy), v[P2ZM] = L(r2, -sz), v[P2YM] = L(r2, -sy);
+  v[P2XP] = L(r2, 1), v[P2ZMXP] = L(r2, -sz + 1), v[P2XM] = L(r2, -1);
+}
 __device__ __forceinline__ void apply3v(const PcgArgs &a, const double v[NSTEN], const double dv[3], int i, int j,
                                         int k, double q[3]) {
   const int Nx = a.g.Nx, Ny = a.g.Ny, Nz = a.g.Nz;
@@ -2003,12 +2005,34 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap(PcgArgs a) {
   const bool live = box_live(a.g, b);
   const int sy = a.g.px, sz = a.g.px * a.g.py;
   const int cl = box_load_c(a.g, b);
+  // the first partials of r·z and r·zold are requested before the stencil's loads (two per
+  // thread: the registers of the 48 stencil values in flight leave room for two), then summed
+  // exactly as sum_parts2 sums them
+  const int np = a.nparts_in;
+  const __amdgpu_buffer_rsrc_t rp = buf_rsrc(region(a.d, PART_RZ), 16 * np);
+  double2 pv[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) pv[u] = buf_load_d2(rp, 16 * min(tid + u * RED_T, np - 1));
   double vz[NSTEN], vp[NSTEN], dv[3];
-  gather24(a.z.c[0], a.z.c[1], a.z.c[2], cl, sy, sz, vz);
-  if (a.q > 0) gather24(a.Pold.c[0], a.Pold.c[1], a.Pold.c[2], cl, sy, sz, vp);
+  gather24b(a.z.c[0], a.z.c[1], a.z.c[2], cl, sy, sz, a.g.n, vz);
+  if (a.q > 0) gather24b(a.Pold.c[0], a.Pold.c[1], a.Pold.c[2], cl, sy, sz, a.g.n, vp);
   for (int comp = 0; comp < 3; ++comp) dv[comp] = a.D[comp][cl];
-  double rz2, rzo;
-  sum_parts2(region(a.d, PART_RZ), a.nparts_in, rz2, rzo, sh, tid);
+  double rz2 = 0.0, rzo = 0.0;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const bool in = tid + u * RED_T < np;
+    rz2 += in ? pv[u].x : 0.0;
+    rzo += in ? pv[u].y : 0.0;
+  }
+  for (int b0 = 2 * RED_T; b0 < np; b0 += 2 * RED_T)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = b0 + tid + u * RED_T;
+      const double2 v = buf_load_d2(rp, 16 * min(e, np - 1));
+      rz2 += e < np ? v.x : 0.0;
+      rzo += e < np ? v.y : 0.0;
+    }
+  block_sum2_t(rz2, rzo, sh, tid);
   double beta = 0.0;
   if (a.q > 0) {
     const double rz = a.d.sc[SC_RZ0 + ((a.q + 1) & 1)];
