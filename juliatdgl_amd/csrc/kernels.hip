@@ -1190,17 +1190,25 @@ __global__ void __launch_bounds__(256) k_plane_mf(PlanePass p, const int *gate) 
 // Blocks go to the XCDs in contiguous runs of the (m tile, plane, component) order, so one
 // plane's blocks share an L2.
 constexpr int PLANE1_T = 512;
-template <int XC, int TJ, int MT> struct PlaneMf1 {
+// LDS rows of 4 (mod 8) 32-bit words: E0 + 4 floats or E0 + 2 doubles
+template <int XC, int TJ, int MT, class TC> struct PlaneMf1 {
+  static constexpr int PADE = 16 / (int)sizeof(TC);
   static constexpr int E0 = 32 * XC, L1P = 32 * TJ, MS = 16 * MT;
-  static constexpr int E0S = E0 + 4, JS = L1P + 4;
+  static constexpr int E0S = E0 + PADE, JS = L1P + PADE;
   static constexpr int NJ = 2 * TJ, NT = MT * NJ, S = (NT + 7) / 8;  // j tiles, tiles, tiles per wave
   static constexpr int NB = E0 * L1P / PLANE1_T, NX = E0 * MS / PLANE1_T, NY = L1P * L1P / PLANE1_T;
   static constexpr int BROW = E0S > JS ? E0S : JS;
   // Xt [MS][E0S] | Bt [L1P][E0S], later Yt [L1P][JS] | St [MS][JS]
   static constexpr int BOFF = MS * E0S, SOFF = BOFF + L1P * BROW;
-  static constexpr size_t lds = (size_t)(SOFF + MS * JS) * sizeof(float);
+  static constexpr size_t lds = (size_t)(SOFF + MS * JS) * sizeof(TC);
   static_assert(NB + NX <= 63 && NY <= 63, "staging loads per thread");
 };
+static size_t plane1_lds(int xc, int tj, int mt, int es) {
+  const int pade = 16 / es, e0s = 32 * xc + pade, js = 32 * tj + pade, ms = 16 * mt;
+  const int brow = e0s > js ? e0s : js;
+  return (size_t)(ms * e0s + 32 * tj * brow + ms * js) * es;
+}
+constexpr size_t PLANE1_LDS_MAX = 160 * 1024;
 constexpr int PLANE1_XC = 5, PLANE1_TJ = 3;  // planes up to 160 x 96
 
 // the hardware block index to the block's place in (m tile, plane, component) order: the
@@ -1245,30 +1253,53 @@ __device__ __forceinline__ void block_sum2_8w(double &a, double &b, double *sh, 
   __syncthreads();
 }
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+// the 16x16x4 matrix instruction of each type: A lane l holds (m = l % 16, k = l / 16), B lane l
+// holds (k = l / 16, n = l % 16); the accumulator's element r of lane l is row mrow(l / 16, r),
+// column l % 16 (FP32: 4 (l / 16) + r; FP64: 4 r + l / 16)
+template <class TC> struct Mf;
+template <> struct Mf<float> {
+  typedef v4f v4;
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ __forceinline__ v4 ld4(const float *p) { return *(const v4f *)p; }
+  static __device__ __forceinline__ int mrow(int lk, int r) { return 4 * lk + r; }
+};
+template <> struct Mf<double> {
+  typedef v4d v4;
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static __device__ __forceinline__ v4 ld4(const double *p) {
+    const v2d lo = *(const v2d *)p, hi = *(const v2d *)(p + 2);
+    return v4d{lo.x, lo.y, hi.x, hi.y};
+  }
+  static __device__ __forceinline__ int mrow(int lk, int r) { return 4 * r + lk; }
+};
+
 // acc[s] += A_s' B_s over G groups of 16 rows for the first NS tiles of a wave (operands in LDS
 // at Ab + ao[s], Bb + bo[s], 16 rows further per group); the next group's operands are read
 // while the current group multiplies
-template <int NS, int S>
-__device__ __forceinline__ void mf_groups(v4f (&acc)[S], const float *Ab, const int (&ao)[S], const float *Bb,
+template <int NS, int S, class TC>
+__device__ __forceinline__ void mf_groups(typename Mf<TC>::v4 (&acc)[S], const TC *Ab, const int (&ao)[S], const TC *Bb,
                                           const int (&bo)[S], int G) {
-  v4f A[NS], B[NS];
+  typedef typename Mf<TC>::v4 V4;
+  V4 A[NS], B[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    A[s] = *(const v4f *)(Ab + ao[s]);
-    B[s] = *(const v4f *)(Bb + bo[s]);
+    A[s] = Mf<TC>::ld4(Ab + ao[s]);
+    B[s] = Mf<TC>::ld4(Bb + bo[s]);
   }
   for (int g = 0; g < G; ++g) {
     const int gn = min(g + 1, G - 1) * 16;
-    v4f An[NS], Bn[NS];
+    V4 An[NS], Bn[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      An[s] = *(const v4f *)(Ab + ao[s] + gn);
-      Bn[s] = *(const v4f *)(Bb + bo[s] + gn);
+      An[s] = Mf<TC>::ld4(Ab + ao[s] + gn);
+      Bn[s] = Mf<TC>::ld4(Bb + bo[s] + gn);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int s = 0; s < NS; ++s) acc[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[s][u], B[s][u], acc[s], 0, 0, 0);
+      for (int s = 0; s < NS; ++s) acc[s] = Mf<TC>::mma(A[s][u], B[s][u], acc[s]);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       A[s] = An[s];
@@ -1278,23 +1309,24 @@ __device__ __forceinline__ void mf_groups(v4f (&acc)[S], const float *Ab, const 
 }
 
 // mf_groups for the ns tiles of a wave (0 <= ns <= S, at least S - 2; wave-uniform)
-template <int S>
-__device__ __forceinline__ void mf_slots(int ns, v4f (&acc)[S], const float *Ab, const int (&ao)[S], const float *Bb,
-                                         const int (&bo)[S], int G) {
+template <int S, class TC>
+__device__ __forceinline__ void mf_slots(int ns, typename Mf<TC>::v4 (&acc)[S], const TC *Ab, const int (&ao)[S],
+                                         const TC *Bb, const int (&bo)[S], int G) {
   if (ns >= S) {
-    mf_groups<S, S>(acc, Ab, ao, Bb, bo, G);
+    mf_groups<S, S, TC>(acc, Ab, ao, Bb, bo, G);
   } else if constexpr (S >= 2) {
     if (ns == S - 1) {
-      mf_groups<S - 1, S>(acc, Ab, ao, Bb, bo, G);
+      mf_groups<S - 1, S, TC>(acc, Ab, ao, Bb, bo, G);
     } else if constexpr (S >= 3) {
-      if (ns == S - 2) mf_groups<S - 2, S>(acc, Ab, ao, Bb, bo, G);
+      if (ns == S - 2) mf_groups<S - 2, S, TC>(acc, Ab, ao, Bb, bo, G);
     }
   }
 }
 
-template <class TI, class TO, int XC, int TJ, int MT>
+template <class TI, class TC, class TO, int XC, int TJ, int MT>
 __global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *gate, int gx, int gy) {
-  using C = PlaneMf1<XC, TJ, MT>;
+  using C = PlaneMf1<XC, TJ, MT, TC>;
+  typedef typename Mf<TC>::v4 V4;
   constexpr int NTH = PLANE1_T, MS = C::MS, E0 = C::E0, L1P = C::L1P, E0S = C::E0S, JS = C::JS, S = C::S;
   extern __shared__ __align__(16) unsigned char plane_smem[];
   __shared__ double sh[16];
@@ -1315,9 +1347,10 @@ __global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *
   const int s1 = p.in[comp].s1;
   const __amdgpu_buffer_rsrc_t rin = buf_rsrc((const TI *)p.in[comp].p + (size_t)p.in[comp].s2 * pl,
                                               (s1 * (e1 - 1) + e0) * (int)sizeof(TI));
-  const __amdgpu_buffer_rsrc_t rX = buf_rsrc(p.XT[comp], e0 * e0 * 4), rY = buf_rsrc(p.YT[comp], e1 * e1 * 4);
+  constexpr int ES = (int)sizeof(TC);
+  const __amdgpu_buffer_rsrc_t rX = buf_rsrc(p.XT[comp], e0 * e0 * ES), rY = buf_rsrc(p.YT[comp], e1 * e1 * ES);
   TI rb[C::NB];
-  float rx[C::NX], ry[C::NY];
+  TC rx[C::NX], ry[C::NY];
 #pragma unroll
   for (int r = 0; r < C::NB; ++r) {
     const int t = tid + NTH * r, ii = t % E0, j = t / E0;
@@ -1326,7 +1359,7 @@ __global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *
 #pragma unroll
   for (int r = 0; r < C::NX; ++r) {
     const int t = tid + NTH * r, ii = t % E0, mm = t / E0;
-    rx[r] = buf_load<float>(rX, (min(m0 + mm, e0 - 1) * e0 + min(ii, e0 - 1)) * 4);
+    rx[r] = buf_load<TC>(rX, (min(m0 + mm, e0 - 1) * e0 + min(ii, e0 - 1)) * ES);
   }
   if (p.conv.mode != CONV_NONE) {
     const double rr = sum_parts_8w(region(p.d, PART_RR), p.conv.nparts, 0, sh, tid);
@@ -1338,22 +1371,22 @@ __global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *
   }
   double d0 = 0.0, d1 = 0.0;
   if (plane < e2 && m0 < e0) {
-    float *Xt = (float *)plane_smem, *Bt = Xt + C::BOFF, *St = Xt + C::SOFF;
+    TC *Xt = (TC *)plane_smem, *Bt = Xt + C::BOFF, *St = Xt + C::SOFF;
 #pragma unroll
     for (int r = 0; r < C::NX; ++r) {
       const int t = tid + NTH * r, ii = t % E0, mm = t / E0;
-      Xt[mm * E0S + ii] = (ii < e0 && m0 + mm < e0) ? rx[r] : 0.0f;
+      Xt[mm * E0S + ii] = (ii < e0 && m0 + mm < e0) ? rx[r] : TC(0);
     }
 #pragma unroll
     for (int r = 0; r < C::NB; ++r) {
       const int t = tid + NTH * r, ii = t % E0, j = t / E0;
-      Bt[j * E0S + ii] = (ii < e0 && j < e1) ? (float)rb[r] : 0.0f;
+      Bt[j * E0S + ii] = (ii < e0 && j < e1) ? (TC)rb[r] : TC(0);
     }
     // Y, for phase 2: in flight during phase 1
 #pragma unroll
     for (int r = 0; r < C::NY; ++r) {
       const int t = tid + NTH * r, jj = t % L1P, q = t / L1P;
-      ry[r] = buf_load<float>(rY, (min(q, e1 - 1) * e1 + min(jj, e1 - 1)) * 4);
+      ry[r] = buf_load<TC>(rY, (min(q, e1 - 1) * e1 + min(jj, e1 - 1)) * ES);
     }
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid % 64, lr = l % 16, lk = l / 16;
@@ -1368,12 +1401,12 @@ __global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *
       ao[s] = ((t % MT) * 16 + lr) * E0S + 4 * lk;
       bo[s] = ((t / MT) * 16 + lr) * E0S + 4 * lk;
     }
-    v4f acc[S];
+    V4 acc[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) acc[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int s = 0; s < S; ++s) acc[s] = V4{0, 0, 0, 0};
     // phase 1: S(m, j) = Σ_i X(i, m) in(i, j), groups of 16 rows of i
     const int G1 = (e0 + 15) / 16;
-    mf_slots<S>(ns, acc, Xt, ao, Bt, bo, G1);
+    mf_slots<S, TC>(ns, acc, Xt, ao, Bt, bo, G1);
     __syncthreads();  // every wave is done with the input plane
     // S to LDS (St [m][j]), Y to LDS (Yt [q][j], over the input plane)
 #pragma unroll
@@ -1381,14 +1414,14 @@ __global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *
       const int t = min(w + 8 * s, NTr - 1), mt = t % MT, jt = t / MT;
       if (s < ns)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) St[(mt * 16 + 4 * lk + r) * JS + jt * 16 + lr] = acc[s][r];
-      acc[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int r = 0; r < 4; ++r) St[(mt * 16 + Mf<TC>::mrow(lk, r)) * JS + jt * 16 + lr] = acc[s][r];
+      acc[s] = V4{0, 0, 0, 0};
     }
-    float *Yt = Bt;
+    TC *Yt = Bt;
 #pragma unroll
     for (int r = 0; r < C::NY; ++r) {
       const int t = tid + NTH * r, jj = t % L1P, q = t / L1P;
-      Yt[q * JS + jj] = (jj < e1 && q < e1) ? ry[r] : 0.0f;
+      Yt[q * JS + jj] = (jj < e1 && q < e1) ? ry[r] : TC(0);
     }
     // the epilogue's operands, in flight during phase 2 (clamped addresses)
     double ev[S][4], zv[S][4];
@@ -1408,7 +1441,7 @@ __global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *
         const int qc = min(qt * 16 + lr, e1 - 1);
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          ev[s][r] = buf_load<double>(re, voff(ve, min(m0 + mt * 16 + 4 * lk + r, e0 - 1), qc, plane) * 8);
+          ev[s][r] = buf_load<double>(re, voff(ve, min(m0 + mt * 16 + Mf<TC>::mrow(lk, r), e0 - 1), qc, plane) * 8);
       }
       if (p.dot == 2)
 #pragma unroll
@@ -1417,7 +1450,7 @@ __global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *
           const int qc = min(qt * 16 + lr, e1 - 1);
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            zv[s][r] = buf_load<double>(rz, voff(vz, min(m0 + mt * 16 + 4 * lk + r, e0 - 1), qc, plane) * 8);
+            zv[s][r] = buf_load<double>(rz, voff(vz, min(m0 + mt * 16 + Mf<TC>::mrow(lk, r), e0 - 1), qc, plane) * 8);
         }
     }
     __syncthreads();
@@ -1429,7 +1462,7 @@ __global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *
       bo[s] = ((t / MT) * 16 + lr) * JS + 4 * lk;
     }
     const int G2 = (e1 + 15) / 16;
-    mf_slots<S>(ns, acc, St, ao, Yt, bo, G2);
+    mf_slots<S, TC>(ns, acc, St, ao, Yt, bo, G2);
     const View vo = p.out[comp];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -1438,7 +1471,7 @@ __global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *
       const int q = qt * 16 + lr;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + mt * 16 + 4 * lk + r;
+        const int m = m0 + mt * 16 + Mf<TC>::mrow(lk, r);
         if (m >= e0 || q >= e1) continue;
         const int qo = voff(vo, m, q, plane);
         if (ec == 1) {
@@ -1479,14 +1512,15 @@ static dim3 plane_grid(const PlanePass &p) {
   }
   return dim3(gx, gy, nc);
 }
-// The one-round Float32 kernel (p.one != 0): planes up to 32 PLANE1_XC x 32 PLANE1_TJ with x
-// extents above 64 and y extents above 32 (smaller planes keep k_plane_mf).  Blocks of 2 m tiles,
-// or of 3 where 2 would need more blocks than the chip has CUs and 3 needs fewer (p.one = 1), or
-// p.one m tiles (2 or 3).
+// The one-round kernel (p.one != 0), FP32 (the preconditioner) or FP64 (the separable solve):
+// planes up to 32 PLANE1_XC x 32 PLANE1_TJ with x extents above 64 and y extents above 32 whose
+// staging fits the LDS (smaller planes, and FP64 planes too large for the LDS, keep k_plane_mf /
+// k_plane).  Blocks of 2 m tiles, or of 3 where 2 would need more blocks than the chip has CUs
+// and 3 needs fewer (p.one = 1), or p.one m tiles (2 or 3).
 struct Plane1Pick {
   int use, xc, tj, mt, gx, gy, nc;
 };
-static Plane1Pick plane1_pick(const PlanePass &p) {
+static Plane1Pick plane1_pick(const PlanePass &p, int es) {
   Plane1Pick k{};
   k.nc = plane_ncomp(p);
   int l0 = 0, l1 = 0;
@@ -1504,15 +1538,15 @@ static Plane1Pick plane1_pick(const PlanePass &p) {
   k.tj = cdiv(l1, 32);
   const int n2 = cdiv(l0, 32) * k.gy * k.nc, n3 = cdiv(l0, 48) * k.gy * k.nc;
   k.mt = p.one >= 2 ? p.one : ((n2 > 256 && n3 < n2) ? 3 : 2);
+  if (plane1_lds(k.xc, k.tj, k.mt, es) > PLANE1_LDS_MAX && p.one < 2) k.mt = 2;
+  if (plane1_lds(k.xc, k.tj, k.mt, es) > PLANE1_LDS_MAX) k.use = 0;
   k.gx = cdiv(l0, 16 * k.mt);
   return k;
 }
 template <class T>
 int plane_parts(const PlanePass &p) {
-  if (sizeof(T) == 4) {
-    const Plane1Pick k = plane1_pick(p);
-    if (k.use) return k.gx * k.gy * k.nc;
-  }
+  const Plane1Pick k = plane1_pick(p, (int)sizeof(T));
+  if (k.use) return k.gx * k.gy * k.nc;
   const dim3 g = plane_grid<T>(p);
   return (int)(g.x * g.y * g.z);
 }
@@ -1534,15 +1568,15 @@ hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
   for (int c = 0; c < nc; ++c) l1 = p.e[c][1] > l1 ? p.e[c][1] : l1;
   const dim3 gr = plane_grid<T>(p);
   const bool mf = sizeof(T) == 4;  // Float32 on the matrix cores
-  if constexpr (sizeof(T) == 4) {
-    const Plane1Pick k = plane1_pick(p);
+  {
+    const Plane1Pick k = plane1_pick(p, (int)sizeof(T));
     if (k.use) {
       const dim3 g1(k.gx * k.gy * k.nc);
       switch ((k.xc * 4 + k.tj) * 4 + k.mt) {
-#define TDGL_PLANE1_CASE(x, t, m)                                                                                 \
-  case (x * 4 + t) * 4 + m:                                                                                       \
-    TDGL_LAUNCH((k_plane_mf1<TI, TO, x, t, m>), g1, dim3(PLANE1_T), (PlaneMf1<x, t, m>::lds), s, p, gate, k.gx, \
-                       k.gy);                                                                                     \
+#define TDGL_PLANE1_CASE(x, t, m)                                                                               \
+  case (x * 4 + t) * 4 + m:                                                                                     \
+    TDGL_LAUNCH((k_plane_mf1<TI, T, TO, x, t, m>), g1, dim3(PLANE1_T), (PlaneMf1<x, t, m, T>::lds), s, p, gate, \
+                k.gx, k.gy);                                                                                    \
     break;
         TDGL_PLANE1_CASE(3, 2, 2) TDGL_PLANE1_CASE(3, 3, 2) TDGL_PLANE1_CASE(3, 2, 3) TDGL_PLANE1_CASE(3, 3, 3)
         TDGL_PLANE1_CASE(4, 2, 2) TDGL_PLANE1_CASE(4, 3, 2) TDGL_PLANE1_CASE(4, 2, 3) TDGL_PLANE1_CASE(4, 3, 3)

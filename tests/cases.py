@@ -72,6 +72,10 @@ CASES = {
     "qp_three": (three_frustrated, {}, dict(qp_exchange=qp_rates(0.5)), 1e-10),
     # x and y extents of the separable solve above 64 and 32: the one-round Float32 x-y launch
     "pcg_wide_f32": (mgb2, dict(nx=68, ny=34, nz=4, buf=2), dict(sigma_vac=0.01), 1e-7),
+    # the FP64 x-y launch of the one-round kernel: the separable solve and the exact PCG on wide planes
+    "sep_wide": (mgb2, dict(nx=68, ny=34, nz=4), {}, 1e-10),
+    "pcg_exact_wide": (mgb2, dict(nx=68, ny=34, nz=4, buf=2), dict(sigma_vac=0.01, pcg_fast=False, pcg_f32=False,
+                                                                  pcg_tol=1e-12), 1e-10),
     "open": (mgb2, {}, dict(selffield_bc="open", open_every=0.02), 1e-10),
     "open_kwt_pcg": (lambda: mgb2(gkwt=10.0), dict(buf=2), dict(sigma_vac=0.01, pcg_tol=1e-12, selffield_bc="open",
                                                                 open_every=0.03), 1e-10),

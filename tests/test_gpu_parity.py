@@ -142,31 +142,36 @@ def test_hip_single_axis_transforms_match_oracle(name, monkeypatch):
 
 
 @pytest.mark.parametrize("one", ["2", "3", "0"])
-def test_hip_one_round_plane_kernel_matches_oracle(one, monkeypatch):
-    """The Float32 x-y launch of the preconditioner with 2 and 3 m tiles per block (and the
-    chunked launch, 0) on planes of 70 x 36 against the oracle: the PCG reaches the same state."""
+@pytest.mark.parametrize("name", ["pcg_wide_f32", "sep_wide", "pcg_exact_wide"])
+def test_hip_one_round_plane_kernel_matches_oracle(name, one, monkeypatch):
+    """The one-round x-y launch, FP32 (the preconditioner) and FP64 (the separable solve and the
+    exact PCG), with 2 and 3 m tiles per block (and the chunked launches, 0) on planes of about
+    70 x 36 against the oracle."""
     from oracle.oracle import Oracle
     HipStepper = _hip()
     monkeypatch.setenv("TDGL_HIP_PLANE1", one)
-    st, tol = make_case("pcg_wide_f32")
+    st, tol = make_case(name)
     s0, pcg, H, HJ = seed_state(st)
     o = Oracle(st)
     o.set_state(s0)
-    o.set_pcg(pcg)
+    if pcg is not None:
+        o.set_pcg(pcg)
     o.step(12, H, HJ)
     ref = o.get_state()
-    print(f"[gpu] one-round plane kernel, mode {one}", flush=True)
+    print(f"[gpu] one-round plane kernel, {name}, mode {one}", flush=True)
     dev = HipStepper(st)
     dev.upload(s0)
-    dev.set_pcg(pcg)
+    if pcg is not None:
+        dev.set_pcg(pcg)
     for _ in range(3):
         dev.step(4, H, HJ)
     errs = compare_states(dev.download(), ref)
     core = {k: v for k, v in errs.items() if not k.startswith(("U", "V"))}
     print(f"    worst {max(core, key=core.get)} {max(core.values()):.3e}", flush=True)
     assert max(core.values()) <= tol, errs
-    gi, oi = dev.get_pcg().iters, o.get_pcg().iters
-    assert abs(gi - oi) <= max(2, oi // 4), (gi, oi)
+    if pcg is not None:
+        gi, oi = dev.get_pcg().iters, o.get_pcg().iters
+        assert abs(gi - oi) <= max(2, oi // 4), (gi, oi)
 
 
 def test_hip_open_boundary_resumes_and_is_odd_in_the_current():
