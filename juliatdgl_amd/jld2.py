@@ -7,9 +7,11 @@ datatypes (JLD2's Julia types), variable-length strings in global heap collectio
 references, and JLD2's ``SerializedDict`` encoding of ``Dict{String,Any}`` (a vector of
 ``Pair{String,Any}`` whose second field is a reference to the value's dataset).
 
-It is only used offline, by ``convert_goldens.py``, to turn the golden files into ``.npz``
-fixtures; no test imports it.  Written from the public HDF5 file format specification
-(version 3.0 of the spec: superblock, object header, message and global-heap layouts).
+juliatdgl_amd.checkpoint reads the reference's checkpoints with it (save_state, gtdgl/src/io.jl:
+131-150: top-level datasets, deflate-compressed), and tests/golden/convert_goldens.py turned the
+reference's golden files into ``.npz`` fixtures with it.  Written from the public HDF5 file format
+specification (version 3.0 of the spec: superblock, object header, message and global-heap
+layouts).
 """
 from __future__ import annotations
 

@@ -10,7 +10,7 @@ the masks and the parameters of the step 1.2 chose.  gtdgl/test/test_regression1
 Each fixture keeps exactly the data that test reads: the parameters (as JSON), the masks, the
 initial fields (Ψ, θ, φ, h, and the links the test checks) and the final fields (Ψ, θ, φ, V, h).
 Arrays keep their Julia (column-major) shapes.  This is data, not code: the fixtures hold the
-reference's numbers, the reader is jld2_reader.py (written from the HDF5 specification).
+reference's numbers, the reader is juliatdgl_amd/jld2.py (written from the HDF5 specification).
 """
 from __future__ import annotations
 
@@ -20,8 +20,8 @@ import sys
 
 import numpy as np
 
-sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-from jld2_reader import JLD2File  # noqa: E402
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", ".."))
+from juliatdgl_amd.jld2 import JLD2File  # noqa: E402
 
 CASES = ("k2_R13_explicit", "k2_R4_prod", "k25_R4_prod")
 INI = ("G1", "G2", "thetax", "thetay", "thetaz", "phix", "phiy", "phiz", "hx", "hy", "hz",
