@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define TDGL_ABI_VERSION 2
+#define TDGL_ABI_VERSION 3
 #define TDGL_MAX_BANDS 4
 #define TDGL_MAX_STAGES 64
 
@@ -136,6 +136,27 @@ typedef struct {
   double open_kappa2;          /* κ² */
   const double *open_lgf;      /* lattice_green(R): (R+1)³, column-major */
   const double *open_Jbar;     /* (Ny+1, Nz+1): κ² curl of the x-face bar field per unit HJ */
+
+  /* Thermal noise (Langevin; gtdgl/src/noise.jl, Stepper's noise / noise_seed / noise_field,
+   * scheme.jl:239-244, 336-360, 424-431).  With noise every band gets the kick
+   * of each step after its update (noise_kick!, noise.jl:40-65: Euler–Maruyama, the polar KWT kick
+   * with its Itô drift), a KWT band on RKC stages its phase kick inside the first stage (SK-ROCK,
+   * noise_draw! / noise_apply!, :99-153), a KWT band steps in log-polar form and the rotating frame
+   * low-passes its rate (bands.jl:210-211, 237-270, 432-455); with noise_field the link phases get
+   * their kick before the implicit solve (field_kick!, :182-198).
+   * The normals: noise_host = 0 draws them on the device from the counter-based Philox4x32-10
+   * generator (Salmon et al., SC'11) keyed by noise_seed, with the counter (node or link index,
+   * step count, band or link family) and the Box–Muller transform; noise_host = 1 takes the
+   * caller's own draws (tdgl_noise_draws), e.g. the reference's stream in its loop order. */
+  int32_t noise;                         /* 1: the Stepper has noise (at T̃ = 0 too: the scheme changes) */
+  int32_t noise_field;                   /* 1: the field kick too */
+  int32_t noise_host;                    /* 1: draws from tdgl_noise_draws */
+  int32_t noise_sk[TDGL_MAX_BANDS];      /* _skrock(st, µ) (bands.jl:446) */
+  double noise_T;                        /* T̃ */
+  uint64_t noise_seed;
+  double noise_sk_w[TDGL_MAX_BANDS][2];  /* RKCCoef.sk = (ν₁, κ₁) = (sω₁/2, sω₁/ω₀) (scheme.jl:80, 99) */
+  double noise_Gamma[TDGL_MAX_BANDS];    /* Γ_µ */
+  double noise_sigma_n;                  /* σn of the field kick */
 } tdgl_desc;
 
 /* Normal-fluid models (tdgl_desc.qp_mode) */
@@ -213,6 +234,23 @@ typedef struct {
 } tdgl_open;
 int tdgl_get_open(tdgl_handle *h, tdgl_open *o);
 int tdgl_set_open(tdgl_handle *h, const tdgl_open *o);
+
+/* The noise stream's position: the steps taken with noise since the stream began (0 at
+ * tdgl_create).  With noise_seed it is the whole state of the generator, what a checkpoint holds
+ * in place of the reference's Xoshiro state (_noise_snap / _noise_back!, noise.jl:200-201;
+ * protocol.jl:382, 659), so that resuming is continuing to the bit. */
+int tdgl_noise_get(tdgl_handle *h, uint64_t *counter);
+int tdgl_noise_set(tdgl_handle *h, uint64_t counter);
+
+/* noise_host = 1: the normals of the next nsteps steps (replacing any not yet used), step after
+ * step, each: the N bands' complex node arrays (Nx+1, Ny+1, Nz+1), ξ = n₁ + i n₂ at every node the
+ * kick visits (interior, and not a hard f = 0 node; anything elsewhere), then with noise_field the
+ * real link arrays (Nx, Ny+1, Nz+1), (Nx+1, Ny, Nz+1), (Nx+1, Ny+1, Nz) with n on the links
+ * field_kick! visits.  Column-major, doubles.  The reference draws these from its seeded Xoshiro
+ * stream on the host in its loop order (noise.jl:40-198, TDGLGPU.jl:714-870), which is how it
+ * runs noise on a GPU: a stream in that order maps onto these arrays one to one.  tdgl_step fails
+ * with TDGL_E_ARG for a step without draws. */
+int tdgl_noise_draws(tdgl_handle *h, int32_t nsteps, const double *draws);
 
 /* The voltmeter of a ladder (ladder, gtdgl/src/protocol.jl:341-352): the x-link planes vi and vo
  * (1-based, x1 + delta_v and x2 - delta_v) and four masks over the lines (j, k), j in y1..y2+1

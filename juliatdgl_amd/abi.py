@@ -10,7 +10,7 @@ import numpy as np
 from .scheme import FIELD_PCG, FIELD_EXPLICIT_SCALED, Stepper
 from .state import FIELDS, State
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAXB = 4
 MAXS = 64
 dp = ct.POINTER(ct.c_double)
@@ -52,7 +52,11 @@ class DescC(ct.Structure):
                 ("qp_Vx", dp), ("qp_Wx", dp), ("qp_ex", dp), ("qp_Vy", dp), ("qp_Wy", dp), ("qp_ey", dp),
                 ("qp_Vz", dp), ("qp_Wz", dp), ("qp_ez", dp),
                 ("open_every", ct.c_int32), ("open_R", ct.c_int32), ("open_fac", ct.c_double),
-                ("open_kappa2", ct.c_double), ("open_lgf", dp), ("open_Jbar", dp)]
+                ("open_kappa2", ct.c_double), ("open_lgf", dp), ("open_Jbar", dp),
+                ("noise", ct.c_int32), ("noise_field", ct.c_int32), ("noise_host", ct.c_int32),
+                ("noise_sk", ct.c_int32 * MAXB), ("noise_T", ct.c_double), ("noise_seed", ct.c_uint64),
+                ("noise_sk_w", (ct.c_double * 2) * MAXB), ("noise_Gamma", ct.c_double * MAXB),
+                ("noise_sigma_n", ct.c_double)]
 
 
 class StateC(ct.Structure):
@@ -178,6 +182,17 @@ class PackedDesc:
         if ob is not None:
             d.open_every, d.open_R, d.open_fac, d.open_kappa2 = ob.every, ob.R, ob.fac, ob.kappa2
             d.open_lgf, d.open_Jbar = hold(ob.lgf), hold(ob.Jbar)
+        nz = st.noise
+        if nz is not None:
+            d.noise, d.noise_T, d.noise_seed = 1, nz.T, nz.seed
+            d.noise_field, d.noise_host = int(nz.field), int(nz.host)
+            for mu in range(N):
+                d.noise_sk[mu] = int(st.sk[mu])
+                kc = st.rkc[mu] if st.rkc is not None else None
+                if kc is not None:
+                    d.noise_sk_w[mu][0], d.noise_sk_w[mu][1] = kc.sk
+                d.noise_Gamma[mu] = st.model.bands[mu].Gamma
+            d.noise_sigma_n = st.model.sigma_n
         self.c = d
         self._keep = keep
 

@@ -50,6 +50,27 @@ __device__ __forceinline__ cplx kwt_update(cplx g, cplx d, double g2) {
   return cadd(g, csub(dp, rmul(s, rmul(g2, g))));
 }
 
+// KWT step of one node with its tangential part as a rotation (kwt_update_rot, bands.jl:150-158)
+__device__ __forceinline__ cplx kwt_update_rot(cplx g, cplx d, double g2) {
+  double a2 = abs2(g);
+  double q = 1 + g2 * a2;
+  cplx dp = rmul(sqrt(q), d);
+  cplx dl = csub(dp, rmul(cmul(cconj(g), dp).x / q, rmul(g2, g)));
+  if (!(a2 > 1e-24)) return cadd(g, dl);
+  double a = sqrt(a2);
+  cplx u = mk(g.x / a, g.y / a);
+  cplx w = cmul(cconj(u), dl);
+  return cmul(u, rmul(a + w.x, dcis(w.y / a)));
+}
+
+// complex quotient (textbook formula), log (log|z| with |z| = hypot, arg z) and exp
+__device__ __forceinline__ cplx cdivz(cplx a, cplx b) {
+  double den = b.x * b.x + b.y * b.y;
+  return mk((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
+}
+__device__ __forceinline__ cplx clogz(cplx z) { return mk(log(hypot(z.x, z.y)), atan2(z.y, z.x)); }
+__device__ __forceinline__ cplx cexpz(cplx z) { return rmul(exp(z.x), dcis(z.y)); }
+
 TDGL_HD double jclamp(double x, double lo, double hi) { return x > hi ? hi : (x < lo ? lo : x); }
 
 // Node-shaped geometry: every 3D array of a run is stored with the strides of the node array

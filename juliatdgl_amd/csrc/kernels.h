@@ -29,7 +29,25 @@ extern thread_local ProfPair *prof_pair;
 
 
 // ---------------------------------------------------------------- band update (bands.jl)
-enum FrameMode { FRAME_NONE = 0, FRAME_RATE = 1, FRAME_TERM = 2 };
+// With noise (bands.jl:210-211, 237-281) the first stage's rate is low-passed before its term
+// (FRAME_RAW: the rate only; k_frame_smooth does the rest) and the later terms are rotations
+// (FRAME_TURNTERM: C e^{−iω̂h}).
+enum FrameMode { FRAME_NONE = 0, FRAME_RATE = 1, FRAME_TERM = 2, FRAME_RAW = 3, FRAME_TURNTERM = 4 };
+
+// Thermal noise (noise.jl): where a kernel takes the normals of the current step.  ctr[0] counts
+// the steps taken with noise (advanced on the device at the end of each step, so a replayed graph
+// draws new numbers), ctr[1] is the step of the first draw in `hd`.  Generator mode: Philox4x32-10
+// keyed by `seed` at counter (index, step, tag); host mode: hd + (ctr[0] − ctr[1])·hstride, each
+// slot N complex node arrays then 3 real link arrays in the device layout.
+struct NoiseArgs {
+  const uint64_t *ctr;
+  const double *hd;
+  long long hstride, nslots;
+  uint64_t seed;
+  int host;
+  int nb;
+  double Tdt, dV;        // T̃ dt and dx dy dz
+};
 
 // How a stage gets its links (computed per use from V, U and Ω, never stored):
 enum LinkSource {
@@ -70,8 +88,38 @@ struct StageArgs {
   const double *qpM[TDGL_MAX_BANDS];
   int qpK, qn1, qnj, qj0, qk0;
   double qpc[TDGL_MAX_BANDS];  // TWO: [0] = m_µ dt; N: P[µ][k]
+  // noise: the KWT increment as a rotation (kwt_update_rot, bands.jl:150-158), the stages
+  // combined in log-polar form (_combine_log!, :439-445), SK-ROCK's turn of the first stage by
+  // e^{i(κ₁−ν₁)δθ} (:487), and after the band's last stage the kick of the step (noise_kick! /
+  // noise_apply!, noise.jl:44-65, 127-153) with the band's Γ and γ̃² (full step)
+  int kwt_rot, logcomb;
+  const double *Qth;           // SK-ROCK's δθ of this band (sk_on, and the kick with nsk)
+  int sk_on;
+  double sk_post;              // κ₁ − ν₁
+  int noise, nsk;
+  double nGm, ng2;
+  NoiseArgs nz;
+  double *Om2;                 // k_frame_smooth: the rate between the two sweeps
 };
 hipError_t launch_stage(const Geo &g, const StageArgs &a, hipStream_t s);
+// With noise and the rotating frame, after the first stage (FRAME_RAW): the two low-pass sweeps
+// of the rate, the frame term as a rotation, and what k_stage would have done after it
+// (_frame_rate_smooth!, bands.jl:243-270)
+hipError_t launch_frame_smooth(const Geo &g, const StageArgs &a, hipStream_t s);
+// SK-ROCK's draw (noise_draw!, noise.jl:99-124, and _sk_turn!, bands.jl:447-453): δθ of the band
+// from Ψⁿ and its normals into Qth, and the first stage's source Ψⁿ e^{iν₁δθ} into Sp
+struct SkArgs {
+  const cplx *G;
+  const double *f;
+  double *Qth;
+  cplx *Sp;
+  int mu, hard;
+  double c, g2, Gm, nu1;
+  NoiseArgs nz;
+};
+hipError_t launch_sk_src(const Geo &g, const SkArgs &a, hipStream_t s);
+// the stream's step counter, after every step with noise
+hipError_t launch_noise_tick(uint64_t *ctr, hipStream_t s);
 
 // ---------------------------------------------------------------- field (field.jl)
 struct FieldArgs {
@@ -91,6 +139,10 @@ struct FieldArgs {
   int nb;
   int scale;             // apply the D scaling
   int gather;            // write B
+  // field_kick! (noise.jl:182-198): φ += (a_c √D) n on the links the step evolves, before B
+  int nkick;
+  double kax, kay, kaz;
+  NoiseArgs nz;
 };
 hipError_t launch_field(const Geo &g, const FieldArgs &a, hipStream_t s);
 int field_blocks(const Geo &g);

@@ -61,6 +61,135 @@ __device__ __forceinline__ cplx stage_link(const Geo &g, const StageArgs &a, int
   return l;
 }
 
+// ============================================================================ noise
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11): ten rounds of two 32x32->64 products, the key bumped by the Weyl constants between them.
+__device__ __forceinline__ void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+  }
+}
+
+// Two standard normals (Box & Muller, 1958) from the block at counter (q, step, tag) under the
+// seed: u in (0, 1) and v in [0, 1) from 52 bits each, √(−2 log u) (cos 2πv, sin 2πv)
+__device__ __forceinline__ cplx normal2(uint64_t seed, uint64_t t, uint32_t tag, uint32_t q) {
+  uint32_t c0 = q, c1 = (uint32_t)t, c2 = (uint32_t)(t >> 32), c3 = tag;
+  philox10(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const double e52 = 2.220446049250313e-16;  // 2^-52
+  const double u = ((double)(((uint64_t)(c0 >> 6) << 26) | (uint64_t)(c1 >> 6)) + 0.5) * e52;
+  const double v = (double)(((uint64_t)(c2 >> 6) << 26) | (uint64_t)(c3 >> 6)) * e52;
+  const double r = sqrt(-2.0 * log(u));
+  double sn, cs;
+  sincos(6.283185307179586 * v, &sn, &cs);
+  return mk(r * cs, r * sn);
+}
+
+// The slot of the caller's draws for this step, or -1 past them (the host refuses such a step
+// before it is launched; this keeps every read inside the buffer regardless)
+__device__ __forceinline__ long long noise_slot(const NoiseArgs &z) {
+  const uint64_t t = z.ctr[0], b = z.ctr[1];
+  return (t >= b && t - b < (uint64_t)z.nslots) ? (long long)(t - b) : -1;
+}
+
+// ξ of band µ at node (i, j, k) (array index c) for this step
+__device__ __forceinline__ cplx noise_xi(const NoiseArgs &z, const Geo &g, int mu, int c, int i, int j, int k) {
+  if (z.host) {
+    const long long sl = noise_slot(z);
+    if (sl < 0) return mk(0.0, 0.0);
+    const double *p = z.hd + sl * z.hstride + (long long)mu * 2 * g.n + 2 * (long long)c;
+    return mk(p[0], p[1]);
+  }
+  const uint32_t q = (uint32_t)((i - 1) + (g.Nx + 1) * ((j - 1) + (g.Ny + 1) * (k - 1)));
+  return normal2(z.seed, z.ctr[0], (uint32_t)mu, q);
+}
+
+// n at the link of family ax at (i, j, k) (array index c) for this step
+__device__ __forceinline__ double noise_link(const NoiseArgs &z, const Geo &g, int ax, int c, int i, int j, int k) {
+  if (z.host) {
+    const long long sl = noise_slot(z);
+    if (sl < 0) return 0.0;
+    return z.hd[sl * z.hstride + (long long)z.nb * 2 * g.n + (long long)ax * g.n + c];
+  }
+  const int e0 = ax == 0 ? g.Nx : g.Nx + 1, e1 = ax == 1 ? g.Ny : g.Ny + 1;
+  const uint32_t q = (uint32_t)((i - 1) + e0 * ((j - 1) + e1 * (k - 1)));
+  return normal2(z.seed, z.ctr[0], 16u + (uint32_t)ax, q).x;
+}
+
+// _kwt_kick (noise.jl:75-87): the polar kick of a KWT node at ψ = Ψⁿ, the Cartesian one where
+// |ψ|² or |F|² is under _ROT_MIN2
+__device__ __forceinline__ cplx kwt_kick(cplx F, cplx psi, cplx xi, double a, double c, double dr, double g2, double Gm,
+                                         double d0) {
+  const double r2 = abs2(psi), sk = sqrt(1 + g2 * r2);
+  if (!(r2 > 1e-24)) return cadd(F, rmul(a, xi));
+  const double sr = sqrt(r2);
+  const cplx u = mk(psi.x / sr, psi.y / sr);
+  const cplx w = cmul(cconj(u), xi);
+  if (r2 > 1e-3 && abs2(F) > 1e-3) {
+    const double rho = sqrt(r2), aF = hypot(F.x, F.y);
+    const double drho = c * w.x / sqrt(Gm * sk) + d0 / (rho * (sk * sk * sk));
+    const double dth = c * w.y * sqrt(sk / Gm) / rho;
+    return cmul(mk(F.x / aF, F.y / aF), rmul(aF + drho, dcis(dth)));
+  }
+  const cplx t = cmul(rmul(c, u), mk(w.x / sqrt(Gm * sk), w.y * sqrt(sk / Gm)));
+  return csub(cadd(F, t), rmul(dr * (1 + sk * sk) / (sk * sk * sk), psi));
+}
+
+// The kick of one node after its band's last stage: noise_kick! (noise.jl:44-65), or after
+// SK-ROCK stages the amplitude part of noise_apply! (:127-153) where the phase went in with them
+__device__ __forceinline__ cplx noise_node(const StageArgs &a, const Geo &g, cplx F, int c, int i, int j, int k) {
+  const NoiseArgs &z = a.nz;
+  const cplx xi = noise_xi(z, g, a.mu, c, i, j, k);
+  const double Tdt = z.Tdt, dV = z.dV, Gm = a.nGm, g2 = a.ng2;
+  const double aa = sqrt(Tdt / (Gm * dV));
+  if (g2 == 0) return cadd(F, rmul(aa, xi));
+  const double cc = sqrt(Tdt / dV);
+  if (a.nsk && a.Qth[c] != 0.0) {
+    const cplx psi = a.G[a.mu][c];
+    const double rho = hypot(psi.x, psi.y), sk = sqrt(1 + g2 * (rho * rho));
+    const cplx u = mk(psi.x / rho, psi.y / rho);
+    const double drho = cc * cmul(cconj(u), xi).x / sqrt(Gm * sk) + Tdt / (2 * dV * Gm) / (rho * (sk * sk * sk));
+    const double aF = hypot(F.x, F.y);
+    return aF > sqrt(1e-3) ? rmul((aF + drho) / aF, F) : cadd(F, rmul(drho, u));
+  }
+  return kwt_kick(F, a.G[a.mu][c], xi, aa, cc, Tdt / (2 * dV) * g2 / Gm, g2, Gm, Tdt / (2 * dV * Gm));
+}
+
+// What follows the frame term and the combination on a stage's node, in the reference's order:
+// SK-ROCK's turn of the first stage (_sk_turn!, bands.jl:487), the turn back of the frame
+// (_frame_turn!, :321-327), the normal-fluid turn (qp_turn!) and the kick of the step
+__device__ __forceinline__ cplx stage_tail(const Geo &g, const StageArgs &a, int i, int j, int k, int c, cplx p, double om) {
+  if (a.sk_on) {
+    const double q = a.Qth[c];
+    if (q != 0.0) p = cmul(p, dcis(a.sk_post * q));
+  }
+  if (a.turn) p = cmul(p, dcis(om * a.dt));
+  if (a.qp) {
+    const int q = (i - 2) + a.qn1 * ((j - a.qj0 - 1) + a.qnj * (k - a.qk0 - 1));
+    double ang;
+    if (a.qp == TDGL_QP_TWO) {
+      ang = -a.qpc[0] * a.qpM[0][q];
+    } else {
+      double pot = 0.0;
+      for (int kk = 0; kk < a.qpK; ++kk) pot += a.qpc[kk] * a.qpM[kk][q];
+      ang = -a.dt * pot;
+    }
+    p = cmul(p, dcis(ang));
+  }
+  if (a.noise && !(a.hard && a.f[c] == 0.0)) p = noise_node(a, g, p, c, i, j, k);
+  return p;
+}
+
 // One (sub)stage of band µ on the interior nodes, fused: the band pass (_band_row!,
 // bands.jl:71-97) with KWT (:135-140), the rotating-frame rate or term (_frame_rate!
 // :213-234 / _frame_term! :274-286), the RKL/RKC combination (_combine! :425-430) and the
@@ -111,35 +240,39 @@ __global__ void __launch_bounds__(256) k_stage(Geo g, StageArgs a) {
       if (nu == a.mu) continue;
       acc = cadd(acc, rmul(a.cpl[nu], a.G[nu][c]));
     }
-    p = a.gamma2 > 0 ? kwt_update(gc, csub(acc, gc), a.gamma2) : acc;
+    p = a.gamma2 > 0 ? (a.kwt_rot ? kwt_update_rot(gc, csub(acc, gc), a.gamma2) : kwt_update(gc, csub(acc, gc), a.gamma2))
+                     : acc;
   }
   double om = 0.0;
-  if (a.frame == FRAME_RATE) {
+  if (a.frame == FRAME_RATE || a.frame == FRAME_RAW) {
     const double a2 = abs2(gc);
     om = a2 > 1e-3 ? jclamp(cmul(cconj(gc), csub(p, gc)).y / (a2 * a.h), -a.wmax, a.wmax) : 0.0;
     a.Om[c] = om;
+    if (a.frame == FRAME_RAW) {
+      // the low-passed rate and everything after it: k_frame_smooth
+      a.D[c] = p;
+      return;
+    }
     p = csub(p, imx_mul(om * a.h, gc));
   } else if (a.frame == FRAME_TERM) {
     om = a.Om[c];
     p = csub(p, imx_mul(om * a.h, gc));
+  } else if (a.frame == FRAME_TURNTERM) {
+    om = a.Om[c];
+    p = cmul(p, dcis(-(om * a.h)));
   } else if (a.turn) {
     om = a.Om[c];
   }
-  if (a.combine) p = cadd(rmul(a.cmu, p), rmul(a.cnu, a.A[c]));
-  if (a.turn) p = cmul(p, dcis(om * a.dt));
-  if (a.qp) {
-    const int q = (i - 2) + a.qn1 * ((j - a.qj0 - 1) + a.qnj * (k - a.qk0 - 1));
-    double ang;
-    if (a.qp == TDGL_QP_TWO) {
-      ang = -a.qpc[0] * a.qpM[0][q];
+  if (a.combine) {
+    if (a.logcomb) {
+      const cplx A = a.A[c];
+      p = abs2(A) > 1e-3 && abs2(p) > 1e-3 ? cmul(A, cexpz(rmul(a.cmu, clogz(cdivz(p, A)))))
+                                            : cadd(rmul(a.cmu, p), rmul(1 - a.cmu, A));
     } else {
-      double pot = 0.0;
-      for (int kk = 0; kk < a.qpK; ++kk) pot += a.qpc[kk] * a.qpM[kk][q];
-      ang = -a.dt * pot;
+      p = cadd(rmul(a.cmu, p), rmul(a.cnu, a.A[c]));
     }
-    p = cmul(p, dcis(ang));
   }
-  a.D[c] = p;
+  a.D[c] = stage_tail(g, a, i, j, k, c, p, om);
 }
 
 hipError_t launch_stage(const Geo &g, const StageArgs &a, hipStream_t s) {
@@ -147,6 +280,81 @@ hipError_t launch_stage(const Geo &g, const StageArgs &a, hipStream_t s) {
   if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
   dim3 b(64, 4, 1), gr(cdiv(ni, 64), cdiv(nj, 4), nk);
   TDGL_LAUNCH(k_stage, gr, b, 0, s, g, a);
+  return hipGetLastError();
+}
+
+// One low-pass sweep of the frame's rate, Ω ← Ω/2 + (sum of the 6 neighbours)/12 with the faces
+// copying the interior (_frame_rate_smooth!, bands.jl:256-262): Om -> Om2, then Om2 -> Om; the
+// second sweep also turns the stage's output by e^{−iω̂h} (:266-268) and finishes it (stage_tail).
+__global__ void __launch_bounds__(256) k_frame_smooth(Geo g, StageArgs a, int pass) {
+  const int i = g.x1 + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = g.y1 + 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = g.z1 + 1 + blockIdx.z;
+  if (i > g.x2 || j > g.y2 || k > g.z2) return;
+  const int c = g.at(i, j, k);
+  const double *T = pass == 1 ? a.Om : a.Om2;
+  const double nb = (om_at(g, T, i + 1, j, k) + om_at(g, T, i - 1, j, k)) +
+                    (om_at(g, T, i, j + 1, k) + om_at(g, T, i, j - 1, k)) +
+                    (om_at(g, T, i, j, k + 1) + om_at(g, T, i, j, k - 1));
+  const double om = 0.5 * T[c] + nb / 12;
+  if (pass == 1) {
+    a.Om2[c] = om;
+    return;
+  }
+  a.Om[c] = om;
+  const cplx p = cmul(a.D[c], dcis(-(om * a.h)));
+  a.D[c] = stage_tail(g, a, i, j, k, c, p, om);
+}
+
+hipError_t launch_frame_smooth(const Geo &g, const StageArgs &a, hipStream_t s) {
+  const int ni = g.x2 - g.x1, nj = g.y2 - g.y1, nk = g.z2 - g.z1;
+  if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
+  if (!a.Om || !a.Om2) return hipErrorInvalidValue;
+  dim3 b(64, 4, 1), gr(cdiv(ni, 64), cdiv(nj, 4), nk);
+  TDGL_LAUNCH(k_frame_smooth, gr, b, 0, s, g, a, 1);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  TDGL_LAUNCH(k_frame_smooth, gr, b, 0, s, g, a, 2);
+  return hipGetLastError();
+}
+
+// noise_draw! for one SK-ROCK band (noise.jl:108-121) and its first stage's source
+// Ψⁿ e^{iν₁δθ} (_sk_turn!, bands.jl:484), on the interior
+__global__ void __launch_bounds__(256) k_sk_src(Geo g, SkArgs a) {
+  const int i = g.x1 + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = g.y1 + 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = g.z1 + 1 + blockIdx.z;
+  if (i > g.x2 || j > g.y2 || k > g.z2) return;
+  const int c = g.at(i, j, k);
+  const cplx psi = a.G[c];
+  double q = 0.0;
+  if (!(a.hard && a.f[c] == 0.0)) {
+    const double r2 = abs2(psi);
+    if (a.g2 > 0 && r2 > 1e-3) {
+      const cplx xi = noise_xi(a.nz, g, a.mu, c, i, j, k);
+      const double sk = sqrt(1 + a.g2 * r2), rho = sqrt(r2);
+      q = a.c * cmul(cconj(mk(psi.x / rho, psi.y / rho)), xi).y * sqrt(sk / a.Gm) / rho;
+    }
+  }
+  a.Qth[c] = q;
+  a.Sp[c] = q == 0.0 ? psi : cmul(psi, dcis(a.nu1 * q));
+}
+
+hipError_t launch_sk_src(const Geo &g, const SkArgs &a, hipStream_t s) {
+  const int ni = g.x2 - g.x1, nj = g.y2 - g.y1, nk = g.z2 - g.z1;
+  if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
+  dim3 b(64, 4, 1), gr(cdiv(ni, 64), cdiv(nj, 4), nk);
+  TDGL_LAUNCH(k_sk_src, gr, b, 0, s, g, a);
+  return hipGetLastError();
+}
+
+// the step count of the noise stream, after all of the step's draws (one lane writes)
+__global__ void __launch_bounds__(64) k_noise_tick(uint64_t *ctr) {
+  if (threadIdx.x == 0) ctr[0] = ctr[0] + 1;
+}
+
+hipError_t launch_noise_tick(uint64_t *ctr, hipStream_t s) {
+  TDGL_LAUNCH(k_noise_tick, dim3(1), dim3(64), 0, s, ctr);
   return hipGetLastError();
 }
 
@@ -183,6 +391,7 @@ __global__ void __launch_bounds__(256) k_field(Geo g, FieldArgs a) {
         a.J[0][c] = J;
         ph = ((a.th[0][c] + (a.d1 * a.tau[0][c]) * J) - a.sxy * (a.h[2][c] - a.h[2][c - sy])) +
              a.sxz * (a.h[1][c] - a.h[1][c - sz]);
+        if (a.nkick) ph += (a.D[0] ? a.kax * sqrt(a.D[0][c]) : a.kax) * noise_link(a.nz, g, 0, c, i, j, k);
         if (a.gather && a.rg.in(0, i, j, k)) {
           const double b = ph - a.th[0][c];
           a.B[0][c] = b;
@@ -203,6 +412,7 @@ __global__ void __launch_bounds__(256) k_field(Geo g, FieldArgs a) {
         a.J[1][c] = J;
         ph = ((a.th[1][c] + (a.d1 * a.tau[1][c]) * J) - a.syz * (a.h[0][c] - a.h[0][c - sz])) +
              a.syx * (a.h[2][c] - a.h[2][c - 1]);
+        if (a.nkick) ph += (a.D[1] ? a.kay * sqrt(a.D[1][c]) : a.kay) * noise_link(a.nz, g, 1, c, i, j, k);
         if (a.gather && a.rg.in(1, i, j, k)) {
           const double b = ph - a.th[1][c];
           a.B[1][c] = b;
@@ -223,6 +433,7 @@ __global__ void __launch_bounds__(256) k_field(Geo g, FieldArgs a) {
         a.J[2][c] = J;
         ph = ((a.th[2][c] + (a.d1 * a.tau[2][c]) * J) - a.szx * (a.h[1][c] - a.h[1][c - 1])) +
              a.szy * (a.h[0][c] - a.h[0][c - sy]);
+        if (a.nkick) ph += (a.D[2] ? a.kaz * sqrt(a.D[2][c]) : a.kaz) * noise_link(a.nz, g, 2, c, i, j, k);
         if (a.gather && a.rg.in(2, i, j, k)) {
           const double b = ph - a.th[2][c];
           a.B[2][c] = b;

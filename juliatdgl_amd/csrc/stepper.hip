@@ -145,6 +145,16 @@ struct tdgl_handle {
   hipGraphExec_t gexec = nullptr;
   HostSnap gend{};
   double *bk[48] = {};     // device copy of a cycle's start (roll back)
+  // thermal noise: the stream's step counter on the device ([0] steps taken, [1] the step of the
+  // first of the caller's draws) and its host copy; the caller's draws; SK-ROCK's δθ and source;
+  // the frame rate between its two low-pass sweeps
+  bool noise = false;
+  uint64_t *nctr = nullptr;
+  double *ndraw = nullptr;
+  long long nstride = 0, nslots = 0;
+  uint64_t nctr_h = 0, nbase_h = 0;
+  cplx *Sp = nullptr;
+  double *Qth = nullptr, *Om2 = nullptr;
   int64_t rollbacks = 0, graph_cycles = 0;
   // counters
   int64_t steps = 0, calls = 0, pcg_iters = 0;
@@ -268,6 +278,20 @@ int prof_mark(tdgl_handle *h, int cls, bool begin) {
   } while (0)
 
 // ---------------------------------------------------------------------------- band update
+NoiseArgs noise_args(const tdgl_handle *h) {
+  NoiseArgs z{};
+  z.ctr = h->nctr;
+  z.hd = h->ndraw;
+  z.hstride = h->nstride;
+  z.nslots = h->nslots;
+  z.seed = h->d.noise_seed;
+  z.host = h->d.noise_host;
+  z.nb = h->d.nbands;
+  z.Tdt = h->d.noise_T * h->d.dt;
+  z.dV = h->d.dx * h->d.dy * h->d.dz;
+  return z;
+}
+
 StageArgs stage_base(tdgl_handle *h, int mu, const tdgl_bandcoef &cf) {
   StageArgs a{};
   for (int nu = 0; nu < h->d.nbands; ++nu) a.G[nu] = h->G[nu];
@@ -291,7 +315,28 @@ StageArgs stage_base(tdgl_handle *h, int mu, const tdgl_bandcoef &cf) {
   a.dt = h->d.dt;
   a.hard = h->d.hard;
   a.lsrc = LS_DIRECT;
+  if (h->noise) {
+    a.kwt_rot = 1;
+    a.nz = noise_args(h);
+    a.Om2 = h->Om2;
+    a.Qth = h->Qth;
+    a.nGm = h->d.noise_Gamma[mu];
+    a.ng2 = h->d.bcs[mu].gamma2;
+    a.nsk = h->d.noise_sk[mu];
+  }
   return a;
+}
+
+// the frame mode of a band's first (sub)stage and of the later ones: with noise the rate is
+// low-passed and the terms are rotations (bands.jl:210-211)
+int frame_first(const tdgl_handle *h) { return h->d.rotframe ? (h->noise ? FRAME_RAW : FRAME_RATE) : FRAME_NONE; }
+int frame_later(const tdgl_handle *h) { return h->d.rotframe ? (h->noise ? FRAME_TURNTERM : FRAME_TERM) : FRAME_NONE; }
+
+// a stage launch, and with FRAME_RAW the low-pass sweeps that finish it
+int stage_launch(tdgl_handle *h, const StageArgs &a) {
+  PL(K_STAGE, launch_stage(h->g, a, h->s));
+  if (a.frame == FRAME_RAW) PL(K_STAGE, launch_frame_smooth(h->g, a, h->s));
+  return 0;
 }
 
 // the turn of band µ by its normal-fluid potential, on the launch that writes its new values
@@ -322,6 +367,8 @@ int band_rkl(tdgl_handle *h, int mu, int ns) {
   const bool rot = d.rotframe, ex = d.extrap;
   cplx *buf[3] = {h->F[mu], h->W, h->W2};
   const cplx *A = h->G[mu], *B = h->G[mu];
+  const bool lg = h->noise && d.bcs[mu].gamma2 > 0;   // _logcomb (bands.jl:455)
+  const bool skq = h->noise && d.noise_sk[mu];         // _skrock (bands.jl:446)
   for (int j = 1; j <= ns; ++j) {
     cplx *C = buf[(((j - ns) % 3) + 3) % 3];
     StageArgs a = stage_base(h, mu, d.bcs[mu]);
@@ -334,16 +381,40 @@ int band_rkl(tdgl_handle *h, int mu, int ns) {
       a.lc = kc ? d.rkc_c[mu][j - 2] : (double)((j - 1) * j) / (double)(ns * (ns + 1));
     }
     a.ghost_fly = j >= 2;
-    a.frame = rot ? (j == 1 ? FRAME_RATE : FRAME_TERM) : FRAME_NONE;
+    if (j == 1 && skq) {
+      // SK-ROCK's first stage: K₁ = E(Ψⁿ e^{iν₁δθ}) e^{i(κ₁−ν₁)δθ} (bands.jl:482-487)
+      SkArgs k{};
+      k.G = h->G[mu];
+      k.f = h->f;
+      k.Qth = h->Qth;
+      k.Sp = h->Sp;
+      k.mu = mu;
+      k.hard = d.hard;
+      k.c = std::sqrt(d.noise_T * d.dt / (d.dx * d.dy * d.dz));
+      k.g2 = d.bcs[mu].gamma2;
+      k.Gm = d.noise_Gamma[mu];
+      k.nu1 = d.noise_sk_w[mu][0];
+      k.nz = noise_args(h);
+      PL(K_STAGE, launch_sk_src(h->g, k, h->s));
+      a.S = h->Sp;
+      a.ghost_fly = 1;
+      a.sk_on = 1;
+      a.sk_post = d.noise_sk_w[mu][1] - d.noise_sk_w[mu][0];
+    }
+    a.frame = j == 1 ? frame_first(h) : frame_later(h);
     a.h = hh;
     if (j >= 2) {
       a.combine = 1;
+      a.logcomb = lg;
       a.cmu = kc ? d.rkc_mu[mu][j - 1] : (2.0 * j - 1) / j;
       a.cnu = kc ? d.rkc_nu[mu][j - 1] : (1.0 - j) / j;
     }
     a.turn = rot && j == ns;
-    if (j == ns) stage_qp(h, a, mu);
-    PL(K_STAGE, launch_stage(h->g, a, h->s));
+    if (j == ns) {
+      stage_qp(h, a, mu);
+      a.noise = h->noise;
+    }
+    CHK(stage_launch(h, a));
     A = B;
     B = C;
   }
@@ -369,11 +440,14 @@ int band_substeps(tdgl_handle *h, int mu, int n) {
       a.lframe = rot;
     }
     a.ghost_fly = m >= 2;
-    a.frame = rot ? (m == 1 ? FRAME_RATE : FRAME_TERM) : FRAME_NONE;
+    a.frame = m == 1 ? frame_first(h) : frame_later(h);
     a.h = hh;
     a.turn = rot && m == n;
-    if (m == n) stage_qp(h, a, mu);
-    PL(K_STAGE, launch_stage(h->g, a, h->s));
+    if (m == n) {
+      stage_qp(h, a, mu);
+      a.noise = h->noise;
+    }
+    CHK(stage_launch(h, a));
     src = dst;
   }
   return 0;
@@ -389,11 +463,12 @@ int update_bands(tdgl_handle *h) {
       StageArgs a = stage_base(h, mu, d.bc[mu]);
       a.D = h->F[mu];
       a.S = h->G[mu];
-      a.frame = d.rotframe ? FRAME_RATE : FRAME_NONE;
+      a.frame = frame_first(h);
       a.h = d.dt;
       a.turn = d.rotframe;
       stage_qp(h, a, mu);
-      PL(K_STAGE, launch_stage(h->g, a, h->s));
+      a.noise = h->noise;
+      CHK(stage_launch(h, a));
     } else {
       CHK(band_substeps(h, mu, d.nsub[mu]));
     }
@@ -1028,6 +1103,15 @@ int step_once(tdgl_handle *h) {
   fa.scale = d.field_mode == TDGL_FIELD_PCG || d.field_mode == TDGL_FIELD_EXPLICIT_SCALED;
   fa.gather = d.field_mode == TDGL_FIELD_PCG || d.field_mode == TDGL_FIELD_SEPARABLE;
   fa.part = d.field_mode == TDGL_FIELD_PCG ? h->pd.part + (size_t)PART_NB * 2 * MAX_PART : nullptr;
+  if (h->noise && d.noise_field) {
+    // field_kick! (noise.jl:184-186): c = √(T̃ dt/(σn dV)), the kick of a link d_l c √D_l n
+    const double c = std::sqrt(d.noise_T * d.dt / (d.noise_sigma_n * d.dx * d.dy * d.dz));
+    fa.nkick = 1;
+    fa.kax = c * d.dx;
+    fa.kay = c * d.dy;
+    fa.kaz = c * d.dz;
+    fa.nz = noise_args(h);
+  }
   PL(K_FIELD, launch_field(h->g, fa, h->s));
   if (d.field_mode == TDGL_FIELD_SEPARABLE) {
     SepOut o{nullptr, true, 0};
@@ -1067,6 +1151,7 @@ int step_once(tdgl_handle *h) {
   for (int c = 0; c < 3; ++c) ga.U[c] = h->U[c];
   ga.nb = d.nbands;
   PL(K_GHOSTS, launch_ghosts(h->g, ga, h->s));
+  if (h->noise) PL(K_VEC, launch_noise_tick(h->nctr, h->s));
   // advance! (field.jl:248-263): F equals G outside the block, so G <- F is a swap
   for (int mu = 0; mu < d.nbands; ++mu) std::swap(h->G[mu], h->F[mu]);
   for (int c = 0; c < 3; ++c) {
@@ -1110,6 +1195,7 @@ CopyList backup_list(tdgl_handle *h, bool save) {
     }
     add((double *)h->ofl, h->bk[q++], 2);  // the clock: four ints
   }
+  if (h->d.field_mode == TDGL_FIELD_PCG && h->noise) add((double *)h->nctr, h->bk[q++], 2);  // the stream's clock
   return c;
 }
 
@@ -1211,6 +1297,10 @@ int cycle_graph(tdgl_handle *h, int n) {
 }
 
 int d_maxit(tdgl_handle *h) { return h->d.pcg_maxit; }
+
+// the most steps one captured graph holds (even, so that the replays of a long call keep the
+// parity of advance!'s swaps and reuse one graph)
+constexpr int GRAPH_MAX_STEPS = 128;
 
 int check_desc(const tdgl_desc *d, char *why, size_t n) {
   auto no = [&](const char *m) {
@@ -1351,6 +1441,17 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     if (stages || subs) ALLOC(h->W, n);
     if (stages) ALLOC(h->W2, n);
     if (d.rotframe) ALLOC(h->Om, n);
+    h->noise = d.noise != 0;
+    if (h->noise) {
+      ALLOC(h->nctr, 2);
+      bool sk = false;
+      for (int mu = 0; mu < d.nbands; ++mu) sk = sk || d.noise_sk[mu];
+      if (sk) {
+        ALLOC(h->Sp, n);
+        ALLOC(h->Qth, n);
+      }
+      if (d.rotframe) ALLOC(h->Om2, n);
+    }
     CHK(put(h, h->f, d.f, d.Nx + 1, d.Ny + 1, d.Nz + 1, 8));
     const double *taus[3] = {d.taux, d.tauy, d.tauz};
     for (int c = 0; c < 3; ++c) CHK(put(h, h->tau[c], taus[c], d.Nx + 1, d.Ny + 1, d.Nz + 1, 8));
@@ -1505,6 +1606,7 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
         for (int k = 0; k < 6; ++k) ALLOC(h->bk[q++], n);
         ALLOC(h->bk[q++], 2);
       }
+      if (h->noise) ALLOC(h->bk[q++], 2);
     }
     // TDGL_HIP_GRAPH=0: every step launched from the host; TDGL_HIP_BUDGET=k: the captured PCG
     // starts with k iterations per solve instead of learning it from an eager first cycle
@@ -1537,6 +1639,7 @@ int tdgl_destroy(tdgl_handle *h) {
   if (h->s) (void)hipStreamSynchronize(h->s);
   for (void *p : h->allocs) (void)hipFree(p);
   for (void *p : h->mallocs) (void)hipFree(p);
+  if (h->ndraw) (void)hipFree(h->ndraw);
   destroy_graph(h);
   if (h->hph) (void)hipHostFree(h->hph);
   if (h->flh) (void)hipHostFree(h->flh);
@@ -1639,11 +1742,23 @@ int tdgl_download_state(tdgl_handle *h, tdgl_state *s) {
   return 0;
 }
 
+// the PCG counters of the cycle whose flags are in flh
+static void pcg_account(tdgl_handle *h) {
+  h->pcg_iters += h->flh[FL_ITSUM];
+  h->last_iters = h->flh[FL_IT];
+  // the graph's iterations per solve: the most one solve took, and a margin
+  if (h->budget == 0) h->budget = std::min(d_maxit(h), std::max(4, h->flh[FL_ITMAX] + 3));
+}
+
 int tdgl_step(tdgl_handle *h, int32_t n, double H, double HJ) {
   if (!h) return fail(TDGL_E_ARG, "null handle");
   if (n < 0) return fail(TDGL_E_ARG, "negative step count");
   if (!std::isfinite(H) || !std::isfinite(HJ)) return fail(TDGL_E_ARG, "non-finite H or HJ");
   const bool pcg = h->d.field_mode == TDGL_FIELD_PCG;
+  if (h->noise && h->d.noise_host && n > 0 && h->nctr_h + (uint64_t)n > h->nbase_h + (uint64_t)h->nslots)
+    return fail(TDGL_E_ARG, "noise: the caller's draws cover %lld more steps, %d asked",
+                (long long)(h->nbase_h + (uint64_t)h->nslots - std::min<uint64_t>(h->nctr_h, h->nbase_h + (uint64_t)h->nslots)),
+                n);
   if (n > 0) h->stepped = true;
   h->hph[0] = H;
   h->hph[1] = HJ;
@@ -1656,22 +1771,29 @@ int tdgl_step(tdgl_handle *h, int32_t n, double H, double HJ) {
       CHK(steps_eager(h, n));
       if (pcg) HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
     } else {
-      CHK(cycle_graph(h, n));
-      h->graph_cycles += 1;
+      // a long call replays a graph of at most GRAPH_MAX_STEPS steps: a captured stream is one
+      // chain of dependent nodes (about 15 a step), which the runtime walks when it instantiates
+      // and launches it, so its length stays bounded
+      for (int left = n; left > 0;) {
+        const int m = std::min(left, GRAPH_MAX_STEPS);
+        CHK(cycle_graph(h, m));
+        h->graph_cycles += 1;
+        left -= m;
+        if (pcg && left > 0) {
+          HIPCHK(hipStreamSynchronize(h->s));
+          pcg_account(h);
+        }
+      }
     }
   }
   HIPCHK(hipEventRecord(h->e1, h->s));
   HIPCHK(hipStreamSynchronize(h->s));
   if (h->prof) CHK(prof_flush(h));
-  if (pcg && n > 0) {
-    h->pcg_iters += h->flh[FL_ITSUM];
-    h->last_iters = h->flh[FL_IT];
-    // the graph's iterations per solve: the most one solve took, and a margin
-    if (h->budget == 0) h->budget = std::min(d_maxit(h), std::max(4, h->flh[FL_ITMAX] + 3));
-  }
+  if (pcg && n > 0) pcg_account(h);
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, h->e0, h->e1));
   h->device_ms += ms;
+  if (h->noise) h->nctr_h += (uint64_t)n;
   h->steps += n;
   h->calls += 1;
   return 0;
@@ -1721,6 +1843,69 @@ int tdgl_set_pcg(tdgl_handle *h, const tdgl_pcg *p) {
   h->warm = p->warm != 0;
   h->have2 = p->have2 != 0;
   h->last_iters = p->iters;
+  return 0;
+}
+
+int tdgl_noise_get(tdgl_handle *h, uint64_t *counter) {
+  if (!h || !counter) return fail(TDGL_E_ARG, "null argument");
+  if (!h->noise) return fail(TDGL_E_ARG, "the stepper has no noise");
+  HIPCHK(hipStreamSynchronize(h->s));
+  uint64_t c[2];
+  HIPCHK(hipMemcpy(c, h->nctr, sizeof c, hipMemcpyDeviceToHost));
+  *counter = c[0];
+  return 0;
+}
+
+int tdgl_noise_set(tdgl_handle *h, uint64_t counter) {
+  if (!h) return fail(TDGL_E_ARG, "null handle");
+  if (!h->noise) return fail(TDGL_E_ARG, "the stepper has no noise");
+  HIPCHK(hipStreamSynchronize(h->s));
+  HIPCHK(hipMemcpy(h->nctr, &counter, sizeof counter, hipMemcpyHostToDevice));
+  h->nctr_h = counter;
+  return 0;
+}
+
+int tdgl_noise_draws(tdgl_handle *h, int32_t nsteps, const double *draws) {
+  if (!h) return fail(TDGL_E_ARG, "null handle");
+  const tdgl_desc &d = h->d;
+  if (!h->noise || !d.noise_host) return fail(TDGL_E_ARG, "the stepper takes no draws of the caller (noise_host)");
+  if (nsteps < 0 || (nsteps > 0 && !draws)) return fail(TDGL_E_ARG, "bad draws");
+  HIPCHK(hipStreamSynchronize(h->s));
+  // the captured graph holds the old buffer's address
+  CHK(destroy_graph(h));
+  if (h->ndraw) {
+    HIPCHK(hipFree(h->ndraw));
+    h->ndraw = nullptr;
+  }
+  const int n = h->g.n;
+  const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz;
+  const long long stride = (long long)d.nbands * 2 * n + (d.noise_field ? 3LL * n : 0);
+  h->nstride = stride;
+  h->nslots = nsteps;
+  if (nsteps > 0) {
+    if (hipMalloc((void **)&h->ndraw, (size_t)stride * nsteps * sizeof(double)) != hipSuccess)
+      return fail(TDGL_E_NOMEM, "hipMalloc of the noise draws failed");
+    HIPCHK(hipMemset(h->ndraw, 0, (size_t)stride * nsteps * sizeof(double)));
+    const size_t nn = (size_t)(Nx + 1) * (Ny + 1) * (Nz + 1);
+    const int sh[3][3] = {{Nx, Ny + 1, Nz + 1}, {Nx + 1, Ny, Nz + 1}, {Nx + 1, Ny + 1, Nz}};
+    const double *p = draws;
+    for (int t = 0; t < nsteps; ++t) {
+      double *slot = h->ndraw + (size_t)t * stride;
+      for (int mu = 0; mu < d.nbands; ++mu, p += 2 * nn) CHK(put(h, slot + (size_t)mu * 2 * n, p, Nx + 1, Ny + 1, Nz + 1, 16));
+      if (d.noise_field)
+        for (int c = 0; c < 3; ++c) {
+          CHK(put(h, slot + (size_t)d.nbands * 2 * n + (size_t)c * n, p, sh[c][0], sh[c][1], sh[c][2], 8));
+          p += (size_t)sh[c][0] * sh[c][1] * sh[c][2];
+        }
+    }
+  }
+  // the first of these draws is for the next step
+  uint64_t c[2];
+  HIPCHK(hipMemcpy(c, h->nctr, sizeof c, hipMemcpyDeviceToHost));
+  c[1] = c[0];
+  HIPCHK(hipMemcpy(h->nctr, c, sizeof c, hipMemcpyHostToDevice));
+  h->nctr_h = c[0];
+  h->nbase_h = c[0];
   return 0;
 }
 

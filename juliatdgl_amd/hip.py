@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "_lib", "libtdglhip.so")
 SYMBOLS = ("tdgl_supported", "tdgl_create", "tdgl_destroy", "tdgl_upload_state", "tdgl_download_state", "tdgl_step",
            "tdgl_get_pcg", "tdgl_set_pcg", "tdgl_get_open", "tdgl_set_open", "tdgl_get_stats", "tdgl_last_error", "tdgl_device_count",
            "tdgl_set_device", "tdgl_profile_enable", "tdgl_profile_get", "tdgl_kernel_name", "tdgl_meter_begin",
-           "tdgl_meter_cycle")
+           "tdgl_meter_cycle", "tdgl_noise_get", "tdgl_noise_set", "tdgl_noise_draws")
 
 _lib = None
 _lock = threading.Lock()
@@ -57,6 +57,9 @@ def load():
             L.tdgl_meter_begin.argtypes = [vp, vp]
             L.tdgl_meter_cycle.argtypes = [vp, ct.c_int32, vp]
             L.tdgl_kernel_name.restype = ct.c_char_p
+            L.tdgl_noise_get.argtypes = [vp, ct.POINTER(ct.c_uint64)]
+            L.tdgl_noise_set.argtypes = [vp, ct.c_uint64]
+            L.tdgl_noise_draws.argtypes = [vp, ct.c_int32, vp]
             _lib = L
         return _lib
 
@@ -172,6 +175,45 @@ class HipStepper:
                 "Tabs": [r.Tabs[k] for k in range(N)], "dmax": r.dmax,
                 "nbig": int(r.nbig), "It": r.It, "Is": r.Is, "dif": [r.dif[k] for k in range(N)], "amax": r.amax,
                 "diffh": r.diffh}
+
+    @property
+    def noise_counter(self) -> int:
+        """The noise stream's position: steps taken with noise since it began (tdgl_noise_get)."""
+        c = ct.c_uint64()
+        _check(load().tdgl_noise_get(self.h, ct.byref(c)), "tdgl_noise_get")
+        return int(c.value)
+
+    @noise_counter.setter
+    def noise_counter(self, c: int) -> None:
+        _check(load().tdgl_noise_set(self.h, int(c)), "tdgl_noise_set")
+
+    def noise_draws(self, flat) -> None:
+        """The caller's normals for the next steps (tdgl_noise_draws; juliatdgl_amd.noise.pack
+        lays them out)."""
+        gr, st = self.st.grid, self.st
+        nn = (gr.Nx + 1) * (gr.Ny + 1) * (gr.Nz + 1)
+        nl = sum(a * b * c for a, b, c in ((gr.Nx, gr.Ny + 1, gr.Nz + 1), (gr.Nx + 1, gr.Ny, gr.Nz + 1),
+                                           (gr.Nx + 1, gr.Ny + 1, gr.Nz)))
+        stride = st.N * 2 * nn + (nl if st.noise is not None and st.noise.field else 0)
+        flat = np.ascontiguousarray(flat, dtype=np.float64)
+        if flat.size % stride:
+            raise ValueError("noise draws: not a whole number of steps")
+        _check(load().tdgl_noise_draws(self.h, flat.size // stride, flat.ctypes.data), "tdgl_noise_draws")
+
+    def noise_extra(self) -> dict:
+        """What a checkpoint holds of the noise stream, in place of the reference's Xoshiro state
+        (noise_rng, protocol.jl:659): its seed and position."""
+        if self.st.noise is None:
+            return {}
+        return {"noise_seed": int(self.st.noise.seed), "noise_counter": self.noise_counter}
+
+    def noise_restore(self, w: dict) -> None:
+        """Resuming the stream where a checkpoint left it (protocol.jl:382)."""
+        if self.st.noise is None or "noise_counter" not in w:
+            return
+        if int(w.get("noise_seed", self.st.noise.seed)) != self.st.noise.seed:
+            raise ValueError("resume: the checkpoint's noise stream has another seed")
+        self.noise_counter = int(w["noise_counter"])
 
     def profile(self, on: bool) -> None:
         """Per-kernel HIP-event timing of every launch while on (clears the totals)."""

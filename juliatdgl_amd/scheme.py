@@ -15,6 +15,7 @@ normal fluids and the open boundary are refused (``Stepper`` raises NotImplement
 from __future__ import annotations
 
 import math
+import warnings
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -310,11 +311,13 @@ def _rkc_cheb(s: int, w0: float):
 
 @dataclass
 class RKCCoef:
-    """RKCCoef (scheme.jl:77-100): q = ω₀/ω₁ and per-stage μ, ν, c."""
+    """RKCCoef (scheme.jl:77-100): q = ω₀/ω₁, per-stage μ, ν, c, and SK-ROCK's weights of the
+    noise in the first stage sk = (ν₁, κ₁) = (sω₁/2, sω₁/ω₀)."""
     q: float
     mu: list
     nu: list
     c: list
+    sk: tuple = (0.0, 0.0)
 
     @staticmethod
     def make(s: int, eps: float = RKC_EPS) -> "RKCCoef":
@@ -331,7 +334,7 @@ class RKCCoef:
                 mu[j - 1] = 2 * w0 * _rkc_cheb(j - 1, w0)[0] / Tj
                 nu[j - 1] = -_rkc_cheb(j - 2, w0)[0] / Tj
         c[s - 1] = 1.0
-        return RKCCoef(w0 / w1, mu, nu, c)
+        return RKCCoef(w0 / w1, mu, nu, c, (s * w1 / 2, s * w1 / w0))
 
 
 def _rkc_ratio(s: int, eps: float = RKC_EPS) -> float:
@@ -756,6 +759,17 @@ def qp_exchange_data(md, gr, nu, split, conductor: bool, dt: float) -> QPData:
                   jr[0] - 1, nj, kr[0] - 1, nk, bases, G)
 
 
+@dataclass(frozen=True)
+class NoiseSpec:
+    """Noise(T̃, seed; field) (noise.jl:32-41) and where the normals come from: `host` False draws
+    them on the device (Philox4x32-10 keyed by the seed), True takes the caller's per step
+    (tdgl_noise_draws)."""
+    T: float
+    seed: int
+    field: bool = False
+    host: bool = False
+
+
 @dataclass
 class Stepper:
     """The coefficients of Stepper{N} (scheme.jl:167-198) that the step reads."""
@@ -789,6 +803,8 @@ class Stepper:
     selffield_bc: str = "bar"
     qp: Optional[QPData] = None
     open: Optional[OpenData] = None
+    noise: Optional[NoiseSpec] = None
+    sk: tuple = ()            # _skrock(st, µ) per band: the phase kick inside the first RKC stage
 
     @property
     def N(self) -> int:
@@ -803,16 +819,26 @@ def make_stepper(md: Model, gr: Grid, mk: Masks, *, implicit=True, substeps=None
                  extrap=True, hard=False, sigma_vac=None, pcg_tol=1e-8, selffield_bc="bar",
                  rkl="rkc", rotframe=None, rkl_margin=None, rkc_damping=RKC_EPS, pcg_fast=True,
                  pcg_f32=True, pcg_extrap=True, dt=None, pcg_maxit=500, qp_exchange=None, qp_split=None,
-                 open_every=1.0, **unsupported) -> Stepper:
+                 open_every=1.0, noise=None, noise_seed=1, noise_field=False, noise_draws="device",
+                 **unsupported) -> Stepper:
     """Stepper(md, gr, mk; kw...) (scheme.jl:250-434) for the options the HIP stepper runs.
 
     `dt` (an extension): a fixed time step in place of the derived one; the substep and stage
     counts follow from it by the same rules (Euler substeps so that each stays within 0.99 of
-    its band's bound, stages from r = rkl_margin·dt/(0.99·bbk))."""
+    its band's bound, stages from r = rkl_margin·dt/(0.99·bbk)).
+
+    `noise = T̃`, `noise_seed`, `noise_field` as the reference's (noise.jl); `noise_draws`
+    "device" (Philox on the device, keyed by the seed) or "host" (the caller's draws per step)."""
     for k, v in unsupported.items():
         if v is not None and v is not False:
             raise NotImplementedError(f"Stepper option {k!r} is not run by the HIP stepper")
     N = md.N
+    if noise is not None and not float(noise) >= 0:
+        raise ValueError("noise: T̃ must be >= 0")
+    if noise_field and noise is None:
+        raise ValueError("noise_field needs noise = T̃ (the same bath for Ψ and A)")
+    if noise_draws not in ("device", "host"):
+        raise ValueError("noise_draws must be 'device' or 'host'")
     substeps = tuple(1 if mu == 0 else 0 for mu in range(N)) if substeps is None else tuple(substeps)
     if isinstance(rkl, bool):
         rkc = False
@@ -879,12 +905,21 @@ def make_stepper(md: Model, gr: Grid, mk: Masks, *, implicit=True, substeps=None
             q = 2
             while _rkc_ratio(q, float(rkc_damping)) < r - 1e-12:
                 q += 1
-            if q < s:
+            # with noise a KWT band takes RKC even at the same count (SK-ROCK)
+            if q < s or (noise is not None and bands[mu].gkwt > 0):
                 s = -q
         stg.append(s)
     usec = [s < 0 for s in stg]
+    skb = tuple(noise is not None and rkc and usec[mu] and bands[mu].gkwt > 0 for mu in range(N))
     stg = [abs(s) for s in stg]
     nsub = [1 if stg[mu] > 0 else nsub[mu] for mu in range(N)]
+    if noise is not None and any(bands[mu].gkwt > 0 and not skb[mu] and stg[mu] > 0 for mu in range(N)):
+        raise ValueError("noise with KWT on RKL stages: the equilibrium comes out biased (10-25 % low at "
+                         "T̃ = 1e-2). Use rkl = 'rkc' (the default: the noise inside the stages, SK-ROCK) or "
+                         "rkl = False (Euler substeps)")
+    if noise is not None and any(bands[mu].gkwt > 0 and not skb[mu] and nsub[mu] > 1 for mu in range(N)):
+        warnings.warn("noise with KWT on Euler substeps: the kick after the step over-heats the stiff phase "
+                      "modes (it converges as dt resolves them); rkl = 'rkc' puts the noise inside the stages")
     rkcc = tuple(RKCCoef.make(stg[mu], float(rkc_damping)) if usec[mu] else None for mu in range(N)) if rkc else None
 
     def divisor(mu):
@@ -914,6 +949,9 @@ def make_stepper(md: Model, gr: Grid, mk: Masks, *, implicit=True, substeps=None
     bar = bar_field(gr) if sbc in ("bar", "open") else None
     ob = open_data(gr, md.kappa, bar, max(1, round(open_every / dtv))) if sbc == "open" else None
     qp = None if qp_exchange is None else qp_exchange_data(md, gr, qp_exchange, qp_split, pad, dtv)
+    nz = None if noise is None else NoiseSpec(float(noise), int(noise_seed) & 0xFFFFFFFFFFFFFFFF, bool(noise_field),
+                                              noise_draws == "host")
     return Stepper(md, gr, mk, dtv, tuple(nsub), tuple(stg), bc, bcs, w, wa, fc, mode, D, bar, bool(extrap), rotframe,
                    bool(hard), limit, float(dt_scale), rkl_margin, rkcc, tuple(float(divisor(mu)) for mu in range(N)),
-                   float(pcg_tol), int(pcg_maxit), bool(pcg_fast), bool(pcg_f32), bool(pcg_extrap), sbc, qp, ob)
+                   float(pcg_tol), int(pcg_maxit), bool(pcg_fast), bool(pcg_f32), bool(pcg_extrap), sbc, qp, ob,
+                   nz, skb)

@@ -9,7 +9,9 @@
 // stages; KWT; the rotating frame; supercurrent; field increment; the implicit curl-curl by the
 // separable solve or by PCG (exact, fast, Float32 preconditioner); explicit and D-scaled
 // explicit field; links; curl; slab and bar field boundaries; Neumann ghosts; advance; two and N
-// normal fluids with a finite exchange (uniform, separable: exchange.jl).
+// normal fluids with a finite exchange (uniform, separable: exchange.jl); thermal noise
+// (noise.jl: Euler–Maruyama and polar KWT kicks, SK-ROCK, the field kick) with the KWT log-polar
+// stepping and the low-passed frame that come with it (bands.jl).
 // Arrays are Julia-shaped (column-major, 1-based accessors) so every index reads as in the
 // reference.  Complex arithmetic is written out with the formulas of Julia's Base (no fused
 // multiply-add: build with -ffp-contract=off).
@@ -19,8 +21,11 @@
 // forms of gtdgl/test/test_physics.jl and TDGLGPU/test/test_invariants.jl (KWT 2x2 solve,
 // amplitude relaxation rates, uniform equilibrium, pure gauge); RKC polynomial identities; the
 // separable solve against a dense inverse of (I + aK); PCG against the separable solve at
-// D = 1; the manufactured rotations of gtdgl/test/rotation_fixture.jl.  The tests that do so
-// are tests/test_oracle_*.py.
+// D = 1; the manufactured rotations of gtdgl/test/rotation_fixture.jl; the noise by the
+// equilibria of gtdgl/test/test_physics.jl:475-535, 697-735 (exp(−F/T̃) of decoupled nodes with and
+// without KWT, the phase and amplitude rates, equipartition of the field's modes) and its normals
+// by the Philox4x32-10 known-answer vectors of Random123.  The tests that do so are
+// tests/test_oracle_*.py.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -62,6 +67,15 @@ struct A3 {
 using A3c = A3<C>;
 using A3r = A3<double>;
 
+template <class T>
+void load(A3<T> &a, const double *p) {
+  std::memcpy(a.v.data(), p, a.v.size() * sizeof(T));
+}
+template <class T>
+void save(const A3<T> &a, double *p) {
+  std::memcpy(p, a.v.data(), a.v.size() * sizeof(T));
+}
+
 const double ROT_MIN2 = 1e-3;  // bands.jl:208
 const double ROT_MAXDT = 0.5;  // bands.jl:209
 
@@ -74,6 +88,19 @@ inline C kwt_update(C g, C d, double g2) {
   C dp = rmul(std::sqrt(q), d);
   double s = cmul(conj(g), dp).re / q;
   return cadd(g, csub(dp, rmul(s, rmul(g2, g))));
+}
+
+// kwt_update_rot, bands.jl:150-158: the increment's part across ψ as a rotation
+inline C kwt_update_rot(C g, C d, double g2) {
+  double a2 = abs2(g);
+  double q = 1 + g2 * a2;
+  C dp = rmul(std::sqrt(q), d);
+  C Dl = csub(dp, rmul(cmul(conj(g), dp).re / q, rmul(g2, g)));
+  if (!(a2 > 1e-24)) return cadd(g, Dl);
+  double a = std::sqrt(a2);
+  C u = {g.re / a, g.im / a};
+  C w = cmul(conj(u), Dl);
+  return cmul(u, rmul(a + w.re, cis(w.im / a)));
 }
 
 // _cis, bands.jl:289
@@ -271,6 +298,18 @@ struct orc {
   // two recomputations (h-shaped)
   int ocount = -1;
   A3r odJ[3], oth[3], oold[3], onew[3];
+  // thermal noise (noise.jl): the stream position, the draws of the current step (per band at the
+  // nodes, per family at the links), SK-ROCK's phase kicks per band, the caller's draws
+  uint64_t nctr = 0;
+  std::vector<A3c> Xi;
+  A3r Nl[3];
+  std::vector<A3r> Qth;
+  A3c Sp;
+  A3r Om2;
+  std::vector<double> hdraws;
+  int hslots = 0;
+  uint64_t hbase = 0;
+  bool noise = false, sk_any = false;
 };
 
 namespace {
@@ -313,7 +352,9 @@ void band_pass(orc &o, A3c &D, const A3c &S, int mu, const A3c &Lx, const A3c &L
           if (nu == mu) continue;
           acc = cadd(acc, rmul(cf.cpl[nu], o.G[nu](i, j, k)));
         }
-        D(i, j, k) = cf.gamma2 > 0 ? kwt_update(g, csub(acc, g), cf.gamma2) : acc;
+        D(i, j, k) = cf.gamma2 > 0 ? (o.noise ? kwt_update_rot(g, csub(acc, g), cf.gamma2)
+                                             : kwt_update(g, csub(acc, g), cf.gamma2))
+                                  : acc;
       }
 }
 
@@ -430,6 +471,90 @@ void combine(orc &o, A3c &Cc, const A3c &A, double mu, double nu) {
       for (int i = o.x1 + 1; i <= o.x2; ++i) Cc(i, j, k) = cadd(rmul(mu, Cc(i, j, k)), rmul(nu, A(i, j, k)));
 }
 
+// faces!(Ω) of _frame_rate_smooth! (bands.jl:250-255): the faces copy the interior, x, y, z
+void om_faces(orc &o, A3r &Om) {
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int j = o.y1 + 1; j <= o.y2; ++j) {
+      Om(o.x1, j, k) = Om(o.x1 + 1, j, k);
+      Om(o.x2 + 1, j, k) = Om(o.x2, j, k);
+    }
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int i = o.x1; i <= o.x2 + 1; ++i) {
+      Om(i, o.y1, k) = Om(i, o.y1 + 1, k);
+      Om(i, o.y2 + 1, k) = Om(i, o.y2, k);
+    }
+  for (int j = o.y1; j <= o.y2 + 1; ++j)
+    for (int i = o.x1; i <= o.x2 + 1; ++i) {
+      Om(i, j, o.z1) = Om(i, j, o.z1 + 1);
+      Om(i, j, o.z2 + 1) = Om(i, j, o.z2);
+    }
+}
+
+// _frame_rate_smooth! (bands.jl:243-270), with noise: the rate, two low-pass sweeps
+// Ω ← Ω/2 + (sum of the 6 neighbours)/12, and the frame term as the rotation C ← C e^{−iω̂h}
+void frame_rate_smooth(orc &o, A3r &Om, A3c &Cc, const A3c &Y, double h, double dt, int passes) {
+  const double wmax = ROT_MAXDT / dt;
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int j = o.y1 + 1; j <= o.y2; ++j)
+      for (int i = o.x1 + 1; i <= o.x2; ++i) {
+        C y = Y(i, j, k);
+        double a2 = abs2(y);
+        Om(i, j, k) = a2 > ROT_MIN2 ? jclamp(cmul(conj(y), csub(Cc(i, j, k), y)).im / (a2 * h), -wmax, wmax) : 0.0;
+      }
+  om_faces(o, Om);
+  for (int p = 0; p < passes; ++p) {
+    o.Om2 = Om;
+    const A3r &T = o.Om2;
+    for (int k = o.z1 + 1; k <= o.z2; ++k)
+      for (int j = o.y1 + 1; j <= o.y2; ++j)
+        for (int i = o.x1 + 1; i <= o.x2; ++i) {
+          double nb = (T(i + 1, j, k) + T(i - 1, j, k)) + (T(i, j + 1, k) + T(i, j - 1, k)) + (T(i, j, k + 1) + T(i, j, k - 1));
+          Om(i, j, k) = 0.5 * T(i, j, k) + nb / 12;
+        }
+    om_faces(o, Om);
+  }
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int j = o.y1 + 1; j <= o.y2; ++j)
+      for (int i = o.x1 + 1; i <= o.x2; ++i) Cc(i, j, k) = cmul(Cc(i, j, k), cis(-(Om(i, j, k) * h)));
+}
+
+// _frame_term!(...; turn = true) (bands.jl:276-281): with noise the frame term is a rotation
+void frame_term_turn(orc &o, A3c &Cc, const A3r &Om, double h) {
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int j = o.y1 + 1; j <= o.y2; ++j)
+      for (int i = o.x1 + 1; i <= o.x2; ++i) Cc(i, j, k) = cmul(Cc(i, j, k), cis(-(Om(i, j, k) * h)));
+}
+
+// complex quotient, log and exp (z/w by the textbook formula; log z = log|z| + i arg z with
+// |z| = hypot; e^z = e^{Re z} e^{i Im z}): the device computes them alike
+inline C cdiv(C a, C b) {
+  double den = b.re * b.re + b.im * b.im;
+  return {(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
+}
+inline C clog(C z) { return {std::log(std::hypot(z.re, z.im)), std::atan2(z.im, z.re)}; }
+inline C cexp(C z) { return rmul(std::exp(z.re), cis(z.im)); }
+
+// _combine_log! (bands.jl:439-445): the stages of a KWT band with noise combined in log-polar form
+void combine_log(orc &o, A3c &Cc, const A3c &A, double mu) {
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int j = o.y1 + 1; j <= o.y2; ++j)
+      for (int i = o.x1 + 1; i <= o.x2; ++i) {
+        C a = A(i, j, k), c = Cc(i, j, k);
+        Cc(i, j, k) = abs2(a) > ROT_MIN2 && abs2(c) > ROT_MIN2 ? cmul(a, cexp(rmul(mu, clog(cdiv(c, a)))))
+                                                              : cadd(rmul(mu, c), rmul(1 - mu, a));
+      }
+}
+
+// _sk_turn! (bands.jl:447-453): D ← X e^{i c δθ} on the interior
+void sk_turn(orc &o, A3c &D, const A3c &X, const A3r &Qt, double c) {
+  for (int k = o.z1 + 1; k <= o.z2; ++k)
+    for (int j = o.y1 + 1; j <= o.y2; ++j)
+      for (int i = o.x1 + 1; i <= o.x2; ++i) {
+        double q = Qt(i, j, k);
+        D(i, j, k) = q == 0.0 ? X(i, j, k) : cmul(X(i, j, k), cis(c * q));
+      }
+}
+
 // ghosts! (bands.jl:524-539)
 void ghosts(orc &o, A3c &X, const A3c &Vx, const A3c &Vy, const A3c &Vz) {
   for (int k = o.z1; k <= o.z2 + 1; ++k)
@@ -474,10 +599,16 @@ void band_substeps(orc &o, int mu, int n) {
     A3c *dst = ((n - m) % 2 == 0) ? &F : &W;
     band_pass(o, *dst, *src, mu, *Lx, *Ly, *Lz, cf);
     if (rt) {
-      if (m == 1)
-        frame_rate(o, o.Om, *dst, *src, h, o.d.dt);
-      else
+      if (m == 1) {
+        if (o.noise)
+          frame_rate_smooth(o, o.Om, *dst, *src, h, o.d.dt, 2);
+        else
+          frame_rate(o, o.Om, *dst, *src, h, o.d.dt);
+      } else if (o.noise) {
+        frame_term_turn(o, *dst, o.Om, h);
+      } else {
         frame_term(o, *dst, *src, o.Om, h);
+      }
     }
     if (m < n) {
       if (o.d.extrap) advance_links(o, *Lx, *Ly, *Lz, o.L[3], o.L[4], o.L[5]);
@@ -489,7 +620,7 @@ void band_substeps(orc &o, int mu, int n) {
   if (rt) frame_turn(o, F, o.Om, o.d.dt);
 }
 
-// _band_rkl! without noise (bands.jl:464-521)
+// _band_rkl! (bands.jl:464-521)
 void band_rkl(orc &o, int mu, int ns) {
   const tdgl_bandcoef &cf = o.d.bcs[mu];
   bool kc = o.d.rkc[mu] != 0;
@@ -514,14 +645,32 @@ void band_rkl(orc &o, int mu, int ns) {
   const A3c *A = &o.G[mu];
   const A3c *B = &o.G[mu];
   A3c *Cc = buf[0];
+  const bool lg = o.noise && o.d.bcs[mu].gamma2 > 0;  // _logcomb (bands.jl:455)
+  const bool skq = o.noise && o.d.noise_sk[mu];       // _skrock (bands.jl:446)
   for (int j = 1; j <= ns; ++j) {
     Cc = buf[(j - 1) % 3];
-    band_pass(o, *Cc, *B, mu, *Lx, *Ly, *Lz, cf);
-    if (rt) {
-      if (j == 1)
-        frame_rate(o, o.Om, *Cc, *B, h, o.d.dt);
-      else
-        frame_term(o, *Cc, *B, o.Om, h);
+    if (j == 1 && skq) {
+      // SK-ROCK's first stage: K₁ = E(Ψⁿ e^{iν₁δθ}) e^{i(κ₁−ν₁)δθ} (bands.jl:482-487)
+      const double nu1 = o.d.noise_sk_w[mu][0], ka1 = o.d.noise_sk_w[mu][1];
+      sk_turn(o, o.Sp, *B, o.Qth[mu], nu1);
+      ghosts(o, o.Sp, *Lx, *Ly, *Lz);
+      band_pass(o, *Cc, o.Sp, mu, *Lx, *Ly, *Lz, cf);
+      if (rt) frame_rate_smooth(o, o.Om, *Cc, o.Sp, h, o.d.dt, 2);
+      sk_turn(o, *Cc, *Cc, o.Qth[mu], ka1 - nu1);
+    } else {
+      band_pass(o, *Cc, *B, mu, *Lx, *Ly, *Lz, cf);
+      if (rt) {
+        if (j == 1) {
+          if (o.noise)
+            frame_rate_smooth(o, o.Om, *Cc, *B, h, o.d.dt, 2);
+          else
+            frame_rate(o, o.Om, *Cc, *B, h, o.d.dt);
+        } else if (o.noise) {
+          frame_term_turn(o, *Cc, o.Om, h);
+        } else {
+          frame_term(o, *Cc, *B, o.Om, h);
+        }
+      }
     }
     if (rt && j == 1 && o.d.extrap && ns > 1) {
       link_rate(o, o.R[0], o.Vx, o.Ux, o.Om, o.d.dt, rx(o), 1, 0, 0);
@@ -537,7 +686,10 @@ void band_rkl(orc &o, int mu, int ns) {
         mj = (2.0 * j - 1) / j;
         nj = (1.0 - j) / j;
       }
-      combine(o, *Cc, *A, mj, nj);
+      if (lg)
+        combine_log(o, *Cc, *A, mj);
+      else
+        combine(o, *Cc, *A, mj, nj);
     }
     if (j < ns) {
       if (o.d.extrap) {
@@ -570,7 +722,10 @@ void update_bands(orc &o) {
     } else if (n == 1) {
       band_pass(o, o.F[mu], o.G[mu], mu, o.Vx, o.Vy, o.Vz, o.d.bc[mu]);
       if (o.d.rotframe) {
-        frame_rate(o, o.Om, o.F[mu], o.G[mu], o.d.dt, o.d.dt);
+        if (o.noise)
+          frame_rate_smooth(o, o.Om, o.F[mu], o.G[mu], o.d.dt, o.d.dt, 2);
+        else
+          frame_rate(o, o.Om, o.F[mu], o.G[mu], o.d.dt, o.d.dt);
         frame_turn(o, o.F[mu], o.Om, o.d.dt);
       }
     } else {
@@ -1217,13 +1372,186 @@ void qp_turn(orc &o) {
         }
 }
 
+// ---------------------------------------------------------------------------- thermal noise
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11): ten rounds of two 32x32->64 multiplications on the counter, the key bumped by the
+// Weyl constants between rounds.
+void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0, hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+  }
+}
+
+// Two standard normals from one Philox block (Box & Muller, Ann. Math. Stat. 29, 610 (1958)):
+// counter (q, step low, step high, tag), key = the seed; u in (0, 1) and v in [0, 1) from 52
+// bits each, r = √(−2 log u), (r cos 2πv, r sin 2πv).
+void normal2(uint64_t seed, uint64_t t, uint32_t tag, uint32_t q, double &n1, double &n2) {
+  uint32_t c[4] = {q, (uint32_t)t, (uint32_t)(t >> 32), tag};
+  philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const double e52 = 2.220446049250313e-16;  // 2^-52
+  const double u = ((double)(((uint64_t)(c[0] >> 6) << 26) | (uint64_t)(c[1] >> 6)) + 0.5) * e52;
+  const double v = (double)(((uint64_t)(c[2] >> 6) << 26) | (uint64_t)(c[3] >> 6)) * e52;
+  const double r = std::sqrt(-2.0 * std::log(u));
+  const double a = 6.283185307179586 * v;
+  n1 = r * std::cos(a);
+  n2 = r * std::sin(a);
+}
+
+inline bool kicked(const orc &o, int i, int j, int k) { return !(o.d.hard && o.f(i, j, k) == 0.0); }
+
+// Every draw of the step: ξ per band at the nodes the kick visits (noise.jl:47-52), n at the
+// links field_kick! visits (:188-196); from the generator or from the caller's draws.
+void noise_fill(orc &o) {
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+  const size_t nn = (size_t)(Nx + 1) * (Ny + 1) * (Nz + 1);
+  const size_t nl[3] = {(size_t)Nx * (Ny + 1) * (Nz + 1), (size_t)(Nx + 1) * Ny * (Nz + 1), (size_t)(Nx + 1) * (Ny + 1) * Nz};
+  if (o.d.noise_host) {
+    const size_t stride = (size_t)o.N * 2 * nn + (o.d.noise_field ? nl[0] + nl[1] + nl[2] : 0);
+    const uint64_t slot = o.nctr - o.hbase;
+    if (o.nctr < o.hbase || slot >= (uint64_t)o.hslots) std::abort();
+    const double *p = o.hdraws.data() + slot * stride;
+    for (int mu = 0; mu < o.N; ++mu, p += 2 * nn) load(o.Xi[mu], p);
+    if (o.d.noise_field)
+      for (int c = 0; c < 3; p += nl[c], ++c) load(o.Nl[c], p);
+    return;
+  }
+  const uint64_t seed = o.d.noise_seed, t = o.nctr;
+  for (int mu = 0; mu < o.N; ++mu)
+    for (int k = o.z1 + 1; k <= o.z2; ++k)
+      for (int j = o.y1 + 1; j <= o.y2; ++j)
+        for (int i = o.x1 + 1; i <= o.x2; ++i) {
+          if (!kicked(o, i, j, k)) continue;
+          const uint32_t q = (uint32_t)((i - 1) + (size_t)(Nx + 1) * ((j - 1) + (size_t)(Ny + 1) * (k - 1)));
+          double a, b;
+          normal2(seed, t, (uint32_t)mu, q, a, b);
+          o.Xi[mu](i, j, k) = {a, b};
+        }
+  if (!o.d.noise_field) return;
+  const int e[3][3] = {{Nx, Ny + 1, Nz + 1}, {Nx + 1, Ny, Nz + 1}, {Nx + 1, Ny + 1, Nz}};
+  const int lo[3][3] = {{1, 2, 2}, {2, 1, 2}, {2, 2, 1}};
+  const int hi[3][3] = {{Nx, Ny, Nz}, {Nx, Ny, Nz}, {Nx, Ny, Nz}};
+  for (int c = 0; c < 3; ++c)
+    for (int k = lo[c][2]; k <= hi[c][2]; ++k)
+      for (int j = lo[c][1]; j <= hi[c][1]; ++j)
+        for (int i = lo[c][0]; i <= hi[c][0]; ++i) {
+          const uint32_t q = (uint32_t)((i - 1) + (size_t)e[c][0] * ((j - 1) + (size_t)e[c][1] * (k - 1)));
+          double a, b;
+          normal2(seed, t, 16u + (uint32_t)c, q, a, b);
+          o.Nl[c](i, j, k) = a;
+        }
+}
+
+// _kwt_kick (noise.jl:75-87): the polar kick of a KWT node at ψ = Ψⁿ, the Cartesian one where
+// |ψ|² or |F|² is under _ROT_MIN2
+C kwt_kick(C F, C psi, C xi, double a, double c, double dr, double g2, double Gm, double d0) {
+  const double r2 = abs2(psi), sk = std::sqrt(1 + g2 * r2);
+  if (!(r2 > 1e-24)) return cadd(F, rmul(a, xi));
+  const double sr = std::sqrt(r2);
+  const C u = {psi.re / sr, psi.im / sr};
+  const C w = cmul(conj(u), xi);
+  if (r2 > ROT_MIN2 && abs2(F) > ROT_MIN2) {
+    const double rho = std::sqrt(r2), aF = std::hypot(F.re, F.im);
+    const double drho = c * w.re / std::sqrt(Gm * sk) + d0 / (rho * (sk * sk * sk));
+    const double dth = c * w.im * std::sqrt(sk / Gm) / rho;
+    return cmul(C{F.re / aF, F.im / aF}, rmul(aF + drho, cis(dth)));
+  }
+  const C t = cmul(rmul(c, u), C{w.re / std::sqrt(Gm * sk), w.im * std::sqrt(sk / Gm)});
+  return csub(cadd(F, t), rmul(dr * (1 + sk * sk) / (sk * sk * sk), psi));
+}
+
+// noise_draw! (noise.jl:99-124): δθ of every SK-ROCK band from Ψⁿ and its draws
+void noise_draw(orc &o) {
+  const double dV = o.d.dx * o.d.dy * o.d.dz;
+  const double c = std::sqrt(o.d.noise_T * o.d.dt / dV);
+  for (int mu = 0; mu < o.N; ++mu) {
+    A3r &Qt = o.Qth[mu];
+    std::fill(Qt.v.begin(), Qt.v.end(), 0.0);
+    if (!o.d.noise_sk[mu]) continue;
+    const double g2 = o.d.bcs[mu].gamma2, Gm = o.d.noise_Gamma[mu];
+    for (int k = o.z1 + 1; k <= o.z2; ++k)
+      for (int j = o.y1 + 1; j <= o.y2; ++j)
+        for (int i = o.x1 + 1; i <= o.x2; ++i) {
+          if (!kicked(o, i, j, k)) continue;
+          const C psi = o.G[mu](i, j, k), xi = o.Xi[mu](i, j, k);
+          const double r2 = abs2(psi);
+          if (!(g2 > 0 && r2 > ROT_MIN2)) continue;
+          const double sk = std::sqrt(1 + g2 * r2), rho = std::sqrt(r2);
+          Qt(i, j, k) = c * cmul(conj(C{psi.re / rho, psi.im / rho}), xi).im * std::sqrt(sk / Gm) / rho;
+        }
+  }
+}
+
+// noise_kick! (noise.jl:44-65) or, after SK-ROCK stages, noise_apply! (:127-153): the kick of the
+// step on every band, after its update
+void noise_kick(orc &o) {
+  const double dV = o.d.dx * o.d.dy * o.d.dz, Tdt = o.d.noise_T * o.d.dt;
+  const double c = std::sqrt(Tdt / dV);
+  for (int mu = 0; mu < o.N; ++mu) {
+    const double Gm = o.d.noise_Gamma[mu], g2 = o.d.bcs[mu].gamma2;
+    const double a = std::sqrt(Tdt / (Gm * dV));
+    const double dr = Tdt / (2 * dV) * g2 / Gm, d0 = Tdt / (2 * dV * Gm);
+    A3c &F = o.F[mu];
+    for (int k = o.z1 + 1; k <= o.z2; ++k)
+      for (int j = o.y1 + 1; j <= o.y2; ++j)
+        for (int i = o.x1 + 1; i <= o.x2; ++i) {
+          if (!kicked(o, i, j, k)) continue;
+          const C xi = o.Xi[mu](i, j, k);
+          if (g2 == 0) {
+            F(i, j, k) = cadd(F(i, j, k), rmul(a, xi));
+          } else if (o.sk_any && o.Qth[mu](i, j, k) != 0.0) {
+            // the phase went in with the stages: the amplitude here
+            const C psi = o.G[mu](i, j, k);
+            const double rho = std::hypot(psi.re, psi.im), sk = std::sqrt(1 + g2 * (rho * rho));
+            const C u = {psi.re / rho, psi.im / rho};
+            const double drho = c * cmul(conj(u), xi).re / std::sqrt(Gm * sk) + Tdt / (2 * dV * Gm) / (rho * (sk * sk * sk));
+            const C f = F(i, j, k);
+            const double aF = std::hypot(f.re, f.im);
+            F(i, j, k) = aF > std::sqrt(ROT_MIN2) ? rmul((aF + drho) / aF, f) : cadd(f, rmul(drho, u));
+          } else {
+            F(i, j, k) = kwt_kick(F(i, j, k), o.G[mu](i, j, k), xi, a, c, dr, g2, Gm, d0);
+          }
+        }
+  }
+}
+
+// field_kick! (noise.jl:182-198): φ_l += d_l √(T̃ dt D_l/(σn dV)) n on the links the field step evolves
+void field_kick(orc &o) {
+  const int Nx = o.Nx, Ny = o.Ny, Nz = o.Nz;
+  const double c = std::sqrt(o.d.noise_T * o.d.dt / (o.d.noise_sigma_n * o.d.dx * o.d.dy * o.d.dz));
+  const double ax = c * o.d.dx, ay = c * o.d.dy, az = c * o.d.dz;
+  const bool hasD = o.d.Dx != nullptr;
+  for (int k = 2; k <= Nz; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 1; i <= Nx; ++i) o.phx(i, j, k) += (hasD ? ax * std::sqrt(o.Dx(i, j, k)) : ax) * o.Nl[0](i, j, k);
+  for (int k = 2; k <= Nz; ++k)
+    for (int j = 1; j <= Ny; ++j)
+      for (int i = 2; i <= Nx; ++i) o.phy(i, j, k) += (hasD ? ay * std::sqrt(o.Dy(i, j, k)) : ay) * o.Nl[1](i, j, k);
+  for (int k = 1; k <= Nz; ++k)
+    for (int j = 2; j <= Ny; ++j)
+      for (int i = 2; i <= Nx; ++i) o.phz(i, j, k) += (hasD ? az * std::sqrt(o.Dz(i, j, k)) : az) * o.Nl[2](i, j, k);
+}
+
 void step(orc &o, double H, double HJ) {
   if (o.d.selffield_bc == TDGL_BC_OPEN) open_update(o, HJ);
   if (o.qp != TDGL_QP_NONE) qp_potential(o);
+  if (o.noise) noise_fill(o);
+  if (o.sk_any) noise_draw(o);
   update_bands(o);
   if (o.qp != TDGL_QP_NONE) qp_turn(o);
+  if (o.noise) noise_kick(o);
   supercurrent(o);
   field_increment(o);
+  if (o.noise && o.d.noise_field) field_kick(o);
   switch (o.d.field_mode) {
     case TDGL_FIELD_SEPARABLE: implicit_sep(o); break;
     case TDGL_FIELD_PCG: implicit_pcg(o); break;
@@ -1235,16 +1563,9 @@ void step(orc &o, double H, double HJ) {
   field_bc(o, H, HJ);
   neumann(o);
   advance(o);
+  if (o.noise) o.nctr += 1;
 }
 
-template <class T>
-void load(A3<T> &a, const double *p) {
-  std::memcpy(a.v.data(), p, a.v.size() * sizeof(T));
-}
-template <class T>
-void save(const A3<T> &a, double *p) {
-  std::memcpy(p, a.v.data(), a.v.size() * sizeof(T));
-}
 
 }  // namespace
 
@@ -1356,6 +1677,19 @@ orc *orc_create(const tdgl_desc *d) {
     o->qT1.assign((size_t)n1 * nj * nk, 0.0);
     o->qT2.assign((size_t)n1 * nj * nk, 0.0);
   }
+  if (d->noise) {
+    o->noise = true;
+    for (int mu = 0; mu < o->N; ++mu) {
+      o->Xi.emplace_back(Nx + 1, Ny + 1, Nz + 1);
+      o->Qth.emplace_back(Nx + 1, Ny + 1, Nz + 1);
+      o->sk_any = o->sk_any || d->noise_sk[mu];
+    }
+    o->Nl[0] = A3r(Nx, Ny + 1, Nz + 1);
+    o->Nl[1] = A3r(Nx + 1, Ny, Nz + 1);
+    o->Nl[2] = A3r(Nx + 1, Ny + 1, Nz);
+    o->Sp = A3c(Nx + 1, Ny + 1, Nz + 1);
+    o->Om2 = A3r(Nx + 1, Ny + 1, Nz + 1);
+  }
   if (d->field_mode == TDGL_FIELD_PCG) {
     o->P[0] = o->Q[0] = A3r(Nx, Ny + 1, Nz + 1);
     o->P[1] = o->Q[1] = A3r(Nx + 1, Ny, Nz + 1);
@@ -1375,6 +1709,28 @@ orc *orc_create(const tdgl_desc *d) {
 }
 
 void orc_destroy(orc *o) { delete o; }
+
+// the noise stream: its position, the caller's draws (the layout of tdgl_noise_draws), and the
+// generator itself for its known-answer vectors
+uint64_t orc_noise_get(orc *o) { return o->nctr; }
+void orc_noise_set(orc *o, uint64_t c) { o->nctr = c; }
+void orc_noise_draws(orc *o, int nsteps, const double *draws) {
+  const size_t nn = (size_t)(o->Nx + 1) * (o->Ny + 1) * (o->Nz + 1);
+  const size_t nl = (size_t)o->Nx * (o->Ny + 1) * (o->Nz + 1) + (size_t)(o->Nx + 1) * o->Ny * (o->Nz + 1) +
+                    (size_t)(o->Nx + 1) * (o->Ny + 1) * o->Nz;
+  const size_t stride = (size_t)o->N * 2 * nn + (o->d.noise_field ? nl : 0);
+  o->hdraws.assign(draws, draws + stride * (size_t)nsteps);
+  o->hslots = nsteps;
+  o->hbase = o->nctr;
+}
+void orc_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  philox(c, key[0], key[1]);
+  for (int q = 0; q < 4; ++q) out[q] = c[q];
+}
+void orc_normal2(uint64_t seed, uint64_t t, uint32_t tag, uint32_t q, double *out) {
+  normal2(seed, t, tag, q, out[0], out[1]);
+}
 
 // the solved modes of the last normal-fluid potential (δ for two bands), region-shaped
 void orc_qp_modes(orc *o, int k, double *out) {

@@ -50,6 +50,12 @@ def lib():
         L.orc_open_state.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.orc_open_state.restype = ct.c_int
         L.orc_open_fields.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.orc_noise_get.argtypes = [ct.c_void_p]
+        L.orc_noise_get.restype = ct.c_uint64
+        L.orc_noise_set.argtypes = [ct.c_void_p, ct.c_uint64]
+        L.orc_noise_draws.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p]
+        L.orc_philox.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.orc_normal2.argtypes = [ct.c_uint64, ct.c_uint64, ct.c_uint32, ct.c_uint32, ct.c_void_p]
         _lib = L
     return _lib
 
@@ -84,7 +90,37 @@ class Oracle:
         return out
 
     def step(self, n: int, H: float, HJ: float) -> None:
+        nz = self.st.noise
+        if nz is not None and nz.host and n > 0:
+            left = getattr(self, "_slots", 0) - (self.noise_counter - getattr(self, "_base", 0))
+            if left < n:
+                raise ValueError(f"noise: draws for {max(left, 0)} more steps, {n} asked")
         lib().orc_step(self.h, int(n), float(H), float(HJ))
+
+    @property
+    def noise_counter(self) -> int:
+        """Steps taken with noise since the stream began."""
+        return int(lib().orc_noise_get(self.h))
+
+    @noise_counter.setter
+    def noise_counter(self, c: int) -> None:
+        lib().orc_noise_set(self.h, int(c))
+
+    def noise_draws(self, flat) -> None:
+        """The caller's normals for the next steps (juliatdgl_amd.noise.pack)."""
+        from juliatdgl_amd.noise import per_step  # noqa: F401  (layout owner)
+        gr, st = self.st.grid, self.st
+        nn = (gr.Nx + 1) * (gr.Ny + 1) * (gr.Nz + 1)
+        nl = sum(a * b * c for a, b, c in ((gr.Nx, gr.Ny + 1, gr.Nz + 1), (gr.Nx + 1, gr.Ny, gr.Nz + 1),
+                                           (gr.Nx + 1, gr.Ny + 1, gr.Nz)))
+        stride = st.N * 2 * nn + (nl if st.noise.field else 0)
+        flat = np.ascontiguousarray(flat, dtype=np.float64)
+        if flat.size % stride:
+            raise ValueError("noise draws: not a whole number of steps")
+        self._draws = flat
+        self._slots = flat.size // stride
+        self._base = self.noise_counter
+        lib().orc_noise_draws(self.h, self._slots, flat.ctypes.data)
 
     def get_pcg(self) -> PcgState:
         p = PcgState(self.st.grid)
@@ -149,6 +185,22 @@ class Oracle:
         out = [np.array(b, dtype=np.float64, order="F", copy=True) for b in (BX, BY, BZ)]
         lib().orc_sep_solve(self.h, *(o.ctypes.data for o in out))
         return out
+
+
+def philox(ctr, key):
+    """Philox4x32-10 of one counter block (four uint32) under a key (two uint32)."""
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return [int(x) for x in out]
+
+
+def normal2(seed: int, t: int, tag: int, q: int):
+    """The two normals of the noise at (step t, tag, index q) under `seed`."""
+    out = np.zeros(2)
+    lib().orc_normal2(int(seed), int(t), int(tag), int(q), out.ctypes.data)
+    return float(out[0]), float(out[1])
 
 
 def _split(a, shapes):

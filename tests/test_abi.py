@@ -74,7 +74,7 @@ def test_create_without_device_or_with_bad_desc_raises():
 def test_stepper_options_outside_the_path_are_refused():
     gr = Grid.make(8, 6, 4, dx=0.5)
     with pytest.raises(NotImplementedError):
-        make_stepper(mgb2(), gr, box_masks(gr), noise=1e-3)
+        make_stepper(mgb2(), gr, box_masks(gr), materials=["metal"])
     with pytest.raises(NotImplementedError):
         make_stepper(mgb2(), gr, box_masks(gr), heat=1.0)
     with pytest.raises(NotImplementedError):

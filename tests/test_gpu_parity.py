@@ -211,3 +211,28 @@ def test_hip_open_boundary_resumes_and_is_odd_in_the_current():
     for mu in range(st.N):
         assert np.max(np.abs(p.G[mu] - np.conj(m.G[mu]))) < 1e-13
     assert np.max(np.abs(p.hz + m.hz)) < 1e-13 and np.max(np.abs(p.hz)) > 1e-4
+
+
+@pytest.mark.parametrize("name", ["production", "kwt10_pcg"])
+def test_hip_long_call_equals_short_calls(name):
+    """A call of more steps than one captured graph holds (128) runs as replays of bounded graphs:
+    the same bits as the same steps in short calls, and the same PCG iterations."""
+    HipStepper = _hip()
+    st, _ = make_case(name)
+    s0, pcg, H, HJ = seed_state(st)
+    print(f"[gpu] long call, {name}", flush=True)
+    runs = []
+    for calls in ((300,), (100, 100, 100)):
+        dev = HipStepper(st)
+        dev.upload(s0)
+        if pcg is not None:
+            dev.set_pcg(pcg)
+        dev.step(2, H, HJ)  # the PCG's budget from an eager cycle
+        for n in calls:
+            dev.step(n, H, HJ)
+        runs.append((dev.download(), dev.stats()))
+    (a, sa), (b, sb) = runs
+    for x, y in zip(a.G, b.G):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.hz, b.hz) and np.array_equal(a.thx, b.thx)
+    assert sa["pcg_iters"] == sb["pcg_iters"] and sa["steps"] == sb["steps"] == 302
