@@ -17,7 +17,7 @@ using TDGL
 using TDGL: State, Stepper, SepSolver, PCGSolver
 
 export GPUStepper, GState, device_state, device_state!, host_state!, gpu_step!, gpu_steps!, supported, enable!,
-       meter_begin!, meter_cycle,
+       meter_begin!, meter_cycle, noise_counter, noise_counter!, noise_draws!, reference_draws,
        disable!, execution_stats
 
 const MAXB = 4
@@ -63,6 +63,10 @@ struct DescC
     # open boundary (openbc.jl)
     open_every::Int32; open_R::Int32; open_fac::Float64; open_kappa2::Float64
     open_lgf::Ptr{Float64}; open_Jbar::Ptr{Float64}
+    # thermal noise (noise.jl)
+    noise::Int32; noise_field::Int32; noise_host::Int32; noise_sk::NTuple{MAXB,Int32}
+    noise_T::Float64; noise_seed::UInt64
+    noise_sk_w::NTuple{MAXB,NTuple{2,Float64}}; noise_Gamma::NTuple{MAXB,Float64}; noise_sigma_n::Float64
 end
 struct StateC
     G::NTuple{MAXB,Ptr{Float64}}; F::NTuple{MAXB,Ptr{Float64}}
@@ -92,7 +96,6 @@ function supported(st::Stepper{N}) where N
     N <= MAXB || return (false, "more than $MAXB bands")
     st.heat === nothing || return (false, "the heat model")
     st.drag === nothing || return (false, "interband drag")
-    st.noise === nothing || return (false, "thermal noise")
     st.mt === nothing || return (false, "materials per region")
     if st.qpx !== nothing
         st.qpx.ps.cond === nothing || return (false, "normal fluids in a sculpted sample")
@@ -120,7 +123,7 @@ function _desc(st::Stepper{N}) where N
                   MAXB, ntuple(_ -> 0.0, MAXS))
     bar = st.bar
     bp(k) = bar === nothing ? Ptr{Float64}(0) : hold(getproperty(bar, k))
-    d = DescC(2, N, gr.Nx, gr.Ny, gr.Nz, gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2, gr.dx, gr.dy, gr.dz, st.dt,
+    d = DescC(3, N, gr.Nx, gr.Ny, gr.Nz, gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2, gr.dx, gr.dy, gr.dz, st.dt,
               _pad(ntuple(µ -> _bc(st.bc[µ], N, µ), N), MAXB, _ZBC), _pad(ntuple(µ -> _bc(st.bcs[µ], N, µ), N), MAXB, _ZBC),
               _pad(Int32.(st.nsub), MAXB, Int32(0)), _pad(Int32.(st.rkl), MAXB, Int32(0)),
               _pad(ntuple(µ -> Int32(rk[µ] !== nothing), N), MAXB, Int32(0)), _pad(div, MAXB, 0.0),
@@ -134,8 +137,21 @@ function _desc(st::Stepper{N}) where N
               D === nothing ? Ptr{Float64}(0) : hold(D[1]), D === nothing ? Ptr{Float64}(0) : hold(D[2]),
               D === nothing ? Ptr{Float64}(0) : hold(D[3]),
               bp(:hy_z), bp(:hy_y), bp(:hz_z), bp(:hz_y), bp(:hz_x), bp(:hy_x),
-              _qp_fields(st, hold)..., _open_fields(st, hold)...)
+              _qp_fields(st, hold)..., _open_fields(st, hold)..., _noise_fields(st, rk)...)
     d, keep
+end
+
+# the Noise of st (noise.jl:32-41) as the description's noise fields: draws on the device unless
+# HOST_DRAWS[] (then the hook hands over the reference's own stream, reference_draws)
+const HOST_DRAWS = Ref(false)
+function _noise_fields(st::Stepper{N}, rk) where N
+    nz = st.noise
+    nz === nothing && return (Int32(0), Int32(0), Int32(0), ntuple(_ -> Int32(0), MAXB), 0.0, UInt64(0),
+                              ntuple(_ -> (0.0, 0.0), MAXB), _Z4, 0.0)
+    sk = _pad(ntuple(µ -> Int32(TDGL._skrock(st, µ)), N), MAXB, Int32(0))
+    w = _pad(ntuple(µ -> rk[µ] === nothing ? (0.0, 0.0) : rk[µ].sk, N), MAXB, (0.0, 0.0))
+    Γ = _pad(ntuple(µ -> st.model.bands[µ].Γ, N), MAXB, 0.0)
+    (Int32(1), Int32(nz.field), Int32(HOST_DRAWS[]), sk, nz.T, nz.seed, w, Γ, st.model.σn)
 end
 
 const _Z4 = ntuple(_ -> 0.0, MAXB)
@@ -305,6 +321,58 @@ function meter_begin!(gs::GPUStepper, st::Stepper, delta_v::Integer)
     (Min, Mou, ok, f1)
 end
 
+"""noise_counter(gs) -> UInt64: the noise stream's position (steps taken with noise); with the seed it is the
+whole generator state, what a checkpoint stores in place of `noise_rng` (protocol.jl:659)."""
+function noise_counter(gs::GPUStepper)
+    c = Ref{UInt64}(0)
+    _check(ccall((:tdgl_noise_get, libpath()), Cint, (Ptr{Cvoid}, Ref{UInt64}), gs.h, c), "tdgl_noise_get")
+    c[]
+end
+noise_counter!(gs::GPUStepper, c::Integer) =
+    _check(ccall((:tdgl_noise_set, libpath()), Cint, (Ptr{Cvoid}, UInt64), gs.h, UInt64(c)), "tdgl_noise_set")
+
+"""noise_draws!(gs, draws, nsteps): the normals of the next nsteps steps (HOST_DRAWS[]), in the layout of
+tdgl_noise_draws; `reference_draws` makes them from the Stepper's own Xoshiro stream."""
+noise_draws!(gs::GPUStepper, draws::Vector{Float64}, nsteps::Integer) =
+    _check(ccall((:tdgl_noise_draws, libpath()), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}), gs.h, Int32(nsteps), draws),
+           "tdgl_noise_draws")
+
+"""
+    reference_draws(st, n) -> Vector{Float64}
+
+The draws of n steps from `st.noise.rng`, taken in the reference's order (noise_kick! / noise_draw!, then
+field_kick!: noise.jl:47-52, 108-111, 188-196) and laid out as tdgl_noise_draws takes them: per step the bands'
+complex node arrays, then with noise_field the x, y, z link arrays.  The stream advances as the CPU step's would.
+"""
+function reference_draws(st::Stepper{N}, n::Integer) where N
+    gr = st.grid; (; Nx, Ny, Nz, x1, x2, y1, y2, z1, z2) = gr
+    rng = st.noise.rng; f = st.masks.f; hard = st.hard
+    nn = (Nx + 1) * (Ny + 1) * (Nz + 1)
+    shp = ((Nx, Ny + 1, Nz + 1), (Nx + 1, Ny, Nz + 1), (Nx + 1, Ny + 1, Nz))
+    nl = st.noise.field ? sum(prod, shp) : 0
+    out = zeros(Float64, n * (2N * nn + nl))
+    p = 0
+    for _ in 1:n
+        for µ in 1:N
+            X = zeros(ComplexF64, Nx + 1, Ny + 1, Nz + 1)
+            for k in z1+1:z2, j in y1+1:y2, i in x1+1:x2
+                (hard && f[i,j,k] == 0.0) && continue
+                X[i,j,k] = complex(randn(rng), randn(rng))
+            end
+            out[p+1:p+2nn] .= reinterpret(Float64, vec(X)); p += 2nn
+        end
+        if st.noise.field
+            rs = ((1:Nx, 2:Ny, 2:Nz), (2:Nx, 1:Ny, 2:Nz), (2:Nx, 2:Ny, 1:Nz))
+            for c in 1:3
+                L = zeros(shp[c])
+                for k in rs[c][3], j in rs[c][2], i in rs[c][1]; L[i,j,k] = randn(rng); end
+                out[p+1:p+length(L)] .= vec(L); p += length(L)
+            end
+        end
+    end
+    out
+end
+
 """meter_cycle(gs, first) -> CycleRecordC: the record of the cycle that just ended; `first` at m == 1."""
 function meter_cycle(gs::GPUStepper, first::Bool)
     r = Ref{CycleRecordC}()
@@ -357,6 +425,8 @@ function _hip_cycle!(s::State, st::Stepper, n::Int; H::Float64, HJ::Float64)
     _DEV[] === nothing || _DEV[][1] !== s ? (_DEV[] = (s, device_state(gs, s))) : device_state!(_DEV[][2], s)
     g = _DEV[][2]
     st.imp isa PCGSolver && _pcg_up!(gs, st.imp)
+    # with noise: the reference's own stream (HOST_DRAWS[]), or the device's stream at its position
+    st.noise !== nothing && HOST_DRAWS[] && noise_draws!(gs, reference_draws(st, n), n)
     gpu_steps!(g, gs, n; H=H, HJ=HJ)
     st.imp isa PCGSolver && _pcg_down!(st.imp, gs)
     host_state!(s, g)

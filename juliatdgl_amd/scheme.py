@@ -227,19 +227,13 @@ def _oround(x: float) -> int:
     return int(math.copysign(math.floor(abs(x) + 0.5), x))
 
 
-def wedge_masks(gr: Grid, wedge_depth=0.35, wedge_half=0.20) -> Masks:
-    """The catalogue WEDGE notch on face y1, through the thickness (geometry.jl:299-308,
-    paint_defect! 'wedge' :121-135, build_defect_list :268-270, z_mode through)."""
-    mk = box_masks(gr)
+def _paint_wedge(mk: Masks, gr: Grid, xc: int, yc: int, d: int, half: int, depth: int) -> None:
+    """paint_defect! of a 'wedge' (geometry.jl:121-135): a notch growing from yc in direction d,
+    a steep wall on -x and a gentle ramp on +x, through the thickness (z_mode through)."""
     x1, x2, y1, y2, z1, z2 = gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2
-    xc = _oround((x1 + x2) / 2)
-    nx_sc, ny_sc = x2 - x1 + 1, y2 - y1 + 1
-    half = _oround(wedge_half * nx_sc)
-    depth = _oround(wedge_depth * ny_sc)
-    yc = y1
     for k in range(z1, z2 + 1):
         for j in range(y1, y2 + 1):
-            layer = j - yc
+            layer = d * (j - yc)
             if 0 <= layer < depth:
                 frac = 1 - layer / depth
                 w_steep = max(1, _oround(2 * frac))
@@ -248,7 +242,45 @@ def wedge_masks(gr: Grid, wedge_depth=0.35, wedge_half=0.20) -> Masks:
                     dxr = i - xc
                     if -w_steep <= dxr <= w_gentle:
                         mk.f[i - 1, j - 1, k - 1] = 0.0
+
+
+def catalogue_masks(gr: Grid, mode: str = "SIM", wedge_depth: float = 0.35, wedge_half: float = 0.20,
+                    wedge_depth2: float = 0.0) -> Masks:
+    """build_masks(GeomSpec(GEOM_MODE=mode, ...)) for the catalogue modes SIM (a clean box), WEDGE
+    (a notch on face y1) and WEDGE_BOTH (and a second on y2), each with its _REF mirror in y
+    (geometry.jl:268-324, mirror_defects! :193-209; J along x, z_mode through, no shifts)."""
+    m = mode.upper()
+    ref = m.endswith("_REF")
+    if ref:
+        m = m[:-4]
+    x1, x2, y1, y2 = gr.x1, gr.x2, gr.y1, gr.y2
+    xc = _oround((x1 + x2) / 2)
+    nx_sc, ny_sc = x2 - x1 + 1, y2 - y1 + 1
+    half = _oround(wedge_half * nx_sc)
+    defects = []
+    if m == "SIM":
+        pass
+    elif m == "WEDGE":
+        defects.append((xc, y1, +1, half, _oround(wedge_depth * ny_sc)))
+    elif m == "WEDGE_BOTH":
+        defects.append((xc, y1, +1, half, _oround(wedge_depth * ny_sc)))
+        if wedge_depth2 > 0:
+            defects.append((xc, y2 + 1, -1, half, _oround(wedge_depth2 * ny_sc)))
+    else:
+        raise NotImplementedError(f"GEOM_MODE {mode!r}: the catalogue modes built here are SIM, WEDGE, WEDGE_BOTH "
+                                  f"(and their _REF mirrors)")
+    if ref:
+        defects = [(x, (y1 + y2 + 1) - y, -d, h, dp) for (x, y, d, h, dp) in defects]
+    mk = box_masks(gr)
+    for x, y, d, h, dp in defects:
+        _paint_wedge(mk, gr, x, y, d, h, dp)
     return mk
+
+
+def wedge_masks(gr: Grid, wedge_depth=0.35, wedge_half=0.20) -> Masks:
+    """The catalogue WEDGE notch on face y1, through the thickness (geometry.jl:299-308,
+    paint_defect! 'wedge' :121-135, build_defect_list :268-270, z_mode through)."""
+    return catalogue_masks(gr, "WEDGE", wedge_depth, wedge_half)
 
 
 def notch_masks(gr: Grid, xr, yr) -> Masks:

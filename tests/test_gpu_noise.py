@@ -181,3 +181,67 @@ def test_device_field_noise_equipartition(kappa, svac, buf, teq, tav):
     r = field_equipartition(_dev_backend, _dev_steps, kappa, svac, buf, teq, tav)
     print(f"    ratio {r:.4f}", flush=True)
     assert r == pytest.approx(1.0, abs=0.01)
+
+
+def test_device_kwt_noise_extended_system_at_the_production_step():
+    """test_physics.jl:544-562: coupled nodes (m = 1, κ = 25) at the production step, where a KWT
+    band takes RKC stages and the noise goes in with them (SK-ROCK, log-polar stages): the mean
+    |Ψ|² does not depend on the mobility, with the rotating frame (the default) and without."""
+    from juliatdgl_amd.scheme import Band, Grid, Model, box_masks, make_stepper
+    from juliatdgl_amd.state import initial_state
+
+    def media(gk, **kw):
+        md = Model.make((Band.make(m=1.0, Gamma=2.0, gkwt=gk),), np.zeros((1, 1)), kappa=25.0)
+        gr = Grid.make(10, 10, 2, dx=0.5)
+        mk = box_masks(gr)
+        st = make_stepper(md, gr, mk, noise=0.01, noise_seed=7, **kw)
+        d = _dev_backend(st, initial_state(md, gr, mk, 0.0))
+        d.step(round(40 / st.dt), 0.0, 0.0)
+        sl = (slice(gr.x1, gr.x2), slice(gr.y1, gr.y2), slice(gr.z1, gr.z2))
+        sm, N = 0.0, 0
+        for _ in range(round(1500 / st.dt) // 13):
+            d.step(13, 0.0, 0.0)
+            x = np.abs(d.download().G[0][sl]) ** 2
+            sm += x.sum()
+            N += x.size
+        return sm / N, st
+
+    print("[gpu] KWT noise, extended system", flush=True)
+    n0, _ = media(0.0)
+    n3, st3 = media(3.0)
+    nf, _ = media(3.0, rotframe=False)
+    print(f"    <|psi|^2> {n0:.4f} (no KWT), {n3:.4f} (KWT), {nf:.4f} (KWT, no frame); stages {st3.rkl}", flush=True)
+    assert st3.rotframe and any(st3.sk)
+    assert n3 == pytest.approx(n0, rel=0.02)
+    assert nf == pytest.approx(n0, rel=0.02)
+
+
+def test_device_kwt_noise_on_the_notched_two_band_box():
+    """test_physics.jl:566-582: the two-band wedge box at κ = 2 with γ̃ = 30 and noise stays finite on
+    RKC stages (5, 13) and keeps ⟨|Ψπ|²⟩ within 5 % of γ̃ = 0."""
+    from juliatdgl_amd.scheme import Grid, make_stepper, two_band, wedge_masks
+    from juliatdgl_amd.state import initial_state
+    gr = Grid.make(24, 12, 6, dx=0.5)
+    mk = wedge_masks(gr)
+
+    def media(gk):
+        md = two_band(xi_ratio=4.0, kappa=2.0, gamma_kwt=gk)
+        st = make_stepper(md, gr, mk, noise=1e-3, noise_seed=3)
+        d = _dev_backend(st, initial_state(md, gr, mk, 0.03))
+        HJ = st.selffield(0.0)
+        sl = (slice(gr.x1, gr.x2), slice(gr.y1, gr.y2), slice(gr.z1, gr.z2))
+        d.step(500, 0.03, HJ)
+        a, c = 0.0, 0
+        for _ in range(100):
+            d.step(10, 0.03, HJ)
+            a += np.sum(np.abs(d.download().G[1][sl]) ** 2)
+            c += 1
+        s = d.download()
+        return all(np.all(np.isfinite(g)) for g in s.G), a / c, st
+
+    print("[gpu] KWT noise, notched two-band box", flush=True)
+    ok0, p0, _ = media(0.0)
+    ok, p, st = media(30.0)
+    print(f"    <|psi_pi|^2> sum {p0:.3f} (no KWT), {p:.3f} (γ̃ = 30); stages {st.rkl}", flush=True)
+    assert ok0 and ok and st.rkl == (5, 13)
+    assert p == pytest.approx(p0, rel=0.05)
