@@ -127,3 +127,12 @@ def test_device_rung_closes_as_the_oracle_rung():
         assert rd.conv == ro.conv and rd.m == ro.m
         assert rd.conv == ("F" if J < 1 else "V")
         assert np.allclose(rd.V, ro.V, rtol=1e-8, atol=1e-8)
+
+
+def test_golden_ladder_on_the_device():
+    """The golden two-polarity ladder of release 1.2 (test_protocol.jl:4-29) with the device stepping
+    and measuring: verdicts, cycles and brackets exactly, the recorded numbers and the final Ψ to 1e-10."""
+    from test_ladder import check_golden_ladder
+    HipStepper = _hip()
+    print("[gpu] golden ladder", flush=True)
+    check_golden_ladder(lambda st: HipStepper(st), report=lambda m: print(m, flush=True))
