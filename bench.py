@@ -136,13 +136,60 @@ def copy_probe(mib: int) -> None:
     print(f"[bench] copy probe: {mib} MiB read + {mib} MiB written", file=sys.stderr)
 
 
+def bench_sweep(args) -> int:
+    """The batched current sweep: 64 independent plain FP64 runs (32 currents in [0.05, 0.4] x 2
+    polarities, grids uniform on multiples of 4 in [32,160]x[16,96]x[4,24], notch depth in [0.1, 0.4]
+    of the width, per-run seeds from one master seed), each `steps` ladder cycles measured on the
+    device.  Runs/hour one after the other and `workers` at a time, whether every run's record is
+    the same alone and in the mix, and p50/p99 latency of the same runs served from a Poisson stream
+    (continuous admission by HBM footprint)."""
+    from juliatdgl_amd import sweep
+    specs = sweep.sweep_specs(32, master=2026)
+    work = sweep.Cycles(cycles=args.steps)
+    cells = sum(s.nx * s.ny * s.nz for s in specs)
+    print(f"[bench] sweep64: {len(specs)} runs, {cells} cells, {args.steps} cycles each", file=sys.stderr, flush=True)
+    sweep.run_batch(specs[:2], sweep.Cycles(cycles=1), workers=2)  # the device and the library warm
+    seq, t_seq = sweep.run_batch(specs, work, workers=1)
+    print(f"[bench] one at a time: {t_seq:.2f} s", file=sys.stderr, flush=True)
+    bat, t_bat = sweep.run_batch(specs, work, workers=args.workers)
+    print(f"[bench] {args.workers} at a time: {t_bat:.2f} s", file=sys.stderr, flush=True)
+    same = all(seq[q] == bat[q] for q in seq)
+    rate = len(specs) / t_bat
+    srv = sweep.Server(budget_bytes=200e9, slots=args.workers)
+    for s, t in zip(specs, sweep.poisson_arrivals(len(specs), rate=0.9 * rate, seed=5)):
+        srv.submit(s, t)
+    t0 = time.perf_counter()
+    done = srv.run(work)
+    t_srv = time.perf_counter() - t0
+    lat = sweep.percentiles(d["latency"] for d in done.values())
+    same_srv = all(done[q]["result"] == seq[q] for q in seq)
+    print(f"[bench] served: {t_srv:.2f} s, p50 {lat['p50']:.3f} s, p99 {lat['p99']:.3f} s", file=sys.stderr, flush=True)
+    line = {"metric": "batched-sweep runs/hr", "value": 3600.0 * len(specs) / t_bat, "unit": "runs/hr", "n_gpus": 1,
+            "steps": args.steps, "warmup": 1, "ms_per_step": 1e3 * t_bat / args.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded psi per run)",
+            "config": {"workload": "64-run current sweep: plain FP64 two-band runs (kappa 2.5, separable solve, "
+                                   "bar field, dt 0.01), grids 32..160 x 16..96 x 4..24, notch depth 0.1..0.4, "
+                                   f"J in +-[0.05, 0.4], {args.steps} ladder cycles of 40 steps per run",
+                       "runs": len(specs), "cells": cells, "workers": args.workers,
+                       "sequential_runs_per_hr": 3600.0 * len(specs) / t_seq, "speedup": t_seq / t_bat,
+                       "cell_cycles_per_s_batched": cells * args.steps / t_bat,
+                       "same_alone_and_in_the_mix": bool(same and same_srv),
+                       "served_p50_latency_s": lat["p50"], "served_p99_latency_s": lat["p99"],
+                       "served_arrival_rate_per_s": 0.9 * rate, "served_runs_per_hr": 3600.0 * len(specs) / t_srv},
+            "roofline": None, "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed ladder cycles")
     ap.add_argument("--warmup", type=int, default=2, help="untimed ladder cycles")
     ap.add_argument("--workload", default="kwt_pcg_128x64x16",
-                    choices=("kwt_pcg_128x64x16", "open_3band_256x128x32", "pcg_wedge_96x48x24", "plain_wedge_96x48x24"))
+                    choices=("kwt_pcg_128x64x16", "open_3band_256x128x32", "pcg_wedge_96x48x24", "plain_wedge_96x48x24",
+                             "sweep64"))
+    ap.add_argument("--workers", type=int, default=8, help="sweep64: runs in flight at once")
     ap.add_argument("--cpu-steps", type=int, default=8, help="time steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-out", default=None, help="write the per-kernel profile (JSON) here")
@@ -156,6 +203,8 @@ def main(argv=None):
     if device_count() < 1:
         print("no HIP device", file=sys.stderr)
         return 1
+    if args.workload == "sweep64":
+        return bench_sweep(args)
     if args.workload == "kwt_pcg_128x64x16":
         st, p = configs.kwt_pcg_128x64x16()
         workload = ("two-band KWT (u=Gamma/sigma_n=5.79, gkwt=10) TDGL, 128x64x16 cells, dx=0.5, buf 2 insulating "

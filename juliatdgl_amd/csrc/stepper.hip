@@ -172,13 +172,36 @@ struct tdgl_handle {
 
 namespace {
 
+#define CHK(expr)          \
+  do {                     \
+    int rc_ = (expr);      \
+    if (rc_) return rc_;   \
+  } while (0)
+
+// Copies and fills of a handle go on its own stream, never on the legacy null stream: that one
+// waits for the work of every blocking stream, and another handle's stream may be capturing a
+// graph in another host thread at that moment.  (None of these runs inside a capture.)
+int hcopy(tdgl_handle *h, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+  return 0;
+}
+int hfill0(tdgl_handle *h, void *dst, size_t bytes) {
+  HIPCHK(hipMemsetAsync(dst, 0, bytes, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+  return 0;
+}
+
 template <class T>
 int dalloc(tdgl_handle *h, T **p, size_t n) {
   void *q = nullptr;
   if (n == 0) n = 1;
   if (hipMalloc(&q, n * sizeof(T)) != hipSuccess) return fail(TDGL_E_NOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
-  if (hipMemset(q, 0, n * sizeof(T)) != hipSuccess) return fail(TDGL_E_DEVICE, "hipMemset failed");
   h->allocs.push_back(q);
+  {
+    const int rc = hfill0(h, q, n * sizeof(T));
+    if (rc) return rc;
+  }
   h->bytes += (int64_t)(n * sizeof(T));
   *p = (T *)q;
   return 0;
@@ -191,9 +214,8 @@ int dalloc(tdgl_handle *h, T **p, size_t n) {
   } while (0)
 
 template <class T>
-int upload_vec(T *dst, const std::vector<T> &v) {
-  HIPCHK(hipMemcpy(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-  return 0;
+int upload_vec(tdgl_handle *h, T *dst, const std::vector<T> &v) {
+  return hcopy(h, dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
 }
 
 // Julia-shaped (n1, n2, n3) host array of `es`-byte elements <-> node-shaped device array
@@ -205,13 +227,13 @@ int put(tdgl_handle *h, void *dev, const void *host, int n1, int n2, int n3, siz
     for (int j = 0; j < n2; ++j)
       std::memcpy(&h->stage[((size_t)g.px * (j + (size_t)g.py * k)) * es], src + ((size_t)n1 * (j + (size_t)n2 * k)) * es,
                   (size_t)n1 * es);
-  HIPCHK(hipMemcpy(dev, h->stage.data(), (size_t)g.n * es, hipMemcpyHostToDevice));
+  CHK(hcopy(h, dev, h->stage.data(), (size_t)g.n * es, hipMemcpyHostToDevice));
   return 0;
 }
 int get(tdgl_handle *h, void *host, const void *dev, int n1, int n2, int n3, size_t es) {
   const Geo &g = h->g;
   h->stage.resize((size_t)g.n * es);
-  HIPCHK(hipMemcpy(h->stage.data(), dev, (size_t)g.n * es, hipMemcpyDeviceToHost));
+  CHK(hcopy(h, h->stage.data(), dev, (size_t)g.n * es, hipMemcpyDeviceToHost));
   unsigned char *dst = (unsigned char *)host;
   for (int k = 0; k < n3; ++k)
     for (int j = 0; j < n2; ++j)
@@ -220,11 +242,7 @@ int get(tdgl_handle *h, void *host, const void *dev, int n1, int n2, int n3, siz
   return 0;
 }
 
-#define CHK(expr)          \
-  do {                     \
-    int rc_ = (expr);      \
-    if (rc_) return rc_;   \
-  } while (0)
+
 
 int prof_flush(tdgl_handle *h) {
   if (h->evn == 0) return 0;
@@ -1256,7 +1274,9 @@ int cycle_graph(tdgl_handle *h, int n) {
   if (!h->gexec || h->gkey != key) {
     CHK(destroy_graph(h));
     h->capturing = true;
-    HIPCHK(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
+    // relaxed: other handles, in other host threads, allocate, free and copy on their own streams
+    // meanwhile; this capture records only the kernels launched on h->s
+    HIPCHK(hipStreamBeginCapture(h->s, hipStreamCaptureModeRelaxed));
     int rc = steps_eager(h, n);
     hipGraph_t gph = nullptr;
     hipError_t e = hipStreamEndCapture(h->s, &gph);
@@ -1470,7 +1490,7 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
       const double *src[6] = {d.bar_hy_z, d.bar_hy_y, d.bar_hz_z, d.bar_hz_y, d.bar_hz_x, d.bar_hy_x};
       for (int q = 0; q < 6; ++q) {
         ALLOC(h->bar[q], nn[q]);
-        CHK(upload_vec(h->bar[q], std::vector<double>(src[q], src[q] + nn[q])));
+        CHK(upload_vec(h, h->bar[q], std::vector<double>(src[q], src[q] + nn[q])));
       }
     }
     if (d.field_mode == TDGL_FIELD_SEPARABLE || d.field_mode == TDGL_FIELD_PCG) {
@@ -1489,17 +1509,17 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
           for (int t = 0; t < 2; ++t) {
             const std::vector<double> &src = t == 0 ? fwd : inv;
             ALLOC(h->Ad[ax][which][t], src.size());
-            CHK(upload_vec(h->Ad[ax][which][t], src));
+            CHK(upload_vec(h, h->Ad[ax][which][t], src));
             std::vector<float> s32(src.begin(), src.end());
             ALLOC(h->Af[ax][which][t], s32.size());
-            CHK(upload_vec(h->Af[ax][which][t], s32));
+            CHK(upload_vec(h, h->Af[ax][which][t], s32));
           }
         }
         ALLOC(h->sd[ax], nn[ax]);
-        CHK(upload_vec(h->sd[ax], b.s));
+        CHK(upload_vec(h, h->sd[ax], b.s));
         std::vector<float> s32(b.s.begin(), b.s.end());
         ALLOC(h->sf[ax], nn[ax]);
-        CHK(upload_vec(h->sf[ax], s32));
+        CHK(upload_vec(h, h->sf[ax], s32));
       }
       for (int c = 0; c < 3; ++c) ALLOC(h->B.c[c], n);
       ALLOC(h->T1, rg.total);
@@ -1517,10 +1537,10 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
       if (pcg_box_blocks(g) > MAX_PART || field_blocks(g) > MAX_PART) return fail(TDGL_E_UNSUPPORTED, "grid too large for the PCG reductions");
       std::vector<double> sc(SC_COUNT, 0.0);
       sc[SC_TOL] = d.pcg_tol;
-      CHK(upload_vec(h->pd.sc, sc));
+      CHK(upload_vec(h, h->pd.sc, sc));
       std::vector<int> fl(FL_COUNT, 0);
       fl[FL_MAXIT] = d.pcg_maxit;
-      CHK(upload_vec(h->pd.fl, fl));
+      CHK(upload_vec(h, h->pd.fl, fl));
     }
     if (d.qp_mode != TDGL_QP_NONE) {
       const int n1 = d.Nx - 1, nj = d.qp_nj, nk = d.qp_nk, K = d.qp_nmodes;
@@ -1536,9 +1556,9 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
         ALLOC(h->qV[a], m);
         ALLOC(h->qW[a], m);
         ALLOC(h->qe[a], (size_t)nn[a]);
-        CHK(upload_vec(h->qV[a], std::vector<double>(V[a], V[a] + m)));
-        CHK(upload_vec(h->qW[a], std::vector<double>(W[a], W[a] + m)));
-        CHK(upload_vec(h->qe[a], std::vector<double>(E[a], E[a] + nn[a])));
+        CHK(upload_vec(h, h->qV[a], std::vector<double>(V[a], V[a] + m)));
+        CHK(upload_vec(h, h->qW[a], std::vector<double>(W[a], W[a] + m)));
+        CHK(upload_vec(h, h->qe[a], std::vector<double>(E[a], E[a] + nn[a])));
       }
       PlanePass t{};
       t.ncomp = K;
@@ -1565,7 +1585,7 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
             tw[2 * k + 1] = -std::sin(2 * pi * k / M);
           }
           ALLOC(h->otw[c][a], (size_t)M);
-          CHK(upload_vec((double *)h->otw[c][a], tw));
+          CHK(upload_vec(h, (double *)h->otw[c][a], tw));
           h->oax[c][a].tw = h->otw[c][a];
         }
         const size_t nb = (size_t)nl[c][0] * nl[c][1] * nl[c][2];
@@ -1577,13 +1597,13 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
         ALLOC(h->ohnew[c], n);
       }
       ALLOC(h->ofl, 4);
-      CHK(upload_vec(h->ofl, std::vector<int>{-1, 0, 0, 0}));
+      CHK(upload_vec(h, h->ofl, std::vector<int>{-1, 0, 0, 0}));
       const size_t nt = (size_t)(d.open_R + 1) * (d.open_R + 1) * (d.open_R + 1);
       ALLOC(h->olgf, nt);
-      CHK(upload_vec(h->olgf, std::vector<double>(d.open_lgf, d.open_lgf + nt)));
+      CHK(upload_vec(h, h->olgf, std::vector<double>(d.open_lgf, d.open_lgf + nt)));
       const size_t nj = (size_t)(d.Ny + 1) * (d.Nz + 1);
       ALLOC(h->oJbar, nj);
-      CHK(upload_vec(h->oJbar, std::vector<double>(d.open_Jbar, d.open_Jbar + nj)));
+      CHK(upload_vec(h, h->oJbar, std::vector<double>(d.open_Jbar, d.open_Jbar + nj)));
       CHK(open_green(h));
     }
     ALLOC(h->hp, 2);
@@ -1809,7 +1829,7 @@ int tdgl_get_pcg(tdgl_handle *h, tdgl_pcg *p) {
     const double *src[2] = {h->x.c[c], h->xprev.c[c]};
     for (int w = 0; w < 2; ++w) {
       if (!dst[w]) continue;
-      HIPCHK(hipMemcpy(node.data(), src[w], node.size() * 8, hipMemcpyDeviceToHost));
+      CHK(hcopy(h, node.data(), src[w], node.size() * 8, hipMemcpyDeviceToHost));
       const int e0 = h->rg.e[c][0], e1 = h->rg.e[c][1], e2 = h->rg.e[c][2];
       for (int k = 0; k < e2; ++k)
         for (int j = 0; j < e1; ++j)
@@ -1837,7 +1857,7 @@ int tdgl_set_pcg(tdgl_handle *h, const tdgl_pcg *p) {
       for (int k = 0; k < e2; ++k)
         for (int j = 0; j < e1; ++j)
           for (int i = 0; i < e0; ++i) node[h->g.at(i + 2, j + 2, k + 2)] = src[w][i + (size_t)e0 * (j + (size_t)e1 * k)];
-      HIPCHK(hipMemcpy(dst[w], node.data(), node.size() * 8, hipMemcpyHostToDevice));
+      CHK(hcopy(h, dst[w], node.data(), node.size() * 8, hipMemcpyHostToDevice));
     }
   }
   h->warm = p->warm != 0;
@@ -1851,7 +1871,7 @@ int tdgl_noise_get(tdgl_handle *h, uint64_t *counter) {
   if (!h->noise) return fail(TDGL_E_ARG, "the stepper has no noise");
   HIPCHK(hipStreamSynchronize(h->s));
   uint64_t c[2];
-  HIPCHK(hipMemcpy(c, h->nctr, sizeof c, hipMemcpyDeviceToHost));
+  CHK(hcopy(h, c, h->nctr, sizeof c, hipMemcpyDeviceToHost));
   *counter = c[0];
   return 0;
 }
@@ -1860,7 +1880,7 @@ int tdgl_noise_set(tdgl_handle *h, uint64_t counter) {
   if (!h) return fail(TDGL_E_ARG, "null handle");
   if (!h->noise) return fail(TDGL_E_ARG, "the stepper has no noise");
   HIPCHK(hipStreamSynchronize(h->s));
-  HIPCHK(hipMemcpy(h->nctr, &counter, sizeof counter, hipMemcpyHostToDevice));
+  CHK(hcopy(h, h->nctr, &counter, sizeof counter, hipMemcpyHostToDevice));
   h->nctr_h = counter;
   return 0;
 }
@@ -1885,7 +1905,7 @@ int tdgl_noise_draws(tdgl_handle *h, int32_t nsteps, const double *draws) {
   if (nsteps > 0) {
     if (hipMalloc((void **)&h->ndraw, (size_t)stride * nsteps * sizeof(double)) != hipSuccess)
       return fail(TDGL_E_NOMEM, "hipMalloc of the noise draws failed");
-    HIPCHK(hipMemset(h->ndraw, 0, (size_t)stride * nsteps * sizeof(double)));
+    CHK(hfill0(h, h->ndraw, (size_t)stride * nsteps * sizeof(double)));
     const size_t nn = (size_t)(Nx + 1) * (Ny + 1) * (Nz + 1);
     const int sh[3][3] = {{Nx, Ny + 1, Nz + 1}, {Nx + 1, Ny, Nz + 1}, {Nx + 1, Ny + 1, Nz}};
     const double *p = draws;
@@ -1901,9 +1921,9 @@ int tdgl_noise_draws(tdgl_handle *h, int32_t nsteps, const double *draws) {
   }
   // the first of these draws is for the next step
   uint64_t c[2];
-  HIPCHK(hipMemcpy(c, h->nctr, sizeof c, hipMemcpyDeviceToHost));
+  CHK(hcopy(h, c, h->nctr, sizeof c, hipMemcpyDeviceToHost));
   c[1] = c[0];
-  HIPCHK(hipMemcpy(h->nctr, c, sizeof c, hipMemcpyHostToDevice));
+  CHK(hcopy(h, h->nctr, c, sizeof c, hipMemcpyHostToDevice));
   h->nctr_h = c[0];
   h->nbase_h = c[0];
   return 0;
@@ -1920,7 +1940,7 @@ int tdgl_get_open(tdgl_handle *h, tdgl_open *o) {
     CHK(get(h, o->hnew[c], h->ohnew[c], sh[c][0], sh[c][1], sh[c][2], 8));
   }
   int fl[4];
-  HIPCHK(hipMemcpy(fl, h->ofl, sizeof fl, hipMemcpyDeviceToHost));
+  CHK(hcopy(h, fl, h->ofl, sizeof fl, hipMemcpyDeviceToHost));
   o->count = fl[OF_COUNT];
   return 0;
 }
@@ -1936,7 +1956,7 @@ int tdgl_set_open(tdgl_handle *h, const tdgl_open *o) {
     CHK(put(h, h->ohnew[c], o->hnew[c], sh[c][0], sh[c][1], sh[c][2], 8));
   }
   const int fl[4] = {o->count, 0, 0, 0};
-  HIPCHK(hipMemcpy(h->ofl, fl, sizeof fl, hipMemcpyHostToDevice));
+  CHK(hcopy(h, h->ofl, fl, sizeof fl, hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -2002,7 +2022,7 @@ int tdgl_meter_begin(tdgl_handle *h, const tdgl_meter *m) {
   const uint8_t *src[4] = {m->Min, m->Mou, m->line_ok, m->line_f1};
   for (int q = 0; q < 4; ++q) {
     CHK(malloc_meter(h, &mk[q], (size_t)nl));
-    HIPCHK(hipMemcpy(mk[q], src[q], (size_t)nl, hipMemcpyHostToDevice));
+    CHK(hcopy(h, mk[q], src[q], (size_t)nl, hipMemcpyHostToDevice));
   }
   a.Min = mk[0];
   a.Mou = mk[1];
@@ -2012,14 +2032,14 @@ int tdgl_meter_begin(tdgl_handle *h, const tdgl_meter *m) {
     CHK(malloc_meter(h, &a.Dprev[mu], (size_t)nl));
     CHK(malloc_meter(h, &a.T[mu], (size_t)nl));
     CHK(malloc_meter(h, &a.cooper[mu], (size_t)nb));
-    HIPCHK(hipMemset(a.Dprev[mu], 0, nl * sizeof(double)));
-    HIPCHK(hipMemset(a.T[mu], 0, nl * sizeof(double)));
+    CHK(hfill0(h, a.Dprev[mu], nl * sizeof(double)));
+    CHK(hfill0(h, a.T[mu], nl * sizeof(double)));
   }
   CHK(malloc_meter(h, &a.hold, (size_t)g.n));
   CHK(malloc_meter(h, &a.part, (size_t)meter_blocks(a) * METER_Q));
   CHK(malloc_meter(h, &h->mrec, (size_t)MR_N));
   CHK(malloc_meter(h, &h->mrung, 2));
-  HIPCHK(hipMemset(h->mrung, 0, 2 * sizeof(double)));
+  CHK(hfill0(h, h->mrung, 2 * sizeof(double)));
   h->ma = a;
   h->meter = true;
   // Cooper0 and fieldOld of the current state

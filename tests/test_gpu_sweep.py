@@ -1,0 +1,56 @@
+"""Several runs on the device at once (juliatdgl_amd.sweep): every run's result is the same alone
+and in the mix, the server's footprint bound holds, and noisy runs served from a Poisson stream end
+at the same first phase slip as alone.  Needs a HIP device."""
+import pytest
+
+from juliatdgl_amd import sweep
+from juliatdgl_amd.sweep import Cycles, FirstSlip, RunSpec, Server, footprint, poisson_arrivals, run_batch, stepper_of
+
+pytestmark = pytest.mark.gpu
+
+
+def _specs():
+    out = []
+    for q, (nx, ny, nz) in enumerate(((32, 16, 4), (40, 24, 8), (64, 16, 4), (48, 32, 12), (36, 20, 8), (56, 24, 4))):
+        out.append(RunSpec(q, nx, ny, nz, (0.1 + 0.05 * q) * (-1) ** q, depth=0.1 + 0.05 * q, seed=100 + q))
+    return out
+
+
+def test_runs_are_the_same_alone_and_in_the_mix():
+    specs = _specs()
+    print("[gpu] sweep: alone and in the mix", flush=True)
+    alone, t1 = run_batch(specs, Cycles(cycles=6), workers=1)
+    mix, t6 = run_batch(specs, Cycles(cycles=6), workers=6)
+    print(f"    one at a time {t1:.2f} s, six at a time {t6:.2f} s", flush=True)
+    for q in alone:
+        assert alone[q] == mix[q], q
+
+
+def test_served_runs_fit_their_footprint_and_match_alone():
+    specs = _specs()[:4]
+    print("[gpu] served runs", flush=True)
+    srv = Server(budget_bytes=2.2 * max(footprint(stepper_of(s)) for s in specs), slots=3)
+    for s, t in zip(specs, poisson_arrivals(len(specs), rate=20.0)):
+        srv.submit(s, t)
+    done = srv.run(Cycles(cycles=4))
+    alone, _ = run_batch(specs, Cycles(cycles=4), workers=1)
+    for s in specs:
+        d = done[s.run_id]
+        assert 0 < d["bytes"] <= footprint(stepper_of(s)), (d["bytes"], footprint(stepper_of(s)))
+        assert d["result"] == alone[s.run_id]
+
+
+def test_noisy_runs_end_at_the_same_first_slip_served():
+    """Requests with thermal noise (each its own counter-based stream) that end at their first phase
+    slip: the same slip time served in the mix as alone."""
+    specs = [RunSpec(q, 40, 20, 4, 1.2 * (-1) ** q, depth=0.2, seed=7 + q, noise=1e-3) for q in range(4)]
+    print("[gpu] noisy runs to the first slip", flush=True)
+    work = FirstSlip(T=60.0, every=10)
+    alone, _ = run_batch(specs, work, workers=1)
+    srv = Server(budget_bytes=200e9, slots=4)
+    for s in specs:
+        srv.submit(s, 0.0)
+    done = srv.run(work)
+    print("    " + ", ".join(f"{q}: {alone[q][0]}" for q in sorted(alone)), flush=True)
+    for q in alone:
+        assert done[q]["result"] == alone[q]
