@@ -198,10 +198,8 @@ int dalloc(tdgl_handle *h, T **p, size_t n) {
   if (n == 0) n = 1;
   if (hipMalloc(&q, n * sizeof(T)) != hipSuccess) return fail(TDGL_E_NOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
   h->allocs.push_back(q);
-  {
-    const int rc = hfill0(h, q, n * sizeof(T));
-    if (rc) return rc;
-  }
+  // zeroed on the handle's stream; tdgl_create waits for the fills once, at its end
+  if (hipMemsetAsync(q, 0, n * sizeof(T), h->s) != hipSuccess) return fail(TDGL_E_DEVICE, "hipMemsetAsync failed");
   h->bytes += (int64_t)(n * sizeof(T));
   *p = (T *)q;
   return 0;
@@ -1650,6 +1648,7 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
   };
   int rc = run();
   if (rc) return bail(rc);
+  if (hipStreamSynchronize(h->s) != hipSuccess) return bail(fail(TDGL_E_DEVICE, "the fills of the allocations"));
   *out = h;
   return 0;
 }
