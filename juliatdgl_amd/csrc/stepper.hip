@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,19 @@ const char *const K_NAMES[K_COUNT] = {"band_stage", "normal_fluids", "current_fi
                                       "pcg_dot", "pcg_finish", "pcg_vector", "cycle_meter"};
 static_assert(K_COUNT == TDGL_NKERNELS, "kernel classes");
 
+// a device or pinned host block of the process's cache
+struct Block {
+  void *p;
+  size_t cap;
+  int dev;  // -1: pinned host
+};
+
+struct PendingGet {
+  void *host;
+  int n1, n2, n3;
+  size_t es;
+};
+
 struct HostSnap {
   cplx *G[TDGL_MAX_BANDS], *F[TDGL_MAX_BANDS];
   double *th[3], *ph[3];
@@ -81,7 +95,7 @@ struct tdgl_handle {
   int dev = 0;
   hipStream_t s = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  std::vector<void *> allocs;
+  std::vector<Block> allocs;   // device blocks, given back to the process's cache by tdgl_destroy
   int64_t bytes = 0;
   // state (state.jl:51-60); advance! swaps G/F, θ/φ, V/U
   cplx *G[TDGL_MAX_BANDS] = {}, *F[TDGL_MAX_BANDS] = {};
@@ -107,7 +121,7 @@ struct tdgl_handle {
   std::vector<double> Fup[TDGL_MAX_BANDS];
   bool stepped = false;
   // the ladder's per-cycle measurements (tdgl_meter_begin): its own allocations
-  std::vector<void *> mallocs;
+  std::vector<Block> mallocs;
   MeterArgs ma{};
   double *mrec = nullptr, *mrung = nullptr;
   bool meter = false;
@@ -151,6 +165,7 @@ struct tdgl_handle {
   bool noise = false;
   uint64_t *nctr = nullptr;
   double *ndraw = nullptr;
+  Block ndraw_b{};
   long long nstride = 0, nslots = 0;
   uint64_t nctr_h = 0, nbase_h = 0;
   cplx *Sp = nullptr;
@@ -161,6 +176,9 @@ struct tdgl_handle {
   double device_ms = 0;
   // host staging
   std::vector<unsigned char> stage;
+  std::vector<Block> stages;   // pinned staging of the copies queued by put_async / get_async
+  std::vector<PendingGet> gets;                    // where get_flush writes the queued downloads
+  std::vector<Block> pins;                         // pinned blocks of flh and hph
   // per-kernel profile (HIP events around each launch while enabled)
   bool prof = false;
   std::vector<hipEvent_t> evs;
@@ -192,12 +210,87 @@ int hfill0(tdgl_handle *h, void *dst, size_t bytes) {
   return 0;
 }
 
+// Device and pinned host blocks are kept by the process once a handle is done with them and
+// handed to the next handle that asks for at most as much (and at least 2/3 of it): a free, of
+// either kind, waits for the whole device and returns the memory to the driver, and mapping it
+// again costs as much, so creating and destroying one run would stall the runs that other host
+// threads drive meanwhile.  A handle waits for its own stream before it gives its blocks back,
+// so a block is idle when another handle takes it.  Above CACHE_MAX idle bytes of one kind, the
+// largest idle blocks are freed.
+constexpr size_t CACHE_MAX = (size_t)64 << 30;
+std::mutex g_cache_mu;
+std::vector<Block> g_cache;
+size_t g_cached[2] = {0, 0};  // idle bytes: device, pinned host
+
+size_t round_cap(size_t b) { return (std::max<size_t>(b, 1) + 255) / 256 * 256; }
+bool cache_take(int dev, size_t bytes, Block *out) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  int best = -1;
+  for (size_t q = 0; q < g_cache.size(); ++q) {
+    const Block &b = g_cache[q];
+    if (b.dev != dev || b.cap < bytes || 2 * b.cap > 3 * bytes + 768) continue;
+    if (best < 0 || b.cap < g_cache[best].cap) best = (int)q;
+  }
+  if (best < 0) return false;
+  *out = g_cache[best];
+  g_cache[best] = g_cache.back();
+  g_cache.pop_back();
+  g_cached[dev < 0] -= out->cap;
+  return true;
+}
+void cache_give(const Block &b) {
+  std::vector<Block> drop;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_cache.push_back(b);
+    const int k = b.dev < 0;
+    g_cached[k] += b.cap;
+    while (g_cached[k] > CACHE_MAX) {
+      int big = -1;
+      for (size_t q = 0; q < g_cache.size(); ++q)
+        if ((g_cache[q].dev < 0) == (k == 1) && (big < 0 || g_cache[q].cap > g_cache[big].cap)) big = (int)q;
+      drop.push_back(g_cache[big]);
+      g_cached[k] -= g_cache[big].cap;
+      g_cache[big] = g_cache.back();
+      g_cache.pop_back();
+    }
+  }
+  for (const Block &x : drop) {
+    if (x.dev < 0) {
+      (void)hipHostFree(x.p);
+    } else {
+      int cur = 0;
+      (void)hipGetDevice(&cur);
+      (void)hipSetDevice(x.dev);
+      (void)hipFree(x.p);
+      (void)hipSetDevice(cur);
+    }
+  }
+}
+// a device block of at least `bytes` on the handle's device
+hipError_t dblock(tdgl_handle *h, size_t bytes, Block *out) {
+  const size_t cap = round_cap(bytes);
+  if (cache_take(h->dev, cap, out)) return hipSuccess;
+  out->cap = cap;
+  out->dev = h->dev;
+  return hipMalloc(&out->p, cap);
+}
+// a pinned host block of at least `bytes`
+hipError_t hblock(size_t bytes, Block *out) {
+  const size_t cap = round_cap(bytes);
+  if (cache_take(-1, cap, out)) return hipSuccess;
+  out->cap = cap;
+  out->dev = -1;
+  return hipHostMalloc(&out->p, cap, hipHostMallocDefault);
+}
+
 template <class T>
 int dalloc(tdgl_handle *h, T **p, size_t n) {
-  void *q = nullptr;
   if (n == 0) n = 1;
-  if (hipMalloc(&q, n * sizeof(T)) != hipSuccess) return fail(TDGL_E_NOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
-  h->allocs.push_back(q);
+  Block b{};
+  if (dblock(h, n * sizeof(T), &b) != hipSuccess) return fail(TDGL_E_NOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
+  h->allocs.push_back(b);
+  void *q = b.p;
   // zeroed on the handle's stream; tdgl_create waits for the fills once, at its end
   if (hipMemsetAsync(q, 0, n * sizeof(T), h->s) != hipSuccess) return fail(TDGL_E_DEVICE, "hipMemsetAsync failed");
   h->bytes += (int64_t)(n * sizeof(T));
@@ -216,16 +309,72 @@ int upload_vec(tdgl_handle *h, T *dst, const std::vector<T> &v) {
   return hcopy(h, dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
 }
 
-// Julia-shaped (n1, n2, n3) host array of `es`-byte elements <-> node-shaped device array
-int put(tdgl_handle *h, void *dev, const void *host, int n1, int n2, int n3, size_t es) {
+// Julia-shaped (n1, n2, n3) host array of `es`-byte elements -> node-shaped device array, the
+// copy queued on the handle's stream from a staging buffer of its own; `flush` waits for the
+// queued copies and drops their staging
+void stages_back(tdgl_handle *h) {
+  for (const Block &b : h->stages) cache_give(b);
+  h->stages.clear();
+}
+int put_async(tdgl_handle *h, void *dev, const void *host, int n1, int n2, int n3, size_t es) {
   const Geo &g = h->g;
-  h->stage.assign((size_t)g.n * es, 0);
+  const size_t bytes = (size_t)g.n * es;
+  Block b{};
+  HIPCHK(hblock(bytes, &b));
+  h->stages.push_back(b);
+  unsigned char *st = (unsigned char *)b.p;
   const unsigned char *src = (const unsigned char *)host;
-  for (int k = 0; k < n3; ++k)
-    for (int j = 0; j < n2; ++j)
-      std::memcpy(&h->stage[((size_t)g.px * (j + (size_t)g.py * k)) * es], src + ((size_t)n1 * (j + (size_t)n2 * k)) * es,
-                  (size_t)n1 * es);
-  CHK(hcopy(h, dev, h->stage.data(), (size_t)g.n * es, hipMemcpyHostToDevice));
+  const size_t row = (size_t)n1 * es, prow = (size_t)g.px * es;
+  for (int k = 0; k < g.Nz + 1; ++k)
+    for (int j = 0; j < g.py; ++j) {
+      unsigned char *dst = st + ((size_t)g.px * (j + (size_t)g.py * k)) * es;
+      if (j < n2 && k < n3) {
+        std::memcpy(dst, src + ((size_t)n1 * (j + (size_t)n2 * k)) * es, row);
+        std::memset(dst + row, 0, prow - row);
+      } else {
+        std::memset(dst, 0, prow);
+      }
+    }
+  HIPCHK(hipMemcpyAsync(dev, st, bytes, hipMemcpyHostToDevice, h->s));
+  return 0;
+}
+int flush(tdgl_handle *h) {
+  const hipError_t e = hipStreamSynchronize(h->s);
+  stages_back(h);
+  if (e != hipSuccess) return fail(TDGL_E_DEVICE, "the queued copies: %s", hipGetErrorString(e));
+  return 0;
+}
+int put(tdgl_handle *h, void *dev, const void *host, int n1, int n2, int n3, size_t es) {
+  CHK(put_async(h, dev, host, n1, n2, n3, es));
+  return flush(h);
+}
+// node-shaped device array -> Julia-shaped host array: the copy queued into a staging buffer of
+// its own, the host array written by get_flush after one wait for all queued copies
+int get_async(tdgl_handle *h, void *host, const void *dev, int n1, int n2, int n3, size_t es) {
+  const Geo &g = h->g;
+  Block b{};
+  HIPCHK(hblock((size_t)g.n * es, &b));
+  h->stages.push_back(b);
+  h->gets.push_back(PendingGet{host, n1, n2, n3, es});
+  HIPCHK(hipMemcpyAsync(b.p, dev, (size_t)g.n * es, hipMemcpyDeviceToHost, h->s));
+  return 0;
+}
+int get_flush(tdgl_handle *h) {
+  const Geo &g = h->g;
+  const hipError_t e = hipStreamSynchronize(h->s);
+  if (e == hipSuccess)
+    for (size_t q = 0; q < h->gets.size(); ++q) {
+      const PendingGet &pg = h->gets[q];
+      const unsigned char *st = (const unsigned char *)h->stages[q].p;
+      unsigned char *dst = (unsigned char *)pg.host;
+      for (int k = 0; k < pg.n3; ++k)
+        for (int j = 0; j < pg.n2; ++j)
+          std::memcpy(dst + ((size_t)pg.n1 * (j + (size_t)pg.n2 * k)) * pg.es,
+                      &st[((size_t)g.px * (j + (size_t)g.py * k)) * pg.es], (size_t)pg.n1 * pg.es);
+    }
+  h->gets.clear();
+  stages_back(h);
+  if (e != hipSuccess) return fail(TDGL_E_DEVICE, "the queued copies: %s", hipGetErrorString(e));
   return 0;
 }
 int get(tdgl_handle *h, void *host, const void *dev, int n1, int n2, int n3, size_t es) {
@@ -945,11 +1094,14 @@ int open_green(tdgl_handle *h) {
     const size_t n = (size_t)M0 * M1 * M2;
     double *g = nullptr;
     cplx *G = nullptr;
-    HIPCHK(hipMalloc((void **)&g, n * sizeof(double)));
-    if (hipMalloc((void **)&G, n * sizeof(cplx)) != hipSuccess) {
-      (void)hipFree(g);
+    Block bg{}, bG{};
+    HIPCHK(dblock(h, n * sizeof(double), &bg));
+    if (dblock(h, n * sizeof(cplx), &bG) != hipSuccess) {
+      cache_give(bg);
       return fail(TDGL_E_NOMEM, "hipMalloc of the padded Green function failed");
     }
+    g = (double *)bg.p;
+    G = (cplx *)bG.p;
     OpenGreen og{};
     og.out = g;
     og.M[0] = M0;
@@ -1007,8 +1159,10 @@ int open_green(tdgl_handle *h) {
       return 0;
     };
     rc = run();
-    (void)hipFree(g);
-    (void)hipFree(G);
+    // run() waits for its stream on success; on failure, wait before the blocks go back
+    if (rc) (void)hipStreamSynchronize(h->s);
+    cache_give(bg);
+    cache_give(bG);
     if (rc) return rc;
   }
   return 0;
@@ -1605,8 +1759,13 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
       CHK(open_green(h));
     }
     ALLOC(h->hp, 2);
-    HIPCHK(hipHostMalloc((void **)&h->hph, 2 * sizeof(double), hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void **)&h->flh, FL_COUNT * sizeof(int), hipHostMallocDefault));
+    Block b1{}, b2{};
+    HIPCHK(hblock(2 * sizeof(double), &b1));
+    h->pins.push_back(b1);
+    HIPCHK(hblock(FL_COUNT * sizeof(int), &b2));
+    h->pins.push_back(b2);
+    h->hph = (double *)b1.p;
+    h->flh = (int *)b2.p;
     // the copy of a cycle's start for the roll back of a captured PCG cycle
     if (d.field_mode == TDGL_FIELD_PCG) {
       int q = 0;
@@ -1656,12 +1815,13 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
 int tdgl_destroy(tdgl_handle *h) {
   if (!h) return 0;
   if (h->s) (void)hipStreamSynchronize(h->s);
-  for (void *p : h->allocs) (void)hipFree(p);
-  for (void *p : h->mallocs) (void)hipFree(p);
-  if (h->ndraw) (void)hipFree(h->ndraw);
   destroy_graph(h);
-  if (h->hph) (void)hipHostFree(h->hph);
-  if (h->flh) (void)hipHostFree(h->flh);
+  // the stream has drained (above): the blocks are idle
+  for (const Block &b : h->allocs) cache_give(b);
+  for (const Block &b : h->mallocs) cache_give(b);
+  if (h->ndraw) cache_give(h->ndraw_b);
+  for (const Block &b : h->pins) cache_give(b);
+  stages_back(h);
   for (hipEvent_t e : h->evs) (void)hipEventDestroy(e);
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
@@ -1677,8 +1837,11 @@ int tdgl_upload_state(tdgl_handle *h, const tdgl_state *s) {
   const Geo &g = h->g;
   HIPCHK(hipStreamSynchronize(h->s));
   for (int mu = 0; mu < d.nbands; ++mu) {
-    if (!s->G[mu] || !s->F[mu]) return fail(TDGL_E_ARG, "null band array");
-    CHK(put(h, h->G[mu], s->G[mu], Nx + 1, Ny + 1, Nz + 1, 16));
+    if (!s->G[mu] || !s->F[mu]) {
+      (void)flush(h);
+      return fail(TDGL_E_ARG, "null band array");
+    }
+    CHK(put_async(h, h->G[mu], s->G[mu], Nx + 1, Ny + 1, Nz + 1, 16));
     // F takes G's values outside the block (they are never stepped and advance! copies only
     // the block), which makes the advance a swap
     std::vector<double> Fh((size_t)(Nx + 1) * (Ny + 1) * (Nz + 1) * 2);
@@ -1692,7 +1855,7 @@ int tdgl_upload_state(tdgl_handle *h, const tdgl_state *s) {
           Fh[2 * q] = s->G[mu][2 * q];
           Fh[2 * q + 1] = s->G[mu][2 * q + 1];
         }
-    CHK(put(h, h->F[mu], Fh.data(), Nx + 1, Ny + 1, Nz + 1, 16));
+    CHK(put_async(h, h->F[mu], Fh.data(), Nx + 1, Ny + 1, Nz + 1, 16));
     h->Fup[mu].assign(s->F[mu], s->F[mu] + Fh.size());
   }
   h->stepped = false;
@@ -1705,33 +1868,55 @@ int tdgl_upload_state(tdgl_handle *h, const tdgl_state *s) {
   const double *U[3] = {s->Ux, s->Uy, s->Uz}, *V[3] = {s->Vx, s->Vy, s->Vz};
   const double *J[3] = {s->Jx, s->Jy, s->Jz}, *hh[3] = {s->hx, s->hy, s->hz};
   for (int c = 0; c < 3; ++c) {
-    if (!th[c] || !ph[c] || !U[c] || !V[c] || !J[c] || !hh[c]) return fail(TDGL_E_ARG, "null state array");
-    CHK(put(h, h->th[c], th[c], ls[c][0], ls[c][1], ls[c][2], 8));
-    CHK(put(h, h->ph[c], ph[c], ls[c][0], ls[c][1], ls[c][2], 8));
-    CHK(put(h, h->U[c], U[c], ls[c][0], ls[c][1], ls[c][2], 16));
-    CHK(put(h, h->V[c], V[c], ls[c][0], ls[c][1], ls[c][2], 16));
-    CHK(put(h, h->J[c], J[c], ls[c][0], ls[c][1], ls[c][2], 8));
-    CHK(put(h, h->h[c], hh[c], hs[c][0], hs[c][1], hs[c][2], 8));
+    if (!th[c] || !ph[c] || !U[c] || !V[c] || !J[c] || !hh[c]) {
+      (void)flush(h);
+      return fail(TDGL_E_ARG, "null state array");
+    }
+    CHK(put_async(h, h->th[c], th[c], ls[c][0], ls[c][1], ls[c][2], 8));
+    CHK(put_async(h, h->ph[c], ph[c], ls[c][0], ls[c][1], ls[c][2], 8));
+    CHK(put_async(h, h->U[c], U[c], ls[c][0], ls[c][1], ls[c][2], 16));
+    CHK(put_async(h, h->V[c], V[c], ls[c][0], ls[c][1], ls[c][2], 16));
+    CHK(put_async(h, h->J[c], J[c], ls[c][0], ls[c][1], ls[c][2], 8));
+    CHK(put_async(h, h->h[c], hh[c], hs[c][0], hs[c][1], hs[c][2], 8));
   }
-  return 0;
+  return flush(h);
 }
 
 int tdgl_download_state(tdgl_handle *h, tdgl_state *s) {
   if (!h || !s) return fail(TDGL_E_ARG, "null argument");
   const tdgl_desc &d = h->d;
   const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz;
-  HIPCHK(hipStreamSynchronize(h->s));
-  for (int mu = 0; mu < d.nbands; ++mu) {
+  for (int mu = 0; mu < d.nbands; ++mu)
     if (!s->G[mu] || !s->F[mu]) return fail(TDGL_E_ARG, "null band array");
-    CHK(get(h, s->G[mu], h->G[mu], Nx + 1, Ny + 1, Nz + 1, 16));
+  const int lx[3] = {Nx, Ny + 1, Nz + 1}, ly[3] = {Nx + 1, Ny, Nz + 1}, lz[3] = {Nx + 1, Ny + 1, Nz};
+  const int *ls[3] = {lx, ly, lz};
+  const int hx[3] = {Nx + 1, Ny, Nz}, hy[3] = {Nx, Ny + 1, Nz}, hz[3] = {Nx, Ny, Nz + 1};
+  const int *hs[3] = {hx, hy, hz};
+  double *th[3] = {s->thx, s->thy, s->thz}, *ph[3] = {s->phx, s->phy, s->phz};
+  double *U[3] = {s->Ux, s->Uy, s->Uz}, *V[3] = {s->Vx, s->Vy, s->Vz};
+  double *J[3] = {s->Jx, s->Jy, s->Jz}, *hh[3] = {s->hx, s->hy, s->hz};
+  for (int c = 0; c < 3; ++c)
+    if (!th[c] || !ph[c] || !U[c] || !V[c] || !J[c] || !hh[c]) return fail(TDGL_E_ARG, "null state array");
+  // every copy queued, one wait
+  for (int mu = 0; mu < d.nbands; ++mu) {
+    CHK(get_async(h, s->G[mu], h->G[mu], Nx + 1, Ny + 1, Nz + 1, 16));
+    if (!h->stepped) CHK(get_async(h, s->F[mu], h->F[mu], Nx + 1, Ny + 1, Nz + 1, 16));
+  }
+  for (int c = 0; c < 3; ++c) {
+    CHK(get_async(h, th[c], h->th[c], ls[c][0], ls[c][1], ls[c][2], 8));
+    CHK(get_async(h, ph[c], h->ph[c], ls[c][0], ls[c][1], ls[c][2], 8));
+    CHK(get_async(h, U[c], h->U[c], ls[c][0], ls[c][1], ls[c][2], 16));
+    CHK(get_async(h, V[c], h->V[c], ls[c][0], ls[c][1], ls[c][2], 16));
+    CHK(get_async(h, J[c], h->J[c], ls[c][0], ls[c][1], ls[c][2], 8));
+    CHK(get_async(h, hh[c], h->h[c], hs[c][0], hs[c][1], hs[c][2], 8));
+  }
+  CHK(get_flush(h));
+  for (int mu = 0; mu < d.nbands; ++mu) {
     // F as the reference leaves it (advance!, field.jl:248-255): after a step it equals G on the
     // block (the swap left the step before in the device's F), and outside the block it is the
     // F that was uploaded
     const size_t ne = (size_t)(Nx + 1) * (Ny + 1) * (Nz + 1) * 2;
-    if (h->stepped)
-      std::memcpy(s->F[mu], s->G[mu], ne * 8);
-    else
-      CHK(get(h, s->F[mu], h->F[mu], Nx + 1, Ny + 1, Nz + 1, 16));
+    if (h->stepped) std::memcpy(s->F[mu], s->G[mu], ne * 8);
     if (h->Fup[mu].size() == ne)
       for (int k = 1; k <= Nz + 1; ++k)
         for (int j = 1; j <= Ny + 1; ++j)
@@ -1741,22 +1926,6 @@ int tdgl_download_state(tdgl_handle *h, tdgl_state *s) {
             s->F[mu][2 * q] = h->Fup[mu][2 * q];
             s->F[mu][2 * q + 1] = h->Fup[mu][2 * q + 1];
           }
-  }
-  const int lx[3] = {Nx, Ny + 1, Nz + 1}, ly[3] = {Nx + 1, Ny, Nz + 1}, lz[3] = {Nx + 1, Ny + 1, Nz};
-  const int *ls[3] = {lx, ly, lz};
-  const int hx[3] = {Nx + 1, Ny, Nz}, hy[3] = {Nx, Ny + 1, Nz}, hz[3] = {Nx, Ny, Nz + 1};
-  const int *hs[3] = {hx, hy, hz};
-  double *th[3] = {s->thx, s->thy, s->thz}, *ph[3] = {s->phx, s->phy, s->phz};
-  double *U[3] = {s->Ux, s->Uy, s->Uz}, *V[3] = {s->Vx, s->Vy, s->Vz};
-  double *J[3] = {s->Jx, s->Jy, s->Jz}, *hh[3] = {s->hx, s->hy, s->hz};
-  for (int c = 0; c < 3; ++c) {
-    if (!th[c] || !ph[c] || !U[c] || !V[c] || !J[c] || !hh[c]) return fail(TDGL_E_ARG, "null state array");
-    CHK(get(h, th[c], h->th[c], ls[c][0], ls[c][1], ls[c][2], 8));
-    CHK(get(h, ph[c], h->ph[c], ls[c][0], ls[c][1], ls[c][2], 8));
-    CHK(get(h, U[c], h->U[c], ls[c][0], ls[c][1], ls[c][2], 16));
-    CHK(get(h, V[c], h->V[c], ls[c][0], ls[c][1], ls[c][2], 16));
-    CHK(get(h, J[c], h->J[c], ls[c][0], ls[c][1], ls[c][2], 8));
-    CHK(get(h, hh[c], h->h[c], hs[c][0], hs[c][1], hs[c][2], 8));
   }
   return 0;
 }
@@ -1893,7 +2062,7 @@ int tdgl_noise_draws(tdgl_handle *h, int32_t nsteps, const double *draws) {
   // the captured graph holds the old buffer's address
   CHK(destroy_graph(h));
   if (h->ndraw) {
-    HIPCHK(hipFree(h->ndraw));
+    cache_give(h->ndraw_b);
     h->ndraw = nullptr;
   }
   const int n = h->g.n;
@@ -1902,8 +2071,9 @@ int tdgl_noise_draws(tdgl_handle *h, int32_t nsteps, const double *draws) {
   h->nstride = stride;
   h->nslots = nsteps;
   if (nsteps > 0) {
-    if (hipMalloc((void **)&h->ndraw, (size_t)stride * nsteps * sizeof(double)) != hipSuccess)
+    if (dblock(h, (size_t)stride * nsteps * sizeof(double), &h->ndraw_b) != hipSuccess)
       return fail(TDGL_E_NOMEM, "hipMalloc of the noise draws failed");
+    h->ndraw = (double *)h->ndraw_b.p;
     CHK(hfill0(h, h->ndraw, (size_t)stride * nsteps * sizeof(double)));
     const size_t nn = (size_t)(Nx + 1) * (Ny + 1) * (Nz + 1);
     const int sh[3][3] = {{Nx, Ny + 1, Nz + 1}, {Nx + 1, Ny, Nz + 1}, {Nx + 1, Ny + 1, Nz}};
@@ -1976,10 +2146,10 @@ MeterArgs meter_args(tdgl_handle *h, int first) {
 }
 template <class T>
 int malloc_meter(tdgl_handle *h, T **p, size_t n) {
-  void *q = nullptr;
-  if (hipMalloc(&q, (n ? n : 1) * sizeof(T)) != hipSuccess) return fail(TDGL_E_NOMEM, "hipMalloc of the meter failed");
-  h->mallocs.push_back(q);
-  *p = (T *)q;
+  Block b{};
+  if (dblock(h, (n ? n : 1) * sizeof(T), &b) != hipSuccess) return fail(TDGL_E_NOMEM, "hipMalloc of the meter failed");
+  h->mallocs.push_back(b);
+  *p = (T *)b.p;
   return 0;
 }
 }  // namespace
@@ -1991,7 +2161,7 @@ int tdgl_meter_begin(tdgl_handle *h, const tdgl_meter *m) {
   const Geo &g = h->g;
   if (m->vi < d.x1 || m->vo > d.x2 || m->vi >= m->vo) return fail(TDGL_E_ARG, "voltmeter planes %d, %d outside x1..x2", m->vi, m->vo);
   HIPCHK(hipStreamSynchronize(h->s));
-  for (void *p : h->mallocs) (void)hipFree(p);
+  for (const Block &b : h->mallocs) cache_give(b);
   h->mallocs.clear();
   h->meter = false;
   MeterArgs a{};

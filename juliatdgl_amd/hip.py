@@ -108,11 +108,11 @@ class HipStepper:
             pass
 
     def upload(self, s: State) -> None:
-        self._shape = s.copy()
         _check(load().tdgl_upload_state(self.h, ct.byref(state_struct(s))), "tdgl_upload_state")
+        self._shape = s.empty_like()
 
     def download(self, into: State = None) -> State:
-        out = into if into is not None else self._shape.copy()
+        out = into if into is not None else self._shape.empty_like()
         _check(load().tdgl_download_state(self.h, ct.byref(state_struct(out))), "tdgl_download_state")
         return out
 

@@ -231,17 +231,16 @@ def _paint_wedge(mk: Masks, gr: Grid, xc: int, yc: int, d: int, half: int, depth
     """paint_defect! of a 'wedge' (geometry.jl:121-135): a notch growing from yc in direction d,
     a steep wall on -x and a gentle ramp on +x, through the thickness (z_mode through)."""
     x1, x2, y1, y2, z1, z2 = gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2
-    for k in range(z1, z2 + 1):
-        for j in range(y1, y2 + 1):
-            layer = d * (j - yc)
-            if 0 <= layer < depth:
-                frac = 1 - layer / depth
-                w_steep = max(1, _oround(2 * frac))
-                w_gentle = _oround(half * frac)
-                for i in range(x1, x2 + 1):
-                    dxr = i - xc
-                    if -w_steep <= dxr <= w_gentle:
-                        mk.f[i - 1, j - 1, k - 1] = 0.0
+    # the same cells for every k (through the thickness): per row j, the x run xc-w_steep..xc+w_gentle
+    for j in range(y1, y2 + 1):
+        layer = d * (j - yc)
+        if 0 <= layer < depth:
+            frac = 1 - layer / depth
+            w_steep = max(1, _oround(2 * frac))
+            w_gentle = _oround(half * frac)
+            lo, hi = max(x1, xc - w_steep), min(x2, xc + w_gentle)
+            if lo <= hi:
+                mk.f[lo - 1:hi, j - 1, z1 - 1:z2] = 0.0
 
 
 def catalogue_masks(gr: Grid, mode: str = "SIM", wedge_depth: float = 0.35, wedge_half: float = 0.20,

@@ -14,7 +14,11 @@ from .scheme import Grid, Masks, Model
 
 
 def emiphi(phi: np.ndarray) -> np.ndarray:
-    """Links e^{-iφ} (state.jl:62)."""
+    """Links e^{-iφ} (state.jl:62).  All +0.0 (a fresh state): 1 − 0i everywhere, as cos and sin of
+    −0.0 give it, without evaluating them."""
+    phi = np.asarray(phi, dtype=np.float64)
+    if not phi.view(np.uint64).any():
+        return np.full(phi.shape, complex(1.0, -0.0), dtype=np.complex128, order="F")
     out = np.empty(phi.shape, dtype=np.complex128, order="F")
     out.real = np.cos(-phi)
     out.imag = np.sin(-phi)
@@ -65,6 +69,11 @@ class State:
                      np.zeros((Nx, Ny + 1, Nz + 1), order="F"), np.zeros((Nx + 1, Ny, Nz + 1), order="F"),
                      np.zeros((Nx + 1, Ny + 1, Nz), order="F"), F(h[0]), F(h[1]), F(h[2]))
 
+    def empty_like(self) -> "State":
+        """A State of uninitialised arrays of the same shapes (what a download fills)."""
+        e = lambda a: np.empty(a.shape, dtype=a.dtype, order="F")
+        return State([e(g) for g in self.G], [e(f) for f in self.F], *(e(getattr(self, k)) for k in FIELDS))
+
     def copy(self) -> "State":
         return State([g.copy(order="F") for g in self.G], [f.copy(order="F") for f in self.F],
                      *(getattr(self, k).copy(order="F") for k in FIELDS))
@@ -75,29 +84,34 @@ class State:
         return out + [(k, getattr(self, k)) for k in FIELDS]
 
 
-def initial_state(md: Model, gr: Grid, mk: Masks, H: float) -> State:
-    """Meissner seed (state.jl:86-100): Ψ_µ = τψ·√(ε_µ/β_µ), A = 0, B_z = H on the z faces."""
+def _fresh(md: Model, gr: Grid, H: float, G) -> State:
+    """The state of fresh arrays G (taken, not copied) with A = 0 and B_z = H on the z faces:
+    F = G, θ = φ = 0, U = V = 1 − 0i (e^{-i·0}), J = 0 (state.jl:69-79, 86-100)."""
     Nx, Ny, Nz = gr.Nx, gr.Ny, gr.Nz
-    G = [np.asarray(mk.tpsi * np.sqrt(max(md.eps(mu), 0.0) / md.bands[mu].beta), dtype=np.complex128, order="F")
-         for mu in range(md.N)]
-    th = (np.zeros((Nx, Ny + 1, Nz + 1), order="F"), np.zeros((Nx + 1, Ny, Nz + 1), order="F"),
-          np.zeros((Nx + 1, Ny + 1, Nz), order="F"))
-    hx = np.zeros((Nx + 1, Ny, Nz), order="F")
-    hy = np.zeros((Nx, Ny + 1, Nz), order="F")
-    hz = np.zeros((Nx, Ny, Nz + 1), order="F")
+    sh = ((Nx, Ny + 1, Nz + 1), (Nx + 1, Ny, Nz + 1), (Nx + 1, Ny + 1, Nz))
+    z = lambda s: np.zeros(s, order="F")
+    one = lambda s: np.full(s, complex(1.0, -0.0), dtype=np.complex128, order="F")
+    hz = z((Nx, Ny, Nz + 1))
     hz[:, :, 0] = H
     hz[:, :, Nz] = H
-    return State.from_fields(gr, G, th, th, (hx, hy, hz))
+    return State(G, [g.copy(order="F") for g in G], *(z(s) for s in sh), *(z(s) for s in sh),
+                 *(one(s) for s in sh), *(one(s) for s in sh), *(z(s) for s in sh),
+                 z((Nx + 1, Ny, Nz)), z((Nx, Ny + 1, Nz)), hz)
+
+
+def initial_state(md: Model, gr: Grid, mk: Masks, H: float) -> State:
+    """Meissner seed (state.jl:86-100): Ψ_µ = τψ·√(ε_µ/β_µ), A = 0, B_z = H on the z faces."""
+    G = [np.asarray(mk.tpsi * np.sqrt(max(md.eps(mu), 0.0) / md.bands[mu].beta), dtype=np.complex128, order="F")
+         for mu in range(md.N)]
+    return _fresh(md, gr, H, [np.array(g, order="F", copy=True) for g in G])
 
 
 def seeded_state(md: Model, gr: Grid, mk: Masks, H: float, seed: int, amp: float = 0.01) -> State:
     """Ψ_µ = τψ·(1 + amp·(n₁ + i n₂)) with n from a seeded normal stream (numpy PCG64), A = 0,
     B_z = H on the z faces: the seeded initial condition of the benchmark configurations."""
-    st = initial_state(md, gr, mk, H)
     rng = np.random.default_rng(seed)
+    G = []
     for mu in range(md.N):
-        n = rng.standard_normal(size=(2,) + st.G[mu].shape)
-        g = mk.tpsi * (1.0 + amp * (n[0] + 1j * n[1]))
-        st.G[mu] = np.asarray(g, dtype=np.complex128, order="F")
-        st.F[mu] = st.G[mu].copy(order="F")
-    return st
+        n = rng.standard_normal(size=(2,) + mk.tpsi.shape)
+        G.append(np.asarray(mk.tpsi * (1.0 + amp * (n[0] + 1j * n[1])), dtype=np.complex128, order="F"))
+    return _fresh(md, gr, H, G)

@@ -15,6 +15,7 @@ normals, its deterministic reductions): the same alone and in the mix.
 """
 from __future__ import annotations
 
+import contextlib
 import heapq
 import math
 import threading
@@ -155,9 +156,20 @@ class FirstSlip:
         return (times[0] if times else None, t, div)
 
 
-def _execute(spec: RunSpec, work: Callable):
+def _one_blas_thread():
+    """BLAS on one thread while host threads drive several runs: each run's set-up (the bar
+    field's quadrature is two small matrix products) would otherwise start a thread team per
+    call, and eight concurrent teams oversubscribe the host's cores many times over."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:  # no control over the BLAS pool: leave it as it is
+        return contextlib.nullcontext()
+    return threadpool_limits(limits=1, user_api="blas")
+
+
+def _execute(spec: RunSpec, work: Callable, st: Optional[Stepper] = None):
     from .hip import HipStepper
-    st = stepper_of(spec)
+    st = st if st is not None else stepper_of(spec)
     dev = HipStepper(st)
     try:
         dev.upload(state_of(spec, st))
@@ -190,13 +202,14 @@ def run_batch(specs: List[RunSpec], work: Callable, workers: int = 8):
             with lock:
                 results[spec.run_id] = out
 
-    t0 = time.perf_counter()
-    ths = [threading.Thread(target=worker) for _ in range(max(1, workers))]
-    for t in ths:
-        t.start()
-    for t in ths:
-        t.join()
-    wall = time.perf_counter() - t0
+    with _one_blas_thread():
+        t0 = time.perf_counter()
+        ths = [threading.Thread(target=worker) for _ in range(max(1, workers))]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        wall = time.perf_counter() - t0
     if errors:
         raise errors[0]
     return results, wall
@@ -232,11 +245,11 @@ class Server:
         t0 = time.perf_counter()
         ths = []
 
-        def launch(req, need):
+        def launch(req, need, st):
             def body():
                 start = time.perf_counter() - t0
                 try:
-                    out, nbytes = _execute(req.spec, work)
+                    out, nbytes = _execute(req.spec, work, st)
                 except BaseException as e:  # noqa: BLE001
                     with cv:
                         state["error"] = e
@@ -255,25 +268,36 @@ class Server:
             ths.append(th)
             th.start()
 
-        with cv:
-            while self.pending and state["error"] is None:
-                req = self.pending[0]
-                now = time.perf_counter() - t0
-                if req.t_arrival > now:
-                    cv.wait(timeout=req.t_arrival - now)
-                    continue
-                need = float(footprint(stepper_of(req.spec)))
-                if need > self.budget:
-                    raise ValueError(f"run {req.run_id} needs {need:.3g} bytes, more than the budget")
-                if state["running"] >= self.slots or state["used"] + need > self.budget:
-                    cv.wait(timeout=0.05)
-                    continue
-                heapq.heappop(self.pending)
-                state["used"] += need
-                state["running"] += 1
-                launch(req, need)
-        for th in ths:
-            th.join()
+        steppers = {}
+        with _one_blas_thread():
+            while True:
+                build = None
+                with cv:
+                    if not self.pending or state["error"] is not None:
+                        break
+                    req = self.pending[0]
+                    now = time.perf_counter() - t0
+                    if req.t_arrival > now:
+                        cv.wait(timeout=req.t_arrival - now)
+                        continue
+                    if req.run_id not in steppers:
+                        build = req  # set up outside the lock: finishing runs take it meanwhile
+                    else:
+                        st = steppers[req.run_id]
+                        need = float(footprint(st))
+                        if need > self.budget:
+                            raise ValueError(f"run {req.run_id} needs {need:.3g} bytes, more than the budget")
+                        if state["running"] >= self.slots or state["used"] + need > self.budget:
+                            cv.wait(timeout=0.05)
+                            continue
+                        heapq.heappop(self.pending)
+                        state["used"] += need
+                        state["running"] += 1
+                        launch(req, need, steppers.pop(req.run_id))
+                if build is not None:
+                    steppers[build.run_id] = stepper_of(build.spec)
+            for th in ths:
+                th.join()
         if state["error"] is not None:
             raise state["error"]
         return self.done

@@ -43,7 +43,7 @@ def test_server_admits_within_budget_and_slots(monkeypatch):
     live, peak = {"n": 0, "bytes": 0}, {"n": 0, "bytes": 0}
     lock = threading.Lock()
 
-    def fake_execute(spec, work):
+    def fake_execute(spec, work, st=None):
         with lock:
             live["n"] += 1
             live["bytes"] += need[spec.run_id]
