@@ -153,7 +153,11 @@ def bench_sweep(args) -> int:
     print(f"[bench] one at a time: {t_seq:.2f} s", file=sys.stderr, flush=True)
     bat, t_bat = sweep.run_batch(specs, work, workers=args.workers)
     print(f"[bench] {args.workers} at a time: {t_bat:.2f} s", file=sys.stderr, flush=True)
-    same = all(seq[q] == bat[q] for q in seq)
+    kb, t_kb = sweep.run_batched(specs, work, group=len(specs), workers=args.workers)
+    print(f"[bench] batched kernels (one launch per kernel over all runs): {t_kb:.2f} s", file=sys.stderr, flush=True)
+    kb16, t_kb16 = sweep.run_batched(specs, work, group=16, workers=args.workers)
+    print(f"[bench] batched kernels, groups of 16 set up while the last runs: {t_kb16:.2f} s", file=sys.stderr, flush=True)
+    same = all(seq[q] == bat[q] == kb[q] == kb16[q] for q in seq)
     rate = len(specs) / t_bat
     srv = sweep.Server(budget_bytes=200e9, slots=args.workers)
     for s, t in zip(specs, sweep.poisson_arrivals(len(specs), rate=0.9 * rate, seed=5)):
@@ -172,6 +176,8 @@ def bench_sweep(args) -> int:
                                    f"J in +-[0.05, 0.4], {args.steps} ladder cycles of 40 steps per run",
                        "runs": len(specs), "cells": cells, "workers": args.workers,
                        "sequential_runs_per_hr": 3600.0 * len(specs) / t_seq, "speedup": t_seq / t_bat,
+                       "batched_kernels_runs_per_hr": 3600.0 * len(specs) / t_kb,
+                       "batched_kernels_groups16_runs_per_hr": 3600.0 * len(specs) / t_kb16,
                        "cell_cycles_per_s_batched": cells * args.steps / t_bat,
                        "same_alone_and_in_the_mix": bool(same and same_srv),
                        "served_p50_latency_s": lat["p50"], "served_p99_latency_s": lat["p99"],

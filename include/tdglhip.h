@@ -309,6 +309,26 @@ int tdgl_profile_enable(tdgl_handle *h, int32_t on);
 int tdgl_profile_get(tdgl_handle *h, tdgl_profile *p);
 const char *tdgl_kernel_name(int32_t k);
 
+/* Batched runs (no counterpart in the reference, whose runs are separate processes; the
+ * batched and server variants its GPU notes leave for later).  A batch advances many handles of
+ * one kind together: every step of every run goes through one launch per kernel over the runs'
+ * own grids (uneven sizes), the steps of a call replayed as one captured graph.  The handles keep
+ * their state, PCG history and meter; after a batch call each is where its own tdgl_step would
+ * have left it, to the bit: each run's kernels are the instances its own step takes.  (Only when
+ * the runs differ in the form of their separable solve do all take its single-axis passes, the
+ * same values to rounding, ~1e-15 a step.)
+ * Supported: the separable field (TDGL_FIELD_SEPARABLE), bar or slab boundary, one normal fluid,
+ * no thermal noise; the runs must have the same bands and stage structure.  A handle belongs to at
+ * most one live batch and is not used by itself while the batch runs (calls are synchronous).
+ * tdgl_batch_step: n steps of every run, run r at (H[r], HJ[r]).  tdgl_batch_meter_cycle: one
+ * tdgl_meter_cycle of every run (each after its own tdgl_meter_begin), out[r] for run r. */
+typedef struct tdgl_batch tdgl_batch;
+#define TDGL_BATCH_MAX 1024
+int tdgl_batch_create(tdgl_handle *const *hs, int32_t n, tdgl_batch **out);
+int tdgl_batch_step(tdgl_batch *b, int32_t n, const double *H, const double *HJ);
+int tdgl_batch_meter_cycle(tdgl_batch *b, int32_t first, tdgl_cycle_record *out);
+int tdgl_batch_destroy(tdgl_batch *b);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *tdgl_last_error(void);
 

@@ -6,6 +6,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <vector>
+
 namespace tdgl {
 
 // Every launch of the library goes through TDGL_LAUNCH.  While a profile pair is set (the per-kernel
@@ -460,6 +462,43 @@ struct MeterArgs {
 int meter_blocks(const MeterArgs &a);
 hipError_t launch_meter(const MeterArgs &a, hipStream_t s);
 hipError_t launch_meter_finish(const MeterArgs &a, double *rec, double *rung, hipStream_t s);
+
+// ---------------------------------------------------------------------------- batched launches
+// One launch runs a kernel over the blocks of many runs: run r owns the blocks off[r] ..
+// off[r+1]-1 of a 1D grid, which it takes as its own grid of dims[3r] x dims[3r+1] x dims[3r+2]
+// blocks (the grid its single-run launch would have), with its own Geo and argument struct.
+enum { BK_STAGE = 0, BK_FIELD, BK_TX, BK_TX2, BK_CURL, BK_GHOSTS, BK_METER, BK_METERFIN, BK_PLANE, BK_ZMODES, BK_N };
+struct BatchTab {
+  const int *off;    // n + 1 block offsets
+  const int *dims;   // 3 per run
+  const Geo *g;      // per run
+  const void *args;  // per run: the kind's argument struct
+  int n;
+};
+struct MeterFinArgs {
+  MeterArgs a;
+  double *rec, *rung;
+};
+// While a Recorder is set (host thread-locally), the launchers of the kinds above append their
+// launch to it instead of launching; any other launcher refuses (hipErrorNotSupported).
+struct BatchRec {
+  int kind;
+  int variant;    // the kernel instance of kinds that have several (the plane and z-mode passes)
+  bool has_grid;  // grid and lds as the launcher chose them (else batch_grid, no dynamic LDS)
+  dim3 grid;
+  size_t lds;
+  Geo g;
+  std::vector<unsigned char> args;
+};
+struct Recorder {
+  std::vector<BatchRec> recs;
+};
+extern thread_local Recorder *g_rec;
+size_t batch_args_size(int kind);
+dim3 batch_grid(int kind, const Geo &g, const void *args);  // the single-run grid of a record
+hipError_t launch_batch(int kind, int variant, size_t lds, const BatchTab &t, int blocks, hipStream_t s);
+dim3 meter_batch_grid(int kind, const void *args);
+hipError_t launch_batch_meter(int kind, const BatchTab &t, int blocks, hipStream_t s);
 
 hipError_t launch_pcg_pap(const PcgArgs &a, hipStream_t s);      // β, P = z + βP, zold = z, Q = A P, p·Ap
 hipError_t launch_pcg_xr(const PcgArgs &a, hipStream_t s);       // α, x += αP, r −= αQ, r·r

@@ -9,6 +9,49 @@
 namespace tdgl {
 
 thread_local ProfPair *prof_pair = nullptr;
+thread_local Recorder *g_rec = nullptr;
+
+static hipError_t record(int kind, const Geo &g, const void *args, size_t bytes, int variant = 0, bool has_grid = false,
+                         dim3 grid = dim3(0, 0, 0), size_t lds = 0) {
+  BatchRec r;
+  r.kind = kind;
+  r.variant = variant;
+  r.has_grid = has_grid;
+  r.grid = grid;
+  r.lds = lds;
+  r.g = g;
+  r.args.assign((const unsigned char *)args, (const unsigned char *)args + bytes);
+  g_rec->recs.push_back(std::move(r));
+  return hipSuccess;
+}
+
+// A kernel body's block: its place in its run's grid and that grid's size
+struct Blk {
+  int x, y, z, nx, ny, nz;
+};
+__device__ __forceinline__ Blk hw_blk() {
+  return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z};
+}
+// The run of a batched launch's block (the last r with off[r] <= block: runs of no blocks are
+// passed over) and the block's place in that run's grid
+struct BatchIdx {
+  int r;
+  Blk b;
+};
+__device__ __forceinline__ BatchIdx batch_idx(const BatchTab &t) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = t.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t.off[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int l = b - t.off[lo];
+  const int nx = t.dims[3 * lo], ny = t.dims[3 * lo + 1], nz = t.dims[3 * lo + 2];
+  return BatchIdx{lo, Blk{l % nx, (l / nx) % ny, l / (nx * ny), nx, ny, nz}};
+}
 
 // ============================================================================ band stage
 // Rate of the frame at a block node: the faces copy the interior (precedence x, y, z), so the
@@ -197,10 +240,10 @@ __device__ __forceinline__ cplx stage_tail(const Geo &g, const StageArgs &a, int
 // Stage ghosts: from the second (sub)stage on, the face values of S are the covariant Neumann
 // ghosts of S with this stage's links (ghosts!, :524-539), which for the nodes a stencil reads are
 // the link times the stencil's own centre: computed here instead of stored.
-__global__ void __launch_bounds__(256) k_stage(Geo g, StageArgs a) {
-  const int i = g.x1 + 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = g.y1 + 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = g.z1 + 1 + blockIdx.z;
+__device__ __forceinline__ void stage_body(const Geo &g, const StageArgs &a, const Blk bk) {
+  const int i = g.x1 + 1 + bk.x * blockDim.x + threadIdx.x;
+  const int j = g.y1 + 1 + bk.y * blockDim.y + threadIdx.y;
+  const int k = g.z1 + 1 + bk.z;
   if (i > g.x2 || j > g.y2 || k > g.z2) return;
   const int c = g.at(i, j, k);
   const int sx = 1, sy = g.px, sz = g.px * g.py;
@@ -275,7 +318,14 @@ __global__ void __launch_bounds__(256) k_stage(Geo g, StageArgs a) {
   a.D[c] = stage_tail(g, a, i, j, k, c, p, om);
 }
 
+__global__ void __launch_bounds__(256) k_stage(Geo g, StageArgs a) { stage_body(g, a, hw_blk()); }
+__global__ void __launch_bounds__(256) k_stage_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  stage_body(t.g[q.r], ((const StageArgs *)t.args)[q.r], q.b);
+}
+
 hipError_t launch_stage(const Geo &g, const StageArgs &a, hipStream_t s) {
+  if (g_rec) return record(BK_STAGE, g, &a, sizeof a);
   const int ni = g.x2 - g.x1, nj = g.y2 - g.y1, nk = g.z2 - g.z1;
   if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
   dim3 b(64, 4, 1), gr(cdiv(ni, 64), cdiv(nj, 4), nk);
@@ -307,6 +357,7 @@ __global__ void __launch_bounds__(256) k_frame_smooth(Geo g, StageArgs a, int pa
 }
 
 hipError_t launch_frame_smooth(const Geo &g, const StageArgs &a, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   const int ni = g.x2 - g.x1, nj = g.y2 - g.y1, nk = g.z2 - g.z1;
   if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
   if (!a.Om || !a.Om2) return hipErrorInvalidValue;
@@ -341,6 +392,7 @@ __global__ void __launch_bounds__(256) k_sk_src(Geo g, SkArgs a) {
 }
 
 hipError_t launch_sk_src(const Geo &g, const SkArgs &a, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   const int ni = g.x2 - g.x1, nj = g.y2 - g.y1, nk = g.z2 - g.z1;
   if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
   dim3 b(64, 4, 1), gr(cdiv(ni, 64), cdiv(nj, 4), nk);
@@ -354,6 +406,7 @@ __global__ void __launch_bounds__(64) k_noise_tick(uint64_t *ctr) {
 }
 
 hipError_t launch_noise_tick(uint64_t *ctr, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   TDGL_LAUNCH(k_noise_tick, dim3(1), dim3(64), 0, s, ctr);
   return hipGetLastError();
 }
@@ -371,12 +424,12 @@ __device__ __forceinline__ double jlink(const FieldArgs &a, int ax, cplx v, int 
 // side φₑ − θ of the implicit solve on its ranges (implicit.jl:93-95, 286-288).
 __device__ __forceinline__ double block_sum_t(double v, double *sh, int tid);
 
-__global__ void __launch_bounds__(256) k_field(Geo g, FieldArgs a) {
+__device__ __forceinline__ void field_body(const Geo &g, const FieldArgs &a, const Blk bk) {
   __shared__ double sh[256];
   const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const int i = 1 + bk.x * blockDim.x + threadIdx.x;
+  const int j = 1 + bk.y * blockDim.y + threadIdx.y;
+  const int k = 1 + bk.z;
   double bb = 0.0;
   if (i <= g.Nx + 1 && j <= g.Ny + 1 && k <= g.Nz + 1) {
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
@@ -449,17 +502,24 @@ __global__ void __launch_bounds__(256) k_field(Geo g, FieldArgs a) {
   if (a.part) {
     const double r = block_sum_t(bb, sh, tid);
     if (tid == 0) {
-      const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+      const int bid = bk.x + bk.nx * (bk.y + bk.ny * bk.z);
       a.part[2 * bid] = r;
       a.part[2 * bid + 1] = 0.0;
     }
   }
 }
 
+__global__ void __launch_bounds__(256) k_field(Geo g, FieldArgs a) { field_body(g, a, hw_blk()); }
+__global__ void __launch_bounds__(256) k_field_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  field_body(t.g[q.r], ((const FieldArgs *)t.args)[q.r], q.b);
+}
+
 int field_blocks(const Geo &g) { return cdiv(g.Nx + 1, 64) * cdiv(g.Ny + 1, 4) * (g.Nz + 1); }
 
 hipError_t launch_field(const Geo &g, const FieldArgs &a, hipStream_t s) {
   if (a.part && field_blocks(g) > MAX_PART) return hipErrorInvalidValue;
+  if (g_rec) return a.part ? hipErrorNotSupported : record(BK_FIELD, g, &a, sizeof a);
   dim3 b(64, 4, 1), gr(cdiv(g.Nx + 1, 64), cdiv(g.Ny + 1, 4), g.Nz + 1);
   TDGL_LAUNCH(k_field, gr, b, 0, s, g, a);
   return hipGetLastError();
@@ -468,10 +528,10 @@ hipError_t launch_field(const Geo &g, const FieldArgs &a, hipStream_t s) {
 // Links U = e^{−iφ} (links!, field.jl:105-121), field h = curl φ (curl_h!, :124-137) and the
 // field boundary (field_bc!, :189-228: applied H, slab or bar self-field; the later assignment
 // of the reference wins where two cover one element).
-__global__ void __launch_bounds__(256) k_curl(Geo g, CurlArgs a) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+__device__ __forceinline__ void curl_body(const Geo &g, const CurlArgs &a, const Blk bk) {
+  const int i = 1 + bk.x * blockDim.x + threadIdx.x;
+  const int j = 1 + bk.y * blockDim.y + threadIdx.y;
+  const int k = 1 + bk.z;
   if (i > g.Nx + 1 || j > g.Ny + 1 || k > g.Nz + 1) return;
   const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
   const int c = g.at(i, j, k);
@@ -569,12 +629,19 @@ __global__ void __launch_bounds__(256) k_curl(Geo g, CurlArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(256) k_curl(Geo g, CurlArgs a) { curl_body(g, a, hw_blk()); }
+__global__ void __launch_bounds__(256) k_curl_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  curl_body(t.g[q.r], ((const CurlArgs *)t.args)[q.r], q.b);
+}
+
 hipError_t launch_curl(const Geo &g, const CurlArgs &a, hipStream_t s) {
   // every table the boundary reads must be there: a missing one is an error, not a device fault
   if (a.bar && !(a.bar_hy_z && a.bar_hy_y && a.bar_hz_z && a.bar_hz_y && a.bar_hz_x && a.bar_hy_x))
     return hipErrorInvalidValue;
   if (a.open && !(a.ofl && a.hold[0] && a.hold[1] && a.hold[2] && a.hnew[0] && a.hnew[1] && a.hnew[2]))
     return hipErrorInvalidValue;
+  if (g_rec) return a.open ? hipErrorNotSupported : record(BK_CURL, g, &a, sizeof a);
   dim3 b(64, 4, 1), gr(cdiv(g.Nx + 1, 64), cdiv(g.Ny + 1, 4), g.Nz + 1);
   TDGL_LAUNCH(k_curl, gr, b, 0, s, g, a);
   return hipGetLastError();
@@ -616,9 +683,9 @@ __device__ __forceinline__ cplx ghost_value(const Geo &g, const cplx *X, const G
 
 // Threads over the six faces: face x (two planes j,k), face y (i,k), face z (i,j); edges are
 // computed by each face that holds them, to the same value.
-__global__ void __launch_bounds__(256) k_ghosts(Geo g, GhostArgs a) {
+__device__ __forceinline__ void ghosts_body(const Geo &g, const GhostArgs &a, const Blk bk) {
   const int ni = g.x2 - g.x1 + 2, nj = g.y2 - g.y1 + 2, nk = g.z2 - g.z1 + 2;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = bk.x * blockDim.x + threadIdx.x;
   const int fx = 2 * nj * nk, fy = 2 * ni * nk, fz = 2 * ni * nj;
   int i, j, k;
   if (t < fx) {
@@ -649,7 +716,14 @@ __global__ void __launch_bounds__(256) k_ghosts(Geo g, GhostArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(256) k_ghosts(Geo g, GhostArgs a) { ghosts_body(g, a, hw_blk()); }
+__global__ void __launch_bounds__(256) k_ghosts_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  ghosts_body(t.g[q.r], ((const GhostArgs *)t.args)[q.r], q.b);
+}
+
 hipError_t launch_ghosts(const Geo &g, const GhostArgs &a, hipStream_t s) {
+  if (g_rec) return record(BK_GHOSTS, g, &a, sizeof a);
   const int ni = g.x2 - g.x1 + 2, nj = g.y2 - g.y1 + 2, nk = g.z2 - g.z1 + 2;
   const int n = 2 * (nj * nk + ni * nk + ni * nj);
   TDGL_LAUNCH(k_ghosts, dim3(cdiv(n, 256)), dim3(256), 0, s, g, a);
@@ -666,6 +740,7 @@ __global__ void __launch_bounds__(256) k_copies(CopyList c) {
 }
 
 hipError_t launch_copies(const CopyList &c, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   if (c.cnt <= 0) return hipSuccess;
   int m = 0;
   for (int a = 0; a < c.cnt; ++a) m = c.n[a] > m ? c.n[a] : m;
@@ -779,7 +854,7 @@ __device__ __forceinline__ void tx_coords(int axis, int e0, int e1, int n, int o
 __device__ __forceinline__ int voff(const View &v, int i0, int i1, int i2) { return i0 + v.s1 * i1 + v.s2 * i2; }
 
 template <class TI, class TA, class TO>
-__global__ void __launch_bounds__(256) k_tx(TxPass p, const int *gate) {
+__device__ __forceinline__ void tx_body(const TxPass &p, const int *gate, const Blk bk) {
   constexpr int TXK = TxDepth<TA>::K;
   constexpr int NL = TXK * TXM / 256;  // loads per thread of each operand per chunk
   static_assert(TXM == 32 && TXO == 32 && NL * 256 == TXK * TXM, "tile shape");
@@ -788,7 +863,7 @@ __global__ void __launch_bounds__(256) k_tx(TxPass p, const int *gate) {
   __shared__ double sh[8];
   __shared__ int colb[TXO];
   const int tid = threadIdx.x;
-  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int bid = bk.x + bk.nx * (bk.y + bk.ny * bk.z);
   if (p.conv.mode != CONV_NONE) {
     // the convergence check that decides whether this solve goes on (implicit.jl:305, 319, 426)
     if (!*p.conv.in) {
@@ -807,13 +882,13 @@ __global__ void __launch_bounds__(256) k_tx(TxPass p, const int *gate) {
   } else if (gate && !*gate) {
     return;
   }
-  const int comp = blockIdx.z;
+  const int comp = bk.z;
   const int e0 = p.e[comp][0], e1 = p.e[comp][1], e2 = p.e[comp][2];
   const int axis = p.axis;
   const int L = axis == 0 ? e0 : (axis == 1 ? e1 : e2);
   const int O = (e0 * e1 * e2) / L;
   const int eo = axis == 0 ? e1 : e0;  // the first "other" extent: o = c1 + eo c2
-  const int m0 = blockIdx.y * TXM, o0 = blockIdx.x * TXO;
+  const int m0 = bk.y * TXM, o0 = bk.x * TXO;
   double d0 = 0.0, d1 = 0.0;
   if (m0 < L && o0 < O) {
     const TI *in = (const TI *)p.in[comp].p;
@@ -938,6 +1013,13 @@ __global__ void __launch_bounds__(256) k_tx(TxPass p, const int *gate) {
   }
 }
 
+template <class TI, class TA, class TO>
+__global__ void __launch_bounds__(256) k_tx(TxPass p, const int *gate) { tx_body<TI, TA, TO>(p, gate, hw_blk()); }
+__global__ void __launch_bounds__(256) k_tx_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  tx_body<double, double, double>(((const TxPass *)t.args)[q.r], nullptr, q.b);
+}
+
 static dim3 tx_grid(const TxPass &p) {
   int mo = 1, ml = 1;
   for (int c = 0; c < 3; ++c) {
@@ -956,6 +1038,11 @@ int tx_parts(const TxPass &p) {
 template <class TI, class TA, class TO>
 hipError_t launch_tx(const TxPass &p, const int *gate, hipStream_t s) {
   if (tx_parts(p) > MAX_PART) return hipErrorInvalidValue;
+  if (g_rec) {
+    constexpr bool f64 = sizeof(TI) == 8 && sizeof(TA) == 8 && sizeof(TO) == 8;
+    if (!f64 || gate || p.conv.mode != CONV_NONE || p.dot) return hipErrorNotSupported;
+    return record(BK_TX, Geo{}, &p, sizeof p);
+  }
   TDGL_LAUNCH((k_tx<TI, TA, TO>), tx_grid(p), dim3(256), 0, s, p, gate);
   return hipGetLastError();
 }
@@ -980,9 +1067,9 @@ __device__ __forceinline__ void project(T k1, T k2, T k3, T a, T &x1, T &y1, T &
 // X[mx-1, my, ·], Y[mx, my-1, ·] and Z[mx, my, ·] are transformed along z to the mode, projected
 // and written; every output element belongs to exactly one mode.
 template <class T>
-__global__ void __launch_bounds__(256) k_tx2_modes(Tx2Modes m, const int *gate) {
+__device__ __forceinline__ void tx2_modes_body(const Tx2Modes &m, const int *gate, const Blk bk) {
   if (gate && !*gate) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = bk.x * blockDim.x + threadIdx.x;
   const int nx = m.nx, ny = m.ny, nz = m.nz;
   if (t >= nx * ny * nz) return;
   const int mx = t % nx, my = (t / nx) % ny, mz = t / (nx * ny);
@@ -1020,7 +1107,15 @@ __global__ void __launch_bounds__(256) k_tx2_modes(Tx2Modes m, const int *gate) 
 }
 
 template <class T>
+__global__ void __launch_bounds__(256) k_tx2_modes(Tx2Modes m, const int *gate) { tx2_modes_body<T>(m, gate, hw_blk()); }
+__global__ void __launch_bounds__(256) k_tx2_modes_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  tx2_modes_body<double>(((const Tx2Modes *)t.args)[q.r], nullptr, q.b);
+}
+
+template <class T>
 hipError_t launch_tx2_modes(const Tx2Modes &m, const int *gate, hipStream_t s) {
+  if (g_rec) return (sizeof(T) != 8 || gate) ? hipErrorNotSupported : record(BK_TX2, Geo{}, &m, sizeof m);
   TDGL_LAUNCH((k_tx2_modes<T>), dim3(cdiv(m.nx * m.ny * m.nz, 256)), dim3(256), 0, s, m, gate);
   return hipGetLastError();
 }
@@ -1070,7 +1165,7 @@ static size_t plane_lds(int l1) {
 // TJ = ceil(y extent / 32) column slots per thread, a compile-time count so that the LDS reads
 // of one step of the contraction are issued together
 template <class TI, class T, class TO, int TJ>
-__global__ void __launch_bounds__(256) k_plane(PlanePass p, const int *gate) {
+__device__ __forceinline__ void plane_body(const PlanePass &p, const int *gate, const Blk bk) {
   constexpr int MS = PlaneCfg<T>::MS, IC = PlaneCfg<T>::IC, RM = PlaneCfg<T>::RM;
   static_assert(MS == 8 * RM && 256 % IC == 0 && (IC * MS) % 256 == 0, "plane tile");
   constexpr int NX = IC * MS / 256;     // staged X values per thread
@@ -1081,9 +1176,9 @@ __global__ void __launch_bounds__(256) k_plane(PlanePass p, const int *gate) {
   extern __shared__ __align__(16) unsigned char plane_smem[];
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int bid = bk.x + bk.nx * (bk.y + bk.ny * bk.z);
   if (!pass_active(p.conv, gate, p.d, sh, tid, bid)) return;
-  const int comp = blockIdx.z, plane = blockIdx.y, m0 = blockIdx.x * MS;
+  const int comp = bk.z, plane = bk.y, m0 = bk.x * MS;
   const int e0 = p.e[comp][0], e1 = p.e[comp][1], e2 = p.e[comp][2];
   double d0 = 0.0, d1 = 0.0;
   if (plane < e2 && m0 < e0) {
@@ -1235,6 +1330,14 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 // Rows of the staged chunks: 64 while the y extent is at most 128 (fewer dependent load rounds),
 // 32 above (the LDS and register budget of the wider tiles)
 template <int TJ> struct PlaneMfCfg { static constexpr int IC = TJ <= 4 ? 64 : 32; };
+
+template <class TI, class T, class TO, int TJ>
+__global__ void __launch_bounds__(256) k_plane(PlanePass p, const int *gate) { plane_body<TI, T, TO, TJ>(p, gate, hw_blk()); }
+template <int TJ>
+__global__ void __launch_bounds__(256) k_plane_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  plane_body<double, double, double, TJ>(((const PlanePass *)t.args)[q.r], nullptr, q.b);
+}
 
 template <class TI, class TO, int TJ>
 __global__ void __launch_bounds__(256) k_plane_mf(PlanePass p, const int *gate) {
@@ -1535,14 +1638,14 @@ __device__ __forceinline__ void mf_slots(int ns, typename Mf<TC>::v4 (&acc)[S], 
 }
 
 template <class TI, class TC, class TO, int XC, int TJ, int MT>
-__global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *gate, int gx, int gy) {
+__device__ __forceinline__ void plane_mf1_body(const PlanePass &p, const int *gate, int gx, int gy, int hwb, int nb) {
   using C = PlaneMf1<XC, TJ, MT, TC>;
   typedef typename Mf<TC>::v4 V4;
   constexpr int NTH = PLANE1_T, MS = C::MS, E0 = C::E0, L1P = C::L1P, E0S = C::E0S, JS = C::JS, S = C::S;
   extern __shared__ __align__(16) unsigned char plane_smem[];
   __shared__ double sh[16];
   const int tid = threadIdx.x;
-  const int nb = gridDim.x, bid = xcd_run(blockIdx.x, nb);
+  const int bid = xcd_run(hwb, nb);
   const int mb = bid % gx, plane = (bid / gx) % gy, comp = bid / (gx * gy), m0 = mb * MS;
   // the gate flag first: a pass the gate turns off loads nothing
   if (p.conv.mode != CONV_NONE) {
@@ -1706,6 +1809,19 @@ __global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *
   }
 }
 
+template <class TI, class TC, class TO, int XC, int TJ, int MT>
+__global__ void __launch_bounds__(PLANE1_T) k_plane_mf1(PlanePass p, const int *gate, int gx, int gy) {
+  plane_mf1_body<TI, TC, TO, XC, TJ, MT>(p, gate, gx, gy, blockIdx.x, gridDim.x);
+}
+// batched: run r's blocks take the run's own (gx, gy) = (dims[3r], dims[3r+1]) and block count
+template <int XC, int TJ, int MT>
+__global__ void __launch_bounds__(PLANE1_T) k_plane_mf1_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  const int l = q.b.x + q.b.nx * (q.b.y + q.b.ny * q.b.z);
+  plane_mf1_body<double, double, double, XC, TJ, MT>(((const PlanePass *)t.args)[q.r], nullptr, q.b.nx, q.b.ny, l,
+                                                      q.b.nx * q.b.ny * q.b.nz);
+}
+
 static size_t plane_mf_lds(int l1) {
   const int l1p = (l1 + 31) / 32 * 32;
   const int ic = l1p <= 128 ? 64 : 32;  // PlaneMfCfg
@@ -1779,6 +1895,18 @@ hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
   for (int c = 0; c < nc; ++c) l1 = p.e[c][1] > l1 ? p.e[c][1] : l1;
   const dim3 gr = plane_grid<T>(p);
   const bool mf = sizeof(T) == 4;  // Float32 on the matrix cores
+  if (g_rec) {
+    // the FP64 passes of the separable solve: the instance this launch would take
+    constexpr bool f64 = sizeof(TI) == 8 && sizeof(T) == 8 && sizeof(TO) == 8;
+    if (!f64 || gate || p.conv.mode != CONV_NONE || p.dot || nc != 3) return hipErrorNotSupported;
+    const Plane1Pick k = plane1_pick(p, 8);
+    if (k.use)
+      return record(BK_PLANE, Geo{}, &p, sizeof p, 1000 + (k.xc * 4 + k.tj) * 4 + k.mt, true, dim3(k.gx, k.gy, k.nc),
+                    plane1_lds(k.xc, k.tj, k.mt, 8));
+    const int tj = (l1 + 31) / 32;
+    if (tj < 1 || tj > 8) return hipErrorInvalidValue;
+    return record(BK_PLANE, Geo{}, &p, sizeof p, tj, true, gr, plane_lds<T>(l1));
+  }
   {
     const Plane1Pick k = plane1_pick(p, (int)sizeof(T));
     if (k.use) {
@@ -1837,7 +1965,7 @@ template bool plane_fits<double>(const int e[][3], int);
 // LB the four z bases too, so every global load is issued before the first wait; the sums run in
 // the same order as from global memory.
 template <class T, bool LB>
-__global__ void __launch_bounds__(256) k_zmodes(ZModes m, const int *gate) {
+__device__ __forceinline__ void zmodes_body(const ZModes &m, const int *gate, const Blk bk) {
   if (gate && !*gate) return;
   extern __shared__ __align__(16) unsigned char zm_smem[];
   const int nx = m.nx, ny = m.ny, nz = m.nz;
@@ -1846,7 +1974,7 @@ __global__ void __launch_bounds__(256) k_zmodes(ZModes m, const int *gate) {
   const int C = 256 / nz;
   const int t = threadIdx.x;
   const int tc = t / nz, mz = t % nz;
-  const int col = blockIdx.x * C + tc;
+  const int col = bk.x * C + tc;
   const bool live = tc < C && col < nx * ny;
   const int mx = live ? col % nx : 0, my = live ? col / nx : 0;
   const bool hx = mx > 0, hy = my > 0, hz = mz > 0;
@@ -1920,12 +2048,23 @@ __global__ void __launch_bounds__(256) k_zmodes(ZModes m, const int *gate) {
 }
 
 constexpr int ZM_LB_MAX = 64;  // z bases in LDS up to this nz
+template <class T, bool LB>
+__global__ void __launch_bounds__(256) k_zmodes(ZModes m, const int *gate) { zmodes_body<T, LB>(m, gate, hw_blk()); }
+template <bool LB>
+__global__ void __launch_bounds__(256) k_zmodes_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  zmodes_body<double, LB>(((const ZModes *)t.args)[q.r], nullptr, q.b);
+}
+
 template <class T>
 hipError_t launch_zmodes(const ZModes &m, const int *gate, hipStream_t s) {
   if (m.nz < 2 || m.nz > 256 || m.nx < 1 || m.ny < 1) return hipErrorInvalidValue;
   const int C = 256 / m.nz;
   const bool lb = m.nz <= ZM_LB_MAX;
   const size_t lds = (6 * 256 + (lb ? 2 * (size_t)m.nz * m.nz + 2 * (size_t)(m.nz - 1) * (m.nz - 1) : 0)) * sizeof(T);
+  if (g_rec)
+    return (sizeof(T) != 8 || gate) ? hipErrorNotSupported
+                                     : record(BK_ZMODES, Geo{}, &m, sizeof m, lb ? 1 : 0, true, dim3(cdiv(m.nx * m.ny, C)), lds);
   if (lb)
     TDGL_LAUNCH((k_zmodes<T, true>), dim3(cdiv(m.nx * m.ny, C)), dim3(256), lds, s, m, gate);
   else
@@ -1986,6 +2125,7 @@ __global__ void __launch_bounds__(256) k_qp_conv(Geo g, QpConv a) {
 }
 
 hipError_t launch_qp_conv(const Geo &g, const QpConv &a, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   const int n = a.n1 * a.nj * a.nk;
   TDGL_LAUNCH(k_qp_conv, dim3(cdiv(n, 256)), dim3(256), 0, s, g, a);
   return hipGetLastError();
@@ -2020,6 +2160,7 @@ __global__ void __launch_bounds__(256) k_qp_zcol(QpZ a) {
 }
 
 hipError_t launch_qp_zcol(const QpZ &a, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   if (a.nk < 1 || a.nk > 256 || a.K < 1 || a.K > TDGL_MAX_BANDS) return hipErrorInvalidValue;
   const int C = 256 / a.nk;
   TDGL_LAUNCH(k_qp_zcol, dim3(cdiv(a.n1 * a.nj, C), a.K), dim3(256), 0, s, a);
@@ -2358,6 +2499,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_finish(PcgArgs a) {
 }
 
 static hipError_t launch_box(void (*k)(PcgArgs), const PcgArgs &a, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   const int nb = pcg_box_blocks(a.g);
   if (nb > MAX_PART) return hipErrorInvalidValue;
   TDGL_LAUNCH(k, dim3(nb), dim3(RED_T), 0, s, a);
@@ -2380,8 +2522,92 @@ __global__ void k_pcg_cycle_reset(PcgDev d) {
 }
 
 hipError_t launch_pcg_cycle_reset(PcgDev d, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   TDGL_LAUNCH(k_pcg_cycle_reset, dim3(1), dim3(64), 0, s, d);
   return hipGetLastError();
 }
 
+// ============================================================================ batched launches
+size_t batch_args_size(int kind) {
+  switch (kind) {
+    case BK_STAGE: return sizeof(StageArgs);
+    case BK_FIELD: return sizeof(FieldArgs);
+    case BK_TX: return sizeof(TxPass);
+    case BK_TX2: return sizeof(Tx2Modes);
+    case BK_CURL: return sizeof(CurlArgs);
+    case BK_GHOSTS: return sizeof(GhostArgs);
+    case BK_METER: return sizeof(MeterArgs);
+    case BK_METERFIN: return sizeof(MeterFinArgs);
+    case BK_PLANE: return sizeof(PlanePass);
+    case BK_ZMODES: return sizeof(ZModes);
+    default: return 0;
+  }
+}
+// the grid of a record's single-run launch (as its launcher computes it)
+dim3 batch_grid(int kind, const Geo &g, const void *args) {
+  switch (kind) {
+    case BK_STAGE: {
+      const int ni = g.x2 - g.x1, nj = g.y2 - g.y1, nk = g.z2 - g.z1;
+      if (ni <= 0 || nj <= 0 || nk <= 0) return dim3(0, 0, 0);
+      return dim3(cdiv(ni, 64), cdiv(nj, 4), nk);
+    }
+    case BK_FIELD:
+    case BK_CURL: return dim3(cdiv(g.Nx + 1, 64), cdiv(g.Ny + 1, 4), g.Nz + 1);
+    case BK_GHOSTS: {
+      const int ni = g.x2 - g.x1 + 2, nj = g.y2 - g.y1 + 2, nk = g.z2 - g.z1 + 2;
+      return dim3(cdiv(2 * (nj * nk + ni * nk + ni * nj), 256));
+    }
+    case BK_TX: return tx_grid(*(const TxPass *)args);
+    case BK_TX2: {
+      const Tx2Modes &m = *(const Tx2Modes *)args;
+      return dim3(cdiv(m.nx * m.ny * m.nz, 256));
+    }
+    default: return meter_batch_grid(kind, args);
+  }
+}
+hipError_t launch_batch(int kind, int variant, size_t lds, const BatchTab &t, int blocks, hipStream_t s) {
+  if (blocks <= 0) return hipSuccess;
+  switch (kind) {
+    case BK_STAGE: TDGL_LAUNCH(k_stage_b, dim3(blocks), dim3(64, 4, 1), 0, s, t); break;
+    case BK_FIELD: TDGL_LAUNCH(k_field_b, dim3(blocks), dim3(64, 4, 1), 0, s, t); break;
+    case BK_CURL: TDGL_LAUNCH(k_curl_b, dim3(blocks), dim3(64, 4, 1), 0, s, t); break;
+    case BK_GHOSTS: TDGL_LAUNCH(k_ghosts_b, dim3(blocks), dim3(256), 0, s, t); break;
+    case BK_TX: TDGL_LAUNCH(k_tx_b, dim3(blocks), dim3(256), 0, s, t); break;
+    case BK_TX2: TDGL_LAUNCH(k_tx2_modes_b, dim3(blocks), dim3(256), 0, s, t); break;
+    case BK_ZMODES:
+      if (variant)
+        TDGL_LAUNCH(k_zmodes_b<true>, dim3(blocks), dim3(256), lds, s, t);
+      else
+        TDGL_LAUNCH(k_zmodes_b<false>, dim3(blocks), dim3(256), lds, s, t);
+      break;
+    case BK_PLANE:
+      if (variant >= 1000) {
+        switch (variant - 1000) {
+#define TDGL_PLANE1B_CASE(x, tt, m)                                                                 \
+  case (x * 4 + tt) * 4 + m:                                                                        \
+    TDGL_LAUNCH((k_plane_mf1_b<x, tt, m>), dim3(blocks), dim3(PLANE1_T), lds, s, t);               \
+    break;
+          TDGL_PLANE1B_CASE(3, 2, 2) TDGL_PLANE1B_CASE(3, 3, 2) TDGL_PLANE1B_CASE(3, 2, 3) TDGL_PLANE1B_CASE(3, 3, 3)
+          TDGL_PLANE1B_CASE(4, 2, 2) TDGL_PLANE1B_CASE(4, 3, 2) TDGL_PLANE1B_CASE(4, 2, 3) TDGL_PLANE1B_CASE(4, 3, 3)
+          TDGL_PLANE1B_CASE(5, 2, 2) TDGL_PLANE1B_CASE(5, 3, 2) TDGL_PLANE1B_CASE(5, 2, 3) TDGL_PLANE1B_CASE(5, 3, 3)
+#undef TDGL_PLANE1B_CASE
+          default: return hipErrorInvalidValue;
+        }
+      } else {
+        switch (variant) {
+#define TDGL_PLANEB_CASE(tj) \
+  case tj: TDGL_LAUNCH(k_plane_b<tj>, dim3(blocks), dim3(256), lds, s, t); break;
+          TDGL_PLANEB_CASE(1) TDGL_PLANEB_CASE(2) TDGL_PLANEB_CASE(3) TDGL_PLANEB_CASE(4)
+          TDGL_PLANEB_CASE(5) TDGL_PLANEB_CASE(6) TDGL_PLANEB_CASE(7) TDGL_PLANEB_CASE(8)
+#undef TDGL_PLANEB_CASE
+          default: return hipErrorInvalidValue;
+        }
+      }
+      break;
+    default: return launch_batch_meter(kind, t, blocks, s);
+  }
+  return hipGetLastError();
+}
+
 }  // namespace tdgl
+

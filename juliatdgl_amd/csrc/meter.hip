@@ -180,8 +180,7 @@ __device__ void meter_mid(const MeterArgs &a, double *part) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_meter(MeterArgs a) {
-  int b = blockIdx.x;
+__device__ __forceinline__ void meter_body(const MeterArgs &a, int b) {
   if (b < a.nlb) {
     if (!a.init) meter_lines(a, a.part, b);
     return;
@@ -199,9 +198,11 @@ __global__ void __launch_bounds__(256) k_meter(MeterArgs a) {
   if (!a.init) meter_mid(a, a.part + (size_t)(a.nlb + a.nnb + a.nhb) * METER_Q);
 }
 
+__global__ void __launch_bounds__(256) k_meter(MeterArgs a) { meter_body(a, blockIdx.x); }
+
 // The record from the partials, each quantity summed by one thread in block order; the rung's
 // dmax and nbig carried in rung[0..1] (zero at its first cycle).
-__global__ void k_meter_finish(MeterArgs a, double *rec, double *rung) {
+__device__ __forceinline__ void meter_finish_body(const MeterArgs &a, double *rec, double *rung) {
   const int t = threadIdx.x;
   const double *pl = a.part, *pn = a.part + (size_t)a.nlb * METER_Q, *ph = a.part + (size_t)(a.nlb + a.nnb) * METER_Q,
                *pm = a.part + (size_t)(a.nlb + a.nnb + a.nhb) * METER_Q;
@@ -244,15 +245,53 @@ __global__ void k_meter_finish(MeterArgs a, double *rec, double *rung) {
   }
 }
 
+__global__ void k_meter_finish(MeterArgs a, double *rec, double *rung) { meter_finish_body(a, rec, rung); }
+
+// batched: run r owns blocks off[r] .. off[r+1]-1 of k_meter's grid, and block r of the finish
+__global__ void __launch_bounds__(256) k_meter_b(BatchTab t) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = t.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t.off[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  meter_body(((const MeterArgs *)t.args)[lo], b - t.off[lo]);
+}
+__global__ void k_meter_finish_b(BatchTab t) {
+  const MeterFinArgs &f = ((const MeterFinArgs *)t.args)[blockIdx.x];
+  meter_finish_body(f.a, f.rec, f.rung);
+}
+
 int meter_blocks(const MeterArgs &a) { return a.nlb + a.nnb + a.nhb + 1; }
 
 hipError_t launch_meter(const MeterArgs &a, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   TDGL_LAUNCH(k_meter, dim3(meter_blocks(a)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_meter_finish(const MeterArgs &a, double *rec, double *rung, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   TDGL_LAUNCH(k_meter_finish, dim3(1), dim3(64), 0, s, a, rec, rung);
   return hipGetLastError();
 }
 
+dim3 meter_batch_grid(int kind, const void *args) {
+  if (kind == BK_METER) return dim3(meter_blocks(*(const MeterArgs *)args));
+  return dim3(1);
+}
+hipError_t launch_batch_meter(int kind, const BatchTab &t, int blocks, hipStream_t s) {
+  if (kind == BK_METER) {
+    TDGL_LAUNCH(k_meter_b, dim3(blocks), dim3(256), 0, s, t);
+  } else if (kind == BK_METERFIN) {
+    TDGL_LAUNCH(k_meter_finish_b, dim3(blocks), dim3(64), 0, s, t);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace tdgl
+

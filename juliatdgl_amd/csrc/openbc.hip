@@ -22,6 +22,7 @@ __global__ void k_open_clock(int *ofl, int every) {
   ofl[OF_COUNT] = re ? 0 : c;
 }
 hipError_t launch_open_clock(int *ofl, int every, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   TDGL_LAUNCH(k_open_clock, dim3(1), dim3(64), 0, s, ofl, every);
   return hipGetLastError();
 }
@@ -58,6 +59,7 @@ __global__ void __launch_bounds__(256) k_open_dj(Geo g, OpenDJ a) {
   a.dj[c][t] = v;
 }
 hipError_t launch_open_dj(const Geo &g, const OpenDJ &a, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   int m = 0;
   for (int c = 0; c < 3; ++c) m = std::max(m, a.n[c][0] * a.n[c][1] * a.n[c][2]);
   TDGL_LAUNCH(k_open_dj, dim3(cdiv(m, 256), 3), dim3(256), 0, s, g, a);
@@ -211,6 +213,7 @@ __global__ void __launch_bounds__(256) k_fft(FftPass p, const int *gate) {
 int fft_lpb(int M) { return std::max(1, std::min(64, FFT_LDS / M)); }
 
 hipError_t launch_fft(FftPass p, const int *gate, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   if (p.ax.M < 1 || p.ax.M > FFT_LDS || p.nin < 1 || p.nin > p.ax.M || p.nout < 1 || p.nout > p.ax.M)
     return hipErrorInvalidValue;
   p.lpb = fft_lpb(p.ax.M);
@@ -269,6 +272,7 @@ __global__ void __launch_bounds__(256) k_open_face(Geo g, OpenFace a) {
     a.hnew[1][q] = ((th(0, i, j, k + 1) - th(0, i, j, k)) - th(2, i + 1, j, k) + th(2, i, j, k)) / (a.dz * a.dx);
 }
 hipError_t launch_open_face(const Geo &g, const OpenFace &a, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
   TDGL_LAUNCH(k_open_face, dim3(cdiv(g.Nx + 1, 64), g.Ny + 1, g.Nz + 1), dim3(64), 0, s, g, a);
   return hipGetLastError();
 }

@@ -1,6 +1,7 @@
 // stepper.hip — the C ABI of include/tdglhip.h: device state, the order of one step
 // (gtdgl/src/step.jl:13-35) and of the band update (bands.jl:330-521), the separable solve
 // and the device-resident PCG (implicit.jl:62-104, 279-446).
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -126,6 +127,7 @@ struct tdgl_handle {
   double *mrec = nullptr, *mrung = nullptr;
   bool meter = false;
   bool plane32 = false, plane64 = false;  // the fused transform launches fit (per precision)
+  bool tx_only = false;                   // the separable solve as single-axis passes (a mixed batch)
   int plane1 = 1;                         // PlanePass::one of the Float32 launches
   // normal fluids: the sources / solved modes (region-shaped, one per mode), work, the bases
   double *qS[TDGL_MAX_BANDS] = {};
@@ -267,14 +269,15 @@ void cache_give(const Block &b) {
     }
   }
 }
-// a device block of at least `bytes` on the handle's device
-hipError_t dblock(tdgl_handle *h, size_t bytes, Block *out) {
+// a device block of at least `bytes` on device `dev` (the current device)
+hipError_t dblock_on(int dev, size_t bytes, Block *out) {
   const size_t cap = round_cap(bytes);
-  if (cache_take(h->dev, cap, out)) return hipSuccess;
+  if (cache_take(dev, cap, out)) return hipSuccess;
   out->cap = cap;
-  out->dev = h->dev;
+  out->dev = dev;
   return hipMalloc(&out->p, cap);
 }
+hipError_t dblock(tdgl_handle *h, size_t bytes, Block *out) { return dblock_on(h->dev, bytes, out); }
 // a pinned host block of at least `bytes`
 hipError_t hblock(size_t bytes, Block *out) {
   const size_t cap = round_cap(bytes);
@@ -713,7 +716,7 @@ int sep_solve(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int *gate, 
   void *Ta = f32 ? (void *)h->T32a : (void *)h->T1;
   void *Tb = f32 ? (void *)h->T32b : (void *)h->T2;
   const size_t es = f32 ? 4 : 8;
-  if (f32 ? h->plane32 : h->plane64) {
+  if (!h->tx_only && (f32 ? h->plane32 : h->plane64)) {
     PlanePass f = plane_pass(h, false, f32);
     for (int c = 0; c < 3; ++c) {
       f.in[c] = node_view(h, in.c[c]);
@@ -834,7 +837,7 @@ int sep_solve(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int *gate, 
 
 // the partials of r·z written by the last launch of a preconditioner solve
 int sep_last_parts(tdgl_handle *h, bool f32) {
-  if (f32 ? h->plane32 : h->plane64) return f32 ? plane_parts<float>(plane_pass(h, true, true)) : plane_parts<double>(plane_pass(h, true, false));
+  if (!h->tx_only && (f32 ? h->plane32 : h->plane64)) return f32 ? plane_parts<float>(plane_pass(h, true, true)) : plane_parts<double>(plane_pass(h, true, false));
   return tx_parts(tx_pass(h, 2, true, false));
 }
 
@@ -2144,6 +2147,24 @@ MeterArgs meter_args(tdgl_handle *h, int first) {
   a.init = 0;
   return a;
 }
+// the record of a cycle from the device's MR_N values
+void to_record(const double *r, tdgl_cycle_record *out) {
+  *out = tdgl_cycle_record{};
+  out->V = r[MR_V];
+  out->TA = r[MR_TA];
+  out->Tf1 = r[MR_TF1];
+  for (int mu = 0; mu < TDGL_MAX_BANDS; ++mu) {
+    out->T[mu] = r[MR_T + mu];
+    out->Tabs[mu] = r[MR_TABS + mu];
+    out->dif[mu] = r[MR_DIF + mu];
+  }
+  out->dmax = r[MR_DMAX];
+  out->nbig = (int32_t)r[MR_NBIG];
+  out->It = r[MR_IT];
+  out->Is = r[MR_IS];
+  out->amax = r[MR_AMAX];
+  out->diffh = r[MR_DIFFH];
+}
 template <class T>
 int malloc_meter(tdgl_handle *h, T **p, size_t n) {
   Block b{};
@@ -2229,21 +2250,7 @@ int tdgl_meter_cycle(tdgl_handle *h, int32_t first, tdgl_cycle_record *out) {
   HIPCHK(hipMemcpyAsync(r, h->mrec, sizeof r, hipMemcpyDeviceToHost, h->s));
   HIPCHK(hipStreamSynchronize(h->s));
   if (h->prof) CHK(prof_flush(h));
-  *out = tdgl_cycle_record{};
-  out->V = r[MR_V];
-  out->TA = r[MR_TA];
-  out->Tf1 = r[MR_TF1];
-  for (int mu = 0; mu < TDGL_MAX_BANDS; ++mu) {
-    out->T[mu] = r[MR_T + mu];
-    out->Tabs[mu] = r[MR_TABS + mu];
-    out->dif[mu] = r[MR_DIF + mu];
-  }
-  out->dmax = r[MR_DMAX];
-  out->nbig = (int32_t)r[MR_NBIG];
-  out->It = r[MR_IT];
-  out->Is = r[MR_IS];
-  out->amax = r[MR_AMAX];
-  out->diffh = r[MR_DIFFH];
+  to_record(r, out);
   return 0;
 }
 
@@ -2284,5 +2291,315 @@ int tdgl_profile_get(tdgl_handle *h, tdgl_profile *p) {
 }
 
 const char *tdgl_kernel_name(int32_t k) { return (k >= 0 && k < K_COUNT) ? K_NAMES[k] : ""; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------- batched runs
+// one batched launch: its kind, blocks, runs and where its tables sit in the batch's block
+struct LaunchPlan {
+  int kind, variant, blocks, n;
+  size_t lds;
+  size_t off, dims, g, args;
+};
+// The steps of a call are recorded once per run (the launches its own step would make, with the
+// single-axis passes of the separable solve), checked to be the same sequence of kernels in
+// every run, laid out as one table per launch (block offsets, each run's grid, Geo and argument
+// struct) and captured as a graph of one batched launch per position; the graph is replayed
+// while the steps per call and every run's parity stay the same.
+struct tdgl_batch {
+  std::vector<tdgl_handle *> hs;
+  int dev = 0;
+  hipStream_t s = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int gn = -1;                 // steps of the captured graph
+  std::vector<int> gpar;       // the parity of each run it was captured from
+  std::vector<HostSnap> gend;  // each run's pointers after it
+  Block tab{};                 // device tables of the graph's launches
+  std::vector<LaunchPlan> plans;  // the launches of gn steps (replayed eagerly under TDGL_HIP_GRAPH=0)
+  bool graphs = true;
+  bool tx_only = false;           // the runs differ in their transform passes: single-axis ones in all
+  Block mtab{}, mstage{};      // device tables of the meter's launches, their pinned staging
+  Block dH{}, hH{};            // (H, HJ) of every run: device, pinned host
+  Block drec{}, hrec{};        // the meter's records: device, pinned host
+};
+
+namespace {
+
+struct TabBuild {
+  std::vector<unsigned char> bytes;
+  size_t put(const void *p, size_t n) {
+    const size_t o = (bytes.size() + 255) / 256 * 256;
+    bytes.resize(o + n);
+    std::memcpy(&bytes[o], p, n);
+    return o;
+  }
+};
+
+// the table of one batched launch of `kind` (instance `variant`, dynamic LDS `lds`): run r with
+// Geo gs[r], argument struct args[r] and the grid grids[r] of its own launch
+int plan_launch(int kind, int variant, size_t lds, const std::vector<Geo> &gs, const std::vector<const void *> &args,
+                const std::vector<dim3> &grids, TabBuild &tb, LaunchPlan &lp) {
+  const int n = (int)gs.size();
+  const size_t es = batch_args_size(kind);
+  if (es == 0) return fail(TDGL_E_ARG, "batch: unknown launch kind %d", kind);
+  std::vector<int> off(n + 1, 0), dims(3 * n, 1);
+  std::vector<unsigned char> ab((size_t)n * es);
+  for (int r = 0; r < n; ++r) {
+    const dim3 gd = grids[r];
+    const long long nb = (long long)gd.x * gd.y * gd.z;
+    dims[3 * r] = gd.x > 0 ? (int)gd.x : 1;
+    dims[3 * r + 1] = gd.y > 0 ? (int)gd.y : 1;
+    dims[3 * r + 2] = gd.z > 0 ? (int)gd.z : 1;
+    if ((long long)off[r] + nb > 0x7fffffffLL) return fail(TDGL_E_ARG, "batch: too many blocks in one launch");
+    off[r + 1] = off[r] + (int)nb;
+    std::memcpy(&ab[(size_t)r * es], args[r], es);
+  }
+  lp.kind = kind;
+  lp.variant = variant;
+  lp.lds = lds;
+  lp.n = n;
+  lp.blocks = off[n];
+  lp.off = tb.put(off.data(), off.size() * sizeof(int));
+  lp.dims = tb.put(dims.data(), dims.size() * sizeof(int));
+  lp.g = tb.put(gs.data(), gs.size() * sizeof(Geo));
+  lp.args = tb.put(ab.data(), ab.size());
+  return 0;
+}
+BatchTab tab_of(const Block &dev, const LaunchPlan &lp) {
+  const unsigned char *base = (const unsigned char *)dev.p;
+  return BatchTab{(const int *)(base + lp.off), (const int *)(base + lp.dims), (const Geo *)(base + lp.g),
+                  (const void *)(base + lp.args), lp.n};
+}
+// the tables into a device block of the batch (grown as needed) through pinned staging
+int upload_tabs(tdgl_batch *b, const TabBuild &tb, Block &dst, Block &stage) {
+  const size_t n = tb.bytes.size() ? tb.bytes.size() : 1;
+  if (!dst.p || dst.cap < n) {
+    if (dst.p) cache_give(dst);
+    dst = Block{};
+    HIPCHK(dblock_on(b->dev, n, &dst));
+  }
+  if (!stage.p || stage.cap < n) {
+    if (stage.p) cache_give(stage);
+    stage = Block{};
+    HIPCHK(hblock(n, &stage));
+  }
+  std::memcpy(stage.p, tb.bytes.data(), tb.bytes.size());
+  HIPCHK(hipMemcpyAsync(dst.p, stage.p, tb.bytes.size(), hipMemcpyHostToDevice, b->s));
+  HIPCHK(hipStreamSynchronize(b->s));
+  return 0;
+}
+
+void batch_drop_graph(tdgl_batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  b->gn = -1;
+}
+
+// the graph of m steps of every run, recorded and captured unless the one there fits
+int batch_graph(tdgl_batch *b, int m) {
+  const int R = (int)b->hs.size();
+  std::vector<int> par(R);
+  for (int r = 0; r < R; ++r) par[r] = b->hs[r]->parity;
+  if (b->gn == m && b->gpar == par && (b->gexec || !b->graphs)) return 0;
+  batch_drop_graph(b);
+  std::vector<Recorder> rec(R);
+  b->gend.assign(R, HostSnap{});
+  for (int r = 0; r < R; ++r) {
+    tdgl_handle *h = b->hs[r];
+    const HostSnap start = snap(h);
+    h->tx_only = b->tx_only;
+    g_rec = &rec[r];
+    const int rc = steps_eager(h, m);
+    g_rec = nullptr;
+    h->tx_only = false;
+    b->gend[r] = snap(h);
+    restore(h, start);
+    if (rc) return fail(rc, "batch: run %d cannot be batched: %s", r, g_err.c_str());
+  }
+  const size_t P = rec[0].recs.size();
+  for (int r = 1; r < R; ++r) {
+    if (rec[r].recs.size() != P) return fail(TDGL_E_ARG, "batch: run %d makes %zu launches a call, run 0 %zu", r, rec[r].recs.size(), P);
+    for (size_t q = 0; q < P; ++q)
+      if (rec[r].recs[q].kind != rec[0].recs[q].kind) return fail(TDGL_E_ARG, "batch: run %d differs from run 0 at launch %zu", r, q);
+  }
+  // one launch per position and kernel instance (the plane and z-mode passes have several)
+  TabBuild tb;
+  std::vector<LaunchPlan> plans;
+  std::vector<CurlArgs> curl(R);
+  for (size_t q = 0; q < P; ++q) {
+    const int kind = rec[0].recs[q].kind;
+    std::vector<int> variants;
+    for (int r = 0; r < R; ++r) {
+      const BatchRec &br = rec[r].recs[q];
+      if (br.args.size() != batch_args_size(kind)) return fail(TDGL_E_ARG, "batch: launch %zu has the wrong argument size", q);
+      if (std::find(variants.begin(), variants.end(), br.variant) == variants.end()) variants.push_back(br.variant);
+    }
+    for (int v : variants) {
+      std::vector<Geo> gs;
+      std::vector<const void *> args;
+      std::vector<dim3> grids;
+      size_t lds = 0;
+      for (int r = 0; r < R; ++r) {
+        const BatchRec &br = rec[r].recs[q];
+        if (br.variant != v) continue;
+        const void *a = br.args.data();
+        if (kind == BK_CURL) {
+          // the field boundary reads the run's (H, HJ) from the batch's array
+          std::memcpy(&curl[r], br.args.data(), sizeof(CurlArgs));
+          curl[r].hp = (double *)b->dH.p + 2 * r;
+          a = &curl[r];
+        }
+        gs.push_back(br.g);
+        args.push_back(a);
+        grids.push_back(br.has_grid ? br.grid : batch_grid(kind, br.g, a));
+        lds = std::max(lds, br.lds);
+      }
+      LaunchPlan lp{};
+      CHK(plan_launch(kind, v, lds, gs, args, grids, tb, lp));
+      plans.push_back(lp);
+    }
+  }
+  Block stage{};
+  const int rc = upload_tabs(b, tb, b->tab, stage);
+  if (stage.p) cache_give(stage);
+  if (rc) return rc;
+  b->plans = plans;
+  if (!b->graphs) {
+    b->gn = m;
+    b->gpar = par;
+    return 0;
+  }
+  HIPCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeRelaxed));
+  hipError_t le = hipSuccess;
+  for (size_t q = 0; q < plans.size() && le == hipSuccess; ++q)
+    le = launch_batch(plans[q].kind, plans[q].variant, plans[q].lds, tab_of(b->tab, plans[q]), plans[q].blocks, b->s);
+  hipGraph_t gph = nullptr;
+  const hipError_t ee = hipStreamEndCapture(b->s, &gph);
+  if (le != hipSuccess || ee != hipSuccess) {
+    if (gph) (void)hipGraphDestroy(gph);
+    return fail(TDGL_E_DEVICE, "batch: capture: %s", hipGetErrorString(le != hipSuccess ? le : ee));
+  }
+  b->graph = gph;
+  HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  b->gn = m;
+  b->gpar = par;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdgl_batch_create(tdgl_handle *const *hs, int32_t n, tdgl_batch **out) {
+  if (!hs || !out || n < 1 || n > TDGL_BATCH_MAX) return fail(TDGL_E_ARG, "batch: 1..%d handles", TDGL_BATCH_MAX);
+  *out = nullptr;
+  for (int r = 0; r < n; ++r) {
+    const tdgl_handle *h = hs[r];
+    if (!h) return fail(TDGL_E_ARG, "batch: null handle %d", r);
+    for (int q = 0; q < r; ++q)
+      if (hs[q] == h) return fail(TDGL_E_ARG, "batch: handle %d given twice", r);
+    if (h->dev != hs[0]->dev) return fail(TDGL_E_ARG, "batch: the handles are on different devices");
+    const tdgl_desc &d = h->d;
+    if (d.field_mode != TDGL_FIELD_SEPARABLE) return fail(TDGL_E_UNSUPPORTED, "batch: run %d: only the separable field is batched", r);
+    if (d.selffield_bc == TDGL_BC_OPEN) return fail(TDGL_E_UNSUPPORTED, "batch: run %d: the open boundary is not batched", r);
+    if (d.qp_mode != TDGL_QP_NONE) return fail(TDGL_E_UNSUPPORTED, "batch: run %d: normal-fluid exchange is not batched", r);
+    if (h->noise) return fail(TDGL_E_UNSUPPORTED, "batch: run %d: thermal noise is not batched", r);
+  }
+  tdgl_batch *b = new tdgl_batch();
+  b->hs.assign(hs, hs + n);
+  b->dev = hs[0]->dev;
+  const char *ge = std::getenv("TDGL_HIP_GRAPH");  // =0: the batched launches from the host (profiling)
+  b->graphs = !(ge && ge[0] == '0');
+  for (int r = 1; r < n; ++r) b->tx_only |= hs[r]->plane64 != hs[0]->plane64;
+  auto bail = [&](int rc) {
+    tdgl_batch_destroy(b);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess) return bail(fail(TDGL_E_DEVICE, "batch: stream"));
+  if (dblock_on(b->dev, 2 * sizeof(double) * n, &b->dH) != hipSuccess || hblock(2 * sizeof(double) * n, &b->hH) != hipSuccess ||
+      dblock_on(b->dev, MR_N * sizeof(double) * n, &b->drec) != hipSuccess || hblock(MR_N * sizeof(double) * n, &b->hrec) != hipSuccess)
+    return bail(fail(TDGL_E_NOMEM, "batch: allocation"));
+  *out = b;
+  return 0;
+}
+
+int tdgl_batch_step(tdgl_batch *b, int32_t n, const double *H, const double *HJ) {
+  if (!b) return fail(TDGL_E_ARG, "null batch");
+  if (n < 0) return fail(TDGL_E_ARG, "negative step count");
+  if (n == 0) return 0;
+  if (!H || !HJ) return fail(TDGL_E_ARG, "batch: null H or HJ");
+  const int R = (int)b->hs.size();
+  double *hh = (double *)b->hH.p;
+  for (int r = 0; r < R; ++r) {
+    if (!std::isfinite(H[r]) || !std::isfinite(HJ[r])) return fail(TDGL_E_ARG, "batch: non-finite H or HJ of run %d", r);
+    if (b->hs[r]->prof) return fail(TDGL_E_ARG, "batch: run %d has its profile on", r);
+    hh[2 * r] = H[r];
+    hh[2 * r + 1] = HJ[r];
+  }
+  HIPCHK(hipMemcpyAsync(b->dH.p, hh, 2 * sizeof(double) * R, hipMemcpyHostToDevice, b->s));
+  for (int left = n; left > 0;) {
+    const int m = std::min(left, GRAPH_MAX_STEPS);
+    CHK(batch_graph(b, m));
+    if (b->graphs) {
+      HIPCHK(hipGraphLaunch(b->gexec, b->s));
+    } else {
+      for (const LaunchPlan &lp : b->plans) HIPCHK(launch_batch(lp.kind, lp.variant, lp.lds, tab_of(b->tab, lp), lp.blocks, b->s));
+    }
+    for (int r = 0; r < R; ++r) restore(b->hs[r], b->gend[r]);
+    left -= m;
+  }
+  HIPCHK(hipStreamSynchronize(b->s));
+  for (tdgl_handle *h : b->hs) {
+    h->stepped = true;
+    h->steps += n;
+    h->calls += 1;
+  }
+  return 0;
+}
+
+int tdgl_batch_meter_cycle(tdgl_batch *b, int32_t first, tdgl_cycle_record *out) {
+  if (!b || !out) return fail(TDGL_E_ARG, "null batch or records");
+  const int R = (int)b->hs.size();
+  std::vector<Geo> gs(R);
+  std::vector<MeterArgs> ma(R);
+  std::vector<MeterFinArgs> mf(R);
+  std::vector<const void *> pa(R), pf(R);
+  for (int r = 0; r < R; ++r) {
+    tdgl_handle *h = b->hs[r];
+    if (!h->meter) return fail(TDGL_E_ARG, "batch: run %d: tdgl_meter_begin has not been called", r);
+    ma[r] = meter_args(h, first);
+    mf[r] = MeterFinArgs{ma[r], (double *)b->drec.p + (size_t)MR_N * r, h->mrung};
+    gs[r] = h->g;
+    pa[r] = &ma[r];
+    pf[r] = &mf[r];
+  }
+  std::vector<dim3> gm(R), gf(R, dim3(1));
+  for (int r = 0; r < R; ++r) gm[r] = meter_batch_grid(BK_METER, &ma[r]);
+  TabBuild tb;
+  LaunchPlan lm{}, lf{};
+  CHK(plan_launch(BK_METER, 0, 0, gs, pa, gm, tb, lm));
+  CHK(plan_launch(BK_METERFIN, 0, 0, gs, pf, gf, tb, lf));
+  CHK(upload_tabs(b, tb, b->mtab, b->mstage));
+  HIPCHK(launch_batch(BK_METER, 0, 0, tab_of(b->mtab, lm), lm.blocks, b->s));
+  HIPCHK(launch_batch(BK_METERFIN, 0, 0, tab_of(b->mtab, lf), lf.blocks, b->s));
+  HIPCHK(hipMemcpyAsync(b->hrec.p, b->drec.p, MR_N * sizeof(double) * R, hipMemcpyDeviceToHost, b->s));
+  HIPCHK(hipStreamSynchronize(b->s));
+  for (int r = 0; r < R; ++r) to_record((const double *)b->hrec.p + (size_t)MR_N * r, &out[r]);
+  return 0;
+}
+
+int tdgl_batch_destroy(tdgl_batch *b) {
+  if (!b) return 0;
+  if (b->s) (void)hipStreamSynchronize(b->s);
+  batch_drop_graph(b);
+  for (Block *x : {&b->tab, &b->mtab, &b->mstage, &b->dH, &b->hH, &b->drec, &b->hrec})
+    if (x->p) cache_give(*x);
+  if (b->s) (void)hipStreamDestroy(b->s);
+  delete b;
+  return 0;
+}
 
 }  // extern "C"

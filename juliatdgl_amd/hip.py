@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "_lib", "libtdglhip.so")
 SYMBOLS = ("tdgl_supported", "tdgl_create", "tdgl_destroy", "tdgl_upload_state", "tdgl_download_state", "tdgl_step",
            "tdgl_get_pcg", "tdgl_set_pcg", "tdgl_get_open", "tdgl_set_open", "tdgl_get_stats", "tdgl_last_error", "tdgl_device_count",
            "tdgl_set_device", "tdgl_profile_enable", "tdgl_profile_get", "tdgl_kernel_name", "tdgl_meter_begin",
-           "tdgl_meter_cycle", "tdgl_noise_get", "tdgl_noise_set", "tdgl_noise_draws")
+           "tdgl_meter_cycle", "tdgl_noise_get", "tdgl_noise_set", "tdgl_noise_draws", "tdgl_batch_create",
+           "tdgl_batch_step", "tdgl_batch_meter_cycle", "tdgl_batch_destroy")
 
 _lib = None
 _lock = threading.Lock()
@@ -60,6 +61,10 @@ def load():
             L.tdgl_noise_get.argtypes = [vp, ct.POINTER(ct.c_uint64)]
             L.tdgl_noise_set.argtypes = [vp, ct.c_uint64]
             L.tdgl_noise_draws.argtypes = [vp, ct.c_int32, vp]
+            L.tdgl_batch_create.argtypes = [vp, ct.c_int32, ct.POINTER(vp)]
+            L.tdgl_batch_step.argtypes = [vp, ct.c_int32, vp, vp]
+            L.tdgl_batch_meter_cycle.argtypes = [vp, ct.c_int32, vp]
+            L.tdgl_batch_destroy.argtypes = [vp]
             _lib = L
         return _lib
 
@@ -80,6 +85,13 @@ def supported_desc(st: Stepper):
     buf = ct.create_string_buffer(256)
     ok = load().tdgl_supported(ct.byref(pd.c), buf, len(buf))
     return bool(ok), buf.value.decode()
+
+
+def _record(r, N: int) -> dict:
+    return {"V": r.V, "TA": r.TA, "Tf1": r.Tf1, "T": [r.T[k] for k in range(N)],
+            "Tabs": [r.Tabs[k] for k in range(N)], "dmax": r.dmax,
+            "nbig": int(r.nbig), "It": r.It, "Is": r.Is, "dif": [r.dif[k] for k in range(N)], "amax": r.amax,
+            "diffh": r.diffh}
 
 
 class HipStepper:
@@ -170,11 +182,7 @@ class HipStepper:
         """The record of the cycle that just ended (tdgl_meter_cycle); `first` starts a rung."""
         r = CycleRecordC()
         _check(load().tdgl_meter_cycle(self.h, int(first), ct.byref(r)), "tdgl_meter_cycle")
-        N = self.st.N
-        return {"V": r.V, "TA": r.TA, "Tf1": r.Tf1, "T": [r.T[k] for k in range(N)],
-                "Tabs": [r.Tabs[k] for k in range(N)], "dmax": r.dmax,
-                "nbig": int(r.nbig), "It": r.It, "Is": r.Is, "dif": [r.dif[k] for k in range(N)], "amax": r.amax,
-                "diffh": r.diffh}
+        return _record(r, self.st.N)
 
     @property
     def noise_counter(self) -> int:
@@ -229,3 +237,42 @@ class HipStepper:
         c = StatsC()
         _check(load().tdgl_get_stats(self.h, ct.byref(c)), "tdgl_get_stats")
         return {k: getattr(c, k) for k, _ in StatsC._fields_}
+
+
+class HipBatch:
+    """Many device steppers advanced together (tdgl_batch_*): one launch per kernel of a step for
+    every run, the steps of a call replayed as one graph.  The steppers keep their state and
+    meter; they are not stepped by themselves while the batch is open."""
+
+    def __init__(self, devs):
+        self.devs = list(devs)
+        n = len(self.devs)
+        arr = (ct.c_void_p * n)(*[d.h for d in self.devs])
+        b = ct.c_void_p()
+        _check(load().tdgl_batch_create(arr, n, ct.byref(b)), "tdgl_batch_create")
+        self.b = b
+
+    def step(self, n: int, H, HJ) -> None:
+        """n steps of every run, run r at (H[r], HJ[r])."""
+        H = np.ascontiguousarray(H, dtype=np.float64)
+        HJ = np.ascontiguousarray(HJ, dtype=np.float64)
+        if H.shape != (len(self.devs),) or HJ.shape != (len(self.devs),):
+            raise ValueError("batch step: one H and one HJ per run")
+        _check(load().tdgl_batch_step(self.b, int(n), H.ctypes.data, HJ.ctypes.data), "tdgl_batch_step")
+
+    def meter_cycle(self, first: bool):
+        """The records of the cycle that just ended, one per run (tdgl_batch_meter_cycle)."""
+        recs = (CycleRecordC * len(self.devs))()
+        _check(load().tdgl_batch_meter_cycle(self.b, int(first), recs), "tdgl_batch_meter_cycle")
+        return [_record(recs[r], d.st.N) for r, d in enumerate(self.devs)]
+
+    def close(self) -> None:
+        if getattr(self, "b", None):
+            load().tdgl_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -215,6 +215,51 @@ def run_batch(specs: List[RunSpec], work: Callable, workers: int = 8):
     return results, wall
 
 
+def run_batched(specs: List[RunSpec], work: "Cycles", group: int = 64, workers: int = 8):
+    """Every run of `specs` with batched kernels (tdgl_batch_*): groups of `group` runs advanced
+    together, one launch per kernel of a step over the whole group.  Host threads set up the next
+    group (stepper, state, upload, meter) while the current one runs.  Returns {run_id: the last
+    cycle's record} and the wall time."""
+    from concurrent.futures import ThreadPoolExecutor
+    from .hip import HipBatch, HipStepper
+    from .meter import Voltmeter
+
+    if not isinstance(work, Cycles):
+        raise ValueError("run_batched: the batched runs take a fixed number of cycles (Cycles)")
+
+    def prep(spec):
+        st = stepper_of(spec)
+        dev = HipStepper(st)
+        dev.upload(state_of(spec, st))
+        dev.meter_begin(Voltmeter.of_ladder(st, work.delta_v))
+        return spec, st, dev
+
+    groups = [specs[q:q + max(1, group)] for q in range(0, len(specs), max(1, group))]
+    results = {}
+    with _one_blas_thread(), ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        t0 = time.perf_counter()
+        pending = [ex.submit(prep, sp) for sp in groups[0]] if groups else []
+        for k in range(len(groups)):
+            ready = [f.result() for f in pending]
+            pending = [ex.submit(prep, sp) for sp in groups[k + 1]] if k + 1 < len(groups) else []
+            batch = HipBatch([dev for _, _, dev in ready])
+            try:
+                H = np.array([sp.H for sp, _, _ in ready])
+                HJ = np.array([st.selffield(sp.J) for sp, st, _ in ready])
+                recs = None
+                for m in range(1, work.cycles + 1):
+                    batch.step(work.lmax, H, HJ)
+                    recs = batch.meter_cycle(m == 1)
+                for (sp, _, _), rec in zip(ready, recs or [None] * len(ready)):
+                    results[sp.run_id] = rec
+            finally:
+                batch.close()
+                for _, _, dev in ready:
+                    dev.close()
+        wall = time.perf_counter() - t0
+    return results, wall
+
+
 @dataclass(order=True)
 class _Req:
     t_arrival: float

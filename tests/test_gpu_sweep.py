@@ -54,3 +54,56 @@ def test_noisy_runs_end_at_the_same_first_slip_served():
     print("    " + ", ".join(f"{q}: {alone[q][0]}" for q in sorted(alone)), flush=True)
     for q in alone:
         assert done[q]["result"] == alone[q]
+
+
+def test_batched_kernels_match_the_runs_alone():
+    """tdgl_batch_*: six uneven runs advanced by one launch per kernel over all of them give the
+    records of their own steppers to the bit (the batch launches the instances each run's own step
+    takes), in a batch of one as in the batch of six."""
+    from juliatdgl_amd.sweep import run_batched
+    specs = _specs()
+    print("[gpu] batched kernels", flush=True)
+    alone, _ = run_batch(specs, Cycles(cycles=3), workers=1)
+    mix, t6 = run_batched(specs, Cycles(cycles=3), group=6)
+    one, _ = run_batched(specs, Cycles(cycles=3), group=1)
+    print(f"    six in one batch {t6:.2f} s", flush=True)
+    for q in alone:
+        assert mix[q] == one[q], q
+        assert mix[q] == alone[q], q
+
+
+def test_batched_kernels_match_the_oracle():
+    """Two uneven runs stepped 40 times in one batch against the oracle's step on the same seeded
+    states: ψ and the links within 1e-10."""
+    import numpy as np
+    from oracle.oracle import Oracle
+    from juliatdgl_amd.hip import HipBatch, HipStepper
+    from juliatdgl_amd.sweep import state_of
+    specs = _specs()[:2]
+    print("[gpu] batched kernels against the oracle", flush=True)
+    sts = [stepper_of(s) for s in specs]
+    devs = []
+    for s, st in zip(specs, sts):
+        d = HipStepper(st)
+        d.upload(state_of(s, st))
+        devs.append(d)
+    b = HipBatch(devs)
+    H = np.array([s.H for s in specs])
+    HJ = np.array([st.selffield(s.J) for s, st in zip(specs, sts)])
+    b.step(40, H, HJ)
+    for s, st, d, hj in zip(specs, sts, devs, HJ):
+        o = Oracle(st)
+        o.set_state(state_of(s, st))
+        o.step(40, s.H, hj)
+        ref, got = o.get_state(), d.download()
+        worst = 0.0
+        for x, y in zip(ref.G, got.G):
+            worst = max(worst, float(np.max(np.abs(x - y)) / max(1.0, np.max(np.abs(x)))))
+        for k in ("phx", "phy", "phz", "hx", "hy", "hz"):
+            x, y = getattr(ref, k), getattr(got, k)
+            worst = max(worst, float(np.max(np.abs(x - y)) / max(1.0, np.max(np.abs(x)))))
+        print(f"    run {s.run_id}: worst {worst:.3e}", flush=True)
+        assert worst <= 1e-10
+    b.close()
+    for d in devs:
+        d.close()
