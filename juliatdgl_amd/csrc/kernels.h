@@ -404,11 +404,13 @@ struct PcgArgs {
   int flexible;
   int warm, extrap, have2;   // start
   int nparts_in;     // partials the kernel consumes
+  int kc;            // levels of k per block of the box kernels (box_kc)
   const double *th[3];       // finish: φ = θ + x
   double *ph[3];
   int act_slot;      // finish: the flag that tells an unfinished solve
 };
-int pcg_box_blocks(const Geo &g);
+int pcg_box_blocks(const Geo &g, int kc);
+int box_kc(const Geo &g);
 hipError_t launch_pcg_start(const PcgArgs &a, hipStream_t s);    // nb, warm start or zero source
 hipError_t launch_pcg_resid(const PcgArgs &a, hipStream_t s);    // Q = A x, r = b − Q, r·r
 hipError_t launch_pcg_p(const PcgArgs &a, hipStream_t s);        // β, P = z + βP, zold = z

@@ -2169,12 +2169,15 @@ hipError_t launch_qp_zcol(const QpZ &a, hipStream_t s) {
 
 // ============================================================================ PCG
 // Kernels over the box i, j, k in 2..N (the union of the three ranges) with 1D blocks of 64 x 4
-// nodes; vectors node-shaped, zero outside their ranges (the direction P is read at neighbours
-// with no test: implicit.jl:300, 406).
+// nodes and kc levels of k each; vectors node-shaped, zero outside their ranges (the direction P
+// is read at neighbours with no test: implicit.jl:300, 406).  Every consumer of a reduction sums
+// all the producer's block partials itself, so the block count enters squared: on large grids a
+// block takes several levels (box_kc), a thread's levels in increasing k.
 struct BoxIdx {
   int i, j, k, c, bid;
+  int nk;  // levels of the block: k .. k + nk - 1
 };
-__device__ __forceinline__ BoxIdx box_idx(const Geo &g) {
+__device__ __forceinline__ BoxIdx box_idx(const Geo &g, int kc) {
   const int gx = cdiv_d(g.Nx - 1, 64), gy = cdiv_d(g.Ny - 1, 4);
   const int b = blockIdx.x;
   BoxIdx r;
@@ -2182,11 +2185,27 @@ __device__ __forceinline__ BoxIdx box_idx(const Geo &g) {
   const int bx = b % gx, by = (b / gx) % gy, bz = b / (gx * gy);
   r.i = 2 + bx * 64 + (int)(threadIdx.x & 63);
   r.j = 2 + by * 4 + (int)(threadIdx.x >> 6);
-  r.k = 2 + bz;
+  r.k = 2 + bz * kc;
+  r.nk = min(kc, g.Nz - 1 - bz * kc);
   r.c = g.at(r.i, r.j, r.k);
   return r;
 }
-int pcg_box_blocks(const Geo &g) { return cdiv(g.Nx - 1, 64) * cdiv(g.Ny - 1, 4) * (g.Nz - 1); }
+// level q of a block (its k + q)
+__device__ __forceinline__ BoxIdx box_level(const Geo &g, BoxIdx b, int q) {
+  b.k += q;
+  b.c += q * g.px * g.py;
+  return b;
+}
+int pcg_box_blocks(const Geo &g, int kc) { return cdiv(g.Nx - 1, 64) * cdiv(g.Ny - 1, 4) * cdiv(g.Nz - 1, kc); }
+// levels per block: one while the box has at most BOX_PARTS blocks of one level, else the least
+// power of two that brings it there
+constexpr int BOX_PARTS = 2048;
+int box_kc(const Geo &g) {
+  const int nxy = cdiv(g.Nx - 1, 64) * cdiv(g.Ny - 1, 4), nz = g.Nz - 1;
+  int kc = 1;
+  while (kc < nz && nxy * cdiv(nz, kc) > BOX_PARTS) kc *= 2;
+  return kc;
+}
 
 // Q = (D + aK) P of the three components at box node (i, j, k) (_apply_A!, implicit.jl:216-234):
 // h = curl P on the plaquettes of its ranges (zero elsewhere), then D p + s (curl h).  The 24
@@ -2264,34 +2283,43 @@ __device__ __forceinline__ int box_load_c(const Geo &g, const BoxIdx &b) { retur
 __global__ void __launch_bounds__(RED_T) k_pcg_start(PcgArgs a) {
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const BoxIdx b = box_idx(a.g);
-  const bool live = box_live(a.g, b);
-  const int cl = box_load_c(a.g, b);
+  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const bool live = box_live(a.g, b0);
+  // the first level's values are requested before the partials are summed
+  const int cl = box_load_c(a.g, b0);
   double xv[3], pv[3];
   for (int comp = 0; comp < 3; ++comp) {
     xv[comp] = a.x.c[comp][cl];
     pv[comp] = a.xprev.c[comp][cl];
   }
   const double nb = sqrt(sum_parts(region(a.d, PART_NB), a.nparts_in, 0, sh, tid));
-  if (b.bid == 0 && tid == 0) {
+  if (b0.bid == 0 && tid == 0) {
     a.d.sc[SC_NB] = nb;
     a.d.fl[FL_NZ] = nb > 0 ? 1 : 0;
     a.d.fl[FL_IT] = 0;
   }
   if (!live) return;
-  for (int comp = 0; comp < 3; ++comp) {
-    if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
-    double *x = a.x.c[comp], *xp = a.xprev.c[comp];
-    const int c = b.c;
-    if (nb == 0) {
-      if (a.warm && a.extrap) xp[c] = xv[comp];
-      x[c] = 0.0;
-    } else if (a.warm && a.extrap) {
-      if (a.have2) {
-        x[c] = 2 * xv[comp] - pv[comp];
-        xp[c] = xv[comp];
-      } else {
-        xp[c] = xv[comp];
+  for (int q = 0; q < b0.nk; ++q) {
+    const BoxIdx b = box_level(a.g, b0, q);
+    if (q > 0)
+      for (int comp = 0; comp < 3; ++comp) {
+        xv[comp] = a.x.c[comp][b.c];
+        pv[comp] = a.xprev.c[comp][b.c];
+      }
+    for (int comp = 0; comp < 3; ++comp) {
+      if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+      double *x = a.x.c[comp], *xp = a.xprev.c[comp];
+      const int c = b.c;
+      if (nb == 0) {
+        if (a.warm && a.extrap) xp[c] = xv[comp];
+        x[c] = 0.0;
+      } else if (a.warm && a.extrap) {
+        if (a.have2) {
+          x[c] = 2 * xv[comp] - pv[comp];
+          xp[c] = xv[comp];
+        } else {
+          xp[c] = xv[comp];
+        }
       }
     }
   }
@@ -2302,25 +2330,27 @@ __global__ void __launch_bounds__(RED_T) k_pcg_resid(PcgArgs a) {
   if (!a.d.fl[FL_NZ]) return;
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const BoxIdx b = box_idx(a.g);
+  const BoxIdx b0 = box_idx(a.g, a.kc);
   double s = 0.0;
-  if (box_live(a.g, b)) {
-    double q[3], pc[3], bv[3];
-    for (int comp = 0; comp < 3; ++comp) bv[comp] = a.b.c[comp][b.c];
-    apply3(a, a.x.c[0], a.x.c[1], a.x.c[2], b.i, b.j, b.k, b.c, q, pc);
-    for (int comp = 0; comp < 3; ++comp) {
-      if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
-      a.Q.c[comp][b.c] = q[comp];
-      const double rv = bv[comp] - q[comp];
-      a.r.c[comp][b.c] = rv;
-      s += rv * rv;
+  if (box_live(a.g, b0))
+    for (int q = 0; q < b0.nk; ++q) {
+      const BoxIdx b = box_level(a.g, b0, q);
+      double qv[3], pc[3], bv[3];
+      for (int comp = 0; comp < 3; ++comp) bv[comp] = a.b.c[comp][b.c];
+      apply3(a, a.x.c[0], a.x.c[1], a.x.c[2], b.i, b.j, b.k, b.c, qv, pc);
+      for (int comp = 0; comp < 3; ++comp) {
+        if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+        a.Q.c[comp][b.c] = qv[comp];
+        const double rv = bv[comp] - qv[comp];
+        a.r.c[comp][b.c] = rv;
+        s += rv * rv;
+      }
     }
-  }
   const double r0 = block_sum_t(s, sh, tid);
   if (tid == 0) {
     double *pr = region_w(a.d, PART_RR);
-    pr[2 * b.bid] = r0;
-    pr[2 * b.bid + 1] = 0.0;
+    pr[2 * b0.bid] = r0;
+    pr[2 * b0.bid + 1] = 0.0;
   }
 }
 
@@ -2330,14 +2360,8 @@ __global__ void __launch_bounds__(RED_T) k_pcg_p(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const BoxIdx b = box_idx(a.g);
-  const bool live = box_live(a.g, b);
-  const int cl = box_load_c(a.g, b);
-  double zv[3], pv[3] = {};
-  for (int comp = 0; comp < 3; ++comp) {
-    zv[comp] = a.z.c[comp][cl];
-    if (a.q > 0) pv[comp] = a.Pold.c[comp][cl];
-  }
+  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const bool live = box_live(a.g, b0);
   double rz2, rzo;
   sum_parts2(region(a.d, PART_RZ), a.nparts_in, rz2, rzo, sh, tid);
   double beta = 0.0;
@@ -2345,13 +2369,21 @@ __global__ void __launch_bounds__(RED_T) k_pcg_p(PcgArgs a) {
     const double rz = a.d.sc[SC_RZ0 + ((a.q + 1) & 1)];
     beta = a.flexible ? (rz2 - rzo) / rz : rz2 / rz;
   }
-  if (b.bid == 0 && tid == 0) a.d.sc[SC_RZ0 + (a.q & 1)] = rz2;
+  if (b0.bid == 0 && tid == 0) a.d.sc[SC_RZ0 + (a.q & 1)] = rz2;
   if (!live) return;
-  for (int comp = 0; comp < 3; ++comp) {
-    if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
-    const int c = b.c;
-    a.Pnew.c[comp][c] = a.q == 0 ? zv[comp] : zv[comp] + beta * pv[comp];
-    if (a.flexible) a.zold.c[comp][c] = zv[comp];
+  for (int q = 0; q < b0.nk; ++q) {
+    const BoxIdx b = box_level(a.g, b0, q);
+    double zv[3], pv[3] = {};
+    for (int comp = 0; comp < 3; ++comp) {
+      zv[comp] = a.z.c[comp][b.c];
+      if (a.q > 0) pv[comp] = a.Pold.c[comp][b.c];
+    }
+    for (int comp = 0; comp < 3; ++comp) {
+      if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+      const int c = b.c;
+      a.Pnew.c[comp][c] = a.q == 0 ? zv[comp] : zv[comp] + beta * pv[comp];
+      if (a.flexible) a.zold.c[comp][c] = zv[comp];
+    }
   }
 }
 
@@ -2360,22 +2392,24 @@ __global__ void __launch_bounds__(RED_T) k_pcg_ap(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const BoxIdx b = box_idx(a.g);
+  const BoxIdx b0 = box_idx(a.g, a.kc);
   double s = 0.0;
-  if (box_live(a.g, b)) {
-    double q[3], pc[3];
-    apply3(a, a.Pnew.c[0], a.Pnew.c[1], a.Pnew.c[2], b.i, b.j, b.k, b.c, q, pc);
-    for (int comp = 0; comp < 3; ++comp) {
-      if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
-      a.Q.c[comp][b.c] = q[comp];
-      s += pc[comp] * q[comp];
+  if (box_live(a.g, b0))
+    for (int q = 0; q < b0.nk; ++q) {
+      const BoxIdx b = box_level(a.g, b0, q);
+      double qv[3], pc[3];
+      apply3(a, a.Pnew.c[0], a.Pnew.c[1], a.Pnew.c[2], b.i, b.j, b.k, b.c, qv, pc);
+      for (int comp = 0; comp < 3; ++comp) {
+        if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+        a.Q.c[comp][b.c] = qv[comp];
+        s += pc[comp] * qv[comp];
+      }
     }
-  }
   const double r0 = block_sum_t(s, sh, tid);
   if (tid == 0) {
     double *pr = region_w(a.d, PART_PQ);
-    pr[2 * b.bid] = r0;
-    pr[2 * b.bid + 1] = 0.0;
+    pr[2 * b0.bid] = r0;
+    pr[2 * b0.bid + 1] = 0.0;
   }
 }
 
@@ -2387,13 +2421,13 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const BoxIdx b = box_idx(a.g);
-  const bool live = box_live(a.g, b);
+  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const bool live = box_live(a.g, b0);
   const int sy = a.g.px, sz = a.g.px * a.g.py;
-  const int cl = box_load_c(a.g, b);
-  // the first partials of r·z and r·zold are requested before the stencil's loads (two per
-  // thread: the registers of the 48 stencil values in flight leave room for two), then summed
-  // exactly as sum_parts2 sums them
+  const int cl = box_load_c(a.g, b0);
+  // the first partials of r·z and r·zold are requested before the first level's stencil loads
+  // (two per thread: the registers of the 48 stencil values in flight leave room for two), then
+  // summed exactly as sum_parts2 sums them
   const int np = a.nparts_in;
   const __amdgpu_buffer_rsrc_t rp = buf_rsrc(region(a.d, PART_RZ), 16 * np);
   double2 pv[2];
@@ -2410,10 +2444,10 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap(PcgArgs a) {
     rz2 += in ? pv[u].x : 0.0;
     rzo += in ? pv[u].y : 0.0;
   }
-  for (int b0 = 2 * RED_T; b0 < np; b0 += 2 * RED_T)
+  for (int b1 = 2 * RED_T; b1 < np; b1 += 2 * RED_T)
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      const int e = b0 + tid + u * RED_T;
+      const int e = b1 + tid + u * RED_T;
       const double2 v = buf_load_d2(rp, 16 * min(e, np - 1));
       rz2 += e < np ? v.x : 0.0;
       rzo += e < np ? v.y : 0.0;
@@ -2424,27 +2458,34 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap(PcgArgs a) {
     const double rz = a.d.sc[SC_RZ0 + ((a.q + 1) & 1)];
     beta = a.flexible ? (rz2 - rzo) / rz : rz2 / rz;
   }
-  if (b.bid == 0 && tid == 0) a.d.sc[SC_RZ0 + (a.q & 1)] = rz2;
+  if (b0.bid == 0 && tid == 0) a.d.sc[SC_RZ0 + (a.q & 1)] = rz2;
   double s = 0.0;
-  if (live) {
-    double v[NSTEN], q[3];
-    for (int n = 0; n < NSTEN; ++n) v[n] = a.q == 0 ? vz[n] : vz[n] + beta * vp[n];
-    apply3v(a, v, dv, b.i, b.j, b.k, q);
-    const int cc[3] = {P0C, P1C, P2C};
-    for (int comp = 0; comp < 3; ++comp) {
-      if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
-      const double pv = v[cc[comp]];
-      a.Pnew.c[comp][b.c] = pv;
-      if (a.flexible) a.zold.c[comp][b.c] = vz[cc[comp]];
-      a.Q.c[comp][b.c] = q[comp];
-      s += pv * q[comp];
+  if (live)
+    for (int q = 0; q < b0.nk; ++q) {
+      const BoxIdx b = box_level(a.g, b0, q);
+      if (q > 0) {
+        gather24b(a.z.c[0], a.z.c[1], a.z.c[2], b.c, sy, sz, a.g.n, vz);
+        if (a.q > 0) gather24b(a.Pold.c[0], a.Pold.c[1], a.Pold.c[2], b.c, sy, sz, a.g.n, vp);
+        for (int comp = 0; comp < 3; ++comp) dv[comp] = a.D[comp][b.c];
+      }
+      double v[NSTEN], qv[3];
+      for (int n = 0; n < NSTEN; ++n) v[n] = a.q == 0 ? vz[n] : vz[n] + beta * vp[n];
+      apply3v(a, v, dv, b.i, b.j, b.k, qv);
+      const int cc[3] = {P0C, P1C, P2C};
+      for (int comp = 0; comp < 3; ++comp) {
+        if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+        const double pn = v[cc[comp]];
+        a.Pnew.c[comp][b.c] = pn;
+        if (a.flexible) a.zold.c[comp][b.c] = vz[cc[comp]];
+        a.Q.c[comp][b.c] = qv[comp];
+        s += pn * qv[comp];
+      }
     }
-  }
   const double r0 = block_sum_t(s, sh, tid);
   if (tid == 0) {
     double *pr = region_w(a.d, PART_PQ);
-    pr[2 * b.bid] = r0;
-    pr[2 * b.bid + 1] = 0.0;
+    pr[2 * b0.bid] = r0;
+    pr[2 * b0.bid + 1] = 0.0;
   }
 }
 
@@ -2453,9 +2494,9 @@ __global__ void __launch_bounds__(RED_T) k_pcg_xr(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const BoxIdx b = box_idx(a.g);
-  const bool live = box_live(a.g, b);
-  const int cl = box_load_c(a.g, b);
+  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const bool live = box_live(a.g, b0);
+  const int cl = box_load_c(a.g, b0);
   double xv[3], pv[3], rv[3], qv[3];
   for (int comp = 0; comp < 3; ++comp) {
     xv[comp] = a.x.c[comp][cl];
@@ -2465,32 +2506,45 @@ __global__ void __launch_bounds__(RED_T) k_pcg_xr(PcgArgs a) {
   }
   const double pAp = sum_parts(region(a.d, PART_PQ), a.nparts_in, 0, sh, tid);
   const double al = a.d.sc[SC_RZ0 + (a.q & 1)] / pAp;
-  if (b.bid == 0 && tid == 0) a.d.fl[FL_IT] += 1;
+  if (b0.bid == 0 && tid == 0) a.d.fl[FL_IT] += 1;
   double s = 0.0;
   if (live)
-    for (int comp = 0; comp < 3; ++comp) {
-      if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
-      const int c = b.c;
-      a.x.c[comp][c] = xv[comp] + al * pv[comp];
-      const double r1 = rv[comp] - al * qv[comp];
-      a.r.c[comp][c] = r1;
-      s += r1 * r1;
+    for (int q = 0; q < b0.nk; ++q) {
+      const BoxIdx b = box_level(a.g, b0, q);
+      if (q > 0)
+        for (int comp = 0; comp < 3; ++comp) {
+          xv[comp] = a.x.c[comp][b.c];
+          pv[comp] = a.Pnew.c[comp][b.c];
+          rv[comp] = a.r.c[comp][b.c];
+          qv[comp] = a.Q.c[comp][b.c];
+        }
+      for (int comp = 0; comp < 3; ++comp) {
+        if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+        const int c = b.c;
+        a.x.c[comp][c] = xv[comp] + al * pv[comp];
+        const double r1 = rv[comp] - al * qv[comp];
+        a.r.c[comp][c] = r1;
+        s += r1 * r1;
+      }
     }
   const double r0 = block_sum_t(s, sh, tid);
   if (tid == 0) {
     double *pr = region_w(a.d, PART_RR);
-    pr[2 * b.bid] = r0;
-    pr[2 * b.bid + 1] = 0.0;
+    pr[2 * b0.bid] = r0;
+    pr[2 * b0.bid + 1] = 0.0;
   }
 }
 
 // φ = θ + x on the ranges (implicit.jl:440-444) and the cycle's counts of the solve
 __global__ void __launch_bounds__(RED_T) k_pcg_finish(PcgArgs a) {
-  const BoxIdx b = box_idx(a.g);
-  if (box_live(a.g, b))
-    for (int comp = 0; comp < 3; ++comp)
-      if (a.rg.in(comp, b.i, b.j, b.k)) a.ph[comp][b.c] = a.th[comp][b.c] + a.x.c[comp][b.c];
-  if (b.bid == 0 && threadIdx.x == 0) {
+  const BoxIdx b0 = box_idx(a.g, a.kc);
+  if (box_live(a.g, b0))
+    for (int q = 0; q < b0.nk; ++q) {
+      const BoxIdx b = box_level(a.g, b0, q);
+      for (int comp = 0; comp < 3; ++comp)
+        if (a.rg.in(comp, b.i, b.j, b.k)) a.ph[comp][b.c] = a.th[comp][b.c] + a.x.c[comp][b.c];
+    }
+  if (b0.bid == 0 && threadIdx.x == 0) {
     int *fl = a.d.fl;
     if (fl[FL_NZ] && fl[FL_ACT0 + (a.act_slot & 1)]) fl[FL_OVF] = 1;
     fl[FL_ITSUM] += fl[FL_IT];
@@ -2500,7 +2554,8 @@ __global__ void __launch_bounds__(RED_T) k_pcg_finish(PcgArgs a) {
 
 static hipError_t launch_box(void (*k)(PcgArgs), const PcgArgs &a, hipStream_t s) {
   if (g_rec) return hipErrorNotSupported;
-  const int nb = pcg_box_blocks(a.g);
+  if (a.kc < 1 || a.kc > a.g.Nz - 1) return hipErrorInvalidValue;
+  const int nb = pcg_box_blocks(a.g, a.kc);
   if (nb > MAX_PART) return hipErrorInvalidValue;
   TDGL_LAUNCH(k, dim3(nb), dim3(RED_T), 0, s, a);
   return hipGetLastError();

@@ -128,6 +128,7 @@ struct tdgl_handle {
   bool meter = false;
   bool plane32 = false, plane64 = false;  // the fused transform launches fit (per precision)
   bool tx_only = false;                   // the separable solve as single-axis passes (a mixed batch)
+  int box_kc = 1;                         // levels of k per block of the PCG box kernels
   int plane1 = 1;                         // PlanePass::one of the Float32 launches
   // normal fluids: the sources / solved modes (region-shaped, one per mode), work, the bases
   double *qS[TDGL_MAX_BANDS] = {};
@@ -848,6 +849,7 @@ PcgArgs pcg_args(tdgl_handle *h) {
   a.g = h->g;
   a.rg = h->rg;
   a.d = h->pd;
+  a.kc = h->box_kc;
   for (int c = 0; c < 3; ++c) {
     a.D[c] = h->D[c];
     a.th[c] = h->th[c];
@@ -883,9 +885,9 @@ int pcg_iteration(tdgl_handle *h, int q, bool flexible) {
   a.Pold = (q % 2 == 0) ? h->P2 : h->P;
   a.nparts_in = sep_last_parts(h, flexible);
   PL(K_APPLYA, launch_pcg_pap(a, h->s));
-  a.nparts_in = pcg_box_blocks(h->g);
+  a.nparts_in = pcg_box_blocks(h->g, h->box_kc);
   PL(K_VEC, launch_pcg_xr(a, h->s));
-  Conv cv{CONV_RR, h->pd.fl + FL_ACT0 + (q & 1), h->pd.fl + FL_ACT0 + ((q + 1) & 1), pcg_box_blocks(h->g)};
+  Conv cv{CONV_RR, h->pd.fl + FL_ACT0 + (q & 1), h->pd.fl + FL_ACT0 + ((q + 1) & 1), pcg_box_blocks(h->g, h->box_kc)};
   SepOut o{&h->z, false, flexible ? 2 : 1};
   CHK(sep_solve(h, h->r, o, nullptr, flexible, cv));
   return 0;
@@ -915,7 +917,7 @@ int implicit_pcg(tdgl_handle *h) {
   h->warm = true;
   PL(K_APPLYA, launch_pcg_resid(a, h->s));
   {
-    Conv cv{CONV_RR0, nz, h->pd.fl + FL_ACT0, pcg_box_blocks(h->g)};
+    Conv cv{CONV_RR0, nz, h->pd.fl + FL_ACT0, pcg_box_blocks(h->g, h->box_kc)};
     SepOut o{&h->z, false, 1};
     CHK(sep_solve(h, h->r, o, nullptr, flexible, cv));
   }
@@ -1689,7 +1691,11 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
       ALLOC(h->pd.sc, SC_COUNT);
       ALLOC(h->pd.fl, FL_COUNT);
       ALLOC(h->pd.part, (size_t)PART_N * 2 * MAX_PART);
-      if (pcg_box_blocks(g) > MAX_PART || field_blocks(g) > MAX_PART) return fail(TDGL_E_UNSUPPORTED, "grid too large for the PCG reductions");
+      // TDGL_HIP_BOX_KC=k: k levels per block of the box kernels instead of box_kc's choice
+      h->box_kc = box_kc(g);
+      const char *bk = std::getenv("TDGL_HIP_BOX_KC");
+      if (bk && std::atoi(bk) >= 1) h->box_kc = std::min(std::atoi(bk), d.Nz - 1);
+      if (pcg_box_blocks(g, h->box_kc) > MAX_PART || field_blocks(g) > MAX_PART) return fail(TDGL_E_UNSUPPORTED, "grid too large for the PCG reductions");
       std::vector<double> sc(SC_COUNT, 0.0);
       sc[SC_TOL] = d.pcg_tol;
       CHK(upload_vec(h, h->pd.sc, sc));

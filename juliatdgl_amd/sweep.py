@@ -357,3 +357,28 @@ def poisson_arrivals(n: int, rate: float, seed: int = 7) -> List[float]:
 def percentiles(latencies) -> dict:
     a = np.sort(np.asarray(list(latencies), dtype=float))
     return {"p50": float(np.percentile(a, 50)), "p99": float(np.percentile(a, 99)), "mean": float(a.mean())}
+
+
+def main(argv=None) -> int:
+    """python -m juliatdgl_amd.sweep [--mode streams|kernels] [--cycles K] [--workers W]: the
+    64-run current sweep (sweep_specs) on the device, one line of runs/hour."""
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", choices=("streams", "kernels"), default="streams",
+                    help="streams: concurrent handles (run_batch); kernels: one launch per kernel over all runs")
+    ap.add_argument("--cycles", type=int, default=10)
+    ap.add_argument("--workers", type=int, default=8)
+    args = ap.parse_args(argv)
+    specs = sweep_specs()
+    work = Cycles(cycles=args.cycles)
+    if args.mode == "streams":
+        res, wall = run_batch(specs, work, workers=args.workers)
+    else:
+        res, wall = run_batched(specs, work, group=len(specs), workers=args.workers)
+    print(f"{len(res)} runs of {args.cycles} cycles ({args.mode}): {wall:.3f} s, {3600.0 * len(res) / wall:.0f} runs/hour",
+          flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -141,6 +141,35 @@ def test_hip_single_axis_transforms_match_oracle(name, monkeypatch):
     assert max(core.values()) <= tol, errs
 
 
+@pytest.mark.parametrize("kc", ["2", "4"])
+@pytest.mark.parametrize("name", ["pcg_fast_f32", "pcg_exact", "kwt10_pcg"])
+def test_hip_pcg_box_levels_per_block_match_oracle(name, kc, monkeypatch):
+    """The PCG box kernels with several levels of k per block (what large grids take, so that the
+    block partials each reduction sums stay few) against the oracle, in eager steps and in graphs."""
+    from oracle.oracle import Oracle
+    HipStepper = _hip()
+    monkeypatch.setenv("TDGL_HIP_BOX_KC", kc)
+    st, tol = make_case(name)
+    s0, pcg, H, HJ = seed_state(st)
+    o = Oracle(st)
+    o.set_state(s0)
+    o.set_pcg(pcg)
+    o.step(12, H, HJ)
+    ref = o.get_state()
+    print(f"[gpu] PCG box kernels, {name}, {kc} levels a block", flush=True)
+    dev = HipStepper(st)
+    dev.upload(s0)
+    dev.set_pcg(pcg)
+    for _ in range(3):
+        dev.step(4, H, HJ)
+    errs = compare_states(dev.download(), ref)
+    core = {k: v for k, v in errs.items() if not k.startswith(("U", "V"))}
+    print(f"    worst {max(core, key=core.get)} {max(core.values()):.3e}", flush=True)
+    assert max(core.values()) <= tol, errs
+    gi, oi = dev.get_pcg().iters, o.get_pcg().iters
+    assert abs(gi - oi) <= max(2, oi // 4), (gi, oi)
+
+
 @pytest.mark.parametrize("one", ["2", "3", "0"])
 @pytest.mark.parametrize("name", ["pcg_wide_f32", "sep_wide", "pcg_exact_wide"])
 def test_hip_one_round_plane_kernel_matches_oracle(name, one, monkeypatch):
