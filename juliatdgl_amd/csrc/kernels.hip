@@ -1866,6 +1866,8 @@ static Plane1Pick plane1_pick(const PlanePass &p, int es) {
   const int n2 = cdiv(l0, 32) * k.gy * k.nc, n3 = cdiv(l0, 48) * k.gy * k.nc;
   k.mt = p.one >= 2 ? p.one : ((n2 > 256 && n3 < n2) ? 3 : 2);
   if (plane1_lds(k.xc, k.tj, k.mt, es) > PLANE1_LDS_MAX && p.one < 2) k.mt = 2;
+  // FP64 planes of up to 160 x 96: one m tile a block where two do not fit the LDS
+  if (plane1_lds(k.xc, k.tj, k.mt, es) > PLANE1_LDS_MAX && p.one < 2 && es == 8) k.mt = 1;
   if (plane1_lds(k.xc, k.tj, k.mt, es) > PLANE1_LDS_MAX) k.use = 0;
   k.gx = cdiv(l0, 16 * k.mt);
   return k;
@@ -1920,6 +1922,7 @@ hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
         TDGL_PLANE1_CASE(3, 2, 2) TDGL_PLANE1_CASE(3, 3, 2) TDGL_PLANE1_CASE(3, 2, 3) TDGL_PLANE1_CASE(3, 3, 3)
         TDGL_PLANE1_CASE(4, 2, 2) TDGL_PLANE1_CASE(4, 3, 2) TDGL_PLANE1_CASE(4, 2, 3) TDGL_PLANE1_CASE(4, 3, 3)
         TDGL_PLANE1_CASE(5, 2, 2) TDGL_PLANE1_CASE(5, 3, 2) TDGL_PLANE1_CASE(5, 2, 3) TDGL_PLANE1_CASE(5, 3, 3)
+        TDGL_PLANE1_CASE(5, 3, 1)
 #undef TDGL_PLANE1_CASE
         default: return hipErrorInvalidValue;
       }
@@ -2645,6 +2648,7 @@ hipError_t launch_batch(int kind, int variant, size_t lds, const BatchTab &t, in
           TDGL_PLANE1B_CASE(3, 2, 2) TDGL_PLANE1B_CASE(3, 3, 2) TDGL_PLANE1B_CASE(3, 2, 3) TDGL_PLANE1B_CASE(3, 3, 3)
           TDGL_PLANE1B_CASE(4, 2, 2) TDGL_PLANE1B_CASE(4, 3, 2) TDGL_PLANE1B_CASE(4, 2, 3) TDGL_PLANE1B_CASE(4, 3, 3)
           TDGL_PLANE1B_CASE(5, 2, 2) TDGL_PLANE1B_CASE(5, 3, 2) TDGL_PLANE1B_CASE(5, 2, 3) TDGL_PLANE1B_CASE(5, 3, 3)
+          TDGL_PLANE1B_CASE(5, 3, 1)
 #undef TDGL_PLANE1B_CASE
           default: return hipErrorInvalidValue;
         }

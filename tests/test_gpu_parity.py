@@ -170,6 +170,27 @@ def test_hip_pcg_box_levels_per_block_match_oracle(name, kc, monkeypatch):
     assert abs(gi - oi) <= max(2, oi // 4), (gi, oi)
 
 
+def test_hip_wide_fp64_planes_match_oracle():
+    """FP64 planes of 141 x 81 (the separable solve), where the one-round x-y kernel holds one m tile
+    a block, against the oracle."""
+    from oracle.oracle import Oracle
+    HipStepper = _hip()
+    st, tol = make_case("sep_wide160")
+    s0, pcg, H, HJ = seed_state(st)
+    o = Oracle(st)
+    o.set_state(s0)
+    o.step(8, H, HJ)
+    ref = o.get_state()
+    print("[gpu] FP64 planes 141 x 81", flush=True)
+    dev = HipStepper(st)
+    dev.upload(s0)
+    dev.step(8, H, HJ)
+    errs = compare_states(dev.download(), ref)
+    core = {k: v for k, v in errs.items() if not k.startswith(("U", "V"))}
+    print(f"    worst {max(core, key=core.get)} {max(core.values()):.3e}", flush=True)
+    assert max(core.values()) <= tol, errs
+
+
 @pytest.mark.parametrize("one", ["2", "3", "0"])
 @pytest.mark.parametrize("name", ["pcg_wide_f32", "sep_wide", "pcg_exact_wide"])
 def test_hip_one_round_plane_kernel_matches_oracle(name, one, monkeypatch):
