@@ -1859,7 +1859,8 @@ static Plane1Pick plane1_pick(const PlanePass &p, int es) {
   }
   bool tr = true;
   for (int c = 0; c < k.nc; ++c) tr = tr && p.XT[c] && p.YT[c];
-  k.use = p.one != 0 && tr && l0 > 64 && l0 <= 32 * PLANE1_XC && l1 > 32 && l1 <= 32 * PLANE1_TJ;
+  // FP64 planes from 33 rows (the FP32 ones from 65: below, the chunked kernel was faster)
+  k.use = p.one != 0 && tr && l0 > (es == 8 ? 32 : 64) && l0 <= 32 * PLANE1_XC && l1 > 32 && l1 <= 32 * PLANE1_TJ;
   if (!k.use) return k;
   k.xc = cdiv(l0, 32);
   k.tj = cdiv(l1, 32);
@@ -1919,6 +1920,7 @@ hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
     TDGL_LAUNCH((k_plane_mf1<TI, T, TO, x, t, m>), g1, dim3(PLANE1_T), (PlaneMf1<x, t, m, T>::lds), s, p, gate, \
                 k.gx, k.gy);                                                                                    \
     break;
+        TDGL_PLANE1_CASE(2, 2, 2) TDGL_PLANE1_CASE(2, 3, 2) TDGL_PLANE1_CASE(2, 2, 3) TDGL_PLANE1_CASE(2, 3, 3)
         TDGL_PLANE1_CASE(3, 2, 2) TDGL_PLANE1_CASE(3, 3, 2) TDGL_PLANE1_CASE(3, 2, 3) TDGL_PLANE1_CASE(3, 3, 3)
         TDGL_PLANE1_CASE(4, 2, 2) TDGL_PLANE1_CASE(4, 3, 2) TDGL_PLANE1_CASE(4, 2, 3) TDGL_PLANE1_CASE(4, 3, 3)
         TDGL_PLANE1_CASE(5, 2, 2) TDGL_PLANE1_CASE(5, 3, 2) TDGL_PLANE1_CASE(5, 2, 3) TDGL_PLANE1_CASE(5, 3, 3)
@@ -2645,6 +2647,7 @@ hipError_t launch_batch(int kind, int variant, size_t lds, const BatchTab &t, in
   case (x * 4 + tt) * 4 + m:                                                                        \
     TDGL_LAUNCH((k_plane_mf1_b<x, tt, m>), dim3(blocks), dim3(PLANE1_T), lds, s, t);               \
     break;
+          TDGL_PLANE1B_CASE(2, 2, 2) TDGL_PLANE1B_CASE(2, 3, 2) TDGL_PLANE1B_CASE(2, 2, 3) TDGL_PLANE1B_CASE(2, 3, 3)
           TDGL_PLANE1B_CASE(3, 2, 2) TDGL_PLANE1B_CASE(3, 3, 2) TDGL_PLANE1B_CASE(3, 2, 3) TDGL_PLANE1B_CASE(3, 3, 3)
           TDGL_PLANE1B_CASE(4, 2, 2) TDGL_PLANE1B_CASE(4, 3, 2) TDGL_PLANE1B_CASE(4, 2, 3) TDGL_PLANE1B_CASE(4, 3, 3)
           TDGL_PLANE1B_CASE(5, 2, 2) TDGL_PLANE1B_CASE(5, 3, 2) TDGL_PLANE1B_CASE(5, 2, 3) TDGL_PLANE1B_CASE(5, 3, 3)

@@ -76,6 +76,8 @@ CASES = {
     "sep_wide": (mgb2, dict(nx=68, ny=34, nz=4), {}, 1e-10),
     # FP64 planes of 141 x 81: the one-round kernel with one m tile a block (two do not fit the LDS)
     "sep_wide160": (mgb2, dict(nx=140, ny=80, nz=4), {}, 1e-10),
+    # FP64 planes of 49 x 41: the one-round kernel with 64-row planes
+    "sep_mid": (mgb2, dict(nx=48, ny=40, nz=4), {}, 1e-10),
     "pcg_exact_wide": (mgb2, dict(nx=68, ny=34, nz=4, buf=2), dict(sigma_vac=0.01, pcg_fast=False, pcg_f32=False,
                                                                   pcg_tol=1e-12), 1e-10),
     "open": (mgb2, {}, dict(selffield_bc="open", open_every=0.02), 1e-10),

@@ -170,18 +170,19 @@ def test_hip_pcg_box_levels_per_block_match_oracle(name, kc, monkeypatch):
     assert abs(gi - oi) <= max(2, oi // 4), (gi, oi)
 
 
-def test_hip_wide_fp64_planes_match_oracle():
-    """FP64 planes of 141 x 81 (the separable solve), where the one-round x-y kernel holds one m tile
-    a block, against the oracle."""
+@pytest.mark.parametrize("name", ["sep_wide160", "sep_mid"])
+def test_hip_wide_fp64_planes_match_oracle(name):
+    """FP64 planes of 141 x 81 (the one-round x-y kernel with one m tile a block) and of 49 x 41 (64-row
+    planes) in the separable solve against the oracle."""
     from oracle.oracle import Oracle
     HipStepper = _hip()
-    st, tol = make_case("sep_wide160")
+    st, tol = make_case(name)
     s0, pcg, H, HJ = seed_state(st)
     o = Oracle(st)
     o.set_state(s0)
     o.step(8, H, HJ)
     ref = o.get_state()
-    print("[gpu] FP64 planes 141 x 81", flush=True)
+    print(f"[gpu] FP64 planes, {name}", flush=True)
     dev = HipStepper(st)
     dev.upload(s0)
     dev.step(8, H, HJ)
