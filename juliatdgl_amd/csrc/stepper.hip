@@ -215,7 +215,7 @@ int hfill0(tdgl_handle *h, void *dst, size_t bytes) {
 
 // Device and pinned host blocks are kept by the process once a handle is done with them and
 // handed to the next handle that asks for at most as much (and at least 2/3 of it): a free, of
-// either kind, waits for the whole device and returns the memory to the driver, and mapping it
+// either kind, waits for the whole device and gives the memory back to the system, and mapping it
 // again costs as much, so creating and destroying one run would stall the runs that other host
 // threads drive meanwhile.  A handle waits for its own stream before it gives its blocks back,
 // so a block is idle when another handle takes it.  Above CACHE_MAX idle bytes of one kind, the
