@@ -319,7 +319,8 @@ const char *tdgl_kernel_name(int32_t k);
  * same values to rounding, ~1e-15 a step.)
  * Supported: the separable field (TDGL_FIELD_SEPARABLE), bar or slab boundary, one normal fluid,
  * no thermal noise; the runs must have the same bands and stage structure.  A handle belongs to at
- * most one live batch and is not used by itself while the batch runs (calls are synchronous).
+ * most one live batch, is not used by itself while the batch runs (calls are synchronous) and is
+ * destroyed only after the batch: the batch's graph holds the addresses of its arrays.
  * tdgl_batch_step: n steps of every run, run r at (H[r], HJ[r]).  tdgl_batch_meter_cycle: one
  * tdgl_meter_cycle of every run (each after its own tdgl_meter_begin), out[r] for run r. */
 typedef struct tdgl_batch tdgl_batch;
