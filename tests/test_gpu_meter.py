@@ -136,3 +136,44 @@ def test_golden_ladder_on_the_device():
     HipStepper = _hip()
     print("[gpu] golden ladder", flush=True)
     check_golden_ladder(lambda st: HipStepper(st), report=lambda m: print(m, flush=True))
+
+
+def test_fp32_preconditioner_against_all_fp64():
+    """The golden ladder's sample with an insulating buffer (buf 2, σvac/σn = 0.01), stepped by the
+    insulating PCG to 1e-10 with the FP32 preconditioner and FP64 outer iteration (the default) and
+    with every part in FP64 (the exact path): 400 steps at J = 0.6 leave ψ and the links within
+    1e-8 of each other, and the two-polarity ladders have the same verdicts, cycles and onset
+    brackets."""
+    from juliatdgl_amd.meter import run_ladders
+    from test_ladder import golden_setup
+    HipStepper = _hip()
+    _, _, st0, pr = golden_setup()
+    gr = Grid.make(24, 12, 4, dx=0.5, buf=2)
+    mk = wedge_masks(gr)
+    kw = dict(selffield_bc="slab", rkl=False, sigma_vac=0.01, pcg_tol=1e-10)
+    st32 = make_stepper(st0.model, gr, mk, **kw)
+    st64 = make_stepper(st0.model, gr, mk, pcg_fast=False, pcg_f32=False, **kw)
+    assert st32.field_mode == st64.field_mode == 2 and st32.pcg_f32 and not st64.pcg_f32
+    print("[gpu] FP32 preconditioner against all-FP64", flush=True)
+    out = []
+    for st in (st32, st64):
+        d = HipStepper(st)
+        d.upload(initial_state(st.model, st.grid, st.masks, 0.03))
+        d.step(400, 0.03, st.selffield(0.6))
+        out.append(d.download())
+        d.close()
+    a, b = out
+    dpsi = max(float(np.max(np.abs(x - y))) for x, y in zip(a.G, b.G))
+    dA = max(float(np.max(np.abs(getattr(a, k) - getattr(b, k)))) for k in ("phx", "phy", "phz"))
+    print(f"    400 steps at J = 0.6: |Δψ| {dpsi:.2e}, |ΔA| {dA:.2e}", flush=True)
+    assert dpsi <= 1e-8 and dA <= 1e-8
+    Js = [0.30, 0.60, 0.90, 1.20, 1.50]
+    r32 = run_ladders(lambda st: HipStepper(st), st32, Js, H=0.03, pr=pr)
+    r64 = run_ladders(lambda st: HipStepper(st), st64, Js, H=0.03, pr=pr)
+    for L32, L64 in zip(r32[:2], r64[:2]):
+        print(f"    {''.join(L32.conv)} {list(L32.m)} / {''.join(L64.conv)} {list(L64.m)}", flush=True)
+        assert L32.conv == L64.conv and list(L32.m) == list(L64.m)
+    print("    brackets " + ", ".join(f"{p} [{r32[2][p]['lo']}, {r32[2][p]['hi']}] / [{r64[2][p]['lo']}, {r64[2][p]['hi']}]"
+                                   for p in ("pos", "neg")), flush=True)
+    for pol in ("pos", "neg"):
+        assert (r32[2][pol]["lo"], r32[2][pol]["hi"]) == (r64[2][pol]["lo"], r64[2][pol]["hi"])
