@@ -157,6 +157,8 @@ struct tdgl_handle {
   bool graphs = true, capturing = false;
   int parity = 0;          // swaps of advance! modulo 2
   int budget = 0;          // PCG iterations per solve in the graph (0: not known yet)
+  bool budget_fixed = false;  // set by TDGL_HIP_BUDGET: the budget only grows (roll backs)
+  int shrink_run = 0, shrink_max = 0;  // cycles in a row far under the budget, their longest solve
   long long gkey = -1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1424,6 +1426,8 @@ int destroy_graph(tdgl_handle *h) {
 // replayed after.  With the PCG, the graph holds `budget` gated iterations per solve; a cycle in
 // which a solve was still active after them is rolled back from a device copy of its start and
 // stepped eagerly, and the budget grows.
+// the budget's way down: BUDGET_RUN cycles in a row BUDGET_SLACK iterations under it
+constexpr int BUDGET_SLACK = 6, BUDGET_RUN = 2;
 int cycle_graph(tdgl_handle *h, int n) {
   const bool pcg = h->d.field_mode == TDGL_FIELD_PCG;
   const long long key = ((((long long)n * 2 + h->parity) * 2 + h->warm) * 2 + h->have2) * 1024 + (pcg ? h->budget : 0);
@@ -1468,6 +1472,26 @@ int cycle_graph(tdgl_handle *h, int n) {
       PL(K_VEC, launch_pcg_cycle_reset(h->pd, h->s));
       CHK(steps_eager(h, n));
       HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
+      h->shrink_run = 0;
+      h->shrink_max = 0;
+    } else if (!h->budget_fixed) {
+      // the budget follows the solves down: after BUDGET_RUN cycles in a row whose longest solve
+      // stayed BUDGET_SLACK or more iterations under it, it becomes that longest + 3 (the graph is
+      // captured again at the next call).  The first budget comes from the first cycle, whose
+      // solves start cold and run longest; every gated iteration past a converged solve still
+      // launches its five kernels.
+      const int itmax = h->flh[FL_ITMAX];
+      if (itmax + BUDGET_SLACK <= h->budget) {
+        h->shrink_max = std::max(h->shrink_max, itmax);
+        if (++h->shrink_run >= BUDGET_RUN) {
+          h->budget = std::max(4, h->shrink_max + 3);
+          h->shrink_run = 0;
+          h->shrink_max = 0;
+        }
+      } else {
+        h->shrink_run = 0;
+        h->shrink_max = 0;
+      }
     }
   }
   return 0;
@@ -1799,7 +1823,10 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     const char *ge = std::getenv("TDGL_HIP_GRAPH");
     h->graphs = !(ge && ge[0] == '0');
     const char *gb = std::getenv("TDGL_HIP_BUDGET");
-    if (gb && d.field_mode == TDGL_FIELD_PCG) h->budget = std::max(1, std::min(d.pcg_maxit, std::atoi(gb)));
+    if (gb && d.field_mode == TDGL_FIELD_PCG) {
+      h->budget = std::max(1, std::min(d.pcg_maxit, std::atoi(gb)));
+      h->budget_fixed = true;
+    }
     // TDGL_HIP_PLANE=0: the separable solve as six single-axis passes even where the fused
     // x-y and z-mode launches fit
     // TDGL_HIP_PLANE1=0: the chunked Float32 x-y kernel also where the one-round kernel fits;
