@@ -264,6 +264,32 @@ def test_hip_open_boundary_resumes_and_is_odd_in_the_current():
     assert np.max(np.abs(p.hz + m.hz)) < 1e-13 and np.max(np.abs(p.hz)) > 1e-4
 
 
+def test_hip_pcg_budget_shrinks_and_changes_nothing(monkeypatch):
+    """The PCG budget learned from the cold first cycle shrinks once the solves run shorter, and the
+    graph cycles that follow give the eager steps' state to the bit."""
+    HipStepper = _hip()
+    st, _ = make_case("kwt10_pcg")
+    s0, pcg, H, HJ = seed_state(st)
+    print("[gpu] PCG budget", flush=True)
+    runs = []
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("TDGL_HIP_GRAPH", graphs)
+        dev = HipStepper(st)
+        dev.upload(s0)
+        dev.set_pcg(pcg)
+        budgets = []
+        for _ in range(5):
+            dev.step(20, H, HJ)
+            budgets.append(dev.stats()["pcg_budget"])
+        runs.append(dev.download())
+        if graphs == "1":
+            print(f"    budgets {budgets}, rollbacks {dev.stats()['rollbacks']}", flush=True)
+            assert budgets[-1] < budgets[0]
+        dev.close()
+    for (k, a), (_, b) in zip(runs[0].arrays(), runs[1].arrays()):
+        assert np.array_equal(a, b), k
+
+
 @pytest.mark.parametrize("name", ["production", "kwt10_pcg"])
 def test_hip_long_call_equals_short_calls(name):
     """A call of more steps than one captured graph holds (128) runs as replays of bounded graphs:
