@@ -1427,7 +1427,7 @@ int destroy_graph(tdgl_handle *h) {
 // which a solve was still active after them is rolled back from a device copy of its start and
 // stepped eagerly, and the budget grows.
 // the budget's way down: BUDGET_RUN cycles in a row BUDGET_SLACK iterations under it
-constexpr int BUDGET_SLACK = 6, BUDGET_RUN = 2;
+constexpr int BUDGET_SLACK = 6, BUDGET_RUN = 1;
 int cycle_graph(tdgl_handle *h, int n) {
   const bool pcg = h->d.field_mode == TDGL_FIELD_PCG;
   const long long key = ((((long long)n * 2 + h->parity) * 2 + h->warm) * 2 + h->have2) * 1024 + (pcg ? h->budget : 0);
@@ -1475,7 +1475,7 @@ int cycle_graph(tdgl_handle *h, int n) {
       h->shrink_run = 0;
       h->shrink_max = 0;
     } else if (!h->budget_fixed) {
-      // the budget follows the solves down: after BUDGET_RUN cycles in a row whose longest solve
+      // the budget follows the solves down: after BUDGET_RUN cycles (in a row) whose longest solve
       // stayed BUDGET_SLACK or more iterations under it, it becomes that longest + 3 (the graph is
       // captured again at the next call).  The first budget comes from the first cycle, whose
       // solves start cold and run longest; every gated iteration past a converged solve still
