@@ -191,7 +191,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed ladder cycles")
-    ap.add_argument("--warmup", type=int, default=2, help="untimed ladder cycles")
+    ap.add_argument("--warmup", type=int, default=5,
+                    help="untimed ladder cycles (the PCG budget settles and its graph is captured in them)")
     ap.add_argument("--workload", default="kwt_pcg_128x64x16",
                     choices=("kwt_pcg_128x64x16", "open_3band_256x128x32", "pcg_wedge_96x48x24", "plain_wedge_96x48x24",
                              "sweep64"))
