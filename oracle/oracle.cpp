@@ -1870,6 +1870,38 @@ void orc_kwt_update(const double *g, const double *d, double g2, double *out) {
   out[1] = r.im;
 }
 
+// The field solve of the step alone (implicit_curl! or scale_increment!, step.jl:25-29) on the
+// state as set: φ holds φₑ, θ the links of the step (tests: the solve against an operator built
+// independently, the zero-source rule of pcg_zero_fixture.jl).
+void orc_field_solve(orc *o) {
+  switch (o->d.field_mode) {
+    case TDGL_FIELD_SEPARABLE: implicit_sep(*o); break;
+    case TDGL_FIELD_PCG: implicit_pcg(*o); break;
+    case TDGL_FIELD_EXPLICIT_SCALED: scale_increment(*o); break;
+    default: break;
+  }
+}
+
+// _link_rate! of one link (bands.jl:303-310): phase of V Ū plus (ω₁ − ω₀) dt
+double orc_link_rate(const double *v, const double *u, double om0, double om1, double dt) {
+  A3c V(1, 1, 1), U(1, 1, 1);
+  A3r Rr(2, 1, 1), Om(2, 1, 1);
+  V(1, 1, 1) = C{v[0], v[1]};
+  U(1, 1, 1) = C{u[0], u[1]};
+  Om(1, 1, 1) = om0;
+  Om(2, 1, 1) = om1;
+  orc o0;
+  link_rate(o0, Rr, V, U, Om, dt, Rng{1, 1, 1, 1, 1, 1}, 1, 0, 0);
+  return Rr(1, 1, 1);
+}
+
+// _cis (bands.jl:288-289): the short series below |a| = 0.05
+void orc_scis(double a, double *out) {
+  C r = scis(a);
+  out[0] = r.re;
+  out[1] = r.im;
+}
+
 // One temporal link rotation (tests: rotation_fixture.jl semantics): mode 0 _rotfrac!, 1 _rot! root
 void orc_rotation(int mode, const double *v, const double *u, double c, int n, double *out) {
   A3c L(1, 1, 1), R(1, 1, 1), V(1, 1, 1), U(1, 1, 1);

@@ -44,6 +44,10 @@ def lib():
         L.orc_basis.argtypes = [ct.c_int] + [ct.c_void_p] * 3
         L.orc_kwt_update.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_void_p]
         L.orc_rotation.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int, ct.c_void_p]
+        L.orc_field_solve.argtypes = [ct.c_void_p]
+        L.orc_link_rate.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_double, ct.c_double]
+        L.orc_link_rate.restype = ct.c_double
+        L.orc_scis.argtypes = [ct.c_double, ct.c_void_p]
         L.orc_qp_modes.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p]
         L.orc_qp_source.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p]
         L.orc_qp_potential.argtypes = [ct.c_void_p]
@@ -180,6 +184,12 @@ class Oracle:
         lib().orc_open_fields(self.h, a.ctypes.data, b.ctypes.data)
         return _split(a, sh), _split(b, sh)
 
+    def field_solve(self) -> None:
+        """The field solve of the step alone on the state as set (φ holding φₑ): implicit_curl! or
+        scale_increment! (step.jl:25-29)."""
+        lib().orc_field_solve(self.h)
+        lib().orc_get_state(self.h, ct.byref(state_struct(self._s)))
+
     def sep_solve(self, BX, BY, BZ):
         """(I + aK)^{-1} on range arrays (the stepper must have the separable or PCG field)."""
         out = [np.array(b, dtype=np.float64, order="F", copy=True) for b in (BX, BY, BZ)]
@@ -225,6 +235,20 @@ def kwt_update(g: complex, d: complex, g2: float) -> complex:
     b = np.array([d.real, d.imag])
     o = np.zeros(2)
     lib().orc_kwt_update(a.ctypes.data, b.ctypes.data, g2, o.ctypes.data)
+    return complex(o[0], o[1])
+
+
+def link_rate(v: complex, u: complex, om0: float, om1: float, dt: float) -> float:
+    """_link_rate! of one link: the phase of V Ū plus (ω₁ − ω₀) dt."""
+    a = np.array([v.real, v.imag])
+    b = np.array([u.real, u.imag])
+    return float(lib().orc_link_rate(a.ctypes.data, b.ctypes.data, float(om0), float(om1), float(dt)))
+
+
+def scis(a: float) -> complex:
+    """_cis: the short series of e^{ia} below |a| = 0.05."""
+    o = np.zeros(2)
+    lib().orc_scis(float(a), o.ctypes.data)
     return complex(o[0], o[1])
 
 
