@@ -20,12 +20,17 @@
 // Euler + substeps + extrapolated links, separable and explicit field, slab BC); the closed
 // forms of gtdgl/test/test_physics.jl and TDGLGPU/test/test_invariants.jl (KWT 2x2 solve,
 // amplitude relaxation rates, uniform equilibrium, pure gauge); RKC polynomial identities; the
-// separable solve against a dense inverse of (I + aK); PCG against the separable solve at
-// D = 1; the manufactured rotations of gtdgl/test/rotation_fixture.jl; the noise by the
-// equilibria of gtdgl/test/test_physics.jl:475-535, 697-735 (exp(−F/T̃) of decoupled nodes with and
-// without KWT, the phase and amplitude rates, equipartition of the field's modes) and its normals
-// by the Philox4x32-10 known-answer vectors of Random123.  The tests that do so are
-// tests/test_oracle_*.py.
+// separable solve and the PCG (exact, fast, Float32 preconditioner) against (I + aK) and
+// (D + aK) assembled in the tests from the circulation of the links around the plaquettes (the
+// true residual within the tolerance, and a sparse direct solve); the PCG at D = 1 equal to the
+// separable solve; the zero-source rule and its checkpoint continuation of
+// gtdgl/test/pcg_zero_fixture.jl; the 93 manufactured rotations of gtdgl/test/rotation_fixture.jl
+// with the bounds of test_link_rotations.jl (_rotfrac!, _rot! roots, _link_rate!, _cis); the
+// anisotropic masses by the Euler factor of discrete Neumann plane waves along each axis; the
+// noise by the equilibria of gtdgl/test/test_physics.jl:475-535, 697-735 (exp(−F/T̃) of decoupled
+// nodes with and without KWT, the phase and amplitude rates, equipartition of the field's modes)
+// and its normals by the Philox4x32-10 known-answer vectors of Random123.  The tests that do so
+// are tests/test_oracle_*.py.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
