@@ -235,6 +235,17 @@ typedef struct {
 int tdgl_get_open(tdgl_handle *h, tdgl_open *o);
 int tdgl_set_open(tdgl_handle *h, const tdgl_open *o);
 
+/* The normal-fluid potentials of the last step, what the host reads from the Stepper after a
+ * cycle (TDGLGPU.jl:2136-2140: st.qpx.δ of QPExchange, st.qpx.μ of QPExchangeN): node-shaped
+ * (Nx+1, Ny+1, Nz+1) Float64 arrays, set on the region of the unknowns (x = 2..Nx,
+ * y = qp_j0+1..qp_j0+qp_nj, z = qp_k0+1..qp_k0+qp_nk) and zero elsewhere.  TDGL_QP_TWO: pot[0] = δ =
+ * μ_π − μ_σ (exchange.jl:383-388); TDGL_QP_N: pot[µ] = μ_µ = Σ_k P[µ][k] m_k for every band
+ * (exchange.jl:506-512).  Zero before the first step.  TDGL_E_ARG for one normal fluid. */
+typedef struct {
+  double *pot[TDGL_MAX_BANDS];
+} tdgl_qp;
+int tdgl_get_qp(tdgl_handle *h, tdgl_qp *q);
+
 /* The noise stream's position: the steps taken with noise since the stream began (0 at
  * tdgl_create).  With noise_seed it is the whole state of the generator, what a checkpoint holds
  * in place of the reference's Xoshiro state (_noise_snap / _noise_back!, noise.jl:200-201;

@@ -72,6 +72,10 @@ class OpenC(ct.Structure):
     _fields_ = [("count", ct.c_int32), ("hold", dp * 3), ("hnew", dp * 3)]
 
 
+class QpC(ct.Structure):
+    _fields_ = [("pot", dp * 4)]
+
+
 class StatsC(ct.Structure):
     _fields_ = [("steps", ct.c_int64), ("calls", ct.c_int64), ("pcg_iters", ct.c_int64),
                 ("last_iters", ct.c_int32), ("device_ms", ct.c_double), ("device_bytes", ct.c_int64),

@@ -38,10 +38,11 @@ def kwt_pcg_128x64x16(nx=128, ny=64, nz=16, pcg_tol=1e-10, J=0.1, dt_scale=0.1):
     """The two bands of notch_12x8x4 (Γ = 1) with KWT: u = Γ/σn = 5.79 (σn = 1/5.79) and γ̃ = 10, on
     128x64x16 cells, dx 0.5, an insulating buffer of 2 cells (σvac/σn = 0.01: the PCG solve) to
     relative residual 1e-10, a wedge cut into face y1 whose sloped face is at 30 degrees, bar
-    field, H = 0.2.  The stepper's own dt rule with dt_scale = 0.1: at the default 0.5 the staged
-    KWT bands with u = 5.79 diverge within a few tens of steps (the explicit supercurrent of the
-    field against the KWT-stiffened phase; 0.2 still diverges on this grid), at 0.1 the run is stable.  J is one rung of the
-    0 → 0.3 ramp."""
+    field, H = 0.2.  The stepper's own dt rule with dt_scale = 0.1: at the default 0.5 (and at 0.25)
+    the staged KWT bands diverge within about a dozen steps, with either reading of u (Γ = 1 and
+    σn = 1/5.79, or Γ = 5.79 and σn = 1), and it is the rotating frame that does (the same step
+    without the frame, or without KWT, stays bounded); at 0.1 the run is stable.  J is one rung of
+    the 0 → 0.3 ramp."""
     sn = 1 / U_KWT
     base = two_band_model(Gamma=1.0, gkwt=GKWT)
     md = Model.make(base.bands, [[0.0, -0.1], [-0.1, 0.0]], kappa=2.5, sigma_n=sn)
@@ -51,19 +52,25 @@ def kwt_pcg_128x64x16(nx=128, ny=64, nz=16, pcg_tol=1e-10, J=0.1, dt_scale=0.1):
     return st, dict(H=0.2, J=J)
 
 
-def open_3band_256x128x32(nx=256, ny=128, nz=32, buf=8, pcg_tol=1e-8, selffield_bc="open"):
+def three_band_aniso_model(kappa=2.5):
     """Three bands (a = 1, 0.6, 0.3; ξ = 1, 1.8, 3; Josephson −0.1, −0.05, −0.02) with
-    m_z = 4 m_xy on 256x128x32 cells and an insulating buffer of 8 cells (σvac = 0.01, PCG with
-    the Float32 preconditioner and the FP64 outer iteration), the open boundary (h_δ by the
-    zero-padded FFT convolution, recomputed every t_GL)."""
+    m_z = 4 m_xy (m_axes = (m, m, 4m), m = 1/(ξ² a))."""
     bands = []
     for a, xi in ((1.0, 1.0), (0.6, 1.8), (0.3, 3.0)):
         m = 1 / (xi * xi * a)
         bands.append(Band.make(a=a, m_axes=(m, m, 4 * m)))
-    md = Model.make(tuple(bands), [[0, -0.1, -0.05], [-0.1, 0, -0.02], [-0.05, -0.02, 0]], kappa=2.5)
+    return Model.make(tuple(bands), [[0, -0.1, -0.05], [-0.1, 0, -0.02], [-0.05, -0.02, 0]], kappa=kappa)
+
+
+def open_3band_256x128x32(nx=256, ny=128, nz=32, buf=8, pcg_tol=1e-8, selffield_bc="open", **kw):
+    """The three anisotropic bands of three_band_aniso_model on 256x128x32 cells and an insulating
+    buffer of 8 cells (σvac = 0.01, PCG with the Float32 preconditioner and the FP64 outer
+    iteration), the open boundary (h_δ by the zero-padded FFT convolution, recomputed every t_GL),
+    a notch at the middle of face y1."""
+    md = three_band_aniso_model()
     gr = Grid.make(nx, ny, nz, dx=0.5, buf=buf)
     mk = notch_masks(gr, (nx // 2 - 4, nx // 2 + 4), (0, ny // 4))
-    st = make_stepper(md, gr, mk, sigma_vac=0.01, pcg_tol=pcg_tol, selffield_bc=selffield_bc)
+    st = make_stepper(md, gr, mk, sigma_vac=0.01, pcg_tol=pcg_tol, selffield_bc=selffield_bc, **kw)
     return st, dict(H=0.2, J=0.1)
 
 

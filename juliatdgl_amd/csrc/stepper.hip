@@ -2165,6 +2165,38 @@ int tdgl_set_open(tdgl_handle *h, const tdgl_open *o) {
   return 0;
 }
 
+int tdgl_get_qp(tdgl_handle *h, tdgl_qp *q) {
+  if (!h || !q) return fail(TDGL_E_ARG, "null handle or normal-fluid potentials");
+  const tdgl_desc &d = h->d;
+  if (d.qp_mode == TDGL_QP_NONE) return fail(TDGL_E_ARG, "the stepper has one normal fluid (no potentials)");
+  const int npot = d.qp_mode == TDGL_QP_TWO ? 1 : d.nbands;
+  for (int p = 0; p < npot; ++p)
+    if (!q->pot[p]) return fail(TDGL_E_ARG, "null potential array %d", p);
+  const int K = d.qp_nmodes, n1 = d.Nx - 1, nj = d.qp_nj, nk = d.qp_nk;
+  const size_t nr = (size_t)h->qn;
+  std::vector<double> M((size_t)K * nr);
+  for (int k = 0; k < K; ++k) CHK(hcopy(h, M.data() + k * nr, h->qS[k], nr * sizeof(double), hipMemcpyDeviceToHost));
+  const size_t NX = (size_t)d.Nx + 1, NY = (size_t)d.Ny + 1, NZ = (size_t)d.Nz + 1;
+  for (int p = 0; p < npot; ++p) {
+    double *pot = q->pot[p];
+    std::fill(pot, pot + NX * NY * NZ, 0.0);
+    for (int kz = 0; kz < nk; ++kz)
+      for (int jy = 0; jy < nj; ++jy)
+        for (int ix = 0; ix < n1; ++ix) {
+          const size_t r = (size_t)ix + (size_t)n1 * (jy + (size_t)nj * kz);
+          double v;
+          if (d.qp_mode == TDGL_QP_TWO) {
+            v = M[r];
+          } else {
+            v = 0.0;  // μ_µ = Σ_k P[µ][k] m_k, in the order of exchange.jl:506-510
+            for (int kk = 0; kk < K; ++kk) v += d.qp_P[p][kk] * M[kk * nr + r];
+          }
+          pot[(size_t)(ix + 1) + NX * ((size_t)(jy + d.qp_j0) + NY * (size_t)(kz + d.qp_k0))] = v;
+        }
+  }
+  return 0;
+}
+
 // ---------------------------------------------------------------------------- the ladder's meter
 extern "C++" {
 namespace {

@@ -9,7 +9,7 @@ import threading
 
 import numpy as np
 
-from .abi import NKERNELS, CycleRecordC, MeterC, OpenC, PackedDesc, PcgState, ProfileC, StatsC, state_struct
+from .abi import NKERNELS, CycleRecordC, MeterC, OpenC, PackedDesc, PcgState, ProfileC, QpC, StatsC, state_struct
 from .scheme import Stepper
 from .state import State
 
@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_lib", "libtdglhip.so")
 
 SYMBOLS = ("tdgl_supported", "tdgl_create", "tdgl_destroy", "tdgl_upload_state", "tdgl_download_state", "tdgl_step",
-           "tdgl_get_pcg", "tdgl_set_pcg", "tdgl_get_open", "tdgl_set_open", "tdgl_get_stats", "tdgl_last_error", "tdgl_device_count",
+           "tdgl_get_pcg", "tdgl_set_pcg", "tdgl_get_open", "tdgl_set_open", "tdgl_get_qp", "tdgl_get_stats", "tdgl_last_error", "tdgl_device_count",
            "tdgl_set_device", "tdgl_profile_enable", "tdgl_profile_get", "tdgl_kernel_name", "tdgl_meter_begin",
            "tdgl_meter_cycle", "tdgl_noise_get", "tdgl_noise_set", "tdgl_noise_draws", "tdgl_batch_create",
            "tdgl_batch_step", "tdgl_batch_meter_cycle", "tdgl_batch_destroy")
@@ -49,6 +49,7 @@ def load():
             L.tdgl_get_pcg.argtypes = [vp, vp]
             L.tdgl_set_pcg.argtypes = [vp, vp]
             L.tdgl_get_stats.argtypes = [vp, vp]
+            L.tdgl_get_qp.argtypes = [vp, vp]
             L.tdgl_last_error.restype = ct.c_char_p
             L.tdgl_device_count.restype = ct.c_int
             L.tdgl_set_device.argtypes = [ct.c_int32]
@@ -167,6 +168,21 @@ class HipStepper:
             c.hold[k] = old[k].ctypes.data_as(ct.POINTER(ct.c_double))
             c.hnew[k] = new[k].ctypes.data_as(ct.POINTER(ct.c_double))
         _check(load().tdgl_set_open(self.h, ct.byref(c)), "tdgl_set_open")
+
+    def get_qp(self):
+        """The normal-fluid potentials of the last step (tdgl_get_qp): [δ] with two fluids, [μ_1..μ_N]
+        with N, node-shaped."""
+        q = self.st.qp
+        if q is None:
+            raise TDGLError("the stepper has one normal fluid")
+        gr = self.st.grid
+        npot = 1 if q.mode == 1 else self.st.N   # TDGL_QP_TWO: δ alone
+        out = [np.zeros((gr.Nx + 1, gr.Ny + 1, gr.Nz + 1), order="F") for _ in range(npot)]
+        c = QpC()
+        for k, a in enumerate(out):
+            c.pot[k] = a.ctypes.data_as(ct.POINTER(ct.c_double))
+        _check(load().tdgl_get_qp(self.h, ct.byref(c)), "tdgl_get_qp")
+        return out
 
     def meter_begin(self, vm) -> None:
         """Uploads the voltmeter `vm` (juliatdgl_amd.meter.Voltmeter) and takes |Ψ| and hz of the

@@ -25,7 +25,7 @@ from typing import Callable, List, Optional
 
 import numpy as np
 
-from .configs import two_band_model
+from .configs import three_band_aniso_model, two_band_model
 from .scheme import Grid, Stepper, catalogue_masks, hole_masks, make_stepper
 from .state import State, seeded_state
 
@@ -50,8 +50,11 @@ def run_seed(master: int, run_id: int) -> int:
 @dataclass(frozen=True)
 class RunSpec:
     """One run: the sample (nx, ny, nz cells, dx 0.5, a notch of depth `depth`·ny on face y1 or a
-    hole of radius `hole` cells), the current J (its sign the polarity), the field H, its seed, and
-    optionally thermal noise T̃."""
+    hole of radius `hole` cells), the current J (its sign the polarity), the field H, its seed,
+    the number of bands (2: the two bands of the 12x8x4 notch case; 3: three anisotropic bands,
+    a = 1, 0.6, 0.3, ξ = 1, 1.8, 3, m_z = 4 m_xy), and optionally thermal noise T̃, its normals
+    drawn on the device (Philox keyed by the seed) or, with noise_host, on the host from the
+    run's own seeded stream and handed over before each call (the validation mode)."""
     run_id: int
     nx: int
     ny: int
@@ -63,6 +66,7 @@ class RunSpec:
     seed: int = 1
     noise: Optional[float] = None
     bands: int = 2
+    noise_host: bool = False
 
 
 def sweep_specs(n_currents: int = 32, master: int = 2026) -> List[RunSpec]:
@@ -83,10 +87,43 @@ def sweep_specs(n_currents: int = 32, master: int = 2026) -> List[RunSpec]:
     return out
 
 
+def serving_specs(n: int, master: int = 4242, lo=(24, 16, 4), hi=(256, 128, 32), noise: float = 1e-3,
+                  noise_host: bool = False) -> List[RunSpec]:
+    """The requests of the served current sweep: per request, from its own seed, two or three bands,
+    cells per axis drawn log-uniform between `lo` and `hi` (rounded to a multiple of 4 in x and y,
+    of 2 in z), a notch (depth 0.1..0.4 of the width) or a hole (radius 0.1..0.25 of the smaller
+    in-plane side), a current |J| in [0.3, 1.5] with a random polarity, and thermal noise T̃ (the
+    noise temperature of the functional; the bath T sits in the bands' a)."""
+    out = []
+    llo, lhi = np.log(np.asarray(lo, float)), np.log(np.asarray(hi, float))
+    for q in range(n):
+        s = run_seed(master, q)
+        rng = np.random.default_rng(s)
+        d = np.exp(rng.uniform(llo, lhi))
+        nx = int(min(hi[0], max(lo[0], 4 * round(d[0] / 4))))
+        ny = int(min(hi[1], max(lo[1], 4 * round(d[1] / 4))))
+        nz = int(min(hi[2], max(lo[2], 2 * round(d[2] / 2))))
+        bands = 2 + int(rng.integers(0, 2))
+        hole = float(rng.uniform(0.1, 0.25) * min(nx, ny)) if rng.random() < 0.5 else 0.0
+        depth = float(rng.uniform(0.1, 0.4))
+        J = float(rng.uniform(0.3, 1.5)) * (1.0 if rng.random() < 0.5 else -1.0)
+        out.append(RunSpec(q, nx, ny, nz, J, depth=depth, hole=hole, seed=s & 0x7FFFFFFF, noise=noise, bands=bands,
+                           noise_host=noise_host))
+    return out
+
+
+def model_of(spec: RunSpec):
+    if spec.bands == 2:
+        return two_band_model()
+    if spec.bands == 3:
+        return three_band_aniso_model()
+    raise ValueError(f"a run has 2 or 3 bands, not {spec.bands}")
+
+
 def stepper_of(spec: RunSpec) -> Stepper:
-    """The plain FP64 stepper of a run: the two bands of the 12x8x4 notch case (κ = 2.5), the
-    separable implicit field, the bar self-field, dt = 0.01 (Euler substeps as the bands need)."""
-    md = two_band_model()
+    """The plain FP64 stepper of a run: its bands (κ = 2.5), the separable implicit field, the bar
+    self-field, dt = 0.01 (Euler substeps or stages as the bands need)."""
+    md = model_of(spec)
     gr = Grid.make(spec.nx, spec.ny, spec.nz, dx=0.5, buf=1)
     if spec.hole > 0:
         mk = hole_masks(gr, (spec.nx / 2, spec.ny / 2), spec.hole)
@@ -94,8 +131,30 @@ def stepper_of(spec: RunSpec) -> Stepper:
         mk = catalogue_masks(gr, "WEDGE", wedge_depth=spec.depth)
     kw = {}
     if spec.noise is not None:
-        kw = dict(noise=spec.noise, noise_seed=spec.seed)
+        kw = dict(noise=spec.noise, noise_seed=spec.seed, noise_draws="host" if spec.noise_host else "device")
     return make_stepper(md, gr, mk, dt=0.01, selffield_bc="bar", **kw)
+
+
+class HostDraws:
+    """A backend (HipStepper, or the oracle's) whose steps take their normals from the run's own
+    seeded stream on the host (numpy PCG64 of the run's seed), handed over before every call:
+    the validation mode of a noisy run, in which the device and the oracle see the same draws."""
+
+    def __init__(self, backend, st: Stepper, seed: int):
+        from .noise import per_step
+        self.b, self.st = backend, st
+        self.rng = np.random.default_rng(seed)
+        self.n = per_step(st)
+
+    def step(self, n: int, H: float, HJ: float) -> None:
+        from .noise import from_stream, pack
+        if n > 0:
+            self.b.noise_draws(pack(self.st, [from_stream(self.st, self.rng.standard_normal(self.n))
+                                              for _ in range(n)]))
+        self.b.step(n, H, HJ)
+
+    def __getattr__(self, name):
+        return getattr(self.b, name)
 
 
 def state_of(spec: RunSpec, st: Stepper) -> State:
@@ -173,7 +232,7 @@ def _execute(spec: RunSpec, work: Callable, st: Optional[Stepper] = None):
     dev = HipStepper(st)
     try:
         dev.upload(state_of(spec, st))
-        out = work(dev, st, spec)
+        out = work(HostDraws(dev, st, spec.seed) if spec.noise_host else dev, st, spec)
         return out, dev.stats()["device_bytes"]
     finally:
         dev.close()

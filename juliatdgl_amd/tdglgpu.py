@@ -21,7 +21,7 @@ from .scheme import Stepper
 from .state import State
 
 __all__ = ["supported", "GPUStepper", "GState", "device_state", "device_state_", "host_state", "gpu_step",
-           "gpu_steps", "enable", "disable", "execution_stats", "STEPS", "cycle_steps"]
+           "gpu_steps", "enable", "disable", "execution_stats", "STEPS", "cycle_steps", "restart_open"]
 
 
 def supported(st: Stepper):
@@ -135,15 +135,50 @@ def _cycle(s: State, st: Stepper, n: int, H: float, HJ: float) -> State:
     else:
         device_state_(dev[1], s)
     g = _state["dev"][1]
-    if st.field_mode == 2 and getattr(st, "pcg_state", None) is not None:
-        gs.dev.set_pcg(st.pcg_state)
+    _stepper_up(gs.dev, st)
     gpu_steps(g, gs, n, H=H, HJ=HJ)
-    if st.field_mode == 2:
-        st.pcg_state = gs.dev.get_pcg()
+    _stepper_down(gs.dev, st)
     host_state(s, g)
     _state["bc"] += 1
     _state["bs"] += n
     return s
+
+
+# The Stepper's own mutable state, which the CPU step keeps in the Stepper and the checkpoints and
+# measurements read there (TDGLGPU.jl:2130-2141): here attributes of the Python Stepper, absent
+# until a cycle has set them (the state of a fresh Stepper).
+#   st.pcg_state   PcgState: x, xprev, warm, have2, iters (_pcg_up! / _pcg_down!)
+#   st.open_state  (count, old, new): the open boundary's clock and ramp (_open_up!; st.open.count,
+#                  old, new); run_ladders restarts a polarity with count = -1 (protocol.jl:852)
+#   st.qp_state    the normal-fluid potentials of the last step (st.qpx.δ or st.qpx.μ)
+#   st.noise_counter  the noise stream's position (with the seed, the generator's whole state: what
+#                  noise_rng is to the CPU step)
+def _stepper_up(dev, st: Stepper) -> None:
+    if st.field_mode == 2 and getattr(st, "pcg_state", None) is not None:
+        dev.set_pcg(st.pcg_state)
+    if st.selffield_bc == "open" and getattr(st, "open_state", None) is not None:
+        dev.set_open(*st.open_state)
+    if st.noise is not None and getattr(st, "noise_counter", None) is not None:
+        dev.noise_counter = st.noise_counter
+
+
+def _stepper_down(dev, st: Stepper) -> None:
+    if st.field_mode == 2:
+        st.pcg_state = dev.get_pcg()
+    if st.selffield_bc == "open":
+        st.open_state = dev.get_open()
+    if st.qp is not None:
+        st.qp_state = dev.get_qp()
+    if st.noise is not None:
+        st.noise_counter = dev.noise_counter
+
+
+def restart_open(st: Stepper) -> None:
+    """What run_ladders does to the open boundary before each polarity (protocol.jl:852): the
+    ramp starts afresh (count = -1) at the next cycle."""
+    if st.selffield_bc == "open" and getattr(st, "open_state", None) is not None:
+        _, old, new = st.open_state
+        st.open_state = (-1, old, new)
 
 
 def cycle_steps(s: State, st: Stepper, n: int, H: float, HJ: float, cpu_step: Optional[Callable] = None) -> State:

@@ -161,6 +161,9 @@ class OracleBackend:
     def step(self, n: int, H: float, HJ: float) -> None:
         self.o.step(n, H, HJ)
 
+    def noise_draws(self, flat) -> None:
+        self.o.noise_draws(flat)
+
     def meter_begin(self, vm) -> None:
         self.meter = OracleMeter(self.st, vm, self.o.get_state())
 
