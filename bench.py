@@ -263,9 +263,11 @@ def main(argv=None):
         return 3
     dev_ms = st1["device_ms"] - st0["device_ms"]
     iters = st1["pcg_iters"] - st0["pcg_iters"]
+    rollbacks = st1["rollbacks"] - st0["rollbacks"]
+    captures = st1["captures"] - st0["captures"]
     nsteps = args.steps * LMAX
     print(f"[bench] timed {args.steps} cycles: wall {wall:.3f}s device {dev_ms / 1e3:.3f}s, pcg iters/step "
-          f"{iters / nsteps:.2f}", file=sys.stderr, flush=True)
+          f"{iters / nsteps:.2f}, roll backs {rollbacks}, graph captures {captures}", file=sys.stderr, flush=True)
     cells = st.grid.cells
     value = cells * args.steps / wall
     # per-kernel device time: the same number of cycles again with HIP events around every launch
@@ -326,6 +328,7 @@ def main(argv=None):
                       "device_ms_per_cycle": dev_ms / args.steps, "padded_nodes": nodes,
                       "cell_steps_per_s": value * LMAX, "b_alg_per_node_step": balg,
                       "hbm_roofline_frac_cycle": hbm_cycle, "ms_per_time_step": 1e3 * wall / nsteps,
+                      "timed_rollbacks": rollbacks, "timed_graph_captures": captures,
                       **({"reference_published": published} if published else {})},
            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out))

@@ -198,6 +198,7 @@ typedef struct {
   int64_t graph_cycles; /* tdgl_step calls replayed from a captured graph */
   int64_t rollbacks;    /* captured cycles redone eagerly (a solve needed more iterations) */
   int32_t pcg_budget;   /* PCG iterations per solve in the captured cycle */
+  int64_t captures;     /* graphs captured and instantiated (the last four are kept and replayed) */
 } tdgl_stats;
 
 typedef struct tdgl_handle tdgl_handle;

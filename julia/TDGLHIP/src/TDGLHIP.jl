@@ -17,7 +17,8 @@ using TDGL
 using TDGL: State, Stepper, SepSolver, PCGSolver
 
 export GPUStepper, GState, device_state, device_state!, host_state!, gpu_step!, gpu_steps!, supported, enable!,
-       meter_begin!, meter_cycle, noise_counter, noise_counter!, noise_draws!, reference_draws,
+       meter_begin!, meter_cycle, noise_counter, noise_counter!, noise_draws!, reference_draws, noise_position,
+       noise_position!,
        disable!, execution_stats
 
 const MAXB = 4
@@ -273,6 +274,41 @@ function _pcg_down!(S::PCGSolver, gs::GPUStepper)
     S
 end
 
+# the ramp of the open boundary to and from the device (_open_up!, TDGLGPU.jl:1214-1217; the host's OpenBC
+# keeps the clock and the last two h_δ, which run_ladders restarts, protocol.jl:852, and open_extra saves)
+struct OpenC
+    count::Int32
+    hold::NTuple{3,Ptr{Float64}}; hnew::NTuple{3,Ptr{Float64}}
+end
+function _open_up!(gs::GPUStepper, ob)
+    o = OpenC(Int32(ob.count), ntuple(c -> _ptr(ob.old[c]), 3), ntuple(c -> _ptr(ob.new[c]), 3))
+    GC.@preserve ob _check(ccall((:tdgl_set_open, libpath()), Cint, (Ptr{Cvoid}, Ref{OpenC}), gs.h, o), "tdgl_set_open")
+end
+function _open_down!(ob, gs::GPUStepper)
+    o = Ref(OpenC(Int32(0), ntuple(c -> _ptr(ob.old[c]), 3), ntuple(c -> _ptr(ob.new[c]), 3)))
+    GC.@preserve ob _check(ccall((:tdgl_get_open, libpath()), Cint, (Ptr{Cvoid}, Ptr{OpenC}), gs.h, o), "tdgl_get_open")
+    ob.count = o[].count
+    ob
+end
+
+# the normal-fluid potentials of the last step into the Stepper (TDGLGPU.jl:2136-2140)
+struct QpC
+    pot::NTuple{MAXB,Ptr{Float64}}
+end
+function _qp_down!(q, gs::GPUStepper)
+    arrs = q isa TDGL.QPExchange ? (q.δ,) : Tuple(q.μ)
+    p = QpC(_pad(ntuple(k -> _ptr(arrs[k]), length(arrs)), MAXB, Ptr{Float64}(0)))
+    GC.@preserve arrs _check(ccall((:tdgl_get_qp, libpath()), Cint, (Ptr{Cvoid}, Ref{QpC}), gs.h, p), "tdgl_get_qp")
+    q
+end
+
+# the device stream's position per Stepper (with its seed the generator's whole state; a checkpoint
+# stores it where the CPU step stores noise_rng, protocol.jl:659)
+const NOISE_POS = WeakKeyDict{Any,UInt64}()
+"""noise_position(st) -> UInt64: the device noise stream's position of `st` after its last hook cycle."""
+noise_position(st::Stepper) = get(NOISE_POS, st, UInt64(0))
+noise_position!(st::Stepper, c::Integer) = (NOISE_POS[st] = UInt64(c); st)
+
 # ---- the ladder's per-cycle measurements on the device (protocol.jl:341-352, 416-464) -------------
 struct MeterC
     vi::Int32
@@ -424,11 +460,18 @@ function _hip_cycle!(s::State, st::Stepper, n::Int; H::Float64, HJ::Float64)
     end
     _DEV[] === nothing || _DEV[][1] !== s ? (_DEV[] = (s, device_state(gs, s))) : device_state!(_DEV[][2], s)
     g = _DEV[][2]
+    # the Stepper's own state to the device, and back after the steps, as the CPU step leaves it
+    # (TDGLGPU.jl:2130-2141): PCG history, the open boundary's clock and ramp, the noise position
     st.imp isa PCGSolver && _pcg_up!(gs, st.imp)
+    st.open === nothing || _open_up!(gs, st.open)
     # with noise: the reference's own stream (HOST_DRAWS[]), or the device's stream at its position
     st.noise !== nothing && HOST_DRAWS[] && noise_draws!(gs, reference_draws(st, n), n)
+    st.noise !== nothing && !HOST_DRAWS[] && haskey(NOISE_POS, st) && noise_counter!(gs, NOISE_POS[st])
     gpu_steps!(g, gs, n; H=H, HJ=HJ)
     st.imp isa PCGSolver && _pcg_down!(st.imp, gs)
+    st.open === nothing || _open_down!(st.open, gs)
+    st.qpx === nothing || _qp_down!(st.qpx, gs)
+    st.noise !== nothing && !HOST_DRAWS[] && (NOISE_POS[st] = noise_counter(gs))
     host_state!(s, g)
     _COUNTS[:bc] += 1; _COUNTS[:bs] += n
     s

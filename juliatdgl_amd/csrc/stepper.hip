@@ -87,6 +87,23 @@ struct HostSnap {
   int parity;
   bool warm, have2;
 };
+// A captured run of steps is valid for the host-side pointer set it was captured with (the parity
+// of the advance's swaps), the PCG's warm-start flags and its iteration budget.
+struct GraphKey {
+  int n, parity, warm, have2, budget;
+  bool operator==(const GraphKey &o) const {
+    return n == o.n && parity == o.parity && warm == o.warm && have2 == o.have2 && budget == o.budget;
+  }
+};
+struct CachedGraph {
+  bool live = false;
+  GraphKey key{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HostSnap end{};        // the host-side pointers and flags after its steps
+  long long used = 0;    // the last use (the least recently used entry is replaced)
+};
+constexpr int GRAPH_CACHE = 4;
 }  // namespace
 
 struct tdgl_handle {
@@ -159,10 +176,10 @@ struct tdgl_handle {
   int budget = 0;          // PCG iterations per solve in the graph (0: not known yet)
   bool budget_fixed = false;  // set by TDGL_HIP_BUDGET: the budget only grows (roll backs)
   int shrink_run = 0, shrink_max = 0;  // cycles in a row far under the budget, their longest solve
-  long long gkey = -1;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  HostSnap gend{};
+  int shrink_hold = 0;     // cycles after a roll back in which the budget does not shrink
+  CachedGraph gc[GRAPH_CACHE];
+  long long gtick = 0;
+  long long captures = 0;  // graphs captured and instantiated
   double *bk[48] = {};     // device copy of a cycle's start (roll back)
   // thermal noise: the stream's step counter on the device ([0] steps taken, [1] the step of the
   // first of the caller's draws) and its host copy; the caller's draws; SK-ROCK's δθ and source;
@@ -1414,26 +1431,38 @@ int steps_eager(tdgl_handle *h, int n) {
   return 0;
 }
 
+void drop_graph(CachedGraph &g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g = CachedGraph{};
+}
+
+// every captured graph (they hold the addresses of the handle's buffers)
 int destroy_graph(tdgl_handle *h) {
-  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-  if (h->graph) (void)hipGraphDestroy(h->graph);
-  h->gexec = nullptr;
-  h->graph = nullptr;
+  for (CachedGraph &g : h->gc) drop_graph(g);
   return 0;
 }
 
-// n steps as one captured graph: captured once per (n, pointer parity, warm, have2, budget),
-// replayed after.  With the PCG, the graph holds `budget` gated iterations per solve; a cycle in
-// which a solve was still active after them is rolled back from a device copy of its start and
-// stepped eagerly, and the budget grows.
-// the budget's way down: BUDGET_RUN cycles in a row BUDGET_SLACK iterations under it
-constexpr int BUDGET_SLACK = 6, BUDGET_RUN = 1;
+// n steps as one captured graph: captured once per key (n, pointer parity, warm, have2, budget)
+// and replayed after; the last GRAPH_CACHE graphs are kept, so cycles that alternate between two
+// budgets or parities replay instead of capturing again.  With the PCG, the graph holds `budget`
+// gated iterations per solve; a cycle in which a solve was still active after them is rolled back
+// from a device copy of its start and stepped eagerly, and the budget grows.
+// The budget's way down: BUDGET_RUN cycles in a row whose longest solve stayed BUDGET_SLACK or more
+// iterations under it, and none in the BUDGET_HOLD cycles after a roll back.
+constexpr int BUDGET_SLACK = 6, BUDGET_RUN = 2, BUDGET_HOLD = 4;
 int cycle_graph(tdgl_handle *h, int n) {
   const bool pcg = h->d.field_mode == TDGL_FIELD_PCG;
-  const long long key = ((((long long)n * 2 + h->parity) * 2 + h->warm) * 2 + h->have2) * 1024 + (pcg ? h->budget : 0);
+  const GraphKey key{n, h->parity, h->warm ? 1 : 0, h->have2 ? 1 : 0, pcg ? h->budget : 0};
   const HostSnap start = snap(h);
-  if (!h->gexec || h->gkey != key) {
-    CHK(destroy_graph(h));
+  CachedGraph *g = nullptr;
+  for (CachedGraph &c : h->gc)
+    if (c.live && c.key == key) g = &c;
+  if (!g) {
+    g = &h->gc[0];
+    for (CachedGraph &c : h->gc)
+      if (!c.live || (g->live && c.used < g->used)) g = &c;
+    drop_graph(*g);
     h->capturing = true;
     // relaxed: other handles, in other host threads, allocate, free and copy on their own streams
     // meanwhile; this capture records only the kernels launched on h->s
@@ -1442,24 +1471,32 @@ int cycle_graph(tdgl_handle *h, int n) {
     hipGraph_t gph = nullptr;
     hipError_t e = hipStreamEndCapture(h->s, &gph);
     h->capturing = false;
-    if (rc) {
+    if (rc || e != hipSuccess) {
+      // nothing ran on the device: the host-side pointers and flags go back to the start
       if (gph) (void)hipGraphDestroy(gph);
       restore(h, start);
-      return rc;
+      return rc ? rc : fail(TDGL_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
     }
-    if (e != hipSuccess) return fail(TDGL_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    h->graph = gph;
-    HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
-    h->gkey = key;
-    h->gend = snap(h);
+    g->graph = gph;
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      drop_graph(*g);
+      restore(h, start);
+      return fail(TDGL_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    g->key = key;
+    g->end = snap(h);
+    g->live = true;
+    h->captures += 1;
     restore(h, start);
   }
+  g->used = ++h->gtick;
   if (pcg) {
     PL(K_VEC, launch_copies(backup_list(h, true), h->s));
     PL(K_VEC, launch_pcg_cycle_reset(h->pd, h->s));
   }
-  HIPCHK(hipGraphLaunch(h->gexec, h->s));
-  restore(h, h->gend);
+  HIPCHK(hipGraphLaunch(g->exec, h->s));
+  restore(h, g->end);
   if (pcg) {
     HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
@@ -1474,6 +1511,9 @@ int cycle_graph(tdgl_handle *h, int n) {
       HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
       h->shrink_run = 0;
       h->shrink_max = 0;
+      h->shrink_hold = BUDGET_HOLD;
+    } else if (h->shrink_hold > 0) {
+      h->shrink_hold -= 1;
     } else if (!h->budget_fixed) {
       // the budget follows the solves down: after BUDGET_RUN cycles (in a row) whose longest solve
       // stayed BUDGET_SLACK or more iterations under it, it becomes that longest + 3 (the graph is
@@ -1524,6 +1564,8 @@ int check_desc(const tdgl_desc *d, char *why, size_t n) {
   }
   if (d->rotframe && !d->extrap) return no("rotframe without extrap");
   if (d->field_mode < 0 || d->field_mode > 3) return no("field mode");
+  if (d->field_mode == TDGL_FIELD_PCG && (d->pcg_maxit < 1 || d->pcg_maxit > (1 << 20) || !(d->pcg_tol >= 0)))
+    return no("PCG iteration limit (1..2^20) or tolerance");
   if ((d->field_mode == TDGL_FIELD_PCG || d->field_mode == TDGL_FIELD_EXPLICIT_SCALED) && !(d->Dx && d->Dy && d->Dz))
     return no("link conductivities D");
   if ((d->field_mode == TDGL_FIELD_SEPARABLE || d->field_mode == TDGL_FIELD_PCG) && !(d->dx == d->dy && d->dy == d->dz))
@@ -2324,6 +2366,7 @@ int tdgl_get_stats(tdgl_handle *h, tdgl_stats *st) {
   st->graph_cycles = h->graph_cycles;
   st->rollbacks = h->rollbacks;
   st->pcg_budget = h->budget;
+  st->captures = h->captures;
   st->steps = h->steps;
   st->calls = h->calls;
   st->pcg_iters = h->pcg_iters;

@@ -353,3 +353,36 @@ def test_hip_long_runs_match_the_oracle(name, cycles):
         errs[k] = np.max(np.abs(getattr(got, k) - getattr(ref, k))) / m
     print("    " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
     assert max(errs.values()) <= 1e-8, errs
+
+
+def test_pcg_budget_settles_on_a_current_ramp():
+    """Captured cycles while the current steps up every cycle (the solves' lengths move): the
+    results equal the eager steps, and the budget neither flips every cycle nor recaptures each
+    time (roll backs hold the budget for a few cycles; the last graphs are kept)."""
+    from juliatdgl_amd.hip import HipStepper
+    _hip()
+    st, tol = make_case("kwt10_pcg")
+    s0, pcg, H, _ = seed_state(st)
+    print("[gpu] PCG budget on a current ramp", flush=True)
+    outs, stats = [], []
+    for graphs in ("1", "0"):
+        import os
+        os.environ["TDGL_HIP_GRAPH"] = graphs
+        try:
+            dev = HipStepper(st)
+        finally:
+            os.environ.pop("TDGL_HIP_GRAPH", None)
+        dev.upload(s0)
+        dev.set_pcg(pcg)
+        for q in range(16):
+            dev.step(8, H, st.selffield(0.05 * q))
+        outs.append(dev.download())
+        stats.append(dev.stats())
+        dev.close()
+    g = stats[0]
+    print(f"    graph cycles {g['graph_cycles']}, roll backs {g['rollbacks']}, captures {g['captures']}, "
+          f"budget {g['pcg_budget']}", flush=True)
+    errs = compare_states(outs[0], outs[1])
+    core = {k: v for k, v in errs.items() if not k.startswith(("U", "V"))}
+    assert max(core.values()) <= 1e-12, errs
+    assert g["rollbacks"] <= 4 and g["captures"] <= 8
