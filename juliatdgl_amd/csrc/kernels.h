@@ -293,6 +293,7 @@ struct ZModes {
   int off[3];
   int nx, ny, nz;
   double a;
+  int columns;           // 1: one block per few whole z-columns (k_zmodes), 0: x-runs (k_zmodes_run) where nz fits
 };
 template <class T>
 hipError_t launch_zmodes(const ZModes &m, const int *gate, hipStream_t s);

@@ -2052,6 +2052,114 @@ __device__ __forceinline__ void zmodes_body(const ZModes &m, const int *gate, co
   if (n < nz - 1) out[m.off[2] + mx + nx * (my + ny * n)] = oz;
 }
 
+// The same z pass, transforms and projector over x-runs of TX modes: a block owns the modes
+// (mx0 .. mx0+TX-1, my, all mz) and stages their columns z-row by z-row, each row a contiguous run
+// of x (coalesced loads and stores), with the forward z bases once per block (the inverse ones are
+// their transposes: the same values, read with the indices swapped).  Thread (p, g) takes column p
+// and the modes (and then the output z indices) g, g + G, g + 2G, ...; every sum runs over n (or q)
+// in the order of zmodes_body, so the results are the same bits.
+constexpr int ZT_NZ_MAX = 64;  // the run kernel holds the z bases in LDS up to this nz
+template <class T, int TX>
+__global__ void __launch_bounds__(256) k_zmodes_run(ZModes m, const int *gate) {
+  if (gate && !*gate) return;
+  constexpr int G = 256 / TX;                          // mode groups
+  constexpr int MZ = (ZT_NZ_MAX + G - 1) / G;          // modes per thread at most
+  extern __shared__ __align__(16) unsigned char zt_smem[];
+  const int nx = m.nx, ny = m.ny, nz = m.nz;
+  T *Cx = (T *)zt_smem, *Cy = Cx + nz * TX, *Cz = Cy + nz * TX;   // [z][TX]
+  T *AV = Cz + nz * TX, *AU = AV + nz * nz;                        // AV[n nz + m] = V(n, m)
+  const int t = threadIdx.x, p = t % TX, g = t / TX;
+  const int mx = blockIdx.x * TX + p, my = blockIdx.y;
+  const bool live = mx < nx;
+  const bool hx = live && mx > 0, hy = live && my > 0;
+  {
+    const T *gV = (const T *)m.Az[0], *gU = (const T *)m.Az[1];
+    for (int e = t; e < nz * nz; e += 256) AV[e] = gV[e];
+    for (int e = t; e < (nz - 1) * (nz - 1); e += 256) AU[e] = gU[e];
+  }
+  {
+    // clamped, in-bounds addresses, loaded unconditionally; the absent values are selected after
+    const T *in = (const T *)m.in;
+    const int mc = min(mx, nx - 1);
+    const size_t stx = (size_t)(nx - 1) * ny, sty = (size_t)nx * (ny - 1), stz = (size_t)nx * ny;
+    const T *colx = in + m.off[0] + max(mc - 1, 0) + (nx - 1) * my;
+    const T *coly = in + m.off[1] + mc + nx * max(my - 1, 0);
+    const T *colz = in + m.off[2] + mc + nx * my;
+    for (int z = g; z < nz; z += G) {
+      const T vx = colx[stx * z], vy = coly[sty * z], vz = colz[stz * min(z, nz - 2)];
+      Cx[z * TX + p] = hx ? vx : T(0);
+      Cy[z * TX + p] = hy ? vy : T(0);
+      Cz[z * TX + p] = (live && z < nz - 1) ? vz : T(0);
+    }
+  }
+  __syncthreads();
+  T ax[MZ], ay[MZ], az[MZ];
+#pragma unroll
+  for (int r = 0; r < MZ; ++r) ax[r] = ay[r] = az[r] = T(0);
+  for (int n = 0; n < nz - 1; ++n) {
+    const T cx = Cx[n * TX + p], cy = Cy[n * TX + p], cz = Cz[n * TX + p];
+#pragma unroll
+    for (int r = 0; r < MZ; ++r) {
+      const int mz = g + G * r;
+      if (mz < nz) {
+        const T v = AV[n * nz + mz];
+        ax[r] += cx * v;
+        ay[r] += cy * v;
+        az[r] += cz * AU[n * (nz - 1) + (mz > 0 ? mz - 1 : 0)];
+      }
+    }
+  }
+  {
+    const T cx = Cx[(nz - 1) * TX + p], cy = Cy[(nz - 1) * TX + p];
+#pragma unroll
+    for (int r = 0; r < MZ; ++r) {
+      const int mz = g + G * r;
+      if (mz < nz) {
+        const T v = AV[(nz - 1) * nz + mz];
+        ax[r] += cx * v;
+        ay[r] += cy * v;
+      }
+    }
+  }
+  __syncthreads();  // the columns are read: their LDS takes the projected modes
+  if (live) {
+    const T k1 = ((const T *)m.sx)[mx], k2 = ((const T *)m.sy)[my];
+#pragma unroll
+    for (int r = 0; r < MZ; ++r) {
+      const int mz = g + G * r;
+      if (mz < nz) {
+        const T k3 = ((const T *)m.sz)[mz];
+        const bool hz = mz > 0;
+        T x1 = hx ? ax[r] : T(0), y1 = hy ? ay[r] : T(0), z1 = hz ? az[r] : T(0);
+        const T k2s = k1 * k1 + k2 * k2 + k3 * k3;
+        if (k2s != 0) project<T>(k1, k2, k3, (T)m.a, x1, y1, z1, hx, hy, hz);
+        Cx[mz * TX + p] = x1;
+        Cy[mz * TX + p] = y1;
+        Cz[mz * TX + p] = z1;
+      }
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  T *out = (T *)m.out;
+#pragma unroll
+  for (int r = 0; r < MZ; ++r) {
+    const int n = g + G * r;  // the z index written
+    if (n >= nz) continue;
+    T ox = 0, oy = 0, oz = 0;
+    for (int q = 0; q < nz; ++q) {
+      const T v = AV[n * nz + q];
+      ox += Cx[q * TX + p] * v;
+      oy += Cy[q * TX + p] * v;
+    }
+    const int nu = n < nz - 1 ? n : 0;
+    for (int q = 1; q < nz; ++q) oz += Cz[q * TX + p] * AU[nu * (nz - 1) + (q - 1)];
+    if (hx) out[m.off[0] + (mx - 1) + (size_t)(nx - 1) * (my + (size_t)ny * n)] = ox;
+    if (hy) out[m.off[1] + mx + (size_t)nx * ((my - 1) + (size_t)(ny - 1) * n)] = oy;
+    if (n < nz - 1) out[m.off[2] + mx + (size_t)nx * (my + (size_t)ny * n)] = oz;
+  }
+}
+
 constexpr int ZM_LB_MAX = 64;  // z bases in LDS up to this nz
 template <class T, bool LB>
 __global__ void __launch_bounds__(256) k_zmodes(ZModes m, const int *gate) { zmodes_body<T, LB>(m, gate, hw_blk()); }
@@ -2070,6 +2178,17 @@ hipError_t launch_zmodes(const ZModes &m, const int *gate, hipStream_t s) {
   if (g_rec)
     return (sizeof(T) != 8 || gate) ? hipErrorNotSupported
                                      : record(BK_ZMODES, Geo{}, &m, sizeof m, lb ? 1 : 0, true, dim3(cdiv(m.nx * m.ny, C)), lds);
+  if (m.nz <= ZT_NZ_MAX && !m.columns) {
+    // x-runs of 64 modes (32 in FP64, or where 64 would leave fewer than two blocks per CU)
+    const bool wide = sizeof(T) == 4 && cdiv(m.nx, 64) * m.ny >= 512;
+    const int TX = wide ? 64 : 32;
+    const size_t ldr = (3 * (size_t)m.nz * TX + (size_t)m.nz * m.nz + (size_t)(m.nz - 1) * (m.nz - 1)) * sizeof(T);
+    if (wide)
+      TDGL_LAUNCH((k_zmodes_run<T, 64>), dim3(cdiv(m.nx, 64), m.ny), dim3(256), ldr, s, m, gate);
+    else
+      TDGL_LAUNCH((k_zmodes_run<T, 32>), dim3(cdiv(m.nx, 32), m.ny), dim3(256), ldr, s, m, gate);
+    return hipGetLastError();
+  }
   if (lb)
     TDGL_LAUNCH((k_zmodes<T, true>), dim3(cdiv(m.nx * m.ny, C)), dim3(256), lds, s, m, gate);
   else

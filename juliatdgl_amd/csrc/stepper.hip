@@ -180,6 +180,7 @@ struct tdgl_handle {
   CachedGraph gc[GRAPH_CACHE];
   long long gtick = 0;
   long long captures = 0;  // graphs captured and instantiated
+  int zcols = 0;           // TDGL_HIP_ZCOL=1: the z pass of whole columns per block (k_zmodes)
   double *bk[48] = {};     // device copy of a cycle's start (roll back)
   // thermal noise: the stream's step counter on the device ([0] steps taken, [1] the step of the
   // first of the caller's draws) and its host copy; the caller's draws; SK-ROCK's δθ and source;
@@ -719,6 +720,7 @@ ZModes zmodes(tdgl_handle *h, const void *in, void *out, bool f32) {
   m.ny = h->g.Ny - 1;
   m.nz = h->g.Nz - 1;
   m.a = h->d.a;
+  m.columns = h->zcols;
   return m;
 }
 
@@ -1873,6 +1875,8 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     // x-y and z-mode launches fit
     // TDGL_HIP_PLANE1=0: the chunked Float32 x-y kernel also where the one-round kernel fits;
     // =2 or =3: the one-round kernel with that many m tiles per block wherever it fits
+    const char *zc = std::getenv("TDGL_HIP_ZCOL");
+    h->zcols = zc && zc[0] == '1' ? 1 : 0;
     const char *g1 = std::getenv("TDGL_HIP_PLANE1");
     if (g1 && (g1[0] == '0' || g1[0] == '2' || g1[0] == '3')) h->plane1 = g1[0] - '0';
     const char *gp = std::getenv("TDGL_HIP_PLANE");

@@ -386,3 +386,25 @@ def test_pcg_budget_settles_on_a_current_ramp():
     core = {k: v for k, v in errs.items() if not k.startswith(("U", "V"))}
     assert max(core.values()) <= 1e-12, errs
     assert g["rollbacks"] <= 4 and g["captures"] <= 8
+
+
+@pytest.mark.parametrize("name", ["production", "pcg_fast_f32", "pcg_exact", "kwt10_pcg", "three_aniso", "pcg_wide_f32"])
+def test_zmodes_runs_equal_whole_columns(name, monkeypatch):
+    """The z pass over x-runs of modes (k_zmodes_run) sums in the order of the whole-column kernel
+    (TDGL_HIP_ZCOL=1): 8 steps give the same bits either way."""
+    HipStepper = _hip()
+    st, _ = make_case(name)
+    s0, pcg, H, HJ = seed_state(st)
+    outs = []
+    print(f"[gpu] z pass runs against columns: {name}", flush=True)
+    for zc in ("1", "0"):
+        monkeypatch.setenv("TDGL_HIP_ZCOL", zc)
+        dev = HipStepper(st)
+        dev.upload(s0)
+        if pcg is not None:
+            dev.set_pcg(pcg)
+        dev.step(8, H, HJ)
+        outs.append(dev.download())
+        dev.close()
+    for (k, a), (_, b) in zip(outs[0].arrays(), outs[1].arrays()):
+        assert np.array_equal(a, b), k
