@@ -1,14 +1,16 @@
 """Ladder-cycle throughput of the HIP stepper on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kwt_pcg_128x64x16]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload open_3band_256x128x32]
 
 A "step" of this benchmark is one ladder cycle: lMax = 40 time steps of the release-2.0 step
 (the n that TDGL's ladder hands to its stepper hook, gtdgl/src/protocol.jl:28, 409), with the
-state resident on the device.  The workload is the two-band KWT stepper on 128x64x16 cells with
-an insulating buffer and the PCG field solve to relative residual 1e-10 (juliatdgl_amd.configs.kwt_pcg_128x64x16).
+state resident on the device.  The default workload is the largest one-GPU configuration: three
+anisotropic bands on 256x128x32 cells with an 8-cell insulating buffer, the open boundary and the
+PCG field solve with the Float32 preconditioner (juliatdgl_amd.configs.open_3band_256x128x32); the
+two-band KWT run on 128x64x16 cells is measured beside it (config.secondary).
 Prints one JSON line: cell-cycles/s (nx·ny·nz × cycles / s), the roofline of the dominant
 kernel (HIP events around its launches over the timed cycles), and the CPU oracle timed on a
-bounded sample of the same workload on this host.
+bounded sample of the same workload on this host, with one thread and with every host thread.
 """
 from __future__ import annotations
 
@@ -88,19 +90,6 @@ def cycle_bytes_per_node_step(st, iters_per_step: float) -> float:
     field = 16 * N + 192
     pcg = 300 + 288 * iters_per_step if st.field_mode == 2 else (168 if st.field_mode == 1 else 0)
     return band + field + pcg + 96
-
-
-def cpu_baseline(st, s0, H, HJ, nsteps: int, threads: int):
-    """The oracle (CPU restatement of the step) on `nsteps` time steps of the same workload."""
-    from oracle.oracle import Oracle, set_threads
-    set_threads(threads)
-    o = Oracle(st)
-    o.set_state(s0)
-    t = time.perf_counter()
-    o.step(nsteps, H, HJ)
-    dt = time.perf_counter() - t
-    cells = st.grid.cells
-    return cells * (nsteps / LMAX) / dt, dt
 
 
 def measured_traffic(kernel: str, workload: str):
@@ -187,102 +176,106 @@ def bench_sweep(args) -> int:
     return 0
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10, help="timed ladder cycles")
-    ap.add_argument("--warmup", type=int, default=5,
-                    help="untimed ladder cycles (the PCG budget settles and its graph is captured in them)")
-    ap.add_argument("--workload", default="kwt_pcg_128x64x16",
-                    choices=("kwt_pcg_128x64x16", "open_3band_256x128x32", "pcg_wedge_96x48x24", "plain_wedge_96x48x24",
-                             "sweep64"))
-    ap.add_argument("--workers", type=int, default=8, help="sweep64: runs in flight at once")
-    ap.add_argument("--cpu-steps", type=int, default=8, help="time steps of the CPU baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-out", default=None, help="write the per-kernel profile (JSON) here")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="first copy a MIB-MiB device buffer (a reference kernel for the counters' scale)")
-    args = ap.parse_args(argv)
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: a run is one device's work and this build has no multi-GPU path "
-              "(--gpus 1 only)", file=sys.stderr)
-        return 2
-    if device_count() < 1:
-        print("no HIP device", file=sys.stderr)
-        return 1
-    if args.workload == "sweep64":
-        return bench_sweep(args)
-    if args.workload == "kwt_pcg_128x64x16":
-        st, p = configs.kwt_pcg_128x64x16()
-        workload = ("two-band KWT (u=Gamma/sigma_n=5.79, gkwt=10) TDGL, 128x64x16 cells, dx=0.5, buf 2 insulating "
-                    "(sigma_vac/sigma_n=0.01), PCG tol 1e-10 (Float32 preconditioner, FP64 outer), 30-degree wedge, "
-                    "H=0.2, J=0.1, dt_scale=0.1, ladder cycle = 40 steps")
-    elif args.workload == "open_3band_256x128x32":
-        st, p = configs.open_3band_256x128x32()
-        workload = ("three-band anisotropic (m_z = 4 m_xy) TDGL, 256x128x32 cells, dx=0.5, buf 8 insulating "
-                    "(sigma_vac/sigma_n=0.01), PCG tol 1e-8 (Float32 preconditioner, FP64 outer), open boundary "
-                    "(FFT convolution, every t_GL), notch, H=0.2, J=0.1, ladder cycle = 40 steps")
-    elif args.workload == "pcg_wedge_96x48x24":
-        st, p = configs.pcg_wedge_96x48x24()
-        workload = ("the reference's validation run file pcg96.toml: mgb2_model_II at kappa=2, 96x48x24 cells, dx=0.5, "
-                    "buf 2 insulating (sigma_vac=0.01), PCG tol 1e-8 (Float32 preconditioner, FP64 outer), WEDGE, "
-                    "J=0.3, H=0.03, ladder cycle = 40 steps")
-    elif args.workload == "plain_wedge_96x48x24":
-        st, p = configs.plain_wedge_96x48x24()
-        workload = ("the reference's validation run file plain96.toml: mgb2_model_II at kappa=2, 96x48x24 cells, "
-                    "dx=0.5, buf 1, separable implicit field solve, WEDGE, J=0.3, H=0.03, ladder cycle = 40 steps")
+WORKLOADS = {
+    "open_3band_256x128x32": (
+        lambda: configs.open_3band_256x128x32(),
+        "three-band anisotropic (m_z = 4 m_xy) TDGL, 256x128x32 cells, dx=0.5, buf 8 insulating (sigma_vac/sigma_n=0.01), "
+        "PCG tol 1e-8 (Float32 preconditioner, FP64 outer), open boundary (FFT convolution, every t_GL), notch, H=0.2, "
+        "J=0.1, ladder cycle = 40 steps", "notch mask"),
+    "kwt_pcg_128x64x16": (
+        lambda: configs.kwt_pcg_128x64x16(),
+        "two-band KWT (u=Gamma/sigma_n=5.79, gkwt=10) TDGL, 128x64x16 cells, dx=0.5, buf 2 insulating "
+        "(sigma_vac/sigma_n=0.01), PCG tol 1e-10 (Float32 preconditioner, FP64 outer), 30-degree wedge, H=0.2, J=0.1, "
+        "dt_scale=0.1, ladder cycle = 40 steps", "30-degree sloped wedge mask"),
+    "pcg_wedge_96x48x24": (
+        lambda: configs.pcg_wedge_96x48x24(),
+        "the reference's validation run file pcg96.toml: mgb2_model_II at kappa=2, 96x48x24 cells, dx=0.5, buf 2 "
+        "insulating (sigma_vac=0.01), PCG tol 1e-8 (Float32 preconditioner, FP64 outer), WEDGE, J=0.3, H=0.03, ladder "
+        "cycle = 40 steps", "catalogue WEDGE mask"),
+    "plain_wedge_96x48x24": (
+        lambda: configs.plain_wedge_96x48x24(),
+        "the reference's validation run file plain96.toml: mgb2_model_II at kappa=2, 96x48x24 cells, dx=0.5, buf 1, "
+        "separable implicit field solve, WEDGE, J=0.3, H=0.03, ladder cycle = 40 steps", "catalogue WEDGE mask"),
+}
+# the reference's published T4 figures for its validation run files (ms per time step, median of three batches of
+# 20 steps; gpu/TDGLGPU/validation/README.md:41-71)
+PUBLISHED = {"pcg_wedge_96x48x24": {"t4_batch_ms_per_step": 26.903, "t4_cycle_ms_per_step": 21.040,
+                                    "laptop_1core_ms_per_step": 146.370},
+             "plain_wedge_96x48x24": {"t4_batch_ms_per_step": 7.295, "t4_cycle_ms_per_step": 4.816,
+                                      "laptop_1core_ms_per_step": 22.574}}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads() -> int:
+    """The host threads this process may use (its affinity set), capped at 16 (the box's CPU share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_sample(name, st, dev, H, HJ, nsteps):
+    """The oracle's legs of the CPU baseline: one thread, then every host thread.  On the workloads
+    whose whole grid the oracle steps in seconds, `nsteps` steps from the device's state and PCG
+    history after the warm-up.  On the 256x128x32 workload one oracle step takes minutes (its dense
+    separable transforms run on the CPU's vector units, and its open-boundary convolution is a
+    direct sum), so the sample is the same workload on 64x32x8 cells: two untimed steps (the cold
+    PCG start and the first open-boundary recomputation, which recurs every 39 steps), then
+    `nsteps` timed ones, scaled by the sample's own cells."""
+    from oracle.oracle import Oracle, set_threads
+    if name == "open_3band_256x128x32":
+        st2, p2 = configs.open_3band_256x128x32(nx=64, ny=32, nz=8)
+        H2, HJ2 = p2["H"], st2.selffield(p2["J"])
+        s0 = seeded_state(st2.model, st2.grid, st2.masks, H2, seed=20261202)
+        pre = Oracle(st2)
+        pre.set_state(s0)
+        set_threads(host_threads())
+        pre.step(2, H2, HJ2)
+        start, pcg, opn = pre.get_state(), pre.get_pcg(), pre.open_state()
+        sample = (f"the same workload on 64x32x8 cells (buf 8: {st2.grid.Nx}x{st2.grid.Ny}x{st2.grid.Nz}), {nsteps} time "
+                  f"steps after two untimed ones, scaled by its {st2.grid.cells} cells; the open boundary's recomputation "
+                  f"(every 39 steps) is outside the timed steps")
+        stx, Hx, HJx, cells = st2, H2, HJ2, st2.grid.cells
     else:
-        raise SystemExit(f"unknown workload {args.workload}")
-    # the reference's published T4 figures for these run files (ms per time step, median of three
-    # batches of 20 steps; gpu/TDGLGPU/validation/README.md:41-71)
-    published = {"pcg_wedge_96x48x24": {"t4_batch_ms_per_step": 26.903, "t4_cycle_ms_per_step": 21.040,
-                                        "laptop_1core_ms_per_step": 146.370},
-                 "plain_wedge_96x48x24": {"t4_batch_ms_per_step": 7.295, "t4_cycle_ms_per_step": 4.816,
-                                          "laptop_1core_ms_per_step": 22.574}}.get(args.workload)
-    if args.copy_probe:
-        copy_probe(args.copy_probe)
-    H, HJ = p["H"], st.selffield(p["J"])
-    s0 = seeded_state(st.model, st.grid, st.masks, H, seed=20261202)
-    dev = HipStepper(st)
-    dev.upload(s0)
-    print(f"[bench] {args.workload}: dt={st.dt:.5f} stages={st.rkl} nsub={st.nsub} grid={st.grid.Nx}x{st.grid.Ny}x"
-          f"{st.grid.Nz} padded", file=sys.stderr, flush=True)
-    for q in range(args.warmup):
-        dev.step(LMAX, H, HJ)
-        print(f"[bench] warmup cycle {q + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-    st0 = dev.stats()
-    t = time.perf_counter()
-    for q in range(args.steps):
-        dev.step(LMAX, H, HJ)
-    wall = time.perf_counter() - t
-    st1 = dev.stats()
-    chk = dev.download()
-    gmax = max(float(np.abs(g).max()) for g in chk.G)
-    if not np.isfinite(gmax) or gmax > 5:
-        print(f"[bench] the state diverged (max|psi| = {gmax})", file=sys.stderr)
-        return 3
-    dev_ms = st1["device_ms"] - st0["device_ms"]
-    iters = st1["pcg_iters"] - st0["pcg_iters"]
-    rollbacks = st1["rollbacks"] - st0["rollbacks"]
-    captures = st1["captures"] - st0["captures"]
-    nsteps = args.steps * LMAX
-    print(f"[bench] timed {args.steps} cycles: wall {wall:.3f}s device {dev_ms / 1e3:.3f}s, pcg iters/step "
-          f"{iters / nsteps:.2f}, roll backs {rollbacks}, graph captures {captures}", file=sys.stderr, flush=True)
-    cells = st.grid.cells
-    value = cells * args.steps / wall
-    # per-kernel device time: the same number of cycles again with HIP events around every launch
-    dev.profile(True)
-    for q in range(args.steps):
-        dev.step(LMAX, H, HJ)
-    prof = dev.profile_get()
-    dev.profile(False)
-    tot = sum(ms for _, ms in prof.values())
-    rows = sorted(prof.items(), key=lambda kv: -kv[1][1])
-    for name, (nl, ms) in rows:
-        if nl:
-            print(f"[bench]   {name:20s} {nl:8d} launches {ms:10.3f} ms ({100 * ms / tot:5.1f}%) "
-                  f"{1e3 * ms / nl:8.2f} us/launch", file=sys.stderr)
-    dom, (nl, ms) = rows[0]
+        start = dev.download()
+        pcg = dev.get_pcg() if st.field_mode == 2 else None
+        opn = dev.get_open() if st.selffield_bc == "open" else None
+        sample = f"{nsteps} time steps (of 40 per cycle) of the same workload from the GPU's state after the warm-up"
+        stx, Hx, HJx, cells = st, H, HJ, st.grid.cells
+    legs = []
+    for threads in (1, host_threads()):
+        set_threads(threads)
+        o = Oracle(stx)
+        o.set_state(start)
+        if pcg is not None:
+            o.set_pcg(pcg)
+        if opn is not None:
+            o.set_open(*opn)
+        t = time.perf_counter()
+        o.step(nsteps, Hx, HJx)
+        dt = time.perf_counter() - t
+        legs.append((threads, cells * (nsteps / LMAX) / dt, dt))
+        print(f"[bench] cpu oracle {threads} thread(s): {legs[-1][1]:.3e} cell-cycles/s ({dt:.1f}s for {nsteps} steps)",
+              file=sys.stderr, flush=True)
+    (t1, v1, d1), (tn, vn, dn) = legs
+    model = cpu_model()
+    return {"value": vn, "unit": "cell-cycles/s", "cores": tn, "kind": "port",
+            "sample": f"{sample}; C++ oracle with OpenMP over (j, k) rows, {tn} threads, {dn:.1f} s; host {model}, "
+                      f"{os.cpu_count()} logical cpus",
+            "single_thread": {"value": v1, "cores": 1, "seconds": d1}, "cpu_model": model}
+
+
+def roofline(st, dom, nl, ms, nsteps, iters, workload):
     avg_s = ms / nl / 1e3
     byts = kernel_bytes(st, dom, nl, nsteps, {"iters_per_solve": iters / nsteps}) / nl
     flops = kernel_flops(st, dom, nl) / nl
@@ -297,40 +290,131 @@ def main(argv=None):
                 "bytes_per_launch": byts, "avg_launch_us": avg_s * 1e6,
                 "note": ("fused x-y transform launch of the Float32 preconditioner on v_mfma_f32_16x16x4f32"
                          if dom == "sep_transform_f32" else
-                         "fused x-y transform launch of the FP64 separable solve on the vector ALUs")
+                         "fused x-y transform launch of the FP64 separable solve")
                         + " (2 (Lx + Ly) flop per element); durations from HIP events recorded at each launch's "
                           "start and end over the same number of cycles as the timed region"}
-    traffic = measured_traffic(dom, args.workload)
-    roof["traffic"] = traffic
-    if args.profile_out:
-        with open(args.profile_out, "w") as fh:
-            json.dump({k: {"launches": v[0], "ms": v[1]} for k, v in prof.items()}, fh, indent=1)
-    cpu = None
-    if not args.no_cpu:
-        s1 = dev.download()
-        v1, t1 = cpu_baseline(st, s1, H, HJ, args.cpu_steps, 1)
-        cpu = {"value": v1, "unit": "cell-cycles/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} time steps (of 40 per cycle) of the same workload from the GPU's state "
-                         f"after warmup, C++ oracle single-threaded, {t1:.1f} s; host {os.cpu_count()} logical cpus"}
-        print(f"[bench] cpu oracle 1 thread: {v1:.3e} cell-cycles/s ({t1:.1f}s for {args.cpu_steps} steps)",
-              file=sys.stderr, flush=True)
+    roof["traffic"] = measured_traffic(dom, workload)
+    return roof
+
+
+def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=None):
+    """Warm-up and timed ladder cycles of one workload with its state resident on the device; then,
+    with `profile`, the same number of cycles again with HIP events around every launch (the
+    roofline kernel), and with `cpu_steps` the CPU legs."""
+    make, desc, mask = WORKLOADS[name]
+    st, p = make()
+    H, HJ = p["H"], st.selffield(p["J"])
+    s0 = seeded_state(st.model, st.grid, st.masks, H, seed=20261202)
+    dev = HipStepper(st)
+    dev.upload(s0)
+    print(f"[bench] {name}: dt={st.dt:.5f} stages={st.rkl} nsub={st.nsub} grid={st.grid.Nx}x{st.grid.Ny}x"
+          f"{st.grid.Nz} padded", file=sys.stderr, flush=True)
+    for q in range(warmup):
+        dev.step(LMAX, H, HJ)
+        print(f"[bench] warmup cycle {q + 1}/{warmup} done", file=sys.stderr, flush=True)
+    st0 = dev.stats()
+    t = time.perf_counter()
+    for q in range(steps):
+        dev.step(LMAX, H, HJ)
+    wall = time.perf_counter() - t
+    st1 = dev.stats()
+    chk = dev.download()
+    gmax = max(float(np.abs(g).max()) for g in chk.G)
+    if not np.isfinite(gmax) or gmax > 5:
+        raise RuntimeError(f"{name}: the state diverged (max|psi| = {gmax})")
+    dev_ms = st1["device_ms"] - st0["device_ms"]
+    iters = st1["pcg_iters"] - st0["pcg_iters"]
+    rollbacks = st1["rollbacks"] - st0["rollbacks"]
+    captures = st1["captures"] - st0["captures"]
+    nsteps = steps * LMAX
+    print(f"[bench] {name}: timed {steps} cycles: wall {wall:.3f}s device {dev_ms / 1e3:.3f}s, pcg iters/step "
+          f"{iters / nsteps:.2f}, roll backs {rollbacks}, graph captures {captures}", file=sys.stderr, flush=True)
+    cells = st.grid.cells
+    value = cells * steps / wall
     nodes = (st.grid.Nx + 1) * (st.grid.Ny + 1) * (st.grid.Nz + 1)
     balg = cycle_bytes_per_node_step(st, iters / nsteps)
-    hbm_cycle = balg * nodes * LMAX * args.steps / wall / HBM_PEAK
+    out = {"value": value, "wall": wall, "steps": steps, "st": st, "nsteps": nsteps, "iters": iters,
+           "config": {"workload": desc, "cells": cells, "lmax": LMAX, "pcg_iters_per_step": iters / nsteps,
+                      "device_ms_per_cycle": dev_ms / steps, "padded_nodes": nodes, "cell_steps_per_s": value * LMAX,
+                      "b_alg_per_node_step": balg,
+                      "hbm_roofline_frac_cycle": balg * nodes * LMAX * steps / wall / HBM_PEAK,
+                      "ms_per_time_step": 1e3 * wall / nsteps, "timed_rollbacks": rollbacks,
+                      "timed_graph_captures": captures},
+           "mask": mask, "roofline": None, "cpu": None}
+    if profile:
+        dev.profile(True)
+        for q in range(steps):
+            dev.step(LMAX, H, HJ)
+        prof = dev.profile_get()
+        dev.profile(False)
+        tot = sum(ms for _, ms in prof.values())
+        rows = sorted(prof.items(), key=lambda kv: -kv[1][1])
+        for kname, (nl, ms) in rows:
+            if nl:
+                print(f"[bench]   {kname:20s} {nl:8d} launches {ms:10.3f} ms ({100 * ms / tot:5.1f}%) "
+                      f"{1e3 * ms / nl:8.2f} us/launch", file=sys.stderr)
+        dom, (nl, ms) = rows[0]
+        out["roofline"] = roofline(st, dom, nl, ms, nsteps, iters, name)
+        out["config"]["kernel_ms_per_cycle"] = {k: v[1] / steps for k, v in prof.items() if v[0]}
+        if profile_out:
+            with open(profile_out, "w") as fh:
+                json.dump({k: {"launches": v[0], "ms": v[1]} for k, v in prof.items()}, fh, indent=1)
+    if cpu_steps:
+        out["cpu"] = cpu_sample(name, st, dev, H, HJ, cpu_steps)
+    dev.close()
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed ladder cycles")
+    ap.add_argument("--warmup", type=int, default=5,
+                    help="untimed ladder cycles (the PCG budget settles and its graph is captured in them)")
+    ap.add_argument("--workload", default="open_3band_256x128x32", choices=tuple(WORKLOADS) + ("sweep64",))
+    ap.add_argument("--workers", type=int, default=8, help="sweep64: runs in flight at once")
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="time steps of the CPU baseline sample (default 8)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="leave out the 128x64x16 KWT workload that the 256x128x32 line carries beside it")
+    ap.add_argument("--profile-out", default=None, help="write the per-kernel profile (JSON) here")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy a MIB-MiB device buffer (a reference kernel for the counters' scale)")
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: a run is one device's work and this build has no multi-GPU path "
+              "(--gpus 1 only)", file=sys.stderr)
+        return 2
+    if device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        return 1
+    if args.workload == "sweep64":
+        return bench_sweep(args)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+    cpu_steps = 0 if args.no_cpu else (args.cpu_steps or 8)
+    r = run_workload(args.workload, args.steps, args.warmup, profile=True, cpu_steps=cpu_steps,
+                     profile_out=args.profile_out)
+    published = PUBLISHED.get(args.workload)
+    cfg = dict(r["config"])
+    if published:
+        cfg["reference_published"] = published
+    if args.workload == "open_3band_256x128x32" and not args.no_secondary:
+        sec = run_workload("kwt_pcg_128x64x16", min(args.steps, 10), 3, profile=False)
+        cfg["secondary"] = {"kwt_pcg_128x64x16": {"cell_cycles_per_s": sec["value"],
+                                                  "ms_per_cycle": 1e3 * sec["wall"] / sec["steps"],
+                                                  **{k: sec["config"][k] for k in ("workload", "cells",
+                                                                                    "pcg_iters_per_step",
+                                                                                    "hbm_roofline_frac_cycle")}}}
+    value = r["value"]
+    st = r["st"]
     out = {"metric": "cell-cycles/s", "value": value, "unit": "cell-cycles/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+           "warmup": args.warmup, "ms_per_step": 1e3 * r["wall"] / args.steps, "higher_is_better": True,
            "scaling": "weak",
-           "vs_baseline": (value / (st.grid.cells / (published["t4_cycle_ms_per_step"] * 1e-3) / LMAX)) if published else None, "dtype": "f64",
-           "data": "synthetic (seeded psi = 1 + 0.01 (n1 + i n2)), " + {"kwt_pcg_128x64x16": "30-degree sloped wedge mask",
-                                                                         "open_3band_256x128x32": "notch mask"}.get(
-               args.workload, "catalogue WEDGE mask"),
-           "config": {"workload": workload, "cells": cells, "lmax": LMAX, "pcg_iters_per_step": iters / nsteps,
-                      "device_ms_per_cycle": dev_ms / args.steps, "padded_nodes": nodes,
-                      "cell_steps_per_s": value * LMAX, "b_alg_per_node_step": balg,
-                      "hbm_roofline_frac_cycle": hbm_cycle, "ms_per_time_step": 1e3 * wall / nsteps,
-                      "timed_rollbacks": rollbacks, "timed_graph_captures": captures,
-                      **({"reference_published": published} if published else {})},
-           "roofline": roof, "cpu_baseline": cpu}
+           "vs_baseline": (value / (st.grid.cells / (published["t4_cycle_ms_per_step"] * 1e-3) / LMAX)) if published else None,
+           "dtype": "f64", "data": "synthetic (seeded psi = 1 + 0.01 (n1 + i n2)), " + r["mask"],
+           "config": cfg, "roofline": r["roofline"], "cpu_baseline": r["cpu"]}
     print(json.dumps(out))
     return 0
 

@@ -1755,6 +1755,17 @@ int orc_open_state(orc *o, double *hold, double *hnew) {
   }
   return o->ocount;
 }
+// the clock and ramp of the open boundary (restore_open!, io.jl:163-171)
+void orc_set_open(orc *o, int count, const double *hold, const double *hnew) {
+  if (o->d.selffield_bc != TDGL_BC_OPEN) return;
+  size_t q = 0;
+  for (int c = 0; c < 3; ++c) {
+    std::memcpy(o->oold[c].v.data(), hold + q, o->oold[c].size() * sizeof(double));
+    std::memcpy(o->onew[c].v.data(), hnew + q, o->onew[c].size() * sizeof(double));
+    q += o->oold[c].size();
+  }
+  o->ocount = count;
+}
 // θ_δ and δJ of the last recomputation (link-shaped x, y, z each)
 void orc_open_fields(orc *o, double *theta, double *dj) {
   size_t q = 0;

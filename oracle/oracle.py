@@ -54,6 +54,7 @@ def lib():
         L.orc_open_state.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.orc_open_state.restype = ct.c_int
         L.orc_open_fields.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.orc_set_open.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p]
         L.orc_noise_get.argtypes = [ct.c_void_p]
         L.orc_noise_get.restype = ct.c_uint64
         L.orc_noise_set.argtypes = [ct.c_void_p, ct.c_uint64]
@@ -174,6 +175,12 @@ class Oracle:
         a, b = np.zeros(n), np.zeros(n)
         cnt = lib().orc_open_state(self.h, a.ctypes.data, b.ctypes.data)
         return cnt, _split(a, sh), _split(b, sh)
+
+    def set_open(self, count: int, old, new) -> None:
+        """The clock and ramp of the open boundary (as open_state returns them)."""
+        a = np.concatenate([np.asarray(x, dtype=np.float64).ravel(order="F") for x in old])
+        b = np.concatenate([np.asarray(x, dtype=np.float64).ravel(order="F") for x in new])
+        lib().orc_set_open(self.h, int(count), a.ctypes.data, b.ctypes.data)
 
     def open_fields(self):
         """θ_δ and δJ of the last recomputation, each the three link-shaped arrays."""
