@@ -51,14 +51,24 @@ def kernel_bytes(st, name: str, launches: int, nsteps: int, stats: dict) -> floa
     if name == "sep_transform_f64":
         return 16.0 * rng * launches
     if name == "sep_transform_f32":
+        if stats.get("folded"):
+            # the quarter transforms: Float32 quarters in and out, 8 B per quarter element
+            return 8.0 * quarter_total(gr) * launches
         # per PCG solve of `it` iterations: it + 1 forward launches (FP64 in, FP32 out: 12 B per
         # element), the first inverse launch (FP32 in, FP64 out, r read for r·z: 20 B) and it
         # inverse launches of the flexible iteration (r and z_old read: 28 B)
         it = stats.get("iters_per_solve", 0.0)
         per = (12.0 * (it + 1) + 20.0 + 28.0 * it) / (2.0 * (it + 1))
         return per * rng * launches
-    if name == "pcg_apply_A":
-        return 24.0 * rng * launches
+    if name == "sep_fold":
+        # the fold (FP64 in, FP32 quarters out: 12 B) and the unfold (FP32 in, FP64 z out, r and
+        # z_old read for the dots: 28 B), one each per preconditioner
+        return 20.0 * rng * launches
+    if name == "sep_modes":
+        return 8.0 * rng * launches
+    if name in ("pcg_apply_A", "pcg_vector"):
+        # apply_A: z, P_old, D read, P, z_old, Q written; vector: x, P, r, Q read, x, r written
+        return 48.0 * rng * launches
     if name == "current_field":
         return (16 * N + 48 + 24 * 6 + 24 + 24) * nodes * launches
     if name == "links_curl_bc":
@@ -70,10 +80,22 @@ def ranges(gr):
     return [(gr.Nx - 2, gr.Ny - 1, gr.Nz - 1), (gr.Nx - 1, gr.Ny - 2, gr.Nz - 1), (gr.Nx - 1, gr.Ny - 1, gr.Nz - 2)]
 
 
-def kernel_flops(st, name: str, launches: int) -> float:
+def quarters(gr):
+    """The parity quarters of the folded transforms: per component (ceil(Lx/2), ceil(Ly/2), Lz)."""
+    return [((ex[0] + 1) // 2, (ex[1] + 1) // 2, ex[2]) for ex in ranges(gr)]
+
+
+def quarter_total(gr):
+    return sum(4 * h0 * h1 * e2 for h0, h1, e2 in quarters(gr))
+
+
+def kernel_flops(st, name: str, launches: int, folded: bool = False) -> float:
     """Algorithmic flops of the launches of one kernel class: a fused x-y transform launch of the
     separable solve multiplies every element by an (Lx x Lx) then an (Ly x Ly) matrix, 2 (Lx + Ly)
-    flop per element."""
+    flop per element; folded, each of the four quarters of a component by its half-size bases,
+    2 (ceil(Lx/2) + ceil(Ly/2)) flop per quarter element (half the work)."""
+    if name == "sep_transform_f32" and folded:
+        return launches * sum(4 * 2.0 * h0 * h1 * e2 * (h0 + h1) for h0, h1, e2 in quarters(st.grid))
     if name.startswith("sep_transform"):
         per = sum(2.0 * ex[0] * ex[1] * ex[2] * (ex[0] + ex[1]) for ex in ranges(st.grid))
         return per * launches
@@ -275,10 +297,10 @@ def cpu_sample(name, st, dev, H, HJ, nsteps):
             "single_thread": {"value": v1, "cores": 1, "seconds": d1}, "cpu_model": model}
 
 
-def roofline(st, dom, nl, ms, nsteps, iters, workload):
+def roofline(st, dom, nl, ms, nsteps, iters, workload, folded=False):
     avg_s = ms / nl / 1e3
-    byts = kernel_bytes(st, dom, nl, nsteps, {"iters_per_solve": iters / nsteps}) / nl
-    flops = kernel_flops(st, dom, nl) / nl
+    byts = kernel_bytes(st, dom, nl, nsteps, {"iters_per_solve": iters / nsteps, "folded": folded}) / nl
+    flops = kernel_flops(st, dom, nl, folded) / nl
     peak = FP32_MFMA_PEAK if dom == "sep_transform_f32" else FP64_PEAK
     if byts > 0 and (flops == 0 or flops / byts < peak / HBM_PEAK):
         roof = {"bound": "hbm", "achieved": byts / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -288,11 +310,15 @@ def roofline(st, dom, nl, ms, nsteps, iters, workload):
         roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                 "frac": flops / avg_s / peak, "traffic": None, "kernel": dom, "flops_per_launch": flops,
                 "bytes_per_launch": byts, "avg_launch_us": avg_s * 1e6,
-                "note": ("fused x-y transform launch of the Float32 preconditioner on v_mfma_f32_16x16x4f32"
+                "note": (("x-y transform launch of the Float32 preconditioner's twelve parity quarters on "
+                          "v_mfma_f32_16x16x4f32 (half-size bases, 2 (ceil(Lx/2) + ceil(Ly/2)) flop per quarter "
+                          "element)" if folded else
+                          "fused x-y transform launch of the Float32 preconditioner on v_mfma_f32_16x16x4f32 "
+                          "(2 (Lx + Ly) flop per element)")
                          if dom == "sep_transform_f32" else
-                         "fused x-y transform launch of the FP64 separable solve")
-                        + " (2 (Lx + Ly) flop per element); durations from HIP events recorded at each launch's "
-                          "start and end over the same number of cycles as the timed region"}
+                         "fused x-y transform launch of the FP64 separable solve (2 (Lx + Ly) flop per element)")
+                        + "; durations from HIP events recorded at each launch's start and end over the same "
+                          "number of cycles as the timed region"}
     roof["traffic"] = measured_traffic(dom, workload)
     return roof
 
@@ -354,7 +380,8 @@ def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=Non
                 print(f"[bench]   {kname:20s} {nl:8d} launches {ms:10.3f} ms ({100 * ms / tot:5.1f}%) "
                       f"{1e3 * ms / nl:8.2f} us/launch", file=sys.stderr)
         dom, (nl, ms) = rows[0]
-        out["roofline"] = roofline(st, dom, nl, ms, nsteps, iters, name)
+        out["roofline"] = roofline(st, dom, nl, ms, nsteps, iters, name, bool(st1.get("folded", 0)))
+        out["config"]["folded_preconditioner"] = bool(st1.get("folded", 0))
         out["config"]["kernel_ms_per_cycle"] = {k: v[1] / steps for k, v in prof.items() if v[0]}
         if profile_out:
             with open(profile_out, "w") as fh:

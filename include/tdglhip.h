@@ -199,6 +199,7 @@ typedef struct {
   int64_t rollbacks;    /* captured cycles redone eagerly (a solve needed more iterations) */
   int32_t pcg_budget;   /* PCG iterations per solve in the captured cycle */
   int64_t captures;     /* graphs captured and instantiated (the last four are kept and replayed) */
+  int32_t folded;       /* 1: the Float32 preconditioner runs its parity-folded transforms */
 } tdgl_stats;
 
 typedef struct tdgl_handle tdgl_handle;
@@ -312,7 +313,7 @@ int tdgl_get_stats(tdgl_handle *h, tdgl_stats *st);
  * While enabled, steps are launched from the host and every launch records a pair of HIP events
  * at the kernel's own start and end; the totals are per kernel class (tdgl_kernel_name).
  * Enabling clears them. */
-#define TDGL_NKERNELS 14
+#define TDGL_NKERNELS 15
 typedef struct {
   int64_t launches[TDGL_NKERNELS];
   double ms[TDGL_NKERNELS];

@@ -80,10 +80,10 @@ class StatsC(ct.Structure):
     _fields_ = [("steps", ct.c_int64), ("calls", ct.c_int64), ("pcg_iters", ct.c_int64),
                 ("last_iters", ct.c_int32), ("device_ms", ct.c_double), ("device_bytes", ct.c_int64),
                 ("graph_cycles", ct.c_int64), ("rollbacks", ct.c_int64), ("pcg_budget", ct.c_int32),
-                ("captures", ct.c_int64)]
+                ("captures", ct.c_int64), ("folded", ct.c_int32)]
 
 
-NKERNELS = 14
+NKERNELS = 15
 
 
 class MeterC(ct.Structure):

@@ -259,7 +259,8 @@ hipError_t launch_tx2_modes(const Tx2Modes &m, const int *gate, hipStream_t s);
 // order, the x result rounded to T as a pass of its own would store it).  One block per
 // (component, plane, slice of PLANE_MS(T) rows of m).  The y extent is at most PLANE_LMAX.
 constexpr int PLANE_LMAX = 256;
-constexpr int PLANE_MAXC = 4;  // components (the three of a link vector, or the modes of a potential)
+constexpr int PLANE_MAXC = 12;  // components (the three of a link vector, the modes of a potential, or the
+                                // four parity quarters of each of the three components)
 struct PlanePass {
   int ncomp;                   // 0 means 3
   int one;                     // Float32: the one-round kernel where it pays (1), forced with 2 or 3 m tiles, off (0)
@@ -280,6 +281,55 @@ template <class T>
 int plane_parts(const PlanePass &p);
 template <class T>
 bool plane_fits(const int e[][3], int ncomp);
+
+// ---------------------------------------------------------------- the parity-folded transforms
+// Every column of the separable bases is even or odd under the mirror i -> L-1-i of its axis (the
+// DCT-II and DST-I columns of parity index p carry the sign (-1)^p), so a transform of length L
+// splits into two of length ceil(L/2): the even columns act on the folded sums a[i] + a[L-1-i],
+// the odd ones on the differences (one radix-2 stage of the symmetric transform, exact for every
+// length, also the prime ones).  Folding x and y turns each component's x-y transform into four
+// quarter transforms with half-size bases.  A component's quarter (px, py) holds the element
+// (i, j, k) of its fold (or, after the forward transforms, the mode (2i+px, 2j+py, k)) at
+// off[c][2 px + py] + i + h[c][0] (j + h[c][1] k), i < h[c][0] = ceil(Lx/2), j < h[c][1], k < e2[c].
+struct Quarters {
+  int L[3][2];     // the components' x and y extents
+  int h[3][2];     // ceil halves
+  int e2[3];       // planes
+  int off[3][4];
+  int total;
+};
+// fold: the three components of a node-shaped FP64 vector into the quarters (FP32), with the
+// convergence check of a first preconditioner pass (conv, as the plane launches take it)
+struct FoldPass {
+  View in[3];
+  float *out;
+  Quarters q;
+  Conv conv;
+  PcgDev d;
+};
+hipError_t launch_fold(const FoldPass &f, const int *gate, hipStream_t s);
+// unfold: the quarters of the inverse x-y transforms back into a node-shaped FP64 vector, with the
+// partials of r·z (dot 1) and r·zold (dot 2) in PART_RZ
+struct UnfoldPass {
+  const float *in;
+  Quarters q;
+  View out[3], r[3], zold[3];
+  int dot;
+  PcgDev d;
+};
+hipError_t launch_unfold(const UnfoldPass &u, const int *gate, hipStream_t s);
+int fold_blocks(const Quarters &q);
+// the z pass, projector and inverse z pass on modes held in quarters, in place
+struct ZModesQ {
+  float *modes;
+  Quarters q;
+  const float *Az[2];    // forward z: [0] V_z, [1] U_z, as ZModes
+  const float *sx, *sy, *sz;
+  int nx, ny, nz;
+  double a;
+};
+hipError_t launch_zmodes_q(const ZModesQ &m, const int *gate, hipStream_t s);
+constexpr int ZQ_NZ_MAX = 64;  // z extents the quarter z pass takes (its bases in LDS)
 
 // The z pass, the per-wavevector projector and the inverse z pass in one launch: a block holds
 // whole z-columns of modes (floor(256 / nz) columns of nz threads), so the inverse pass reads
@@ -409,6 +459,7 @@ struct PcgArgs {
   const double *th[3];       // finish: φ = θ + x
   double *ph[3];
   int act_slot;      // finish: the flag that tells an unfinished solve
+  int tile;          // pap: the LDS-tiled kernel (k_pcg_pap_tile), else one gather per stencil value
 };
 int pcg_box_blocks(const Geo &g, int kc);
 int box_kc(const Geo &g);

@@ -1956,6 +1956,7 @@ hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
 template hipError_t launch_plane<double, float, float>(const PlanePass &, const int *, hipStream_t);
 template hipError_t launch_plane<float, float, double>(const PlanePass &, const int *, hipStream_t);
 template hipError_t launch_plane<double, double, double>(const PlanePass &, const int *, hipStream_t);
+template hipError_t launch_plane<float, float, float>(const PlanePass &, const int *, hipStream_t);
 template int plane_parts<float>(const PlanePass &);
 template int plane_parts<double>(const PlanePass &);
 template bool plane_fits<float>(const int e[][3], int);
@@ -2158,6 +2159,258 @@ __global__ void __launch_bounds__(256) k_zmodes_run(ZModes m, const int *gate) {
     if (hy) out[m.off[1] + mx + (size_t)nx * ((my - 1) + (size_t)(ny - 1) * n)] = oy;
     if (n < nz - 1) out[m.off[2] + mx + (size_t)nx * (my + (size_t)ny * n)] = oz;
   }
+}
+
+// ---------------------------------------------------------------------------- parity folds
+// Blocks of 64 x 4 threads: 64 consecutive i of one j row of one component, four groups of k.
+// Thread (i, j) folds the four mirror images (i, j), (L0-1-i, j), (i, L1-1-j), (L0-1-i, L1-1-j) of
+// every k (a centre row or column, on an odd length, is its own image: it goes to the even
+// quarter alone).
+__device__ __forceinline__ void fold_index(const Quarters &q, int &c, int &i, int &j, int &kg) {
+  c = blockIdx.z;
+  j = blockIdx.y;
+  i = blockIdx.x * 64 + (int)(threadIdx.x & 63);
+  kg = (int)(threadIdx.x >> 6);
+}
+int fold_blocks(const Quarters &q) {
+  int hx = 1, hy = 1;
+  for (int c = 0; c < 3; ++c) {
+    hx = q.h[c][0] > hx ? q.h[c][0] : hx;
+    hy = q.h[c][1] > hy ? q.h[c][1] : hy;
+  }
+  return cdiv(hx, 64) * hy * 3;
+}
+static dim3 fold_grid(const Quarters &q) {
+  int hx = 1, hy = 1;
+  for (int c = 0; c < 3; ++c) {
+    hx = q.h[c][0] > hx ? q.h[c][0] : hx;
+    hy = q.h[c][1] > hy ? q.h[c][1] : hy;
+  }
+  return dim3(cdiv(hx, 64), hy, 3);
+}
+
+__global__ void __launch_bounds__(256) k_fold(FoldPass f, const int *gate) {
+  __shared__ double sh[8];
+  const int tid = threadIdx.x;
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (!pass_active(f.conv, gate, f.d, sh, tid, bid)) return;
+  int c, i, j, kg;
+  fold_index(f.q, c, i, j, kg);
+  const int L0 = f.q.L[c][0], L1 = f.q.L[c][1], h0 = f.q.h[c][0], h1 = f.q.h[c][1], e2 = f.q.e2[c];
+  if (i >= h0 || j >= h1) return;
+  const int is = L0 - 1 - i, js = L1 - 1 - j;
+  const bool ci = is == i, cj = js == j;
+  const View v = f.in[c];
+  const double *base = (const double *)v.p;
+  const size_t o_ij = (size_t)i + (size_t)v.s1 * j, o_sj = (size_t)is + (size_t)v.s1 * j;
+  const size_t o_is = (size_t)i + (size_t)v.s1 * js, o_ss = (size_t)is + (size_t)v.s1 * js;
+  const size_t qp = (size_t)i + (size_t)h0 * j, qs = (size_t)h0 * h1;
+  float *q00 = f.out + f.q.off[c][0] + qp, *q01 = f.out + f.q.off[c][1] + qp;
+  float *q10 = f.out + f.q.off[c][2] + qp, *q11 = f.out + f.q.off[c][3] + qp;
+  for (int k = kg; k < e2; k += 4) {
+    const double *pk = base + (size_t)v.s2 * k;
+    const double a = pk[o_ij], b = pk[o_sj], cc = pk[o_is], d = pk[o_ss];
+    const double sj = ci ? a : a + b, dj = ci ? 0.0 : a - b;
+    const double ss = ci ? cc : cc + d, ds = ci ? 0.0 : cc - d;
+    const size_t qk = qs * k;
+    q00[qk] = (float)(cj ? sj : sj + ss);
+    q01[qk] = (float)(cj ? 0.0 : sj - ss);
+    q10[qk] = (float)(cj ? dj : dj + ds);
+    q11[qk] = (float)(cj ? 0.0 : dj - ds);
+  }
+}
+
+hipError_t launch_fold(const FoldPass &f, const int *gate, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
+  if (fold_blocks(f.q) > MAX_PART) return hipErrorInvalidValue;
+  TDGL_LAUNCH(k_fold, fold_grid(f.q), dim3(256), 0, s, f, gate);
+  return hipGetLastError();
+}
+
+// The inverse of the fold: the element at each of the four mirror images is the sum of the four
+// quarters, the odd quarter of an axis with the sign of the image (-1 on the mirrored side); the
+// products r·z and r·zold of the values written, partials per block.
+__global__ void __launch_bounds__(256) k_unfold(UnfoldPass u, const int *gate) {
+  if (gate && !*gate) return;
+  __shared__ double sh[8];
+  const int tid = threadIdx.x;
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  int c, i, j, kg;
+  fold_index(u.q, c, i, j, kg);
+  const int L0 = u.q.L[c][0], L1 = u.q.L[c][1], h0 = u.q.h[c][0], h1 = u.q.h[c][1], e2 = u.q.e2[c];
+  double d0 = 0.0, d1 = 0.0;
+  if (i < h0 && j < h1) {
+    const int is = L0 - 1 - i, js = L1 - 1 - j;
+    const bool ci = is == i, cj = js == j;
+    const size_t qp = (size_t)i + (size_t)h0 * j, qs = (size_t)h0 * h1;
+    const float *q00 = u.in + u.q.off[c][0] + qp, *q01 = u.in + u.q.off[c][1] + qp;
+    const float *q10 = u.in + u.q.off[c][2] + qp, *q11 = u.in + u.q.off[c][3] + qp;
+    const View vo = u.out[c];
+    const size_t o[4] = {(size_t)i + (size_t)vo.s1 * j, (size_t)is + (size_t)vo.s1 * j, (size_t)i + (size_t)vo.s1 * js,
+                         (size_t)is + (size_t)vo.s1 * js};
+    const bool w[4] = {true, !ci, !cj, !ci && !cj};
+    for (int k = kg; k < e2; k += 4) {
+      const size_t qk = qs * k;
+      const double r00 = q00[qk], r01 = q01[qk], r10 = q10[qk], r11 = q11[qk];
+      const double val[4] = {(r00 + r10) + (r01 + r11), (r00 - r10) + (r01 - r11), (r00 + r10) - (r01 + r11),
+                             (r00 - r10) - (r01 - r11)};
+      const size_t zk = (size_t)vo.s2 * k;
+      double rv[4] = {0, 0, 0, 0}, zv[4] = {0, 0, 0, 0};
+      if (u.dot)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          if (!w[t]) continue;
+          rv[t] = ((const double *)u.r[c].p)[o[t] + zk];
+          if (u.dot == 2) zv[t] = ((const double *)u.zold[c].p)[o[t] + zk];
+        }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (!w[t]) continue;
+        ((double *)vo.p)[o[t] + zk] = val[t];
+        if (u.dot) {
+          d0 += rv[t] * val[t];
+          if (u.dot == 2) d1 += rv[t] * zv[t];
+        }
+      }
+    }
+  }
+  if (u.dot) {
+    block_sum2_t(d0, d1, sh, tid);
+    if (tid == 0) {
+      double *pr = region_w(u.d, PART_RZ);
+      pr[2 * bid] = d0;
+      pr[2 * bid + 1] = d1;
+    }
+  }
+}
+
+hipError_t launch_unfold(const UnfoldPass &u, const int *gate, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
+  if (fold_blocks(u.q) > MAX_PART) return hipErrorInvalidValue;
+  TDGL_LAUNCH(k_unfold, fold_grid(u.q), dim3(256), 0, s, u, gate);
+  return hipGetLastError();
+}
+
+// The z pass and projector of k_zmodes_run on modes held in quarters.  A block owns TX modes
+// (mx = 2 mm + px, my = 2 qq + py, all mz) of one parity quarter (px, py): Z of those modes sits in
+// Z's quarter (px, py) at (mm, qq), Y (at my - 1) in Y's quarter (px, 1 - py) at (mm, qq - 1 + py),
+// X (at mx - 1) in X's quarter (1 - px, py) at (mm - 1 + px, qq), each a contiguous run; the results
+// go back to the same places.
+template <int TX>
+__global__ void __launch_bounds__(256) k_zmodes_q(ZModesQ m, const int *gate) {
+  if (gate && !*gate) return;
+  using T = float;
+  constexpr int G = 256 / TX;
+  constexpr int MZ = (ZT_NZ_MAX + G - 1) / G;
+  extern __shared__ __align__(16) unsigned char zq_smem[];
+  const int nx = m.nx, ny = m.ny, nz = m.nz;
+  T *Cx = (T *)zq_smem, *Cy = Cx + nz * TX, *Cz = Cy + nz * TX;
+  T *AV = Cz + nz * TX, *AU = AV + nz * nz;
+  const int t = threadIdx.x, p = t % TX, g = t / TX;
+  const int px = (int)(blockIdx.z >> 1), py = (int)(blockIdx.z & 1);
+  const int mm = blockIdx.x * TX + p, qq = blockIdx.y;
+  const int mx = 2 * mm + px, my = 2 * qq + py;
+  const bool live = mx < nx && my < ny;
+  const bool hx = live && mx > 0, hy = live && my > 0;
+  const Quarters &q = m.q;
+  // positions in the three quarters (clamped into each quarter; the absent values selected after)
+  const int xi = min(max(mm - 1 + px, 0), q.h[0][0] - 1), xj = min(qq, q.h[0][1] - 1);
+  const int yi = min(mm, q.h[1][0] - 1), yj = min(max(qq - 1 + py, 0), q.h[1][1] - 1);
+  const int zi = min(mm, q.h[2][0] - 1), zj = min(qq, q.h[2][1] - 1);
+  T *colx = m.modes + q.off[0][2 * (1 - px) + py] + xi + (size_t)q.h[0][0] * xj;
+  T *coly = m.modes + q.off[1][2 * px + (1 - py)] + yi + (size_t)q.h[1][0] * yj;
+  T *colz = m.modes + q.off[2][2 * px + py] + zi + (size_t)q.h[2][0] * zj;
+  const size_t stx = (size_t)q.h[0][0] * q.h[0][1], sty = (size_t)q.h[1][0] * q.h[1][1], stz = (size_t)q.h[2][0] * q.h[2][1];
+  {
+    const T *gV = m.Az[0], *gU = m.Az[1];
+    for (int e = t; e < nz * nz; e += 256) AV[e] = gV[e];
+    for (int e = t; e < (nz - 1) * (nz - 1); e += 256) AU[e] = gU[e];
+  }
+  for (int z = g; z < nz; z += G) {
+    const T vx = colx[stx * z], vy = coly[sty * z], vz = colz[stz * min(z, nz - 2)];
+    Cx[z * TX + p] = hx ? vx : T(0);
+    Cy[z * TX + p] = hy ? vy : T(0);
+    Cz[z * TX + p] = (live && z < nz - 1) ? vz : T(0);
+  }
+  __syncthreads();
+  T ax[MZ], ay[MZ], az[MZ];
+#pragma unroll
+  for (int r = 0; r < MZ; ++r) ax[r] = ay[r] = az[r] = T(0);
+  for (int n = 0; n < nz - 1; ++n) {
+    const T cx = Cx[n * TX + p], cy = Cy[n * TX + p], cz = Cz[n * TX + p];
+#pragma unroll
+    for (int r = 0; r < MZ; ++r) {
+      const int mz = g + G * r;
+      if (mz < nz) {
+        const T v = AV[n * nz + mz];
+        ax[r] += cx * v;
+        ay[r] += cy * v;
+        az[r] += cz * AU[n * (nz - 1) + (mz > 0 ? mz - 1 : 0)];
+      }
+    }
+  }
+  {
+    const T cx = Cx[(nz - 1) * TX + p], cy = Cy[(nz - 1) * TX + p];
+#pragma unroll
+    for (int r = 0; r < MZ; ++r) {
+      const int mz = g + G * r;
+      if (mz < nz) {
+        const T v = AV[(nz - 1) * nz + mz];
+        ax[r] += cx * v;
+        ay[r] += cy * v;
+      }
+    }
+  }
+  __syncthreads();
+  if (live) {
+    const T k1 = m.sx[mx], k2 = m.sy[my];
+#pragma unroll
+    for (int r = 0; r < MZ; ++r) {
+      const int mz = g + G * r;
+      if (mz < nz) {
+        const T k3 = m.sz[mz];
+        const bool hz = mz > 0;
+        T x1 = hx ? ax[r] : T(0), y1 = hy ? ay[r] : T(0), z1 = hz ? az[r] : T(0);
+        const T k2s = k1 * k1 + k2 * k2 + k3 * k3;
+        if (k2s != 0) project<T>(k1, k2, k3, (T)m.a, x1, y1, z1, hx, hy, hz);
+        Cx[mz * TX + p] = x1;
+        Cy[mz * TX + p] = y1;
+        Cz[mz * TX + p] = z1;
+      }
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+#pragma unroll
+  for (int r = 0; r < MZ; ++r) {
+    const int n = g + G * r;
+    if (n >= nz) continue;
+    T ox = 0, oy = 0, oz = 0;
+    for (int qm = 0; qm < nz; ++qm) {
+      const T v = AV[n * nz + qm];
+      ox += Cx[qm * TX + p] * v;
+      oy += Cy[qm * TX + p] * v;
+    }
+    const int nu = n < nz - 1 ? n : 0;
+    for (int qm = 1; qm < nz; ++qm) oz += Cz[qm * TX + p] * AU[nu * (nz - 1) + (qm - 1)];
+    if (hx) colx[stx * n] = ox;
+    if (hy) coly[sty * n] = oy;
+    if (n < nz - 1) colz[stz * n] = oz;
+  }
+}
+
+hipError_t launch_zmodes_q(const ZModesQ &m, const int *gate, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
+  if (m.nz < 2 || m.nz > ZT_NZ_MAX || m.nx < 1 || m.ny < 1) return hipErrorInvalidValue;
+  const int hzx = m.q.h[2][0], hzy = m.q.h[2][1];
+  const bool wide = cdiv(hzx, 64) * hzy * 4 >= 512;
+  const int TX = wide ? 64 : 32;
+  const size_t ldr = (3 * (size_t)m.nz * TX + (size_t)m.nz * m.nz + (size_t)(m.nz - 1) * (m.nz - 1)) * sizeof(float);
+  if (wide)
+    TDGL_LAUNCH((k_zmodes_q<64>), dim3(cdiv(hzx, 64), hzy, 4), dim3(256), ldr, s, m, gate);
+  else
+    TDGL_LAUNCH((k_zmodes_q<32>), dim3(cdiv(hzx, 32), hzy, 4), dim3(256), ldr, s, m, gate);
+  return hipGetLastError();
 }
 
 constexpr int ZM_LB_MAX = 64;  // z bases in LDS up to this nz
@@ -2613,6 +2866,125 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap(PcgArgs a) {
   }
 }
 
+// The same iteration as k_pcg_pap over an LDS tile: the block's 64 x 4 nodes of one k level and
+// their halo (66 x 6) hold P = z + βPold of the three components for the levels k-1, k, k+1, so
+// that z and Pold are read once per node (and halo node) instead of once per stencil use; the next
+// level's plane is requested before the current level's operator is evaluated.  P at each node
+// is the value every old-kernel thread computed for it, the operator and the block's partials
+// take the same values in the same order: the results are the same bits.
+constexpr int PT_W = 66, PT_H = 6, PT_N = PT_W * PT_H;  // the tile with its halo
+__global__ void __launch_bounds__(RED_T) k_pcg_pap_tile(PcgArgs a) {
+  if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
+  __shared__ double sh[8];
+  __shared__ double Pl[3][3][PT_N];  // [component][level slot][row * 66 + col]
+  const int tid = threadIdx.x;
+  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const bool live = box_live(a.g, b0);
+  const int i0 = b0.i - (tid & 63) - 1, j0 = b0.j - (tid >> 6) - 1;  // the tile's first (halo) node
+  const bool pold = a.q > 0;
+  // this thread's two tile points (p = tid, tid + 256 when below 396) as node indices of level 1
+  int pidx[2];
+  bool pok[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int p = tid + u * RED_T;
+    pok[u] = p < PT_N;
+    const int col = p % PT_W, row = p / PT_W;
+    const int ii = min(max(i0 + col, 1), a.g.Nx + 1), jj = min(max(j0 + row, 1), a.g.Ny + 1);
+    pidx[u] = a.g.at(ii, jj, 1);
+  }
+  const int sz = a.g.px * a.g.py;
+  double rz_[2][3], rp_[2][3];  // a plane in flight: z and Pold of the two points, three components
+  auto fetch = [&](int kk) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int c = pidx[u] + sz * (kk - 1);
+#pragma unroll
+      for (int comp = 0; comp < 3; ++comp) {  // clamped addresses: every load in bounds
+        rz_[u][comp] = a.z.c[comp][c];
+        rp_[u][comp] = pold ? a.Pold.c[comp][c] : 0.0;
+      }
+    }
+  };
+  auto stash = [&](int kk, double beta) {
+    const int slot = kk % 3;
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      if (pok[u])
+#pragma unroll
+        for (int comp = 0; comp < 3; ++comp)
+          Pl[comp][slot][tid + u * RED_T] = pold ? rz_[u][comp] + beta * rp_[u][comp] : rz_[u][comp];
+  };
+  // β from the partials of r·z and r·zold, summed as k_pcg_pap sums them, while the first plane
+  // is in flight
+  const int kf = b0.k;  // the block's first level
+  fetch(kf - 1);
+  const int np = a.nparts_in;
+  const __amdgpu_buffer_rsrc_t rp = buf_rsrc(region(a.d, PART_RZ), 16 * np);
+  double rz2 = 0.0, rzo = 0.0;
+  for (int b1 = 0; b1 < np; b1 += 2 * RED_T)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = b1 + tid + u * RED_T;
+      const double2 v = buf_load_d2(rp, 16 * min(e, np - 1));
+      rz2 += e < np ? v.x : 0.0;
+      rzo += e < np ? v.y : 0.0;
+    }
+  block_sum2_t(rz2, rzo, sh, tid);
+  double beta = 0.0;
+  if (pold) {
+    const double rz = a.d.sc[SC_RZ0 + ((a.q + 1) & 1)];
+    beta = a.flexible ? (rz2 - rzo) / rz : rz2 / rz;
+  }
+  if (b0.bid == 0 && tid == 0) a.d.sc[SC_RZ0 + (a.q & 1)] = rz2;
+  stash(kf - 1, beta);
+  fetch(kf);
+  stash(kf, beta);
+  fetch(kf + 1);
+  stash(kf + 1, beta);
+  const int tx = (tid & 63) + 1, ty = (tid >> 6) + 1;
+  auto L = [&](int comp, int kk, int row, int col) { return Pl[comp][kk % 3][row * PT_W + col]; };
+  double s = 0.0;
+  for (int q = 0; q < b0.nk; ++q) {
+    const int k = kf + q;
+    __syncthreads();
+    const bool more = q + 1 < b0.nk;
+    if (more) fetch(k + 2);
+    if (live) {
+      const BoxIdx b = box_level(a.g, b0, q);
+      double v[NSTEN], qv[3];
+      v[P0C] = L(0, k, ty, tx), v[P0ZP] = L(0, k + 1, ty, tx), v[P0ZM] = L(0, k - 1, ty, tx);
+      v[P0XMZP] = L(0, k + 1, ty, tx - 1), v[P0XM] = L(0, k, ty, tx - 1), v[P0YP] = L(0, k, ty + 1, tx);
+      v[P0YM] = L(0, k, ty - 1, tx), v[P0XMYP] = L(0, k, ty + 1, tx - 1);
+      v[P1C] = L(1, k, ty, tx), v[P1ZP] = L(1, k + 1, ty, tx), v[P1ZM] = L(1, k - 1, ty, tx);
+      v[P1YMZP] = L(1, k + 1, ty - 1, tx), v[P1YM] = L(1, k, ty - 1, tx), v[P1XP] = L(1, k, ty, tx + 1);
+      v[P1YMXP] = L(1, k, ty - 1, tx + 1), v[P1XM] = L(1, k, ty, tx - 1);
+      v[P2C] = L(2, k, ty, tx), v[P2YP] = L(2, k, ty + 1, tx), v[P2ZMYP] = L(2, k - 1, ty + 1, tx);
+      v[P2ZM] = L(2, k - 1, ty, tx), v[P2YM] = L(2, k, ty - 1, tx), v[P2XP] = L(2, k, ty, tx + 1);
+      v[P2ZMXP] = L(2, k - 1, ty, tx + 1), v[P2XM] = L(2, k, ty, tx - 1);
+      const double dv[3] = {a.D[0][b.c], a.D[1][b.c], a.D[2][b.c]};
+      apply3v(a, v, dv, b.i, b.j, b.k, qv);
+      const int cc[3] = {P0C, P1C, P2C};
+      for (int comp = 0; comp < 3; ++comp) {
+        if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+        const double pn = v[cc[comp]];
+        a.Pnew.c[comp][b.c] = pn;
+        if (a.flexible) a.zold.c[comp][b.c] = a.z.c[comp][b.c];
+        a.Q.c[comp][b.c] = qv[comp];
+        s += pn * qv[comp];
+      }
+    }
+    __syncthreads();  // level k-1 is read: its slot takes level k+2
+    if (more) stash(k + 2, beta);
+  }
+  const double r0 = block_sum_t(s, sh, tid);
+  if (tid == 0) {
+    double *pr = region_w(a.d, PART_PQ);
+    pr[2 * b0.bid] = r0;
+    pr[2 * b0.bid + 1] = 0.0;
+  }
+}
+
 // iteration q: α = rz/pAp, x += αP, r −= αQ and the partials of r·r; IT += 1
 __global__ void __launch_bounds__(RED_T) k_pcg_xr(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
@@ -2635,13 +3007,18 @@ __global__ void __launch_bounds__(RED_T) k_pcg_xr(PcgArgs a) {
   if (live)
     for (int q = 0; q < b0.nk; ++q) {
       const BoxIdx b = box_level(a.g, b0, q);
-      if (q > 0)
+      // the next level's values are requested before this level's are used
+      double xn[3], pn[3], rn[3], qn[3];
+      const bool more = q + 1 < b0.nk;
+      if (more) {
+        const int cn = b.c + a.g.px * a.g.py;
         for (int comp = 0; comp < 3; ++comp) {
-          xv[comp] = a.x.c[comp][b.c];
-          pv[comp] = a.Pnew.c[comp][b.c];
-          rv[comp] = a.r.c[comp][b.c];
-          qv[comp] = a.Q.c[comp][b.c];
+          xn[comp] = a.x.c[comp][cn];
+          pn[comp] = a.Pnew.c[comp][cn];
+          rn[comp] = a.r.c[comp][cn];
+          qn[comp] = a.Q.c[comp][cn];
         }
+      }
       for (int comp = 0; comp < 3; ++comp) {
         if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
         const int c = b.c;
@@ -2650,6 +3027,13 @@ __global__ void __launch_bounds__(RED_T) k_pcg_xr(PcgArgs a) {
         a.r.c[comp][c] = r1;
         s += r1 * r1;
       }
+      if (more)
+        for (int comp = 0; comp < 3; ++comp) {
+          xv[comp] = xn[comp];
+          pv[comp] = pn[comp];
+          rv[comp] = rn[comp];
+          qv[comp] = qn[comp];
+        }
     }
   const double r0 = block_sum_t(s, sh, tid);
   if (tid == 0) {
@@ -2688,7 +3072,7 @@ hipError_t launch_pcg_start(const PcgArgs &a, hipStream_t s) { return launch_box
 hipError_t launch_pcg_resid(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_resid, a, s); }
 hipError_t launch_pcg_p(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_p, a, s); }
 hipError_t launch_pcg_ap(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_ap, a, s); }
-hipError_t launch_pcg_pap(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_pap, a, s); }
+hipError_t launch_pcg_pap(const PcgArgs &a, hipStream_t s) { return launch_box(a.tile ? k_pcg_pap_tile : k_pcg_pap, a, s); }
 hipError_t launch_pcg_xr(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_xr, a, s); }
 hipError_t launch_pcg_finish(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_finish, a, s); }
 

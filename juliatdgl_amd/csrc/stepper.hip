@@ -61,10 +61,11 @@ Bases make_bases(int nn) {
 }
 
 // kernel classes of the per-kernel profile (tdgl_profile_get)
-enum { K_STAGE = 0, K_QP, K_FIELD, K_CURL, K_GHOSTS, K_SCATTER, K_TX, K_TX32, K_MODES, K_APPLYA, K_DOT, K_FINISH, K_VEC, K_METER, K_COUNT };
+enum { K_STAGE = 0, K_QP, K_FIELD, K_CURL, K_GHOSTS, K_SCATTER, K_TX, K_TX32, K_MODES, K_APPLYA, K_DOT, K_FINISH, K_VEC, K_METER,
+       K_FOLD, K_COUNT };
 const char *const K_NAMES[K_COUNT] = {"band_stage", "normal_fluids", "current_field", "links_curl_bc", "neumann_ghosts",
                                       "solve_scatter", "sep_transform_f64", "sep_transform_f32", "sep_modes", "pcg_apply_A",
-                                      "pcg_dot", "pcg_finish", "pcg_vector", "cycle_meter"};
+                                      "pcg_dot", "pcg_finish", "pcg_vector", "cycle_meter", "sep_fold"};
 static_assert(K_COUNT == TDGL_NKERNELS, "kernel classes");
 
 // a device or pinned host block of the process's cache
@@ -144,6 +145,12 @@ struct tdgl_handle {
   double *mrec = nullptr, *mrung = nullptr;
   bool meter = false;
   bool plane32 = false, plane64 = false;  // the fused transform launches fit (per precision)
+  // the parity-folded Float32 preconditioner: quarters, their two buffers, the half-size bases
+  // [axis][0=V,1=U][parity][0=fwd,1=inv]
+  bool fold = false;
+  Quarters q{};
+  float *qa = nullptr, *qb = nullptr;
+  float *Hq[2][2][2][2] = {};
   bool tx_only = false;                   // the separable solve as single-axis passes (a mixed batch)
   int box_kc = 1;                         // levels of k per block of the PCG box kernels
   int plane1 = 1;                         // PlanePass::one of the Float32 launches
@@ -181,6 +188,7 @@ struct tdgl_handle {
   long long gtick = 0;
   long long captures = 0;  // graphs captured and instantiated
   int zcols = 0;           // TDGL_HIP_ZCOL=1: the z pass of whole columns per block (k_zmodes)
+  int paptile = 1;         // TDGL_HIP_PAPTILE=0: the PCG operator with a gather per stencil value (k_pcg_pap)
   double *bk[48] = {};     // device copy of a cycle's start (roll back)
   // thermal noise: the stream's step counter on the device ([0] steps taken, [1] the step of the
   // first of the caller's draws) and its host copy; the caller's draws; SK-ROCK's δθ and source;
@@ -734,7 +742,79 @@ struct SepOut {
 // (I + aK)^{-1} of the node-shaped vector `in` (_sep_solve!, implicit.jl:62-83, or in Float32
 // _precond32!, :353-373): forward passes along x and y, the pass along z fused with the mode
 // projector, the three inverse passes.  `conv`: the convergence check of the first pass.
+// The x-y passes of the folded preconditioner over the twelve quarters (three components, four
+// parities each): the quarter (c, px, py) takes the half-size bases of parity px along x and py
+// along y of the component's own bases (U on its own axis, V elsewhere).
+PlanePass quarter_pass(tdgl_handle *h, bool inv, float *in, float *out) {
+  PlanePass p{};
+  p.d = h->pd;
+  p.one = h->plane1;
+  p.ncomp = 12;
+  const Quarters &q = h->q;
+  for (int c = 0; c < 3; ++c)
+    for (int px = 0; px < 2; ++px)
+      for (int py = 0; py < 2; ++py) {
+        const int k = 4 * c + 2 * px + py, qo = q.off[c][2 * px + py];
+        p.e[k][0] = q.h[c][0];
+        p.e[k][1] = q.h[c][1];
+        p.e[k][2] = q.e2[c];
+        p.in[k] = View{(void *)(in + qo), q.h[c][0], q.h[c][0] * q.h[c][1]};
+        p.out[k] = View{(void *)(out + qo), q.h[c][0], q.h[c][0] * q.h[c][1]};
+        const int wx = c == 0 ? 1 : 0, wy = c == 1 ? 1 : 0;
+        p.X[k] = h->Hq[0][wx][px][inv];
+        p.Y[k] = h->Hq[1][wy][py][inv];
+        p.XT[k] = h->Hq[0][wx][px][!inv];
+        p.YT[k] = h->Hq[1][wy][py][!inv];
+      }
+  return p;
+}
+
+// (I + aK)^{-1} in Float32 with the folds (_precond32!, implicit.jl:353-373): fold x and y, the x-y
+// passes of the quarters, the z pass with the projector on the quarters, the inverse x-y passes,
+// unfold into the node-shaped output with the partials of its dots.
+int sep_solve_folded(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int *gate, Conv conv) {
+  if (o.addth) return fail(TDGL_E_ARG, "the folded preconditioner writes z, not φ");
+  FoldPass f{};
+  for (int c = 0; c < 3; ++c) f.in[c] = node_view(h, in.c[c]);
+  f.out = h->qa;
+  f.q = h->q;
+  f.conv = conv;
+  f.d = h->pd;
+  PL(K_FOLD, launch_fold(f, gate, h->s));
+  const int *g2 = conv.mode != CONV_NONE ? conv.out : gate;
+  PL(K_TX32, (launch_plane<float, float, float>(quarter_pass(h, false, h->qa, h->qb), g2, h->s)));
+  ZModesQ zq{};
+  zq.modes = h->qb;
+  zq.q = h->q;
+  zq.Az[0] = h->Af[2][0][0];
+  zq.Az[1] = h->Af[2][1][0];
+  zq.sx = h->sf[0];
+  zq.sy = h->sf[1];
+  zq.sz = h->sf[2];
+  zq.nx = h->g.Nx - 1;
+  zq.ny = h->g.Ny - 1;
+  zq.nz = h->g.Nz - 1;
+  zq.a = h->d.a;
+  PL(K_MODES, launch_zmodes_q(zq, g2, h->s));
+  PL(K_TX32, (launch_plane<float, float, float>(quarter_pass(h, true, h->qb, h->qa), g2, h->s)));
+  UnfoldPass u{};
+  u.in = h->qa;
+  u.q = h->q;
+  for (int c = 0; c < 3; ++c) {
+    u.out[c] = node_view(h, o.out->c[c]);
+    if (o.dot) {
+      u.r[c] = node_view(h, h->r.c[c]);
+      u.zold[c] = node_view(h, h->zold.c[c]);
+    }
+  }
+  u.dot = o.dot;
+  u.d = h->pd;
+  PL(K_FOLD, launch_unfold(u, g2, h->s));
+  return 0;
+}
+
 int sep_solve(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int *gate, bool f32, Conv conv) {
+  if (f32 && h->fold) return sep_solve_folded(h, in, o, gate, conv);
   void *Ta = f32 ? (void *)h->T32a : (void *)h->T1;
   void *Tb = f32 ? (void *)h->T32b : (void *)h->T2;
   const size_t es = f32 ? 4 : 8;
@@ -859,6 +939,7 @@ int sep_solve(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int *gate, 
 
 // the partials of r·z written by the last launch of a preconditioner solve
 int sep_last_parts(tdgl_handle *h, bool f32) {
+  if (f32 && h->fold) return fold_blocks(h->q);
   if (!h->tx_only && (f32 ? h->plane32 : h->plane64)) return f32 ? plane_parts<float>(plane_pass(h, true, true)) : plane_parts<double>(plane_pass(h, true, false));
   return tx_parts(tx_pass(h, 2, true, false));
 }
@@ -871,6 +952,7 @@ PcgArgs pcg_args(tdgl_handle *h) {
   a.rg = h->rg;
   a.d = h->pd;
   a.kc = h->box_kc;
+  a.tile = h->paptile;
   for (int c = 0; c < 3; ++c) {
     a.D[c] = h->D[c];
     a.th[c] = h->th[c];
@@ -1875,6 +1957,8 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     // x-y and z-mode launches fit
     // TDGL_HIP_PLANE1=0: the chunked Float32 x-y kernel also where the one-round kernel fits;
     // =2 or =3: the one-round kernel with that many m tiles per block wherever it fits
+    const char *pt = std::getenv("TDGL_HIP_PAPTILE");
+    h->paptile = pt && pt[0] == '0' ? 0 : 1;
     const char *zc = std::getenv("TDGL_HIP_ZCOL");
     h->zcols = zc && zc[0] == '1' ? 1 : 0;
     const char *g1 = std::getenv("TDGL_HIP_PLANE1");
@@ -1884,6 +1968,58 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     if (!(gp && gp[0] == '0') && zfits) {
       h->plane32 = plane_fits<float>(rg.e, 3);
       h->plane64 = plane_fits<double>(rg.e, 3);
+    }
+    // the parity-folded Float32 preconditioner (TDGL_HIP_FOLD=0: the full-length transforms)
+    const char *gf = std::getenv("TDGL_HIP_FOLD");
+    if (d.field_mode == TDGL_FIELD_PCG && d.pcg_f32 && !(gf && gf[0] == '0') && !(gp && gp[0] == '0') &&
+        d.Nz - 1 >= 2 && d.Nz - 1 <= ZQ_NZ_MAX) {
+      Quarters q{};
+      int off = 0;
+      int qe[12][3];
+      for (int c = 0; c < 3; ++c) {
+        q.L[c][0] = rg.e[c][0];
+        q.L[c][1] = rg.e[c][1];
+        q.h[c][0] = (rg.e[c][0] + 1) / 2;
+        q.h[c][1] = (rg.e[c][1] + 1) / 2;
+        q.e2[c] = rg.e[c][2];
+        for (int k = 0; k < 4; ++k) {
+          q.off[c][k] = off;
+          off += q.h[c][0] * q.h[c][1] * q.e2[c];
+          qe[4 * c + k][0] = q.h[c][0];
+          qe[4 * c + k][1] = q.h[c][1];
+          qe[4 * c + k][2] = q.e2[c];
+        }
+      }
+      q.total = off;
+      if (plane_fits<float>(qe, 12) && fold_blocks(q) <= MAX_PART) {
+        h->q = q;
+        ALLOC(h->qa, q.total);
+        ALLOC(h->qb, q.total);
+        const int nn[2] = {d.Nx - 1, d.Ny - 1};
+        for (int ax = 0; ax < 2; ++ax) {
+          const Bases b = make_bases(nn[ax]);
+          for (int which = 0; which < 2; ++which) {
+            const std::vector<double> &M = which == 0 ? b.V : b.U;
+            const int L = which == 0 ? nn[ax] : nn[ax] - 1, Lh = (L + 1) / 2;
+            for (int par = 0; par < 2; ++par) {
+              // the columns of parity `par` on the first ceil(L/2) rows: Mh(i, cc) = M(i, 2 cc + par)
+              std::vector<float> hf((size_t)Lh * Lh, 0.0f), hi((size_t)Lh * Lh, 0.0f);
+              for (int i = 0; i < Lh; ++i)
+                for (int cc = 0; cc < Lh; ++cc) {
+                  const int col = 2 * cc + par;
+                  const float v = col < L ? (float)M[i + (size_t)L * col] : 0.0f;
+                  hf[(size_t)i * Lh + cc] = v;  // Mh(i, cc)
+                  hi[(size_t)cc * Lh + i] = v;  // Hi[q][m] = Mh(m, q)
+                }
+              ALLOC(h->Hq[ax][which][par][0], hf.size());
+              CHK(upload_vec(h, h->Hq[ax][which][par][0], hf));
+              ALLOC(h->Hq[ax][which][par][1], hi.size());
+              CHK(upload_vec(h, h->Hq[ax][which][par][1], hi));
+            }
+          }
+        }
+        h->fold = true;
+      }
     }
     return 0;
   };
@@ -2371,6 +2507,7 @@ int tdgl_get_stats(tdgl_handle *h, tdgl_stats *st) {
   st->rollbacks = h->rollbacks;
   st->pcg_budget = h->budget;
   st->captures = h->captures;
+  st->folded = h->fold ? 1 : 0;
   st->steps = h->steps;
   st->calls = h->calls;
   st->pcg_iters = h->pcg_iters;

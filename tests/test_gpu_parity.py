@@ -388,17 +388,25 @@ def test_pcg_budget_settles_on_a_current_ramp():
     assert g["rollbacks"] <= 4 and g["captures"] <= 8
 
 
-@pytest.mark.parametrize("name", ["production", "pcg_fast_f32", "pcg_exact", "kwt10_pcg", "three_aniso", "pcg_wide_f32"])
-def test_zmodes_runs_equal_whole_columns(name, monkeypatch):
-    """The z pass over x-runs of modes (k_zmodes_run) sums in the order of the whole-column kernel
-    (TDGL_HIP_ZCOL=1): 8 steps give the same bits either way."""
+VARIANTS = [("TDGL_HIP_ZCOL", "1", "0"), ("TDGL_HIP_PAPTILE", "0", "1")]
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=lambda v: v[0])
+@pytest.mark.parametrize("name", ["production", "pcg_fast_f32", "pcg_exact", "kwt10_pcg", "three_aniso", "pcg_wide_f32",
+                                  "open_kwt_pcg"])
+def test_kernel_variants_give_the_same_bits(name, variant, monkeypatch):
+    """Kernels rewritten for speed keep the arithmetic of the ones they replace: the z pass over
+    x-runs of modes (k_zmodes_run) against whole columns per block (TDGL_HIP_ZCOL=1), the LDS-tiled
+    PCG operator (k_pcg_pap_tile) against one gather per stencil value (TDGL_HIP_PAPTILE=0).  Eight
+    steps give the same bits either way."""
     HipStepper = _hip()
     st, _ = make_case(name)
     s0, pcg, H, HJ = seed_state(st)
     outs = []
-    print(f"[gpu] z pass runs against columns: {name}", flush=True)
-    for zc in ("1", "0"):
-        monkeypatch.setenv("TDGL_HIP_ZCOL", zc)
+    env, old, new = variant
+    print(f"[gpu] {env}={old} against {new}: {name}", flush=True)
+    for zc in (old, new):
+        monkeypatch.setenv(env, zc)
         dev = HipStepper(st)
         dev.upload(s0)
         if pcg is not None:
@@ -408,3 +416,30 @@ def test_zmodes_runs_equal_whole_columns(name, monkeypatch):
         dev.close()
     for (k, a), (_, b) in zip(outs[0].arrays(), outs[1].arrays()):
         assert np.array_equal(a, b), k
+
+
+@pytest.mark.parametrize("name", ["pcg_fast_f32", "pcg_tight_f32", "kwt10_pcg", "pcg_wide_f32", "open_kwt_pcg",
+                                  "qp_two_kwt_pcg"])
+def test_folded_preconditioner_matches_full_length(name, monkeypatch):
+    """The Float32 preconditioner with parity-folded transforms (quarter planes with half-size bases)
+    against the full-length transforms (TDGL_HIP_FOLD=0): the same solutions to the PCG tolerance
+    (its rounding differs, so not the same bits), and the folded path is the one that ran."""
+    HipStepper = _hip()
+    st, tol = make_case(name)
+    s0, pcg, H, HJ = seed_state(st)
+    outs, folded = [], []
+    print(f"[gpu] folded preconditioner against full length: {name}", flush=True)
+    for fv in ("0", "1"):
+        monkeypatch.setenv("TDGL_HIP_FOLD", fv)
+        dev = HipStepper(st)
+        dev.upload(s0)
+        dev.set_pcg(pcg)
+        dev.step(16, H, HJ)
+        outs.append(dev.download())
+        folded.append(dev.stats()["folded"])
+        dev.close()
+    assert folded == [0, 1]
+    errs = compare_states(outs[1], outs[0])
+    core = {k: v for k, v in errs.items() if not k.startswith(("U", "V"))}
+    print(f"    worst {max(core, key=core.get)} {max(core.values()):.3e}", flush=True)
+    assert max(core.values()) <= max(tol, 1e-12) * 10, errs
