@@ -221,6 +221,15 @@ int tdgl_download_state(tdgl_handle *h, tdgl_state *s);
  * (gtdgl/src/step.jl:13-35), synchronous. */
 int tdgl_step(tdgl_handle *h, int32_t n, double H, double HJ);
 
+/* No counterpart in the reference: tdgl_step for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  The same steps and the same results as
+ * tdgl_step, launched from the host (no graph is captured or instantiated), with the PCG's
+ * reduction partials, scalars and flags in a fixed device array of the library (about 1 MiB per
+ * GPU, held until the process ends) instead of the handle's; grids whose reductions need more than
+ * its 16384 partials per region are refused with TDGL_E_ARG.  Calls that overlap on one GPU share
+ * that array and race: serialise them.  The per-kernel profile must be off. */
+int tdgl_step_noalloc(tdgl_handle *h, int32_t n, double H, double HJ);
+
 /* Replaces _pcg_up! / _pcg_down! (TDGLGPU.jl:217-224): PCG history in and out. */
 int tdgl_get_pcg(tdgl_handle *h, tdgl_pcg *p);
 int tdgl_set_pcg(tdgl_handle *h, const tdgl_pcg *p);

@@ -2205,6 +2205,81 @@ int tdgl_step(tdgl_handle *h, int32_t n, double H, double HJ) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------- no allocation
+// The PCG's reduction partials, scalars and flags for tdgl_step_noalloc: a file-scope device
+// array in the layout of a handle's own (PART_N regions of 2 MAX_PART doubles, then SC_COUNT
+// doubles, then FL_COUNT ints), one copy per GPU for the life of the process.  Host code uses the
+// address hipGetSymbolAddress gives on each device, never the array's name.
+}  // extern "C"
+constexpr size_t SCRATCH_PART = (size_t)PART_N * 2 * MAX_PART;  // doubles
+constexpr size_t SCRATCH_BYTES = (SCRATCH_PART + SC_COUNT) * sizeof(double) + FL_COUNT * sizeof(int);
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+void *g_scratch[SCRATCH_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// the scratch's address on device `dev`, looked up once per device with that device current
+int scratch_on(int dev, void **out) {
+  if (dev < 0 || dev >= SCRATCH_DEVICES) return fail(TDGL_E_ARG, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) return fail(TDGL_E_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+}  // namespace
+extern "C" {
+
+int tdgl_step_noalloc(tdgl_handle *h, int32_t n, double H, double HJ) {
+  if (!h) return fail(TDGL_E_ARG, "null handle");
+  if (h->prof) return fail(TDGL_E_ARG, "tdgl_step_noalloc: the per-kernel profile is on (it records events per launch)");
+  const bool pcg = h->d.field_mode == TDGL_FIELD_PCG;
+  if (pcg) {
+    const bool f32 = h->d.pcg_f32 && (h->d.pcg_fast || h->d.pcg_f32);
+    const int need = std::max({field_blocks(h->g), pcg_box_blocks(h->g, h->box_kc), sep_last_parts(h, f32),
+                               sep_last_parts(h, false)});
+    if (need > MAX_PART)
+      return fail(TDGL_E_ARG, "tdgl_step_noalloc: the grid's reductions need %d partials, the fixed scratch holds %d",
+                  need, MAX_PART);
+  }
+  void *scr = nullptr;
+  CHK(scratch_on(h->dev, &scr));
+  const PcgDev own = h->pd;
+  if (pcg) {
+    // the solver's scalars and flags in, every call: the array keeps the last call's contents
+    PcgDev sp{};
+    sp.part = (double *)scr;
+    sp.sc = sp.part + SCRATCH_PART;
+    sp.fl = (int *)(sp.sc + SC_COUNT);
+    HIPCHK(hipMemcpyAsync(sp.sc, own.sc, SC_COUNT * sizeof(double), hipMemcpyDeviceToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(sp.fl, own.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToDevice, h->s));
+    h->pd = sp;
+  }
+  // launched from the host: a captured graph would be instantiated (an allocation)
+  const bool graphs = h->graphs;
+  h->graphs = false;
+  const int rc = tdgl_step(h, n, H, HJ);
+  h->graphs = graphs;
+  if (pcg) {
+    const PcgDev sp = h->pd;
+    h->pd = own;
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(own.sc, sp.sc, SC_COUNT * sizeof(double), hipMemcpyDeviceToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(own.fl, sp.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToDevice, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+  }
+  return rc;
+}
+
 int tdgl_get_pcg(tdgl_handle *h, tdgl_pcg *p) {
   if (!h || !p) return fail(TDGL_E_ARG, "null argument");
   if (h->d.field_mode != TDGL_FIELD_PCG) return fail(TDGL_E_ARG, "the stepper has no PCG solver");

@@ -20,7 +20,7 @@ SYMBOLS = ("tdgl_supported", "tdgl_create", "tdgl_destroy", "tdgl_upload_state",
            "tdgl_get_pcg", "tdgl_set_pcg", "tdgl_get_open", "tdgl_set_open", "tdgl_get_qp", "tdgl_get_stats", "tdgl_last_error", "tdgl_device_count",
            "tdgl_set_device", "tdgl_profile_enable", "tdgl_profile_get", "tdgl_kernel_name", "tdgl_meter_begin",
            "tdgl_meter_cycle", "tdgl_noise_get", "tdgl_noise_set", "tdgl_noise_draws", "tdgl_batch_create",
-           "tdgl_batch_step", "tdgl_batch_meter_cycle", "tdgl_batch_destroy")
+           "tdgl_batch_step", "tdgl_batch_meter_cycle", "tdgl_batch_destroy", "tdgl_step_noalloc")
 
 _lib = None
 _lock = threading.Lock()
@@ -46,6 +46,7 @@ def load():
             L.tdgl_upload_state.argtypes = [vp, vp]
             L.tdgl_download_state.argtypes = [vp, vp]
             L.tdgl_step.argtypes = [vp, ct.c_int32, ct.c_double, ct.c_double]
+            L.tdgl_step_noalloc.argtypes = [vp, ct.c_int32, ct.c_double, ct.c_double]
             L.tdgl_get_pcg.argtypes = [vp, vp]
             L.tdgl_set_pcg.argtypes = [vp, vp]
             L.tdgl_get_stats.argtypes = [vp, vp]
@@ -131,6 +132,11 @@ class HipStepper:
 
     def step(self, n: int, H: float, HJ: float) -> None:
         _check(load().tdgl_step(self.h, int(n), float(H), float(HJ)), "tdgl_step")
+
+    def step_noalloc(self, n: int, H: float, HJ: float) -> None:
+        """n steps through tdgl_step_noalloc: the same steps with the reductions' scratch in the
+        library's fixed device array, nothing allocated."""
+        _check(load().tdgl_step_noalloc(self.h, int(n), float(H), float(HJ)), "tdgl_step_noalloc")
 
     def get_pcg(self) -> PcgState:
         p = PcgState(self.st.grid)
