@@ -323,10 +323,11 @@ def roofline(st, dom, nl, ms, nsteps, iters, workload, folded=False):
     return roof
 
 
-def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=None):
+def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=None, tune=False):
     """Warm-up and timed ladder cycles of one workload with its state resident on the device; then,
     with `profile`, the same number of cycles again with HIP events around every launch (the
-    roofline kernel), and with `cpu_steps` the CPU legs."""
+    roofline kernel), and with `cpu_steps` the CPU legs.  With `tune`, the library's tuner picks the
+    launch shapes on the seeded state first (untimed; the state is left as it was)."""
     make, desc, mask = WORKLOADS[name]
     st, p = make()
     H, HJ = p["H"], st.selffield(p["J"])
@@ -335,6 +336,12 @@ def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=Non
     dev.upload(s0)
     print(f"[bench] {name}: dt={st.dt:.5f} stages={st.rkl} nsub={st.nsub} grid={st.grid.Nx}x{st.grid.Ny}x"
           f"{st.grid.Nz} padded", file=sys.stderr, flush=True)
+    tuned = None
+    if tune and st.field_mode == 2:
+        tuned = dev.autotune(H, HJ)
+        print(f"[bench] {name}: tuned box_kc={tuned['box_kc']} plane_mt={tuned['plane_mt']} "
+              f"zmodes_waves={tuned['zmodes_waves']}: " + ", ".join(f"{k}={v}:{us:.1f}us" for k, v, us in tuned["candidates"]),
+              file=sys.stderr, flush=True)
     for q in range(warmup):
         dev.step(LMAX, H, HJ)
         print(f"[bench] warmup cycle {q + 1}/{warmup} done", file=sys.stderr, flush=True)
@@ -365,7 +372,8 @@ def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=Non
                       "b_alg_per_node_step": balg,
                       "hbm_roofline_frac_cycle": balg * nodes * LMAX * steps / wall / HBM_PEAK,
                       "ms_per_time_step": 1e3 * wall / nsteps, "timed_rollbacks": rollbacks,
-                      "timed_graph_captures": captures},
+                      "timed_graph_captures": captures,
+                      "tuned": None if tuned is None else {k: tuned[k] for k in ("box_kc", "plane_mt", "zmodes_waves")}},
            "mask": mask, "roofline": None, "cpu": None}
     if profile:
         dev.profile(True)
@@ -392,19 +400,61 @@ def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=Non
     return out
 
 
+def bench_serve(args) -> int:
+    """The served current sweep: a Poisson stream of requests (juliatdgl_amd.sweep.serving_specs: two or
+    three bands, grids log-uniform from 24x16x4 to 256x128x32, a notch or a hole, a random current and
+    polarity, thermal noise T~ = 1e-3 drawn on the device), admitted by HBM footprint as slots free up,
+    each run ending at its first phase slip across the voltmeter (crossing_times with stop = 1) or at
+    t = T.  Runs/hour of the stream and p50/p99 latency (finish - arrival)."""
+    from juliatdgl_amd import sweep
+    n = args.serve_runs
+    specs = sweep.serving_specs(n, master=4242)
+    cells = sum(s.nx * s.ny * s.nz for s in specs)
+    work = sweep.FirstSlip(T=args.serve_T, every=10)
+    print(f"[bench] serve: {n} requests, {cells} cells, first slip or t = {args.serve_T}", file=sys.stderr, flush=True)
+    sweep.run_batch(specs[:2], sweep.Cycles(cycles=1), workers=2)  # the device and the library warm
+    srv = sweep.Server(budget_bytes=200e9, slots=args.workers)
+    arrivals = sweep.poisson_arrivals(n, rate=args.serve_rate, seed=11)
+    for s, t in zip(specs, arrivals):
+        srv.submit(s, t)
+    t0 = time.perf_counter()
+    done = srv.run(work)
+    wall = time.perf_counter() - t0
+    lat = sweep.percentiles(d["latency"] for d in done.values())
+    slipped = sum(1 for d in done.values() if d["result"][0] is not None)
+    print(f"[bench] served {n} runs in {wall:.2f} s: p50 {lat['p50']:.3f} s, p99 {lat['p99']:.3f} s, {slipped} slipped",
+          file=sys.stderr, flush=True)
+    line = {"metric": "served runs/hr", "value": 3600.0 * n / wall, "unit": "runs/hr", "n_gpus": 1, "steps": n,
+            "warmup": 1, "ms_per_step": 1e3 * wall / n, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded psi per run, thermal noise from the device RNG)",
+            "config": {"workload": "served current sweep: Poisson stream of two- and three-band runs, grids log-uniform "
+                                   "24x16x4..256x128x32, notch or hole, |J| in [0.3, 1.5] of either polarity, noise "
+                                   f"T~ = 1e-3, separable field, dt 0.01, each to its first phase slip or t = {args.serve_T}",
+                       "runs": n, "cells": cells, "slots": args.workers, "arrival_rate_per_s": args.serve_rate,
+                       "p50_latency_s": lat["p50"], "p99_latency_s": lat["p99"], "mean_latency_s": lat["mean"],
+                       "slipped": slipped, "bands": {str(b): sum(1 for s in specs if s.bands == b) for b in (2, 3)}},
+            "roofline": None, "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed ladder cycles")
     ap.add_argument("--warmup", type=int, default=5,
                     help="untimed ladder cycles (the PCG budget settles and its graph is captured in them)")
-    ap.add_argument("--workload", default="open_3band_256x128x32", choices=tuple(WORKLOADS) + ("sweep64",))
+    ap.add_argument("--workload", default="open_3band_256x128x32", choices=tuple(WORKLOADS) + ("sweep64", "serve"))
+    ap.add_argument("--serve-runs", type=int, default=32, help="serve: requests of the stream")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-T", type=float, default=40.0, help="serve: a run ends at its first slip or at this time")
     ap.add_argument("--workers", type=int, default=8, help="sweep64: runs in flight at once")
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="time steps of the CPU baseline sample (default 8)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
                     help="leave out the 128x64x16 KWT workload that the 256x128x32 line carries beside it")
+    ap.add_argument("--tune", action="store_true", help="run the library's launch-shape tuner before the warm-up")
     ap.add_argument("--profile-out", default=None, help="write the per-kernel profile (JSON) here")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy a MIB-MiB device buffer (a reference kernel for the counters' scale)")
@@ -418,17 +468,19 @@ def main(argv=None):
         return 1
     if args.workload == "sweep64":
         return bench_sweep(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     if args.copy_probe:
         copy_probe(args.copy_probe)
     cpu_steps = 0 if args.no_cpu else (args.cpu_steps or 8)
-    r = run_workload(args.workload, args.steps, args.warmup, profile=True, cpu_steps=cpu_steps,
+    r = run_workload(args.workload, args.steps, args.warmup, profile=True, cpu_steps=cpu_steps, tune=args.tune,
                      profile_out=args.profile_out)
     published = PUBLISHED.get(args.workload)
     cfg = dict(r["config"])
     if published:
         cfg["reference_published"] = published
     if args.workload == "open_3band_256x128x32" and not args.no_secondary:
-        sec = run_workload("kwt_pcg_128x64x16", min(args.steps, 10), 3, profile=False)
+        sec = run_workload("kwt_pcg_128x64x16", min(args.steps, 10), 3, profile=False, tune=args.tune)
         cfg["secondary"] = {"kwt_pcg_128x64x16": {"cell_cycles_per_s": sec["value"],
                                                   "ms_per_cycle": 1e3 * sec["wall"] / sec["steps"],
                                                   **{k: sec["config"][k] for k in ("workload", "cells",

@@ -230,6 +230,28 @@ int tdgl_step(tdgl_handle *h, int32_t n, double H, double HJ);
  * that array and race: serialise them.  The per-kernel profile must be off. */
 int tdgl_step_noalloc(tdgl_handle *h, int32_t n, double H, double HJ);
 
+/* No counterpart in the reference (its kernels have fixed shapes): the library's tuner.  For a PCG
+ * stepper it times, on the handle's own state and at (H, HJ), one time step per candidate launch
+ * shape and keeps the fastest: the PCG box kernels' k levels per block (box_kc: 1, 2, 4, ... while
+ * the reductions' partials fit), the m tiles per block of the one-round x-y transform (2 or 3), and
+ * the waves per block (groups of z modes) of the folded preconditioner's z pass (4 or 8).  The
+ * state (F and the normal-fluid potentials included), the PCG history, the open boundary and the
+ * noise position are put back; captured cycles are dropped.  box_kc
+ * changes how the PCG's block partials fall (its sums round differently), the others change no
+ * result.  cand[] lists every candidate timed (us per launch of the kernels it shapes). */
+#define TDGL_TUNE_MAX 16
+enum { TDGL_TUNE_BOX_KC = 1, TDGL_TUNE_PLANE_MT = 2, TDGL_TUNE_ZMODES_WAVES = 3 };
+typedef struct {
+  int32_t kind, value;
+  double us;
+} tdgl_tune_cand;
+typedef struct {
+  int32_t box_kc, plane_mt, zmodes_waves;
+  int32_t ncand;
+  tdgl_tune_cand cand[TDGL_TUNE_MAX];
+} tdgl_tuning;
+int tdgl_autotune(tdgl_handle *h, double H, double HJ, tdgl_tuning *out);
+
 /* Replaces _pcg_up! / _pcg_down! (TDGLGPU.jl:217-224): PCG history in and out. */
 int tdgl_get_pcg(tdgl_handle *h, tdgl_pcg *p);
 int tdgl_set_pcg(tdgl_handle *h, const tdgl_pcg *p);

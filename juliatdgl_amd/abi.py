@@ -84,6 +84,17 @@ class StatsC(ct.Structure):
 
 
 NKERNELS = 15
+TUNE_MAX = 16
+TUNE_KINDS = {1: "box_kc", 2: "plane_mt", 3: "zmodes_waves"}
+
+
+class TuneCandC(ct.Structure):
+    _fields_ = [("kind", ct.c_int32), ("value", ct.c_int32), ("us", ct.c_double)]
+
+
+class TuningC(ct.Structure):
+    _fields_ = [("box_kc", ct.c_int32), ("plane_mt", ct.c_int32), ("zmodes_waves", ct.c_int32),
+                ("ncand", ct.c_int32), ("cand", TuneCandC * TUNE_MAX)]
 
 
 class MeterC(ct.Structure):

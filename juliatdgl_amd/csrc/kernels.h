@@ -298,14 +298,12 @@ struct Quarters {
   int off[3][4];
   int total;
 };
-// fold: the three components of a node-shaped FP64 vector into the quarters (FP32), with the
-// convergence check of a first preconditioner pass (conv, as the plane launches take it)
+// fold: the three components of a node-shaped FP64 vector into the quarters (FP32); the
+// convergence check of a preconditioner's first pass is the forward x-y launch's
 struct FoldPass {
   View in[3];
   float *out;
   Quarters q;
-  Conv conv;
-  PcgDev d;
 };
 hipError_t launch_fold(const FoldPass &f, const int *gate, hipStream_t s);
 // unfold: the quarters of the inverse x-y transforms back into a node-shaped FP64 vector, with the
@@ -323,13 +321,18 @@ int fold_blocks(const Quarters &q);
 struct ZModesQ {
   float *modes;
   Quarters q;
-  const float *Az[2];    // forward z: [0] V_z, [1] U_z, as ZModes
+  const float *zb[4];    // the padded z bases (ZQ_BASES): V and U by rows (position, mode), then by columns
   const float *sx, *sy, *sz;
   int nx, ny, nz;
   double a;
+  int waves;             // waves per block, each a group of z modes: 4 or 8, 0 the library's choice
 };
 hipError_t launch_zmodes_q(const ZModesQ &m, const int *gate, hipStream_t s);
-constexpr int ZQ_NZ_MAX = 64;  // z extents the quarter z pass takes (its bases in LDS)
+constexpr int ZQ_NZ_MAX = 64;  // z extents the quarter z pass takes (16 modes a wave at 4 waves a block)
+// The quarter z pass's bases, float, each nz x nz followed by ZQ_SLACK zeros: [0] V(n, mz) at
+// n nz + mz, [1] U(n, mz - 1) at n nz + mz (zero for mz = 0 or n = nz - 1), [2] and [3] the same at
+// mz nz + n.  The slack covers the reads of the last group past nz.
+constexpr int ZQ_SLACK = 128;
 
 // The z pass, the per-wavevector projector and the inverse z pass in one launch: a block holds
 // whole z-columns of modes (floor(256 / nz) columns of nz threads), so the inverse pass reads
@@ -556,6 +559,7 @@ hipError_t launch_batch_meter(int kind, const BatchTab &t, int blocks, hipStream
 
 hipError_t launch_pcg_pap(const PcgArgs &a, hipStream_t s);      // β, P = z + βP, zold = z, Q = A P, p·Ap
 hipError_t launch_pcg_xr(const PcgArgs &a, hipStream_t s);       // α, x += αP, r −= αQ, r·r
+hipError_t launch_pcg_xr_fold(const PcgArgs &a, const FoldPass &f, hipStream_t s);  // the same, and the fold of r
 hipError_t launch_pcg_finish(const PcgArgs &a, hipStream_t s);   // φ = θ + x, the cycle's counts
 hipError_t launch_pcg_cycle_reset(PcgDev d, hipStream_t s);
 

@@ -2190,10 +2190,7 @@ static dim3 fold_grid(const Quarters &q) {
 }
 
 __global__ void __launch_bounds__(256) k_fold(FoldPass f, const int *gate) {
-  __shared__ double sh[8];
-  const int tid = threadIdx.x;
-  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  if (!pass_active(f.conv, gate, f.d, sh, tid, bid)) return;
+  if (gate && !*gate) return;
   int c, i, j, kg;
   fold_index(f.q, c, i, j, kg);
   const int L0 = f.q.L[c][0], L1 = f.q.L[c][1], h0 = f.q.h[c][0], h1 = f.q.h[c][1], e2 = f.q.e2[c];
@@ -2218,6 +2215,79 @@ __global__ void __launch_bounds__(256) k_fold(FoldPass f, const int *gate) {
     q10[qk] = (float)(cj ? dj : dj + ds);
     q11[qk] = (float)(cj ? 0.0 : dj - ds);
   }
+}
+
+// Iteration q of the PCG with the folded preconditioner: α = rz/pAp, x += αP and r −= αQ at the
+// four mirror images of each quarter point (each range element is one image of exactly one
+// point), the partials of r·r, and the fold of the new r into the quarters, in one pass.  IT += 1.
+// The convergence check that k_fold's first pass would take moves to the forward x-y launch, which
+// reads these partials (fold_blocks of them).
+__global__ void __launch_bounds__(256) k_pcg_xr_fold(PcgArgs a, FoldPass f) {
+  if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
+  __shared__ double sh[8];
+  const int tid = threadIdx.x;
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const double pAp = sum_parts(region(a.d, PART_PQ), a.nparts_in, 0, sh, tid);
+  const double al = a.d.sc[SC_RZ0 + (a.q & 1)] / pAp;
+  if (bid == 0 && tid == 0) a.d.fl[FL_IT] += 1;
+  int c, i, j, kg;
+  fold_index(f.q, c, i, j, kg);
+  const int L0 = f.q.L[c][0], L1 = f.q.L[c][1], h0 = f.q.h[c][0], h1 = f.q.h[c][1], e2 = f.q.e2[c];
+  double s = 0.0;
+  if (i < h0 && j < h1) {
+    const int is = L0 - 1 - i, js = L1 - 1 - j;
+    const bool ci = is == i, cj = js == j;
+    const View v = f.in[c];  // r's view; x, P, Q share its layout (node-shaped vectors)
+    const size_t base = (size_t)((const double *)v.p - a.r.c[c]);
+    const size_t o[4] = {base + i + (size_t)v.s1 * j, base + is + (size_t)v.s1 * j, base + i + (size_t)v.s1 * js,
+                         base + is + (size_t)v.s1 * js};
+    const bool w[4] = {true, !ci, !cj, !ci && !cj};
+    double *X = a.x.c[c], *R = a.r.c[c];
+    const double *P = a.Pnew.c[c], *Q = a.Q.c[c];
+    const size_t qp = (size_t)i + (size_t)h0 * j, qs = (size_t)h0 * h1;
+    float *q00 = f.out + f.q.off[c][0] + qp, *q01 = f.out + f.q.off[c][1] + qp;
+    float *q10 = f.out + f.q.off[c][2] + qp, *q11 = f.out + f.q.off[c][3] + qp;
+    for (int k = kg; k < e2; k += 4) {
+      const size_t zk = (size_t)v.s2 * k;
+      double xv[4], pv[4], rv[4], qv[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        xv[t] = X[o[t] + zk];
+        pv[t] = P[o[t] + zk];
+        rv[t] = R[o[t] + zk];
+        qv[t] = Q[o[t] + zk];
+      }
+      double rn[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        rn[t] = rv[t] - al * qv[t];
+        if (!w[t]) continue;
+        X[o[t] + zk] = xv[t] + al * pv[t];
+        R[o[t] + zk] = rn[t];
+        s += rn[t] * rn[t];
+      }
+      const double sj = ci ? rn[0] : rn[0] + rn[1], dj = ci ? 0.0 : rn[0] - rn[1];
+      const double ss = ci ? rn[2] : rn[2] + rn[3], ds = ci ? 0.0 : rn[2] - rn[3];
+      const size_t qk = qs * k;
+      q00[qk] = (float)(cj ? sj : sj + ss);
+      q01[qk] = (float)(cj ? 0.0 : sj - ss);
+      q10[qk] = (float)(cj ? dj : dj + ds);
+      q11[qk] = (float)(cj ? 0.0 : dj - ds);
+    }
+  }
+  const double r0 = block_sum_t(s, sh, tid);
+  if (tid == 0) {
+    double *pr = region_w(a.d, PART_RR);
+    pr[2 * bid] = r0;
+    pr[2 * bid + 1] = 0.0;
+  }
+}
+
+hipError_t launch_pcg_xr_fold(const PcgArgs &a, const FoldPass &f, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
+  if (fold_blocks(f.q) > MAX_PART) return hipErrorInvalidValue;
+  TDGL_LAUNCH(k_pcg_xr_fold, fold_grid(f.q), dim3(256), 0, s, a, f);
+  return hipGetLastError();
 }
 
 hipError_t launch_fold(const FoldPass &f, const int *gate, hipStream_t s) {
@@ -2291,28 +2361,33 @@ hipError_t launch_unfold(const UnfoldPass &u, const int *gate, hipStream_t s) {
   return hipGetLastError();
 }
 
-// The z pass and projector of k_zmodes_run on modes held in quarters.  A block owns TX modes
-// (mx = 2 mm + px, my = 2 qq + py, all mz) of one parity quarter (px, py): Z of those modes sits in
-// Z's quarter (px, py) at (mm, qq), Y (at my - 1) in Y's quarter (px, 1 - py) at (mm, qq - 1 + py),
-// X (at mx - 1) in X's quarter (1 - px, py) at (mm - 1 + px, qq), each a contiguous run; the results
-// go back to the same places.
-template <int TX>
-__global__ void __launch_bounds__(256) k_zmodes_q(ZModesQ m, const int *gate) {
+// The z pass and projector of k_zmodes_run on modes held in quarters.  A block owns 64 modes
+// (mx = 2 mm + px, my = 2 qq + py, all mz) of one parity quarter (px, py), one per lane, taken in
+// the order of (mm, qq) flattened along mm (so rows of the quarter share blocks): Z of those
+// modes sits in Z's quarter (px, py) at (mm, qq), Y (at my - 1) in Y's quarter (px, 1 - py) at
+// (mm, qq - 1 + py), X (at mx - 1) in X's quarter (1 - px, py) at (mm - 1 + px, qq), each a
+// contiguous run; the results go back to the same places.  The block's G waves split the z modes
+// (and, going back, the z positions) into contiguous groups of mzc: a wave's group index is
+// uniform, so the bases are read with uniform addresses (scalar loads, no LDS), and the columns are
+// shared through LDS.  Every global store comes after the last load of the bases.
+template <int G, int MZ>
+__global__ void __launch_bounds__(64 * G) k_zmodes_q(ZModesQ m, const int *gate) {
   if (gate && !*gate) return;
   using T = float;
-  constexpr int G = 256 / TX;
-  constexpr int MZ = (ZT_NZ_MAX + G - 1) / G;
   extern __shared__ __align__(16) unsigned char zq_smem[];
   const int nx = m.nx, ny = m.ny, nz = m.nz;
-  T *Cx = (T *)zq_smem, *Cy = Cx + nz * TX, *Cz = Cy + nz * TX;
-  T *AV = Cz + nz * TX, *AU = AV + nz * nz;
-  const int t = threadIdx.x, p = t % TX, g = t / TX;
+  T *Cx = (T *)zq_smem, *Cy = Cx + nz * 64, *Cz = Cy + nz * 64;
+  const int p = (int)(threadIdx.x & 63);
+  const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int mzc = (nz + G - 1) / G;  // modes (positions) of a group, at most MZ
+  const int m0 = g * mzc;
   const int px = (int)(blockIdx.z >> 1), py = (int)(blockIdx.z & 1);
-  const int mm = blockIdx.x * TX + p, qq = blockIdx.y;
-  const int mx = 2 * mm + px, my = 2 * qq + py;
-  const bool live = mx < nx && my < ny;
-  const bool hx = live && mx > 0, hy = live && my > 0;
   const Quarters &q = m.q;
+  const int f = blockIdx.x * 64 + p;  // the mode (mm, qq) of Z's quarter, flattened along mm
+  const int mm = f % q.h[2][0], qq = f / q.h[2][0];
+  const int mx = 2 * mm + px, my = 2 * qq + py;
+  const bool live = qq < q.h[2][1] && mx < nx && my < ny;
+  const bool hx = live && mx > 0, hy = live && my > 0;
   // positions in the three quarters (clamped into each quarter; the absent values selected after)
   const int xi = min(max(mm - 1 + px, 0), q.h[0][0] - 1), xj = min(qq, q.h[0][1] - 1);
   const int yi = min(mm, q.h[1][0] - 1), yj = min(max(qq - 1 + py, 0), q.h[1][1] - 1);
@@ -2321,96 +2396,98 @@ __global__ void __launch_bounds__(256) k_zmodes_q(ZModesQ m, const int *gate) {
   T *coly = m.modes + q.off[1][2 * px + (1 - py)] + yi + (size_t)q.h[1][0] * yj;
   T *colz = m.modes + q.off[2][2 * px + py] + zi + (size_t)q.h[2][0] * zj;
   const size_t stx = (size_t)q.h[0][0] * q.h[0][1], sty = (size_t)q.h[1][0] * q.h[1][1], stz = (size_t)q.h[2][0] * q.h[2][1];
-  {
-    const T *gV = m.Az[0], *gU = m.Az[1];
-    for (int e = t; e < nz * nz; e += 256) AV[e] = gV[e];
-    for (int e = t; e < (nz - 1) * (nz - 1); e += 256) AU[e] = gU[e];
-  }
   for (int z = g; z < nz; z += G) {
     const T vx = colx[stx * z], vy = coly[sty * z], vz = colz[stz * min(z, nz - 2)];
-    Cx[z * TX + p] = hx ? vx : T(0);
-    Cy[z * TX + p] = hy ? vy : T(0);
-    Cz[z * TX + p] = (live && z < nz - 1) ? vz : T(0);
+    Cx[z * 64 + p] = hx ? vx : T(0);
+    Cy[z * 64 + p] = hy ? vy : T(0);
+    Cz[z * 64 + p] = (live && z < nz - 1) ? vz : T(0);
   }
   __syncthreads();
+  // forward: the modes m0 .. m0 + MZ - 1 of the three columns (those past the group's mzc are the
+  // next group's, or none: computed and dropped); the padded bases by rows, V(n, mz) and U(n, mz - 1)
+  const T *VR = m.zb[0], *UR = m.zb[1];
   T ax[MZ], ay[MZ], az[MZ];
 #pragma unroll
   for (int r = 0; r < MZ; ++r) ax[r] = ay[r] = az[r] = T(0);
-  for (int n = 0; n < nz - 1; ++n) {
-    const T cx = Cx[n * TX + p], cy = Cy[n * TX + p], cz = Cz[n * TX + p];
+  for (int n = 0; n < nz; ++n) {
+    const T cx = Cx[n * 64 + p], cy = Cy[n * 64 + p], cz = Cz[n * 64 + p];
+    const T *v = VR + (size_t)n * nz + m0, *u = UR + (size_t)n * nz + m0;
 #pragma unroll
     for (int r = 0; r < MZ; ++r) {
-      const int mz = g + G * r;
-      if (mz < nz) {
-        const T v = AV[n * nz + mz];
-        ax[r] += cx * v;
-        ay[r] += cy * v;
-        az[r] += cz * AU[n * (nz - 1) + (mz > 0 ? mz - 1 : 0)];
-      }
-    }
-  }
-  {
-    const T cx = Cx[(nz - 1) * TX + p], cy = Cy[(nz - 1) * TX + p];
-#pragma unroll
-    for (int r = 0; r < MZ; ++r) {
-      const int mz = g + G * r;
-      if (mz < nz) {
-        const T v = AV[(nz - 1) * nz + mz];
-        ax[r] += cx * v;
-        ay[r] += cy * v;
-      }
+      ax[r] = __builtin_fmaf(cx, v[r], ax[r]);
+      ay[r] = __builtin_fmaf(cy, v[r], ay[r]);
+      az[r] = __builtin_fmaf(cz, u[r], az[r]);
     }
   }
   __syncthreads();
-  if (live) {
-    const T k1 = m.sx[mx], k2 = m.sy[my];
+  {
+    const T k1 = live ? m.sx[mx] : T(0), k2 = live ? m.sy[my] : T(0);
 #pragma unroll
     for (int r = 0; r < MZ; ++r) {
-      const int mz = g + G * r;
-      if (mz < nz) {
+      const int mz = m0 + r;
+      if (r < mzc && mz < nz) {
         const T k3 = m.sz[mz];
         const bool hz = mz > 0;
-        T x1 = hx ? ax[r] : T(0), y1 = hy ? ay[r] : T(0), z1 = hz ? az[r] : T(0);
+        T x1 = hx ? ax[r] : T(0), y1 = hy ? ay[r] : T(0), z1 = (live && hz) ? az[r] : T(0);
         const T k2s = k1 * k1 + k2 * k2 + k3 * k3;
-        if (k2s != 0) project<T>(k1, k2, k3, (T)m.a, x1, y1, z1, hx, hy, hz);
-        Cx[mz * TX + p] = x1;
-        Cy[mz * TX + p] = y1;
-        Cz[mz * TX + p] = z1;
+        if (live && k2s != 0) project<T>(k1, k2, k3, (T)m.a, x1, y1, z1, hx, hy, hz);
+        Cx[mz * 64 + p] = x1;
+        Cy[mz * 64 + p] = y1;
+        Cz[mz * 64 + p] = z1;
       }
     }
   }
   __syncthreads();
+  // back: the positions m0 .. m0 + MZ - 1; the padded bases by columns, V(n, qm) and U(n, qm - 1)
+  const T *VC = m.zb[2], *UC = m.zb[3];
+  T ox[MZ], oy[MZ], oz[MZ];
+#pragma unroll
+  for (int r = 0; r < MZ; ++r) ox[r] = oy[r] = oz[r] = T(0);
+  for (int qm = 0; qm < nz; ++qm) {
+    const T cx = Cx[qm * 64 + p], cy = Cy[qm * 64 + p], cz = Cz[qm * 64 + p];
+    const T *v = VC + (size_t)qm * nz + m0, *u = UC + (size_t)qm * nz + m0;
+#pragma unroll
+    for (int r = 0; r < MZ; ++r) {
+      ox[r] = __builtin_fmaf(cx, v[r], ox[r]);
+      oy[r] = __builtin_fmaf(cy, v[r], oy[r]);
+      oz[r] = __builtin_fmaf(cz, u[r], oz[r]);
+    }
+  }
   if (!live) return;
 #pragma unroll
   for (int r = 0; r < MZ; ++r) {
-    const int n = g + G * r;
-    if (n >= nz) continue;
-    T ox = 0, oy = 0, oz = 0;
-    for (int qm = 0; qm < nz; ++qm) {
-      const T v = AV[n * nz + qm];
-      ox += Cx[qm * TX + p] * v;
-      oy += Cy[qm * TX + p] * v;
+    const int n = m0 + r;
+    if (r < mzc && n < nz) {
+      if (hx) colx[stx * n] = ox[r];
+      if (hy) coly[sty * n] = oy[r];
+      if (n < nz - 1) colz[stz * n] = oz[r];
     }
-    const int nu = n < nz - 1 ? n : 0;
-    for (int qm = 1; qm < nz; ++qm) oz += Cz[qm * TX + p] * AU[nu * (nz - 1) + (qm - 1)];
-    if (hx) colx[stx * n] = ox;
-    if (hy) coly[sty * n] = oy;
-    if (n < nz - 1) colz[stz * n] = oz;
   }
+}
+
+template <int G>
+static hipError_t zmodes_q_g(const ZModesQ &m, const int *gate, hipStream_t s) {
+  const int mzc = (m.nz + G - 1) / G;
+  const dim3 grid(cdiv(m.q.h[2][0] * m.q.h[2][1], 64), 1, 4), blk(64 * G);
+  const size_t ldr = 3 * (size_t)m.nz * 64 * sizeof(float);
+  switch (mzc <= 2 ? 2 : mzc <= 4 ? 4 : mzc <= 6 ? 6 : mzc <= 8 ? 8 : mzc <= 12 ? 12 : 16) {
+    case 2: TDGL_LAUNCH((k_zmodes_q<G, 2>), grid, blk, ldr, s, m, gate); break;
+    case 4: TDGL_LAUNCH((k_zmodes_q<G, 4>), grid, blk, ldr, s, m, gate); break;
+    case 6: TDGL_LAUNCH((k_zmodes_q<G, 6>), grid, blk, ldr, s, m, gate); break;
+    case 8: TDGL_LAUNCH((k_zmodes_q<G, 8>), grid, blk, ldr, s, m, gate); break;
+    case 12: TDGL_LAUNCH((k_zmodes_q<G, 12>), grid, blk, ldr, s, m, gate); break;
+    default: TDGL_LAUNCH((k_zmodes_q<G, 16>), grid, blk, ldr, s, m, gate); break;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_zmodes_q(const ZModesQ &m, const int *gate, hipStream_t s) {
   if (g_rec) return hipErrorNotSupported;
-  if (m.nz < 2 || m.nz > ZT_NZ_MAX || m.nx < 1 || m.ny < 1) return hipErrorInvalidValue;
-  const int hzx = m.q.h[2][0], hzy = m.q.h[2][1];
-  const bool wide = cdiv(hzx, 64) * hzy * 4 >= 512;
-  const int TX = wide ? 64 : 32;
-  const size_t ldr = (3 * (size_t)m.nz * TX + (size_t)m.nz * m.nz + (size_t)(m.nz - 1) * (m.nz - 1)) * sizeof(float);
-  if (wide)
-    TDGL_LAUNCH((k_zmodes_q<64>), dim3(cdiv(hzx, 64), hzy, 4), dim3(256), ldr, s, m, gate);
-  else
-    TDGL_LAUNCH((k_zmodes_q<32>), dim3(cdiv(hzx, 32), hzy, 4), dim3(256), ldr, s, m, gate);
-  return hipGetLastError();
+  if (m.nz < 2 || m.nz > ZQ_NZ_MAX || m.nx < 1 || m.ny < 1) return hipErrorInvalidValue;
+  const int waves = m.waves ? m.waves : 4;
+  if (waves == 8) return zmodes_q_g<8>(m, gate, s);
+  if (waves == 4) return zmodes_q_g<4>(m, gate, s);
+  return hipErrorInvalidValue;
 }
 
 constexpr int ZM_LB_MAX = 64;  // z bases in LDS up to this nz

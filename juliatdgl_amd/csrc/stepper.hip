@@ -151,6 +151,8 @@ struct tdgl_handle {
   Quarters q{};
   float *qa = nullptr, *qb = nullptr;
   float *Hq[2][2][2][2] = {};
+  int zq_waves = 0;        // waves per block of the quarter z pass (0: the library's choice)
+  float *zqb[4] = {};      // its padded z bases (ZQ_BASES)
   bool tx_only = false;                   // the separable solve as single-axis passes (a mixed batch)
   int box_kc = 1;                         // levels of k per block of the PCG box kernels
   int plane1 = 1;                         // PlanePass::one of the Float32 launches
@@ -772,22 +774,25 @@ PlanePass quarter_pass(tdgl_handle *h, bool inv, float *in, float *out) {
 // (I + aK)^{-1} in Float32 with the folds (_precond32!, implicit.jl:353-373): fold x and y, the x-y
 // passes of the quarters, the z pass with the projector on the quarters, the inverse x-y passes,
 // unfold into the node-shaped output with the partials of its dots.
-int sep_solve_folded(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int *gate, Conv conv) {
-  if (o.addth) return fail(TDGL_E_ARG, "the folded preconditioner writes z, not φ");
+FoldPass fold_pass(tdgl_handle *h, const Vec3 &in) {
   FoldPass f{};
   for (int c = 0; c < 3; ++c) f.in[c] = node_view(h, in.c[c]);
   f.out = h->qa;
   f.q = h->q;
-  f.conv = conv;
-  f.d = h->pd;
-  PL(K_FOLD, launch_fold(f, gate, h->s));
+  return f;
+}
+
+// what follows the fold (h->qa holds the folded input): the forward x-y launch, which takes the
+// convergence check of a first pass, the z pass, the inverse x-y launch, the unfold
+int sep_folded_tail(tdgl_handle *h, const SepOut &o, const int *gate, Conv conv) {
+  PlanePass fw = quarter_pass(h, false, h->qa, h->qb);
+  fw.conv = conv;
+  PL(K_TX32, (launch_plane<float, float, float>(fw, gate, h->s)));
   const int *g2 = conv.mode != CONV_NONE ? conv.out : gate;
-  PL(K_TX32, (launch_plane<float, float, float>(quarter_pass(h, false, h->qa, h->qb), g2, h->s)));
   ZModesQ zq{};
   zq.modes = h->qb;
   zq.q = h->q;
-  zq.Az[0] = h->Af[2][0][0];
-  zq.Az[1] = h->Af[2][1][0];
+  for (int k = 0; k < 4; ++k) zq.zb[k] = h->zqb[k];
   zq.sx = h->sf[0];
   zq.sy = h->sf[1];
   zq.sz = h->sf[2];
@@ -795,6 +800,7 @@ int sep_solve_folded(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int 
   zq.ny = h->g.Ny - 1;
   zq.nz = h->g.Nz - 1;
   zq.a = h->d.a;
+  zq.waves = h->zq_waves;
   PL(K_MODES, launch_zmodes_q(zq, g2, h->s));
   PL(K_TX32, (launch_plane<float, float, float>(quarter_pass(h, true, h->qb, h->qa), g2, h->s)));
   UnfoldPass u{};
@@ -811,6 +817,14 @@ int sep_solve_folded(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int 
   u.d = h->pd;
   PL(K_FOLD, launch_unfold(u, g2, h->s));
   return 0;
+}
+
+int sep_solve_folded(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int *gate, Conv conv) {
+  if (o.addth) return fail(TDGL_E_ARG, "the folded preconditioner writes z, not φ");
+  // the fold runs where the solve does (the incoming flag of a first pass); the forward x-y launch
+  // decides on convergence
+  PL(K_FOLD, launch_fold(fold_pass(h, in), conv.mode != CONV_NONE ? conv.in : gate, h->s));
+  return sep_folded_tail(h, o, gate, conv);
 }
 
 int sep_solve(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int *gate, bool f32, Conv conv) {
@@ -989,9 +1003,16 @@ int pcg_iteration(tdgl_handle *h, int q, bool flexible) {
   a.nparts_in = sep_last_parts(h, flexible);
   PL(K_APPLYA, launch_pcg_pap(a, h->s));
   a.nparts_in = pcg_box_blocks(h->g, h->box_kc);
+  SepOut o{&h->z, false, flexible ? 2 : 1};
+  if (flexible && h->fold) {
+    // the update of x and r and the fold of r in one pass; its partials of r·r feed the forward
+    // x-y launch's convergence check
+    PL(K_VEC, launch_pcg_xr_fold(a, fold_pass(h, h->r), h->s));
+    Conv cv{CONV_RR, h->pd.fl + FL_ACT0 + (q & 1), h->pd.fl + FL_ACT0 + ((q + 1) & 1), fold_blocks(h->q)};
+    return sep_folded_tail(h, o, nullptr, cv);
+  }
   PL(K_VEC, launch_pcg_xr(a, h->s));
   Conv cv{CONV_RR, h->pd.fl + FL_ACT0 + (q & 1), h->pd.fl + FL_ACT0 + ((q + 1) & 1), pcg_box_blocks(h->g, h->box_kc)};
-  SepOut o{&h->z, false, flexible ? 2 : 1};
   CHK(sep_solve(h, h->r, o, nullptr, flexible, cv));
   return 0;
 }
@@ -1961,6 +1982,9 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     h->paptile = pt && pt[0] == '0' ? 0 : 1;
     const char *zc = std::getenv("TDGL_HIP_ZCOL");
     h->zcols = zc && zc[0] == '1' ? 1 : 0;
+    // TDGL_HIP_ZQ_WAVES=4 or 8: the waves per block of the quarter z pass
+    const char *zw = std::getenv("TDGL_HIP_ZQ_WAVES");
+    if (zw && (std::atoi(zw) == 4 || std::atoi(zw) == 8)) h->zq_waves = std::atoi(zw);
     const char *g1 = std::getenv("TDGL_HIP_PLANE1");
     if (g1 && (g1[0] == '0' || g1[0] == '2' || g1[0] == '3')) h->plane1 = g1[0] - '0';
     const char *gp = std::getenv("TDGL_HIP_PLANE");
@@ -2016,6 +2040,26 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
               ALLOC(h->Hq[ax][which][par][1], hi.size());
               CHK(upload_vec(h, h->Hq[ax][which][par][1], hi));
             }
+          }
+        }
+        {
+          // the z pass's padded bases: V(n, mz), and U(n, mz - 1) with its zero column and row
+          const int nz = d.Nz - 1, Lu = nz - 1;
+          const Bases b = make_bases(nz);
+          std::vector<float> zb[4];
+          for (int k = 0; k < 4; ++k) zb[k].assign((size_t)nz * nz + ZQ_SLACK, 0.0f);
+          for (int n = 0; n < nz; ++n)
+            for (int mz = 0; mz < nz; ++mz) {
+              const float v = (float)b.V[n + (size_t)nz * mz];
+              const float u = (mz >= 1 && n < Lu) ? (float)b.U[n + (size_t)Lu * (mz - 1)] : 0.0f;
+              zb[0][(size_t)n * nz + mz] = v;
+              zb[1][(size_t)n * nz + mz] = u;
+              zb[2][(size_t)mz * nz + n] = v;
+              zb[3][(size_t)mz * nz + n] = u;
+            }
+          for (int k = 0; k < 4; ++k) {
+            ALLOC(h->zqb[k], zb[k].size());
+            CHK(upload_vec(h, h->zqb[k], zb[k]));
           }
         }
         h->fold = true;
@@ -2278,6 +2322,116 @@ int tdgl_step_noalloc(tdgl_handle *h, int32_t n, double H, double HJ) {
     HIPCHK(hipStreamSynchronize(h->s));
   }
   return rc;
+}
+
+// ---------------------------------------------------------------------------- the tuner
+// Launch shapes chosen by timing them on the handle's own state: per candidate one time step,
+// launched from the host with HIP events around every launch, from a device copy of the state (the
+// roll-back buffers), which is put back after the last.  The PCG's box levels per block (box_kc)
+// decide how the block partials of its reductions fall, the others only how the work is cut.
+int tdgl_autotune(tdgl_handle *h, double H, double HJ, tdgl_tuning *out) {
+  if (!h || !out) return fail(TDGL_E_ARG, "null argument");
+  if (h->d.field_mode != TDGL_FIELD_PCG) return fail(TDGL_E_ARG, "the tuner times the PCG's kernels (no PCG here)");
+  if (h->prof) return fail(TDGL_E_ARG, "the per-kernel profile is on");
+  if (h->noise && h->d.noise_host) return fail(TDGL_E_ARG, "the tuner does not step on the caller's draws");
+  if (!std::isfinite(H) || !std::isfinite(HJ)) return fail(TDGL_E_ARG, "non-finite H or HJ");
+  *out = tdgl_tuning{};
+  HIPCHK(hipStreamSynchronize(h->s));
+  h->hph[0] = H;
+  h->hph[1] = HJ;
+  HIPCHK(hipMemcpyAsync(h->hp, h->hph, 2 * sizeof(double), hipMemcpyHostToDevice, h->s));
+  const HostSnap start = snap(h);
+  const int last_iters = h->last_iters;
+  PL(K_VEC, launch_copies(backup_list(h, true), h->s));
+  // what a step rewrites before reading it but the getters return (F of every band, the normal
+  // fluids' potentials) goes to the host and back
+  std::vector<std::pair<void *, std::vector<char>>> kept;
+  for (int mu = 0; mu < h->d.nbands; ++mu) kept.push_back({(void *)h->F[mu], std::vector<char>(2 * sizeof(double) * h->g.n)});
+  for (int k = 0; k < h->d.qp_nmodes && h->d.qp_mode != TDGL_QP_NONE; ++k)
+    kept.push_back({(void *)h->qS[k], std::vector<char>(sizeof(double) * (size_t)h->qn)});
+  for (auto &kv : kept) CHK(hcopy(h, kv.second.data(), kv.first, kv.second.size(), hipMemcpyDeviceToHost));
+  // one step of a candidate; the class times of it (ms per launch of `cls`, and of the second)
+  auto trial = [&](int kind, int value, int cls, int cls2, double *us) -> int {
+    restore(h, start);
+    PL(K_VEC, launch_copies(backup_list(h, false), h->s));
+    h->last_iters = last_iters;
+    PL(K_VEC, launch_pcg_cycle_reset(h->pd, h->s));
+    CHK(tdgl_profile_enable(h, 1));
+    const int rc = steps_eager(h, 1);
+    tdgl_profile p{};
+    const int rc2 = rc ? rc : tdgl_profile_get(h, &p);
+    (void)tdgl_profile_enable(h, 0);
+    if (rc2) return rc2;
+    const double n = (double)std::max<int64_t>(1, p.launches[cls]);
+    *us = 1e3 * (p.ms[cls] + (cls2 >= 0 ? p.ms[cls2] : 0.0)) / n;
+    if (out->ncand < TDGL_TUNE_MAX) out->cand[out->ncand++] = tdgl_tune_cand{kind, value, *us};
+    return 0;
+  };
+  int rc = 0;
+  // the PCG's levels per block: the operator and the vector update per iteration
+  const int kc0 = h->box_kc;
+  int best_kc = kc0;
+  double best = 1e300;
+  for (int kc = 1; kc <= 32 && rc == 0; kc *= 2) {
+    if (kc > h->g.Nz - 1 || pcg_box_blocks(h->g, kc) > MAX_PART) continue;
+    h->box_kc = kc;
+    double us = 0;
+    rc = trial(TDGL_TUNE_BOX_KC, kc, K_APPLYA, K_VEC, &us);
+    if (rc == 0 && us < best) {
+      best = us;
+      best_kc = kc;
+    }
+  }
+  h->box_kc = best_kc;
+  // m tiles per block of the one-round x-y kernel
+  const int mt0 = h->plane1;
+  int best_mt = mt0;
+  best = 1e300;
+  for (int mt = 2; mt <= 3 && rc == 0; ++mt) {
+    h->plane1 = mt;
+    double us = 0;
+    rc = trial(TDGL_TUNE_PLANE_MT, mt, K_TX32, -1, &us);
+    if (rc == 0 && us < best) {
+      best = us;
+      best_mt = mt;
+    }
+  }
+  h->plane1 = best_mt;
+  // waves (groups of z modes) per block of the quarter z pass
+  const int zw0 = h->zq_waves;
+  int best_zw = zw0;
+  if (h->fold) {
+    best = 1e300;
+    for (int zw = 4; zw <= 8 && rc == 0; zw *= 2) {
+      h->zq_waves = zw;
+      double us = 0;
+      rc = trial(TDGL_TUNE_ZMODES_WAVES, zw, K_MODES, -1, &us);
+      if (rc == 0 && us < best) {
+        best = us;
+        best_zw = zw;
+      }
+    }
+    h->zq_waves = best_zw;
+  }
+  // the state back where it was, whatever happened
+  restore(h, start);
+  h->last_iters = last_iters;
+  const hipError_t e = launch_copies(backup_list(h, false), h->s);
+  hipError_t e2 = hipStreamSynchronize(h->s);
+  for (auto &kv : kept)
+    if (e2 == hipSuccess) e2 = hipMemcpy(kv.first, kv.second.data(), kv.second.size(), hipMemcpyHostToDevice);
+  CHK(destroy_graph(h));  // captured cycles hold the old shapes
+  if (rc) {
+    h->box_kc = kc0;
+    h->plane1 = mt0;
+    h->zq_waves = zw0;
+    return rc;
+  }
+  if (e != hipSuccess || e2 != hipSuccess) return fail(TDGL_E_DEVICE, "the tuner's restore of the state");
+  out->box_kc = h->box_kc;
+  out->plane_mt = h->plane1;
+  out->zmodes_waves = h->zq_waves;
+  return 0;
 }
 
 int tdgl_get_pcg(tdgl_handle *h, tdgl_pcg *p) {

@@ -9,7 +9,8 @@ import threading
 
 import numpy as np
 
-from .abi import NKERNELS, CycleRecordC, MeterC, OpenC, PackedDesc, PcgState, ProfileC, QpC, StatsC, state_struct
+from .abi import (NKERNELS, TUNE_KINDS, CycleRecordC, MeterC, OpenC, PackedDesc, PcgState, ProfileC, QpC, StatsC, TuningC,
+                  state_struct)
 from .scheme import Stepper
 from .state import State
 
@@ -20,7 +21,7 @@ SYMBOLS = ("tdgl_supported", "tdgl_create", "tdgl_destroy", "tdgl_upload_state",
            "tdgl_get_pcg", "tdgl_set_pcg", "tdgl_get_open", "tdgl_set_open", "tdgl_get_qp", "tdgl_get_stats", "tdgl_last_error", "tdgl_device_count",
            "tdgl_set_device", "tdgl_profile_enable", "tdgl_profile_get", "tdgl_kernel_name", "tdgl_meter_begin",
            "tdgl_meter_cycle", "tdgl_noise_get", "tdgl_noise_set", "tdgl_noise_draws", "tdgl_batch_create",
-           "tdgl_batch_step", "tdgl_batch_meter_cycle", "tdgl_batch_destroy", "tdgl_step_noalloc")
+           "tdgl_batch_step", "tdgl_batch_meter_cycle", "tdgl_batch_destroy", "tdgl_step_noalloc", "tdgl_autotune")
 
 _lib = None
 _lock = threading.Lock()
@@ -51,6 +52,7 @@ def load():
             L.tdgl_set_pcg.argtypes = [vp, vp]
             L.tdgl_get_stats.argtypes = [vp, vp]
             L.tdgl_get_qp.argtypes = [vp, vp]
+            L.tdgl_autotune.argtypes = [vp, ct.c_double, ct.c_double, vp]
             L.tdgl_last_error.restype = ct.c_char_p
             L.tdgl_device_count.restype = ct.c_int
             L.tdgl_set_device.argtypes = [ct.c_int32]
@@ -189,6 +191,15 @@ class HipStepper:
             c.pot[k] = a.ctypes.data_as(ct.POINTER(ct.c_double))
         _check(load().tdgl_get_qp(self.h, ct.byref(c)), "tdgl_get_qp")
         return out
+
+    def autotune(self, H: float, HJ: float) -> dict:
+        """Times the candidate launch shapes on the handle's own state (tdgl_autotune) and keeps the
+        fastest; the state is left as it was.  Returns the choice and every candidate's time."""
+        c = TuningC()
+        _check(load().tdgl_autotune(self.h, float(H), float(HJ), ct.byref(c)), "tdgl_autotune")
+        return {"box_kc": c.box_kc, "plane_mt": c.plane_mt, "zmodes_waves": c.zmodes_waves,
+                "candidates": [(TUNE_KINDS.get(c.cand[i].kind, "?"), c.cand[i].value, c.cand[i].us)
+                               for i in range(c.ncand)]}
 
     def meter_begin(self, vm) -> None:
         """Uploads the voltmeter `vm` (juliatdgl_amd.meter.Voltmeter) and takes |Ψ| and hz of the

@@ -388,7 +388,7 @@ def test_pcg_budget_settles_on_a_current_ramp():
     assert g["rollbacks"] <= 4 and g["captures"] <= 8
 
 
-VARIANTS = [("TDGL_HIP_ZCOL", "1", "0"), ("TDGL_HIP_PAPTILE", "0", "1")]
+VARIANTS = [("TDGL_HIP_ZCOL", "1", "0"), ("TDGL_HIP_PAPTILE", "0", "1"), ("TDGL_HIP_ZQ_WAVES", "8", "4")]
 
 
 @pytest.mark.parametrize("variant", VARIANTS, ids=lambda v: v[0])
@@ -397,8 +397,9 @@ VARIANTS = [("TDGL_HIP_ZCOL", "1", "0"), ("TDGL_HIP_PAPTILE", "0", "1")]
 def test_kernel_variants_give_the_same_bits(name, variant, monkeypatch):
     """Kernels rewritten for speed keep the arithmetic of the ones they replace: the z pass over
     x-runs of modes (k_zmodes_run) against whole columns per block (TDGL_HIP_ZCOL=1), the LDS-tiled
-    PCG operator (k_pcg_pap_tile) against one gather per stencil value (TDGL_HIP_PAPTILE=0).  Eight
-    steps give the same bits either way."""
+    PCG operator (k_pcg_pap_tile) against one gather per stencil value (TDGL_HIP_PAPTILE=0), and the
+    quarter z pass (k_zmodes_q) with 8 against 4 waves a block (each mode's sums in the same order).
+    Eight steps give the same bits either way."""
     HipStepper = _hip()
     st, _ = make_case(name)
     s0, pcg, H, HJ = seed_state(st)
