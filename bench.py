@@ -61,18 +61,26 @@ def kernel_bytes(st, name: str, launches: int, nsteps: int, stats: dict) -> floa
         per = (12.0 * (it + 1) + 20.0 + 28.0 * it) / (2.0 * (it + 1))
         return per * rng * launches
     if name == "sep_fold":
+        if stats.get("folded"):
+            # per solve of `it` iterations: one fold (FP64 in, FP32 quarters out: 12 B, the start's;
+            # the iterations' folds ride on the x/r update), the start's unfold (FP32 in, FP64 z
+            # out, r read for r·z: 20 B) and `it` unfolds that also read z_old (28 B)
+            it = stats.get("iters_per_solve", 0.0)
+            return (12.0 + 20.0 + 28.0 * it) / (it + 2.0) * rng * launches
         # the fold (FP64 in, FP32 quarters out: 12 B) and the unfold (FP32 in, FP64 z out, r and
         # z_old read for the dots: 28 B), one each per preconditioner
         return 20.0 * rng * launches
     if name == "sep_modes":
-        return 8.0 * rng * launches
-    if name in ("pcg_apply_A", "pcg_vector"):
-        # apply_A: z, P_old, D read, P, z_old, Q written; vector: x, P, r, Q read, x, r written
-        return 48.0 * rng * launches
+        return 8.0 * (quarter_total(gr) if stats.get("folded") else rng) * launches
+    if name == "pcg_apply_A":
+        # z, P_old, D read, P, Q written (and z_old = z written unless the folded solve alternates
+        # the two buffers)
+        return (40.0 if stats.get("folded") else 48.0) * rng * launches
+    if name == "pcg_vector":
+        # x, P, r, Q read, x, r written (folded: and the Float32 quarters of r's fold written)
+        return (52.0 if stats.get("folded") else 48.0) * rng * launches
     if name == "current_field":
         return (16 * N + 48 + 24 * 6 + 24 + 24) * nodes * launches
-    if name == "links_curl_bc":
-        return 96.0 * nodes * launches
     return 0.0
 
 
@@ -305,7 +313,10 @@ def roofline(st, dom, nl, ms, nsteps, iters, workload, folded=False):
     if byts > 0 and (flops == 0 or flops / byts < peak / HBM_PEAK):
         roof = {"bound": "hbm", "achieved": byts / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": byts / avg_s / HBM_PEAK, "traffic": None, "kernel": dom, "bytes_per_launch": byts,
-                "avg_launch_us": avg_s * 1e6}
+                "avg_launch_us": avg_s * 1e6,
+                "note": "algorithmic bytes: each array the launch must touch counted once (FP64 node-shaped vectors, "
+                        "FP32 quarters); durations from HIP events recorded at each launch's start and end over the "
+                        "same number of cycles as the timed region"}
     else:
         roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                 "frac": flops / avg_s / peak, "traffic": None, "kernel": dom, "flops_per_launch": flops,
@@ -391,6 +402,25 @@ def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=Non
         out["roofline"] = roofline(st, dom, nl, ms, nsteps, iters, name, bool(st1.get("folded", 0)))
         out["config"]["folded_preconditioner"] = bool(st1.get("folded", 0))
         out["config"]["kernel_ms_per_cycle"] = {k: v[1] / steps for k, v in prof.items() if v[0]}
+        # every class with algorithmic bytes: its average launch against the HBM peak (and the
+        # transforms against the matrix peak)
+        per = {}
+        for kname, (nl_k, ms_k) in prof.items():
+            if not nl_k:
+                continue
+            b = kernel_bytes(st, kname, nl_k, nsteps, {"iters_per_solve": iters / nsteps,
+                                                       "folded": bool(st1.get("folded", 0))}) / nl_k
+            if b <= 0:
+                continue
+            us = 1e3 * ms_k / nl_k
+            row = {"us_per_launch": us, "bytes_per_launch": b, "GB_s": b / us / 1e3, "hbm_frac": b / us / 1e3 / (HBM_PEAK / 1e9)}
+            f = kernel_flops(st, kname, nl_k, bool(st1.get("folded", 0))) / nl_k
+            if f > 0:
+                pk = FP32_MFMA_PEAK if kname == "sep_transform_f32" else FP64_PEAK
+                row["TFLOP_s"] = f / us / 1e6
+                row["mfma_frac"] = f / us / 1e6 / (pk / 1e12)
+            per[kname] = row
+        out["config"]["kernel_roofline"] = per
         if profile_out:
             with open(profile_out, "w") as fh:
                 json.dump({k: {"launches": v[0], "ms": v[1]} for k, v in prof.items()}, fh, indent=1)
@@ -454,7 +484,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
                     help="leave out the 128x64x16 KWT workload that the 256x128x32 line carries beside it")
-    ap.add_argument("--tune", action="store_true", help="run the library's launch-shape tuner before the warm-up")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the library's default launch shapes (by default its tuner picks them before the warm-up)")
     ap.add_argument("--profile-out", default=None, help="write the per-kernel profile (JSON) here")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy a MIB-MiB device buffer (a reference kernel for the counters' scale)")
@@ -473,14 +504,14 @@ def main(argv=None):
     if args.copy_probe:
         copy_probe(args.copy_probe)
     cpu_steps = 0 if args.no_cpu else (args.cpu_steps or 8)
-    r = run_workload(args.workload, args.steps, args.warmup, profile=True, cpu_steps=cpu_steps, tune=args.tune,
+    r = run_workload(args.workload, args.steps, args.warmup, profile=True, cpu_steps=cpu_steps, tune=not args.no_tune,
                      profile_out=args.profile_out)
     published = PUBLISHED.get(args.workload)
     cfg = dict(r["config"])
     if published:
         cfg["reference_published"] = published
     if args.workload == "open_3band_256x128x32" and not args.no_secondary:
-        sec = run_workload("kwt_pcg_128x64x16", min(args.steps, 10), 3, profile=False, tune=args.tune)
+        sec = run_workload("kwt_pcg_128x64x16", min(args.steps, 10), 3, profile=False, tune=not args.no_tune)
         cfg["secondary"] = {"kwt_pcg_128x64x16": {"cell_cycles_per_s": sec["value"],
                                                   "ms_per_cycle": 1e3 * sec["wall"] / sec["steps"],
                                                   **{k: sec["config"][k] for k in ("workload", "cells",

@@ -328,6 +328,23 @@ struct ZModesQ {
   int waves;             // waves per block, each a group of z modes: 4 or 8, 0 the library's choice
 };
 hipError_t launch_zmodes_q(const ZModesQ &m, const int *gate, hipStream_t s);
+
+// The x-y transforms of the twelve quarters (three components by four parities) in one launch:
+// out(m, q) = Σ_i Σ_j X(i, m) in(i, j) Y(j, q) per z plane, with the half-size bases given
+// transposed and zero-padded to whole 16 x 16 tiles: XT[m QS_x + i] = X(i, m), YT[q QS_y + j] =
+// Y(j, q), QS = 16 ceil(L/16) for a basis of L = e rows.  A block per (quarter, plane), a
+// wavefront per 16 rows of m.  x extents up to 16 QP_MT, y extents up to 16 QP_JT.
+constexpr int QP_MT = 10, QP_JT = 6;
+struct QPass {
+  View in[12], out[12];
+  const float *XT[12], *YT[12];
+  int e[12][3];
+  Conv conv;
+  PcgDev d;
+};
+bool qplane_fits(const QPass &p);
+int qplane_parts(const QPass &p);
+hipError_t launch_qplane(const QPass &p, const int *gate, hipStream_t s);
 constexpr int ZQ_NZ_MAX = 64;  // z extents the quarter z pass takes (16 modes a wave at 4 waves a block)
 // The quarter z pass's bases, float, each nz x nz followed by ZQ_SLACK zeros: [0] V(n, mz) at
 // n nz + mz, [1] U(n, mz - 1) at n nz + mz (zero for mz = 0 or n = nz - 1), [2] and [3] the same at
@@ -463,6 +480,8 @@ struct PcgArgs {
   double *ph[3];
   int act_slot;      // finish: the flag that tells an unfinished solve
   int tile;          // pap: the LDS-tiled kernel (k_pcg_pap_tile), else one gather per stencil value
+  int zcopy;         // pap, flexible: store zold = z (0: the caller alternates z and zold instead)
+  int xcd;           // box kernels: blocks to the XCDs in contiguous runs (box_idx)
 };
 int pcg_box_blocks(const Geo &g, int kc);
 int box_kc(const Geo &g);

@@ -1957,6 +1957,172 @@ template hipError_t launch_plane<double, float, float>(const PlanePass &, const 
 template hipError_t launch_plane<float, float, double>(const PlanePass &, const int *, hipStream_t);
 template hipError_t launch_plane<double, double, double>(const PlanePass &, const int *, hipStream_t);
 template hipError_t launch_plane<float, float, float>(const PlanePass &, const int *, hipStream_t);
+
+// ---------------------------------------------------------------- the quarter-plane transforms
+// k_qplane: one block per (quarter, z plane) holds the whole plane in LDS once (Bt[j][i], rows of
+// 4 (mod 8) words), and wave w computes the 16 rows m = 16 w .. 16 w + 15 of both products on
+// v_mfma_f32_16x16x4f32: phase 1 S(m, j) = Σ_i X(i, m) in(i, j) with A = X read from the padded
+// transposed basis in global memory (16 bytes a lane, every block reads the same few KB: cache
+// hits) and B = the plane from LDS; S goes to LDS over the plane (St[m][j]); phase 2 the
+// transposed product out(q, m) = Σ_j Y(j, q) S(m, j) with A = Y from global memory and B = S from
+// LDS, so that a lane's results lie along m, the fastest index of the output.  The operands of
+// one instruction and the order of the instructions are those of k_plane_mf1 (group g, step u,
+// lane k index lk: k = 16 g + 4 lk + u), and the padding is zero in both: the same bits.
+// The next group's global operands are requested while the current group multiplies.
+template <int JT>
+__global__ void __launch_bounds__(64 * QP_MT) k_qplane(QPass p, const int *gate, int e2max, int mtx, int jtx) {
+  typedef v4f V4;
+  extern __shared__ __align__(16) unsigned char qplane_smem[];
+  __shared__ double sh[16];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int bid = blockIdx.x, comp = bid / e2max, plane = bid % e2max;
+  if (p.conv.mode != CONV_NONE) {
+    if (!*p.conv.in) {
+      if (bid == 0 && tid == 0) *p.conv.out = 0;
+      return;
+    }
+  } else if (gate && !*gate) {
+    return;
+  }
+  const int e0 = p.e[comp][0], e1 = p.e[comp][1], e2 = p.e[comp][2];
+  if (p.conv.mode != CONV_NONE) {
+    const double rr = sum_parts_8w(region(p.d, PART_RR), p.conv.nparts, 0, sh, tid);
+    const double tol = p.d.sc[SC_TOL], nrm = p.d.sc[SC_NB];
+    const int active = p.conv.mode == CONV_RR0 ? ((nrm > 0 && sqrt(rr) > tol * nrm) ? 1 : 0)
+                                               : ((sqrt(rr) <= tol * nrm || p.d.fl[FL_IT] >= p.d.fl[FL_MAXIT]) ? 0 : 1);
+    if (bid == 0 && tid == 0) *p.conv.out = active;
+    if (!active) return;
+  }
+  if (plane >= e2) return;
+  const int mt = (e0 + 15) / 16, jt = (e1 + 15) / 16;  // this quarter's tiles along m (and i), j (and q)
+  const int E0S = 16 * mtx + 4, JS = 16 * jtx + 4;     // LDS row strides (the launch's largest tiles)
+  const int xs = 16 * mt, ys = 16 * jt;                // the padded bases' row strides
+  float *Bt = (float *)qplane_smem, *St = Bt;
+  {
+    // the plane, zero beyond e0 x e1 up to whole tiles; eight loads in flight per thread
+    const float *src = (const float *)p.in[comp].p + (size_t)p.in[comp].s2 * plane;
+    const int s1 = p.in[comp].s1, W = 16 * mt, n = W * 16 * jt;
+    for (int t0 = 0; t0 < n; t0 += 8 * nth) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = t0 + tid + u * nth, ii = t % W, j = t / W;
+        v[u] = (t < n && ii < e0 && j < e1) ? src[ii + (size_t)s1 * j] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = t0 + tid + u * nth, ii = t % W, j = t / W;
+        if (t < n) Bt[j * E0S + ii] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, lr = l & 15, lk = l >> 4;
+  const bool on = w < mt;
+  V4 acc[JT];
+#pragma unroll
+  for (int c = 0; c < JT; ++c) acc[c] = V4{0, 0, 0, 0};
+  if (on) {
+    // phase 1: A = X(16 g + 4 lk + u, 16 w + lr) from XT, B = in(16 g + 4 lk + u, 16 c + lr) from Bt
+    const float *xa = p.XT[comp] + (size_t)(16 * w + lr) * xs + 4 * lk;
+    V4 A = *(const V4 *)xa;
+    for (int g = 0; g < mt; ++g) {
+      const V4 An = *(const V4 *)(xa + 16 * min(g + 1, mt - 1));
+      V4 B[JT];
+#pragma unroll
+      for (int c = 0; c < JT; ++c)
+        B[c] = c < jt ? *(const V4 *)(Bt + (16 * c + lr) * E0S + 16 * g + 4 * lk) : V4{0, 0, 0, 0};
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int c = 0; c < JT; ++c)
+          if (c < jt) acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[u], B[c][u], acc[c], 0, 0, 0);
+      A = An;
+    }
+  }
+  __syncthreads();  // every wave is done with the plane: S goes over it
+  if (on)
+#pragma unroll
+    for (int c = 0; c < JT; ++c)
+      if (c < jt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) St[(16 * w + 4 * lk + r) * JS + 16 * c + lr] = acc[c][r];
+  __syncthreads();
+  if (!on) return;
+  // phase 2: A = Y(16 g + 4 lk + u, 16 c + lr) from YT, B = S(16 w + lr, 16 g + 4 lk + u) from St
+#pragma unroll
+  for (int c = 0; c < JT; ++c) acc[c] = V4{0, 0, 0, 0};
+  const float *ya = p.YT[comp] + (size_t)lr * ys + 4 * lk;
+  V4 Y[JT];
+#pragma unroll
+  for (int c = 0; c < JT; ++c) Y[c] = c < jt ? *(const V4 *)(ya + (size_t)16 * c * ys) : V4{0, 0, 0, 0};
+  for (int g = 0; g < jt; ++g) {
+    const int gn = 16 * min(g + 1, jt - 1);
+    V4 Yn[JT];
+#pragma unroll
+    for (int c = 0; c < JT; ++c) Yn[c] = c < jt ? *(const V4 *)(ya + (size_t)16 * c * ys + gn) : V4{0, 0, 0, 0};
+    const V4 B = *(const V4 *)(St + (16 * w + lr) * JS + 16 * g + 4 * lk);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < JT; ++c)
+        if (c < jt) acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(Y[c][u], B[u], acc[c], 0, 0, 0);
+#pragma unroll
+    for (int c = 0; c < JT; ++c) Y[c] = Yn[c];
+  }
+  // out(q, m): element r of lane l is q = 16 c + 4 lk + r, m = 16 w + lr
+  const View vo = p.out[comp];
+  float *dst = (float *)vo.p + (size_t)vo.s2 * plane;
+  const int m = 16 * w + lr;
+#pragma unroll
+  for (int c = 0; c < JT; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = 16 * c + 4 * lk + r;
+      if (c < jt && m < e0 && q < e1) dst[m + (size_t)vo.s1 * q] = acc[c][r];
+    }
+}
+
+static void qplane_tiles(const QPass &p, int &mtx, int &jtx, int &e2max) {
+  mtx = jtx = e2max = 1;
+  for (int c = 0; c < 12; ++c) {
+    mtx = std::max(mtx, cdiv(p.e[c][0], 16));
+    jtx = std::max(jtx, cdiv(p.e[c][1], 16));
+    e2max = std::max(e2max, p.e[c][2]);
+  }
+}
+static size_t qplane_lds(int mtx, int jtx) {
+  return (size_t)std::max(16 * jtx * (16 * mtx + 4), 16 * mtx * (16 * jtx + 4)) * sizeof(float);
+}
+bool qplane_fits(const QPass &p) {
+  int mtx, jtx, e2max;
+  qplane_tiles(p, mtx, jtx, e2max);
+  for (int c = 0; c < 12; ++c)
+    if (p.e[c][0] < 1 || p.e[c][1] < 1 || p.e[c][2] < 1) return false;
+  return mtx <= QP_MT && jtx <= QP_JT && qplane_lds(mtx, jtx) <= 160 * 1024;
+}
+int qplane_parts(const QPass &p) {
+  int mtx, jtx, e2max;
+  qplane_tiles(p, mtx, jtx, e2max);
+  return 12 * e2max;
+}
+hipError_t launch_qplane(const QPass &p, const int *gate, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
+  if (!qplane_fits(p)) return hipErrorInvalidValue;
+  int mtx, jtx, e2max;
+  qplane_tiles(p, mtx, jtx, e2max);
+  const dim3 grid(12 * e2max), blk(64 * std::max(4, mtx));
+  const size_t lds = qplane_lds(mtx, jtx);
+  switch (jtx) {
+    case 1: TDGL_LAUNCH((k_qplane<1>), grid, blk, lds, s, p, gate, e2max, mtx, jtx); break;
+    case 2: TDGL_LAUNCH((k_qplane<2>), grid, blk, lds, s, p, gate, e2max, mtx, jtx); break;
+    case 3: TDGL_LAUNCH((k_qplane<3>), grid, blk, lds, s, p, gate, e2max, mtx, jtx); break;
+    case 4: TDGL_LAUNCH((k_qplane<4>), grid, blk, lds, s, p, gate, e2max, mtx, jtx); break;
+    case 5: TDGL_LAUNCH((k_qplane<5>), grid, blk, lds, s, p, gate, e2max, mtx, jtx); break;
+    default: TDGL_LAUNCH((k_qplane<6>), grid, blk, lds, s, p, gate, e2max, mtx, jtx); break;
+  }
+  return hipGetLastError();
+}
 template int plane_parts<float>(const PlanePass &);
 template int plane_parts<double>(const PlanePass &);
 template bool plane_fits<float>(const int e[][3], int);
@@ -2631,9 +2797,12 @@ struct BoxIdx {
   int i, j, k, c, bid;
   int nk;  // levels of the block: k .. k + nk - 1
 };
-__device__ __forceinline__ BoxIdx box_idx(const Geo &g, int kc) {
+// With `xcd`, the hardware's block order is taken through xcd_run: each XCD receives a contiguous
+// run of the (x, y, level) order, so that the neighbours whose halos a stencil block shares mostly
+// sit behind the same L2.  The block's partial slot is its place in that order either way.
+__device__ __forceinline__ BoxIdx box_idx(const Geo &g, int kc, int xcd) {
   const int gx = cdiv_d(g.Nx - 1, 64), gy = cdiv_d(g.Ny - 1, 4);
-  const int b = blockIdx.x;
+  const int b = xcd ? xcd_run((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
   BoxIdx r;
   r.bid = b;
   const int bx = b % gx, by = (b / gx) % gy, bz = b / (gx * gy);
@@ -2737,7 +2906,7 @@ __device__ __forceinline__ int box_load_c(const Geo &g, const BoxIdx &b) { retur
 __global__ void __launch_bounds__(RED_T) k_pcg_start(PcgArgs a) {
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const BoxIdx b0 = box_idx(a.g, a.kc, a.xcd);
   const bool live = box_live(a.g, b0);
   // the first level's values are requested before the partials are summed
   const int cl = box_load_c(a.g, b0);
@@ -2784,7 +2953,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_resid(PcgArgs a) {
   if (!a.d.fl[FL_NZ]) return;
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const BoxIdx b0 = box_idx(a.g, a.kc, a.xcd);
   double s = 0.0;
   if (box_live(a.g, b0))
     for (int q = 0; q < b0.nk; ++q) {
@@ -2814,7 +2983,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_p(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const BoxIdx b0 = box_idx(a.g, a.kc, a.xcd);
   const bool live = box_live(a.g, b0);
   double rz2, rzo;
   sum_parts2(region(a.d, PART_RZ), a.nparts_in, rz2, rzo, sh, tid);
@@ -2846,7 +3015,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_ap(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const BoxIdx b0 = box_idx(a.g, a.kc, a.xcd);
   double s = 0.0;
   if (box_live(a.g, b0))
     for (int q = 0; q < b0.nk; ++q) {
@@ -2875,7 +3044,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const BoxIdx b0 = box_idx(a.g, a.kc, a.xcd);
   const bool live = box_live(a.g, b0);
   const int sy = a.g.px, sz = a.g.px * a.g.py;
   const int cl = box_load_c(a.g, b0);
@@ -2930,7 +3099,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap(PcgArgs a) {
         if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
         const double pn = v[cc[comp]];
         a.Pnew.c[comp][b.c] = pn;
-        if (a.flexible) a.zold.c[comp][b.c] = vz[cc[comp]];
+        if (a.flexible && a.zcopy) a.zold.c[comp][b.c] = vz[cc[comp]];
         a.Q.c[comp][b.c] = qv[comp];
         s += pn * qv[comp];
       }
@@ -2955,7 +3124,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap_tile(PcgArgs a) {
   __shared__ double sh[8];
   __shared__ double Pl[3][3][PT_N];  // [component][level slot][row * 66 + col]
   const int tid = threadIdx.x;
-  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const BoxIdx b0 = box_idx(a.g, a.kc, a.xcd);
   const bool live = box_live(a.g, b0);
   const int i0 = b0.i - (tid & 63) - 1, j0 = b0.j - (tid >> 6) - 1;  // the tile's first (halo) node
   const bool pold = a.q > 0;
@@ -3046,7 +3215,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap_tile(PcgArgs a) {
         if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
         const double pn = v[cc[comp]];
         a.Pnew.c[comp][b.c] = pn;
-        if (a.flexible) a.zold.c[comp][b.c] = a.z.c[comp][b.c];
+        if (a.flexible && a.zcopy) a.zold.c[comp][b.c] = a.z.c[comp][b.c];
         a.Q.c[comp][b.c] = qv[comp];
         s += pn * qv[comp];
       }
@@ -3067,7 +3236,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_xr(PcgArgs a) {
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const BoxIdx b0 = box_idx(a.g, a.kc, a.xcd);
   const bool live = box_live(a.g, b0);
   const int cl = box_load_c(a.g, b0);
   double xv[3], pv[3], rv[3], qv[3];
@@ -3122,7 +3291,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_xr(PcgArgs a) {
 
 // φ = θ + x on the ranges (implicit.jl:440-444) and the cycle's counts of the solve
 __global__ void __launch_bounds__(RED_T) k_pcg_finish(PcgArgs a) {
-  const BoxIdx b0 = box_idx(a.g, a.kc);
+  const BoxIdx b0 = box_idx(a.g, a.kc, a.xcd);
   if (box_live(a.g, b0))
     for (int q = 0; q < b0.nk; ++q) {
       const BoxIdx b = box_level(a.g, b0, q);

@@ -151,6 +151,9 @@ struct tdgl_handle {
   Quarters q{};
   float *qa = nullptr, *qb = nullptr;
   float *Hq[2][2][2][2] = {};
+  float *Hp[2][2][2][2] = {};  // Hq zero-padded to whole 16 x 16 tiles
+  int HpL[2][2] = {};          // their unpadded order
+  int qplane = 1;              // TDGL_HIP_QPLANE=0: the quarter x-y transforms on k_plane_mf1 / k_plane_mf
   int zq_waves = 0;        // waves per block of the quarter z pass (0: the library's choice)
   float *zqb[4] = {};      // its padded z bases (ZQ_BASES)
   bool tx_only = false;                   // the separable solve as single-axis passes (a mixed batch)
@@ -191,6 +194,8 @@ struct tdgl_handle {
   long long captures = 0;  // graphs captured and instantiated
   int zcols = 0;           // TDGL_HIP_ZCOL=1: the z pass of whole columns per block (k_zmodes)
   int paptile = 1;         // TDGL_HIP_PAPTILE=0: the PCG operator with a gather per stencil value (k_pcg_pap)
+  int boxxcd = 1;          // TDGL_HIP_BOXXCD=0: the box kernels' blocks in the hardware's order
+  int zswap = 1;           // TDGL_HIP_ZSWAP=0: the folded PCG copies z into zold instead of alternating them
   double *bk[48] = {};     // device copy of a cycle's start (roll back)
   // thermal noise: the stream's step counter on the device ([0] steps taken, [1] the step of the
   // first of the caller's draws) and its host copy; the caller's draws; SK-ROCK's δθ and source;
@@ -739,6 +744,7 @@ struct SepOut {
   Vec3 *out;          // a node-shaped vector
   bool addth;         // or φ = θ + value
   int dot;            // with the partials of r·z (1) and r·zold (2)
+  const Vec3 *zold = nullptr;  // the zold of the dot (the handle's when null)
 };
 
 // (I + aK)^{-1} of the node-shaped vector `in` (_sep_solve!, implicit.jl:62-83, or in Float32
@@ -771,6 +777,43 @@ PlanePass quarter_pass(tdgl_handle *h, bool inv, float *in, float *out) {
   return p;
 }
 
+// the same passes for k_qplane (the padded bases)
+QPass qplane_pass(tdgl_handle *h, bool inv, float *in, float *out) {
+  const PlanePass p = quarter_pass(h, inv, in, out);
+  QPass qp{};
+  qp.d = h->pd;
+  for (int c = 0; c < 3; ++c)
+    for (int px = 0; px < 2; ++px)
+      for (int py = 0; py < 2; ++py) {
+        const int k = 4 * c + 2 * px + py;
+        qp.in[k] = p.in[k];
+        qp.out[k] = p.out[k];
+        for (int e = 0; e < 3; ++e) qp.e[k][e] = p.e[k][e];
+        const int wx = c == 0 ? 1 : 0, wy = c == 1 ? 1 : 0;
+        qp.XT[k] = h->Hp[0][wx][px][!inv];
+        qp.YT[k] = h->Hp[1][wy][py][!inv];
+        // the padded bases are 16 ceil(e/16) square: a quarter of another extent stays off k_qplane
+        if (h->HpL[0][wx] != p.e[k][0] || h->HpL[1][wy] != p.e[k][1]) qp.e[k][2] = 0;
+      }
+  return qp;
+}
+
+// one x-y launch of the quarters: k_qplane where it takes them, else the plane kernels
+int quarter_xy(tdgl_handle *h, bool inv, float *in, float *out, const int *gate, Conv conv) {
+  if (h->qplane) {
+    QPass qp = qplane_pass(h, inv, in, out);
+    qp.conv = conv;
+    if (qplane_fits(qp)) {
+      PL(K_TX32, launch_qplane(qp, gate, h->s));
+      return 0;
+    }
+  }
+  PlanePass p = quarter_pass(h, inv, in, out);
+  p.conv = conv;
+  PL(K_TX32, (launch_plane<float, float, float>(p, gate, h->s)));
+  return 0;
+}
+
 // (I + aK)^{-1} in Float32 with the folds (_precond32!, implicit.jl:353-373): fold x and y, the x-y
 // passes of the quarters, the z pass with the projector on the quarters, the inverse x-y passes,
 // unfold into the node-shaped output with the partials of its dots.
@@ -785,9 +828,7 @@ FoldPass fold_pass(tdgl_handle *h, const Vec3 &in) {
 // what follows the fold (h->qa holds the folded input): the forward x-y launch, which takes the
 // convergence check of a first pass, the z pass, the inverse x-y launch, the unfold
 int sep_folded_tail(tdgl_handle *h, const SepOut &o, const int *gate, Conv conv) {
-  PlanePass fw = quarter_pass(h, false, h->qa, h->qb);
-  fw.conv = conv;
-  PL(K_TX32, (launch_plane<float, float, float>(fw, gate, h->s)));
+  CHK(quarter_xy(h, false, h->qa, h->qb, gate, conv));
   const int *g2 = conv.mode != CONV_NONE ? conv.out : gate;
   ZModesQ zq{};
   zq.modes = h->qb;
@@ -802,7 +843,7 @@ int sep_folded_tail(tdgl_handle *h, const SepOut &o, const int *gate, Conv conv)
   zq.a = h->d.a;
   zq.waves = h->zq_waves;
   PL(K_MODES, launch_zmodes_q(zq, g2, h->s));
-  PL(K_TX32, (launch_plane<float, float, float>(quarter_pass(h, true, h->qb, h->qa), g2, h->s)));
+  CHK(quarter_xy(h, true, h->qb, h->qa, g2, Conv{}));
   UnfoldPass u{};
   u.in = h->qa;
   u.q = h->q;
@@ -810,7 +851,7 @@ int sep_folded_tail(tdgl_handle *h, const SepOut &o, const int *gate, Conv conv)
     u.out[c] = node_view(h, o.out->c[c]);
     if (o.dot) {
       u.r[c] = node_view(h, h->r.c[c]);
-      u.zold[c] = node_view(h, h->zold.c[c]);
+      u.zold[c] = node_view(h, (o.zold ? o.zold : &h->zold)->c[c]);
     }
   }
   u.dot = o.dot;
@@ -967,6 +1008,8 @@ PcgArgs pcg_args(tdgl_handle *h) {
   a.d = h->pd;
   a.kc = h->box_kc;
   a.tile = h->paptile;
+  a.zcopy = 1;
+  a.xcd = h->boxxcd;
   for (int c = 0; c < 3; ++c) {
     a.D[c] = h->D[c];
     a.th[c] = h->th[c];
@@ -1001,16 +1044,26 @@ int pcg_iteration(tdgl_handle *h, int q, bool flexible) {
   a.Pnew = (q % 2 == 0) ? h->P : h->P2;
   a.Pold = (q % 2 == 0) ? h->P2 : h->P;
   a.nparts_in = sep_last_parts(h, flexible);
-  PL(K_APPLYA, launch_pcg_pap(a, h->s));
-  a.nparts_in = pcg_box_blocks(h->g, h->box_kc);
-  SepOut o{&h->z, false, flexible ? 2 : 1};
   if (flexible && h->fold) {
+    // z and zold alternate between the two buffers instead of a copy: iteration q's z is the
+    // preconditioner's output of iteration q - 1 (of the start for q = 0, in h->z), and iteration
+    // q's preconditioner writes the other buffer, its r·zold read from this one
+    const bool swap = h->zswap;
+    Vec3 *zq = (swap && (q & 1)) ? &h->zold : &h->z, *zn = (swap && !(q & 1)) ? &h->zold : &h->z;
+    a.z = *zq;
+    a.zcopy = swap ? 0 : 1;
+    PL(K_APPLYA, launch_pcg_pap(a, h->s));
+    a.nparts_in = pcg_box_blocks(h->g, h->box_kc);
     // the update of x and r and the fold of r in one pass; its partials of r·r feed the forward
     // x-y launch's convergence check
     PL(K_VEC, launch_pcg_xr_fold(a, fold_pass(h, h->r), h->s));
     Conv cv{CONV_RR, h->pd.fl + FL_ACT0 + (q & 1), h->pd.fl + FL_ACT0 + ((q + 1) & 1), fold_blocks(h->q)};
+    SepOut o{zn, false, 2, swap ? zq : nullptr};
     return sep_folded_tail(h, o, nullptr, cv);
   }
+  PL(K_APPLYA, launch_pcg_pap(a, h->s));
+  a.nparts_in = pcg_box_blocks(h->g, h->box_kc);
+  SepOut o{&h->z, false, flexible ? 2 : 1};
   PL(K_VEC, launch_pcg_xr(a, h->s));
   Conv cv{CONV_RR, h->pd.fl + FL_ACT0 + (q & 1), h->pd.fl + FL_ACT0 + ((q + 1) & 1), pcg_box_blocks(h->g, h->box_kc)};
   CHK(sep_solve(h, h->r, o, nullptr, flexible, cv));
@@ -1980,8 +2033,14 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     // =2 or =3: the one-round kernel with that many m tiles per block wherever it fits
     const char *pt = std::getenv("TDGL_HIP_PAPTILE");
     h->paptile = pt && pt[0] == '0' ? 0 : 1;
+    const char *bx = std::getenv("TDGL_HIP_BOXXCD");
+    h->boxxcd = bx && bx[0] == '0' ? 0 : 1;
+    const char *zs = std::getenv("TDGL_HIP_ZSWAP");
+    h->zswap = zs && zs[0] == '0' ? 0 : 1;
     const char *zc = std::getenv("TDGL_HIP_ZCOL");
     h->zcols = zc && zc[0] == '1' ? 1 : 0;
+    const char *qp = std::getenv("TDGL_HIP_QPLANE");
+    h->qplane = qp && qp[0] == '0' ? 0 : 1;
     // TDGL_HIP_ZQ_WAVES=4 or 8: the waves per block of the quarter z pass
     const char *zw = std::getenv("TDGL_HIP_ZQ_WAVES");
     if (zw && (std::atoi(zw) == 4 || std::atoi(zw) == 8)) h->zq_waves = std::atoi(zw);
@@ -2039,6 +2098,17 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
               CHK(upload_vec(h, h->Hq[ax][which][par][0], hf));
               ALLOC(h->Hq[ax][which][par][1], hi.size());
               CHK(upload_vec(h, h->Hq[ax][which][par][1], hi));
+              // the same, zero-padded to whole 16 x 16 tiles (k_qplane)
+              const int LP = 16 * cdiv(Lh, 16);
+              for (int t = 0; t < 2; ++t) {
+                const std::vector<float> &src = t == 0 ? hf : hi;
+                std::vector<float> pd((size_t)LP * LP, 0.0f);
+                for (int r = 0; r < Lh; ++r)
+                  for (int c = 0; c < Lh; ++c) pd[(size_t)r * LP + c] = src[(size_t)r * Lh + c];
+                ALLOC(h->Hp[ax][which][par][t], pd.size());
+                CHK(upload_vec(h, h->Hp[ax][which][par][t], pd));
+              }
+              h->HpL[ax][which] = Lh;
             }
           }
         }

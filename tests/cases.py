@@ -80,6 +80,10 @@ CASES = {
     "sep_mid": (mgb2, dict(nx=48, ny=40, nz=4), {}, 1e-10),
     "pcg_exact_wide": (mgb2, dict(nx=68, ny=34, nz=4, buf=2), dict(sigma_vac=0.01, pcg_fast=False, pcg_f32=False,
                                                                   pcg_tol=1e-12), 1e-10),
+    # quarters of 71 x 37 (wider than 64 and 32): the folded preconditioner's one-round x-y kernels
+    "pcg_mf1_f32": (mgb2, dict(nx=136, ny=68, nz=4, buf=2), dict(sigma_vac=0.01), 1e-7),
+    "kwt10_mf1_pcg": (lambda: mgb2(gkwt=10.0), dict(nx=136, ny=68, nz=4, buf=2), dict(sigma_vac=0.01, pcg_tol=1e-12),
+                      1e-10),
     "open": (mgb2, {}, dict(selffield_bc="open", open_every=0.02), 1e-10),
     "open_kwt_pcg": (lambda: mgb2(gkwt=10.0), dict(buf=2), dict(sigma_vac=0.01, pcg_tol=1e-12, selffield_bc="open",
                                                                 open_every=0.03), 1e-10),

@@ -388,7 +388,8 @@ def test_pcg_budget_settles_on_a_current_ramp():
     assert g["rollbacks"] <= 4 and g["captures"] <= 8
 
 
-VARIANTS = [("TDGL_HIP_ZCOL", "1", "0"), ("TDGL_HIP_PAPTILE", "0", "1"), ("TDGL_HIP_ZQ_WAVES", "8", "4")]
+VARIANTS = [("TDGL_HIP_ZCOL", "1", "0"), ("TDGL_HIP_PAPTILE", "0", "1"), ("TDGL_HIP_ZQ_WAVES", "8", "4"),
+            ("TDGL_HIP_BOXXCD", "0", "1"), ("TDGL_HIP_ZSWAP", "0", "1")]
 
 
 @pytest.mark.parametrize("variant", VARIANTS, ids=lambda v: v[0])
@@ -398,8 +399,10 @@ def test_kernel_variants_give_the_same_bits(name, variant, monkeypatch):
     """Kernels rewritten for speed keep the arithmetic of the ones they replace: the z pass over
     x-runs of modes (k_zmodes_run) against whole columns per block (TDGL_HIP_ZCOL=1), the LDS-tiled
     PCG operator (k_pcg_pap_tile) against one gather per stencil value (TDGL_HIP_PAPTILE=0), and the
-    quarter z pass (k_zmodes_q) with 8 against 4 waves a block (each mode's sums in the same order).
-    Eight steps give the same bits either way."""
+    quarter z pass (k_zmodes_q) with 8 against 4 waves a block (each mode's sums in the same order),
+    the box kernels' blocks in XCD runs against the hardware's order (each block's partial in the
+    same slot), and the folded PCG's alternating z and zold against the copy.  Eight steps give the
+    same bits either way."""
     HipStepper = _hip()
     st, _ = make_case(name)
     s0, pcg, H, HJ = seed_state(st)
@@ -419,8 +422,32 @@ def test_kernel_variants_give_the_same_bits(name, variant, monkeypatch):
         assert np.array_equal(a, b), k
 
 
+@pytest.mark.parametrize("name", ["pcg_mf1_f32", "kwt10_mf1_pcg"])
+def test_quarter_plane_kernel_gives_the_same_bits(name, monkeypatch):
+    """The quarter x-y transforms on k_qplane (a block per quarter plane, the bases read from
+    global memory) against k_plane_mf1 (TDGL_HIP_QPLANE=0; quarters wider than 64 and 32, so the
+    one-round kernel takes them): the same operands of each matrix instruction in the same order,
+    so eight steps give the same bits."""
+    HipStepper = _hip()
+    st, _ = make_case(name)
+    s0, pcg, H, HJ = seed_state(st)
+    outs = []
+    for v in ("0", "1"):
+        print(f"[gpu] TDGL_HIP_QPLANE={v}: {name}", flush=True)
+        monkeypatch.setenv("TDGL_HIP_QPLANE", v)
+        dev = HipStepper(st)
+        dev.upload(s0)
+        dev.set_pcg(pcg)
+        dev.step(8, H, HJ)
+        assert dev.stats()["folded"] == 1
+        outs.append(dev.download())
+        dev.close()
+    for (k, a), (_, b) in zip(outs[0].arrays(), outs[1].arrays()):
+        assert np.array_equal(a, b), k
+
+
 @pytest.mark.parametrize("name", ["pcg_fast_f32", "pcg_tight_f32", "kwt10_pcg", "pcg_wide_f32", "open_kwt_pcg",
-                                  "qp_two_kwt_pcg"])
+                                  "qp_two_kwt_pcg", "pcg_mf1_f32"])
 def test_folded_preconditioner_matches_full_length(name, monkeypatch):
     """The Float32 preconditioner with parity-folded transforms (quarter planes with half-size bases)
     against the full-length transforms (TDGL_HIP_FOLD=0): the same solutions to the PCG tolerance
