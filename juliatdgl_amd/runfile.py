@@ -10,8 +10,11 @@ The tables and keys are the reference's:
                 (a number or "dirty"), T, gamma12 | gamma; [[material.band]] m | g | xi_ratio, Gamma, a,
                 beta, gamma_kwt, name; m_x, m_y, m_z or xi_ratio_x, ... for an anisotropic band
     [mesh]      nx, ny, nz, dx, buf (bufx, bufy, bufz)
-    [geometry]  GEOM_MODE (SIM, WEDGE, WEDGE_BOTH and their _REF mirrors), wedge_depth, wedge_half,
-                wedge_depth2 (and the keys that do not change those masks: inj_margin, ...)
+    [geometry]  the GeomSpec keys: GEOM_MODE (the catalogue SIM, ESFERA, ARRAY, WEDGE, WEDGE_BOTH,
+                ESF_CUNA, FGRAD, SAWROW, SPHROW, SAWSPH, STAGGER, FUNNEL, FUNNEL_IN, TRI_Y1, TRI_Y1_INV,
+                RECT_Y1, HEX, HEX_STAG and their _REF mirrors, or CUSTOM / CUSTOM_REF with geom_file, a
+                shape file next to the run file or in TDGL_SHAPES), wedge_depth, wedge_half, wedge_depth2,
+                fgrad_*, x_shift, y_shift, z_mode, prism_phi, inj_margin, ...
     [scheme]    implicit, dt_scale, substeps, extrap, hard, rkl, rotframe, rkl_margin, selffield_bc,
                 open_every, sigma_vac, qp_exchange, qp_split, noise, noise_seed, noise_field, pcg_tol,
                 pcg_fast, pcg_f32 (threads is the CPU step's and is ignored)
@@ -20,8 +23,8 @@ The tables and keys are the reference's:
 
 A material `file` is looked for as given, next to the run file, and in the directories of
 TDGL_MATERIALS (the reference's gtdgl/materials, say); the material of the reference's validation
-run files is built in.  Options this stepper does not run (heat, drag, materials per region, other
-geometry modes) raise NotImplementedError, as make_stepper does.
+run files is built in.  Options this stepper does not run (heat, drag, materials per region, a
+sculpted sample, a current along y) raise NotImplementedError, as make_stepper does.
 """
 from __future__ import annotations
 
@@ -39,7 +42,7 @@ except ModuleNotFoundError:  # Python < 3.11
     import tomli as _toml
 
 from .meter import Protocol
-from .scheme import Band, Grid, Masks, Model, Stepper, catalogue_masks, make_stepper
+from .scheme import Band, Grid, Masks, Model, Stepper, make_stepper
 from .state import State, initial_state
 
 # the material of the reference's validation run files (gtdgl/materials/mgb2_model_II.toml): the
@@ -127,18 +130,31 @@ def material(d: dict, base_dir: str = "."):
     return md, info
 
 
-def geometry_masks(gk: dict, gr: Grid) -> Masks:
-    """build_masks(GeomSpec(; gk...), gr) for the modes catalogue_masks builds."""
-    mode = str(gk.get("GEOM_MODE", "SIM"))
-    for k in ("geom_file", "x_shift", "y_shift", "prism_phi"):
-        if k in gk and gk[k] not in ("", 0, 0.0):
-            raise NotImplementedError(f"[geometry] {k} is not built here")
-    if str(gk.get("z_mode", "through")) != "through":
-        raise NotImplementedError("[geometry] z_mode other than 'through' is not built here")
+def shape_path(f: str, base_dir: str) -> str:
+    """A [geometry] geom_file (config.jl:182-186): as given when absolute, else next to the run file,
+    else in the directories of TDGL_SHAPES (the reference's gtdgl/shapes, say)."""
+    if os.path.isabs(f):
+        return f
+    cands = [os.path.join(base_dir, f)] + [os.path.join(d, f) for d in os.environ.get("TDGL_SHAPES", "").split(os.pathsep)
+                                           if d]
+    for c in cands:
+        if os.path.isfile(c):
+            return c
+    raise ValueError(f"geom_file {f!r}: not next to the run file nor in TDGL_SHAPES")
+
+
+def geometry_masks(gk: dict, gr: Grid, base_dir: str = ".") -> Masks:
+    """build_masks(GeomSpec(; gk...), gr) (geometry.jl:485-527; juliatdgl_amd.geometry): every
+    catalogue mode and its _REF mirror, the shifts, z_mode, prism_phi and the shape files (CUSTOM,
+    CUSTOM_REF).  The current runs along x (current_axis = "x", the stepper's injection)."""
+    from .geometry import GeomSpec, build_masks
     if str(gk.get("current_axis", "x")).lower() != "x":
-        raise NotImplementedError("[geometry] current_axis other than x is not built here")
-    return catalogue_masks(gr, mode, float(gk.get("wedge_depth", 0.35)), float(gk.get("wedge_half", 0.20)),
-                           float(gk.get("wedge_depth2", 0.0)))
+        raise NotImplementedError("[geometry] current_axis other than x is not run here (the current is injected along x)")
+    g = dict(gk)
+    if g.get("geom_file"):
+        g["geom_file"] = shape_path(str(g["geom_file"]), base_dir)
+    mk, _ = build_masks(GeomSpec.from_keys(g), gr)
+    return mk
 
 
 def _holds_ladder(od: str) -> bool:
@@ -193,7 +209,7 @@ def load_config(path: str) -> RunConfig:
     if "materials" in raw:
         raise NotImplementedError("[materials.<name>]: materials per region are not run here")
     gk = dict(raw.get("geometry", {}))
-    mk = geometry_masks(gk, gr)
+    mk = geometry_masks(gk, gr, base)
     if ms.get("ghost_links", True) is False:
         raise NotImplementedError("[mesh] ghost_links = false is not run here")
     sk = dict(raw.get("scheme", {}))

@@ -198,12 +198,15 @@ class Grid:
 
 @dataclass
 class Masks:
-    """Masks (grid.jl:135-143), node-shaped Float64 arrays, Fortran order."""
+    """Masks (grid.jl:135-143), node-shaped Float64 arrays, Fortran order; `vac` the vacuum of a
+    sculpted sample and `mat` the material index per node (juliatdgl_amd.geometry), None without."""
     f: np.ndarray
     tx: np.ndarray
     ty: np.ndarray
     tz: np.ndarray
     tpsi: np.ndarray
+    vac: Optional[np.ndarray] = None
+    mat: Optional[np.ndarray] = None
 
 
 def box_masks(gr: Grid) -> Masks:
@@ -222,57 +225,14 @@ def box_masks(gr: Grid) -> Masks:
     return Masks(f, tx, ty, tz, tp)
 
 
-def _oround(x: float) -> int:
-    """round(Int, x, RoundNearestTiesAway) (geometry.jl:12)."""
-    return int(math.copysign(math.floor(abs(x) + 0.5), x))
-
-
-def _paint_wedge(mk: Masks, gr: Grid, xc: int, yc: int, d: int, half: int, depth: int) -> None:
-    """paint_defect! of a 'wedge' (geometry.jl:121-135): a notch growing from yc in direction d,
-    a steep wall on -x and a gentle ramp on +x, through the thickness (z_mode through)."""
-    x1, x2, y1, y2, z1, z2 = gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2
-    # the same cells for every k (through the thickness): per row j, the x run xc-w_steep..xc+w_gentle
-    for j in range(y1, y2 + 1):
-        layer = d * (j - yc)
-        if 0 <= layer < depth:
-            frac = 1 - layer / depth
-            w_steep = max(1, _oround(2 * frac))
-            w_gentle = _oround(half * frac)
-            lo, hi = max(x1, xc - w_steep), min(x2, xc + w_gentle)
-            if lo <= hi:
-                mk.f[lo - 1:hi, j - 1, z1 - 1:z2] = 0.0
-
-
 def catalogue_masks(gr: Grid, mode: str = "SIM", wedge_depth: float = 0.35, wedge_half: float = 0.20,
                     wedge_depth2: float = 0.0) -> Masks:
-    """build_masks(GeomSpec(GEOM_MODE=mode, ...)) for the catalogue modes SIM (a clean box), WEDGE
-    (a notch on face y1) and WEDGE_BOTH (and a second on y2), each with its _REF mirror in y
-    (geometry.jl:268-324, mirror_defects! :193-209; J along x, z_mode through, no shifts)."""
-    m = mode.upper()
-    ref = m.endswith("_REF")
-    if ref:
-        m = m[:-4]
-    x1, x2, y1, y2 = gr.x1, gr.x2, gr.y1, gr.y2
-    xc = _oround((x1 + x2) / 2)
-    nx_sc, ny_sc = x2 - x1 + 1, y2 - y1 + 1
-    half = _oround(wedge_half * nx_sc)
-    defects = []
-    if m == "SIM":
-        pass
-    elif m == "WEDGE":
-        defects.append((xc, y1, +1, half, _oround(wedge_depth * ny_sc)))
-    elif m == "WEDGE_BOTH":
-        defects.append((xc, y1, +1, half, _oround(wedge_depth * ny_sc)))
-        if wedge_depth2 > 0:
-            defects.append((xc, y2 + 1, -1, half, _oround(wedge_depth2 * ny_sc)))
-    else:
-        raise NotImplementedError(f"GEOM_MODE {mode!r}: the catalogue modes built here are SIM, WEDGE, WEDGE_BOTH "
-                                  f"(and their _REF mirrors)")
-    if ref:
-        defects = [(x, (y1 + y2 + 1) - y, -d, h, dp) for (x, y, d, h, dp) in defects]
-    mk = box_masks(gr)
-    for x, y, d, h, dp in defects:
-        _paint_wedge(mk, gr, x, y, d, h, dp)
+    """build_masks(GeomSpec(GEOM_MODE=mode, wedge_depth, wedge_half, wedge_depth2)) of a catalogue
+    mode or its _REF mirror (geometry.jl:268-476; juliatdgl_amd.geometry), J along x, z_mode through,
+    no shifts."""
+    from .geometry import GeomSpec, build_masks
+    mk, _ = build_masks(GeomSpec(GEOM_MODE=mode, wedge_depth=wedge_depth, wedge_half=wedge_half,
+                                 wedge_depth2=wedge_depth2), gr)
     return mk
 
 
@@ -863,6 +823,11 @@ def make_stepper(md: Model, gr: Grid, mk: Masks, *, implicit=True, substeps=None
     for k, v in unsupported.items():
         if v is not None and v is not False:
             raise NotImplementedError(f"Stepper option {k!r} is not run by the HIP stepper")
+    if mk.vac is not None and mk.vac.any():
+        raise NotImplementedError("a sculpted sample (vacuum = true in the shape file) is not run by the HIP stepper")
+    if mk.mat is not None and (mk.mat != 1).any():
+        raise NotImplementedError("materials per region (material = \"name\" in the shape file) are not run by the "
+                                  "HIP stepper")
     N = md.N
     if noise is not None and not float(noise) >= 0:
         raise ValueError("noise: T̃ must be >= 0")

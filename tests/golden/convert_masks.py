@@ -1,17 +1,17 @@
-"""Fixtures of the run-file front end from the reference's golden files.
+"""Fixtures of the geometry and the run-file front end from the reference's golden files.
 
     python tests/golden/convert_masks.py [/root/reference]
 
-* catalogue_masks.npz: the condensation masks (fmask) that juliaTDGL 1.2 wrote for the catalogue
-  modes the front end builds (gtdgl/test/golden/masks.jld2, made by make_masks.jl), with the mesh
-  and geometry keys of each case as JSON.
-* ladder_k25_scalars.json: the scalars of gtdgl/test/golden/ladder_k25.jld2 that
-  gtdgl/test/test_config.jl compares a run file against (mr, dt, n_pi), with eps_c and the brackets.
-* ladder_k25.npz: the two polarities of that golden ladder, point by point (J, verdicts, cycles, V,
-  the band-2 slope, the static voltage, the half-window slopes, slips, magnetization, residuals)
-  and the final Ψ of each, which gtdgl/test/test_protocol.jl:4-29 compares a ladder against.
+* catalogue_masks.npz: every condensation mask (fmask) that juliaTDGL 1.2 wrote in
+  gtdgl/test/golden/masks.jld2 (make_masks.jl: every catalogue mode and its _REF mirror on two
+  meshes, the z modes, the shifts, the prism angle, the FGRAD keys, and every shape file of
+  tdgl/shapes as CUSTOM and CUSTOM_REF), the box's link and node masks of its one case, and as
+  JSON the mesh and geometry keys of each case and the parsed content of each shape file.
+* ladder_k25_scalars.json, ladder_k25.npz: the golden ladder (gtdgl/test/golden/ladder_k25.jld2)
+  that gtdgl/test/test_config.jl and test_protocol.jl compare against.
 
-Data only: the masks and numbers the reference's files hold, read with juliatdgl_amd/jld2.py.
+Data only: the masks, numbers and shape descriptions the reference's files hold, read with
+juliatdgl_amd/jld2.py and tomli.
 """
 from __future__ import annotations
 
@@ -24,20 +24,26 @@ import numpy as np
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", ".."))
 from juliatdgl_amd.jld2 import JLD2File  # noqa: E402
 
-CASES = ("SIM_40x24x6", "SIM_REF_30x20x8", "WEDGE_40x24x6", "WEDGE_30x20x8", "WEDGE_REF_40x24x6",
-         "WEDGE_REF_30x20x8", "WEDGE_BOTH_40x24x6", "WEDGE_BOTH_REF_30x20x8", "WEDGE_deep")
-
 
 def main(ref: str) -> None:
+    import tomli
     out = os.path.dirname(os.path.abspath(__file__))
     gdir = os.path.join(ref, "gtdgl", "test", "golden")
     c = JLD2File(os.path.join(gdir, "masks.jld2"))["casos"]
-    arrs, meta = {}, {}
-    for k in CASES:
+    arrs, meta, shapes = {}, {}, {}
+    for k in sorted(c.keys()):
+        if k.startswith("_box"):
+            for t in ("taux", "tauy", "tauz", "taupsi"):
+                arrs[f"{k}_{t}"] = np.asarray(c[k][t], dtype=np.uint8)
+            continue
         arrs[k] = np.asarray(c[k]["fmask"], dtype=np.float64)
         meta[k] = {"malla": c[k]["malla"], "geo": c[k]["geo"]}
-    np.savez_compressed(os.path.join(out, "catalogue_masks.npz"), meta_json=np.frombuffer(json.dumps(meta).encode(), np.uint8),
-                        **arrs)
+        gf = c[k]["geo"].get("geom_file")
+        if gf and gf not in shapes:
+            with open(os.path.join(ref, "tdgl", "shapes", gf), "rb") as fh:
+                shapes[gf] = tomli.load(fh)
+    blob = json.dumps({"cases": meta, "shapes": shapes}).encode()
+    np.savez_compressed(os.path.join(out, "catalogue_masks.npz"), meta_json=np.frombuffer(blob, np.uint8), **arrs)
     f = JLD2File(os.path.join(gdir, "ladder_k25.jld2"))
     sc = {k: f[k] for k in ("mr", "dt", "n_pi", "eps_c", "lo_pos", "hi_pos", "lo_neg", "hi_neg")}
     lad = {}

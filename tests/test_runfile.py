@@ -81,20 +81,41 @@ def test_the_reference_test_run_file(tmp_path):
 
 
 def test_catalogue_masks_are_release_12s():
-    """The masks of SIM, WEDGE and WEDGE_BOTH and their mirrors on the meshes of make_masks.jl
-    (and WEDGE_deep) equal the golden ones to the bit."""
+    """The [geometry] table's masks (geometry_masks, the run file's path into juliatdgl_amd.geometry)
+    equal the golden ones of every catalogue case to the bit (the shape files: tests/test_geometry.py)."""
     z = np.load(os.path.join(GOLDEN, "catalogue_masks.npz"))
-    meta = json.loads(bytes(z["meta_json"]).decode())
-    assert len(meta) == 9
+    meta = json.loads(bytes(z["meta_json"]).decode())["cases"]
+    n = 0
     for k, m in meta.items():
         ma, g = m["malla"], m["geo"]
+        if "geom_file" in g:
+            continue
         gr = Grid.make(ma["Nx_sc"], ma["Ny_sc"], ma["Nz_sc"], dx=ma["dx"], buf=ma["buf"])
         mk = geometry_masks(dict(g), gr)
         assert np.array_equal(mk.f, z[k]), k
-    with pytest.raises(NotImplementedError, match="GEOM_MODE"):
-        catalogue_masks(Grid.make(8, 6, 4, dx=0.5), "ESFERA")
-    with pytest.raises(NotImplementedError, match="z_mode"):
-        geometry_masks({"GEOM_MODE": "WEDGE", "z_mode": "half"}, Grid.make(8, 6, 4, dx=0.5))
+        n += 1
+    assert n == 82
+    mk = catalogue_masks(Grid.make(40, 24, 6, dx=0.5, buf=1), "WEDGE_BOTH_REF", wedge_depth2=0.0)
+    assert np.array_equal(mk.f, z["WEDGE_BOTH_REF_40x24x6"])
+    with pytest.raises(NotImplementedError, match="current_axis"):
+        geometry_masks({"GEOM_MODE": "WEDGE", "current_axis": "y"}, Grid.make(8, 6, 4, dx=0.5))
+
+
+def test_run_file_with_a_shape_file(tmp_path):
+    """A run file whose [geometry] names a shape file next to it (config.jl:182-186): its masks are
+    the shape's, mirrored for CUSTOM_REF; a missing file is named."""
+    (tmp_path / "tri.toml").write_text('[[shape]]\ntype = "triangle"\nx = 8.0\ny = 0.0\nw = 5.0\nh = 3.0\nf = 0.0\n')
+    base = ('[material]\nkappa = 2.0\n[[material.band]]\nm = 1.0\n[mesh]\nnx = 24\nny = 12\nnz = 4\ndx = 0.5\n'
+            '[run]\nJ = [0.5]\nH = 0.03\n')
+    from juliatdgl_amd.geometry import GeomSpec, build_masks, read_shape
+    for mode in ("CUSTOM", "CUSTOM_REF"):
+        (tmp_path / "run.toml").write_text(base + f'[geometry]\nGEOM_MODE = "{mode}"\ngeom_file = "tri.toml"\n')
+        c = load_config(str(tmp_path / "run.toml"))
+        want, _ = build_masks(GeomSpec(GEOM_MODE=mode), c.grid, read_shape(str(tmp_path / "tri.toml")))
+        assert np.array_equal(c.masks.f, want.f) and (c.masks.f == 0).sum() > 10
+    (tmp_path / "run.toml").write_text(base + '[geometry]\nGEOM_MODE = "CUSTOM"\ngeom_file = "none.toml"\n')
+    with pytest.raises(ValueError, match="none.toml"):
+        load_config(str(tmp_path / "run.toml"))
 
 
 def test_material_keys():
