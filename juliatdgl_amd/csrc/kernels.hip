@@ -1969,9 +1969,80 @@ template hipError_t launch_plane<float, float, float>(const PlanePass &, const i
 // one instruction and the order of the instructions are those of k_plane_mf1 (group g, step u,
 // lane k index lk: k = 16 g + 4 lk + u), and the padding is zero in both: the same bits.
 // The next group's global operands are requested while the current group multiplies.
+// the two products of k_qplane for one quarter plane held in LDS (Bt); FULL: the quarter has the
+// launch's JT tiles along j, so no tile is guarded
+template <int JT, bool FULL>
+__device__ __forceinline__ void qplane_products(const QPass &p, int comp, int plane, int tid, int mt, int jt, int e0,
+                                                int e1, int E0S, int JS, int xs, int ys, float *Bt, float *St) {
+  typedef v4f V4;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, lr = l & 15, lk = l >> 4;
+  const bool on = w < mt;
+  V4 acc[JT];
+#pragma unroll
+  for (int c = 0; c < JT; ++c) acc[c] = V4{0, 0, 0, 0};
+  if (on) {
+    // phase 1: A = X(16 g + 4 lk + u, 16 w + lr) from XT, B = in(16 g + 4 lk + u, 16 c + lr) from Bt
+    const float *xa = p.XT[comp] + (size_t)(16 * w + lr) * xs + 4 * lk;
+    V4 A = *(const V4 *)xa;
+    for (int g = 0; g < mt; ++g) {
+      const V4 An = *(const V4 *)(xa + 16 * min(g + 1, mt - 1));
+      V4 B[JT];
+#pragma unroll
+      for (int c = 0; c < JT; ++c)
+        B[c] = (FULL || c < jt) ? *(const V4 *)(Bt + (16 * c + lr) * E0S + 16 * g + 4 * lk) : V4{0, 0, 0, 0};
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int c = 0; c < JT; ++c)
+          if ((FULL || c < jt)) acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[u], B[c][u], acc[c], 0, 0, 0);
+      A = An;
+    }
+  }
+  __syncthreads();  // every wave is done with the plane: S goes over it
+  if (on)
+#pragma unroll
+    for (int c = 0; c < JT; ++c)
+      if ((FULL || c < jt))
+#pragma unroll
+        for (int r = 0; r < 4; ++r) St[(16 * w + 4 * lk + r) * JS + 16 * c + lr] = acc[c][r];
+  __syncthreads();
+  if (!on) return;
+  // phase 2: A = Y(16 g + 4 lk + u, 16 c + lr) from YT, B = S(16 w + lr, 16 g + 4 lk + u) from St
+#pragma unroll
+  for (int c = 0; c < JT; ++c) acc[c] = V4{0, 0, 0, 0};
+  const float *ya = p.YT[comp] + (size_t)lr * ys + 4 * lk;
+  V4 Y[JT];
+#pragma unroll
+  for (int c = 0; c < JT; ++c) Y[c] = (FULL || c < jt) ? *(const V4 *)(ya + (size_t)16 * c * ys) : V4{0, 0, 0, 0};
+  for (int g = 0; g < jt; ++g) {
+    const int gn = 16 * min(g + 1, jt - 1);
+    V4 Yn[JT];
+#pragma unroll
+    for (int c = 0; c < JT; ++c) Yn[c] = (FULL || c < jt) ? *(const V4 *)(ya + (size_t)16 * c * ys + gn) : V4{0, 0, 0, 0};
+    const V4 B = *(const V4 *)(St + (16 * w + lr) * JS + 16 * g + 4 * lk);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < JT; ++c)
+        if ((FULL || c < jt)) acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(Y[c][u], B[u], acc[c], 0, 0, 0);
+#pragma unroll
+    for (int c = 0; c < JT; ++c) Y[c] = Yn[c];
+  }
+  // out(q, m): element r of lane l is q = 16 c + 4 lk + r, m = 16 w + lr
+  const View vo = p.out[comp];
+  float *dst = (float *)vo.p + (size_t)vo.s2 * plane;
+  const int m = 16 * w + lr;
+#pragma unroll
+  for (int c = 0; c < JT; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = 16 * c + 4 * lk + r;
+      if ((FULL || c < jt) && m < e0 && q < e1) dst[m + (size_t)vo.s1 * q] = acc[c][r];
+    }
+}
+
 template <int JT>
 __global__ void __launch_bounds__(64 * QP_MT) k_qplane(QPass p, const int *gate, int e2max, int mtx, int jtx) {
-  typedef v4f V4;
   extern __shared__ __align__(16) unsigned char qplane_smem[];
   __shared__ double sh[16];
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -2017,70 +2088,10 @@ __global__ void __launch_bounds__(64 * QP_MT) k_qplane(QPass p, const int *gate,
     }
   }
   __syncthreads();
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, lr = l & 15, lk = l >> 4;
-  const bool on = w < mt;
-  V4 acc[JT];
-#pragma unroll
-  for (int c = 0; c < JT; ++c) acc[c] = V4{0, 0, 0, 0};
-  if (on) {
-    // phase 1: A = X(16 g + 4 lk + u, 16 w + lr) from XT, B = in(16 g + 4 lk + u, 16 c + lr) from Bt
-    const float *xa = p.XT[comp] + (size_t)(16 * w + lr) * xs + 4 * lk;
-    V4 A = *(const V4 *)xa;
-    for (int g = 0; g < mt; ++g) {
-      const V4 An = *(const V4 *)(xa + 16 * min(g + 1, mt - 1));
-      V4 B[JT];
-#pragma unroll
-      for (int c = 0; c < JT; ++c)
-        B[c] = c < jt ? *(const V4 *)(Bt + (16 * c + lr) * E0S + 16 * g + 4 * lk) : V4{0, 0, 0, 0};
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int c = 0; c < JT; ++c)
-          if (c < jt) acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[u], B[c][u], acc[c], 0, 0, 0);
-      A = An;
-    }
-  }
-  __syncthreads();  // every wave is done with the plane: S goes over it
-  if (on)
-#pragma unroll
-    for (int c = 0; c < JT; ++c)
-      if (c < jt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) St[(16 * w + 4 * lk + r) * JS + 16 * c + lr] = acc[c][r];
-  __syncthreads();
-  if (!on) return;
-  // phase 2: A = Y(16 g + 4 lk + u, 16 c + lr) from YT, B = S(16 w + lr, 16 g + 4 lk + u) from St
-#pragma unroll
-  for (int c = 0; c < JT; ++c) acc[c] = V4{0, 0, 0, 0};
-  const float *ya = p.YT[comp] + (size_t)lr * ys + 4 * lk;
-  V4 Y[JT];
-#pragma unroll
-  for (int c = 0; c < JT; ++c) Y[c] = c < jt ? *(const V4 *)(ya + (size_t)16 * c * ys) : V4{0, 0, 0, 0};
-  for (int g = 0; g < jt; ++g) {
-    const int gn = 16 * min(g + 1, jt - 1);
-    V4 Yn[JT];
-#pragma unroll
-    for (int c = 0; c < JT; ++c) Yn[c] = c < jt ? *(const V4 *)(ya + (size_t)16 * c * ys + gn) : V4{0, 0, 0, 0};
-    const V4 B = *(const V4 *)(St + (16 * w + lr) * JS + 16 * g + 4 * lk);
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int c = 0; c < JT; ++c)
-        if (c < jt) acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(Y[c][u], B[u], acc[c], 0, 0, 0);
-#pragma unroll
-    for (int c = 0; c < JT; ++c) Y[c] = Yn[c];
-  }
-  // out(q, m): element r of lane l is q = 16 c + 4 lk + r, m = 16 w + lr
-  const View vo = p.out[comp];
-  float *dst = (float *)vo.p + (size_t)vo.s2 * plane;
-  const int m = 16 * w + lr;
-#pragma unroll
-  for (int c = 0; c < JT; ++c)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int q = 16 * c + 4 * lk + r;
-      if (c < jt && m < e0 && q < e1) dst[m + (size_t)vo.s1 * q] = acc[c][r];
-    }
+  if (jt == JT)
+    qplane_products<JT, true>(p, comp, plane, tid, mt, jt, e0, e1, E0S, JS, xs, ys, Bt, St);
+  else
+    qplane_products<JT, false>(p, comp, plane, tid, mt, jt, e0, e1, E0S, JS, xs, ys, Bt, St);
 }
 
 static void qplane_tiles(const QPass &p, int &mtx, int &jtx, int &e2max) {
