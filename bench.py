@@ -40,6 +40,7 @@ def kernel_bytes(st, name: str, launches: int, nsteps: int, stats: dict) -> floa
     of one kernel class in `nsteps` time steps."""
     gr = st.grid
     N = st.N
+    refine = getattr(st, "pcg_refine", False)
     interior = (gr.x2 - gr.x1) * (gr.y2 - gr.y1) * (gr.z2 - gr.z1)
     nodes = (gr.Nx + 1) * (gr.Ny + 1) * (gr.Nz + 1)
     rng = (gr.Nx - 2) * (gr.Ny - 1) * (gr.Nz - 1) + (gr.Nx - 1) * (gr.Ny - 2) * (gr.Nz - 1) + \
@@ -61,6 +62,9 @@ def kernel_bytes(st, name: str, launches: int, nsteps: int, stats: dict) -> floa
         per = (12.0 * (it + 1) + 20.0 + 28.0 * it) / (2.0 * (it + 1))
         return per * rng * launches
     if name == "sep_fold":
+        if refine:
+            # the inner solves' unfolds: Float32 quarters, r, z_old in, z out (16 B)
+            return 16.0 * rng * launches
         if stats.get("folded"):
             # per solve of `it` iterations: one fold (FP64 in, FP32 quarters out: 12 B, the start's;
             # the iterations' folds ride on the x/r update), the start's unfold (FP32 in, FP64 z
@@ -74,11 +78,11 @@ def kernel_bytes(st, name: str, launches: int, nsteps: int, stats: dict) -> floa
         return 8.0 * (quarter_total(gr) if stats.get("folded") else rng) * launches
     if name == "pcg_apply_A":
         # z, P_old, D read, P, Q written (and z_old = z written unless the folded solve alternates
-        # the two buffers)
-        return (40.0 if stats.get("folded") else 48.0) * rng * launches
+        # the two buffers); the refinement's inner iterations in Float32
+        return (20.0 if refine else 40.0 if stats.get("folded") else 48.0) * rng * launches
     if name == "pcg_vector":
         # x, P, r, Q read, x, r written (folded: and the Float32 quarters of r's fold written)
-        return (52.0 if stats.get("folded") else 48.0) * rng * launches
+        return (28.0 if refine else 52.0 if stats.get("folded") else 48.0) * rng * launches
     if name == "current_field":
         return (16 * N + 48 + 24 * 6 + 24 + 24) * nodes * launches
     return 0.0
@@ -118,7 +122,12 @@ def cycle_bytes_per_node_step(st, iters_per_step: float) -> float:
     N = st.N
     band = 2 * (16 * (N + 1) + 8 + 72) + 152 * (s - 2)
     field = 16 * N + 192
-    pcg = 300 + 288 * iters_per_step if st.field_mode == 2 else (168 if st.field_mode == 1 else 0)
+    if st.field_mode == 2 and getattr(st, "pcg_refine", False):
+        # Float32 inner iterations (their vectors, operator and D at half the bytes of the FP64
+        # ones: 176 for 288), and about three FP64 rounds a step (residual, check, x += d)
+        pcg = 300 + 176 * iters_per_step + 3 * 150
+    else:
+        pcg = 300 + 288 * iters_per_step if st.field_mode == 2 else (168 if st.field_mode == 1 else 0)
     return band + field + pcg + 96
 
 
@@ -212,6 +221,12 @@ WORKLOADS = {
         "three-band anisotropic (m_z = 4 m_xy) TDGL, 256x128x32 cells, dx=0.5, buf 8 insulating (sigma_vac/sigma_n=0.01), "
         "PCG tol 1e-8 (Float32 preconditioner, FP64 outer), open boundary (FFT convolution, every t_GL), notch, H=0.2, "
         "J=0.1, ladder cycle = 40 steps", "notch mask"),
+    "open_3band_256x128x32_refine": (
+        lambda: configs.open_3band_256x128x32(pcg_refine=True),
+        "three-band anisotropic (m_z = 4 m_xy) TDGL, 256x128x32 cells, dx=0.5, buf 8 insulating (sigma_vac/sigma_n=0.01), "
+        "PCG tol 1e-8 by FP64 iterative refinement around Float32 inner PCG solves (each to 1e-4 of its starting "
+        "residual, Float32 preconditioner), open boundary (FFT convolution, every t_GL), notch, H=0.2, J=0.1, ladder "
+        "cycle = 40 steps", "notch mask"),
     "kwt_pcg_128x64x16": (
         lambda: configs.kwt_pcg_128x64x16(),
         "two-band KWT (u=Gamma/sigma_n=5.79, gkwt=10) TDGL, 128x64x16 cells, dx=0.5, buf 2 insulating "

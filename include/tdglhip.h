@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define TDGL_ABI_VERSION 3
+#define TDGL_ABI_VERSION 4
 #define TDGL_MAX_BANDS 4
 #define TDGL_MAX_STAGES 64
 
@@ -157,6 +157,15 @@ typedef struct {
   double noise_sk_w[TDGL_MAX_BANDS][2];  /* RKCCoef.sk = (ν₁, κ₁) = (sω₁/2, sω₁/ω₀) (scheme.jl:80, 99) */
   double noise_Gamma[TDGL_MAX_BANDS];    /* Γ_µ */
   double noise_sigma_n;                  /* σn of the field kick */
+
+  /* The precision of the PCG (field_mode PCG; an extension): with pcg_refine the FP64 operator,
+   * residual and solution sit around inner PCG solves in Float32 (their vectors, operator and the
+   * folded Float32 preconditioner): each round solves A d = b − A x to pcg_inner_tol of its starting
+   * residual, x += d, until ||b − A x|| <= pcg_tol ||b|| in FP64.  A captured cycle holds pcg_rounds
+   * rounds (1..16).  Needs pcg_f32 and the folded preconditioner; 0 keeps the reference's solver. */
+  int32_t pcg_refine;
+  int32_t pcg_rounds;
+  double pcg_inner_tol;
 } tdgl_desc;
 
 /* Normal-fluid models (tdgl_desc.qp_mode) */

@@ -68,6 +68,8 @@ struct DescC
     noise::Int32; noise_field::Int32; noise_host::Int32; noise_sk::NTuple{MAXB,Int32}
     noise_T::Float64; noise_seed::UInt64
     noise_sk_w::NTuple{MAXB,NTuple{2,Float64}}; noise_Gamma::NTuple{MAXB,Float64}; noise_sigma_n::Float64
+    # the PCG's precision (an extension: FP64 refinement around Float32 inner solves)
+    pcg_refine::Int32; pcg_rounds::Int32; pcg_inner_tol::Float64
 end
 struct StateC
     G::NTuple{MAXB,Ptr{Float64}}; F::NTuple{MAXB,Ptr{Float64}}
@@ -124,7 +126,7 @@ function _desc(st::Stepper{N}) where N
                   MAXB, ntuple(_ -> 0.0, MAXS))
     bar = st.bar
     bp(k) = bar === nothing ? Ptr{Float64}(0) : hold(getproperty(bar, k))
-    d = DescC(3, N, gr.Nx, gr.Ny, gr.Nz, gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2, gr.dx, gr.dy, gr.dz, st.dt,
+    d = DescC(4, N, gr.Nx, gr.Ny, gr.Nz, gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2, gr.dx, gr.dy, gr.dz, st.dt,
               _pad(ntuple(µ -> _bc(st.bc[µ], N, µ), N), MAXB, _ZBC), _pad(ntuple(µ -> _bc(st.bcs[µ], N, µ), N), MAXB, _ZBC),
               _pad(Int32.(st.nsub), MAXB, Int32(0)), _pad(Int32.(st.rkl), MAXB, Int32(0)),
               _pad(ntuple(µ -> Int32(rk[µ] !== nothing), N), MAXB, Int32(0)), _pad(div, MAXB, 0.0),
@@ -138,9 +140,17 @@ function _desc(st::Stepper{N}) where N
               D === nothing ? Ptr{Float64}(0) : hold(D[1]), D === nothing ? Ptr{Float64}(0) : hold(D[2]),
               D === nothing ? Ptr{Float64}(0) : hold(D[3]),
               bp(:hy_z), bp(:hy_y), bp(:hz_z), bp(:hz_y), bp(:hz_x), bp(:hy_x),
-              _qp_fields(st, hold)..., _open_fields(st, hold)..., _noise_fields(st, rk)...)
+              _qp_fields(st, hold)..., _open_fields(st, hold)..., _noise_fields(st, rk)...,
+              Int32(PCG_REFINE[] && fast && pcg.fast.f32), Int32(PCG_ROUNDS[]), PCG_INNER_TOL[])
     d, keep
 end
+
+# The PCG's precision on the device (an extension of the reference's solver): with PCG_REFINE[] the
+# FP64 operator, residual and solution refine around Float32 inner solves, each to PCG_INNER_TOL[]
+# of its starting residual, PCG_ROUNDS[] rounds per captured solve.  Read when a handle is created.
+const PCG_REFINE = Ref(false)
+const PCG_ROUNDS = Ref(3)
+const PCG_INNER_TOL = Ref(1e-4)
 
 # the Noise of st (noise.jl:32-41) as the description's noise fields: draws on the device unless
 # HOST_DRAWS[] (then the hook hands over the reference's own stream, reference_draws)

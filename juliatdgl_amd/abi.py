@@ -10,7 +10,7 @@ import numpy as np
 from .scheme import FIELD_PCG, FIELD_EXPLICIT_SCALED, Stepper
 from .state import FIELDS, State
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAXB = 4
 MAXS = 64
 dp = ct.POINTER(ct.c_double)
@@ -56,7 +56,8 @@ class DescC(ct.Structure):
                 ("noise", ct.c_int32), ("noise_field", ct.c_int32), ("noise_host", ct.c_int32),
                 ("noise_sk", ct.c_int32 * MAXB), ("noise_T", ct.c_double), ("noise_seed", ct.c_uint64),
                 ("noise_sk_w", (ct.c_double * 2) * MAXB), ("noise_Gamma", ct.c_double * MAXB),
-                ("noise_sigma_n", ct.c_double)]
+                ("noise_sigma_n", ct.c_double),
+                ("pcg_refine", ct.c_int32), ("pcg_rounds", ct.c_int32), ("pcg_inner_tol", ct.c_double)]
 
 
 class StateC(ct.Structure):
@@ -209,6 +210,7 @@ class PackedDesc:
                     d.noise_sk_w[mu][0], d.noise_sk_w[mu][1] = kc.sk
                 d.noise_Gamma[mu] = st.model.bands[mu].Gamma
             d.noise_sigma_n = st.model.sigma_n
+        d.pcg_refine, d.pcg_rounds, d.pcg_inner_tol = int(st.pcg_refine), int(st.pcg_rounds), float(st.pcg_inner_tol)
         self.c = d
         self._keep = keep
 

@@ -185,7 +185,9 @@ hipError_t launch_copies(const CopyList &c, hipStream_t s);
 
 // ---------------------------------------------------------------- implicit solve (implicit.jl)
 // Scalars, flags and reduction partials of one PCG solve
-enum { SC_NB = 0, SC_RZ0, SC_RZ1, SC_TOL, SC_COUNT };
+// SC_TOLI, SC_NBI: the relative tolerance and the starting residual norm of an inner Float32
+// solve of the refinement (d.pcg_refine)
+enum { SC_NB = 0, SC_RZ0, SC_RZ1, SC_TOL, SC_TOLI, SC_NBI, SC_COUNT };
 enum {
   FL_NZ = 0,      // the source is not zero
   FL_ACT0,        // the iterations run while set: iteration q reads ACT[q % 2], its convergence
@@ -195,6 +197,7 @@ enum {
   FL_OVF,         // a solve of the cycle was still active after its launched iterations
   FL_ITSUM,       // iterations summed over the cycle
   FL_ITMAX,       // the most iterations of one solve in the cycle
+  FL_OACT,        // refinement: the FP64 residual is still above the tolerance (the rounds run while set)
   FL_COUNT
 };
 // partial sums, two per block, one region per reduction (every consumer re-sums them in block
@@ -220,6 +223,7 @@ struct Conv {
   const int *in;   // the incoming flag (NZ for RR0, ACT[q % 2] for RR)
   int *out;        // ACT of the iterations that follow
   int nparts;      // partials of r·r
+  int inner;       // against SC_TOLI x SC_NBI (an inner solve of the refinement), else SC_TOL x SC_NB
 };
 
 // One pass of the 1D transforms along `axis` for the three components, out = M' in with
@@ -304,8 +308,20 @@ struct FoldPass {
   View in[3];
   float *out;
   Quarters q;
+  int f32;         // the input is Float32 (an inner solve's residual), else FP64
 };
 hipError_t launch_fold(const FoldPass &f, const int *gate, hipStream_t s);
+// The check of a refinement round and the fold of its Float32 residual: from the partials of
+// ||b − A x||² (PART_RR, nparts of them) the round goes on while the residual is above SC_TOL x
+// SC_NB and the iterations below FL_MAXIT (FL_OACT; SC_NBI = the norm; ACT0 of the inner solve),
+// and then, with `fold`, r32 is folded into the quarters.  Off unless *gate.
+struct RefFold {
+  FoldPass f;
+  const int *gate;
+  int nparts, fold;
+  PcgDev d;
+};
+hipError_t launch_ref_fold(const RefFold &r, hipStream_t s);
 // unfold: the quarters of the inverse x-y transforms back into a node-shaped FP64 vector, with the
 // partials of r·z (dot 1) and r·zold (dot 2) in PART_RZ
 struct UnfoldPass {
@@ -313,6 +329,7 @@ struct UnfoldPass {
   Quarters q;
   View out[3], r[3], zold[3];
   int dot;
+  int f32;         // out, r and zold are Float32 (an inner solve), else FP64
   PcgDev d;
 };
 hipError_t launch_unfold(const UnfoldPass &u, const int *gate, hipStream_t s);
@@ -464,6 +481,15 @@ hipError_t launch_qp_zcol(const QpZ &a, hipStream_t s);
 struct Vec3 {
   double *c[3];
 };
+struct Vec3f {
+  float *c[3];
+};
+// the Float32 vectors of an inner solve of the refinement (d.pcg_refine): its solution increment
+// d, residual r, preconditioned z and z_old, the directions P (new and old) and Q = A P, and D
+struct Pcg32 {
+  Vec3f d, r, z, zold, Pnew, Pold, Q;
+  const float *D[3];
+};
 struct PcgArgs {
   Geo g;
   Ranges rg;
@@ -482,6 +508,10 @@ struct PcgArgs {
   int tile;          // pap: the LDS-tiled kernel (k_pcg_pap_tile), else one gather per stencil value
   int zcopy;         // pap, flexible: store zold = z (0: the caller alternates z and zold instead)
   int xcd;           // box kernels: blocks to the XCDs in contiguous runs (box_idx)
+  Pcg32 f;           // the Float32 vectors (the refinement's inner solves)
+  int gate_fl;       // resid32, xupdate: the flag that gates them
+  int norm_only;     // resid32: only the partials of ||b − A x||²
+  int ovf_fl;        // finish: the flag that tells an unfinished solve (ACT[act_slot] when 0)
 };
 int pcg_box_blocks(const Geo &g, int kc);
 int box_kc(const Geo &g);
@@ -508,6 +538,9 @@ __device__ __forceinline__ double2 buf_load_d2(__amdgpu_buffer_rsrc_t r, int off
 template <class T> __device__ __forceinline__ T buf_load_s(__amdgpu_buffer_rsrc_t r, int off, int soff);
 template <> __device__ __forceinline__ double buf_load_s<double>(__amdgpu_buffer_rsrc_t r, int off, int soff) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+}
+template <> __device__ __forceinline__ float buf_load_s<float>(__amdgpu_buffer_rsrc_t r, int off, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
 }
 
 // ---------------------------------------------------------------- the ladder's per-cycle measurements
@@ -579,6 +612,13 @@ hipError_t launch_batch_meter(int kind, const BatchTab &t, int blocks, hipStream
 hipError_t launch_pcg_pap(const PcgArgs &a, hipStream_t s);      // β, P = z + βP, zold = z, Q = A P, p·Ap
 hipError_t launch_pcg_xr(const PcgArgs &a, hipStream_t s);       // α, x += αP, r −= αQ, r·r
 hipError_t launch_pcg_xr_fold(const PcgArgs &a, const FoldPass &f, hipStream_t s);  // the same, and the fold of r
+// the refinement's Float32 inner iteration: pap and x/r update with fold on a.f (x is a.f.d)
+hipError_t launch_pcg32_pap(const PcgArgs &a, hipStream_t s);
+hipError_t launch_pcg32_xr_fold(const PcgArgs &a, const FoldPass &f, hipStream_t s);
+// r32 = b − A x (FP64 operator, rounded) and the partials of ||b − A x||²; gated by fl[gate_fl]
+hipError_t launch_pcg_resid32(const PcgArgs &a, hipStream_t s);
+// x += d32, gated by fl[gate_fl]
+hipError_t launch_pcg_xupdate(const PcgArgs &a, hipStream_t s);
 hipError_t launch_pcg_finish(const PcgArgs &a, hipStream_t s);   // φ = θ + x, the cycle's counts
 hipError_t launch_pcg_cycle_reset(PcgDev d, hipStream_t s);
 

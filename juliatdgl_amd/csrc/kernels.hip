@@ -871,7 +871,7 @@ __device__ __forceinline__ void tx_body(const TxPass &p, const int *gate, const 
       return;
     }
     const double rr = sum_parts(region(p.d, PART_RR), p.conv.nparts, 0, sh, tid);
-    const double tol = p.d.sc[SC_TOL], nb = p.d.sc[SC_NB];
+    const double tol = p.d.sc[p.conv.inner ? SC_TOLI : SC_TOL], nb = p.d.sc[p.conv.inner ? SC_NBI : SC_NB];
     int active;
     if (p.conv.mode == CONV_RR0)
       active = (nb > 0 && sqrt(rr) > tol * nb) ? 1 : 0;
@@ -1133,7 +1133,7 @@ __device__ __forceinline__ bool pass_active(const Conv &cv, const int *gate, con
       return false;
     }
     const double rr = sum_parts(region(d, PART_RR), cv.nparts, 0, sh, tid);
-    const double tol = d.sc[SC_TOL], nb = d.sc[SC_NB];
+    const double tol = d.sc[cv.inner ? SC_TOLI : SC_TOL], nb = d.sc[cv.inner ? SC_NBI : SC_NB];
     int active;
     if (cv.mode == CONV_RR0)
       active = (nb > 0 && sqrt(rr) > tol * nb) ? 1 : 0;
@@ -1677,7 +1677,7 @@ __device__ __forceinline__ void plane_mf1_body(const PlanePass &p, const int *ga
   }
   if (p.conv.mode != CONV_NONE) {
     const double rr = sum_parts_8w(region(p.d, PART_RR), p.conv.nparts, 0, sh, tid);
-    const double tol = p.d.sc[SC_TOL], nrm = p.d.sc[SC_NB];
+    const double tol = p.d.sc[p.conv.inner ? SC_TOLI : SC_TOL], nrm = p.d.sc[p.conv.inner ? SC_NBI : SC_NB];
     const int active = p.conv.mode == CONV_RR0 ? ((nrm > 0 && sqrt(rr) > tol * nrm) ? 1 : 0)
                                                : ((sqrt(rr) <= tol * nrm || p.d.fl[FL_IT] >= p.d.fl[FL_MAXIT]) ? 0 : 1);
     if (bid == 0 && tid == 0) *p.conv.out = active;
@@ -2058,7 +2058,7 @@ __global__ void __launch_bounds__(64 * QP_MT) k_qplane(QPass p, const int *gate,
   const int e0 = p.e[comp][0], e1 = p.e[comp][1], e2 = p.e[comp][2];
   if (p.conv.mode != CONV_NONE) {
     const double rr = sum_parts_8w(region(p.d, PART_RR), p.conv.nparts, 0, sh, tid);
-    const double tol = p.d.sc[SC_TOL], nrm = p.d.sc[SC_NB];
+    const double tol = p.d.sc[p.conv.inner ? SC_TOLI : SC_TOL], nrm = p.d.sc[p.conv.inner ? SC_NBI : SC_NB];
     const int active = p.conv.mode == CONV_RR0 ? ((nrm > 0 && sqrt(rr) > tol * nrm) ? 1 : 0)
                                                : ((sqrt(rr) <= tol * nrm || p.d.fl[FL_IT] >= p.d.fl[FL_MAXIT]) ? 0 : 1);
     if (bid == 0 && tid == 0) *p.conv.out = active;
@@ -2375,15 +2375,21 @@ __global__ void __launch_bounds__(256) k_fold(FoldPass f, const int *gate) {
   const int is = L0 - 1 - i, js = L1 - 1 - j;
   const bool ci = is == i, cj = js == j;
   const View v = f.in[c];
-  const double *base = (const double *)v.p;
   const size_t o_ij = (size_t)i + (size_t)v.s1 * j, o_sj = (size_t)is + (size_t)v.s1 * j;
   const size_t o_is = (size_t)i + (size_t)v.s1 * js, o_ss = (size_t)is + (size_t)v.s1 * js;
   const size_t qp = (size_t)i + (size_t)h0 * j, qs = (size_t)h0 * h1;
   float *q00 = f.out + f.q.off[c][0] + qp, *q01 = f.out + f.q.off[c][1] + qp;
   float *q10 = f.out + f.q.off[c][2] + qp, *q11 = f.out + f.q.off[c][3] + qp;
   for (int k = kg; k < e2; k += 4) {
-    const double *pk = base + (size_t)v.s2 * k;
-    const double a = pk[o_ij], b = pk[o_sj], cc = pk[o_is], d = pk[o_ss];
+    const size_t zk = (size_t)v.s2 * k;
+    double a, b, cc, d;
+    if (f.f32) {
+      const float *pk = (const float *)v.p + zk;
+      a = pk[o_ij], b = pk[o_sj], cc = pk[o_is], d = pk[o_ss];
+    } else {
+      const double *pk = (const double *)v.p + zk;
+      a = pk[o_ij], b = pk[o_sj], cc = pk[o_is], d = pk[o_ss];
+    }
     const double sj = ci ? a : a + b, dj = ci ? 0.0 : a - b;
     const double ss = ci ? cc : cc + d, ds = ci ? 0.0 : cc - d;
     const size_t qk = qs * k;
@@ -2399,7 +2405,10 @@ __global__ void __launch_bounds__(256) k_fold(FoldPass f, const int *gate) {
 // point), the partials of r·r, and the fold of the new r into the quarters, in one pass.  IT += 1.
 // The convergence check that k_fold's first pass would take moves to the forward x-y launch, which
 // reads these partials (fold_blocks of them).
+// T = float: the refinement's inner iteration on a.f, its x the increment d (set at q = 0)
+template <class T>
 __global__ void __launch_bounds__(256) k_pcg_xr_fold(PcgArgs a, FoldPass f) {
+  constexpr bool F32 = sizeof(T) == 4;
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
   __shared__ double sh[8];
   const int tid = threadIdx.x;
@@ -2415,12 +2424,12 @@ __global__ void __launch_bounds__(256) k_pcg_xr_fold(PcgArgs a, FoldPass f) {
     const int is = L0 - 1 - i, js = L1 - 1 - j;
     const bool ci = is == i, cj = js == j;
     const View v = f.in[c];  // r's view; x, P, Q share its layout (node-shaped vectors)
-    const size_t base = (size_t)((const double *)v.p - a.r.c[c]);
+    T *X = F32 ? (T *)a.f.d.c[c] : (T *)a.x.c[c], *R = F32 ? (T *)a.f.r.c[c] : (T *)a.r.c[c];
+    const T *P = F32 ? (const T *)a.f.Pnew.c[c] : (const T *)a.Pnew.c[c], *Q = F32 ? (const T *)a.f.Q.c[c] : (const T *)a.Q.c[c];
+    const size_t base = (size_t)((const T *)v.p - R);
     const size_t o[4] = {base + i + (size_t)v.s1 * j, base + is + (size_t)v.s1 * j, base + i + (size_t)v.s1 * js,
                          base + is + (size_t)v.s1 * js};
     const bool w[4] = {true, !ci, !cj, !ci && !cj};
-    double *X = a.x.c[c], *R = a.r.c[c];
-    const double *P = a.Pnew.c[c], *Q = a.Q.c[c];
     const size_t qp = (size_t)i + (size_t)h0 * j, qs = (size_t)h0 * h1;
     float *q00 = f.out + f.q.off[c][0] + qp, *q01 = f.out + f.q.off[c][1] + qp;
     float *q10 = f.out + f.q.off[c][2] + qp, *q11 = f.out + f.q.off[c][3] + qp;
@@ -2438,9 +2447,10 @@ __global__ void __launch_bounds__(256) k_pcg_xr_fold(PcgArgs a, FoldPass f) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         rn[t] = rv[t] - al * qv[t];
+        if (F32) rn[t] = (double)(T)rn[t];  // the value stored, in the sums and the fold too
         if (!w[t]) continue;
-        X[o[t] + zk] = xv[t] + al * pv[t];
-        R[o[t] + zk] = rn[t];
+        X[o[t] + zk] = (F32 && a.q == 0) ? (T)(al * pv[t]) : (T)(xv[t] + al * pv[t]);
+        R[o[t] + zk] = (T)rn[t];
         s += rn[t] * rn[t];
       }
       const double sj = ci ? rn[0] : rn[0] + rn[1], dj = ci ? 0.0 : rn[0] - rn[1];
@@ -2463,7 +2473,70 @@ __global__ void __launch_bounds__(256) k_pcg_xr_fold(PcgArgs a, FoldPass f) {
 hipError_t launch_pcg_xr_fold(const PcgArgs &a, const FoldPass &f, hipStream_t s) {
   if (g_rec) return hipErrorNotSupported;
   if (fold_blocks(f.q) > MAX_PART) return hipErrorInvalidValue;
-  TDGL_LAUNCH(k_pcg_xr_fold, fold_grid(f.q), dim3(256), 0, s, a, f);
+  TDGL_LAUNCH(k_pcg_xr_fold<double>, fold_grid(f.q), dim3(256), 0, s, a, f);
+  return hipGetLastError();
+}
+hipError_t launch_pcg32_xr_fold(const PcgArgs &a, const FoldPass &f, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
+  if (fold_blocks(f.q) > MAX_PART) return hipErrorInvalidValue;
+  TDGL_LAUNCH(k_pcg_xr_fold<float>, fold_grid(f.q), dim3(256), 0, s, a, f);
+  return hipGetLastError();
+}
+
+// The check of a refinement round (RefFold) and the fold of the inner solve's Float32 residual.
+__global__ void __launch_bounds__(256) k_ref_fold(RefFold r) {
+  __shared__ double sh[8];
+  const int tid = threadIdx.x;
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  int *fl = r.d.fl;
+  if (!*r.gate) {
+    if (bid == 0 && tid == 0) {
+      fl[FL_OACT] = 0;
+      fl[FL_ACT0] = 0;
+      fl[FL_ACT1] = 0;
+    }
+    return;
+  }
+  const double nr = sqrt(sum_parts(region(r.d, PART_RR), r.nparts, 0, sh, tid));
+  const double tol = r.d.sc[SC_TOL], nb = r.d.sc[SC_NB];
+  const int active = (nr > tol * nb && fl[FL_IT] < fl[FL_MAXIT]) ? 1 : 0;
+  if (bid == 0 && tid == 0) {
+    fl[FL_OACT] = active;
+    fl[FL_ACT0] = active;
+    fl[FL_ACT1] = 0;
+    r.d.sc[SC_NBI] = nr;
+  }
+  if (!active || !r.fold) return;
+  const FoldPass &f = r.f;
+  int c, i, j, kg;
+  fold_index(f.q, c, i, j, kg);
+  const int L0 = f.q.L[c][0], L1 = f.q.L[c][1], h0 = f.q.h[c][0], h1 = f.q.h[c][1], e2 = f.q.e2[c];
+  if (i >= h0 || j >= h1) return;
+  const int is = L0 - 1 - i, js = L1 - 1 - j;
+  const bool ci = is == i, cj = js == j;
+  const View v = f.in[c];
+  const float *base = (const float *)v.p;
+  const size_t o_ij = (size_t)i + (size_t)v.s1 * j, o_sj = (size_t)is + (size_t)v.s1 * j;
+  const size_t o_is = (size_t)i + (size_t)v.s1 * js, o_ss = (size_t)is + (size_t)v.s1 * js;
+  const size_t qp = (size_t)i + (size_t)h0 * j, qs = (size_t)h0 * h1;
+  float *q00 = f.out + f.q.off[c][0] + qp, *q01 = f.out + f.q.off[c][1] + qp;
+  float *q10 = f.out + f.q.off[c][2] + qp, *q11 = f.out + f.q.off[c][3] + qp;
+  for (int k = kg; k < e2; k += 4) {
+    const float *pk = base + (size_t)v.s2 * k;
+    const double a = pk[o_ij], b = pk[o_sj], cc = pk[o_is], d = pk[o_ss];
+    const double sj = ci ? a : a + b, dj = ci ? 0.0 : a - b;
+    const double ss = ci ? cc : cc + d, ds = ci ? 0.0 : cc - d;
+    const size_t qk = qs * k;
+    q00[qk] = (float)(cj ? sj : sj + ss);
+    q01[qk] = (float)(cj ? 0.0 : sj - ss);
+    q10[qk] = (float)(cj ? dj : dj + ds);
+    q11[qk] = (float)(cj ? 0.0 : dj - ds);
+  }
+}
+hipError_t launch_ref_fold(const RefFold &r, hipStream_t s) {
+  if (g_rec) return hipErrorNotSupported;
+  if (fold_blocks(r.f.q) > MAX_PART) return hipErrorInvalidValue;
+  TDGL_LAUNCH(k_ref_fold, fold_grid(r.f.q), dim3(256), 0, s, r);
   return hipGetLastError();
 }
 
@@ -2507,15 +2580,21 @@ __global__ void __launch_bounds__(256) k_unfold(UnfoldPass u, const int *gate) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           if (!w[t]) continue;
-          rv[t] = ((const double *)u.r[c].p)[o[t] + zk];
-          if (u.dot == 2) zv[t] = ((const double *)u.zold[c].p)[o[t] + zk];
+          rv[t] = u.f32 ? (double)((const float *)u.r[c].p)[o[t] + zk] : ((const double *)u.r[c].p)[o[t] + zk];
+          if (u.dot == 2)
+            zv[t] = u.f32 ? (double)((const float *)u.zold[c].p)[o[t] + zk] : ((const double *)u.zold[c].p)[o[t] + zk];
         }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         if (!w[t]) continue;
-        ((double *)vo.p)[o[t] + zk] = val[t];
+        // Float32: the value as written (rounded), in the dots too
+        const double vw = u.f32 ? (double)(float)val[t] : val[t];
+        if (u.f32)
+          ((float *)vo.p)[o[t] + zk] = (float)val[t];
+        else
+          ((double *)vo.p)[o[t] + zk] = val[t];
         if (u.dot) {
-          d0 += rv[t] * val[t];
+          d0 += rv[t] * vw;
           if (u.dot == 2) d1 += rv[t] * zv[t];
         }
       }
@@ -2875,32 +2954,34 @@ __device__ __forceinline__ void gather24b(const double *P0, const double *P1, co
   v[P2C] = L(r2, 0), v[P2YP] = L(r2, sy), v[P2ZMYP] = L(r2, -sz + sy), v[P2ZM] = L(r2, -sz), v[P2YM] = L(r2, -sy);
   v[P2XP] = L(r2, 1), v[P2ZMXP] = L(r2, -sz + 1), v[P2XM] = L(r2, -1);
 }
-__device__ __forceinline__ void apply3v(const PcgArgs &a, const double v[NSTEN], const double dv[3], int i, int j,
-                                        int k, double q[3]) {
+template <class T>
+__device__ __forceinline__ void apply3v(const PcgArgs &a, const T v[NSTEN], const T dv[3], int i, int j, int k, T q[3]) {
   const int Nx = a.g.Nx, Ny = a.g.Ny, Nz = a.g.Nz;
+  const T dydz = (T)a.dydz, dzdx = (T)a.dzdx, dxdy = (T)a.dxdy;
+  const T sxy = (T)a.sxy, sxz = (T)a.sxz, syx = (T)a.syx, syz = (T)a.syz, szx = (T)a.szx, szy = (T)a.szy;
   // the ranges of hx (2:Nx, 2:Ny-1, 2:Nz-1), hy (2:Nx-1, 2:Ny, 2:Nz-1), hz (2:Nx-1, 2:Ny-1, 2:Nz)
   auto okx = [&](int ii, int jj, int kk) { return ii >= 2 && ii <= Nx && jj >= 2 && jj <= Ny - 1 && kk >= 2 && kk <= Nz - 1; };
   auto oky = [&](int ii, int jj, int kk) { return ii >= 2 && ii <= Nx - 1 && jj >= 2 && jj <= Ny && kk >= 2 && kk <= Nz - 1; };
   auto okz = [&](int ii, int jj, int kk) { return ii >= 2 && ii <= Nx - 1 && jj >= 2 && jj <= Ny - 1 && kk >= 2 && kk <= Nz; };
-  const double hx_c = okx(i, j, k) ? ((v[P2YP] - v[P2C]) - v[P1ZP] + v[P1C]) / a.dydz : 0.0;
-  const double hx_zm = okx(i, j, k - 1) ? ((v[P2ZMYP] - v[P2ZM]) - v[P1C] + v[P1ZM]) / a.dydz : 0.0;
-  const double hx_ym = okx(i, j - 1, k) ? ((v[P2C] - v[P2YM]) - v[P1YMZP] + v[P1YM]) / a.dydz : 0.0;
-  const double hy_c = oky(i, j, k) ? ((v[P0ZP] - v[P0C]) - v[P2XP] + v[P2C]) / a.dzdx : 0.0;
-  const double hy_zm = oky(i, j, k - 1) ? ((v[P0C] - v[P0ZM]) - v[P2ZMXP] + v[P2ZM]) / a.dzdx : 0.0;
-  const double hy_xm = oky(i - 1, j, k) ? ((v[P0XMZP] - v[P0XM]) - v[P2C] + v[P2XM]) / a.dzdx : 0.0;
-  const double hz_c = okz(i, j, k) ? ((v[P1XP] - v[P1C]) - v[P0YP] + v[P0C]) / a.dxdy : 0.0;
-  const double hz_ym = okz(i, j - 1, k) ? ((v[P1YMXP] - v[P1YM]) - v[P0C] + v[P0YM]) / a.dxdy : 0.0;
-  const double hz_xm = okz(i - 1, j, k) ? ((v[P1C] - v[P1XM]) - v[P0XMYP] + v[P0XM]) / a.dxdy : 0.0;
-  q[0] = dv[0] * v[P0C] + a.sxy * (hz_c - hz_ym) - a.sxz * (hy_c - hy_zm);
-  q[1] = dv[1] * v[P1C] + a.syz * (hx_c - hx_zm) - a.syx * (hz_c - hz_xm);
-  q[2] = dv[2] * v[P2C] + a.szx * (hy_c - hy_xm) - a.szy * (hx_c - hx_ym);
+  const T hx_c = okx(i, j, k) ? ((v[P2YP] - v[P2C]) - v[P1ZP] + v[P1C]) / dydz : T(0);
+  const T hx_zm = okx(i, j, k - 1) ? ((v[P2ZMYP] - v[P2ZM]) - v[P1C] + v[P1ZM]) / dydz : T(0);
+  const T hx_ym = okx(i, j - 1, k) ? ((v[P2C] - v[P2YM]) - v[P1YMZP] + v[P1YM]) / dydz : T(0);
+  const T hy_c = oky(i, j, k) ? ((v[P0ZP] - v[P0C]) - v[P2XP] + v[P2C]) / dzdx : T(0);
+  const T hy_zm = oky(i, j, k - 1) ? ((v[P0C] - v[P0ZM]) - v[P2ZMXP] + v[P2ZM]) / dzdx : T(0);
+  const T hy_xm = oky(i - 1, j, k) ? ((v[P0XMZP] - v[P0XM]) - v[P2C] + v[P2XM]) / dzdx : T(0);
+  const T hz_c = okz(i, j, k) ? ((v[P1XP] - v[P1C]) - v[P0YP] + v[P0C]) / dxdy : T(0);
+  const T hz_ym = okz(i, j - 1, k) ? ((v[P1YMXP] - v[P1YM]) - v[P0C] + v[P0YM]) / dxdy : T(0);
+  const T hz_xm = okz(i - 1, j, k) ? ((v[P1C] - v[P1XM]) - v[P0XMYP] + v[P0XM]) / dxdy : T(0);
+  q[0] = dv[0] * v[P0C] + sxy * (hz_c - hz_ym) - sxz * (hy_c - hy_zm);
+  q[1] = dv[1] * v[P1C] + syz * (hx_c - hx_zm) - syx * (hz_c - hz_xm);
+  q[2] = dv[2] * v[P2C] + szx * (hy_c - hy_xm) - szy * (hx_c - hx_ym);
 }
 __device__ __forceinline__ void apply3(const PcgArgs &a, const double *P0, const double *P1, const double *P2, int i,
                                        int j, int k, int c, double q[3], double pc[3]) {
   double v[NSTEN];
   gather24(P0, P1, P2, c, a.g.px, a.g.px * a.g.py, v);
   const double dv[3] = {a.D[0][c], a.D[1][c], a.D[2][c]};
-  apply3v(a, v, dv, i, j, k, q);
+  apply3v<double>(a, v, dv, i, j, k, q);
   pc[0] = v[P0C];
   pc[1] = v[P1C];
   pc[2] = v[P2C];
@@ -3104,7 +3185,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap(PcgArgs a) {
       }
       double v[NSTEN], qv[3];
       for (int n = 0; n < NSTEN; ++n) v[n] = a.q == 0 ? vz[n] : vz[n] + beta * vp[n];
-      apply3v(a, v, dv, b.i, b.j, b.k, qv);
+      apply3v<double>(a, v, dv, b.i, b.j, b.k, qv);
       const int cc[3] = {P0C, P1C, P2C};
       for (int comp = 0; comp < 3; ++comp) {
         if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
@@ -3130,10 +3211,19 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap(PcgArgs a) {
 // is the value every old-kernel thread computed for it, the operator and the block's partials
 // take the same values in the same order: the results are the same bits.
 constexpr int PT_W = 66, PT_H = 6, PT_N = PT_W * PT_H;  // the tile with its halo
+// T = float: the refinement's inner iteration on a.f (its P, z, Q and D in Float32; β and the
+// partials in FP64)
+template <class T>
 __global__ void __launch_bounds__(RED_T) k_pcg_pap_tile(PcgArgs a) {
+  constexpr bool F32 = sizeof(T) == 4;
   if (!a.d.fl[FL_ACT0 + (a.q & 1)]) return;
   __shared__ double sh[8];
-  __shared__ double Pl[3][3][PT_N];  // [component][level slot][row * 66 + col]
+  __shared__ T Pl[3][3][PT_N];  // [component][level slot][row * 66 + col]
+  const T *const *Zc = F32 ? (const T *const *)a.f.z.c : (const T *const *)a.z.c;
+  const T *const *Pc = F32 ? (const T *const *)a.f.Pold.c : (const T *const *)a.Pold.c;
+  T *const *Nc = F32 ? (T *const *)a.f.Pnew.c : (T *const *)a.Pnew.c;
+  T *const *Qc = F32 ? (T *const *)a.f.Q.c : (T *const *)a.Q.c;
+  const T *const *Dc = F32 ? (const T *const *)a.f.D : (const T *const *)a.D;
   const int tid = threadIdx.x;
   const BoxIdx b0 = box_idx(a.g, a.kc, a.xcd);
   const bool live = box_live(a.g, b0);
@@ -3151,19 +3241,19 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap_tile(PcgArgs a) {
     pidx[u] = a.g.at(ii, jj, 1);
   }
   const int sz = a.g.px * a.g.py;
-  double rz_[2][3], rp_[2][3];  // a plane in flight: z and Pold of the two points, three components
+  T rz_[2][3], rp_[2][3];  // a plane in flight: z and Pold of the two points, three components
   auto fetch = [&](int kk) {
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const int c = pidx[u] + sz * (kk - 1);
 #pragma unroll
       for (int comp = 0; comp < 3; ++comp) {  // clamped addresses: every load in bounds
-        rz_[u][comp] = a.z.c[comp][c];
-        rp_[u][comp] = pold ? a.Pold.c[comp][c] : 0.0;
+        rz_[u][comp] = Zc[comp][c];
+        rp_[u][comp] = pold ? Pc[comp][c] : T(0);
       }
     }
   };
-  auto stash = [&](int kk, double beta) {
+  auto stash = [&](int kk, T beta) {
     const int slot = kk % 3;
 #pragma unroll
     for (int u = 0; u < 2; ++u)
@@ -3194,11 +3284,12 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap_tile(PcgArgs a) {
     beta = a.flexible ? (rz2 - rzo) / rz : rz2 / rz;
   }
   if (b0.bid == 0 && tid == 0) a.d.sc[SC_RZ0 + (a.q & 1)] = rz2;
-  stash(kf - 1, beta);
+  const T betaT = (T)beta;
+  stash(kf - 1, betaT);
   fetch(kf);
-  stash(kf, beta);
+  stash(kf, betaT);
   fetch(kf + 1);
-  stash(kf + 1, beta);
+  stash(kf + 1, betaT);
   const int tx = (tid & 63) + 1, ty = (tid >> 6) + 1;
   auto L = [&](int comp, int kk, int row, int col) { return Pl[comp][kk % 3][row * PT_W + col]; };
   double s = 0.0;
@@ -3209,7 +3300,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap_tile(PcgArgs a) {
     if (more) fetch(k + 2);
     if (live) {
       const BoxIdx b = box_level(a.g, b0, q);
-      double v[NSTEN], qv[3];
+      T v[NSTEN], qv[3];
       v[P0C] = L(0, k, ty, tx), v[P0ZP] = L(0, k + 1, ty, tx), v[P0ZM] = L(0, k - 1, ty, tx);
       v[P0XMZP] = L(0, k + 1, ty, tx - 1), v[P0XM] = L(0, k, ty, tx - 1), v[P0YP] = L(0, k, ty + 1, tx);
       v[P0YM] = L(0, k, ty - 1, tx), v[P0XMYP] = L(0, k, ty + 1, tx - 1);
@@ -3219,20 +3310,20 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap_tile(PcgArgs a) {
       v[P2C] = L(2, k, ty, tx), v[P2YP] = L(2, k, ty + 1, tx), v[P2ZMYP] = L(2, k - 1, ty + 1, tx);
       v[P2ZM] = L(2, k - 1, ty, tx), v[P2YM] = L(2, k, ty - 1, tx), v[P2XP] = L(2, k, ty, tx + 1);
       v[P2ZMXP] = L(2, k - 1, ty, tx + 1), v[P2XM] = L(2, k, ty, tx - 1);
-      const double dv[3] = {a.D[0][b.c], a.D[1][b.c], a.D[2][b.c]};
-      apply3v(a, v, dv, b.i, b.j, b.k, qv);
+      const T dv[3] = {Dc[0][b.c], Dc[1][b.c], Dc[2][b.c]};
+      apply3v<T>(a, v, dv, b.i, b.j, b.k, qv);
       const int cc[3] = {P0C, P1C, P2C};
       for (int comp = 0; comp < 3; ++comp) {
         if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
-        const double pn = v[cc[comp]];
-        a.Pnew.c[comp][b.c] = pn;
-        if (a.flexible && a.zcopy) a.zold.c[comp][b.c] = a.z.c[comp][b.c];
-        a.Q.c[comp][b.c] = qv[comp];
-        s += pn * qv[comp];
+        const T pn = v[cc[comp]];
+        Nc[comp][b.c] = pn;
+        if (!F32 && a.flexible && a.zcopy) a.zold.c[comp][b.c] = a.z.c[comp][b.c];
+        Qc[comp][b.c] = qv[comp];
+        s += (double)pn * (double)qv[comp];
       }
     }
     __syncthreads();  // level k-1 is read: its slot takes level k+2
-    if (more) stash(k + 2, beta);
+    if (more) stash(k + 2, betaT);
   }
   const double r0 = block_sum_t(s, sh, tid);
   if (tid == 0) {
@@ -3301,6 +3392,46 @@ __global__ void __launch_bounds__(RED_T) k_pcg_xr(PcgArgs a) {
 }
 
 // φ = θ + x on the ranges (implicit.jl:440-444) and the cycle's counts of the solve
+// A refinement round's residual: r32 = b − A x with the FP64 operator (rounded to Float32 when
+// stored) and the partials of ||b − A x||² in FP64 (only those with norm_only); gated by
+// fl[gate_fl]
+__global__ void __launch_bounds__(RED_T) k_pcg_resid32(PcgArgs a) {
+  if (!a.d.fl[a.gate_fl]) return;
+  __shared__ double sh[8];
+  const int tid = threadIdx.x;
+  const BoxIdx b0 = box_idx(a.g, a.kc, a.xcd);
+  double s = 0.0;
+  if (box_live(a.g, b0))
+    for (int q = 0; q < b0.nk; ++q) {
+      const BoxIdx b = box_level(a.g, b0, q);
+      double qv[3], pc[3], bv[3];
+      for (int comp = 0; comp < 3; ++comp) bv[comp] = a.b.c[comp][b.c];
+      apply3(a, a.x.c[0], a.x.c[1], a.x.c[2], b.i, b.j, b.k, b.c, qv, pc);
+      for (int comp = 0; comp < 3; ++comp) {
+        if (!a.rg.in(comp, b.i, b.j, b.k)) continue;
+        const double rv = bv[comp] - qv[comp];
+        if (!a.norm_only) a.f.r.c[comp][b.c] = (float)rv;
+        s += rv * rv;
+      }
+    }
+  const double r0 = block_sum_t(s, sh, tid);
+  if (tid == 0) {
+    double *pr = region_w(a.d, PART_RR);
+    pr[2 * b0.bid] = r0;
+    pr[2 * b0.bid + 1] = 0.0;
+  }
+}
+// the end of a refinement round: x += d (FP64 += Float32), gated by fl[gate_fl]
+__global__ void __launch_bounds__(RED_T) k_pcg_xupdate(PcgArgs a) {
+  if (!a.d.fl[a.gate_fl]) return;
+  const BoxIdx b0 = box_idx(a.g, a.kc, a.xcd);
+  if (!box_live(a.g, b0)) return;
+  for (int q = 0; q < b0.nk; ++q) {
+    const BoxIdx b = box_level(a.g, b0, q);
+    for (int comp = 0; comp < 3; ++comp)
+      if (a.rg.in(comp, b.i, b.j, b.k)) a.x.c[comp][b.c] += (double)a.f.d.c[comp][b.c];
+  }
+}
 __global__ void __launch_bounds__(RED_T) k_pcg_finish(PcgArgs a) {
   const BoxIdx b0 = box_idx(a.g, a.kc, a.xcd);
   if (box_live(a.g, b0))
@@ -3311,7 +3442,7 @@ __global__ void __launch_bounds__(RED_T) k_pcg_finish(PcgArgs a) {
     }
   if (b0.bid == 0 && threadIdx.x == 0) {
     int *fl = a.d.fl;
-    if (fl[FL_NZ] && fl[FL_ACT0 + (a.act_slot & 1)]) fl[FL_OVF] = 1;
+    if (fl[FL_NZ] && fl[a.ovf_fl ? a.ovf_fl : FL_ACT0 + (a.act_slot & 1)]) fl[FL_OVF] = 1;
     fl[FL_ITSUM] += fl[FL_IT];
     if (fl[FL_IT] > fl[FL_ITMAX]) fl[FL_ITMAX] = fl[FL_IT];
   }
@@ -3329,9 +3460,14 @@ hipError_t launch_pcg_start(const PcgArgs &a, hipStream_t s) { return launch_box
 hipError_t launch_pcg_resid(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_resid, a, s); }
 hipError_t launch_pcg_p(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_p, a, s); }
 hipError_t launch_pcg_ap(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_ap, a, s); }
-hipError_t launch_pcg_pap(const PcgArgs &a, hipStream_t s) { return launch_box(a.tile ? k_pcg_pap_tile : k_pcg_pap, a, s); }
+hipError_t launch_pcg_pap(const PcgArgs &a, hipStream_t s) {
+  return launch_box(a.tile ? k_pcg_pap_tile<double> : k_pcg_pap, a, s);
+}
+hipError_t launch_pcg32_pap(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_pap_tile<float>, a, s); }
 hipError_t launch_pcg_xr(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_xr, a, s); }
 hipError_t launch_pcg_finish(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_finish, a, s); }
+hipError_t launch_pcg_resid32(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_resid32, a, s); }
+hipError_t launch_pcg_xupdate(const PcgArgs &a, hipStream_t s) { return launch_box(k_pcg_xupdate, a, s); }
 
 __global__ void k_pcg_cycle_reset(PcgDev d) {
   if (threadIdx.x == 0) {

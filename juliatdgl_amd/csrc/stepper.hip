@@ -176,6 +176,11 @@ struct tdgl_handle {
   double *olgf = nullptr, *oJbar = nullptr;
   // right-hand side of the implicit solve and the PCG vectors: node-shaped, zero outside the ranges
   Vec3 B{}, x{}, r{}, z{}, P{}, P2{}, Qv{}, xprev{}, zold{};
+  // the refinement (d.pcg_refine): its Float32 vectors and the rounds a captured solve holds
+  Pcg32 f32v{};
+  float *D32[3] = {};
+  bool refine = false;
+  int rounds = 3;
   PcgDev pd{};
   bool warm = false, have2 = false;
   int last_iters = 0;
@@ -684,6 +689,9 @@ int update_bands(tdgl_handle *h) {
 View node_view(tdgl_handle *h, const double *base) {
   return View{(void *)(base + h->g.at(2, 2, 2)), h->g.px, h->g.px * h->g.py};
 }
+View node_view(tdgl_handle *h, const float *base) {
+  return View{(void *)(base + h->g.at(2, 2, 2)), h->g.px, h->g.px * h->g.py};
+}
 View compact_view(tdgl_handle *h, const void *base, int c, size_t es) {
   return View{(void *)((const unsigned char *)base + (size_t)h->rg.off[c] * es), h->rg.e[c][0],
               h->rg.e[c][0] * h->rg.e[c][1]};
@@ -745,6 +753,9 @@ struct SepOut {
   bool addth;         // or φ = θ + value
   int dot;            // with the partials of r·z (1) and r·zold (2)
   const Vec3 *zold = nullptr;  // the zold of the dot (the handle's when null)
+  // Float32 (the refinement's inner solve; the folded path only): out, its r and zold
+  Vec3f *out32 = nullptr;
+  const Vec3f *r32 = nullptr, *zold32 = nullptr;
 };
 
 // (I + aK)^{-1} of the node-shaped vector `in` (_sep_solve!, implicit.jl:62-83, or in Float32
@@ -824,6 +835,14 @@ FoldPass fold_pass(tdgl_handle *h, const Vec3 &in) {
   f.q = h->q;
   return f;
 }
+FoldPass fold_pass(tdgl_handle *h, const Vec3f &in) {
+  FoldPass f{};
+  for (int c = 0; c < 3; ++c) f.in[c] = node_view(h, in.c[c]);
+  f.out = h->qa;
+  f.q = h->q;
+  f.f32 = 1;
+  return f;
+}
 
 // what follows the fold (h->qa holds the folded input): the forward x-y launch, which takes the
 // convergence check of a first pass, the z pass, the inverse x-y launch, the unfold
@@ -848,6 +867,15 @@ int sep_folded_tail(tdgl_handle *h, const SepOut &o, const int *gate, Conv conv)
   u.in = h->qa;
   u.q = h->q;
   for (int c = 0; c < 3; ++c) {
+    if (o.out32) {
+      u.f32 = 1;
+      u.out[c] = node_view(h, o.out32->c[c]);
+      if (o.dot) {
+        u.r[c] = node_view(h, o.r32->c[c]);
+        if (o.zold32) u.zold[c] = node_view(h, o.zold32->c[c]);
+      }
+      continue;
+    }
     u.out[c] = node_view(h, o.out->c[c]);
     if (o.dot) {
       u.r[c] = node_view(h, h->r.c[c]);
@@ -1010,6 +1038,7 @@ PcgArgs pcg_args(tdgl_handle *h) {
   a.tile = h->paptile;
   a.zcopy = 1;
   a.xcd = h->boxxcd;
+  a.f = h->f32v;
   for (int c = 0; c < 3; ++c) {
     a.D[c] = h->D[c];
     a.th[c] = h->th[c];
@@ -1070,10 +1099,127 @@ int pcg_iteration(tdgl_handle *h, int q, bool flexible) {
   return 0;
 }
 
+// Iteration q of a Float32 inner solve of the refinement: the iteration of pcg_iteration's folded
+// flexible path on the Float32 vectors (x the round's increment d), its convergence against
+// SC_TOLI x SC_NBI (the round's starting residual).  Gated by ACT[q % 2].
+int pcg32_iteration(tdgl_handle *h, int q) {
+  PcgArgs a = pcg_args(h);
+  a.q = q;
+  a.flexible = 1;
+  a.f.Pnew = (q % 2 == 0) ? h->f32v.Pnew : h->f32v.Pold;
+  a.f.Pold = (q % 2 == 0) ? h->f32v.Pold : h->f32v.Pnew;
+  Vec3f *zq = (q & 1) ? &h->f32v.zold : &h->f32v.z, *zn = (q & 1) ? &h->f32v.z : &h->f32v.zold;
+  a.f.z = *zq;
+  a.nparts_in = sep_last_parts(h, true);
+  PL(K_APPLYA, launch_pcg32_pap(a, h->s));
+  a.nparts_in = pcg_box_blocks(h->g, h->box_kc);
+  PL(K_VEC, launch_pcg32_xr_fold(a, fold_pass(h, h->f32v.r), h->s));
+  Conv cv{CONV_RR, h->pd.fl + FL_ACT0 + (q & 1), h->pd.fl + FL_ACT0 + ((q + 1) & 1), fold_blocks(h->q), 1};
+  SepOut o{nullptr, false, 2, nullptr, zn, &h->f32v.r, zq};
+  return sep_folded_tail(h, o, nullptr, cv);
+}
+
+// One round of the refinement: r32 = b − A x (FP64), the round's check and the fold of r32, the
+// Float32 solve A d = r32 from d = 0 (its first preconditioner, then `budget` gated iterations, or
+// eagerly while active), x += d.  Round 0 is gated by NZ, the others by OACT.
+int refine_round(tdgl_handle *h, int round) {
+  const tdgl_desc &d = h->d;
+  PcgArgs a = pcg_args(h);
+  a.gate_fl = round == 0 ? FL_NZ : FL_OACT;
+  PL(K_APPLYA, launch_pcg_resid32(a, h->s));
+  RefFold rf{};
+  rf.f = fold_pass(h, h->f32v.r);
+  rf.gate = h->pd.fl + a.gate_fl;
+  rf.nparts = pcg_box_blocks(h->g, h->box_kc);
+  rf.fold = 1;
+  rf.d = h->pd;
+  PL(K_FOLD, launch_ref_fold(rf, h->s));
+  const int *oact = h->pd.fl + FL_OACT;
+  {
+    SepOut o{nullptr, false, 1, nullptr, &h->f32v.z, &h->f32v.r, nullptr};
+    CHK(sep_folded_tail(h, o, oact, Conv{}));
+  }
+  int q = 0;
+  if (h->capturing) {
+    for (; q < h->budget; ++q) CHK(pcg32_iteration(h, q));
+  } else {
+    int batch = 4;
+    for (;;) {
+      HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
+      HIPCHK(hipStreamSynchronize(h->s));
+      if (!h->flh[FL_ACT0 + (q & 1)] || h->flh[FL_IT] >= d.pcg_maxit) break;
+      for (int t = 0; t < batch; ++t, ++q) CHK(pcg32_iteration(h, q));
+      batch = 2;
+    }
+  }
+  a.gate_fl = FL_OACT;
+  PL(K_VEC, launch_pcg_xupdate(a, h->s));
+  return 0;
+}
+
+// The refinement (d.pcg_refine): the FP64 operator, residual and solution around Float32 inner
+// PCG solves with the folded Float32 preconditioner.  Each round solves A d = b − A x to SC_TOLI of
+// its starting residual; the rounds stop when ||b − A x|| <= tol ||b|| (FP64) or at the iteration
+// limit.  Eager: rounds while the check says so.  Captured: `rounds` rounds, each with `budget`
+// gated inner iterations, then a last FP64 check; a solve still above the tolerance overflows
+// (the cycle is rolled back and stepped eagerly).
+int implicit_pcg_refine(tdgl_handle *h) {
+  const tdgl_desc &d = h->d;
+  const bool extrap = d.pcg_extrap && d.pcg_fast;
+  const int *nz = h->pd.fl + FL_NZ;
+  PcgArgs a = pcg_args(h);
+  a.warm = h->warm;
+  a.extrap = extrap;
+  a.have2 = h->have2;
+  a.nparts_in = field_blocks(h->g);
+  PL(K_VEC, launch_pcg_start(a, h->s));
+  if (!h->warm) {  // cold start: x = M⁻¹ b in Float64
+    SepOut o{&h->x, false, 0};
+    CHK(sep_solve(h, h->B, o, nz, false, Conv{CONV_NONE, nullptr, nullptr, 0}));
+  }
+  h->have2 = h->warm && extrap;
+  h->warm = true;
+  if (h->capturing) {
+    for (int r = 0; r < h->rounds; ++r) CHK(refine_round(h, r));
+  } else {
+    for (int r = 0;; ++r) {
+      if (r > 0) {
+        HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
+        HIPCHK(hipStreamSynchronize(h->s));
+        if (!h->flh[FL_OACT] || h->flh[FL_IT] >= d.pcg_maxit) break;
+      }
+      CHK(refine_round(h, r));
+    }
+  }
+  // the last check: ||b − A x|| of the final x (OACT set when still above the tolerance)
+  a = pcg_args(h);
+  a.gate_fl = h->capturing ? FL_OACT : FL_NZ;
+  a.norm_only = 1;
+  PL(K_APPLYA, launch_pcg_resid32(a, h->s));
+  RefFold rf{};
+  rf.f = fold_pass(h, h->f32v.r);
+  rf.gate = h->pd.fl + a.gate_fl;
+  rf.nparts = pcg_box_blocks(h->g, h->box_kc);
+  rf.fold = 0;
+  rf.d = h->pd;
+  PL(K_FOLD, launch_ref_fold(rf, h->s));
+  if (!h->capturing) {
+    HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    h->last_iters = h->flh[FL_IT];
+  }
+  a.ovf_fl = h->capturing ? FL_OACT : 0;
+  a.act_slot = 0;
+  if (!h->capturing) a.ovf_fl = FL_ACT1;  // eager: the solve ran to its end (ACT1 is 0 after a round's check)
+  PL(K_SCATTER, launch_pcg_finish(a, h->s));
+  return 0;
+}
+
 // implicit_curl!(s, ::PCGSolver) / _implicit_curl_fast! (implicit.jl:279-446), after the field
 // kernel has written b = φₑ − θ and the partials of b·b, and scaled φ by D.  Eager: the host
 // reads the flags between batches of iterations.  Captured: `budget` iterations, each gated.
 int implicit_pcg(tdgl_handle *h) {
+  if (h->refine) return implicit_pcg_refine(h);
   const tdgl_desc &d = h->d;
   const bool fast = d.pcg_fast || d.pcg_f32;
   const bool f32 = d.pcg_f32 != 0;
@@ -1724,6 +1870,9 @@ int check_desc(const tdgl_desc *d, char *why, size_t n) {
   if (d->field_mode < 0 || d->field_mode > 3) return no("field mode");
   if (d->field_mode == TDGL_FIELD_PCG && (d->pcg_maxit < 1 || d->pcg_maxit > (1 << 20) || !(d->pcg_tol >= 0)))
     return no("PCG iteration limit (1..2^20) or tolerance");
+  if (d->pcg_refine && (d->field_mode != TDGL_FIELD_PCG || !d->pcg_f32 || d->pcg_rounds < 1 || d->pcg_rounds > 16 ||
+                        !(d->pcg_inner_tol > 0 && d->pcg_inner_tol < 1)))
+    return no("pcg_refine needs the PCG with pcg_f32, pcg_rounds in 1..16 and pcg_inner_tol in (0, 1)");
   if ((d->field_mode == TDGL_FIELD_PCG || d->field_mode == TDGL_FIELD_EXPLICIT_SCALED) && !(d->Dx && d->Dy && d->Dz))
     return no("link conductivities D");
   if ((d->field_mode == TDGL_FIELD_SEPARABLE || d->field_mode == TDGL_FIELD_PCG) && !(d->dx == d->dy && d->dy == d->dz))
@@ -1922,6 +2071,7 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
       if (pcg_box_blocks(g, h->box_kc) > MAX_PART || field_blocks(g) > MAX_PART) return fail(TDGL_E_UNSUPPORTED, "grid too large for the PCG reductions");
       std::vector<double> sc(SC_COUNT, 0.0);
       sc[SC_TOL] = d.pcg_tol;
+      sc[SC_TOLI] = d.pcg_inner_tol;
       CHK(upload_vec(h, h->pd.sc, sc));
       std::vector<int> fl(FL_COUNT, 0);
       fl[FL_MAXIT] = d.pcg_maxit;
@@ -2134,6 +2284,27 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
         }
         h->fold = true;
       }
+    }
+    if (d.field_mode == TDGL_FIELD_PCG && d.pcg_refine) {
+      // FP64 iterative refinement around Float32 inner solves: their vectors, D in Float32
+      if (!h->fold || !d.pcg_f32 || !(d.pcg_fast || d.pcg_f32))
+        return fail(TDGL_E_UNSUPPORTED, "the refinement needs the folded Float32 preconditioner (pcg_f32)");
+      if (d.pcg_rounds < 1 || d.pcg_rounds > 16) return fail(TDGL_E_ARG, "pcg_rounds must be 1..16");
+      Vec3f *fv[7] = {&h->f32v.d, &h->f32v.r, &h->f32v.z, &h->f32v.zold, &h->f32v.Pnew, &h->f32v.Pold, &h->f32v.Q};
+      for (Vec3f *v : fv)
+        for (int c = 0; c < 3; ++c) {
+          ALLOC(v->c[c], n);
+          CHK(upload_vec(h, v->c[c], std::vector<float>((size_t)n, 0.0f)));
+        }
+      for (int c = 0; c < 3; ++c) {
+        std::vector<double> dd((size_t)n);
+        CHK(hcopy(h, dd.data(), h->D[c], (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        ALLOC(h->D32[c], n);
+        CHK(upload_vec(h, h->D32[c], std::vector<float>(dd.begin(), dd.end())));
+        h->f32v.D[c] = h->D32[c];
+      }
+      h->refine = true;
+      h->rounds = d.pcg_rounds;
     }
     return 0;
   };

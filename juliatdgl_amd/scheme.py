@@ -796,6 +796,11 @@ class Stepper:
     open: Optional[OpenData] = None
     noise: Optional[NoiseSpec] = None
     sk: tuple = ()            # _skrock(st, µ) per band: the phase kick inside the first RKC stage
+    # the PCG's precision (an extension): FP64 refinement around Float32 inner solves, each to
+    # pcg_inner_tol of its starting residual; pcg_rounds rounds per captured solve
+    pcg_refine: bool = False
+    pcg_inner_tol: float = 1e-4
+    pcg_rounds: int = 3
 
     @property
     def N(self) -> int:
@@ -810,6 +815,7 @@ def make_stepper(md: Model, gr: Grid, mk: Masks, *, implicit=True, substeps=None
                  extrap=True, hard=False, sigma_vac=None, pcg_tol=1e-8, selffield_bc="bar",
                  rkl="rkc", rotframe=None, rkl_margin=None, rkc_damping=RKC_EPS, pcg_fast=True,
                  pcg_f32=True, pcg_extrap=True, dt=None, pcg_maxit=500, qp_exchange=None, qp_split=None,
+                 pcg_refine=False, pcg_inner_tol=1e-4, pcg_rounds=3,
                  open_every=1.0, noise=None, noise_seed=1, noise_field=False, noise_draws="device",
                  **unsupported) -> Stepper:
     """Stepper(md, gr, mk; kw...) (scheme.jl:250-434) for the options the HIP stepper runs.
@@ -819,10 +825,19 @@ def make_stepper(md: Model, gr: Grid, mk: Masks, *, implicit=True, substeps=None
     its band's bound, stages from r = rkl_margin·dt/(0.99·bbk)).
 
     `noise = T̃`, `noise_seed`, `noise_field` as the reference's (noise.jl); `noise_draws`
-    "device" (Philox on the device, keyed by the seed) or "host" (the caller's draws per step)."""
+    "device" (Philox on the device, keyed by the seed) or "host" (the caller's draws per step).
+
+    `pcg_refine` (an extension of the PCG solver, with pcg_f32 and the insulating padding): the FP64
+    operator, residual and solution refine around Float32 inner PCG solves, each to `pcg_inner_tol`
+    of its starting residual, until ||b − A x|| <= pcg_tol ||b||; `pcg_rounds` rounds a captured
+    solve holds."""
     for k, v in unsupported.items():
         if v is not None and v is not False:
             raise NotImplementedError(f"Stepper option {k!r} is not run by the HIP stepper")
+    if pcg_refine and not (pcg_f32 and sigma_vac is not None and float(sigma_vac) != md.sigma_n):
+        raise ValueError("pcg_refine needs the PCG (sigma_vac) with its Float32 preconditioner (pcg_f32)")
+    if not (0 < float(pcg_inner_tol) < 1) or not (1 <= int(pcg_rounds) <= 16):
+        raise ValueError("pcg_inner_tol must be in (0, 1) and pcg_rounds in 1..16")
     if mk.vac is not None and mk.vac.any():
         raise NotImplementedError("a sculpted sample (vacuum = true in the shape file) is not run by the HIP stepper")
     if mk.mat is not None and (mk.mat != 1).any():
@@ -950,4 +965,4 @@ def make_stepper(md: Model, gr: Grid, mk: Masks, *, implicit=True, substeps=None
     return Stepper(md, gr, mk, dtv, tuple(nsub), tuple(stg), bc, bcs, w, wa, fc, mode, D, bar, bool(extrap), rotframe,
                    bool(hard), limit, float(dt_scale), rkl_margin, rkcc, tuple(float(divisor(mu)) for mu in range(N)),
                    float(pcg_tol), int(pcg_maxit), bool(pcg_fast), bool(pcg_f32), bool(pcg_extrap), sbc, qp, ob,
-                   nz, skb)
+                   nz, skb, bool(pcg_refine), float(pcg_inner_tol), int(pcg_rounds))
