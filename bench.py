@@ -215,6 +215,8 @@ def bench_sweep(args) -> int:
     return 0
 
 
+REFINE = {"pcg_inner_tol": 1e-4, "pcg_rounds": 3}  # the refinement workload's inner tolerance and rounds
+
 WORKLOADS = {
     "open_3band_256x128x32": (
         lambda: configs.open_3band_256x128x32(),
@@ -222,10 +224,10 @@ WORKLOADS = {
         "PCG tol 1e-8 (Float32 preconditioner, FP64 outer), open boundary (FFT convolution, every t_GL), notch, H=0.2, "
         "J=0.1, ladder cycle = 40 steps", "notch mask"),
     "open_3band_256x128x32_refine": (
-        lambda: configs.open_3band_256x128x32(pcg_refine=True),
+        lambda: configs.open_3band_256x128x32(pcg_refine=True, **REFINE),
         "three-band anisotropic (m_z = 4 m_xy) TDGL, 256x128x32 cells, dx=0.5, buf 8 insulating (sigma_vac/sigma_n=0.01), "
         "PCG tol 1e-8 by FP64 iterative refinement around Float32 inner PCG solves (each to 1e-4 of its starting "
-        "residual, Float32 preconditioner), open boundary (FFT convolution, every t_GL), notch, H=0.2, J=0.1, ladder "
+        "residual by default, Float32 preconditioner), open boundary (FFT convolution, every t_GL), notch, H=0.2, J=0.1, ladder "
         "cycle = 40 steps", "notch mask"),
     "kwt_pcg_128x64x16": (
         lambda: configs.kwt_pcg_128x64x16(),
@@ -398,8 +400,10 @@ def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=Non
                       "b_alg_per_node_step": balg,
                       "hbm_roofline_frac_cycle": balg * nodes * LMAX * steps / wall / HBM_PEAK,
                       "ms_per_time_step": 1e3 * wall / nsteps, "timed_rollbacks": rollbacks,
-                      "timed_graph_captures": captures,
-                      "tuned": None if tuned is None else {k: tuned[k] for k in ("box_kc", "plane_mt", "zmodes_waves")}},
+                      "timed_graph_captures": captures, "pcg_budget": st1.get("pcg_budget"),
+                      "tuned": None if tuned is None else {k: tuned[k] for k in ("box_kc", "plane_mt", "zmodes_waves")},
+                      **({"pcg_refine": {"inner_tol": st.pcg_inner_tol, "rounds": st.pcg_rounds}}
+                         if getattr(st, "pcg_refine", False) else {})},
            "mask": mask, "roofline": None, "cpu": None}
     if profile:
         dev.profile(True)
@@ -499,6 +503,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
                     help="leave out the 128x64x16 KWT workload that the 256x128x32 line carries beside it")
+    ap.add_argument("--inner-tol", type=float, default=1e-4,
+                    help="open_3band_256x128x32_refine: the inner solves' relative tolerance")
+    ap.add_argument("--rounds", type=int, default=3, help="open_3band_256x128x32_refine: rounds per captured solve")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default launch shapes (by default its tuner picks them before the warm-up)")
     ap.add_argument("--profile-out", default=None, help="write the per-kernel profile (JSON) here")
@@ -516,6 +523,7 @@ def main(argv=None):
         return bench_sweep(args)
     if args.workload == "serve":
         return bench_serve(args)
+    REFINE.update(pcg_inner_tol=args.inner_tol, pcg_rounds=args.rounds)
     if args.copy_probe:
         copy_probe(args.copy_probe)
     cpu_steps = 0 if args.no_cpu else (args.cpu_steps or 8)

@@ -104,32 +104,35 @@ def test_open_3band_reduced_against_the_oracle(mode):
 
 
 def test_open_3band_256x128x32_fp32_against_all_fp64():
-    """The full 256x128x32 workload with the Float32 preconditioner inside the FP64 PCG against the
-    same run with the preconditioner in FP64: ψ and A after five ladder cycles (200 steps from the
-    seeded state, five recomputations of the open boundary).  The tolerance of both solves is the
-    workload's 1e-8; the states agree to a few times it."""
+    """The full 256x128x32 workload against the same run with every part of the PCG in FP64 (the
+    preconditioner too): with the Float32 preconditioner inside the FP64 PCG (the reference's solver)
+    and with the FP64 iterative refinement around Float32 inner solves (pcg_refine), ψ and A after
+    five ladder cycles (200 steps from the seeded state, five recomputations of the open boundary).
+    The tolerance of every solve is the workload's 1e-8; the states agree to a few times it."""
     HipStepper = _hip()
-    out = []
-    for f32 in (True, False):
-        st, p = configs.open_3band_256x128x32(pcg_f32=f32)
+    out = {}
+    for name, kw in (("f64", dict(pcg_f32=False)), ("f32", {}), ("refine", dict(pcg_refine=True))):
+        st, p = configs.open_3band_256x128x32(**kw)
         H, HJ = p["H"], st.selffield(p["J"])
         s0 = seeded_state(st.model, st.grid, st.masks, H, seed=20261202)
-        print(f"[gpu] open_3band 256x128x32 f32={f32}: 5 cycles", flush=True)
+        print(f"[gpu] open_3band 256x128x32 {name}: 5 cycles", flush=True)
         dev = HipStepper(st)
         dev.upload(s0)
         for _ in range(5):
             dev.step(40, H, HJ)
-        out.append((dev.download(), dev.stats()))
+        out[name] = (dev.download(), dev.stats())
         dev.close()
-    (a, sa), (b, sb) = out
-    dpsi = max(float(np.max(np.abs(x - y)) / np.max(np.abs(y))) for x, y in zip(a.G, b.G))
-    dA = max(float(np.max(np.abs(getattr(a, k) - getattr(b, k)))) for k in ("phx", "phy", "phz")) / \
-        max(float(np.max(np.abs(getattr(b, k)))) for k in ("phx", "phy", "phz"))
-    dh = max(float(np.max(np.abs(getattr(a, k) - getattr(b, k)))) for k in ("hx", "hy", "hz"))
-    print(f"    |Δψ|/|ψ| {dpsi:.2e}, |ΔA|/|A| {dA:.2e}, |Δh| {dh:.2e}; PCG iterations f32 {sa['pcg_iters']}, "
-          f"f64 {sb['pcg_iters']}", flush=True)
-    assert all(np.isfinite(g).all() for g in a.G)
-    assert dpsi <= 1e-6 and dA <= 1e-6
+    b, sb = out["f64"]
+    for name in ("f32", "refine"):
+        a, sa = out[name]
+        dpsi = max(float(np.max(np.abs(x - y)) / np.max(np.abs(y))) for x, y in zip(a.G, b.G))
+        dA = max(float(np.max(np.abs(getattr(a, k) - getattr(b, k)))) for k in ("phx", "phy", "phz")) / \
+            max(float(np.max(np.abs(getattr(b, k)))) for k in ("phx", "phy", "phz"))
+        dh = max(float(np.max(np.abs(getattr(a, k) - getattr(b, k)))) for k in ("hx", "hy", "hz"))
+        print(f"    {name} against all-FP64: |Δψ|/|ψ| {dpsi:.2e}, |ΔA|/|A| {dA:.2e}, |Δh| {dh:.2e}; PCG "
+              f"iterations {sa['pcg_iters']} (all-FP64 {sb['pcg_iters']})", flush=True)
+        assert all(np.isfinite(g).all() for g in a.G)
+        assert dpsi <= 1e-6 and dA <= 1e-6
 
 
 def test_anisotropic_plane_wave_on_the_device():

@@ -96,3 +96,44 @@ def test_refinement_against_all_fp64_on_the_three_band_workload():
     print("[gpu] refinement vs all-FP64, 48x24x8 three bands: " +
           ", ".join(f"{k} {errs[k]:.2e}" for k in ("G1", "G2", "G3", "thx", "thy", "thz")), flush=True)
     assert max(v for k, v in errs.items() if not k.startswith(("U", "V"))) <= 1e-6, errs
+
+
+def test_refinement_onset_bins_against_all_fp64():
+    """The golden ladder's sample with an insulating buffer (buf 2, σvac/σn = 0.01, PCG tol 1e-10):
+    the two-polarity ladders stepped with the refinement have the verdicts, cycles and onset
+    brackets of the same ladders stepped with every part of the PCG in FP64, and 400 steps at J = 0.6
+    leave ψ and A within 1e-8 of them."""
+    from juliatdgl_amd.meter import run_ladders
+    from juliatdgl_amd.scheme import Grid, make_stepper, wedge_masks
+    from juliatdgl_amd.state import initial_state
+    from test_ladder import golden_setup
+    HipStepper = _hip()
+    _, _, st0, pr = golden_setup()
+    gr = Grid.make(24, 12, 4, dx=0.5, buf=2)
+    mk = wedge_masks(gr)
+    kw = dict(selffield_bc="slab", rkl=False, sigma_vac=0.01, pcg_tol=1e-10)
+    str_ = make_stepper(st0.model, gr, mk, pcg_refine=True, **kw)
+    st64 = make_stepper(st0.model, gr, mk, pcg_fast=False, pcg_f32=False, **kw)
+    print("[gpu] refinement against all-FP64: golden ladder sample", flush=True)
+    out = []
+    for st in (str_, st64):
+        d = HipStepper(st)
+        d.upload(initial_state(st.model, st.grid, st.masks, 0.03))
+        d.step(400, 0.03, st.selffield(0.6))
+        out.append(d.download())
+        d.close()
+    a, b = out
+    dpsi = max(float(np.max(np.abs(x - y))) for x, y in zip(a.G, b.G))
+    dA = max(float(np.max(np.abs(getattr(a, k) - getattr(b, k)))) for k in ("phx", "phy", "phz"))
+    print(f"    400 steps at J = 0.6: |Δψ| {dpsi:.2e}, |ΔA| {dA:.2e}", flush=True)
+    assert dpsi <= 1e-8 and dA <= 1e-8
+    Js = [0.30, 0.60, 0.90, 1.20, 1.50]
+    rr = run_ladders(lambda st: HipStepper(st), str_, Js, H=0.03, pr=pr)
+    r64 = run_ladders(lambda st: HipStepper(st), st64, Js, H=0.03, pr=pr)
+    for La, Lb in zip(rr[:2], r64[:2]):
+        print(f"    {''.join(La.conv)} {list(La.m)} / {''.join(Lb.conv)} {list(Lb.m)}", flush=True)
+        assert La.conv == Lb.conv and list(La.m) == list(Lb.m)
+    print("    brackets " + ", ".join(f"{p} [{rr[2][p]['lo']}, {rr[2][p]['hi']}] / [{r64[2][p]['lo']}, {r64[2][p]['hi']}]"
+                                   for p in ("pos", "neg")), flush=True)
+    for pol in ("pos", "neg"):
+        assert (rr[2][pol]["lo"], rr[2][pol]["hi"]) == (r64[2][pol]["lo"], r64[2][pol]["hi"])
