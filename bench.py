@@ -450,11 +450,13 @@ def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=Non
 
 
 def bench_serve(args) -> int:
-    """The served current sweep: a Poisson stream of requests (juliatdgl_amd.sweep.serving_specs: two or
-    three bands, grids log-uniform from 24x16x4 to 256x128x32, a notch or a hole, a random current and
-    polarity, thermal noise T~ = 1e-3 drawn on the device), admitted by HBM footprint as slots free up,
-    each run ending at its first phase slip across the voltmeter (crossing_times with stop = 1) or at
-    t = T.  Runs/hour of the stream and p50/p99 latency (finish - arrival)."""
+    """The served current sweep: requests (juliatdgl_amd.sweep.serving_specs: two or three bands, grids
+    log-uniform from 24x16x4 to 256x128x32, a notch or a hole, a random current and polarity, thermal
+    noise T~ = 1e-3 drawn on the device) admitted by HBM footprint as slots free up, each run ending
+    at its first phase slip across the voltmeter (crossing_times with stop = 1) or at t = T.  First the
+    stream's capacity (every request there at t = 0: runs/hour), then a Poisson stream at
+    `serve_load` of that rate (or at --serve-rate requests/s): its p50/p99 latency (finish - arrival)
+    and whether every result equals the run's own alone (the first phase slip's time)."""
     from juliatdgl_amd import sweep
     n = args.serve_runs
     specs = sweep.serving_specs(n, master=4242)
@@ -463,7 +465,16 @@ def bench_serve(args) -> int:
     print(f"[bench] serve: {n} requests, {cells} cells, first slip or t = {args.serve_T}", file=sys.stderr, flush=True)
     sweep.run_batch(specs[:2], sweep.Cycles(cycles=1), workers=2)  # the device and the library warm
     srv = sweep.Server(budget_bytes=200e9, slots=args.workers)
-    arrivals = sweep.poisson_arrivals(n, rate=args.serve_rate, seed=11)
+    for s in specs:
+        srv.submit(s, 0.0)
+    t0 = time.perf_counter()
+    full = srv.run(work)
+    t_cap = time.perf_counter() - t0
+    cap = n / t_cap
+    print(f"[bench] capacity: {n} runs in {t_cap:.2f} s ({3600 * cap:.0f} runs/hr)", file=sys.stderr, flush=True)
+    rate = args.serve_rate if args.serve_rate > 0 else args.serve_load * cap
+    srv = sweep.Server(budget_bytes=200e9, slots=args.workers)
+    arrivals = sweep.poisson_arrivals(n, rate=rate, seed=11)
     for s, t in zip(specs, arrivals):
         srv.submit(s, t)
     t0 = time.perf_counter()
@@ -471,17 +482,21 @@ def bench_serve(args) -> int:
     wall = time.perf_counter() - t0
     lat = sweep.percentiles(d["latency"] for d in done.values())
     slipped = sum(1 for d in done.values() if d["result"][0] is not None)
-    print(f"[bench] served {n} runs in {wall:.2f} s: p50 {lat['p50']:.3f} s, p99 {lat['p99']:.3f} s, {slipped} slipped",
-          file=sys.stderr, flush=True)
-    line = {"metric": "served runs/hr", "value": 3600.0 * n / wall, "unit": "runs/hr", "n_gpus": 1, "steps": n,
-            "warmup": 1, "ms_per_step": 1e3 * wall / n, "higher_is_better": True, "scaling": "weak",
+    same = all(done[q]["result"] == full[q]["result"] for q in done)
+    print(f"[bench] Poisson at {rate:.2f}/s: {n} runs in {wall:.2f} s: p50 {lat['p50']:.3f} s, p99 {lat['p99']:.3f} s, "
+          f"{slipped} slipped, results as at capacity: {same}", file=sys.stderr, flush=True)
+    line = {"metric": "served runs/hr", "value": 3600.0 * cap, "unit": "runs/hr", "n_gpus": 1, "steps": n,
+            "warmup": 1, "ms_per_step": 1e3 * t_cap / n, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded psi per run, thermal noise from the device RNG)",
-            "config": {"workload": "served current sweep: Poisson stream of two- and three-band runs, grids log-uniform "
-                                   "24x16x4..256x128x32, notch or hole, |J| in [0.3, 1.5] of either polarity, noise "
-                                   f"T~ = 1e-3, separable field, dt 0.01, each to its first phase slip or t = {args.serve_T}",
-                       "runs": n, "cells": cells, "slots": args.workers, "arrival_rate_per_s": args.serve_rate,
+            "config": {"workload": "served current sweep: two- and three-band runs, grids log-uniform 24x16x4..256x128x32, "
+                                   "notch or hole, |J| in [0.3, 1.5] of either polarity, noise T~ = 1e-3, separable "
+                                   f"field, dt 0.01, each to its first phase slip or t = {args.serve_T}; value: every "
+                                   "request at t = 0 (capacity); latency: a Poisson stream below it",
+                       "runs": n, "cells": cells, "slots": args.workers, "capacity_s": t_cap,
+                       "poisson_rate_per_s": rate, "poisson_wall_s": wall, "poisson_runs_per_hr": 3600.0 * n / wall,
                        "p50_latency_s": lat["p50"], "p99_latency_s": lat["p99"], "mean_latency_s": lat["mean"],
-                       "slipped": slipped, "bands": {str(b): sum(1 for s in specs if s.bands == b) for b in (2, 3)}},
+                       "slipped": slipped, "same_as_at_capacity": same,
+                       "bands": {str(b): sum(1 for s in specs if s.bands == b) for b in (2, 3)}},
             "roofline": None, "cpu_baseline": None}
     print(json.dumps(line), flush=True)
     return 0
@@ -494,8 +509,10 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5,
                     help="untimed ladder cycles (the PCG budget settles and its graph is captured in them)")
     ap.add_argument("--workload", default="open_3band_256x128x32", choices=tuple(WORKLOADS) + ("sweep64", "serve"))
-    ap.add_argument("--serve-runs", type=int, default=32, help="serve: requests of the stream")
-    ap.add_argument("--serve-rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-runs", type=int, default=64, help="serve: requests of the stream")
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="serve: Poisson arrival rate (requests/s); 0: --serve-load of the measured capacity")
+    ap.add_argument("--serve-load", type=float, default=0.8, help="serve: the Poisson rate as a fraction of capacity")
     ap.add_argument("--serve-T", type=float, default=40.0, help="serve: a run ends at its first slip or at this time")
     ap.add_argument("--workers", type=int, default=8, help="sweep64: runs in flight at once")
     ap.add_argument("--cpu-steps", type=int, default=0,

@@ -243,11 +243,12 @@ int tdgl_step_noalloc(tdgl_handle *h, int32_t n, double H, double HJ);
  * stepper it times, on the handle's own state and at (H, HJ), one time step per candidate launch
  * shape and keeps the fastest: the PCG box kernels' k levels per block (box_kc: 1, 2, 4, ... while
  * the reductions' partials fit), the m tiles per block of the one-round x-y transform (2 or 3), and
- * the waves per block (groups of z modes) of the folded preconditioner's z pass (4 or 8).  The
- * state (F and the normal-fluid potentials included), the PCG history, the open boundary and the
- * noise position are put back; captured cycles are dropped.  box_kc
- * changes how the PCG's block partials fall (its sums round differently), the others change no
- * result.  cand[] lists every candidate timed (us per launch of the kernels it shapes). */
+ * the folded preconditioner's z pass: on the matrix cores (0) or with 4 or 8 waves per block
+ * (groups of z modes) on the vector ALUs.  The state (F and the normal-fluid potentials included),
+ * the PCG history, the open boundary and the noise position are put back; captured cycles are
+ * dropped.  box_kc changes how the PCG's block partials fall and the z pass's kernel how its sums
+ * are grouped (both round differently); the m tiles and the vector ALUs' waves change no result.
+ * cand[] lists every candidate timed (us per launch of the kernels it shapes). */
 #define TDGL_TUNE_MAX 16
 enum { TDGL_TUNE_BOX_KC = 1, TDGL_TUNE_PLANE_MT = 2, TDGL_TUNE_ZMODES_WAVES = 3 };
 typedef struct {

@@ -342,7 +342,7 @@ struct ZModesQ {
   const float *sx, *sy, *sz;
   int nx, ny, nz;
   double a;
-  int waves;             // waves per block, each a group of z modes: 4 or 8, 0 the library's choice
+  int waves;             // k_zmodes_q: waves per block, each a group of z modes (4 or 8); 0: k_zmodes_mf (matrix cores)
 };
 hipError_t launch_zmodes_q(const ZModesQ &m, const int *gate, hipStream_t s);
 
@@ -512,7 +512,13 @@ struct PcgArgs {
   int gate_fl;       // resid32, xupdate: the flag that gates them
   int norm_only;     // resid32: only the partials of ||b − A x||²
   int ovf_fl;        // finish: the flag that tells an unfinished solve (ACT[act_slot] when 0)
+  // D as a palette (pap): byte k of Dcode[node] indexes Dpal[k] (FP64) / Dpal32[k] (Float32); null
+  // without (each component of D takes at most DPAL_MAX values: the bar's σn and the padding's σvac)
+  const uint32_t *Dcode;
+  const double *Dpal;
+  const float *Dpal32;
 };
+constexpr int DPAL_MAX = 16;  // values per component of D in the palette (held in LDS by the operator)
 int pcg_box_blocks(const Geo &g, int kc);
 int box_kc(const Geo &g);
 hipError_t launch_pcg_start(const PcgArgs &a, hipStream_t s);    // nb, warm start or zero source

@@ -2737,10 +2737,192 @@ static hipError_t zmodes_q_g(const ZModesQ &m, const int *gate, hipStream_t s) {
   return hipGetLastError();
 }
 
+// The z pass and projector of k_zmodes_q on v_mfma_f32_16x16x4f32.  A block of 4 waves owns 64
+// modes of one parity quarter (the columns, flattened along mm as in k_zmodes_q); wave w takes the
+// columns 16 w .. 16 w + 15.  With NZP = 16 NT >= nz (zero rows and columns beyond nz):
+//   forward  X~(mz, col) = Σ_n V(n, mz) X(n, col)  (and Y with V, Z with U(n, mz − 1)),
+//   inverse  x(n, col)   = Σ_mz V(n, mz) X~(mz, col),
+// each a 16 x 16 tile per (row tile, this wave's columns) with A the basis and B the columns, both
+// from LDS with k contiguous: the instruction u of a 16-k group g takes k = 16 g + 4 lk + u (lane
+// l = 16 lk + lr), so one 16-byte LDS read gives a lane its four k-steps.  The forward tiles of X,
+// Y and Z of the same (mz, col) sit in the same lane and register, where the projector runs; the
+// projected modes go back to LDS (over the columns) as the inverse's B.  LDS rows are NZP + 4 words
+// (4 mod 8: 16 lanes' 16-byte reads are bank-conflict free).  The sums are grouped four products at
+// a time (as the plane kernels'), not in k_zmodes_q's order: the results differ from it by Float32
+// rounding.
+template <int NT>
+__global__ void __launch_bounds__(256) k_zmodes_mf(ZModesQ m, const int *gate) {
+  if (gate && !*gate) return;
+  typedef v4f V4;
+  constexpr int NZP = 16 * NT, S = NZP + 4;
+  extern __shared__ __align__(16) unsigned char zm_smem[];
+  float *B4 = (float *)zm_smem;              // 4 bases [row][k]: Vt (mz, n), Ut (mz, n), V (n, mz), U (n, mz)
+  float *Ct = B4 + 4 * NZP * S;              // [comp][col][k]: the columns by n, later the modes by mz
+  const int nx = m.nx, ny = m.ny, nz = m.nz, tid = threadIdx.x;
+  const Quarters &q = m.q;
+  const int px = (int)(blockIdx.z >> 1), py = (int)(blockIdx.z & 1);
+  // the bases, zero beyond nz (the padded copies: [0] V(n, mz), [1] U(n, mz − 1), [2], [3] by columns)
+  for (int e = tid; e < 4 * NZP * NZP; e += 256) {
+    const int b = e / (NZP * NZP), r = (e / NZP) % NZP, k = e % NZP;
+    // b 0: Vt[mz][n] = zb[2][mz nz + n]; 1: Ut = zb[3]; 2: V[n][mz] = zb[0][n nz + mz]; 3: U = zb[1]
+    const float *src = m.zb[b == 0 ? 2 : b == 1 ? 3 : b == 2 ? 0 : 1];
+    B4[b * NZP * S + r * S + k] = (r < nz && k < nz) ? src[(size_t)r * nz + k] : 0.0f;
+  }
+  // the columns: thread (col, g4) loads n = g4, g4 + 4, ... of X, Y, Z of its column
+  const int col = tid & 63, g4 = tid >> 6;
+  {
+    const int f = blockIdx.x * 64 + col;
+    const int mm = f % q.h[2][0], qq = f / q.h[2][0];
+    const int mx = 2 * mm + px, my = 2 * qq + py;
+    const bool live = qq < q.h[2][1] && mx < nx && my < ny;
+    const bool hx = live && mx > 0, hy = live && my > 0;
+    const int xi = min(max(mm - 1 + px, 0), q.h[0][0] - 1), xj = min(qq, q.h[0][1] - 1);
+    const int yi = min(mm, q.h[1][0] - 1), yj = min(max(qq - 1 + py, 0), q.h[1][1] - 1);
+    const int zi = min(mm, q.h[2][0] - 1), zj = min(qq, q.h[2][1] - 1);
+    const float *colx = m.modes + q.off[0][2 * (1 - px) + py] + xi + (size_t)q.h[0][0] * xj;
+    const float *coly = m.modes + q.off[1][2 * px + (1 - py)] + yi + (size_t)q.h[1][0] * yj;
+    const float *colz = m.modes + q.off[2][2 * px + py] + zi + (size_t)q.h[2][0] * zj;
+    const size_t stx = (size_t)q.h[0][0] * q.h[0][1], sty = (size_t)q.h[1][0] * q.h[1][1], stz = (size_t)q.h[2][0] * q.h[2][1];
+    for (int n = g4; n < NZP; n += 4) {
+      const int nn = min(n, nz - 1);
+      const float vx = colx[stx * nn], vy = coly[sty * nn], vz = colz[stz * min(n, nz - 2)];
+      Ct[(0 * 64 + col) * S + n] = (hx && n < nz) ? vx : 0.0f;
+      Ct[(1 * 64 + col) * S + n] = (hy && n < nz) ? vy : 0.0f;
+      Ct[(2 * 64 + col) * S + n] = (live && n < nz - 1) ? vz : 0.0f;
+    }
+  }
+  __syncthreads();
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, lr = l & 15, lk = l >> 4;
+  const float *Vt = B4, *Ut = B4 + NZP * S, *Vr = B4 + 2 * NZP * S, *Ur = B4 + 3 * NZP * S;
+  // forward: acc[c][t] is the tile of rows mz = 16 t .. 16 t + 15 and this wave's 16 columns
+  V4 acc[3][NT];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[c][t] = V4{0, 0, 0, 0};
+  const int cb = (16 * w + lr) * S + 4 * lk;  // this lane's column row in Ct, its k offset
+#pragma unroll
+  for (int g = 0; g < NT; ++g) {
+    V4 Bc[3], Av[NT], Au[NT];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Bc[c] = *(const V4 *)(Ct + c * 64 * S + cb + 16 * g);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      Av[t] = *(const V4 *)(Vt + (16 * t + lr) * S + 16 * g + 4 * lk);
+      Au[t] = *(const V4 *)(Ut + (16 * t + lr) * S + 16 * g + 4 * lk);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Av[t][u], Bc[0][u], acc[0][t], 0, 0, 0);
+        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Av[t][u], Bc[1][u], acc[1][t], 0, 0, 0);
+        acc[2][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Au[t][u], Bc[2][u], acc[2][t], 0, 0, 0);
+      }
+  }
+  // the projector on this lane's modes: element r of tile t is mz = 16 t + 4 lk + r, column 16 w + lr
+  {
+    const int f = blockIdx.x * 64 + 16 * w + lr;
+    const int mm = f % q.h[2][0], qq = f / q.h[2][0];
+    const int mx = 2 * mm + px, my = 2 * qq + py;
+    const bool live = qq < q.h[2][1] && mx < nx && my < ny;
+    const bool hx = live && mx > 0, hy = live && my > 0;
+    const float k1 = live ? m.sx[mx] : 0.0f, k2 = live ? m.sy[my] : 0.0f;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int mz = 16 * t + 4 * lk + r;
+        if (mz >= nz) continue;
+        const float k3 = m.sz[mz];
+        const bool hz = mz > 0;
+        float x1 = hx ? acc[0][t][r] : 0.0f, y1 = hy ? acc[1][t][r] : 0.0f, z1 = (live && hz) ? acc[2][t][r] : 0.0f;
+        const float k2s = k1 * k1 + k2 * k2 + k3 * k3;
+        if (live && k2s != 0) project<float>(k1, k2, k3, (float)m.a, x1, y1, z1, hx, hy, hz);
+        acc[0][t][r] = x1;
+        acc[1][t][r] = y1;
+        acc[2][t][r] = z1;
+      }
+  }
+  __syncthreads();  // every wave is done with the columns: the modes go over them
+  // this lane's four rows mz = 16 t + 4 lk .. + 3 of its column, one 16-byte write per tile
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) *(V4 *)(Ct + c * 64 * S + (16 * w + lr) * S + 16 * t + 4 * lk) = acc[c][t];
+  __syncthreads();
+  // inverse: acc[c][t] is the tile of rows n = 16 t .. 16 t + 15
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[c][t] = V4{0, 0, 0, 0};
+#pragma unroll
+  for (int g = 0; g < NT; ++g) {
+    V4 Bc[3], Av[NT], Au[NT];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Bc[c] = *(const V4 *)(Ct + c * 64 * S + cb + 16 * g);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      Av[t] = *(const V4 *)(Vr + (16 * t + lr) * S + 16 * g + 4 * lk);
+      Au[t] = *(const V4 *)(Ur + (16 * t + lr) * S + 16 * g + 4 * lk);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Av[t][u], Bc[0][u], acc[0][t], 0, 0, 0);
+        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Av[t][u], Bc[1][u], acc[1][t], 0, 0, 0);
+        acc[2][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Au[t][u], Bc[2][u], acc[2][t], 0, 0, 0);
+      }
+  }
+  // the positions back into the quarters: element r of tile t is n = 16 t + 4 lk + r of column
+  // 16 w + lr (lanes 16 consecutive columns: runs along mm)
+  {
+    const int f = blockIdx.x * 64 + 16 * w + lr;
+    const int mm = f % q.h[2][0], qq = f / q.h[2][0];
+    const int mx = 2 * mm + px, my = 2 * qq + py;
+    const bool live = qq < q.h[2][1] && mx < nx && my < ny;
+    if (!live) return;
+    const bool hx = mx > 0, hy = my > 0;
+    const int xi = min(max(mm - 1 + px, 0), q.h[0][0] - 1), xj = min(qq, q.h[0][1] - 1);
+    const int yi = min(mm, q.h[1][0] - 1), yj = min(max(qq - 1 + py, 0), q.h[1][1] - 1);
+    const int zi = min(mm, q.h[2][0] - 1), zj = min(qq, q.h[2][1] - 1);
+    float *colx = m.modes + q.off[0][2 * (1 - px) + py] + xi + (size_t)q.h[0][0] * xj;
+    float *coly = m.modes + q.off[1][2 * px + (1 - py)] + yi + (size_t)q.h[1][0] * yj;
+    float *colz = m.modes + q.off[2][2 * px + py] + zi + (size_t)q.h[2][0] * zj;
+    const size_t stx = (size_t)q.h[0][0] * q.h[0][1], sty = (size_t)q.h[1][0] * q.h[1][1], stz = (size_t)q.h[2][0] * q.h[2][1];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = 16 * t + 4 * lk + r;
+        if (n >= nz) continue;
+        if (hx) colx[stx * n] = acc[0][t][r];
+        if (hy) coly[sty * n] = acc[1][t][r];
+        if (n < nz - 1) colz[stz * n] = acc[2][t][r];
+      }
+  }
+}
+
+static hipError_t launch_zmodes_mf(const ZModesQ &m, const int *gate, hipStream_t s) {
+  const int NT = (m.nz + 15) / 16;
+  const dim3 grid(cdiv(m.q.h[2][0] * m.q.h[2][1], 64), 1, 4), blk(256);
+  const size_t lds = (size_t)(4 * 16 * NT + 3 * 64) * (16 * NT + 4) * sizeof(float);
+  switch (NT) {
+    case 1: TDGL_LAUNCH((k_zmodes_mf<1>), grid, blk, lds, s, m, gate); break;
+    case 2: TDGL_LAUNCH((k_zmodes_mf<2>), grid, blk, lds, s, m, gate); break;
+    case 3: TDGL_LAUNCH((k_zmodes_mf<3>), grid, blk, lds, s, m, gate); break;
+    case 4: TDGL_LAUNCH((k_zmodes_mf<4>), grid, blk, lds, s, m, gate); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_zmodes_q(const ZModesQ &m, const int *gate, hipStream_t s) {
   if (g_rec) return hipErrorNotSupported;
   if (m.nz < 2 || m.nz > ZQ_NZ_MAX || m.nx < 1 || m.ny < 1) return hipErrorInvalidValue;
-  const int waves = m.waves ? m.waves : 4;
+  if (m.waves == 0) return launch_zmodes_mf(m, gate, s);
+  const int waves = m.waves;
   if (waves == 8) return zmodes_q_g<8>(m, gate, s);
   if (waves == 4) return zmodes_q_g<4>(m, gate, s);
   return hipErrorInvalidValue;
@@ -3292,12 +3474,33 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap_tile(PcgArgs a) {
   stash(kf + 1, betaT);
   const int tx = (tid & 63) + 1, ty = (tid >> 6) + 1;
   auto L = [&](int comp, int kk, int row, int col) { return Pl[comp][kk % 3][row * PT_W + col]; };
+  // D of the node: the palette in LDS and the level's code requested a level ahead, or the three
+  // arrays a level ahead
+  __shared__ T palL[3 * DPAL_MAX];
+  const bool pal = a.Dcode != nullptr;
+  if (pal && tid < 3 * DPAL_MAX) palL[tid] = F32 ? (T)a.Dpal32[tid] : (T)a.Dpal[tid];
+  const int cnode = box_load_c(a.g, b0), slev = a.g.px * a.g.py;
+  uint32_t code_n = pal ? a.Dcode[cnode] : 0u;
+  T dn[3] = {T(0), T(0), T(0)};
+  if (!pal)
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) dn[comp] = Dc[comp][cnode];
   double s = 0.0;
   for (int q = 0; q < b0.nk; ++q) {
     const int k = kf + q;
     __syncthreads();
     const bool more = q + 1 < b0.nk;
     if (more) fetch(k + 2);
+    const uint32_t code = code_n;
+    const T dcur[3] = {dn[0], dn[1], dn[2]};
+    if (more) {
+      const int cn = cnode + slev * (q + 1);
+      if (pal)
+        code_n = a.Dcode[cn];
+      else
+#pragma unroll
+        for (int comp = 0; comp < 3; ++comp) dn[comp] = Dc[comp][cn];
+    }
     if (live) {
       const BoxIdx b = box_level(a.g, b0, q);
       T v[NSTEN], qv[3];
@@ -3310,7 +3513,10 @@ __global__ void __launch_bounds__(RED_T) k_pcg_pap_tile(PcgArgs a) {
       v[P2C] = L(2, k, ty, tx), v[P2YP] = L(2, k, ty + 1, tx), v[P2ZMYP] = L(2, k - 1, ty + 1, tx);
       v[P2ZM] = L(2, k - 1, ty, tx), v[P2YM] = L(2, k, ty - 1, tx), v[P2XP] = L(2, k, ty, tx + 1);
       v[P2ZMXP] = L(2, k - 1, ty, tx + 1), v[P2XM] = L(2, k, ty, tx - 1);
-      const T dv[3] = {Dc[0][b.c], Dc[1][b.c], Dc[2][b.c]};
+      T dv[3];
+#pragma unroll
+      for (int comp = 0; comp < 3; ++comp)
+        dv[comp] = pal ? palL[DPAL_MAX * comp + (int)((code >> (8 * comp)) & 255u)] : dcur[comp];
       apply3v<T>(a, v, dv, b.i, b.j, b.k, qv);
       const int cc[3] = {P0C, P1C, P2C};
       for (int comp = 0; comp < 3; ++comp) {

@@ -154,7 +154,7 @@ struct tdgl_handle {
   float *Hp[2][2][2][2] = {};  // Hq zero-padded to whole 16 x 16 tiles
   int HpL[2][2] = {};          // their unpadded order
   int qplane = 1;              // TDGL_HIP_QPLANE=0: the quarter x-y transforms on k_plane_mf1 / k_plane_mf
-  int zq_waves = 0;        // waves per block of the quarter z pass (0: the library's choice)
+  int zq_waves = 8;        // the quarter z pass: 4 or 8 waves a block on the vector ALUs, 0 on the matrix cores (k_zmodes_mf)
   float *zqb[4] = {};      // its padded z bases (ZQ_BASES)
   bool tx_only = false;                   // the separable solve as single-axis passes (a mixed batch)
   int box_kc = 1;                         // levels of k per block of the PCG box kernels
@@ -201,6 +201,9 @@ struct tdgl_handle {
   int paptile = 1;         // TDGL_HIP_PAPTILE=0: the PCG operator with a gather per stencil value (k_pcg_pap)
   int boxxcd = 1;          // TDGL_HIP_BOXXCD=0: the box kernels' blocks in the hardware's order
   int zswap = 1;           // TDGL_HIP_ZSWAP=0: the folded PCG copies z into zold instead of alternating them
+  uint32_t *Dcode = nullptr;   // D as a palette for the PCG operator (TDGL_HIP_DPAL=0: off)
+  double *Dpal = nullptr;
+  float *Dpal32 = nullptr;
   double *bk[48] = {};     // device copy of a cycle's start (roll back)
   // thermal noise: the stream's step counter on the device ([0] steps taken, [1] the step of the
   // first of the caller's draws) and its host copy; the caller's draws; SK-ROCK's δθ and source;
@@ -1039,6 +1042,9 @@ PcgArgs pcg_args(tdgl_handle *h) {
   a.zcopy = 1;
   a.xcd = h->boxxcd;
   a.f = h->f32v;
+  a.Dcode = h->Dcode;
+  a.Dpal = h->Dpal;
+  a.Dpal32 = h->Dpal32;
   for (int c = 0; c < 3; ++c) {
     a.D[c] = h->D[c];
     a.th[c] = h->th[c];
@@ -2191,9 +2197,10 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     h->zcols = zc && zc[0] == '1' ? 1 : 0;
     const char *qp = std::getenv("TDGL_HIP_QPLANE");
     h->qplane = qp && qp[0] == '0' ? 0 : 1;
-    // TDGL_HIP_ZQ_WAVES=4 or 8: the waves per block of the quarter z pass
+    // TDGL_HIP_ZQ_WAVES=4 or 8: the quarter z pass on the vector ALUs with that many waves a block
+    // (8 by default), 0: on the matrix cores
     const char *zw = std::getenv("TDGL_HIP_ZQ_WAVES");
-    if (zw && (std::atoi(zw) == 4 || std::atoi(zw) == 8)) h->zq_waves = std::atoi(zw);
+    if (zw && (std::atoi(zw) == 0 || std::atoi(zw) == 4 || std::atoi(zw) == 8)) h->zq_waves = std::atoi(zw);
     const char *g1 = std::getenv("TDGL_HIP_PLANE1");
     if (g1 && (g1[0] == '0' || g1[0] == '2' || g1[0] == '3')) h->plane1 = g1[0] - '0';
     const char *gp = std::getenv("TDGL_HIP_PLANE");
@@ -2283,6 +2290,39 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
           }
         }
         h->fold = true;
+      }
+    }
+    if (d.field_mode == TDGL_FIELD_PCG && !(std::getenv("TDGL_HIP_DPAL") && std::getenv("TDGL_HIP_DPAL")[0] == '0')) {
+      // D as a palette when each component takes at most DPAL_MAX values: one 32-bit code per
+      // node instead of three doubles in the PCG operator (the same values, so the same bits)
+      std::vector<double> pal((size_t)3 * DPAL_MAX, 0.0);
+      std::vector<uint32_t> code((size_t)n, 0u);
+      bool fits = true;
+      for (int c = 0; c < 3 && fits; ++c) {
+        std::vector<double> dd((size_t)n);
+        CHK(hcopy(h, dd.data(), h->D[c], (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<double> vals;
+        for (size_t e = 0; e < (size_t)n && fits; ++e) {
+          auto it = std::find(vals.begin(), vals.end(), dd[e]);
+          size_t ix = (size_t)(it - vals.begin());
+          if (it == vals.end()) {
+            if (vals.size() == (size_t)DPAL_MAX) {
+              fits = false;
+              break;
+            }
+            vals.push_back(dd[e]);
+          }
+          code[e] |= (uint32_t)ix << (8 * c);
+        }
+        for (size_t k = 0; k < vals.size(); ++k) pal[(size_t)DPAL_MAX * c + k] = vals[k];
+      }
+      if (fits) {
+        ALLOC(h->Dcode, n);
+        CHK(upload_vec(h, h->Dcode, code));
+        ALLOC(h->Dpal, pal.size());
+        CHK(upload_vec(h, h->Dpal, pal));
+        ALLOC(h->Dpal32, pal.size());
+        CHK(upload_vec(h, h->Dpal32, std::vector<float>(pal.begin(), pal.end())));
       }
     }
     if (d.field_mode == TDGL_FIELD_PCG && d.pcg_refine) {
@@ -2643,7 +2683,8 @@ int tdgl_autotune(tdgl_handle *h, double H, double HJ, tdgl_tuning *out) {
   int best_zw = zw0;
   if (h->fold) {
     best = 1e300;
-    for (int zw = 4; zw <= 8 && rc == 0; zw *= 2) {
+    for (int zw : {8, 4, 0}) {
+      if (rc) break;
       h->zq_waves = zw;
       double us = 0;
       rc = trial(TDGL_TUNE_ZMODES_WAVES, zw, K_MODES, -1, &us);

@@ -389,7 +389,7 @@ def test_pcg_budget_settles_on_a_current_ramp():
 
 
 VARIANTS = [("TDGL_HIP_ZCOL", "1", "0"), ("TDGL_HIP_PAPTILE", "0", "1"), ("TDGL_HIP_ZQ_WAVES", "8", "4"),
-            ("TDGL_HIP_BOXXCD", "0", "1"), ("TDGL_HIP_ZSWAP", "0", "1")]
+            ("TDGL_HIP_BOXXCD", "0", "1"), ("TDGL_HIP_ZSWAP", "0", "1"), ("TDGL_HIP_DPAL", "0", "1")]
 
 
 @pytest.mark.parametrize("variant", VARIANTS, ids=lambda v: v[0])
@@ -401,8 +401,9 @@ def test_kernel_variants_give_the_same_bits(name, variant, monkeypatch):
     PCG operator (k_pcg_pap_tile) against one gather per stencil value (TDGL_HIP_PAPTILE=0), and the
     quarter z pass (k_zmodes_q) with 8 against 4 waves a block (each mode's sums in the same order),
     the box kernels' blocks in XCD runs against the hardware's order (each block's partial in the
-    same slot), and the folded PCG's alternating z and zold against the copy.  Eight steps give the
-    same bits either way."""
+    same slot), the folded PCG's alternating z and zold against the copy, and the operator's D read
+    through a palette of its values against the three arrays.  Eight steps give the same bits either
+    way."""
     HipStepper = _hip()
     st, _ = make_case(name)
     s0, pcg, H, HJ = seed_state(st)
@@ -420,6 +421,30 @@ def test_kernel_variants_give_the_same_bits(name, variant, monkeypatch):
         dev.close()
     for (k, a), (_, b) in zip(outs[0].arrays(), outs[1].arrays()):
         assert np.array_equal(a, b), k
+
+
+@pytest.mark.parametrize("name", ["pcg_fast_f32", "kwt10_pcg", "pcg_mf1_f32", "open_kwt_pcg"])
+def test_matrix_core_z_pass_matches_the_vector_one(name, monkeypatch):
+    """The folded preconditioner's z pass on the matrix cores (k_zmodes_mf, TDGL_HIP_ZQ_WAVES=0) against
+    the vector ALUs' (k_zmodes_q): its sums are grouped four products at a time, so the two agree to
+    the PCG tolerance, not to the bit; both against the oracle's tolerance of the case."""
+    HipStepper = _hip()
+    st, tol = make_case(name)
+    s0, pcg, H, HJ = seed_state(st)
+    outs = []
+    for v in ("8", "0"):
+        print(f"[gpu] TDGL_HIP_ZQ_WAVES={v}: {name}", flush=True)
+        monkeypatch.setenv("TDGL_HIP_ZQ_WAVES", v)
+        dev = HipStepper(st)
+        dev.upload(s0)
+        dev.set_pcg(pcg)
+        dev.step(12, H, HJ)
+        outs.append(dev.download())
+        dev.close()
+    errs = compare_states(outs[1], outs[0])
+    core = {k: e for k, e in errs.items() if not k.startswith(("U", "V"))}
+    print(f"    worst {max(core, key=core.get)} {max(core.values()):.2e}", flush=True)
+    assert max(core.values()) <= tol, errs
 
 
 @pytest.mark.parametrize("name", ["pcg_mf1_f32", "kwt10_mf1_pcg"])
