@@ -185,7 +185,11 @@ def bench_sweep(args) -> int:
     print(f"[bench] batched kernels (one launch per kernel over all runs): {t_kb:.2f} s", file=sys.stderr, flush=True)
     kb16, t_kb16 = sweep.run_batched(specs, work, group=16, workers=args.workers)
     print(f"[bench] batched kernels, groups of 16 set up while the last runs: {t_kb16:.2f} s", file=sys.stderr, flush=True)
-    same = all(seq[q] == bat[q] == kb[q] == kb16[q] for q in seq)
+    kbl, t_kbl = sweep.run_batched(specs, work, group=16, workers=args.workers, lanes=4)
+    print(f"[bench] batched kernels, four groups of 16 at once: {t_kbl:.2f} s", file=sys.stderr, flush=True)
+    kbl8, t_kbl8 = sweep.run_batched(specs, work, group=8, workers=args.workers, lanes=4)
+    print(f"[bench] batched kernels, groups of 8, four at once: {t_kbl8:.2f} s", file=sys.stderr, flush=True)
+    same = all(seq[q] == bat[q] == kb[q] == kb16[q] == kbl[q] == kbl8[q] for q in seq)
     rate = len(specs) / t_bat
     srv = sweep.Server(budget_bytes=200e9, slots=args.workers)
     for s, t in zip(specs, sweep.poisson_arrivals(len(specs), rate=0.9 * rate, seed=5)):
@@ -206,6 +210,8 @@ def bench_sweep(args) -> int:
                        "sequential_runs_per_hr": 3600.0 * len(specs) / t_seq, "speedup": t_seq / t_bat,
                        "batched_kernels_runs_per_hr": 3600.0 * len(specs) / t_kb,
                        "batched_kernels_groups16_runs_per_hr": 3600.0 * len(specs) / t_kb16,
+                       "batched_kernels_4x16_at_once_runs_per_hr": 3600.0 * len(specs) / t_kbl,
+                       "batched_kernels_4x8_at_once_runs_per_hr": 3600.0 * len(specs) / t_kbl8,
                        "cell_cycles_per_s_batched": cells * args.steps / t_bat,
                        "same_alone_and_in_the_mix": bool(same and same_srv),
                        "served_p50_latency_s": lat["p50"], "served_p99_latency_s": lat["p99"],

@@ -302,13 +302,15 @@ struct Quarters {
   int off[3][4];
   int total;
 };
-// fold: the three components of a node-shaped FP64 vector into the quarters (FP32); the
-// convergence check of a preconditioner's first pass is the forward x-y launch's
+// fold: the three components of a node-shaped FP64 vector into the quarters (FP32, or FP64 into
+// out64 where it is set: the FP64 separable solve); the convergence check of a preconditioner's
+// first pass is the forward x-y launch's
 struct FoldPass {
   View in[3];
   float *out;
   Quarters q;
   int f32;         // the input is Float32 (an inner solve's residual), else FP64
+  double *out64;   // FP64 quarters (then out is unused)
 };
 hipError_t launch_fold(const FoldPass &f, const int *gate, hipStream_t s);
 // The check of a refinement round and the fold of its Float32 residual: from the partials of
@@ -331,6 +333,9 @@ struct UnfoldPass {
   int dot;
   int f32;         // out, r and zold are Float32 (an inner solve), else FP64
   PcgDev d;
+  const double *in64;  // FP64 quarters (the FP64 separable solve; then in is unused)
+  int addth;           // out = th + the value (φ = θ + the solve)
+  View th[3];
 };
 hipError_t launch_unfold(const UnfoldPass &u, const int *gate, hipStream_t s);
 int fold_blocks(const Quarters &q);
@@ -345,6 +350,17 @@ struct ZModesQ {
   int waves;             // k_zmodes_q: waves per block, each a group of z modes (4 or 8); 0: k_zmodes_mf (matrix cores)
 };
 hipError_t launch_zmodes_q(const ZModesQ &m, const int *gate, hipStream_t s);
+// the same in FP64 (the FP64 separable solve): the bases by rows and columns as ZQ_BASES, double
+struct ZModesQ64 {
+  double *modes;
+  Quarters q;
+  const double *zb[4];
+  const double *sx, *sy, *sz;
+  int nx, ny, nz;
+  double a;
+  int waves;             // 4 or 8
+};
+hipError_t launch_zmodes_q64(const ZModesQ64 &m, hipStream_t s);
 
 // The x-y transforms of the twelve quarters (three components by four parities) in one launch:
 // out(m, q) = Σ_i Σ_j X(i, m) in(i, j) Y(j, q) per z plane, with the half-size bases given
@@ -352,17 +368,25 @@ hipError_t launch_zmodes_q(const ZModesQ &m, const int *gate, hipStream_t s);
 // Y(j, q), QS = 16 ceil(L/16) for a basis of L = e rows.  A block per (quarter, plane), a
 // wavefront per 16 rows of m.  x extents up to 16 QP_MT, y extents up to 16 QP_JT.
 constexpr int QP_MT = 10, QP_JT = 6;
-struct QPass {
+// T = float: the Float32 preconditioner's quarters; T = double: the FP64 separable solve's
+template <class T>
+struct QPassT {
   View in[12], out[12];
-  const float *XT[12], *YT[12];
+  const T *XT[12], *YT[12];
   int e[12][3];
   Conv conv;
   PcgDev d;
+  int e2max, mtx, jtx;  // the launch's planes and largest tiles (set by the launcher)
 };
+typedef QPassT<float> QPass;
+typedef QPassT<double> QPass64;
 bool qplane_fits(const QPass &p);
 int qplane_parts(const QPass &p);
 hipError_t launch_qplane(const QPass &p, const int *gate, hipStream_t s);
+bool qplane64_fits(const QPass64 &p);
+hipError_t launch_qplane64(const QPass64 &p, hipStream_t s);
 constexpr int ZQ_NZ_MAX = 64;  // z extents the quarter z pass takes (16 modes a wave at 4 waves a block)
+constexpr int ZQ64_NZ_MAX = 48;  // in FP64, with its bases in LDS beside the columns (3 nz 64 + 4 (nz² + 64) doubles)
 // The quarter z pass's bases, float, each nz x nz followed by ZQ_SLACK zeros: [0] V(n, mz) at
 // n nz + mz, [1] U(n, mz - 1) at n nz + mz (zero for mz = 0 or n = nz - 1), [2] and [3] the same at
 // mz nz + n.  The slack covers the reads of the last group past nz.
@@ -582,7 +606,8 @@ hipError_t launch_meter_finish(const MeterArgs &a, double *rec, double *rung, hi
 // One launch runs a kernel over the blocks of many runs: run r owns the blocks off[r] ..
 // off[r+1]-1 of a 1D grid, which it takes as its own grid of dims[3r] x dims[3r+1] x dims[3r+2]
 // blocks (the grid its single-run launch would have), with its own Geo and argument struct.
-enum { BK_STAGE = 0, BK_FIELD, BK_TX, BK_TX2, BK_CURL, BK_GHOSTS, BK_METER, BK_METERFIN, BK_PLANE, BK_ZMODES, BK_N };
+enum { BK_STAGE = 0, BK_FIELD, BK_TX, BK_TX2, BK_CURL, BK_GHOSTS, BK_METER, BK_METERFIN, BK_PLANE, BK_ZMODES, BK_FOLD, BK_UNFOLD,
+       BK_ZMODESQ, BK_QPLANE, BK_N };
 struct BatchTab {
   const int *off;    // n + 1 block offsets
   const int *dims;   // 3 per run

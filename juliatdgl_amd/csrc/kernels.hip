@@ -1847,6 +1847,14 @@ static dim3 plane_grid(const PlanePass &p) {
 struct Plane1Pick {
   int use, xc, tj, mt, gx, gy, nc;
 };
+// the instances of the one-round kernel: (2..5) x (2, 3) x (2, 3) and (5, 3, 1) in both types, and
+// in FP64 also those of one 32-row chunk along x or y (TDGL_PLANE1_SMALL)
+static bool plane1_has(int xc, int tj, int mt) {
+  if (xc >= 2 && xc <= 5 && tj >= 2 && tj <= 3 && mt >= 2 && mt <= 3) return true;
+  if (xc == 5 && tj == 3 && mt == 1) return true;
+  if (xc == 1 && tj >= 1 && tj <= 3 && mt == 2) return true;
+  return xc >= 2 && xc <= 5 && tj == 1 && mt >= 2 && mt <= 3;
+}
 static Plane1Pick plane1_pick(const PlanePass &p, int es) {
   Plane1Pick k{};
   k.nc = plane_ncomp(p);
@@ -1859,8 +1867,10 @@ static Plane1Pick plane1_pick(const PlanePass &p, int es) {
   }
   bool tr = true;
   for (int c = 0; c < k.nc; ++c) tr = tr && p.XT[c] && p.YT[c];
-  // FP64 planes from 33 rows (the FP32 ones from 65: below, the chunked kernel was faster)
-  k.use = p.one != 0 && tr && l0 > (es == 8 ? 32 : 64) && l0 <= 32 * PLANE1_XC && l1 > 32 && l1 <= 32 * PLANE1_TJ;
+  // FP64 planes of any extent up to 32 PLANE1_XC x 32 PLANE1_TJ (the parity quarters of the FP64
+  // separable solve are mostly below 33); FP32 ones from 65 x 33 (below, the chunked kernel was faster)
+  k.use = p.one != 0 && tr && l0 > (es == 8 ? 0 : 64) && l0 <= 32 * PLANE1_XC && l1 > (es == 8 ? 0 : 32) &&
+          l1 <= 32 * PLANE1_TJ;
   if (!k.use) return k;
   k.xc = cdiv(l0, 32);
   k.tj = cdiv(l1, 32);
@@ -1870,6 +1880,7 @@ static Plane1Pick plane1_pick(const PlanePass &p, int es) {
   // FP64 planes of up to 160 x 96: one m tile a block where two do not fit the LDS
   if (plane1_lds(k.xc, k.tj, k.mt, es) > PLANE1_LDS_MAX && p.one < 2 && es == 8) k.mt = 1;
   if (plane1_lds(k.xc, k.tj, k.mt, es) > PLANE1_LDS_MAX) k.use = 0;
+  if (!plane1_has(k.xc, k.tj, k.mt)) k.use = 0;
   k.gx = cdiv(l0, 16 * k.mt);
   return k;
 }
@@ -1890,6 +1901,10 @@ bool plane_fits(const int e[][3], int ncomp) {
   return (sizeof(T) == 4 ? plane_mf_lds(l1) : plane_lds<T>(l1)) <= 64 * 1024;
 }
 
+// the FP64 instances of one 32-row chunk along x or y
+#define TDGL_PLANE1_SMALL(C) \
+  C(1, 1, 2) C(1, 2, 2) C(1, 3, 2) C(2, 1, 2) C(2, 1, 3) C(3, 1, 2) C(3, 1, 3) C(4, 1, 2) C(4, 1, 3) C(5, 1, 2) C(5, 1, 3)
+
 template <class TI, class T, class TO>
 hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
   const int nc = plane_ncomp(p);
@@ -1901,7 +1916,8 @@ hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
   if (g_rec) {
     // the FP64 passes of the separable solve: the instance this launch would take
     constexpr bool f64 = sizeof(TI) == 8 && sizeof(T) == 8 && sizeof(TO) == 8;
-    if (!f64 || gate || p.conv.mode != CONV_NONE || p.dot || nc != 3) return hipErrorNotSupported;
+    if (!f64 || gate || p.conv.mode != CONV_NONE || p.dot || (nc != 3 && nc != 12) || (nc == 12 && p.addth))
+      return hipErrorNotSupported;
     const Plane1Pick k = plane1_pick(p, 8);
     if (k.use)
       return record(BK_PLANE, Geo{}, &p, sizeof p, 1000 + (k.xc * 4 + k.tj) * 4 + k.mt, true, dim3(k.gx, k.gy, k.nc),
@@ -1925,8 +1941,16 @@ hipError_t launch_plane(const PlanePass &p, const int *gate, hipStream_t s) {
         TDGL_PLANE1_CASE(4, 2, 2) TDGL_PLANE1_CASE(4, 3, 2) TDGL_PLANE1_CASE(4, 2, 3) TDGL_PLANE1_CASE(4, 3, 3)
         TDGL_PLANE1_CASE(5, 2, 2) TDGL_PLANE1_CASE(5, 3, 2) TDGL_PLANE1_CASE(5, 2, 3) TDGL_PLANE1_CASE(5, 3, 3)
         TDGL_PLANE1_CASE(5, 3, 1)
+        default:
+          if constexpr (sizeof(T) == 8) {
+            switch ((k.xc * 4 + k.tj) * 4 + k.mt) {
+              TDGL_PLANE1_SMALL(TDGL_PLANE1_CASE)
+              default: return hipErrorInvalidValue;
+            }
+          } else {
+            return hipErrorInvalidValue;
+          }
 #undef TDGL_PLANE1_CASE
-        default: return hipErrorInvalidValue;
       }
       return hipGetLastError();
     }
@@ -1970,11 +1994,12 @@ template hipError_t launch_plane<float, float, float>(const PlanePass &, const i
 // lane k index lk: k = 16 g + 4 lk + u), and the padding is zero in both: the same bits.
 // The next group's global operands are requested while the current group multiplies.
 // the two products of k_qplane for one quarter plane held in LDS (Bt); FULL: the quarter has the
-// launch's JT tiles along j, so no tile is guarded
-template <int JT, bool FULL>
-__device__ __forceinline__ void qplane_products(const QPass &p, int comp, int plane, int tid, int mt, int jt, int e0,
-                                                int e1, int E0S, int JS, int xs, int ys, float *Bt, float *St) {
-  typedef v4f V4;
+// launch's JT tiles along j, so no tile is guarded.  Mf<T> gives the instruction of each type and
+// where its results lie (the FP64 accumulator's element r of lane l is row 4 r + l / 16).
+template <class T, int JT, bool FULL>
+__device__ __forceinline__ void qplane_products(const QPassT<T> &p, int comp, int plane, int tid, int mt, int jt, int e0,
+                                                int e1, int E0S, int JS, int xs, int ys, T *Bt, T *St) {
+  typedef typename Mf<T>::v4 V4;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, lr = l & 15, lk = l >> 4;
   const bool on = w < mt;
   V4 acc[JT];
@@ -1982,19 +2007,19 @@ __device__ __forceinline__ void qplane_products(const QPass &p, int comp, int pl
   for (int c = 0; c < JT; ++c) acc[c] = V4{0, 0, 0, 0};
   if (on) {
     // phase 1: A = X(16 g + 4 lk + u, 16 w + lr) from XT, B = in(16 g + 4 lk + u, 16 c + lr) from Bt
-    const float *xa = p.XT[comp] + (size_t)(16 * w + lr) * xs + 4 * lk;
-    V4 A = *(const V4 *)xa;
+    const T *xa = p.XT[comp] + (size_t)(16 * w + lr) * xs + 4 * lk;
+    V4 A = Mf<T>::ld4(xa);
     for (int g = 0; g < mt; ++g) {
-      const V4 An = *(const V4 *)(xa + 16 * min(g + 1, mt - 1));
+      const V4 An = Mf<T>::ld4(xa + 16 * min(g + 1, mt - 1));
       V4 B[JT];
 #pragma unroll
       for (int c = 0; c < JT; ++c)
-        B[c] = (FULL || c < jt) ? *(const V4 *)(Bt + (16 * c + lr) * E0S + 16 * g + 4 * lk) : V4{0, 0, 0, 0};
+        B[c] = (FULL || c < jt) ? Mf<T>::ld4(Bt + (16 * c + lr) * E0S + 16 * g + 4 * lk) : V4{0, 0, 0, 0};
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
         for (int c = 0; c < JT; ++c)
-          if ((FULL || c < jt)) acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[u], B[c][u], acc[c], 0, 0, 0);
+          if ((FULL || c < jt)) acc[c] = Mf<T>::mma(A[u], B[c][u], acc[c]);
       A = An;
     }
   }
@@ -2004,49 +2029,54 @@ __device__ __forceinline__ void qplane_products(const QPass &p, int comp, int pl
     for (int c = 0; c < JT; ++c)
       if ((FULL || c < jt))
 #pragma unroll
-        for (int r = 0; r < 4; ++r) St[(16 * w + 4 * lk + r) * JS + 16 * c + lr] = acc[c][r];
+        for (int r = 0; r < 4; ++r) St[(16 * w + Mf<T>::mrow(lk, r)) * JS + 16 * c + lr] = acc[c][r];
   __syncthreads();
   if (!on) return;
   // phase 2: A = Y(16 g + 4 lk + u, 16 c + lr) from YT, B = S(16 w + lr, 16 g + 4 lk + u) from St
 #pragma unroll
   for (int c = 0; c < JT; ++c) acc[c] = V4{0, 0, 0, 0};
-  const float *ya = p.YT[comp] + (size_t)lr * ys + 4 * lk;
+  const T *ya = p.YT[comp] + (size_t)lr * ys + 4 * lk;
   V4 Y[JT];
 #pragma unroll
-  for (int c = 0; c < JT; ++c) Y[c] = (FULL || c < jt) ? *(const V4 *)(ya + (size_t)16 * c * ys) : V4{0, 0, 0, 0};
+  for (int c = 0; c < JT; ++c) Y[c] = (FULL || c < jt) ? Mf<T>::ld4(ya + (size_t)16 * c * ys) : V4{0, 0, 0, 0};
   for (int g = 0; g < jt; ++g) {
     const int gn = 16 * min(g + 1, jt - 1);
     V4 Yn[JT];
 #pragma unroll
-    for (int c = 0; c < JT; ++c) Yn[c] = (FULL || c < jt) ? *(const V4 *)(ya + (size_t)16 * c * ys + gn) : V4{0, 0, 0, 0};
-    const V4 B = *(const V4 *)(St + (16 * w + lr) * JS + 16 * g + 4 * lk);
+    for (int c = 0; c < JT; ++c) Yn[c] = (FULL || c < jt) ? Mf<T>::ld4(ya + (size_t)16 * c * ys + gn) : V4{0, 0, 0, 0};
+    const V4 B = Mf<T>::ld4(St + (16 * w + lr) * JS + 16 * g + 4 * lk);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int c = 0; c < JT; ++c)
-        if ((FULL || c < jt)) acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(Y[c][u], B[u], acc[c], 0, 0, 0);
+        if ((FULL || c < jt)) acc[c] = Mf<T>::mma(Y[c][u], B[u], acc[c]);
 #pragma unroll
     for (int c = 0; c < JT; ++c) Y[c] = Yn[c];
   }
-  // out(q, m): element r of lane l is q = 16 c + 4 lk + r, m = 16 w + lr
+  // out(q, m): element r of lane l is q = 16 c + mrow(l / 16, r), m = 16 w + lr
   const View vo = p.out[comp];
-  float *dst = (float *)vo.p + (size_t)vo.s2 * plane;
+  T *dst = (T *)vo.p + (size_t)vo.s2 * plane;
   const int m = 16 * w + lr;
 #pragma unroll
   for (int c = 0; c < JT; ++c)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int q = 16 * c + 4 * lk + r;
+      const int q = 16 * c + Mf<T>::mrow(lk, r);
       if ((FULL || c < jt) && m < e0 && q < e1) dst[m + (size_t)vo.s1 * q] = acc[c][r];
     }
 }
 
-template <int JT>
-__global__ void __launch_bounds__(64 * QP_MT) k_qplane(QPass p, const int *gate, int e2max, int mtx, int jtx) {
+// LDS rows of 4 (mod 8) words: 4 floats or 2 doubles of padding
+template <class T> constexpr int qplane_pad() { return 16 / (int)sizeof(T); }
+
+// block `bid` of a k_qplane launch: (quarter, plane) = (bid / e2max, bid % e2max)
+template <class T, int JT>
+__device__ __forceinline__ void qplane_body(const QPassT<T> &p, const int *gate, int bid) {
   extern __shared__ __align__(16) unsigned char qplane_smem[];
   __shared__ double sh[16];
   const int tid = threadIdx.x, nth = blockDim.x;
-  const int bid = blockIdx.x, comp = bid / e2max, plane = bid % e2max;
+  const int e2max = p.e2max, mtx = p.mtx, jtx = p.jtx;
+  const int comp = bid / e2max, plane = bid % e2max;
   if (p.conv.mode != CONV_NONE) {
     if (!*p.conv.in) {
       if (bid == 0 && tid == 0) *p.conv.out = 0;
@@ -2066,19 +2096,19 @@ __global__ void __launch_bounds__(64 * QP_MT) k_qplane(QPass p, const int *gate,
   }
   if (plane >= e2) return;
   const int mt = (e0 + 15) / 16, jt = (e1 + 15) / 16;  // this quarter's tiles along m (and i), j (and q)
-  const int E0S = 16 * mtx + 4, JS = 16 * jtx + 4;     // LDS row strides (the launch's largest tiles)
+  const int E0S = 16 * mtx + qplane_pad<T>(), JS = 16 * jtx + qplane_pad<T>();  // LDS row strides
   const int xs = 16 * mt, ys = 16 * jt;                // the padded bases' row strides
-  float *Bt = (float *)qplane_smem, *St = Bt;
+  T *Bt = (T *)qplane_smem, *St = Bt;
   {
     // the plane, zero beyond e0 x e1 up to whole tiles; eight loads in flight per thread
-    const float *src = (const float *)p.in[comp].p + (size_t)p.in[comp].s2 * plane;
+    const T *src = (const T *)p.in[comp].p + (size_t)p.in[comp].s2 * plane;
     const int s1 = p.in[comp].s1, W = 16 * mt, n = W * 16 * jt;
     for (int t0 = 0; t0 < n; t0 += 8 * nth) {
-      float v[8];
+      T v[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int t = t0 + tid + u * nth, ii = t % W, j = t / W;
-        v[u] = (t < n && ii < e0 && j < e1) ? src[ii + (size_t)s1 * j] : 0.0f;
+        v[u] = (t < n && ii < e0 && j < e1) ? src[ii + (size_t)s1 * j] : T(0);
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -2089,12 +2119,32 @@ __global__ void __launch_bounds__(64 * QP_MT) k_qplane(QPass p, const int *gate,
   }
   __syncthreads();
   if (jt == JT)
-    qplane_products<JT, true>(p, comp, plane, tid, mt, jt, e0, e1, E0S, JS, xs, ys, Bt, St);
+    qplane_products<T, JT, true>(p, comp, plane, tid, mt, jt, e0, e1, E0S, JS, xs, ys, Bt, St);
   else
-    qplane_products<JT, false>(p, comp, plane, tid, mt, jt, e0, e1, E0S, JS, xs, ys, Bt, St);
+    qplane_products<T, JT, false>(p, comp, plane, tid, mt, jt, e0, e1, E0S, JS, xs, ys, Bt, St);
 }
 
-static void qplane_tiles(const QPass &p, int &mtx, int &jtx, int &e2max) {
+template <int JT>
+__global__ void __launch_bounds__(64 * QP_MT) k_qplane(QPass p, const int *gate) {
+  qplane_body<float, JT>(p, gate, (int)blockIdx.x);
+}
+// FP64: at most 8 waves (quarters up to 128 rows), so that the doubled operands of 4 j tiles fit
+// the registers without spilling
+constexpr int QP64_MT = 8;
+template <int JT>
+__global__ void __launch_bounds__(64 * QP64_MT) k_qplane64(QPass64 p) {
+  qplane_body<double, JT>(p, nullptr, (int)blockIdx.x);
+}
+// batched: run r's blocks are its (quarter, plane) blocks; the launch's block has the waves of
+// the largest m tile count among its runs (a run's surplus waves stage the plane and wait)
+template <int JT>
+__global__ void __launch_bounds__(64 * QP64_MT) k_qplane64_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  qplane_body<double, JT>(((const QPass64 *)t.args)[q.r], nullptr, q.b.x);
+}
+
+template <class T>
+static void qplane_tiles(const QPassT<T> &p, int &mtx, int &jtx, int &e2max) {
   mtx = jtx = e2max = 1;
   for (int c = 0; c < 12; ++c) {
     mtx = std::max(mtx, cdiv(p.e[c][0], 16));
@@ -2102,35 +2152,59 @@ static void qplane_tiles(const QPass &p, int &mtx, int &jtx, int &e2max) {
     e2max = std::max(e2max, p.e[c][2]);
   }
 }
+template <class T>
 static size_t qplane_lds(int mtx, int jtx) {
-  return (size_t)std::max(16 * jtx * (16 * mtx + 4), 16 * mtx * (16 * jtx + 4)) * sizeof(float);
+  const int pad = qplane_pad<T>();
+  return (size_t)std::max(16 * jtx * (16 * mtx + pad), 16 * mtx * (16 * jtx + pad)) * sizeof(T);
 }
-bool qplane_fits(const QPass &p) {
+template <class T>
+static bool qplane_fits_t(const QPassT<T> &p) {
   int mtx, jtx, e2max;
   qplane_tiles(p, mtx, jtx, e2max);
   for (int c = 0; c < 12; ++c)
     if (p.e[c][0] < 1 || p.e[c][1] < 1 || p.e[c][2] < 1) return false;
-  return mtx <= QP_MT && jtx <= QP_JT && qplane_lds(mtx, jtx) <= 160 * 1024;
+  return mtx <= (sizeof(T) == 8 ? QP64_MT : QP_MT) && jtx <= QP_JT && qplane_lds<T>(mtx, jtx) <= 160 * 1024;
 }
+bool qplane_fits(const QPass &p) { return qplane_fits_t(p); }
+bool qplane64_fits(const QPass64 &p) { return qplane_fits_t(p); }
 int qplane_parts(const QPass &p) {
   int mtx, jtx, e2max;
   qplane_tiles(p, mtx, jtx, e2max);
   return 12 * e2max;
 }
-hipError_t launch_qplane(const QPass &p, const int *gate, hipStream_t s) {
+hipError_t launch_qplane(const QPass &p0, const int *gate, hipStream_t s) {
   if (g_rec) return hipErrorNotSupported;
-  if (!qplane_fits(p)) return hipErrorInvalidValue;
-  int mtx, jtx, e2max;
-  qplane_tiles(p, mtx, jtx, e2max);
-  const dim3 grid(12 * e2max), blk(64 * std::max(4, mtx));
-  const size_t lds = qplane_lds(mtx, jtx);
-  switch (jtx) {
-    case 1: TDGL_LAUNCH((k_qplane<1>), grid, blk, lds, s, p, gate, e2max, mtx, jtx); break;
-    case 2: TDGL_LAUNCH((k_qplane<2>), grid, blk, lds, s, p, gate, e2max, mtx, jtx); break;
-    case 3: TDGL_LAUNCH((k_qplane<3>), grid, blk, lds, s, p, gate, e2max, mtx, jtx); break;
-    case 4: TDGL_LAUNCH((k_qplane<4>), grid, blk, lds, s, p, gate, e2max, mtx, jtx); break;
-    case 5: TDGL_LAUNCH((k_qplane<5>), grid, blk, lds, s, p, gate, e2max, mtx, jtx); break;
-    default: TDGL_LAUNCH((k_qplane<6>), grid, blk, lds, s, p, gate, e2max, mtx, jtx); break;
+  if (!qplane_fits(p0)) return hipErrorInvalidValue;
+  QPass p = p0;
+  qplane_tiles(p, p.mtx, p.jtx, p.e2max);
+  const dim3 grid(12 * p.e2max), blk(64 * std::max(4, p.mtx));
+  const size_t lds = qplane_lds<float>(p.mtx, p.jtx);
+  switch (p.jtx) {
+    case 1: TDGL_LAUNCH((k_qplane<1>), grid, blk, lds, s, p, gate); break;
+    case 2: TDGL_LAUNCH((k_qplane<2>), grid, blk, lds, s, p, gate); break;
+    case 3: TDGL_LAUNCH((k_qplane<3>), grid, blk, lds, s, p, gate); break;
+    case 4: TDGL_LAUNCH((k_qplane<4>), grid, blk, lds, s, p, gate); break;
+    case 5: TDGL_LAUNCH((k_qplane<5>), grid, blk, lds, s, p, gate); break;
+    default: TDGL_LAUNCH((k_qplane<6>), grid, blk, lds, s, p, gate); break;
+  }
+  return hipGetLastError();
+}
+// FP64: recorded with the instance 100 JT + waves (the batch launches each pair on its own)
+hipError_t launch_qplane64(const QPass64 &p0, hipStream_t s) {
+  if (!qplane64_fits(p0)) return hipErrorInvalidValue;
+  QPass64 p = p0;
+  qplane_tiles(p, p.mtx, p.jtx, p.e2max);
+  const int waves = std::max(4, p.mtx);
+  const dim3 grid(12 * p.e2max), blk(64 * waves);
+  const size_t lds = qplane_lds<double>(p.mtx, p.jtx);
+  if (g_rec) return record(BK_QPLANE, Geo{}, &p, sizeof p, 100 * p.jtx + waves, true, grid, lds);
+  switch (p.jtx) {
+    case 1: TDGL_LAUNCH((k_qplane64<1>), grid, blk, lds, s, p); break;
+    case 2: TDGL_LAUNCH((k_qplane64<2>), grid, blk, lds, s, p); break;
+    case 3: TDGL_LAUNCH((k_qplane64<3>), grid, blk, lds, s, p); break;
+    case 4: TDGL_LAUNCH((k_qplane64<4>), grid, blk, lds, s, p); break;
+    case 5: TDGL_LAUNCH((k_qplane64<5>), grid, blk, lds, s, p); break;
+    default: TDGL_LAUNCH((k_qplane64<6>), grid, blk, lds, s, p); break;
   }
   return hipGetLastError();
 }
@@ -2343,11 +2417,14 @@ __global__ void __launch_bounds__(256) k_zmodes_run(ZModes m, const int *gate) {
 // Thread (i, j) folds the four mirror images (i, j), (L0-1-i, j), (i, L1-1-j), (L0-1-i, L1-1-j) of
 // every k (a centre row or column, on an odd length, is its own image: it goes to the even
 // quarter alone).
-__device__ __forceinline__ void fold_index(const Quarters &q, int &c, int &i, int &j, int &kg) {
-  c = blockIdx.z;
-  j = blockIdx.y;
-  i = blockIdx.x * 64 + (int)(threadIdx.x & 63);
+__device__ __forceinline__ void fold_index(const Blk &b, int &c, int &i, int &j, int &kg) {
+  c = b.z;
+  j = b.y;
+  i = b.x * 64 + (int)(threadIdx.x & 63);
   kg = (int)(threadIdx.x >> 6);
+}
+__device__ __forceinline__ void fold_index(const Quarters &, int &c, int &i, int &j, int &kg) {
+  fold_index(hw_blk(), c, i, j, kg);
 }
 int fold_blocks(const Quarters &q) {
   int hx = 1, hy = 1;
@@ -2366,10 +2443,11 @@ static dim3 fold_grid(const Quarters &q) {
   return dim3(cdiv(hx, 64), hy, 3);
 }
 
-__global__ void __launch_bounds__(256) k_fold(FoldPass f, const int *gate) {
-  if (gate && !*gate) return;
+// the fold of block b into quarters of TQ (FP32: the preconditioner's; FP64: the separable solve's)
+template <class TQ>
+__device__ __forceinline__ void fold_body(const FoldPass &f, TQ *out, const Blk &b) {
   int c, i, j, kg;
-  fold_index(f.q, c, i, j, kg);
+  fold_index(b, c, i, j, kg);
   const int L0 = f.q.L[c][0], L1 = f.q.L[c][1], h0 = f.q.h[c][0], h1 = f.q.h[c][1], e2 = f.q.e2[c];
   if (i >= h0 || j >= h1) return;
   const int is = L0 - 1 - i, js = L1 - 1 - j;
@@ -2378,8 +2456,8 @@ __global__ void __launch_bounds__(256) k_fold(FoldPass f, const int *gate) {
   const size_t o_ij = (size_t)i + (size_t)v.s1 * j, o_sj = (size_t)is + (size_t)v.s1 * j;
   const size_t o_is = (size_t)i + (size_t)v.s1 * js, o_ss = (size_t)is + (size_t)v.s1 * js;
   const size_t qp = (size_t)i + (size_t)h0 * j, qs = (size_t)h0 * h1;
-  float *q00 = f.out + f.q.off[c][0] + qp, *q01 = f.out + f.q.off[c][1] + qp;
-  float *q10 = f.out + f.q.off[c][2] + qp, *q11 = f.out + f.q.off[c][3] + qp;
+  TQ *q00 = out + f.q.off[c][0] + qp, *q01 = out + f.q.off[c][1] + qp;
+  TQ *q10 = out + f.q.off[c][2] + qp, *q11 = out + f.q.off[c][3] + qp;
   for (int k = kg; k < e2; k += 4) {
     const size_t zk = (size_t)v.s2 * k;
     double a, b, cc, d;
@@ -2393,11 +2471,24 @@ __global__ void __launch_bounds__(256) k_fold(FoldPass f, const int *gate) {
     const double sj = ci ? a : a + b, dj = ci ? 0.0 : a - b;
     const double ss = ci ? cc : cc + d, ds = ci ? 0.0 : cc - d;
     const size_t qk = qs * k;
-    q00[qk] = (float)(cj ? sj : sj + ss);
-    q01[qk] = (float)(cj ? 0.0 : sj - ss);
-    q10[qk] = (float)(cj ? dj : dj + ds);
-    q11[qk] = (float)(cj ? 0.0 : dj - ds);
+    q00[qk] = (TQ)(cj ? sj : sj + ss);
+    q01[qk] = (TQ)(cj ? 0.0 : sj - ss);
+    q10[qk] = (TQ)(cj ? dj : dj + ds);
+    q11[qk] = (TQ)(cj ? 0.0 : dj - ds);
   }
+}
+__global__ void __launch_bounds__(256) k_fold(FoldPass f, const int *gate) {
+  if (gate && !*gate) return;
+  if (f.out64)
+    fold_body<double>(f, f.out64, hw_blk());
+  else
+    fold_body<float>(f, f.out, hw_blk());
+}
+// batched: the FP64 fold of each run's FoldPass
+__global__ void __launch_bounds__(256) k_fold_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  const FoldPass &f = ((const FoldPass *)t.args)[q.r];
+  fold_body<double>(f, f.out64, q.b);
 }
 
 // Iteration q of the PCG with the folded preconditioner: α = rz/pAp, x += αP and r −= αQ at the
@@ -2541,8 +2632,8 @@ hipError_t launch_ref_fold(const RefFold &r, hipStream_t s) {
 }
 
 hipError_t launch_fold(const FoldPass &f, const int *gate, hipStream_t s) {
-  if (g_rec) return hipErrorNotSupported;
   if (fold_blocks(f.q) > MAX_PART) return hipErrorInvalidValue;
+  if (g_rec) return (f.out64 && !gate) ? record(BK_FOLD, Geo{}, &f, sizeof f, 0, true, fold_grid(f.q), 0) : hipErrorNotSupported;
   TDGL_LAUNCH(k_fold, fold_grid(f.q), dim3(256), 0, s, f, gate);
   return hipGetLastError();
 }
@@ -2550,24 +2641,25 @@ hipError_t launch_fold(const FoldPass &f, const int *gate, hipStream_t s) {
 // The inverse of the fold: the element at each of the four mirror images is the sum of the four
 // quarters, the odd quarter of an axis with the sign of the image (-1 on the mirrored side); the
 // products r·z and r·zold of the values written, partials per block.
-__global__ void __launch_bounds__(256) k_unfold(UnfoldPass u, const int *gate) {
-  if (gate && !*gate) return;
+template <class TQ>
+__device__ __forceinline__ void unfold_body(const UnfoldPass &u, const TQ *in, const Blk &b, int bid) {
   __shared__ double sh[8];
   const int tid = threadIdx.x;
-  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   int c, i, j, kg;
-  fold_index(u.q, c, i, j, kg);
+  fold_index(b, c, i, j, kg);
   const int L0 = u.q.L[c][0], L1 = u.q.L[c][1], h0 = u.q.h[c][0], h1 = u.q.h[c][1], e2 = u.q.e2[c];
   double d0 = 0.0, d1 = 0.0;
   if (i < h0 && j < h1) {
     const int is = L0 - 1 - i, js = L1 - 1 - j;
     const bool ci = is == i, cj = js == j;
     const size_t qp = (size_t)i + (size_t)h0 * j, qs = (size_t)h0 * h1;
-    const float *q00 = u.in + u.q.off[c][0] + qp, *q01 = u.in + u.q.off[c][1] + qp;
-    const float *q10 = u.in + u.q.off[c][2] + qp, *q11 = u.in + u.q.off[c][3] + qp;
-    const View vo = u.out[c];
+    const TQ *q00 = in + u.q.off[c][0] + qp, *q01 = in + u.q.off[c][1] + qp;
+    const TQ *q10 = in + u.q.off[c][2] + qp, *q11 = in + u.q.off[c][3] + qp;
+    const View vo = u.out[c], vt = u.th[c];
     const size_t o[4] = {(size_t)i + (size_t)vo.s1 * j, (size_t)is + (size_t)vo.s1 * j, (size_t)i + (size_t)vo.s1 * js,
                          (size_t)is + (size_t)vo.s1 * js};
+    const size_t ot[4] = {(size_t)i + (size_t)vt.s1 * j, (size_t)is + (size_t)vt.s1 * j, (size_t)i + (size_t)vt.s1 * js,
+                          (size_t)is + (size_t)vt.s1 * js};
     const bool w[4] = {true, !ci, !cj, !ci && !cj};
     for (int k = kg; k < e2; k += 4) {
       const size_t qk = qs * k;
@@ -2575,7 +2667,7 @@ __global__ void __launch_bounds__(256) k_unfold(UnfoldPass u, const int *gate) {
       const double val[4] = {(r00 + r10) + (r01 + r11), (r00 - r10) + (r01 - r11), (r00 + r10) - (r01 + r11),
                              (r00 - r10) - (r01 - r11)};
       const size_t zk = (size_t)vo.s2 * k;
-      double rv[4] = {0, 0, 0, 0}, zv[4] = {0, 0, 0, 0};
+      double rv[4] = {0, 0, 0, 0}, zv[4] = {0, 0, 0, 0}, tv[4] = {0, 0, 0, 0};
       if (u.dot)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -2584,6 +2676,10 @@ __global__ void __launch_bounds__(256) k_unfold(UnfoldPass u, const int *gate) {
           if (u.dot == 2)
             zv[t] = u.f32 ? (double)((const float *)u.zold[c].p)[o[t] + zk] : ((const double *)u.zold[c].p)[o[t] + zk];
         }
+      if (u.addth)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (w[t]) tv[t] = ((const double *)vt.p)[ot[t] + (size_t)vt.s2 * k];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         if (!w[t]) continue;
@@ -2591,6 +2687,8 @@ __global__ void __launch_bounds__(256) k_unfold(UnfoldPass u, const int *gate) {
         const double vw = u.f32 ? (double)(float)val[t] : val[t];
         if (u.f32)
           ((float *)vo.p)[o[t] + zk] = (float)val[t];
+        else if (u.addth)
+          ((double *)vo.p)[o[t] + zk] = tv[t] + val[t];
         else
           ((double *)vo.p)[o[t] + zk] = val[t];
         if (u.dot) {
@@ -2609,10 +2707,26 @@ __global__ void __launch_bounds__(256) k_unfold(UnfoldPass u, const int *gate) {
     }
   }
 }
+__global__ void __launch_bounds__(256) k_unfold(UnfoldPass u, const int *gate) {
+  if (gate && !*gate) return;
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (u.in64)
+    unfold_body<double>(u, u.in64, hw_blk(), bid);
+  else
+    unfold_body<float>(u, u.in, hw_blk(), bid);
+}
+// batched: the FP64 unfold of each run's UnfoldPass (no dots)
+__global__ void __launch_bounds__(256) k_unfold_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  const UnfoldPass &u = ((const UnfoldPass *)t.args)[q.r];
+  unfold_body<double>(u, u.in64, q.b, 0);
+}
 
 hipError_t launch_unfold(const UnfoldPass &u, const int *gate, hipStream_t s) {
-  if (g_rec) return hipErrorNotSupported;
   if (fold_blocks(u.q) > MAX_PART) return hipErrorInvalidValue;
+  if (g_rec)
+    return (u.in64 && !gate && !u.dot) ? record(BK_UNFOLD, Geo{}, &u, sizeof u, 0, true, fold_grid(u.q), 0)
+                                       : hipErrorNotSupported;
   TDGL_LAUNCH(k_unfold, fold_grid(u.q), dim3(256), 0, s, u, gate);
   return hipGetLastError();
 }
@@ -2626,20 +2740,35 @@ hipError_t launch_unfold(const UnfoldPass &u, const int *gate, hipStream_t s) {
 // (and, going back, the z positions) into contiguous groups of mzc: a wave's group index is
 // uniform, so the bases are read with uniform addresses (scalar loads, no LDS), and the columns are
 // shared through LDS.  Every global store comes after the last load of the bases.
-template <int G, int MZ>
-__global__ void __launch_bounds__(64 * G) k_zmodes_q(ZModesQ m, const int *gate) {
-  if (gate && !*gate) return;
-  using T = float;
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// T = float with ZModesQ (the Float32 preconditioner), T = double with ZModesQ64 (the FP64
+// separable solve); block (bx, bz) of the launch's grid
+// LB: the four bases staged in LDS after the columns (zq_lds_bases elements each) and read there
+// with wave-uniform addresses (broadcasts), instead of through the scalar cache
+__host__ __device__ constexpr int zq_lds_bases(int nz) { return nz * nz + 64; }
+template <int G, int MZ, class T, class A, bool LB = false>
+__device__ __forceinline__ void zmodes_q_body(const A &m, int bx, int bz) {
   extern __shared__ __align__(16) unsigned char zq_smem[];
   const int nx = m.nx, ny = m.ny, nz = m.nz;
   T *Cx = (T *)zq_smem, *Cy = Cx + nz * 64, *Cz = Cy + nz * 64;
+  const T *zb[4] = {m.zb[0], m.zb[1], m.zb[2], m.zb[3]};
+  if constexpr (LB) {
+    // every base and 64 elements of its slack (the reads of the last group past nz)
+    const int nb = zq_lds_bases(nz);
+    T *Bs = Cz + nz * 64;
+    for (int k = 0; k < 4; ++k) {
+      for (int t = (int)threadIdx.x; t < nb; t += 64 * G) Bs[k * nb + t] = m.zb[k][t];
+      zb[k] = Bs + k * nb;
+    }
+  }
   const int p = (int)(threadIdx.x & 63);
   const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int mzc = (nz + G - 1) / G;  // modes (positions) of a group, at most MZ
   const int m0 = g * mzc;
-  const int px = (int)(blockIdx.z >> 1), py = (int)(blockIdx.z & 1);
+  const int px = bz >> 1, py = bz & 1;
   const Quarters &q = m.q;
-  const int f = blockIdx.x * 64 + p;  // the mode (mm, qq) of Z's quarter, flattened along mm
+  const int f = bx * 64 + p;  // the mode (mm, qq) of Z's quarter, flattened along mm
   const int mm = f % q.h[2][0], qq = f / q.h[2][0];
   const int mx = 2 * mm + px, my = 2 * qq + py;
   const bool live = qq < q.h[2][1] && mx < nx && my < ny;
@@ -2661,7 +2790,7 @@ __global__ void __launch_bounds__(64 * G) k_zmodes_q(ZModesQ m, const int *gate)
   __syncthreads();
   // forward: the modes m0 .. m0 + MZ - 1 of the three columns (those past the group's mzc are the
   // next group's, or none: computed and dropped); the padded bases by rows, V(n, mz) and U(n, mz - 1)
-  const T *VR = m.zb[0], *UR = m.zb[1];
+  const T *VR = zb[0], *UR = zb[1];
   T ax[MZ], ay[MZ], az[MZ];
 #pragma unroll
   for (int r = 0; r < MZ; ++r) ax[r] = ay[r] = az[r] = T(0);
@@ -2670,9 +2799,9 @@ __global__ void __launch_bounds__(64 * G) k_zmodes_q(ZModesQ m, const int *gate)
     const T *v = VR + (size_t)n * nz + m0, *u = UR + (size_t)n * nz + m0;
 #pragma unroll
     for (int r = 0; r < MZ; ++r) {
-      ax[r] = __builtin_fmaf(cx, v[r], ax[r]);
-      ay[r] = __builtin_fmaf(cy, v[r], ay[r]);
-      az[r] = __builtin_fmaf(cz, u[r], az[r]);
+      ax[r] = fma_t(cx, v[r], ax[r]);
+      ay[r] = fma_t(cy, v[r], ay[r]);
+      az[r] = fma_t(cz, u[r], az[r]);
     }
   }
   __syncthreads();
@@ -2695,7 +2824,7 @@ __global__ void __launch_bounds__(64 * G) k_zmodes_q(ZModesQ m, const int *gate)
   }
   __syncthreads();
   // back: the positions m0 .. m0 + MZ - 1; the padded bases by columns, V(n, qm) and U(n, qm - 1)
-  const T *VC = m.zb[2], *UC = m.zb[3];
+  const T *VC = zb[2], *UC = zb[3];
   T ox[MZ], oy[MZ], oz[MZ];
 #pragma unroll
   for (int r = 0; r < MZ; ++r) ox[r] = oy[r] = oz[r] = T(0);
@@ -2704,9 +2833,9 @@ __global__ void __launch_bounds__(64 * G) k_zmodes_q(ZModesQ m, const int *gate)
     const T *v = VC + (size_t)qm * nz + m0, *u = UC + (size_t)qm * nz + m0;
 #pragma unroll
     for (int r = 0; r < MZ; ++r) {
-      ox[r] = __builtin_fmaf(cx, v[r], ox[r]);
-      oy[r] = __builtin_fmaf(cy, v[r], oy[r]);
-      oz[r] = __builtin_fmaf(cz, u[r], oz[r]);
+      ox[r] = fma_t(cx, v[r], ox[r]);
+      oy[r] = fma_t(cy, v[r], oy[r]);
+      oz[r] = fma_t(cz, u[r], oz[r]);
     }
   }
   if (!live) return;
@@ -2719,6 +2848,21 @@ __global__ void __launch_bounds__(64 * G) k_zmodes_q(ZModesQ m, const int *gate)
       if (n < nz - 1) colz[stz * n] = oz[r];
     }
   }
+}
+template <int G, int MZ>
+__global__ void __launch_bounds__(64 * G) k_zmodes_q(ZModesQ m, const int *gate) {
+  if (gate && !*gate) return;
+  zmodes_q_body<G, MZ, float, ZModesQ>(m, (int)blockIdx.x, (int)blockIdx.z);
+}
+template <int G, int MZ>
+__global__ void __launch_bounds__(64 * G) k_zmodes_q64(ZModesQ64 m) {
+  zmodes_q_body<G, MZ, double, ZModesQ64, true>(m, (int)blockIdx.x, (int)blockIdx.z);
+}
+// batched: run r's blocks take its own (mode block, quarter) grid
+template <int G, int MZ>
+__global__ void __launch_bounds__(64 * G) k_zmodes_q64_b(BatchTab t) {
+  const BatchIdx q = batch_idx(t);
+  zmodes_q_body<G, MZ, double, ZModesQ64, true>(((const ZModesQ64 *)t.args)[q.r], q.b.x, q.b.z);
 }
 
 template <int G>
@@ -2916,6 +3060,30 @@ static hipError_t launch_zmodes_mf(const ZModesQ &m, const int *gate, hipStream_
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+static int zq_mz(int mzc) { return mzc <= 2 ? 2 : mzc <= 4 ? 4 : mzc <= 6 ? 6 : mzc <= 8 ? 8 : mzc <= 12 ? 12 : 16; }
+template <int G>
+static hipError_t zmodes_q64_g(const ZModesQ64 &m, hipStream_t s) {
+  const int mz = zq_mz((m.nz + G - 1) / G);
+  const dim3 grid(cdiv(m.q.h[2][0] * m.q.h[2][1], 64), 1, 4), blk(64 * G);
+  const size_t ldr = (3 * (size_t)m.nz * 64 + 4 * (size_t)zq_lds_bases(m.nz)) * sizeof(double);
+  if (g_rec) return record(BK_ZMODESQ, Geo{}, &m, sizeof m, 100 * G + mz, true, grid, ldr);
+  switch (mz) {
+    case 2: TDGL_LAUNCH((k_zmodes_q64<G, 2>), grid, blk, ldr, s, m); break;
+    case 4: TDGL_LAUNCH((k_zmodes_q64<G, 4>), grid, blk, ldr, s, m); break;
+    case 6: TDGL_LAUNCH((k_zmodes_q64<G, 6>), grid, blk, ldr, s, m); break;
+    case 8: TDGL_LAUNCH((k_zmodes_q64<G, 8>), grid, blk, ldr, s, m); break;
+    case 12: TDGL_LAUNCH((k_zmodes_q64<G, 12>), grid, blk, ldr, s, m); break;
+    default: TDGL_LAUNCH((k_zmodes_q64<G, 16>), grid, blk, ldr, s, m); break;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_zmodes_q64(const ZModesQ64 &m, hipStream_t s) {
+  if (m.nz < 2 || m.nz > ZQ64_NZ_MAX || m.nx < 1 || m.ny < 1) return hipErrorInvalidValue;
+  if (m.waves == 8) return zmodes_q64_g<8>(m, s);
+  if (m.waves == 4) return zmodes_q64_g<4>(m, s);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_zmodes_q(const ZModesQ &m, const int *gate, hipStream_t s) {
@@ -3702,6 +3870,10 @@ size_t batch_args_size(int kind) {
     case BK_METERFIN: return sizeof(MeterFinArgs);
     case BK_PLANE: return sizeof(PlanePass);
     case BK_ZMODES: return sizeof(ZModes);
+    case BK_FOLD: return sizeof(FoldPass);
+    case BK_UNFOLD: return sizeof(UnfoldPass);
+    case BK_ZMODESQ: return sizeof(ZModesQ64);
+    case BK_QPLANE: return sizeof(QPass64);
     default: return 0;
   }
 }
@@ -3742,6 +3914,33 @@ hipError_t launch_batch(int kind, int variant, size_t lds, const BatchTab &t, in
       else
         TDGL_LAUNCH(k_zmodes_b<false>, dim3(blocks), dim3(256), lds, s, t);
       break;
+    case BK_QPLANE: {
+      const int jt = variant / 100, waves = variant % 100;
+      if (waves < 4 || waves > QP64_MT) return hipErrorInvalidValue;
+      switch (jt) {
+        case 1: TDGL_LAUNCH((k_qplane64_b<1>), dim3(blocks), dim3(64 * waves), lds, s, t); break;
+        case 2: TDGL_LAUNCH((k_qplane64_b<2>), dim3(blocks), dim3(64 * waves), lds, s, t); break;
+        case 3: TDGL_LAUNCH((k_qplane64_b<3>), dim3(blocks), dim3(64 * waves), lds, s, t); break;
+        case 4: TDGL_LAUNCH((k_qplane64_b<4>), dim3(blocks), dim3(64 * waves), lds, s, t); break;
+        case 5: TDGL_LAUNCH((k_qplane64_b<5>), dim3(blocks), dim3(64 * waves), lds, s, t); break;
+        case 6: TDGL_LAUNCH((k_qplane64_b<6>), dim3(blocks), dim3(64 * waves), lds, s, t); break;
+        default: return hipErrorInvalidValue;
+      }
+      break;
+    }
+    case BK_FOLD: TDGL_LAUNCH(k_fold_b, dim3(blocks), dim3(256), 0, s, t); break;
+    case BK_UNFOLD: TDGL_LAUNCH(k_unfold_b, dim3(blocks), dim3(256), 0, s, t); break;
+    case BK_ZMODESQ:
+      switch (variant) {
+#define TDGL_ZQB_CASE(g, mz) \
+  case 100 * g + mz: TDGL_LAUNCH((k_zmodes_q64_b<g, mz>), dim3(blocks), dim3(64 * g), lds, s, t); break;
+        TDGL_ZQB_CASE(8, 2) TDGL_ZQB_CASE(8, 4) TDGL_ZQB_CASE(8, 6) TDGL_ZQB_CASE(8, 8) TDGL_ZQB_CASE(8, 12)
+        TDGL_ZQB_CASE(8, 16) TDGL_ZQB_CASE(4, 2) TDGL_ZQB_CASE(4, 4) TDGL_ZQB_CASE(4, 6) TDGL_ZQB_CASE(4, 8)
+        TDGL_ZQB_CASE(4, 12) TDGL_ZQB_CASE(4, 16)
+#undef TDGL_ZQB_CASE
+        default: return hipErrorInvalidValue;
+      }
+      break;
     case BK_PLANE:
       if (variant >= 1000) {
         switch (variant - 1000) {
@@ -3754,6 +3953,7 @@ hipError_t launch_batch(int kind, int variant, size_t lds, const BatchTab &t, in
           TDGL_PLANE1B_CASE(4, 2, 2) TDGL_PLANE1B_CASE(4, 3, 2) TDGL_PLANE1B_CASE(4, 2, 3) TDGL_PLANE1B_CASE(4, 3, 3)
           TDGL_PLANE1B_CASE(5, 2, 2) TDGL_PLANE1B_CASE(5, 3, 2) TDGL_PLANE1B_CASE(5, 2, 3) TDGL_PLANE1B_CASE(5, 3, 3)
           TDGL_PLANE1B_CASE(5, 3, 1)
+          TDGL_PLANE1_SMALL(TDGL_PLANE1B_CASE)
 #undef TDGL_PLANE1B_CASE
           default: return hipErrorInvalidValue;
         }

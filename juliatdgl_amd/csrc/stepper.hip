@@ -156,6 +156,13 @@ struct tdgl_handle {
   int qplane = 1;              // TDGL_HIP_QPLANE=0: the quarter x-y transforms on k_plane_mf1 / k_plane_mf
   int zq_waves = 8;        // the quarter z pass: 4 or 8 waves a block on the vector ALUs, 0 on the matrix cores (k_zmodes_mf)
   float *zqb[4] = {};      // its padded z bases (ZQ_BASES)
+  // the parity-folded FP64 separable solve (field_mode SEPARABLE): the same quarters, in T1 and T2,
+  // the FP64 half-size bases [axis][0=V,1=U][parity][0=fwd,1=inv] and padded z bases
+  bool fold64 = false;
+  size_t t12cap = 0;           // the elements of T1 and T2
+  double *Hq64[2][2][2][2] = {};
+  double *Hp64[2][2][2][2] = {};  // Hq64 zero-padded to whole 16 x 16 tiles (k_qplane64)
+  double *zqb64[4] = {};
   bool tx_only = false;                   // the separable solve as single-axis passes (a mixed batch)
   int box_kc = 1;                         // levels of k per block of the PCG box kernels
   int plane1 = 1;                         // PlanePass::one of the Float32 launches
@@ -351,9 +358,20 @@ int dalloc(tdgl_handle *h, T **p, size_t n) {
     if (rc_) return rc_;                     \
   } while (0)
 
+// a host vector to the device: staged in a pinned block of the handle's and queued on its stream
+// (tdgl_create waits once, at its end, for every copy and fill it queued, then gives the staging
+// back); a synchronous pageable copy each would wait on the device's copy queue, shared with the
+// runs other host threads drive
 template <class T>
 int upload_vec(tdgl_handle *h, T *dst, const std::vector<T> &v) {
-  return hcopy(h, dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  const size_t bytes = v.size() * sizeof(T);
+  if (bytes == 0) return 0;
+  Block b{};
+  HIPCHK(hblock(bytes, &b));
+  h->stages.push_back(b);
+  std::memcpy(b.p, v.data(), bytes);
+  HIPCHK(hipMemcpyAsync(dst, b.p, bytes, hipMemcpyHostToDevice, h->s));
+  return 0;
 }
 
 // Julia-shaped (n1, n2, n3) host array of `es`-byte elements -> node-shaped device array, the
@@ -761,6 +779,29 @@ struct SepOut {
   const Vec3f *r32 = nullptr, *zold32 = nullptr;
 };
 
+// The quarters of the parity folds of the solve region (Quarters): four per component, in the
+// order (component, 2 px + py); qe[4 c + k] their extents
+Quarters quarters_of(const Ranges &rg, int qe[12][3]) {
+  Quarters q{};
+  int off = 0;
+  for (int c = 0; c < 3; ++c) {
+    q.L[c][0] = rg.e[c][0];
+    q.L[c][1] = rg.e[c][1];
+    q.h[c][0] = (rg.e[c][0] + 1) / 2;
+    q.h[c][1] = (rg.e[c][1] + 1) / 2;
+    q.e2[c] = rg.e[c][2];
+    for (int k = 0; k < 4; ++k) {
+      q.off[c][k] = off;
+      off += q.h[c][0] * q.h[c][1] * q.e2[c];
+      qe[4 * c + k][0] = q.h[c][0];
+      qe[4 * c + k][1] = q.h[c][1];
+      qe[4 * c + k][2] = q.e2[c];
+    }
+  }
+  q.total = off;
+  return q;
+}
+
 // (I + aK)^{-1} of the node-shaped vector `in` (_sep_solve!, implicit.jl:62-83, or in Float32
 // _precond32!, :353-373): forward passes along x and y, the pass along z fused with the mode
 // projector, the three inverse passes.  `conv`: the convergence check of the first pass.
@@ -899,8 +940,100 @@ int sep_solve_folded(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int 
   return sep_folded_tail(h, o, gate, conv);
 }
 
+// The FP64 separable solve with the parity folds (_sep_solve!, implicit.jl:62-83): fold x and y
+// of the three components into the quarters (T1), their x-y transforms with the FP64 half-size
+// bases (T1 -> T2), the z pass with the projector on the quarters (in T2), the inverse x-y
+// transforms (T2 -> T1), the unfold into the output (φ = θ + the solve with addth).
+PlanePass quarter_pass64(tdgl_handle *h, bool inv, double *in, double *out) {
+  PlanePass p{};
+  p.d = h->pd;
+  p.one = h->plane1;
+  p.ncomp = 12;
+  const Quarters &q = h->q;
+  for (int c = 0; c < 3; ++c)
+    for (int px = 0; px < 2; ++px)
+      for (int py = 0; py < 2; ++py) {
+        const int k = 4 * c + 2 * px + py, qo = q.off[c][2 * px + py];
+        p.e[k][0] = q.h[c][0];
+        p.e[k][1] = q.h[c][1];
+        p.e[k][2] = q.e2[c];
+        p.in[k] = View{(void *)(in + qo), q.h[c][0], q.h[c][0] * q.h[c][1]};
+        p.out[k] = View{(void *)(out + qo), q.h[c][0], q.h[c][0] * q.h[c][1]};
+        const int wx = c == 0 ? 1 : 0, wy = c == 1 ? 1 : 0;
+        p.X[k] = h->Hq64[0][wx][px][inv];
+        p.Y[k] = h->Hq64[1][wy][py][inv];
+        p.XT[k] = h->Hq64[0][wx][px][!inv];
+        p.YT[k] = h->Hq64[1][wy][py][!inv];
+      }
+  return p;
+}
+// one FP64 x-y launch of the quarters: k_qplane64 where it takes them (TDGL_HIP_QPLANE=0: never),
+// else the plane kernels
+int quarter_xy64(tdgl_handle *h, bool inv, double *in, double *out) {
+  const PlanePass p = quarter_pass64(h, inv, in, out);
+  if (h->qplane) {
+    QPass64 qp{};
+    for (int k = 0; k < 12; ++k) {
+      qp.in[k] = p.in[k];
+      qp.out[k] = p.out[k];
+      for (int e = 0; e < 3; ++e) qp.e[k][e] = p.e[k][e];
+      const int c = k / 4, px = (k / 2) % 2, py = k % 2;
+      const int wx = c == 0 ? 1 : 0, wy = c == 1 ? 1 : 0;
+      qp.XT[k] = h->Hp64[0][wx][px][!inv];
+      qp.YT[k] = h->Hp64[1][wy][py][!inv];
+      // the padded bases are 16 ceil(e/16) square: a quarter of another extent stays off k_qplane64
+      if (h->HpL[0][wx] != p.e[k][0] || h->HpL[1][wy] != p.e[k][1]) qp.e[k][2] = 0;
+    }
+    if (qplane64_fits(qp)) {
+      PL(K_TX, launch_qplane64(qp, h->s));
+      return 0;
+    }
+  }
+  PL(K_TX, (launch_plane<double, double, double>(p, nullptr, h->s)));
+  return 0;
+}
+int sep_solve_folded64(tdgl_handle *h, const Vec3 &in, const SepOut &o) {
+  FoldPass f{};
+  for (int c = 0; c < 3; ++c) f.in[c] = node_view(h, in.c[c]);
+  f.out64 = h->T1;
+  f.q = h->q;
+  PL(K_FOLD, launch_fold(f, nullptr, h->s));
+  CHK(quarter_xy64(h, false, h->T1, h->T2));
+  ZModesQ64 zq{};
+  zq.modes = h->T2;
+  zq.q = h->q;
+  for (int k = 0; k < 4; ++k) zq.zb[k] = h->zqb64[k];
+  zq.sx = h->sd[0];
+  zq.sy = h->sd[1];
+  zq.sz = h->sd[2];
+  zq.nx = h->g.Nx - 1;
+  zq.ny = h->g.Ny - 1;
+  zq.nz = h->g.Nz - 1;
+  zq.a = h->d.a;
+  zq.waves = h->zq_waves == 4 ? 4 : 8;
+  PL(K_MODES, launch_zmodes_q64(zq, h->s));
+  CHK(quarter_xy64(h, true, h->T2, h->T1));
+  UnfoldPass u{};
+  u.in64 = h->T1;
+  u.q = h->q;
+  u.addth = o.addth;
+  for (int c = 0; c < 3; ++c) {
+    if (o.addth) {
+      u.out[c] = node_view(h, h->ph[c]);
+      u.th[c] = node_view(h, h->th[c]);
+    } else {
+      u.out[c] = node_view(h, o.out->c[c]);
+    }
+  }
+  u.d = h->pd;
+  PL(K_FOLD, launch_unfold(u, nullptr, h->s));
+  return 0;
+}
+
 int sep_solve(tdgl_handle *h, const Vec3 &in, const SepOut &o, const int *gate, bool f32, Conv conv) {
   if (f32 && h->fold) return sep_solve_folded(h, in, o, gate, conv);
+  if (!f32 && h->fold64 && !h->tx_only && !gate && conv.mode == CONV_NONE && !o.dot && !o.out32)
+    return sep_solve_folded64(h, in, o);
   void *Ta = f32 ? (void *)h->T32a : (void *)h->T1;
   void *Tb = f32 ? (void *)h->T32b : (void *)h->T2;
   const size_t es = f32 ? 4 : 8;
@@ -2058,8 +2191,11 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
         CHK(upload_vec(h, h->sf[ax], s32));
       }
       for (int c = 0; c < 3; ++c) ALLOC(h->B.c[c], n);
-      ALLOC(h->T1, rg.total);
-      ALLOC(h->T2, rg.total);
+      // T1 and T2 also hold the FP64 quarters (a little more than the region: ceil halves)
+      int qe[12][3];
+      h->t12cap = std::max((size_t)rg.total, (size_t)quarters_of(rg, qe).total);
+      ALLOC(h->T1, h->t12cap);
+      ALLOC(h->T2, h->t12cap);
     }
     if (d.field_mode == TDGL_FIELD_PCG) {
       Vec3 *vs[8] = {&h->x, &h->r, &h->z, &h->P, &h->P2, &h->Qv, &h->xprev, &h->zold};
@@ -2213,24 +2349,8 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     const char *gf = std::getenv("TDGL_HIP_FOLD");
     if (d.field_mode == TDGL_FIELD_PCG && d.pcg_f32 && !(gf && gf[0] == '0') && !(gp && gp[0] == '0') &&
         d.Nz - 1 >= 2 && d.Nz - 1 <= ZQ_NZ_MAX) {
-      Quarters q{};
-      int off = 0;
       int qe[12][3];
-      for (int c = 0; c < 3; ++c) {
-        q.L[c][0] = rg.e[c][0];
-        q.L[c][1] = rg.e[c][1];
-        q.h[c][0] = (rg.e[c][0] + 1) / 2;
-        q.h[c][1] = (rg.e[c][1] + 1) / 2;
-        q.e2[c] = rg.e[c][2];
-        for (int k = 0; k < 4; ++k) {
-          q.off[c][k] = off;
-          off += q.h[c][0] * q.h[c][1] * q.e2[c];
-          qe[4 * c + k][0] = q.h[c][0];
-          qe[4 * c + k][1] = q.h[c][1];
-          qe[4 * c + k][2] = q.e2[c];
-        }
-      }
-      q.total = off;
+      const Quarters q = quarters_of(rg, qe);
       if (plane_fits<float>(qe, 12) && fold_blocks(q) <= MAX_PART) {
         h->q = q;
         ALLOC(h->qa, q.total);
@@ -2292,6 +2412,67 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
         h->fold = true;
       }
     }
+    // the parity-folded FP64 separable solve (TDGL_HIP_FOLD64=0: the full-length transforms)
+    const char *gf64 = std::getenv("TDGL_HIP_FOLD64");
+    if (d.field_mode == TDGL_FIELD_SEPARABLE && h->plane64 && !(gf64 && gf64[0] == '0') && d.Nz - 1 >= 2 &&
+        d.Nz - 1 <= ZQ64_NZ_MAX) {
+      int qe[12][3];
+      const Quarters q = quarters_of(rg, qe);
+      if (plane_fits<double>(qe, 12) && fold_blocks(q) <= MAX_PART && (size_t)q.total <= h->t12cap) {
+        h->q = q;
+        const int nn[2] = {d.Nx - 1, d.Ny - 1};
+        for (int ax = 0; ax < 2; ++ax) {
+          const Bases b = make_bases(nn[ax]);
+          for (int which = 0; which < 2; ++which) {
+            const std::vector<double> &M = which == 0 ? b.V : b.U;
+            const int L = which == 0 ? nn[ax] : nn[ax] - 1, Lh = (L + 1) / 2;
+            for (int par = 0; par < 2; ++par) {
+              // Mh(i, cc) = M(i, 2 cc + par) on the first ceil(L/2) rows, and its transpose
+              std::vector<double> hf((size_t)Lh * Lh, 0.0), hi((size_t)Lh * Lh, 0.0);
+              for (int i = 0; i < Lh; ++i)
+                for (int cc = 0; cc < Lh; ++cc) {
+                  const int col = 2 * cc + par;
+                  const double v = col < L ? M[i + (size_t)L * col] : 0.0;
+                  hf[(size_t)i * Lh + cc] = v;
+                  hi[(size_t)cc * Lh + i] = v;
+                }
+              ALLOC(h->Hq64[ax][which][par][0], hf.size());
+              CHK(upload_vec(h, h->Hq64[ax][which][par][0], hf));
+              ALLOC(h->Hq64[ax][which][par][1], hi.size());
+              CHK(upload_vec(h, h->Hq64[ax][which][par][1], hi));
+              const int LP = 16 * cdiv(Lh, 16);
+              for (int t = 0; t < 2; ++t) {
+                const std::vector<double> &src = t == 0 ? hf : hi;
+                std::vector<double> pd((size_t)LP * LP, 0.0);
+                for (int r = 0; r < Lh; ++r)
+                  for (int c = 0; c < Lh; ++c) pd[(size_t)r * LP + c] = src[(size_t)r * Lh + c];
+                ALLOC(h->Hp64[ax][which][par][t], pd.size());
+                CHK(upload_vec(h, h->Hp64[ax][which][par][t], pd));
+              }
+              h->HpL[ax][which] = Lh;
+            }
+          }
+        }
+        const int nz = d.Nz - 1, Lu = nz - 1;
+        const Bases b = make_bases(nz);
+        std::vector<double> zb[4];
+        for (int k = 0; k < 4; ++k) zb[k].assign((size_t)nz * nz + ZQ_SLACK, 0.0);
+        for (int n = 0; n < nz; ++n)
+          for (int mz = 0; mz < nz; ++mz) {
+            const double v = b.V[n + (size_t)nz * mz];
+            const double u = (mz >= 1 && n < Lu) ? b.U[n + (size_t)Lu * (mz - 1)] : 0.0;
+            zb[0][(size_t)n * nz + mz] = v;
+            zb[1][(size_t)n * nz + mz] = u;
+            zb[2][(size_t)mz * nz + n] = v;
+            zb[3][(size_t)mz * nz + n] = u;
+          }
+        for (int k = 0; k < 4; ++k) {
+          ALLOC(h->zqb64[k], zb[k].size());
+          CHK(upload_vec(h, h->zqb64[k], zb[k]));
+        }
+        h->fold64 = true;
+      }
+    }
     if (d.field_mode == TDGL_FIELD_PCG && !(std::getenv("TDGL_HIP_DPAL") && std::getenv("TDGL_HIP_DPAL")[0] == '0')) {
       // D as a palette when each component takes at most DPAL_MAX values: one 32-bit code per
       // node instead of three doubles in the PCG operator (the same values, so the same bits)
@@ -2350,7 +2531,8 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
   };
   int rc = run();
   if (rc) return bail(rc);
-  if (hipStreamSynchronize(h->s) != hipSuccess) return bail(fail(TDGL_E_DEVICE, "the fills of the allocations"));
+  if (hipStreamSynchronize(h->s) != hipSuccess) return bail(fail(TDGL_E_DEVICE, "the fills and uploads of the allocations"));
+  stages_back(h);
   *out = h;
   return 0;
 }
@@ -2968,9 +3150,10 @@ int tdgl_meter_begin(tdgl_handle *h, const tdgl_meter *m) {
   a.nhb = cdiv(d.Nx * d.Ny * (d.Nz + 1), 256);
   unsigned char *mk[4];
   const uint8_t *src[4] = {m->Min, m->Mou, m->line_ok, m->line_f1};
+  // the masks staged and queued, the fills queued: one wait at the end
   for (int q = 0; q < 4; ++q) {
     CHK(malloc_meter(h, &mk[q], (size_t)nl));
-    CHK(hcopy(h, mk[q], src[q], (size_t)nl, hipMemcpyHostToDevice));
+    CHK(upload_vec(h, mk[q], std::vector<unsigned char>(src[q], src[q] + nl)));
   }
   a.Min = mk[0];
   a.Mou = mk[1];
@@ -2980,22 +3163,21 @@ int tdgl_meter_begin(tdgl_handle *h, const tdgl_meter *m) {
     CHK(malloc_meter(h, &a.Dprev[mu], (size_t)nl));
     CHK(malloc_meter(h, &a.T[mu], (size_t)nl));
     CHK(malloc_meter(h, &a.cooper[mu], (size_t)nb));
-    CHK(hfill0(h, a.Dprev[mu], nl * sizeof(double)));
-    CHK(hfill0(h, a.T[mu], nl * sizeof(double)));
+    HIPCHK(hipMemsetAsync(a.Dprev[mu], 0, nl * sizeof(double), h->s));
+    HIPCHK(hipMemsetAsync(a.T[mu], 0, nl * sizeof(double), h->s));
   }
   CHK(malloc_meter(h, &a.hold, (size_t)g.n));
   CHK(malloc_meter(h, &a.part, (size_t)meter_blocks(a) * METER_Q));
   CHK(malloc_meter(h, &h->mrec, (size_t)MR_N));
   CHK(malloc_meter(h, &h->mrung, 2));
-  CHK(hfill0(h, h->mrung, 2 * sizeof(double)));
+  HIPCHK(hipMemsetAsync(h->mrung, 0, 2 * sizeof(double), h->s));
   h->ma = a;
   h->meter = true;
   // Cooper0 and fieldOld of the current state
   MeterArgs b = meter_args(h, 0);
   b.init = 1;
   LCHK(launch_meter(b, h->s));
-  HIPCHK(hipStreamSynchronize(h->s));
-  return 0;
+  return flush(h);
 }
 
 int tdgl_meter_cycle(tdgl_handle *h, int32_t first, tdgl_cycle_record *out) {
@@ -3018,7 +3200,7 @@ int tdgl_get_stats(tdgl_handle *h, tdgl_stats *st) {
   st->rollbacks = h->rollbacks;
   st->pcg_budget = h->budget;
   st->captures = h->captures;
-  st->folded = h->fold ? 1 : 0;
+  st->folded = (h->fold || h->fold64) ? 1 : 0;
   st->steps = h->steps;
   st->calls = h->calls;
   st->pcg_iters = h->pcg_iters;
@@ -3060,6 +3242,7 @@ struct LaunchPlan {
   int kind, variant, blocks, n;
   size_t lds;
   size_t off, dims, g, args;
+  int pos;  // the position in the step sequence: the instances of one position are independent
 };
 // The steps of a call are recorded once per run (the launches its own step would make, with the
 // single-axis passes of the separable solve), checked to be the same sequence of kernels in
@@ -3082,6 +3265,11 @@ struct tdgl_batch {
   Block mtab{}, mstage{};      // device tables of the meter's launches, their pinned staging
   Block dH{}, hH{};            // (H, HJ) of every run: device, pinned host
   Block drec{}, hrec{};        // the meter's records: device, pinned host
+  // the graph's parallel branches: the kernel instances of one position run side by side, each
+  // on a side stream that forks from s at an event and joins it again (default priority)
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
 };
 
 namespace {
@@ -3219,6 +3407,7 @@ int batch_graph(tdgl_batch *b, int m) {
       }
       LaunchPlan lp{};
       CHK(plan_launch(kind, v, lds, gs, args, grids, tb, lp));
+      lp.pos = (int)q;
       plans.push_back(lp);
     }
   }
@@ -3232,10 +3421,51 @@ int batch_graph(tdgl_batch *b, int m) {
     b->gpar = par;
     return 0;
   }
+  // side streams and events for the widest position (TDGL_HIP_BATCH_FORK=0: one chain)
+  const char *bf = std::getenv("TDGL_HIP_BATCH_FORK");
+  const bool forks = !(bf && bf[0] == '0');
+  size_t width = 1;
+  for (size_t q = 0; q < plans.size();) {
+    size_t e = q;
+    while (e < plans.size() && plans[e].pos == plans[q].pos) ++e;
+    width = std::max(width, e - q);
+    q = e;
+  }
+  if (forks) {
+    if (!b->fork) HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    while (b->side.size() + 1 < width) {
+      hipStream_t st = nullptr;
+      hipEvent_t ev = nullptr;
+      HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      b->side.push_back(st);
+      HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      b->join.push_back(ev);
+    }
+  }
   HIPCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeRelaxed));
   hipError_t le = hipSuccess;
-  for (size_t q = 0; q < plans.size() && le == hipSuccess; ++q)
-    le = launch_batch(plans[q].kind, plans[q].variant, plans[q].lds, tab_of(b->tab, plans[q]), plans[q].blocks, b->s);
+  for (size_t q = 0; q < plans.size() && le == hipSuccess;) {
+    size_t e = q;
+    while (e < plans.size() && plans[e].pos == plans[q].pos) ++e;
+    if (!forks || e - q == 1) {
+      for (; q < e && le == hipSuccess; ++q)
+        le = launch_batch(plans[q].kind, plans[q].variant, plans[q].lds, tab_of(b->tab, plans[q]), plans[q].blocks, b->s);
+      continue;
+    }
+    // fork: instance k >= 1 on side stream k - 1 after the fork event, the first on s; join
+    le = hipEventRecord(b->fork, b->s);
+    for (size_t k = 1; k < e - q && le == hipSuccess; ++k) {
+      const LaunchPlan &lp = plans[q + k];
+      hipStream_t st = b->side[k - 1];
+      le = hipStreamWaitEvent(st, b->fork, 0);
+      if (le == hipSuccess) le = launch_batch(lp.kind, lp.variant, lp.lds, tab_of(b->tab, lp), lp.blocks, st);
+      if (le == hipSuccess) le = hipEventRecord(b->join[k - 1], st);
+    }
+    if (le == hipSuccess)
+      le = launch_batch(plans[q].kind, plans[q].variant, plans[q].lds, tab_of(b->tab, plans[q]), plans[q].blocks, b->s);
+    for (size_t k = 1; k < e - q && le == hipSuccess; ++k) le = hipStreamWaitEvent(b->s, b->join[k - 1], 0);
+    q = e;
+  }
   hipGraph_t gph = nullptr;
   const hipError_t ee = hipStreamEndCapture(b->s, &gph);
   if (le != hipSuccess || ee != hipSuccess) {
@@ -3273,7 +3503,7 @@ int tdgl_batch_create(tdgl_handle *const *hs, int32_t n, tdgl_batch **out) {
   b->dev = hs[0]->dev;
   const char *ge = std::getenv("TDGL_HIP_GRAPH");  // =0: the batched launches from the host (profiling)
   b->graphs = !(ge && ge[0] == '0');
-  for (int r = 1; r < n; ++r) b->tx_only |= hs[r]->plane64 != hs[0]->plane64;
+  for (int r = 1; r < n; ++r) b->tx_only |= hs[r]->plane64 != hs[0]->plane64 || hs[r]->fold64 != hs[0]->fold64;
   auto bail = [&](int rc) {
     tdgl_batch_destroy(b);
     return rc;
@@ -3357,6 +3587,9 @@ int tdgl_batch_destroy(tdgl_batch *b) {
   batch_drop_graph(b);
   for (Block *x : {&b->tab, &b->mtab, &b->mstage, &b->dH, &b->hH, &b->drec, &b->hrec})
     if (x->p) cache_give(*x);
+  for (hipStream_t st : b->side) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : b->join) (void)hipEventDestroy(ev);
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->s) (void)hipStreamDestroy(b->s);
   delete b;
   return 0;

@@ -274,11 +274,13 @@ def run_batch(specs: List[RunSpec], work: Callable, workers: int = 8):
     return results, wall
 
 
-def run_batched(specs: List[RunSpec], work: "Cycles", group: int = 64, workers: int = 8):
+def run_batched(specs: List[RunSpec], work: "Cycles", group: int = 64, workers: int = 8, lanes: int = 1):
     """Every run of `specs` with batched kernels (tdgl_batch_*): groups of `group` runs advanced
-    together, one launch per kernel of a step over the whole group.  Host threads set up the next
-    group (stepper, state, upload, meter) while the current one runs.  Returns {run_id: the last
-    cycle's record} and the wall time."""
+    together, one launch per kernel of a step over the whole group.  With lanes = 1 host threads
+    set up the next group (stepper, state, upload, meter) while the current one runs; with
+    lanes > 1 that many groups step at once, each batch on its own stream (their launches overlap
+    on the device, so one batch's slowest blocks do not idle it), each lane setting up its next
+    group as it finishes one.  Returns {run_id: the last cycle's record} and the wall time."""
     from concurrent.futures import ThreadPoolExecutor
     from .hip import HipBatch, HipStepper
     from .meter import Voltmeter
@@ -293,28 +295,58 @@ def run_batched(specs: List[RunSpec], work: "Cycles", group: int = 64, workers: 
         dev.meter_begin(Voltmeter.of_ladder(st, work.delta_v))
         return spec, st, dev
 
+    def advance(ready):
+        batch = HipBatch([dev for _, _, dev in ready])
+        try:
+            H = np.array([sp.H for sp, _, _ in ready])
+            HJ = np.array([st.selffield(sp.J) for sp, st, _ in ready])
+            recs = None
+            for m in range(1, work.cycles + 1):
+                batch.step(work.lmax, H, HJ)
+                recs = batch.meter_cycle(m == 1)
+            return {sp.run_id: rec for (sp, _, _), rec in zip(ready, recs or [None] * len(ready))}
+        finally:
+            batch.close()
+            for _, _, dev in ready:
+                dev.close()
+
     groups = [specs[q:q + max(1, group)] for q in range(0, len(specs), max(1, group))]
     results = {}
     with _one_blas_thread(), ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
         t0 = time.perf_counter()
-        pending = [ex.submit(prep, sp) for sp in groups[0]] if groups else []
-        for k in range(len(groups)):
-            ready = [f.result() for f in pending]
-            pending = [ex.submit(prep, sp) for sp in groups[k + 1]] if k + 1 < len(groups) else []
-            batch = HipBatch([dev for _, _, dev in ready])
-            try:
-                H = np.array([sp.H for sp, _, _ in ready])
-                HJ = np.array([st.selffield(sp.J) for sp, st, _ in ready])
-                recs = None
-                for m in range(1, work.cycles + 1):
-                    batch.step(work.lmax, H, HJ)
-                    recs = batch.meter_cycle(m == 1)
-                for (sp, _, _), rec in zip(ready, recs or [None] * len(ready)):
-                    results[sp.run_id] = rec
-            finally:
-                batch.close()
-                for _, _, dev in ready:
-                    dev.close()
+        if lanes <= 1:
+            pending = [ex.submit(prep, sp) for sp in groups[0]] if groups else []
+            for k in range(len(groups)):
+                ready = [f.result() for f in pending]
+                pending = [ex.submit(prep, sp) for sp in groups[k + 1]] if k + 1 < len(groups) else []
+                results.update(advance(ready))
+        else:
+            lock = threading.Lock()
+            queue = list(range(len(groups)))
+            errors = []
+
+            def lane():
+                while True:
+                    with lock:
+                        if not queue or errors:
+                            return
+                        k = queue.pop(0)
+                    try:
+                        out = advance(list(ex.map(prep, groups[k])))
+                    except BaseException as e:  # noqa: BLE001 - handed to the caller
+                        with lock:
+                            errors.append(e)
+                        return
+                    with lock:
+                        results.update(out)
+
+            ths = [threading.Thread(target=lane) for _ in range(lanes)]
+            for t in ths:
+                t.start()
+            for t in ths:
+                t.join()
+            if errors:
+                raise errors[0]
         wall = time.perf_counter() - t0
     return results, wall
 

@@ -78,6 +78,8 @@ CASES = {
     "sep_wide160": (mgb2, dict(nx=140, ny=80, nz=4), {}, 1e-10),
     # FP64 planes of 49 x 41: the one-round kernel with 64-row planes
     "sep_mid": (mgb2, dict(nx=48, ny=40, nz=4), {}, 1e-10),
+    # 21 planes deep: the FP64 quarter z pass with several modes a wave
+    "sep_deep": (mgb2, dict(nx=40, ny=28, nz=20), {}, 1e-10),
     "pcg_exact_wide": (mgb2, dict(nx=68, ny=34, nz=4, buf=2), dict(sigma_vac=0.01, pcg_fast=False, pcg_f32=False,
                                                                   pcg_tol=1e-12), 1e-10),
     # quarters of 71 x 37 (wider than 64 and 32): the folded preconditioner's one-round x-y kernels

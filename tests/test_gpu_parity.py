@@ -447,12 +447,13 @@ def test_matrix_core_z_pass_matches_the_vector_one(name, monkeypatch):
     assert max(core.values()) <= tol, errs
 
 
-@pytest.mark.parametrize("name", ["pcg_mf1_f32", "kwt10_mf1_pcg"])
+@pytest.mark.parametrize("name", ["pcg_mf1_f32", "kwt10_mf1_pcg", "production", "sep_mid", "sep_wide160", "sep_deep"])
 def test_quarter_plane_kernel_gives_the_same_bits(name, monkeypatch):
-    """The quarter x-y transforms on k_qplane (a block per quarter plane, the bases read from
-    global memory) against k_plane_mf1 (TDGL_HIP_QPLANE=0; quarters wider than 64 and 32, so the
-    one-round kernel takes them): the same operands of each matrix instruction in the same order,
-    so eight steps give the same bits."""
+    """The quarter x-y transforms on k_qplane / k_qplane64 (a block per quarter plane, the bases
+    read from global memory) against k_plane_mf1 (TDGL_HIP_QPLANE=0; Float32 quarters wider than 64
+    and 32, FP64 quarters of any extent, so the one-round kernel takes them): the same operands of
+    each matrix instruction in the same order, so eight steps give the same bits, in the Float32
+    preconditioner and in the FP64 separable solve."""
     HipStepper = _hip()
     st, _ = make_case(name)
     s0, pcg, H, HJ = seed_state(st)
@@ -462,7 +463,8 @@ def test_quarter_plane_kernel_gives_the_same_bits(name, monkeypatch):
         monkeypatch.setenv("TDGL_HIP_QPLANE", v)
         dev = HipStepper(st)
         dev.upload(s0)
-        dev.set_pcg(pcg)
+        if pcg is not None:
+            dev.set_pcg(pcg)
         dev.step(8, H, HJ)
         assert dev.stats()["folded"] == 1
         outs.append(dev.download())
@@ -496,3 +498,30 @@ def test_folded_preconditioner_matches_full_length(name, monkeypatch):
     core = {k: v for k, v in errs.items() if not k.startswith(("U", "V"))}
     print(f"    worst {max(core, key=core.get)} {max(core.values()):.3e}", flush=True)
     assert max(core.values()) <= max(tol, 1e-12) * 10, errs
+
+
+@pytest.mark.parametrize("name", ["production", "slab", "three_aniso", "qp_two", "sep_wide", "sep_wide160", "sep_mid",
+                                  "sep_deep"])
+def test_folded_fp64_solve_matches_full_length(name, monkeypatch):
+    """The FP64 separable solve with parity-folded transforms (quarter planes with half-size bases,
+    the z pass on the quarters) against the full-length transforms (TDGL_HIP_FOLD64=0): 40 steps
+    agree within 1e-12 of each field's scale (only the grouping of the sums differs), and the
+    folded path is the one that ran."""
+    HipStepper = _hip()
+    st, tol = make_case(name)
+    s0, pcg, H, HJ = seed_state(st)
+    outs, folded = [], []
+    print(f"[gpu] folded FP64 separable solve against full length: {name}", flush=True)
+    for fv in ("0", "1"):
+        monkeypatch.setenv("TDGL_HIP_FOLD64", fv)
+        dev = HipStepper(st)
+        dev.upload(s0)
+        dev.step(40, H, HJ)
+        outs.append(dev.download())
+        folded.append(dev.stats()["folded"])
+        dev.close()
+    assert folded == [0, 1]
+    errs = compare_states(outs[1], outs[0])
+    core = {k: v for k, v in errs.items() if not k.startswith(("U", "V"))}
+    print(f"    worst {max(core, key=core.get)} {max(core.values()):.3e}", flush=True)
+    assert max(core.values()) <= 1e-12, errs
