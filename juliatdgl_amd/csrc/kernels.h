@@ -102,6 +102,7 @@ struct StageArgs {
   double nGm, ng2;
   NoiseArgs nz;
   double *Om2;                 // k_frame_smooth: the rate between the two sweeps
+  int xcd;  // the blocks to the XCDs in contiguous runs of (x, y, k) order (each block's nodes unchanged)
 };
 hipError_t launch_stage(const Geo &g, const StageArgs &a, hipStream_t s);
 // With noise and the rotating frame, after the first stage (FRAME_RAW): the two low-pass sweeps
@@ -145,6 +146,7 @@ struct FieldArgs {
   int nkick;
   double kax, kay, kaz;
   NoiseArgs nz;
+  int xcd;  // the blocks to the XCDs in contiguous runs of (x, y, k) order (each block's nodes unchanged)
 };
 hipError_t launch_field(const Geo &g, const FieldArgs &a, hipStream_t s);
 int field_blocks(const Geo &g);
@@ -163,6 +165,7 @@ struct CurlArgs {
   const double *hold[3], *hnew[3];
   const int *ofl;
   int every;
+  int xcd;  // the blocks to the XCDs in contiguous runs of (x, y, k) order (each block's nodes unchanged)
 };
 hipError_t launch_curl(const Geo &g, const CurlArgs &a, hipStream_t s);
 

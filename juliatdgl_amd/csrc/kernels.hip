@@ -32,14 +32,27 @@ struct Blk {
 __device__ __forceinline__ Blk hw_blk() {
   return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z};
 }
-// The run of a batched launch's block (the last r with off[r] <= block: runs of no blocks are
+// the hardware block index to the block's place in a logical order (the plane kernels' (m tile,
+// plane, component), the stencils' (x, y, k)): the hardware deals blocks to the 8 XCDs in turn,
+// XCD x receives a contiguous run of that order, so blocks that share halo planes share an L2
+__device__ __forceinline__ int xcd_run(int hw, int nb) {
+  const int q = nb / 8, r = nb % 8, x = hw % 8, s = hw / 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + s;
+}
+// hw_blk with the blocks in XCD runs of the (x, y, z) order
+__device__ __forceinline__ Blk hw_blk_xcd() {
+  const int nx = gridDim.x, ny = gridDim.y, nz = gridDim.z;
+  const int l = xcd_run((int)blockIdx.x + nx * ((int)blockIdx.y + ny * (int)blockIdx.z), nx * ny * nz);
+  return Blk{l % nx, (l / nx) % ny, l / (nx * ny), nx, ny, nz};
+}
+// The run of a batched launch's block b (the last r with off[r] <= b: runs of no blocks are
 // passed over) and the block's place in that run's grid
 struct BatchIdx {
   int r;
   Blk b;
 };
-__device__ __forceinline__ BatchIdx batch_idx(const BatchTab &t) {
-  const int b = blockIdx.x;
+__device__ __forceinline__ BatchIdx batch_idx(const BatchTab &t, int b = -1) {
+  if (b < 0) b = blockIdx.x;
   int lo = 0, hi = t.n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -318,9 +331,10 @@ __device__ __forceinline__ void stage_body(const Geo &g, const StageArgs &a, con
   a.D[c] = stage_tail(g, a, i, j, k, c, p, om);
 }
 
-__global__ void __launch_bounds__(256) k_stage(Geo g, StageArgs a) { stage_body(g, a, hw_blk()); }
+__global__ void __launch_bounds__(256) k_stage(Geo g, StageArgs a) { stage_body(g, a, a.xcd ? hw_blk_xcd() : hw_blk()); }
+// batched: the whole launch's blocks in XCD runs, so each XCD takes whole runs' neighbouring planes
 __global__ void __launch_bounds__(256) k_stage_b(BatchTab t) {
-  const BatchIdx q = batch_idx(t);
+  const BatchIdx q = batch_idx(t, xcd_run((int)blockIdx.x, (int)gridDim.x));
   stage_body(t.g[q.r], ((const StageArgs *)t.args)[q.r], q.b);
 }
 
@@ -509,9 +523,9 @@ __device__ __forceinline__ void field_body(const Geo &g, const FieldArgs &a, con
   }
 }
 
-__global__ void __launch_bounds__(256) k_field(Geo g, FieldArgs a) { field_body(g, a, hw_blk()); }
+__global__ void __launch_bounds__(256) k_field(Geo g, FieldArgs a) { field_body(g, a, a.xcd ? hw_blk_xcd() : hw_blk()); }
 __global__ void __launch_bounds__(256) k_field_b(BatchTab t) {
-  const BatchIdx q = batch_idx(t);
+  const BatchIdx q = batch_idx(t, xcd_run((int)blockIdx.x, (int)gridDim.x));
   field_body(t.g[q.r], ((const FieldArgs *)t.args)[q.r], q.b);
 }
 
@@ -629,9 +643,9 @@ __device__ __forceinline__ void curl_body(const Geo &g, const CurlArgs &a, const
   }
 }
 
-__global__ void __launch_bounds__(256) k_curl(Geo g, CurlArgs a) { curl_body(g, a, hw_blk()); }
+__global__ void __launch_bounds__(256) k_curl(Geo g, CurlArgs a) { curl_body(g, a, a.xcd ? hw_blk_xcd() : hw_blk()); }
 __global__ void __launch_bounds__(256) k_curl_b(BatchTab t) {
-  const BatchIdx q = batch_idx(t);
+  const BatchIdx q = batch_idx(t, xcd_run((int)blockIdx.x, (int)gridDim.x));
   curl_body(t.g[q.r], ((const CurlArgs *)t.args)[q.r], q.b);
 }
 
@@ -1525,12 +1539,7 @@ static size_t plane1_lds(int xc, int tj, int mt, int es) {
 constexpr size_t PLANE1_LDS_MAX = 160 * 1024;
 constexpr int PLANE1_XC = 5, PLANE1_TJ = 3;  // planes up to 160 x 96
 
-// the hardware block index to the block's place in (m tile, plane, component) order: the
-// hardware deals blocks to the 8 XCDs in turn, XCD x receives a contiguous run of that order
-__device__ __forceinline__ int xcd_run(int hw, int nb) {
-  const int q = nb / 8, r = nb % 8, x = hw % 8, s = hw / 8;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + s;
-}
+
 
 // sum_parts in a block of eight wavefronts: the first four load and sum as a block of RED_T
 // threads does (the same value, bit for bit, as every other reader of the partials)

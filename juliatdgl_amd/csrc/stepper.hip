@@ -155,6 +155,7 @@ struct tdgl_handle {
   int HpL[2][2] = {};          // their unpadded order
   int qplane = 1;              // TDGL_HIP_QPLANE=0: the quarter x-y transforms on k_plane_mf1 / k_plane_mf
   int zq_waves = 8;        // the quarter z pass: 4 or 8 waves a block on the vector ALUs, 0 on the matrix cores (k_zmodes_mf)
+  int stxcd = 1;           // the stencil kernels' blocks in XCD runs (TDGL_HIP_STENCILXCD=0: the hardware's order)
   float *zqb[4] = {};      // its padded z bases (ZQ_BASES)
   // the parity-folded FP64 separable solve (field_mode SEPARABLE): the same quarters, in T1 and T2,
   // the FP64 half-size bases [axis][0=V,1=U][parity][0=fwd,1=inv] and padded z bases
@@ -524,6 +525,7 @@ NoiseArgs noise_args(const tdgl_handle *h) {
 
 StageArgs stage_base(tdgl_handle *h, int mu, const tdgl_bandcoef &cf) {
   StageArgs a{};
+  a.xcd = h->stxcd;
   for (int nu = 0; nu < h->d.nbands; ++nu) a.G[nu] = h->G[nu];
   for (int c = 0; c < 3; ++c) {
     a.V[c] = h->V[c];
@@ -1715,6 +1717,7 @@ int step_once(tdgl_handle *h) {
   if (d.qp_mode != TDGL_QP_NONE) CHK(qp_potential(h));
   CHK(update_bands(h));
   FieldArgs fa{};
+  fa.xcd = h->stxcd;
   for (int mu = 0; mu < d.nbands; ++mu) fa.G[mu] = h->G[mu];
   for (int c = 0; c < 3; ++c) {
     fa.V[c] = h->V[c];
@@ -1757,6 +1760,7 @@ int step_once(tdgl_handle *h) {
     CHK(implicit_pcg(h));
   }
   CurlArgs ca{};
+  ca.xcd = h->stxcd;
   for (int c = 0; c < 3; ++c) {
     ca.ph[c] = h->ph[c];
     ca.U[c] = h->U[c];
@@ -2335,6 +2339,8 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     h->qplane = qp && qp[0] == '0' ? 0 : 1;
     // TDGL_HIP_ZQ_WAVES=4 or 8: the quarter z pass on the vector ALUs with that many waves a block
     // (8 by default), 0: on the matrix cores
+    const char *sx = std::getenv("TDGL_HIP_STENCILXCD");
+    h->stxcd = sx && sx[0] == '0' ? 0 : 1;
     const char *zw = std::getenv("TDGL_HIP_ZQ_WAVES");
     if (zw && (std::atoi(zw) == 0 || std::atoi(zw) == 4 || std::atoi(zw) == 8)) h->zq_waves = std::atoi(zw);
     const char *g1 = std::getenv("TDGL_HIP_PLANE1");

@@ -389,7 +389,8 @@ def test_pcg_budget_settles_on_a_current_ramp():
 
 
 VARIANTS = [("TDGL_HIP_ZCOL", "1", "0"), ("TDGL_HIP_PAPTILE", "0", "1"), ("TDGL_HIP_ZQ_WAVES", "8", "4"),
-            ("TDGL_HIP_BOXXCD", "0", "1"), ("TDGL_HIP_ZSWAP", "0", "1"), ("TDGL_HIP_DPAL", "0", "1")]
+            ("TDGL_HIP_BOXXCD", "0", "1"), ("TDGL_HIP_ZSWAP", "0", "1"), ("TDGL_HIP_DPAL", "0", "1"),
+            ("TDGL_HIP_STENCILXCD", "0", "1")]
 
 
 @pytest.mark.parametrize("variant", VARIANTS, ids=lambda v: v[0])
@@ -401,9 +402,10 @@ def test_kernel_variants_give_the_same_bits(name, variant, monkeypatch):
     PCG operator (k_pcg_pap_tile) against one gather per stencil value (TDGL_HIP_PAPTILE=0), and the
     quarter z pass (k_zmodes_q) with 8 against 4 waves a block (each mode's sums in the same order),
     the box kernels' blocks in XCD runs against the hardware's order (each block's partial in the
-    same slot), the folded PCG's alternating z and zold against the copy, and the operator's D read
-    through a palette of its values against the three arrays.  Eight steps give the same bits either
-    way."""
+    same slot), the folded PCG's alternating z and zold against the copy, the operator's D read
+    through a palette of its values against the three arrays, and the stencil kernels' blocks in XCD
+    runs against the hardware's order (each block's nodes and partial slot unchanged).  Eight steps
+    give the same bits either way."""
     HipStepper = _hip()
     st, _ = make_case(name)
     s0, pcg, H, HJ = seed_state(st)
