@@ -191,6 +191,10 @@ hipError_t launch_copies(const CopyList &c, hipStream_t s);
 // SC_TOLI, SC_NBI: the relative tolerance and the starting residual norm of an inner Float32
 // solve of the refinement (d.pcg_refine)
 enum { SC_NB = 0, SC_RZ0, SC_RZ1, SC_TOL, SC_TOLI, SC_NBI, SC_COUNT };
+// classes of a step for the captured PCG's budgets (PcgArgs::cls): with the open boundary, the
+// steps since its last recomputation of h_δ (0 .. PCG_NCLS - 2, then the rest), whose solves start
+// further from their extrapolated guesses the nearer they are to it; without it, one class
+constexpr int PCG_NCLS = 6;
 enum {
   FL_NZ = 0,      // the source is not zero
   FL_ACT0,        // the iterations run while set: iteration q reads ACT[q % 2], its convergence
@@ -201,7 +205,8 @@ enum {
   FL_ITSUM,       // iterations summed over the cycle
   FL_ITMAX,       // the most iterations of one solve in the cycle
   FL_OACT,        // refinement: the FP64 residual is still above the tolerance (the rounds run while set)
-  FL_COUNT
+  FL_ITMAXC,      // the most iterations of one solve of each step class in the cycle (PCG_NCLS slots)
+  FL_COUNT = FL_ITMAXC + PCG_NCLS
 };
 // partial sums, two per block, one region per reduction (every consumer re-sums them in block
 // order: the same value in every block, no fence and no extra launch)
@@ -521,6 +526,7 @@ struct PcgArgs {
   Geo g;
   Ranges rg;
   PcgDev d;
+  int cls;  // the step's class (k_pcg_finish's slot of FL_ITMAXC)
   const double *D[3];
   double sxy, sxz, syx, syz, szx, szy, dydz, dzdx, dxdy;
   Vec3 x, r, b, z, Pold, Pnew, Q, zold, xprev;

@@ -3828,6 +3828,8 @@ __global__ void __launch_bounds__(RED_T) k_pcg_finish(PcgArgs a) {
     if (fl[FL_NZ] && fl[a.ovf_fl ? a.ovf_fl : FL_ACT0 + (a.act_slot & 1)]) fl[FL_OVF] = 1;
     fl[FL_ITSUM] += fl[FL_IT];
     if (fl[FL_IT] > fl[FL_ITMAX]) fl[FL_ITMAX] = fl[FL_IT];
+    const int c = a.cls >= 0 && a.cls < PCG_NCLS ? a.cls : 0;
+    if (fl[FL_IT] > fl[FL_ITMAXC + c]) fl[FL_ITMAXC + c] = fl[FL_IT];
   }
 }
 
@@ -3857,6 +3859,7 @@ __global__ void k_pcg_cycle_reset(PcgDev d) {
     d.fl[FL_OVF] = 0;
     d.fl[FL_ITSUM] = 0;
     d.fl[FL_ITMAX] = 0;
+    for (int c = 0; c < PCG_NCLS; ++c) d.fl[FL_ITMAXC + c] = 0;
   }
 }
 

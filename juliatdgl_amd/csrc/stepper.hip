@@ -91,9 +91,9 @@ struct HostSnap {
 // A captured run of steps is valid for the host-side pointer set it was captured with (the parity
 // of the advance's swaps), the PCG's warm-start flags and its iteration budget.
 struct GraphKey {
-  int n, parity, warm, have2, budget;
+  int n, parity, warm, have2, budget, cls;
   bool operator==(const GraphKey &o) const {
-    return n == o.n && parity == o.parity && warm == o.warm && have2 == o.have2 && budget == o.budget;
+    return n == o.n && parity == o.parity && warm == o.warm && have2 == o.have2 && budget == o.budget && cls == o.cls;
   }
 };
 struct CachedGraph {
@@ -104,7 +104,7 @@ struct CachedGraph {
   HostSnap end{};        // the host-side pointers and flags after its steps
   long long used = 0;    // the last use (the least recently used entry is replaced)
 };
-constexpr int GRAPH_CACHE = 4;
+constexpr int GRAPH_CACHE = 16;  // the one-step graphs of the step classes take two parities each
 }  // namespace
 
 struct tdgl_handle {
@@ -202,6 +202,16 @@ struct tdgl_handle {
   bool budget_fixed = false;  // set by TDGL_HIP_BUDGET: the budget only grows (roll backs)
   int shrink_run = 0, shrink_max = 0;  // cycles in a row far under the budget, their longest solve
   int shrink_hold = 0;     // cycles after a roll back in which the budget does not shrink
+  // budgets per step class (PCG_NCLS): with the open boundary, a call's steps replay one-step graphs,
+  // each with the budget of its class (the steps since the last recomputation of h_δ, which the host
+  // follows on its copy of the device's clock); TDGL_HIP_STEPWISE=0: one budget, one graph a call
+  bool stepwise = false;
+  int cur_cls = 0;         // the class of the step being launched or captured
+  int oclock = -1;         // the host's copy of the clock's count (ofl[OF_COUNT]), read at each call
+  int *oclk_h = nullptr;   // pinned: the clock read back
+  int cbudget[PCG_NCLS] = {};
+  int cshrink_run[PCG_NCLS] = {}, cshrink_max[PCG_NCLS] = {}, cshrink_hold[PCG_NCLS] = {};
+  bool cseen[PCG_NCLS] = {};  // a step of the class has run (the others' budgets are never used)
   CachedGraph gc[GRAPH_CACHE];
   long long gtick = 0;
   long long captures = 0;  // graphs captured and instantiated
@@ -1169,6 +1179,7 @@ int sep_last_parts(tdgl_handle *h, bool f32) {
 PcgArgs pcg_args(tdgl_handle *h) {
   const tdgl_desc &d = h->d;
   PcgArgs a{};
+  a.cls = h->cur_cls;
   a.g = h->g;
   a.rg = h->rg;
   a.d = h->pd;
@@ -1387,7 +1398,8 @@ int implicit_pcg(tdgl_handle *h) {
   }
   int q = 0;
   if (h->capturing) {
-    for (; q < h->budget; ++q) CHK(pcg_iteration(h, q, flexible));
+    const int budget = h->stepwise ? h->cbudget[h->cur_cls] : h->budget;
+    for (; q < budget; ++q) CHK(pcg_iteration(h, q, flexible));
   } else {
     int batch = h->last_iters > 0 ? h->last_iters : 4;
     for (;;) {
@@ -1873,8 +1885,20 @@ void restore(tdgl_handle *h, const HostSnap &s) {
   h->have2 = s.have2;
 }
 
+// the class of the next step, the host's copy of the clock advanced by it as k_open_clock
+// advances the device's (a recomputation when the count reaches `every`, or on the first step)
+int step_class(tdgl_handle *h) {
+  if (!h->stepwise) return 0;
+  const int c = h->oclock + 1;
+  const bool re = c >= h->d.open_every || h->oclock < 0;
+  h->oclock = re ? 0 : c;
+  return std::min(h->oclock, PCG_NCLS - 1);
+}
 int steps_eager(tdgl_handle *h, int n) {
-  for (int q = 0; q < n; ++q) CHK(step_once(h));
+  for (int q = 0; q < n; ++q) {
+    if (h->stepwise && !h->capturing) h->cur_cls = step_class(h);
+    CHK(step_once(h));
+  }
   return 0;
 }
 
@@ -1985,6 +2009,98 @@ int cycle_graph(tdgl_handle *h, int n) {
 }
 
 int d_maxit(tdgl_handle *h) { return h->d.pcg_maxit; }
+
+// A call of n steps of a stepwise handle: step s replays the one-step graph of its class c_s
+// (captured once per key (1, parity, warm, have2, budget of c_s, c_s)); the whole call rolls back
+// as cycle_graph's when a solve outran its budget, and the budgets of the classes whose longest
+// solve reached theirs grow.  Each class's budget comes down as the single budget does.
+int stepwise_graphs(tdgl_handle *h, int n) {
+  const HostSnap start = snap(h);
+  const int clock0 = h->oclock;
+  PL(K_VEC, launch_copies(backup_list(h, true), h->s));
+  PL(K_VEC, launch_pcg_cycle_reset(h->pd, h->s));
+  for (int q = 0; q < n; ++q) {
+    const int c = step_class(h);
+    h->cur_cls = c;
+    const GraphKey key{1, h->parity, h->warm ? 1 : 0, h->have2 ? 1 : 0, h->cbudget[c], c};
+    const HostSnap before = snap(h);
+    CachedGraph *g = nullptr;
+    for (CachedGraph &cg : h->gc)
+      if (cg.live && cg.key == key) g = &cg;
+    if (!g) {
+      g = &h->gc[0];
+      for (CachedGraph &cg : h->gc)
+        if (!cg.live || (g->live && cg.used < g->used)) g = &cg;
+      drop_graph(*g);
+      h->capturing = true;
+      HIPCHK(hipStreamBeginCapture(h->s, hipStreamCaptureModeRelaxed));
+      int rc = step_once(h);
+      hipGraph_t gph = nullptr;
+      hipError_t e = hipStreamEndCapture(h->s, &gph);
+      h->capturing = false;
+      if (rc || e != hipSuccess) {
+        if (gph) (void)hipGraphDestroy(gph);
+        restore(h, before);
+        return rc ? rc : fail(TDGL_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      }
+      g->graph = gph;
+      e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        drop_graph(*g);
+        restore(h, before);
+        return fail(TDGL_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      }
+      g->key = key;
+      g->end = snap(h);
+      g->live = true;
+      h->captures += 1;
+      restore(h, before);
+    }
+    g->used = ++h->gtick;
+    HIPCHK(hipGraphLaunch(g->exec, h->s));
+    restore(h, g->end);
+  }
+  HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+  const int *itc = h->flh + FL_ITMAXC;
+  if (h->flh[FL_OVF]) {
+    restore(h, start);
+    h->oclock = clock0;
+    PL(K_VEC, launch_copies(backup_list(h, false), h->s));
+    for (int c = 0; c < PCG_NCLS; ++c) {
+      if (itc[c] >= h->cbudget[c]) h->cbudget[c] = std::min(h->d.pcg_maxit, std::max(2 * h->cbudget[c], itc[c] + 4));
+      h->cshrink_run[c] = 0;
+      h->cshrink_max[c] = 0;
+      h->cshrink_hold[c] = BUDGET_HOLD;
+    }
+    h->rollbacks += 1;
+    PL(K_VEC, launch_pcg_cycle_reset(h->pd, h->s));
+    CHK(steps_eager(h, n));
+    HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
+    return 0;
+  }
+  for (int c = 0; c < PCG_NCLS; ++c) h->cseen[c] = h->cseen[c] || itc[c] > 0;
+  if (h->budget_fixed) return 0;
+  for (int c = 0; c < PCG_NCLS; ++c) {
+    if (h->cshrink_hold[c] > 0) {
+      h->cshrink_hold[c] -= 1;
+      continue;
+    }
+    if (itc[c] == 0) continue;  // no step of this class in the call
+    if (itc[c] + BUDGET_SLACK <= h->cbudget[c]) {
+      h->cshrink_max[c] = std::max(h->cshrink_max[c], itc[c]);
+      if (++h->cshrink_run[c] >= BUDGET_RUN) {
+        h->cbudget[c] = std::max(4, h->cshrink_max[c] + 3);
+        h->cshrink_run[c] = 0;
+        h->cshrink_max[c] = 0;
+      }
+    } else {
+      h->cshrink_run[c] = 0;
+      h->cshrink_max[c] = 0;
+    }
+  }
+  return 0;
+}
 
 // the most steps one captured graph holds (even, so that the replays of a long call keep the
 // parity of advance!'s swaps and reuse one graph)
@@ -2533,6 +2649,17 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
       h->refine = true;
       h->rounds = d.pcg_rounds;
     }
+    // budgets per step class with the open boundary (its recomputations of h_δ lengthen the next
+    // few solves; TDGL_HIP_STEPWISE=0: one budget and one graph a call)
+    const char *sw = std::getenv("TDGL_HIP_STEPWISE");
+    if (d.field_mode == TDGL_FIELD_PCG && d.selffield_bc == TDGL_BC_OPEN && h->ofl && !h->refine && !(sw && sw[0] == '0')) {
+      Block b{};
+      HIPCHK(hblock(sizeof(int), &b));
+      h->pins.push_back(b);
+      h->oclk_h = (int *)b.p;
+      h->stepwise = true;
+      for (int c = 0; c < PCG_NCLS; ++c) h->cbudget[c] = h->budget;  // TDGL_HIP_BUDGET's, else learnt
+    }
     return 0;
   };
   int rc = run();
@@ -2665,8 +2792,16 @@ int tdgl_download_state(tdgl_handle *h, tdgl_state *s) {
 static void pcg_account(tdgl_handle *h) {
   h->pcg_iters += h->flh[FL_ITSUM];
   h->last_iters = h->flh[FL_IT];
-  // the graph's iterations per solve: the most one solve took, and a margin
-  if (h->budget == 0) h->budget = std::min(d_maxit(h), std::max(4, h->flh[FL_ITMAX] + 3));
+  // the graph's iterations per solve: the most one solve took, and a margin (per class: the
+  // classes no step of the first call had take the overall one)
+  if (h->budget == 0) {
+    h->budget = std::min(d_maxit(h), std::max(4, h->flh[FL_ITMAX] + 3));
+    for (int c = 0; c < PCG_NCLS; ++c) {
+      const int m = h->flh[FL_ITMAXC + c];
+      h->cbudget[c] = (m > 0 && !h->budget_fixed) ? std::min(d_maxit(h), std::max(4, m + 3)) : h->budget;
+      h->cseen[c] = h->cseen[c] || m > 0;
+    }
+  }
 }
 
 int tdgl_step(tdgl_handle *h, int32_t n, double H, double HJ) {
@@ -2683,9 +2818,18 @@ int tdgl_step(tdgl_handle *h, int32_t n, double H, double HJ) {
   h->hph[1] = HJ;
   HIPCHK(hipMemcpyAsync(h->hp, h->hph, 2 * sizeof(double), hipMemcpyHostToDevice, h->s));
   HIPCHK(hipEventRecord(h->e0, h->s));
+  if (n > 0 && h->stepwise) {
+    // the device clock's count: the classes of this call's steps
+    HIPCHK(hipMemcpyAsync(h->oclk_h, h->ofl + OF_COUNT, sizeof(int), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    h->oclock = *h->oclk_h;
+  }
   if (n > 0) {
     const bool eager = h->prof || !h->graphs || (pcg && h->budget == 0);
-    if (eager) {
+    if (!eager && h->stepwise) {
+      CHK(stepwise_graphs(h, n));
+      h->graph_cycles += 1;
+    } else if (eager) {
       if (pcg) PL(K_VEC, launch_pcg_cycle_reset(h->pd, h->s));
       CHK(steps_eager(h, n));
       if (pcg) HIPCHK(hipMemcpyAsync(h->flh, h->pd.fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, h->s));
@@ -3205,6 +3349,12 @@ int tdgl_get_stats(tdgl_handle *h, tdgl_stats *st) {
   st->graph_cycles = h->graph_cycles;
   st->rollbacks = h->rollbacks;
   st->pcg_budget = h->budget;
+  if (h->stepwise) {  // the largest budget of the classes that have run (the single one is not used)
+    int m = 0;
+    for (int c = 0; c < PCG_NCLS; ++c)
+      if (h->cseen[c]) m = std::max(m, h->cbudget[c]);
+    if (m > 0) st->pcg_budget = m;
+  }
   st->captures = h->captures;
   st->folded = (h->fold || h->fold64) ? 1 : 0;
   st->steps = h->steps;

@@ -527,3 +527,37 @@ def test_folded_fp64_solve_matches_full_length(name, monkeypatch):
     core = {k: v for k, v in errs.items() if not k.startswith(("U", "V"))}
     print(f"    worst {max(core, key=core.get)} {max(core.values()):.3e}", flush=True)
     assert max(core.values()) <= 1e-12, errs
+
+
+def test_open_pcg_step_class_budgets_give_the_same_bits(monkeypatch):
+    """With the open boundary, a call's steps replay one-step graphs whose gated PCG budgets follow
+    the steps since the last recomputation of h_δ (the default), against one graph a call with one
+    budget (TDGL_HIP_STEPWISE=0) and against steps launched from the host (TDGL_HIP_GRAPH=0): eight
+    calls of 13 steps, so the recomputations fall at every offset of a call, give the same bits in
+    all three (a gated iteration past convergence changes nothing), and the class budgets stay at
+    most the single one."""
+    HipStepper = _hip()
+    st, _ = make_case("open_kwt_pcg")
+    s0, pcg, H, HJ = seed_state(st)
+    outs, stats = [], []
+    for env in ({}, {"TDGL_HIP_STEPWISE": "0"}, {"TDGL_HIP_GRAPH": "0"}):
+        print(f"[gpu] open PCG class budgets: {env or 'default'}", flush=True)
+        for k in ("TDGL_HIP_STEPWISE", "TDGL_HIP_GRAPH"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        dev = HipStepper(st)
+        dev.upload(s0)
+        dev.set_pcg(pcg)
+        for _ in range(8):
+            dev.step(13, H, HJ)
+        outs.append(dev.download())
+        stats.append(dev.stats())
+        dev.close()
+    for o in outs[1:]:
+        for (k, a), (_, b) in zip(outs[0].arrays(), o.arrays()):
+            assert np.array_equal(a, b), k
+    print(f"    budgets {stats[0]['pcg_budget']} (classes, the largest) / {stats[1]['pcg_budget']} (one); "
+          f"roll backs {stats[0]['rollbacks']} / {stats[1]['rollbacks']}", flush=True)
+    assert stats[0]["pcg_budget"] <= stats[1]["pcg_budget"]
+    assert stats[0]["rollbacks"] <= 4
