@@ -394,10 +394,10 @@ hipError_t launch_qplane(const QPass &p, const int *gate, hipStream_t s);
 bool qplane64_fits(const QPass64 &p);
 hipError_t launch_qplane64(const QPass64 &p, hipStream_t s);
 constexpr int ZQ_NZ_MAX = 64;  // z extents the quarter z pass takes (16 modes a wave at 4 waves a block)
-constexpr int ZQ64_NZ_MAX = 48;  // in FP64, with its bases in LDS beside the columns (3 nz 64 + 4 (nz² + 64) doubles)
+constexpr int ZQ64_NZ_MAX = 48;  // in FP64, with its bases in LDS beside the columns (3 nz 64 + 4 (nz² + nz + 32) doubles)
 // The quarter z pass's bases, float, each nz x nz followed by ZQ_SLACK zeros: [0] V(n, mz) at
 // n nz + mz, [1] U(n, mz - 1) at n nz + mz (zero for mz = 0 or n = nz - 1), [2] and [3] the same at
-// mz nz + n.  The slack covers the reads of the last group past nz.
+// mz nz + n.  The slack covers the reads of the last group past nz and of the prefetched row nz.
 constexpr int ZQ_SLACK = 128;
 
 // The z pass, the per-wavevector projector and the inverse z pass in one launch: a block holds

@@ -2755,7 +2755,7 @@ __device__ __forceinline__ double fma_t(double a, double b, double c) { return _
 // separable solve); block (bx, bz) of the launch's grid
 // LB: the four bases staged in LDS after the columns (zq_lds_bases elements each) and read there
 // with wave-uniform addresses (broadcasts), instead of through the scalar cache
-__host__ __device__ constexpr int zq_lds_bases(int nz) { return nz * nz + 64; }
+__host__ __device__ constexpr int zq_lds_bases(int nz) { return nz * nz + nz + 32; }
 template <int G, int MZ, class T, class A, bool LB = false>
 __device__ __forceinline__ void zmodes_q_body(const A &m, int bx, int bz) {
   extern __shared__ __align__(16) unsigned char zq_smem[];
@@ -2763,7 +2763,8 @@ __device__ __forceinline__ void zmodes_q_body(const A &m, int bx, int bz) {
   T *Cx = (T *)zq_smem, *Cy = Cx + nz * 64, *Cz = Cy + nz * 64;
   const T *zb[4] = {m.zb[0], m.zb[1], m.zb[2], m.zb[3]};
   if constexpr (LB) {
-    // every base and 64 elements of its slack (the reads of the last group past nz)
+    // every base and nz + 32 elements of its slack (the last group's reads past nz, and the
+    // prefetch of row nz)
     const int nb = zq_lds_bases(nz);
     T *Bs = Cz + nz * 64;
     for (int k = 0; k < 4; ++k) {
@@ -2803,9 +2804,28 @@ __device__ __forceinline__ void zmodes_q_body(const A &m, int bx, int bz) {
   T ax[MZ], ay[MZ], az[MZ];
 #pragma unroll
   for (int r = 0; r < MZ; ++r) ax[r] = ay[r] = az[r] = T(0);
+  // the next row of the bases is requested before this row's products (row nz, read past the last,
+  // lies in the bases' slack), so the uniform loads' latency overlaps the arithmetic
+  T vn[MZ], un[MZ];
+#pragma unroll
+  for (int r = 0; r < MZ; ++r) {
+    vn[r] = VR[m0 + r];
+    un[r] = UR[m0 + r];
+  }
   for (int n = 0; n < nz; ++n) {
     const T cx = Cx[n * 64 + p], cy = Cy[n * 64 + p], cz = Cz[n * 64 + p];
-    const T *v = VR + (size_t)n * nz + m0, *u = UR + (size_t)n * nz + m0;
+    T v[MZ], u[MZ];
+#pragma unroll
+    for (int r = 0; r < MZ; ++r) {
+      v[r] = vn[r];
+      u[r] = un[r];
+    }
+    const T *vq = VR + (size_t)(n + 1) * nz + m0, *uq = UR + (size_t)(n + 1) * nz + m0;
+#pragma unroll
+    for (int r = 0; r < MZ; ++r) {
+      vn[r] = vq[r];
+      un[r] = uq[r];
+    }
 #pragma unroll
     for (int r = 0; r < MZ; ++r) {
       ax[r] = fma_t(cx, v[r], ax[r]);
@@ -2837,9 +2857,25 @@ __device__ __forceinline__ void zmodes_q_body(const A &m, int bx, int bz) {
   T ox[MZ], oy[MZ], oz[MZ];
 #pragma unroll
   for (int r = 0; r < MZ; ++r) ox[r] = oy[r] = oz[r] = T(0);
+#pragma unroll
+  for (int r = 0; r < MZ; ++r) {
+    vn[r] = VC[m0 + r];
+    un[r] = UC[m0 + r];
+  }
   for (int qm = 0; qm < nz; ++qm) {
     const T cx = Cx[qm * 64 + p], cy = Cy[qm * 64 + p], cz = Cz[qm * 64 + p];
-    const T *v = VC + (size_t)qm * nz + m0, *u = UC + (size_t)qm * nz + m0;
+    T v[MZ], u[MZ];
+#pragma unroll
+    for (int r = 0; r < MZ; ++r) {
+      v[r] = vn[r];
+      u[r] = un[r];
+    }
+    const T *vq = VC + (size_t)(qm + 1) * nz + m0, *uq = UC + (size_t)(qm + 1) * nz + m0;
+#pragma unroll
+    for (int r = 0; r < MZ; ++r) {
+      vn[r] = vq[r];
+      un[r] = uq[r];
+    }
 #pragma unroll
     for (int r = 0; r < MZ; ++r) {
       ox[r] = fma_t(cx, v[r], ox[r]);
