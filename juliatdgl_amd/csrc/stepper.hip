@@ -208,6 +208,7 @@ struct tdgl_handle {
   bool stepwise = false;
   int cur_cls = 0;         // the class of the step being launched or captured
   int oclock = -1;         // the host's copy of the clock's count (ofl[OF_COUNT]), read at each call
+  bool clock_known = false;  // inside tdgl_step, after the read: cur_cls is the step's own class
   int *oclk_h = nullptr;   // pinned: the clock read back
   int cbudget[PCG_NCLS] = {};
   int cshrink_run[PCG_NCLS] = {}, cshrink_max[PCG_NCLS] = {}, cshrink_hold[PCG_NCLS] = {};
@@ -1527,6 +1528,9 @@ int open_update(tdgl_handle *h) {
   const tdgl_desc &d = h->d;
   const int *re = h->ofl + OF_RECOMP, *first = h->ofl + OF_FIRST;
   PL(K_CURL, launch_open_clock(h->ofl, d.open_every, h->s));
+  // a step the host knows not to recompute (the class of a step after the recomputation, its
+  // copy of the clock read at this call) leaves out the launches the clock would gate off
+  if (h->stepwise && h->clock_known && h->cur_cls != 0) return 0;
   CopyList keep{};
   for (int c = 0; c < 3; ++c) {
     keep.src[c] = h->ohnew[c];
@@ -1896,7 +1900,7 @@ int step_class(tdgl_handle *h) {
 }
 int steps_eager(tdgl_handle *h, int n) {
   for (int q = 0; q < n; ++q) {
-    if (h->stepwise && !h->capturing) h->cur_cls = step_class(h);
+    if (h->stepwise && h->clock_known && !h->capturing) h->cur_cls = step_class(h);
     CHK(step_once(h));
   }
   return 0;
@@ -2823,7 +2827,16 @@ int tdgl_step(tdgl_handle *h, int32_t n, double H, double HJ) {
     HIPCHK(hipMemcpyAsync(h->oclk_h, h->ofl + OF_COUNT, sizeof(int), hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
     h->oclock = *h->oclk_h;
+    h->clock_known = true;
   }
+  // the classes hold for this call's steps only (the tuner and the snapshots step otherwise)
+  struct ClockScope {
+    tdgl_handle *h;
+    ~ClockScope() {
+      h->clock_known = false;
+      h->cur_cls = 0;
+    }
+  } clock_scope{h};
   if (n > 0) {
     const bool eager = h->prof || !h->graphs || (pcg && h->budget == 0);
     if (!eager && h->stepwise) {
