@@ -2488,11 +2488,9 @@ __device__ __forceinline__ void fold_body(const FoldPass &f, TQ *out, const Blk 
 }
 __global__ void __launch_bounds__(256) k_fold(FoldPass f, const int *gate) {
   if (gate && !*gate) return;
-  if (f.out64)
-    fold_body<double>(f, f.out64, hw_blk());
-  else
-    fold_body<float>(f, f.out, hw_blk());
+  fold_body<float>(f, f.out, hw_blk());
 }
+__global__ void __launch_bounds__(256) k_fold64(FoldPass f) { fold_body<double>(f, f.out64, hw_blk()); }
 // batched: the FP64 fold of each run's FoldPass
 __global__ void __launch_bounds__(256) k_fold_b(BatchTab t) {
   const BatchIdx q = batch_idx(t);
@@ -2643,14 +2641,20 @@ hipError_t launch_ref_fold(const RefFold &r, hipStream_t s) {
 hipError_t launch_fold(const FoldPass &f, const int *gate, hipStream_t s) {
   if (fold_blocks(f.q) > MAX_PART) return hipErrorInvalidValue;
   if (g_rec) return (f.out64 && !gate) ? record(BK_FOLD, Geo{}, &f, sizeof f, 0, true, fold_grid(f.q), 0) : hipErrorNotSupported;
-  TDGL_LAUNCH(k_fold, fold_grid(f.q), dim3(256), 0, s, f, gate);
+  if (f.out64) {
+    if (gate) return hipErrorInvalidValue;  // the FP64 fold is never gated
+    TDGL_LAUNCH(k_fold64, fold_grid(f.q), dim3(256), 0, s, f);
+  } else {
+    TDGL_LAUNCH(k_fold, fold_grid(f.q), dim3(256), 0, s, f, gate);
+  }
   return hipGetLastError();
 }
 
 // The inverse of the fold: the element at each of the four mirror images is the sum of the four
 // quarters, the odd quarter of an axis with the sign of the image (-1 on the mirrored side); the
 // products r·z and r·zold of the values written, partials per block.
-template <class TQ>
+// F32: u.f32 (out, r and zold Float32); ADDTH: u.addth (out = th + the value); fixed per instance
+template <class TQ, bool F32, bool ADDTH>
 __device__ __forceinline__ void unfold_body(const UnfoldPass &u, const TQ *in, const Blk &b, int bid) {
   __shared__ double sh[8];
   const int tid = threadIdx.x;
@@ -2681,11 +2685,11 @@ __device__ __forceinline__ void unfold_body(const UnfoldPass &u, const TQ *in, c
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           if (!w[t]) continue;
-          rv[t] = u.f32 ? (double)((const float *)u.r[c].p)[o[t] + zk] : ((const double *)u.r[c].p)[o[t] + zk];
+          rv[t] = F32 ? (double)((const float *)u.r[c].p)[o[t] + zk] : ((const double *)u.r[c].p)[o[t] + zk];
           if (u.dot == 2)
-            zv[t] = u.f32 ? (double)((const float *)u.zold[c].p)[o[t] + zk] : ((const double *)u.zold[c].p)[o[t] + zk];
+            zv[t] = F32 ? (double)((const float *)u.zold[c].p)[o[t] + zk] : ((const double *)u.zold[c].p)[o[t] + zk];
         }
-      if (u.addth)
+      if (ADDTH)
 #pragma unroll
         for (int t = 0; t < 4; ++t)
           if (w[t]) tv[t] = ((const double *)vt.p)[ot[t] + (size_t)vt.s2 * k];
@@ -2693,10 +2697,10 @@ __device__ __forceinline__ void unfold_body(const UnfoldPass &u, const TQ *in, c
       for (int t = 0; t < 4; ++t) {
         if (!w[t]) continue;
         // Float32: the value as written (rounded), in the dots too
-        const double vw = u.f32 ? (double)(float)val[t] : val[t];
-        if (u.f32)
+        const double vw = F32 ? (double)(float)val[t] : val[t];
+        if (F32)
           ((float *)vo.p)[o[t] + zk] = (float)val[t];
-        else if (u.addth)
+        else if (ADDTH)
           ((double *)vo.p)[o[t] + zk] = tv[t] + val[t];
         else
           ((double *)vo.p)[o[t] + zk] = val[t];
@@ -2716,27 +2720,44 @@ __device__ __forceinline__ void unfold_body(const UnfoldPass &u, const TQ *in, c
     }
   }
 }
+// Float32 quarters (the preconditioner's) and FP64 ones (the separable solve's) in kernels of their
+// own: one kernel holding both would take the registers of the larger
+template <bool F32>
 __global__ void __launch_bounds__(256) k_unfold(UnfoldPass u, const int *gate) {
   if (gate && !*gate) return;
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  if (u.in64)
-    unfold_body<double>(u, u.in64, hw_blk(), bid);
-  else
-    unfold_body<float>(u, u.in, hw_blk(), bid);
+  unfold_body<float, F32, false>(u, u.in, hw_blk(), bid);
+}
+template <bool ADDTH>
+__global__ void __launch_bounds__(256) k_unfold64(UnfoldPass u) {
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  unfold_body<double, false, ADDTH>(u, u.in64, hw_blk(), bid);
 }
 // batched: the FP64 unfold of each run's UnfoldPass (no dots)
 __global__ void __launch_bounds__(256) k_unfold_b(BatchTab t) {
   const BatchIdx q = batch_idx(t);
   const UnfoldPass &u = ((const UnfoldPass *)t.args)[q.r];
-  unfold_body<double>(u, u.in64, q.b, 0);
+  unfold_body<double, false, true>(u, u.in64, q.b, 0);
 }
 
 hipError_t launch_unfold(const UnfoldPass &u, const int *gate, hipStream_t s) {
   if (fold_blocks(u.q) > MAX_PART) return hipErrorInvalidValue;
   if (g_rec)
-    return (u.in64 && !gate && !u.dot) ? record(BK_UNFOLD, Geo{}, &u, sizeof u, 0, true, fold_grid(u.q), 0)
-                                       : hipErrorNotSupported;
-  TDGL_LAUNCH(k_unfold, fold_grid(u.q), dim3(256), 0, s, u, gate);
+    return (u.in64 && u.addth && !gate && !u.dot) ? record(BK_UNFOLD, Geo{}, &u, sizeof u, 0, true, fold_grid(u.q), 0)
+                                                  : hipErrorNotSupported;
+  if (u.in64) {
+    if (gate || u.f32) return hipErrorInvalidValue;  // the FP64 unfold is never gated, its output FP64
+    if (u.addth)
+      TDGL_LAUNCH(k_unfold64<true>, fold_grid(u.q), dim3(256), 0, s, u);
+    else
+      TDGL_LAUNCH(k_unfold64<false>, fold_grid(u.q), dim3(256), 0, s, u);
+  } else {
+    if (u.addth) return hipErrorInvalidValue;  // the preconditioner writes z
+    if (u.f32)
+      TDGL_LAUNCH(k_unfold<true>, fold_grid(u.q), dim3(256), 0, s, u, gate);
+    else
+      TDGL_LAUNCH(k_unfold<false>, fold_grid(u.q), dim3(256), 0, s, u, gate);
+  }
   return hipGetLastError();
 }
 
