@@ -105,6 +105,7 @@ struct CachedGraph {
   long long used = 0;    // the last use (the least recently used entry is replaced)
 };
 constexpr int GRAPH_CACHE = 16;  // the one-step graphs of the step classes take two parities each
+constexpr int GRAPH_CACHE_CALL = 4;  // graphs of whole calls (up to GRAPH_MAX_STEPS steps each)
 }  // namespace
 
 struct tdgl_handle {
@@ -1928,15 +1929,16 @@ int destroy_graph(tdgl_handle *h) {
 constexpr int BUDGET_SLACK = 6, BUDGET_RUN = 2, BUDGET_HOLD = 4;
 int cycle_graph(tdgl_handle *h, int n) {
   const bool pcg = h->d.field_mode == TDGL_FIELD_PCG;
-  const GraphKey key{n, h->parity, h->warm ? 1 : 0, h->have2 ? 1 : 0, pcg ? h->budget : 0};
+  const GraphKey key{n, h->parity, h->warm ? 1 : 0, h->have2 ? 1 : 0, pcg ? h->budget : 0, 0};
   const HostSnap start = snap(h);
   CachedGraph *g = nullptr;
-  for (CachedGraph &c : h->gc)
-    if (c.live && c.key == key) g = &c;
+  CachedGraph *const gc0 = h->gc, *const gc1 = h->gc + GRAPH_CACHE_CALL;
+  for (CachedGraph *c = gc0; c < gc1; ++c)
+    if (c->live && c->key == key) g = c;
   if (!g) {
-    g = &h->gc[0];
-    for (CachedGraph &c : h->gc)
-      if (!c.live || (g->live && c.used < g->used)) g = &c;
+    g = gc0;
+    for (CachedGraph *c = gc0; c < gc1; ++c)
+      if (!c->live || (g->live && c->used < g->used)) g = c;
     drop_graph(*g);
     h->capturing = true;
     // relaxed: other handles, in other host threads, allocate, free and copy on their own streams
