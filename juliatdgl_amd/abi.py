@@ -86,7 +86,7 @@ class StatsC(ct.Structure):
 
 NKERNELS = 15
 TUNE_MAX = 16
-TUNE_KINDS = {1: "box_kc", 2: "plane_mt", 3: "zmodes_waves"}
+TUNE_KINDS = {1: "box_kc", 2: "plane_mt", 3: "zmodes_waves", 4: "fold", 5: "zcol"}
 
 
 class TuneCandC(ct.Structure):
@@ -95,7 +95,7 @@ class TuneCandC(ct.Structure):
 
 class TuningC(ct.Structure):
     _fields_ = [("box_kc", ct.c_int32), ("plane_mt", ct.c_int32), ("zmodes_waves", ct.c_int32),
-                ("ncand", ct.c_int32), ("cand", TuneCandC * TUNE_MAX)]
+                ("ncand", ct.c_int32), ("cand", TuneCandC * TUNE_MAX), ("fold", ct.c_int32), ("zcol", ct.c_int32)]
 
 
 class MeterC(ct.Structure):

@@ -2990,8 +2990,15 @@ int tdgl_autotune(tdgl_handle *h, double H, double HJ, tdgl_tuning *out) {
     const int rc2 = rc ? rc : tdgl_profile_get(h, &p);
     (void)tdgl_profile_enable(h, 0);
     if (rc2) return rc2;
-    const double n = (double)std::max<int64_t>(1, p.launches[cls]);
-    *us = 1e3 * (p.ms[cls] + (cls2 >= 0 ? p.ms[cls2] : 0.0)) / n;
+    if (cls < 0) {
+      // a whole PCG iteration: the kernels of the iterations (and the preconditioner's passes)
+      // over the step, per iteration (one operator launch each)
+      const double n = (double)std::max<int64_t>(1, p.launches[K_APPLYA]);
+      *us = 1e3 * (p.ms[K_APPLYA] + p.ms[K_VEC] + p.ms[K_TX32] + p.ms[K_MODES] + p.ms[K_FOLD]) / n;
+    } else {
+      const double n = (double)std::max<int64_t>(1, p.launches[cls]);
+      *us = 1e3 * (p.ms[cls] + (cls2 >= 0 ? p.ms[cls2] : 0.0)) / n;
+    }
     if (out->ncand < TDGL_TUNE_MAX) out->cand[out->ncand++] = tdgl_tune_cand{kind, value, *us};
     return 0;
   };
@@ -3042,6 +3049,40 @@ int tdgl_autotune(tdgl_handle *h, double H, double HJ, tdgl_tuning *out) {
     }
     h->zq_waves = best_zw;
   }
+  // the preconditioner's path (a whole PCG iteration each): the folds, or the full-length
+  // transforms with the z pass over x-runs or whole columns; the folds stay where a refinement
+  // needs them
+  const bool fold0 = h->fold;
+  const int zcol0 = h->zcols;
+  bool best_fold = fold0;
+  int best_zc = zcol0;
+  if (h->d.pcg_f32 && rc == 0) {
+    best = 1e300;
+    struct Path {
+      bool fold;
+      int zcol;
+    };
+    std::vector<Path> paths;
+    if (fold0) paths.push_back({true, zcol0});
+    if (!h->refine) {
+      paths.push_back({false, 0});
+      paths.push_back({false, 1});
+    }
+    for (const Path &pa : paths) {
+      if (rc) break;
+      h->fold = pa.fold;
+      h->zcols = pa.zcol;
+      double us = 0;
+      rc = pa.fold ? trial(TDGL_TUNE_FOLD, 1, -1, -1, &us) : trial(TDGL_TUNE_ZCOL, pa.zcol, -1, -1, &us);
+      if (rc == 0 && us < best) {
+        best = us;
+        best_fold = pa.fold;
+        best_zc = pa.zcol;
+      }
+    }
+    h->fold = best_fold;
+    h->zcols = best_zc;
+  }
   // the state back where it was, whatever happened
   restore(h, start);
   h->last_iters = last_iters;
@@ -3054,12 +3095,16 @@ int tdgl_autotune(tdgl_handle *h, double H, double HJ, tdgl_tuning *out) {
     h->box_kc = kc0;
     h->plane1 = mt0;
     h->zq_waves = zw0;
+    h->fold = fold0;
+    h->zcols = zcol0;
     return rc;
   }
   if (e != hipSuccess || e2 != hipSuccess) return fail(TDGL_E_DEVICE, "the tuner's restore of the state");
   out->box_kc = h->box_kc;
   out->plane_mt = h->plane1;
   out->zmodes_waves = h->zq_waves;
+  out->fold = h->fold ? 1 : 0;
+  out->zcol = h->zcols;
   return 0;
 }
 

@@ -561,3 +561,27 @@ def test_open_pcg_step_class_budgets_give_the_same_bits(monkeypatch):
           f"roll backs {stats[0]['rollbacks']} / {stats[1]['rollbacks']}", flush=True)
     assert stats[0]["pcg_budget"] <= stats[1]["pcg_budget"]
     assert stats[0]["rollbacks"] <= 4
+
+
+@pytest.mark.parametrize("name", ["pcg_fast_f32", "kwt10_pcg", "pcg_wide_f32"])
+def test_full_length_preconditioner_z_passes_give_the_same_bits(name, monkeypatch):
+    """The Float32 preconditioner's full-length path (TDGL_HIP_FOLD=0, which the tuner takes where it
+    is the faster) with its z pass over whole z-columns (TDGL_HIP_ZCOL=1) against x-runs of modes:
+    each mode's sums in the same order, so eight steps give the same bits."""
+    HipStepper = _hip()
+    st, _ = make_case(name)
+    s0, pcg, H, HJ = seed_state(st)
+    monkeypatch.setenv("TDGL_HIP_FOLD", "0")
+    outs = []
+    for zc in ("0", "1"):
+        print(f"[gpu] TDGL_HIP_FOLD=0 TDGL_HIP_ZCOL={zc}: {name}", flush=True)
+        monkeypatch.setenv("TDGL_HIP_ZCOL", zc)
+        dev = HipStepper(st)
+        dev.upload(s0)
+        dev.set_pcg(pcg)
+        dev.step(8, H, HJ)
+        assert dev.stats()["folded"] == 0
+        outs.append(dev.download())
+        dev.close()
+    for (k, a), (_, b) in zip(outs[0].arrays(), outs[1].arrays()):
+        assert np.array_equal(a, b), k

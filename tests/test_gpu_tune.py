@@ -1,8 +1,9 @@
 """tdgl_autotune: the state, the PCG history and the open boundary come back as they were after the
 tuner's timed steps, every candidate it names was timed, and the steps after a tune agree with the
 steps of an untuned handle.  The PCG box kernels' levels per block decide how the reductions' block
-partials are summed, so the two agree to the case's tolerance against the oracle, not bit for bit;
-the other two choices change no bit (test_gpu_parity's kernel-variant test).  Needs a HIP device."""
+partials are summed and the preconditioner's folds how it rounds, so the two agree to the case's
+tolerance against the oracle, not bit for bit; the m tiles, the z pass's waves and its columns or
+x-runs change no bit (test_gpu_parity's kernel-variant test).  Needs a HIP device."""
 import numpy as np
 import pytest
 
@@ -43,13 +44,21 @@ def test_tune_restores_state_and_steps_agree(name):
             assert us > 0
             timed.setdefault(kind, set()).add(value)
         assert t["box_kc"] in timed["box_kc"] and t["plane_mt"] in timed["plane_mt"]
-        if a.stats()["folded"]:
+        if "zmodes_waves" in timed:
             assert t["zmodes_waves"] in timed["zmodes_waves"]
+        # the preconditioner's path: the folds (timed where the handle has them) or the full-length
+        # transforms with one of the two z passes
+        if t["fold"]:
+            assert 1 in timed["fold"]
+        else:
+            assert t["zcol"] in timed["zcol"]
+        assert a.stats()["folded"] == t["fold"]
         a.step(12, H, HJ)
         b.step(12, H, HJ)
         errs = compare_states(a.download(), b.download())
         worst = max(errs.values())
-        print(f"[gpu]   tuned {t['box_kc'], t['plane_mt'], t['zmodes_waves']}: worst {worst:.2e}", flush=True)
+        print(f"[gpu]   tuned {t['box_kc'], t['plane_mt'], t['zmodes_waves'], t['fold'], t['zcol']}: worst {worst:.2e}",
+              flush=True)
         assert worst <= tol, errs
     finally:
         a.close()
