@@ -182,7 +182,8 @@ def bench_sweep(args) -> int:
     bat, t_bat = sweep.run_batch(specs, work, workers=args.workers)
     print(f"[bench] {args.workers} at a time: {t_bat:.2f} s", file=sys.stderr, flush=True)
     kb, t_kb = sweep.run_batched(specs, work, group=len(specs), workers=args.workers)
-    print(f"[bench] batched kernels (one launch per kernel over all runs): {t_kb:.2f} s", file=sys.stderr, flush=True)
+    print(f"[bench] batched kernels (one launch per kernel over all runs of a solve path): {t_kb:.2f} s", file=sys.stderr,
+          flush=True)
     kb16, t_kb16 = sweep.run_batched(specs, work, group=16, workers=args.workers)
     print(f"[bench] batched kernels, groups of 16 set up while the last runs: {t_kb16:.2f} s", file=sys.stderr, flush=True)
     kbl, t_kbl = sweep.run_batched(specs, work, group=16, workers=args.workers, lanes=4)

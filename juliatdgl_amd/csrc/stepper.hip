@@ -2541,9 +2541,20 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
       }
     }
     // the parity-folded FP64 separable solve (TDGL_HIP_FOLD64=0: the full-length transforms)
+    // the folds by default where the full-length FP64 planes are too large for the one-round
+    // kernel (wider than 160 x 96: they would fall to the vector ALUs); on smaller planes the two
+    // launches the folds add cost more than their halved products save (TDGL_HIP_FOLD64=1: always,
+    // =0: never).  The separable handles' z pass takes whole z-columns (TDGL_HIP_ZCOL=0: x-runs).
     const char *gf64 = std::getenv("TDGL_HIP_FOLD64");
-    if (d.field_mode == TDGL_FIELD_SEPARABLE && h->plane64 && !(gf64 && gf64[0] == '0') && d.Nz - 1 >= 2 &&
-        d.Nz - 1 <= ZQ64_NZ_MAX) {
+    int l0max = 0, l1max = 0;
+    for (int c = 0; c < 3; ++c) {
+      l0max = std::max(l0max, rg.e[c][0]);
+      l1max = std::max(l1max, rg.e[c][1]);
+    }
+    const bool fold64_pays = l0max > 160 || l1max > 96;
+    const bool fold64_on = gf64 ? gf64[0] == '1' : fold64_pays;
+    if (d.field_mode == TDGL_FIELD_SEPARABLE && !(zc && zc[0] == '0')) h->zcols = 1;
+    if (d.field_mode == TDGL_FIELD_SEPARABLE && h->plane64 && fold64_on && d.Nz - 1 >= 2 && d.Nz - 1 <= ZQ64_NZ_MAX) {
       int qe[12][3];
       const Quarters q = quarters_of(rg, qe);
       if (plane_fits<double>(qe, 12) && fold_blocks(q) <= MAX_PART && (size_t)q.total <= h->t12cap) {

@@ -276,7 +276,8 @@ def run_batch(specs: List[RunSpec], work: Callable, workers: int = 8):
 
 def run_batched(specs: List[RunSpec], work: "Cycles", group: int = 64, workers: int = 8, lanes: int = 1):
     """Every run of `specs` with batched kernels (tdgl_batch_*): groups of `group` runs advanced
-    together, one launch per kernel of a step over the whole group.  With lanes = 1 host threads
+    together, one launch per kernel of a step over the group's runs of each path of the separable
+    solve (with or without the parity folds: one batch per path).  With lanes = 1 host threads
     set up the next group (stepper, state, upload, meter) while the current one runs; with
     lanes > 1 that many groups step at once, each batch on its own stream (their launches overlap
     on the device, so one batch's slowest blocks do not idle it), each lane setting up its next
@@ -295,18 +296,32 @@ def run_batched(specs: List[RunSpec], work: "Cycles", group: int = 64, workers: 
         dev.meter_begin(Voltmeter.of_ladder(st, work.delta_v))
         return spec, st, dev
 
-    def advance(ready):
-        batch = HipBatch([dev for _, _, dev in ready])
+    def advance_part(part):
+        batch = HipBatch([dev for _, _, dev in part])
         try:
-            H = np.array([sp.H for sp, _, _ in ready])
-            HJ = np.array([st.selffield(sp.J) for sp, st, _ in ready])
+            H = np.array([sp.H for sp, _, _ in part])
+            HJ = np.array([st.selffield(sp.J) for sp, st, _ in part])
             recs = None
             for m in range(1, work.cycles + 1):
                 batch.step(work.lmax, H, HJ)
                 recs = batch.meter_cycle(m == 1)
-            return {sp.run_id: rec for (sp, _, _), rec in zip(ready, recs or [None] * len(ready))}
+            return {sp.run_id: rec for (sp, _, _), rec in zip(part, recs or [None] * len(part))}
         finally:
             batch.close()
+
+    def advance(ready):
+        # one batch per path of the separable solve (the parity folds, which a handle takes on planes
+        # too large for the full-length kernel, or not): a batch of one path launches what each run's
+        # own step launches, so every record equals the run's alone
+        out = {}
+        try:
+            parts = {}
+            for r in ready:
+                parts.setdefault(r[2].stats()["folded"], []).append(r)
+            for part in parts.values():
+                out.update(advance_part(part))
+            return out
+        finally:
             for _, _, dev in ready:
                 dev.close()
 

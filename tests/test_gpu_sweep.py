@@ -56,13 +56,18 @@ def test_noisy_runs_end_at_the_same_first_slip_served():
         assert done[q]["result"] == alone[q]
 
 
-def test_batched_kernels_match_the_runs_alone():
+@pytest.mark.parametrize("fold64", ["default", "1"])
+def test_batched_kernels_match_the_runs_alone(fold64, monkeypatch):
     """tdgl_batch_*: six uneven runs advanced by one launch per kernel over all of them give the
     records of their own steppers to the bit (the batch launches the instances each run's own step
-    takes), in a batch of one as in the batch of six."""
+    takes), in a batch of one as in the batch of six; with the full-length FP64 solve these small
+    runs take by default, and with the parity folds (TDGL_HIP_FOLD64=1: the batched fold, quarter
+    planes, quarter z pass and unfold)."""
     from juliatdgl_amd.sweep import run_batched
+    if fold64 != "default":
+        monkeypatch.setenv("TDGL_HIP_FOLD64", fold64)
     specs = _specs()
-    print("[gpu] batched kernels", flush=True)
+    print(f"[gpu] batched kernels, FP64 folds {fold64}", flush=True)
     alone, _ = run_batch(specs, Cycles(cycles=3), workers=1)
     mix, t6 = run_batched(specs, Cycles(cycles=3), group=6)
     one, _ = run_batched(specs, Cycles(cycles=3), group=1)
