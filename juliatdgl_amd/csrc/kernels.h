@@ -356,8 +356,12 @@ struct ZModesQ {
   int nx, ny, nz;
   double a;
   int waves;             // k_zmodes_q: waves per block, each a group of z modes (4 or 8); 0: k_zmodes_mf (matrix cores)
+  const float *zmf;      // k_zmodes_mf's padded image of the bases (zmodes_mf_image(nz) floats)
 };
 hipError_t launch_zmodes_q(const ZModesQ &m, const int *gate, hipStream_t s);
+// the floats of k_zmodes_mf's image: Vt (mz, n), Ut (mz, n), V (n, mz), U (n, mz), each NZP = 16 ceil(nz / 16)
+// rows of NZP + 4 (zero beyond nz)
+int zmodes_mf_image(int nz);
 // the same in FP64 (the FP64 separable solve): the bases by rows and columns as ZQ_BASES, double
 struct ZModesQ64 {
   double *modes;

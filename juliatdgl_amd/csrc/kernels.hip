@@ -2948,61 +2948,60 @@ static hipError_t zmodes_q_g(const ZModesQ &m, const int *gate, hipStream_t s) {
 }
 
 // The z pass and projector of k_zmodes_q on v_mfma_f32_16x16x4f32.  A block of 4 waves owns 64
-// modes of one parity quarter (the columns, flattened along mm as in k_zmodes_q); wave w takes the
-// columns 16 w .. 16 w + 15.  With NZP = 16 NT >= nz (zero rows and columns beyond nz):
+// columns of one parity quarter (flattened along mm as in k_zmodes_q); wave w takes the columns
+// 16 w .. 16 w + 15.  With NZP = 16 NT >= nz (zero rows and columns beyond nz):
 //   forward  X~(mz, col) = Σ_n V(n, mz) X(n, col)  (and Y with V, Z with U(n, mz − 1)),
 //   inverse  x(n, col)   = Σ_mz V(n, mz) X~(mz, col),
-// each a 16 x 16 tile per (row tile, this wave's columns) with A the basis and B the columns, both
-// from LDS with k contiguous: the instruction u of a 16-k group g takes k = 16 g + 4 lk + u (lane
-// l = 16 lk + lr), so one 16-byte LDS read gives a lane its four k-steps.  The forward tiles of X,
-// Y and Z of the same (mz, col) sit in the same lane and register, where the projector runs; the
-// projected modes go back to LDS (over the columns) as the inverse's B.  LDS rows are NZP + 4 words
-// (4 mod 8: 16 lanes' 16-byte reads are bank-conflict free).  The sums are grouped four products at
-// a time (as the plane kernels'), not in k_zmodes_q's order: the results differ from it by Float32
-// rounding.
+// each a 16 x 16 tile per (row tile, this wave's columns) with A the basis, from LDS, k contiguous:
+// the instruction u of a 16-k group g takes k = 16 g + 4 lk + u (lane l = 16 lk + lr), so one
+// 16-byte LDS read gives a lane its four k-steps.  The B operands never pass through LDS: the
+// forward's are the lane's own loads of its column at n = 16 g + 4 lk + u, and a forward tile
+// leaves in the lane exactly the modes mz = 16 g + 4 lk + u the inverse's instruction (g, u) takes
+// as B, so the projected tiles are the inverse's operands where they lie.  The forward tiles of X,
+// Y and Z of the same (mz, col) sit in the same lane and register, where the projector runs.  The
+// block copies the bases once from their padded image (m.zmf: Vt, Ut, V, U, each NZP rows of
+// NZP + 4 words, 4 mod 8: 16 lanes' 16-byte reads are bank-conflict free) while its columns load.
+// The sums are grouped four products at a time (as the plane kernels'), not in k_zmodes_q's order:
+// the results differ from it by Float32 rounding.
 template <int NT>
 __global__ void __launch_bounds__(256) k_zmodes_mf(ZModesQ m, const int *gate) {
   if (gate && !*gate) return;
   typedef v4f V4;
   constexpr int NZP = 16 * NT, S = NZP + 4;
   extern __shared__ __align__(16) unsigned char zm_smem[];
-  float *B4 = (float *)zm_smem;              // 4 bases [row][k]: Vt (mz, n), Ut (mz, n), V (n, mz), U (n, mz)
-  float *Ct = B4 + 4 * NZP * S;              // [comp][col][k]: the columns by n, later the modes by mz
+  float *B4 = (float *)zm_smem;  // 4 bases [row][k]: Vt (mz, n), Ut (mz, n), V (n, mz), U (n, mz)
   const int nx = m.nx, ny = m.ny, nz = m.nz, tid = threadIdx.x;
   const Quarters &q = m.q;
   const int px = (int)(blockIdx.z >> 1), py = (int)(blockIdx.z & 1);
-  // the bases, zero beyond nz (the padded copies: [0] V(n, mz), [1] U(n, mz − 1), [2], [3] by columns)
-  for (int e = tid; e < 4 * NZP * NZP; e += 256) {
-    const int b = e / (NZP * NZP), r = (e / NZP) % NZP, k = e % NZP;
-    // b 0: Vt[mz][n] = zb[2][mz nz + n]; 1: Ut = zb[3]; 2: V[n][mz] = zb[0][n nz + mz]; 3: U = zb[1]
-    const float *src = m.zb[b == 0 ? 2 : b == 1 ? 3 : b == 2 ? 0 : 1];
-    B4[b * NZP * S + r * S + k] = (r < nz && k < nz) ? src[(size_t)r * nz + k] : 0.0f;
-  }
-  // the columns: thread (col, g4) loads n = g4, g4 + 4, ... of X, Y, Z of its column
-  const int col = tid & 63, g4 = tid >> 6;
-  {
-    const int f = blockIdx.x * 64 + col;
-    const int mm = f % q.h[2][0], qq = f / q.h[2][0];
-    const int mx = 2 * mm + px, my = 2 * qq + py;
-    const bool live = qq < q.h[2][1] && mx < nx && my < ny;
-    const bool hx = live && mx > 0, hy = live && my > 0;
-    const int xi = min(max(mm - 1 + px, 0), q.h[0][0] - 1), xj = min(qq, q.h[0][1] - 1);
-    const int yi = min(mm, q.h[1][0] - 1), yj = min(max(qq - 1 + py, 0), q.h[1][1] - 1);
-    const int zi = min(mm, q.h[2][0] - 1), zj = min(qq, q.h[2][1] - 1);
-    const float *colx = m.modes + q.off[0][2 * (1 - px) + py] + xi + (size_t)q.h[0][0] * xj;
-    const float *coly = m.modes + q.off[1][2 * px + (1 - py)] + yi + (size_t)q.h[1][0] * yj;
-    const float *colz = m.modes + q.off[2][2 * px + py] + zi + (size_t)q.h[2][0] * zj;
-    const size_t stx = (size_t)q.h[0][0] * q.h[0][1], sty = (size_t)q.h[1][0] * q.h[1][1], stz = (size_t)q.h[2][0] * q.h[2][1];
-    for (int n = g4; n < NZP; n += 4) {
-      const int nn = min(n, nz - 1);
-      const float vx = colx[stx * nn], vy = coly[sty * nn], vz = colz[stz * min(n, nz - 2)];
-      Ct[(0 * 64 + col) * S + n] = (hx && n < nz) ? vx : 0.0f;
-      Ct[(1 * 64 + col) * S + n] = (hy && n < nz) ? vy : 0.0f;
-      Ct[(2 * 64 + col) * S + n] = (live && n < nz - 1) ? vz : 0.0f;
-    }
-  }
-  __syncthreads();
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, lr = l & 15, lk = l >> 4;
+  // this lane's column (lanes 16 consecutive columns: runs along mm) and its three component columns
+  const int f = blockIdx.x * 64 + 16 * w + lr;
+  const int mm = f % q.h[2][0], qq = f / q.h[2][0];
+  const int mx = 2 * mm + px, my = 2 * qq + py;
+  const bool live = qq < q.h[2][1] && mx < nx && my < ny;
+  const bool hx = live && mx > 0, hy = live && my > 0;
+  const int xi = min(max(mm - 1 + px, 0), q.h[0][0] - 1), xj = min(qq, q.h[0][1] - 1);
+  const int yi = min(mm, q.h[1][0] - 1), yj = min(max(qq - 1 + py, 0), q.h[1][1] - 1);
+  const int zi = min(mm, q.h[2][0] - 1), zj = min(qq, q.h[2][1] - 1);
+  float *colx = m.modes + q.off[0][2 * (1 - px) + py] + xi + (size_t)q.h[0][0] * xj;
+  float *coly = m.modes + q.off[1][2 * px + (1 - py)] + yi + (size_t)q.h[1][0] * yj;
+  float *colz = m.modes + q.off[2][2 * px + py] + zi + (size_t)q.h[2][0] * zj;
+  const size_t stx = (size_t)q.h[0][0] * q.h[0][1], sty = (size_t)q.h[1][0] * q.h[1][1], stz = (size_t)q.h[2][0] * q.h[2][1];
+  // the forward's B operands: n = 16 g + 4 lk + u of the three columns (clamped reads, zero where absent)
+  V4 Bx[NT], By[NT], Bz[NT];
+#pragma unroll
+  for (int g = 0; g < NT; ++g)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int n = 16 * g + 4 * lk + u, nn = min(n, nz - 1);
+      const float vx = colx[stx * nn], vy = coly[sty * nn], vz = colz[stz * min(n, nz - 2)];
+      Bx[g][u] = (hx && n < nz) ? vx : 0.0f;
+      By[g][u] = (hy && n < nz) ? vy : 0.0f;
+      Bz[g][u] = (live && n < nz - 1) ? vz : 0.0f;
+    }
+  // the bases: a straight copy of the padded image
+  for (int e = tid; e < NZP * S; e += 256) ((V4 *)B4)[e] = ((const V4 *)m.zmf)[e];
+  __syncthreads();
   const float *Vt = B4, *Ut = B4 + NZP * S, *Vr = B4 + 2 * NZP * S, *Ur = B4 + 3 * NZP * S;
   // forward: acc[c][t] is the tile of rows mz = 16 t .. 16 t + 15 and this wave's 16 columns
   V4 acc[3][NT];
@@ -3010,12 +3009,9 @@ __global__ void __launch_bounds__(256) k_zmodes_mf(ZModesQ m, const int *gate) {
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[c][t] = V4{0, 0, 0, 0};
-  const int cb = (16 * w + lr) * S + 4 * lk;  // this lane's column row in Ct, its k offset
 #pragma unroll
   for (int g = 0; g < NT; ++g) {
-    V4 Bc[3], Av[NT], Au[NT];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) Bc[c] = *(const V4 *)(Ct + c * 64 * S + cb + 16 * g);
+    V4 Av[NT], Au[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       Av[t] = *(const V4 *)(Vt + (16 * t + lr) * S + 16 * g + 4 * lk);
@@ -3025,18 +3021,13 @@ __global__ void __launch_bounds__(256) k_zmodes_mf(ZModesQ m, const int *gate) {
     for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Av[t][u], Bc[0][u], acc[0][t], 0, 0, 0);
-        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Av[t][u], Bc[1][u], acc[1][t], 0, 0, 0);
-        acc[2][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Au[t][u], Bc[2][u], acc[2][t], 0, 0, 0);
+        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Av[t][u], Bx[g][u], acc[0][t], 0, 0, 0);
+        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Av[t][u], By[g][u], acc[1][t], 0, 0, 0);
+        acc[2][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Au[t][u], Bz[g][u], acc[2][t], 0, 0, 0);
       }
   }
   // the projector on this lane's modes: element r of tile t is mz = 16 t + 4 lk + r, column 16 w + lr
   {
-    const int f = blockIdx.x * 64 + 16 * w + lr;
-    const int mm = f % q.h[2][0], qq = f / q.h[2][0];
-    const int mx = 2 * mm + px, my = 2 * qq + py;
-    const bool live = qq < q.h[2][1] && mx < nx && my < ny;
-    const bool hx = live && mx > 0, hy = live && my > 0;
     const float k1 = live ? m.sx[mx] : 0.0f, k2 = live ? m.sy[my] : 0.0f;
 #pragma unroll
     for (int t = 0; t < NT; ++t)
@@ -3054,23 +3045,16 @@ __global__ void __launch_bounds__(256) k_zmodes_mf(ZModesQ m, const int *gate) {
         acc[2][t][r] = z1;
       }
   }
-  __syncthreads();  // every wave is done with the columns: the modes go over them
-  // this lane's four rows mz = 16 t + 4 lk .. + 3 of its column, one 16-byte write per tile
+  // inverse: out[c][t] is the tile of rows n = 16 t .. 16 t + 15; the instruction (g, u) takes the
+  // modes mz = 16 g + 4 lk + u, element u of the lane's projected tile g
+  V4 out[3][NT];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int t = 0; t < NT; ++t) *(V4 *)(Ct + c * 64 * S + (16 * w + lr) * S + 16 * t + 4 * lk) = acc[c][t];
-  __syncthreads();
-  // inverse: acc[c][t] is the tile of rows n = 16 t .. 16 t + 15
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[c][t] = V4{0, 0, 0, 0};
+    for (int t = 0; t < NT; ++t) out[c][t] = V4{0, 0, 0, 0};
 #pragma unroll
   for (int g = 0; g < NT; ++g) {
-    V4 Bc[3], Av[NT], Au[NT];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) Bc[c] = *(const V4 *)(Ct + c * 64 * S + cb + 16 * g);
+    V4 Av[NT], Au[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       Av[t] = *(const V4 *)(Vr + (16 * t + lr) * S + 16 * g + 4 * lk);
@@ -3080,44 +3064,36 @@ __global__ void __launch_bounds__(256) k_zmodes_mf(ZModesQ m, const int *gate) {
     for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Av[t][u], Bc[0][u], acc[0][t], 0, 0, 0);
-        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Av[t][u], Bc[1][u], acc[1][t], 0, 0, 0);
-        acc[2][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Au[t][u], Bc[2][u], acc[2][t], 0, 0, 0);
+        out[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Av[t][u], acc[0][g][u], out[0][t], 0, 0, 0);
+        out[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Av[t][u], acc[1][g][u], out[1][t], 0, 0, 0);
+        out[2][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Au[t][u], acc[2][g][u], out[2][t], 0, 0, 0);
       }
   }
-  // the positions back into the quarters: element r of tile t is n = 16 t + 4 lk + r of column
-  // 16 w + lr (lanes 16 consecutive columns: runs along mm)
-  {
-    const int f = blockIdx.x * 64 + 16 * w + lr;
-    const int mm = f % q.h[2][0], qq = f / q.h[2][0];
-    const int mx = 2 * mm + px, my = 2 * qq + py;
-    const bool live = qq < q.h[2][1] && mx < nx && my < ny;
-    if (!live) return;
-    const bool hx = mx > 0, hy = my > 0;
-    const int xi = min(max(mm - 1 + px, 0), q.h[0][0] - 1), xj = min(qq, q.h[0][1] - 1);
-    const int yi = min(mm, q.h[1][0] - 1), yj = min(max(qq - 1 + py, 0), q.h[1][1] - 1);
-    const int zi = min(mm, q.h[2][0] - 1), zj = min(qq, q.h[2][1] - 1);
-    float *colx = m.modes + q.off[0][2 * (1 - px) + py] + xi + (size_t)q.h[0][0] * xj;
-    float *coly = m.modes + q.off[1][2 * px + (1 - py)] + yi + (size_t)q.h[1][0] * yj;
-    float *colz = m.modes + q.off[2][2 * px + py] + zi + (size_t)q.h[2][0] * zj;
-    const size_t stx = (size_t)q.h[0][0] * q.h[0][1], sty = (size_t)q.h[1][0] * q.h[1][1], stz = (size_t)q.h[2][0] * q.h[2][1];
+  // the positions back into the quarters: element r of tile t is n = 16 t + 4 lk + r
+  if (!live) return;
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+  for (int t = 0; t < NT; ++t)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int n = 16 * t + 4 * lk + r;
-        if (n >= nz) continue;
-        if (hx) colx[stx * n] = acc[0][t][r];
-        if (hy) coly[sty * n] = acc[1][t][r];
-        if (n < nz - 1) colz[stz * n] = acc[2][t][r];
-      }
-  }
+    for (int r = 0; r < 4; ++r) {
+      const int n = 16 * t + 4 * lk + r;
+      if (n >= nz) continue;
+      if (hx) colx[stx * n] = out[0][t][r];
+      if (hy) coly[sty * n] = out[1][t][r];
+      if (n < nz - 1) colz[stz * n] = out[2][t][r];
+    }
+}
+
+// the floats of the padded image k_zmodes_mf copies into LDS (four bases of NZP rows of NZP + 4)
+int zmodes_mf_image(int nz) {
+  const int nzp = 16 * ((nz + 15) / 16);
+  return 4 * nzp * (nzp + 4);
 }
 
 static hipError_t launch_zmodes_mf(const ZModesQ &m, const int *gate, hipStream_t s) {
   const int NT = (m.nz + 15) / 16;
+  if (!m.zmf) return hipErrorInvalidValue;
   const dim3 grid(cdiv(m.q.h[2][0] * m.q.h[2][1], 64), 1, 4), blk(256);
-  const size_t lds = (size_t)(4 * 16 * NT + 3 * 64) * (16 * NT + 4) * sizeof(float);
+  const size_t lds = (size_t)zmodes_mf_image(m.nz) * sizeof(float);
   switch (NT) {
     case 1: TDGL_LAUNCH((k_zmodes_mf<1>), grid, blk, lds, s, m, gate); break;
     case 2: TDGL_LAUNCH((k_zmodes_mf<2>), grid, blk, lds, s, m, gate); break;

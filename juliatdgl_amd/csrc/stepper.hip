@@ -158,6 +158,7 @@ struct tdgl_handle {
   int zq_waves = 8;        // the quarter z pass: 4 or 8 waves a block on the vector ALUs, 0 on the matrix cores (k_zmodes_mf)
   int stxcd = 1;           // the stencil kernels' blocks in XCD runs (TDGL_HIP_STENCILXCD=0: the hardware's order)
   float *zqb[4] = {};      // its padded z bases (ZQ_BASES)
+  float *zqmf = nullptr;   // the same as k_zmodes_mf's image (zmodes_mf_image floats)
   // the parity-folded FP64 separable solve (field_mode SEPARABLE): the same quarters, in T1 and T2,
   // the FP64 half-size bases [axis][0=V,1=U][parity][0=fwd,1=inv] and padded z bases
   bool fold64 = false;
@@ -911,6 +912,7 @@ int sep_folded_tail(tdgl_handle *h, const SepOut &o, const int *gate, Conv conv)
   zq.modes = h->qb;
   zq.q = h->q;
   for (int k = 0; k < 4; ++k) zq.zb[k] = h->zqb[k];
+  zq.zmf = h->zqmf;
   zq.sx = h->sf[0];
   zq.sy = h->sf[1];
   zq.sz = h->sf[2];
@@ -2536,6 +2538,16 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
             ALLOC(h->zqb[k], zb[k].size());
             CHK(upload_vec(h, h->zqb[k], zb[k]));
           }
+          // k_zmodes_mf's image: Vt, Ut (by columns), V, U (by rows), NZP rows of NZP + 4, zero beyond nz
+          const int nzp = 16 * ((nz + 15) / 16), sp = nzp + 4;
+          std::vector<float> img((size_t)zmodes_mf_image(nz), 0.0f);
+          for (int b = 0; b < 4; ++b) {
+            const std::vector<float> &src = zb[b == 0 ? 2 : b == 1 ? 3 : b == 2 ? 0 : 1];
+            for (int r = 0; r < nz; ++r)
+              for (int k = 0; k < nz; ++k) img[((size_t)b * nzp + r) * sp + k] = src[(size_t)r * nz + k];
+          }
+          ALLOC(h->zqmf, img.size());
+          CHK(upload_vec(h, h->zqmf, img));
         }
         h->fold = true;
       }
