@@ -2109,20 +2109,32 @@ __device__ __forceinline__ void qplane_body(const QPassT<T> &p, const int *gate,
   const int xs = 16 * mt, ys = 16 * jt;                // the padded bases' row strides
   T *Bt = (T *)qplane_smem, *St = Bt;
   {
-    // the plane, zero beyond e0 x e1 up to whole tiles; eight loads in flight per thread
+    // the plane, zero beyond e0 x e1 up to whole tiles: wave w takes the rows j = w, w + waves, ...
+    // four at a time, its lanes along the row (ii = l, l + 64, l + 128: W <= 16 QP_MT = 160), so
+    // that no index is divided; twelve loads in flight per lane
     const T *src = (const T *)p.in[comp].p + (size_t)p.in[comp].s2 * plane;
-    const int s1 = p.in[comp].s1, W = 16 * mt, n = W * 16 * jt;
-    for (int t0 = 0; t0 < n; t0 += 8 * nth) {
-      T v[8];
+    const int s1 = p.in[comp].s1, W = 16 * mt, H = 16 * jt;
+    const int nw = nth >> 6, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
+    constexpr int R = 4, LR = (16 * QP_MT + 63) / 64;
+    for (int j0 = w; j0 < H; j0 += R * nw) {
+      T v[R][LR];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int t = t0 + tid + u * nth, ii = t % W, j = t / W;
-        v[u] = (t < n && ii < e0 && j < e1) ? src[ii + (size_t)s1 * j] : T(0);
+      for (int r = 0; r < R; ++r) {
+        const int j = j0 + r * nw;
+#pragma unroll
+        for (int c = 0; c < LR; ++c) {
+          const int ii = l + 64 * c;
+          v[r][c] = (j < e1 && ii < e0) ? src[ii + (size_t)s1 * j] : T(0);
+        }
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int t = t0 + tid + u * nth, ii = t % W, j = t / W;
-        if (t < n) Bt[j * E0S + ii] = v[u];
+      for (int r = 0; r < R; ++r) {
+        const int j = j0 + r * nw;
+#pragma unroll
+        for (int c = 0; c < LR; ++c) {
+          const int ii = l + 64 * c;
+          if (j < H && ii < W) Bt[j * E0S + ii] = v[r][c];
+        }
       }
     }
   }
