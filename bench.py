@@ -105,7 +105,11 @@ def kernel_flops(st, name: str, launches: int, folded: bool = False) -> float:
     """Algorithmic flops of the launches of one kernel class: a fused x-y transform launch of the
     separable solve multiplies every element by an (Lx x Lx) then an (Ly x Ly) matrix, 2 (Lx + Ly)
     flop per element; folded, each of the four quarters of a component by its half-size bases,
-    2 (ceil(Lx/2) + ceil(Ly/2)) flop per quarter element (half the work)."""
+    2 (ceil(Lx/2) + ceil(Ly/2)) flop per quarter element (half the work).  The folded Float32
+    preconditioner's z pass takes every quarter element by an (Lz x Lz) matrix forward and back,
+    4 Lz flop per element."""
+    if name == "sep_modes" and folded and st.field_mode == 2:
+        return launches * sum(4 * h0 * h1 * e2 * 4.0 * e2 for h0, h1, e2 in quarters(st.grid))
     if name == "sep_transform_f32" and folded:
         return launches * sum(4 * 2.0 * h0 * h1 * e2 * (h0 + h1) for h0, h1, e2 in quarters(st.grid))
     if name.startswith("sep_transform"):
@@ -333,7 +337,7 @@ def roofline(st, dom, nl, ms, nsteps, iters, workload, folded=False):
     avg_s = ms / nl / 1e3
     byts = kernel_bytes(st, dom, nl, nsteps, {"iters_per_solve": iters / nsteps, "folded": folded}) / nl
     flops = kernel_flops(st, dom, nl, folded) / nl
-    peak = FP32_MFMA_PEAK if dom == "sep_transform_f32" else FP64_PEAK
+    peak = FP32_MFMA_PEAK if dom in ("sep_transform_f32", "sep_modes") else FP64_PEAK
     if byts > 0 and (flops == 0 or flops / byts < peak / HBM_PEAK):
         roof = {"bound": "hbm", "achieved": byts / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": byts / avg_s / HBM_PEAK, "traffic": None, "kernel": dom, "bytes_per_launch": byts,
@@ -444,7 +448,7 @@ def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=Non
             row = {"us_per_launch": us, "bytes_per_launch": b, "GB_s": b / us / 1e3, "hbm_frac": b / us / 1e3 / (HBM_PEAK / 1e9)}
             f = kernel_flops(st, kname, nl_k, bool(st1.get("folded", 0))) / nl_k
             if f > 0:
-                pk = FP32_MFMA_PEAK if kname == "sep_transform_f32" else FP64_PEAK
+                pk = FP32_MFMA_PEAK if kname in ("sep_transform_f32", "sep_modes") else FP64_PEAK
                 row["TFLOP_s"] = f / us / 1e6
                 row["mfma_frac"] = f / us / 1e6 / (pk / 1e12)
             per[kname] = row
