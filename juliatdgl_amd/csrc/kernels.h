@@ -393,7 +393,6 @@ struct QPassT {
 typedef QPassT<float> QPass;
 typedef QPassT<double> QPass64;
 bool qplane_fits(const QPass &p);
-int qplane_parts(const QPass &p);
 hipError_t launch_qplane(const QPass &p, const int *gate, hipStream_t s);
 bool qplane64_fits(const QPass64 &p);
 hipError_t launch_qplane64(const QPass64 &p, hipStream_t s);

@@ -2188,11 +2188,6 @@ static bool qplane_fits_t(const QPassT<T> &p) {
 }
 bool qplane_fits(const QPass &p) { return qplane_fits_t(p); }
 bool qplane64_fits(const QPass64 &p) { return qplane_fits_t(p); }
-int qplane_parts(const QPass &p) {
-  int mtx, jtx, e2max;
-  qplane_tiles(p, mtx, jtx, e2max);
-  return 12 * e2max;
-}
 hipError_t launch_qplane(const QPass &p0, const int *gate, hipStream_t s) {
   if (g_rec) return hipErrorNotSupported;
   if (!qplane_fits(p0)) return hipErrorInvalidValue;
