@@ -379,7 +379,7 @@ def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=Non
     if tune and st.field_mode == 2:
         tuned = dev.autotune(H, HJ)
         print(f"[bench] {name}: tuned box_kc={tuned['box_kc']} plane_mt={tuned['plane_mt']} "
-              f"zmodes_waves={tuned['zmodes_waves']} fold={tuned['fold']} zcol={tuned['zcol']}: " +
+              f"zmodes_waves={tuned['zmodes_waves']} qparts={tuned['qparts']} fold={tuned['fold']} zcol={tuned['zcol']}: " +
               ", ".join(f"{k}={v}:{us:.1f}us" for k, v, us in tuned["candidates"]),
               file=sys.stderr, flush=True)
     for q in range(warmup):
@@ -414,7 +414,7 @@ def run_workload(name, steps, warmup, profile=True, cpu_steps=0, profile_out=Non
                       "ms_per_time_step": 1e3 * wall / nsteps, "timed_rollbacks": rollbacks,
                       "timed_graph_captures": captures, "pcg_budget": st1.get("pcg_budget"),
                       "tuned": None if tuned is None else {k: tuned[k] for k in ("box_kc", "plane_mt", "zmodes_waves",
-                                                                                 "fold", "zcol")},
+                                                                                 "qparts", "fold", "zcol")},
                       **({"pcg_refine": {"inner_tol": st.pcg_inner_tol, "rounds": st.pcg_rounds}}
                          if getattr(st, "pcg_refine", False) else {})},
            "mask": mask, "roofline": None, "cpu": None}

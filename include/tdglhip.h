@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define TDGL_ABI_VERSION 4
+#define TDGL_ABI_VERSION 5
 #define TDGL_MAX_BANDS 4
 #define TDGL_MAX_STAGES 64
 
@@ -244,18 +244,19 @@ int tdgl_step_noalloc(tdgl_handle *h, int32_t n, double H, double HJ);
  * shape and keeps the fastest: the PCG box kernels' k levels per block (box_kc: 1, 2, 4, ... while
  * the reductions' partials fit), the m tiles per block of the one-round x-y transform (2 or 3), and
  * the folded preconditioner's z pass: on the matrix cores (0) or with 4 or 8 waves per block
- * (groups of z modes) on the vector ALUs; then the Float32 preconditioner's path, timed as whole
+ * (groups of z modes) on the vector ALUs, the blocks a folded quarter plane's x-y transform is split
+ * over (qparts 1 or 2, by its m tiles); then the Float32 preconditioner's path, timed as whole
  * PCG iterations: the parity folds (fold 1) or the full-length transforms (fold 0, where a
  * refinement does not need the folds), and for the latter the z pass over whole z-columns (zcol 1)
  * or over x-runs of modes (zcol 0).  The state (F and the normal-fluid potentials included), the
  * PCG history, the open boundary and the noise position are put back; captured cycles are dropped.
  * box_kc changes how the PCG's block partials fall, the z pass's kernel how its sums are grouped
  * and the folds how the preconditioner rounds (each agrees with the others to the PCG tolerance);
- * the m tiles, the vector ALUs' waves and zcol change no result.  cand[] lists every candidate
+ * the m tiles, the vector ALUs' waves, qparts and zcol change no result.  cand[] lists every candidate
  * timed (us per launch of the kernels it shapes; per PCG iteration for fold and zcol). */
 #define TDGL_TUNE_MAX 16
 enum { TDGL_TUNE_BOX_KC = 1, TDGL_TUNE_PLANE_MT = 2, TDGL_TUNE_ZMODES_WAVES = 3, TDGL_TUNE_FOLD = 4,
-       TDGL_TUNE_ZCOL = 5 };
+       TDGL_TUNE_ZCOL = 5, TDGL_TUNE_QPARTS = 6 };
 typedef struct {
   int32_t kind, value;
   double us;
@@ -265,6 +266,7 @@ typedef struct {
   int32_t ncand;
   tdgl_tune_cand cand[TDGL_TUNE_MAX];
   int32_t fold, zcol;  /* the preconditioner's path chosen */
+  int32_t qparts;      /* blocks a folded quarter plane's x-y transform is split over */
 } tdgl_tuning;
 int tdgl_autotune(tdgl_handle *h, double H, double HJ, tdgl_tuning *out);
 

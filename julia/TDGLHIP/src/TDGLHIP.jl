@@ -126,7 +126,7 @@ function _desc(st::Stepper{N}) where N
                   MAXB, ntuple(_ -> 0.0, MAXS))
     bar = st.bar
     bp(k) = bar === nothing ? Ptr{Float64}(0) : hold(getproperty(bar, k))
-    d = DescC(4, N, gr.Nx, gr.Ny, gr.Nz, gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2, gr.dx, gr.dy, gr.dz, st.dt,
+    d = DescC(5, N, gr.Nx, gr.Ny, gr.Nz, gr.x1, gr.x2, gr.y1, gr.y2, gr.z1, gr.z2, gr.dx, gr.dy, gr.dz, st.dt,
               _pad(ntuple(µ -> _bc(st.bc[µ], N, µ), N), MAXB, _ZBC), _pad(ntuple(µ -> _bc(st.bcs[µ], N, µ), N), MAXB, _ZBC),
               _pad(Int32.(st.nsub), MAXB, Int32(0)), _pad(Int32.(st.rkl), MAXB, Int32(0)),
               _pad(ntuple(µ -> Int32(rk[µ] !== nothing), N), MAXB, Int32(0)), _pad(div, MAXB, 0.0),

@@ -10,7 +10,7 @@ import numpy as np
 from .scheme import FIELD_PCG, FIELD_EXPLICIT_SCALED, Stepper
 from .state import FIELDS, State
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 MAXB = 4
 MAXS = 64
 dp = ct.POINTER(ct.c_double)
@@ -86,7 +86,7 @@ class StatsC(ct.Structure):
 
 NKERNELS = 15
 TUNE_MAX = 16
-TUNE_KINDS = {1: "box_kc", 2: "plane_mt", 3: "zmodes_waves", 4: "fold", 5: "zcol"}
+TUNE_KINDS = {1: "box_kc", 2: "plane_mt", 3: "zmodes_waves", 4: "fold", 5: "zcol", 6: "qparts"}
 
 
 class TuneCandC(ct.Structure):
@@ -95,7 +95,8 @@ class TuneCandC(ct.Structure):
 
 class TuningC(ct.Structure):
     _fields_ = [("box_kc", ct.c_int32), ("plane_mt", ct.c_int32), ("zmodes_waves", ct.c_int32),
-                ("ncand", ct.c_int32), ("cand", TuneCandC * TUNE_MAX), ("fold", ct.c_int32), ("zcol", ct.c_int32)]
+                ("ncand", ct.c_int32), ("cand", TuneCandC * TUNE_MAX), ("fold", ct.c_int32), ("zcol", ct.c_int32),
+                ("qparts", ct.c_int32)]
 
 
 class MeterC(ct.Structure):

@@ -389,6 +389,7 @@ struct QPassT {
   Conv conv;
   PcgDev d;
   int e2max, mtx, jtx;  // the launch's planes and largest tiles (set by the launcher)
+  int parts;            // blocks a quarter plane's m tiles are split over (1: one block a plane; Float32 only)
 };
 typedef QPassT<float> QPass;
 typedef QPassT<double> QPass64;

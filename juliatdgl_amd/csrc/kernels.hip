@@ -2007,16 +2007,18 @@ template hipError_t launch_plane<float, float, float>(const PlanePass &, const i
 // where its results lie (the FP64 accumulator's element r of lane l is row 4 r + l / 16).
 template <class T, int JT, bool FULL>
 __device__ __forceinline__ void qplane_products(const QPassT<T> &p, int comp, int plane, int tid, int mt, int jt, int e0,
-                                                int e1, int E0S, int JS, int xs, int ys, T *Bt, T *St) {
+                                                int e1, int E0S, int JS, int xs, int ys, T *Bt, T *St, int m0t, int mh) {
   typedef typename Mf<T>::v4 V4;
+  // wave w takes the m tile m0t + w of the block's mh (its rows of S in LDS are the block's own)
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, lr = l & 15, lk = l >> 4;
-  const bool on = w < mt;
+  const int wm = m0t + w;
+  const bool on = w < mh && wm < mt;
   V4 acc[JT];
 #pragma unroll
   for (int c = 0; c < JT; ++c) acc[c] = V4{0, 0, 0, 0};
   if (on) {
     // phase 1: A = X(16 g + 4 lk + u, 16 w + lr) from XT, B = in(16 g + 4 lk + u, 16 c + lr) from Bt
-    const T *xa = p.XT[comp] + (size_t)(16 * w + lr) * xs + 4 * lk;
+    const T *xa = p.XT[comp] + (size_t)(16 * wm + lr) * xs + 4 * lk;
     V4 A = Mf<T>::ld4(xa);
     for (int g = 0; g < mt; ++g) {
       const V4 An = Mf<T>::ld4(xa + 16 * min(g + 1, mt - 1));
@@ -2065,7 +2067,7 @@ __device__ __forceinline__ void qplane_products(const QPassT<T> &p, int comp, in
   // out(q, m): element r of lane l is q = 16 c + mrow(l / 16, r), m = 16 w + lr
   const View vo = p.out[comp];
   T *dst = (T *)vo.p + (size_t)vo.s2 * plane;
-  const int m = 16 * w + lr;
+  const int m = 16 * wm + lr;
 #pragma unroll
   for (int c = 0; c < JT; ++c)
 #pragma unroll
@@ -2078,14 +2080,16 @@ __device__ __forceinline__ void qplane_products(const QPassT<T> &p, int comp, in
 // LDS rows of 4 (mod 8) words: 4 floats or 2 doubles of padding
 template <class T> constexpr int qplane_pad() { return 16 / (int)sizeof(T); }
 
-// block `bid` of a k_qplane launch: (quarter, plane) = (bid / e2max, bid % e2max)
+// block `bid` of a k_qplane launch: (quarter, plane, part) = (u / e2max, u % e2max, bid % parts) with
+// u = bid / parts; part h takes the m tiles h mh .. h mh + mh - 1, mh = ceil(mt / parts), of the plane
 template <class T, int JT>
 __device__ __forceinline__ void qplane_body(const QPassT<T> &p, const int *gate, int bid) {
   extern __shared__ __align__(16) unsigned char qplane_smem[];
   __shared__ double sh[16];
   const int tid = threadIdx.x, nth = blockDim.x;
-  const int e2max = p.e2max, mtx = p.mtx, jtx = p.jtx;
-  const int comp = bid / e2max, plane = bid % e2max;
+  const int e2max = p.e2max, mtx = p.mtx, jtx = p.jtx, parts = p.parts > 1 ? p.parts : 1;
+  const int unit = bid / parts, part = bid - unit * parts;
+  const int comp = unit / e2max, plane = unit % e2max;
   if (p.conv.mode != CONV_NONE) {
     if (!*p.conv.in) {
       if (bid == 0 && tid == 0) *p.conv.out = 0;
@@ -2139,10 +2143,11 @@ __device__ __forceinline__ void qplane_body(const QPassT<T> &p, const int *gate,
     }
   }
   __syncthreads();
+  const int mh = (mt + parts - 1) / parts;
   if (jt == JT)
-    qplane_products<T, JT, true>(p, comp, plane, tid, mt, jt, e0, e1, E0S, JS, xs, ys, Bt, St);
+    qplane_products<T, JT, true>(p, comp, plane, tid, mt, jt, e0, e1, E0S, JS, xs, ys, Bt, St, part * mh, mh);
   else
-    qplane_products<T, JT, false>(p, comp, plane, tid, mt, jt, e0, e1, E0S, JS, xs, ys, Bt, St);
+    qplane_products<T, JT, false>(p, comp, plane, tid, mt, jt, e0, e1, E0S, JS, xs, ys, Bt, St, part * mh, mh);
 }
 
 template <int JT>
@@ -2193,7 +2198,8 @@ hipError_t launch_qplane(const QPass &p0, const int *gate, hipStream_t s) {
   if (!qplane_fits(p0)) return hipErrorInvalidValue;
   QPass p = p0;
   qplane_tiles(p, p.mtx, p.jtx, p.e2max);
-  const dim3 grid(12 * p.e2max), blk(64 * std::max(4, p.mtx));
+  p.parts = std::max(1, std::min(p.parts, p.mtx));
+  const dim3 grid(12 * p.e2max * p.parts), blk(64 * std::max(4, cdiv(p.mtx, p.parts)));
   const size_t lds = qplane_lds<float>(p.mtx, p.jtx);
   switch (p.jtx) {
     case 1: TDGL_LAUNCH((k_qplane<1>), grid, blk, lds, s, p, gate); break;
@@ -2210,6 +2216,7 @@ hipError_t launch_qplane64(const QPass64 &p0, hipStream_t s) {
   if (!qplane64_fits(p0)) return hipErrorInvalidValue;
   QPass64 p = p0;
   qplane_tiles(p, p.mtx, p.jtx, p.e2max);
+  p.parts = 1;
   const int waves = std::max(4, p.mtx);
   const dim3 grid(12 * p.e2max), blk(64 * waves);
   const size_t lds = qplane_lds<double>(p.mtx, p.jtx);

@@ -155,6 +155,7 @@ struct tdgl_handle {
   float *Hp[2][2][2][2] = {};  // Hq zero-padded to whole 16 x 16 tiles
   int HpL[2][2] = {};          // their unpadded order
   int qplane = 1;              // TDGL_HIP_QPLANE=0: the quarter x-y transforms on k_plane_mf1 / k_plane_mf
+  int qparts = 1;              // k_qplane's blocks a quarter plane (its m tiles split; TDGL_HIP_QPARTS)
   int zq_waves = 8;        // the quarter z pass: 4 or 8 waves a block on the vector ALUs, 0 on the matrix cores (k_zmodes_mf)
   int stxcd = 1;           // the stencil kernels' blocks in XCD runs (TDGL_HIP_STENCILXCD=0: the hardware's order)
   float *zqb[4] = {};      // its padded z bases (ZQ_BASES)
@@ -852,6 +853,7 @@ QPass qplane_pass(tdgl_handle *h, bool inv, float *in, float *out) {
   const PlanePass p = quarter_pass(h, inv, in, out);
   QPass qp{};
   qp.d = h->pd;
+  qp.parts = h->qparts;
   for (int c = 0; c < 3; ++c)
     for (int px = 0; px < 2; ++px)
       for (int py = 0; py < 2; ++py) {
@@ -2461,6 +2463,8 @@ int tdgl_create(const tdgl_desc *desc, tdgl_handle **out) {
     h->zcols = zc && zc[0] == '1' ? 1 : 0;
     const char *qp = std::getenv("TDGL_HIP_QPLANE");
     h->qplane = qp && qp[0] == '0' ? 0 : 1;
+    const char *qs = std::getenv("TDGL_HIP_QPARTS");
+    if (qs && qs[0] >= '1' && qs[0] <= '3' && !qs[1]) h->qparts = qs[0] - '0';
     // TDGL_HIP_ZQ_WAVES=4 or 8: the quarter z pass on the vector ALUs with that many waves a block
     // (8 by default), 0: on the matrix cores
     const char *sx = std::getenv("TDGL_HIP_STENCILXCD");
@@ -3041,10 +3045,13 @@ int tdgl_autotune(tdgl_handle *h, double H, double HJ, tdgl_tuning *out) {
     }
   }
   h->box_kc = best_kc;
-  // m tiles per block of the one-round x-y kernel
+  // m tiles per block of the one-round x-y kernel, timed on the full-length path that launches it
+  // (a folded handle's x-y launches are the quarter kernel's), unless a refinement keeps the folds
   const int mt0 = h->plane1;
   int best_mt = mt0;
   best = 1e300;
+  const bool fold_in = h->fold;
+  if (!h->refine) h->fold = false;
   for (int mt = 2; mt <= 3 && rc == 0; ++mt) {
     h->plane1 = mt;
     double us = 0;
@@ -3054,6 +3061,7 @@ int tdgl_autotune(tdgl_handle *h, double H, double HJ, tdgl_tuning *out) {
       best_mt = mt;
     }
   }
+  h->fold = fold_in;
   h->plane1 = best_mt;
   // waves (groups of z modes) per block of the quarter z pass
   const int zw0 = h->zq_waves;
@@ -3071,6 +3079,22 @@ int tdgl_autotune(tdgl_handle *h, double H, double HJ, tdgl_tuning *out) {
       }
     }
     h->zq_waves = best_zw;
+  }
+  // blocks a folded quarter plane's x-y transform is split over (its m tiles)
+  const int qp0 = h->qparts;
+  if (h->fold && h->qplane) {
+    int best_qp = qp0;
+    best = 1e300;
+    for (int qp = 1; qp <= 2 && rc == 0; ++qp) {
+      h->qparts = qp;
+      double us = 0;
+      rc = trial(TDGL_TUNE_QPARTS, qp, K_TX32, -1, &us);
+      if (rc == 0 && us < best) {
+        best = us;
+        best_qp = qp;
+      }
+    }
+    h->qparts = best_qp;
   }
   // the preconditioner's path (a whole PCG iteration each): the folds, or the full-length
   // transforms with the z pass over x-runs or whole columns; the folds stay where a refinement
@@ -3118,6 +3142,7 @@ int tdgl_autotune(tdgl_handle *h, double H, double HJ, tdgl_tuning *out) {
     h->box_kc = kc0;
     h->plane1 = mt0;
     h->zq_waves = zw0;
+    h->qparts = qp0;
     h->fold = fold0;
     h->zcols = zcol0;
     return rc;
@@ -3128,6 +3153,7 @@ int tdgl_autotune(tdgl_handle *h, double H, double HJ, tdgl_tuning *out) {
   out->zmodes_waves = h->zq_waves;
   out->fold = h->fold ? 1 : 0;
   out->zcol = h->zcols;
+  out->qparts = h->qparts;
   return 0;
 }
 

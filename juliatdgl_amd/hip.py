@@ -198,6 +198,7 @@ class HipStepper:
         c = TuningC()
         _check(load().tdgl_autotune(self.h, float(H), float(HJ), ct.byref(c)), "tdgl_autotune")
         return {"box_kc": c.box_kc, "plane_mt": c.plane_mt, "zmodes_waves": c.zmodes_waves, "fold": c.fold, "zcol": c.zcol,
+                "qparts": c.qparts,
                 "candidates": [(TUNE_KINDS.get(c.cand[i].kind, "?"), c.cand[i].value, c.cand[i].us)
                                for i in range(c.ncand)]}
 

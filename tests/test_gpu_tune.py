@@ -46,6 +46,10 @@ def test_tune_restores_state_and_steps_agree(name):
         assert t["box_kc"] in timed["box_kc"] and t["plane_mt"] in timed["plane_mt"]
         if "zmodes_waves" in timed:
             assert t["zmodes_waves"] in timed["zmodes_waves"]
+        if "qparts" in timed:
+            assert t["qparts"] in timed["qparts"] and timed["qparts"] == {1, 2}
+        else:
+            assert t["qparts"] == 1
         # the preconditioner's path: the folds (timed where the handle has them) or the full-length
         # transforms with one of the two z passes
         if t["fold"]:
