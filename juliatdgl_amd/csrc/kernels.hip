@@ -2080,15 +2080,20 @@ __device__ __forceinline__ void qplane_products(const QPassT<T> &p, int comp, in
 // LDS rows of 4 (mod 8) words: 4 floats or 2 doubles of padding
 template <class T> constexpr int qplane_pad() { return 16 / (int)sizeof(T); }
 
-// block `bid` of a k_qplane launch: (quarter, plane, part) = (u / e2max, u % e2max, bid % parts) with
-// u = bid / parts; part h takes the m tiles h mh .. h mh + mh - 1, mh = ceil(mt / parts), of the plane
+// block `bid` of a k_qplane launch: unit u = (quarter, plane) = (u / e2max, u % e2max) and its part h,
+// which takes the m tiles h mh .. h mh + mh - 1, mh = ceil(mt / parts), of the plane.  The parts of a
+// unit are 8 block indices apart (u = 8 (bid / 8 parts) + bid % 8, h = (bid / 8) % parts): the
+// hardware deals blocks to the eight XCDs in turn, so they share one XCD and the later part finds
+// the plane in its L2
 template <class T, int JT>
 __device__ __forceinline__ void qplane_body(const QPassT<T> &p, const int *gate, int bid) {
   extern __shared__ __align__(16) unsigned char qplane_smem[];
   __shared__ double sh[16];
   const int tid = threadIdx.x, nth = blockDim.x;
   const int e2max = p.e2max, mtx = p.mtx, jtx = p.jtx, parts = p.parts > 1 ? p.parts : 1;
-  const int unit = bid / parts, part = bid - unit * parts;
+  const int grp = bid / (8 * parts), r8 = bid - grp * 8 * parts;
+  const int unit = 8 * grp + (r8 & 7), part = r8 >> 3;
+  if (unit >= 12 * e2max) return;
   const int comp = unit / e2max, plane = unit % e2max;
   if (p.conv.mode != CONV_NONE) {
     if (!*p.conv.in) {
@@ -2199,7 +2204,7 @@ hipError_t launch_qplane(const QPass &p0, const int *gate, hipStream_t s) {
   QPass p = p0;
   qplane_tiles(p, p.mtx, p.jtx, p.e2max);
   p.parts = std::max(1, std::min(p.parts, p.mtx));
-  const dim3 grid(12 * p.e2max * p.parts), blk(64 * std::max(4, cdiv(p.mtx, p.parts)));
+  const dim3 grid(p.parts > 1 ? 8 * p.parts * cdiv(12 * p.e2max, 8) : 12 * p.e2max), blk(64 * std::max(4, cdiv(p.mtx, p.parts)));
   const size_t lds = qplane_lds<float>(p.mtx, p.jtx);
   switch (p.jtx) {
     case 1: TDGL_LAUNCH((k_qplane<1>), grid, blk, lds, s, p, gate); break;
