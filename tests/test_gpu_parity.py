@@ -478,14 +478,14 @@ def test_quarter_plane_kernel_gives_the_same_bits(name, monkeypatch):
 
 @pytest.mark.parametrize("name", ["pcg_mf1_f32", "kwt10_mf1_pcg"])
 def test_quarter_planes_split_over_two_blocks_give_the_same_bits(name, monkeypatch):
-    """k_qplane with each quarter plane's m tiles split over two blocks (TDGL_HIP_QPARTS=2: each
-    block stages the whole plane and computes its own rows) against one block a plane: every output
-    element takes the same operands in the same order, so eight steps give the same bits."""
+    """k_qplane with each quarter plane's m tiles split over two or three blocks (TDGL_HIP_QPARTS=2, 3:
+    each block stages the whole plane and computes its own rows) against one block a plane: every
+    output element takes the same operands in the same order, so eight steps give the same bits."""
     HipStepper = _hip()
     st, _ = make_case(name)
     s0, pcg, H, HJ = seed_state(st)
     outs = []
-    for v in ("1", "2"):
+    for v in ("1", "2", "3"):
         print(f"[gpu] TDGL_HIP_QPARTS={v}: {name}", flush=True)
         monkeypatch.setenv("TDGL_HIP_QPARTS", v)
         dev = HipStepper(st)
@@ -496,8 +496,9 @@ def test_quarter_planes_split_over_two_blocks_give_the_same_bits(name, monkeypat
         assert dev.stats()["folded"] == 1
         outs.append(dev.download())
         dev.close()
-    for (k, a), (_, b) in zip(outs[0].arrays(), outs[1].arrays()):
-        assert np.array_equal(a, b), k
+    for other in outs[1:]:
+        for (k, a), (_, b) in zip(outs[0].arrays(), other.arrays()):
+            assert np.array_equal(a, b), k
 
 
 @pytest.mark.parametrize("name", ["pcg_fast_f32", "pcg_tight_f32", "kwt10_pcg", "pcg_wide_f32", "open_kwt_pcg",
